@@ -1,0 +1,218 @@
+"""Benchmark: batched frequency-domain Helmholtz solves on one MI355X.
+
+Workload (default): 512^2 interior + 32-cell stretched PML (576^2 nodes),
+3 MHz, dx = lambda_min/10, seeded smooth random medium (c, rho, power-law
+alpha), 11 plane-wave angles in [-15, +15] deg from a 192-element line array,
+192 receivers; complex64 (F)GMRES(30) with FP64 inner products and FP64
+true-residual refinement to relative residual 1e-6 on every angle.
+
+A "step" = one full batched solve of the 11 angles from a zero start:
+coefficients + multigrid set-up on the device, source injection, the Krylov
+solve to 1e-6, receiver sampling.  Inputs (c, rho, alpha maps) are resident
+in HBM before the timed region.  ``value`` = solves/s = 11 * K / seconds.
+
+Prints ONE JSON line (rank 0, N=1).  --gpus N>1 exits with a message: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precond", default="mg", choices=["none", "mg", "learned"])
+    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iterations", type=int, default=6000)
+    ap.add_argument("--plain-max-iterations", type=int, default=1500,
+                    help="iteration cap of the plain (unpreconditioned) GMRES comparison leg")
+    ap.add_argument("--interior", type=int, default=512)
+    ap.add_argument("--pml", type=int, default=32)
+    ap.add_argument("--freq", type=float, default=3.0)
+    ap.add_argument("--angles", type=int, default=11)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--mg-levels", type=int, default=5)
+    ap.add_argument("--mg-shift", type=float, default=0.5)
+    ap.add_argument("--mg-omega", type=float, default=0.6)
+    ap.add_argument("--mg-pre", type=int, default=2)
+    ap.add_argument("--mg-post", type=int, default=2)
+    ap.add_argument("--mg-coarse", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
+    ap.add_argument("--cpu-interior", type=int, default=None,
+                    help="interior size of the CPU baseline sample (default: the full workload)")
+    return ap.parse_args()
+
+
+class Bench:
+    def __init__(self, a):
+        import numpy as np
+        import torch
+        from nsrno_amd import media
+        from nsrno_amd.helmholtz import HelmholtzPlan, Ports
+        from nsrno_amd.theranostic import Grid, Solver
+
+        self.a = a
+        self.torch = torch
+        self.work = w = media.smooth_random_medium(seed=a.seed, interior=a.interior, pml=a.pml,
+                                                   frequency_mhz=a.freq)
+        self.angles = np.linspace(-15.0, 15.0, a.angles)
+        pos, nrm, wid = media.line_array(w, 192)
+        grid = Grid(w.x_mm, w.z_mm, w.pml_mm)
+        self.weights = Solver(grid, c0_m_s=w.c0).source_weights(pos, nrm, wid)
+        self.amp = media.plane_wave_amplitudes(pos, self.angles, w.frequency_mhz, w.c0)
+        dev = torch.device("cuda")
+        self.ports = Ports(self.weights, device="cuda")
+        self.amp_d = torch.from_numpy(self.amp).to(dev)
+        self.c = torch.from_numpy(w.c).to(dev)
+        self.rho = torch.from_numpy(w.rho).to(dev)
+        self.alpha = torch.from_numpy(w.alpha_np_m).to(dev)
+        self.plan = HelmholtzPlan(w.nz, w.nx, w.h_mm, w.frequency_mhz, w.pml_mm, c0_m_s=w.c0,
+                                  nrhs=a.angles, restart=a.restart)
+        self.R = a.angles
+        self.n = w.nz * w.nx
+
+    def step(self, precond, max_iterations):
+        a, p = self.a, self.plan
+        p.set_medium_physical(self.c, self.rho, self.alpha, rho0=self.work.rho0)
+        if precond != "none":
+            p.setup_multigrid(levels=a.mg_levels, shift=a.mg_shift, pre=a.mg_pre, post=a.mg_post,
+                              coarse_sweeps=a.mg_coarse, omega=a.mg_omega)
+        rhs = p.inject(self.ports, self.amp_d)
+        x, infos = p.solve(rhs, tolerance=a.tol, restart=a.restart, max_iterations=max_iterations,
+                           precond=precond)
+        traces = p.sample(self.ports, x)
+        return x, infos, traces
+
+    def timed(self, precond, steps, warmup, max_iterations):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step(precond, max_iterations)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            x, infos, traces = self.step(precond, max_iterations)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, x, infos, traces
+
+    def stencil_roofline(self, reps=50):
+        """Average duration of the FP32 stencil kernel on the bench's batch,
+        timed with HIP events on the stream it is launched on."""
+        torch = self.torch
+        p = self.plan
+        p.set_medium_physical(self.c, self.rho, self.alpha, rho0=self.work.rho0)
+        u = torch.randn(self.R, self.work.nz, self.work.nx, dtype=torch.complex64, device="cuda")
+        out = torch.empty_like(u)
+        for _ in range(5):
+            p.apply(u, out=out)
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            p.apply(u, out=out)
+        e1.record(s)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / reps
+        bytes_alg = (16.0 + 12.0 / self.R) * self.n * self.R
+        return t, bytes_alg
+
+
+def cpu_baseline(bench, a):
+    """Direct complex128 sparse LU of the same operator and RHS (the
+    theranostic CPU algorithm, wave.py:240-273), via the CPU oracle."""
+    import numpy as np
+    from oracle import helmholtz as O
+    from nsrno_amd import media
+    w = bench.work
+    if a.cpu_interior is not None and a.cpu_interior != a.interior:
+        w = media.smooth_random_medium(seed=a.seed, interior=a.cpu_interior, pml=a.pml, frequency_mhz=a.freq)
+    grid = O.Grid(w.x_mm, w.z_mm, w.pml_mm)
+    pos, nrm, wid = media.line_array(w, 192)
+    W = O.source_weights(grid, pos, nrm, wid)
+    amp = media.plane_wave_amplitudes(pos, bench.angles, w.frequency_mhz, w.c0)
+    rhs = -(W @ amp.T) / grid.h_m ** 2
+    t0 = time.perf_counter()
+    x, diag = O.solve(grid, w.frequency_mhz, None, w.rho0 * w.c0 ** 2 / (w.rho * w.c ** 2) - 1,
+                      w.rho0 / w.rho - 1, kappa_factor=(1 + 1j * w.alpha_np_m * w.c / (2 * np.pi * w.frequency_mhz * 1e6)) ** 2,
+                      rhs=rhs, c0=w.c0, check_pml=False)
+    dt = time.perf_counter() - t0
+    return dict(seconds=dt, solves=rhs.shape[1], nodes=grid.size, residual=diag["relative_linear_residual"],
+                fields=x, grid_shape=grid.shape)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    b = Bench(a)
+    print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)
+    dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations)
+    solves_per_s = b.R * a.steps / dt
+    iters = [i.iterations for i in infos]
+    rel = [i.relative_residual for i in infos]
+    print(f"[bench] {a.precond}: {dt / a.steps * 1e3:.1f} ms/step, iterations {iters}, max rel {max(rel):.2e}",
+          flush=True)
+    result = {}
+    if not a.no_plain and a.precond != "none":
+        pdt, _, pinf, _ = b.timed("none", 1, 0, a.plain_max_iterations)
+        result["plain_gmres"] = dict(
+            seconds_per_step=pdt, solves_per_s=b.R / pdt, iterations=[i.iterations for i in pinf],
+            max_relative_residual=max(i.relative_residual for i in pinf),
+            reasons=sorted(set(i.reason for i in pinf)), max_iterations=a.plain_max_iterations)
+        print(f"[bench] plain: {pdt * 1e3:.1f} ms, iterations {result['plain_gmres']['iterations'][:3]}..., "
+              f"max rel {result['plain_gmres']['max_relative_residual']:.2e}", flush=True)
+    t_st, bytes_alg = b.stencil_roofline()
+    achieved = bytes_alg / t_st / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_apply<float,float2,0> (FP32 stencil, 11 RHS)",
+                "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "avg_launch_us": round(t_st * 1e6, 2), "algorithmic_bytes": int(bytes_alg)}
+    cpu = None
+    if not a.no_cpu:
+        c = cpu_baseline(b, a)
+        cpu = {"value": round(c["solves"] / c["seconds"], 5), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"complex128 splu (oracle restatement of wave.py:240-273) of the same "
+                         f"{c['grid_shape'][0]}x{c['grid_shape'][1]} operator, {c['solves']} RHS, "
+                         f"assembly+factor+solve {c['seconds']:.2f} s, residual {c['residual']:.1e}"}
+        if c["grid_shape"] == (b.work.nz, b.work.nx):
+            xf = x.reshape(b.R, -1).T.cpu().numpy()
+            err = np.linalg.norm(xf - c["fields"], axis=0) / np.linalg.norm(c["fields"], axis=0)
+            result["field_error_vs_c128_direct"] = [float(f"{e:.3e}") for e in err]
+        print(f"[bench] cpu baseline {c['seconds']:.2f} s", flush=True)
+    line = {
+        "metric": "solves/s to rel-res 1e-6, 512²×11 angles: learned FGMRES vs plain GMRES",
+        "value": round(solves_per_s, 4), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (FP64 reductions/refinement)",
+        "data": "synthetic (seeded smooth random medium)",
+        "config": {"workload": f"helmholtz {a.interior}^2+{a.pml}PML {a.freq}MHz x{a.angles} angles",
+                   "nodes": b.n, "rhs": b.R, "tolerance": a.tol, "method": f"fgmres({a.restart})",
+                   "precond": a.precond, "parallelism": "single GPU"},
+        "iterations": iters, "max_relative_residual": max(rel),
+        "reasons": sorted(set(i.reason for i in infos)),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

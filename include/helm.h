@@ -1,0 +1,159 @@
+/*
+ * helm.h -- C ABI of libhelm.so, the MI355X frequency-domain Helmholtz solver.
+ *
+ * 2-D variable-density viscoacoustic Helmholtz operator in flux form with a
+ * complex-coordinate-stretched quadratic PML, multiplied by sx*sz so that the
+ * matrix is complex symmetric (A^T = A), exp(-i omega t) convention:
+ *
+ *   (A u)[i,j] = k0^2 sz_i sx_j kappa_c[i,j] u[i,j]
+ *              + sum_{4 edges e} F_e (u_neighbour - u[i,j]),     u_ghost = 0
+ *   F_e = stretch_e * mean(eta_rel over the edge ends, ghost end = 1) / h^2
+ *
+ * eta_rel = 1 + m_eta = rho0/rho,  kappa_c = (1 + m_kappa) (1 + i alpha c/omega)^2.
+ * This is the operator of nsrno/theranostic/wave.py:127-188 (Solver.matrix),
+ * extended by the complex kappa_c for attenuation.
+ *
+ * Conventions: arrays are [rhs][z][x] row-major, complex numbers are
+ * interleaved (re, im); c64 = 2 x float32, c128 = 2 x float64.  All data
+ * pointers are DEVICE pointers unless stated.  Every call is asynchronous on
+ * the given HIP stream (hipStream_t passed as void*, NULL = default stream),
+ * except helm_solve, which synchronises once per restart cycle.  Every entry
+ * point returns 0 on success or a HELM_ERR_* code; helm_error_string() turns
+ * it into text and helm_last_error() gives the detail of the last failure on
+ * the calling thread.  A plan is not thread safe: one plan per stream.
+ *
+ * The reference has no C ABI on this path; the native precedent it follows is
+ * nsrno/native_time.cu:117-155 (extern "C" int entry points, device pointers
+ * from torch, a stream argument, an error-string function).
+ */
+#ifndef HELM_H
+#define HELM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HELM_OK 0
+#define HELM_ERR_ARG 1     /* invalid argument (shape, size, value)        */
+#define HELM_ERR_HIP 2     /* a HIP runtime call failed                    */
+#define HELM_ERR_ALLOC 3   /* device allocation failed                     */
+#define HELM_ERR_STATE 4   /* call order (e.g. solve before set_medium)    */
+
+/* termination reasons, nsrno/wave_solver.py:17,36 and nsrno/dense_wave.py:131-145 */
+#define HELM_CONVERGED 0
+#define HELM_ITERATION_BUDGET 1
+#define HELM_BREAKDOWN 2
+#define HELM_STAGNATION 3
+
+#define HELM_GMRES 0
+#define HELM_BICGSTAB 1
+
+#define HELM_PC_NONE 0     /* plain (unpreconditioned) Krylov               */
+#define HELM_PC_MG 1       /* complex-shifted-Laplacian multigrid V-cycle   */
+#define HELM_PC_LEARNED 2  /* learned conv-net correction + MG V-cycle      */
+
+typedef struct helm_plan helm_plan;
+
+/* Grid, frequency and PML: the arguments of Grid/Solver/coefficients,
+ * nsrno/theranostic/wave.py:29-156.  Coordinates in mm, frequency in MHz. */
+typedef struct {
+    int32_t nz, nx;            /* nodes including the PML                    */
+    int32_t nrhs;              /* capacity: right-hand sides per call        */
+    int32_t restart;           /* capacity: Krylov basis length m (0: none)  */
+    double x0_mm, z0_mm;       /* coordinate of node [0,0]                   */
+    double h_mm;               /* equal spacing in x and z                   */
+    double pml_mm;             /* PML thickness on every side (0: none)      */
+    double frequency_mhz;
+    double c0_m_s;             /* k0 = omega/c0 and sigma_max reference      */
+    double pml_strength;       /* sigma_max = pml_strength*c0/pml_width      */
+} helm_desc;
+
+typedef struct {
+    int32_t method;            /* HELM_GMRES or HELM_BICGSTAB                */
+    int32_t precond;           /* HELM_PC_*                                  */
+    int32_t restart;           /* m, 1 <= m <= helm_desc.restart             */
+    int32_t max_iterations;    /* budget of Krylov steps per right-hand side */
+    double tolerance;          /* on the true FP64 ||b - A x|| / ||b||       */
+    int32_t adjoint;           /* nonzero: solve A^H x = b                   */
+    int32_t use_x0;            /* nonzero: x holds the initial guess         */
+    /* called once per restart with the true relative residuals (host array
+     * [nrhs]) and the operator applications so far, like the callback of
+     * nsrno/wave_solver.py:9-17; may be NULL */
+    void (*callback)(void* user, int32_t applications, const double* relative, int32_t nrhs);
+    void* user;
+} helm_solve_opts;
+
+typedef struct {
+    int32_t reason;            /* HELM_CONVERGED ...                         */
+    int32_t iterations;        /* Krylov steps (preconditioned applies)      */
+    int32_t applications;      /* all operator applications incl. residuals  */
+    int32_t cycles;            /* restarts / refinement cycles               */
+    double relative_residual;  /* true FP64 ||b - A x|| / ||b|| at return    */
+} helm_solve_info;
+
+/* ---- library ---------------------------------------------------------- */
+const char* helm_version(void);
+const char* helm_error_string(int32_t code);
+const char* helm_last_error(void);
+
+/* ---- plan: replaces Solver.__init__/coefficients/matrix (wave.py:99-188) */
+int32_t helm_plan_create(const helm_desc* desc, helm_plan** plan);
+int32_t helm_plan_destroy(helm_plan* plan);
+/* eta_rel [N] float64 and kappa_c [N] c128 (device).  Replaces the
+ * m_kappa/m_eta arguments of Solver.matrix (wave.py:174-179). */
+int32_t helm_set_medium(helm_plan* plan, const double* eta_rel, const double* kappa_c, void* stream);
+/* Physical maps (device float64 [N]): c [m/s], rho [kg/m^3], alpha [Np/m];
+ * eta_rel = rho0/rho, kappa_c = rho0 c0^2/(rho c^2) (1 + i alpha c/omega)^2.
+ * Replaces setup.physical_maps (setup.py:299-307) run backwards and the
+ * alpha power law of nsrno/wave_operator.py:56,67. alpha may be NULL. */
+int32_t helm_set_medium_physical(helm_plan* plan, const double* c, const double* rho,
+                                 const double* alpha, double rho0, void* stream);
+
+/* ---- operator: replaces Solver.matrix(...) @ u (wave.py:174-188) ------- */
+/* out = A u (adjoint != 0: A^H u) for nrhs fields; precision 32: c64 in/out
+ * computed in FP32; 64: c128 in/out computed in FP64. */
+int32_t helm_apply(helm_plan* plan, const void* u, void* out, int32_t nrhs,
+                   int32_t precision, int32_t adjoint, void* stream);
+
+/* ---- ports: replaces Solver.solve's RHS = -W/h^2 and Solver.sample ------
+ * (wave.py:260, 275-278).  The sparse port weights W (N x n_ports, real) are
+ * passed twice: node-major CSR (node_ptr [n_nodes+1], node_idx [n_nodes],
+ * port_of_tap [nnz], w_node [nnz]) for injection, port-major CSC (port_ptr
+ * [n_ports+1], node_of_tap [nnz], w_port [nnz]) for sampling; int32/float64. */
+/* rhs[r] = -(1/h^2) sum_p amp[r][p] W[:,p]  (rhs c128 [nrhs][N], zeroed
+ * first; amp c128 [nrhs][n_ports]) */
+int32_t helm_inject(helm_plan* plan, int32_t n_ports, int32_t n_nodes, const int32_t* node_ptr,
+                    const int32_t* node_idx, const int32_t* port_of_tap, const double* w_node,
+                    const void* amp, void* rhs, int32_t nrhs, void* stream);
+/* traces[r][p] = sum_n W[n,p] field[r][n]; field c128 (precision 64) or c64
+ * (precision 32), traces c128 [nrhs][n_ports] */
+int32_t helm_sample(helm_plan* plan, int32_t n_ports, const int32_t* port_ptr,
+                    const int32_t* node_of_tap, const double* w_port, const void* field,
+                    int32_t precision, void* traces, int32_t nrhs, void* stream);
+
+/* ---- Krylov solve: replaces Solver.solve's splu (wave.py:253-261) and
+ * nsrno/wave_solver.gmres (wave_solver.py:9-36) / dense_wave.solve_system
+ * (dense_wave.py:167-225).  rhs and x are c128 [nrhs][N]; info is a HOST
+ * array [nrhs].  Independent Krylov spaces per right-hand side, advanced in
+ * lock step; FP32 stencil and basis vectors, FP64 inner products,
+ * Hessenberg/Givens least squares and true-residual refinement at every
+ * restart. */
+int32_t helm_solve(helm_plan* plan, const void* rhs, void* x, int32_t nrhs,
+                   const helm_solve_opts* opts, helm_solve_info* info, void* stream);
+
+/* ---- preconditioners --------------------------------------------------- */
+/* Complex-shifted-Laplacian multigrid: kappa_c -> kappa_c (1 + i shift).
+ * levels >= 1 (1: smoother only), pre/post damped-Jacobi sweeps with weight
+ * omega, coarse sweeps on the coarsest level. */
+int32_t helm_mg_setup(helm_plan* plan, int32_t levels, double shift, int32_t pre, int32_t post,
+                      int32_t coarse_sweeps, double omega, void* stream);
+/* z = M^{-1} v for nrhs c64 fields (the preconditioner the solve applies). */
+int32_t helm_precond_apply(helm_plan* plan, int32_t precond, const void* v, void* z, int32_t nrhs,
+                           void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

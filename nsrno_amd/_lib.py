@@ -1,0 +1,84 @@
+"""ctypes binding of libhelm.so (the C ABI declared in include/helm.h).
+
+The library is built in-tree by ``python -m nsrno_amd.build``.  There is no
+CPU fallback: if the library is missing every entry point raises.
+"""
+import ctypes
+from ctypes import POINTER, c_double, c_int32, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhelm.so"
+
+HELM_CONVERGED, HELM_ITERATION_BUDGET, HELM_BREAKDOWN, HELM_STAGNATION = 0, 1, 2, 3
+REASONS = {HELM_CONVERGED: "converged", HELM_ITERATION_BUDGET: "iteration_budget",
+           HELM_BREAKDOWN: "breakdown", HELM_STAGNATION: "stagnation"}
+HELM_GMRES, HELM_BICGSTAB = 0, 1
+PC_NONE, PC_MG, PC_LEARNED = 0, 1, 2
+
+
+class HelmDesc(ctypes.Structure):
+    _fields_ = [("nz", c_int32), ("nx", c_int32), ("nrhs", c_int32), ("restart", c_int32),
+                ("x0_mm", c_double), ("z0_mm", c_double), ("h_mm", c_double), ("pml_mm", c_double),
+                ("frequency_mhz", c_double), ("c0_m_s", c_double), ("pml_strength", c_double)]
+
+
+CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), c_int32)
+
+
+class HelmSolveOpts(ctypes.Structure):
+    _fields_ = [("method", c_int32), ("precond", c_int32), ("restart", c_int32), ("max_iterations", c_int32),
+                ("tolerance", c_double), ("adjoint", c_int32), ("use_x0", c_int32),
+                ("callback", CALLBACK), ("user", c_void_p)]
+
+
+class HelmSolveInfo(ctypes.Structure):
+    _fields_ = [("reason", c_int32), ("iterations", c_int32), ("applications", c_int32), ("cycles", c_int32),
+                ("relative_residual", c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "helm_version": (ctypes.c_char_p, []),
+    "helm_error_string": (ctypes.c_char_p, [c_int32]),
+    "helm_last_error": (ctypes.c_char_p, []),
+    "helm_plan_create": (c_int32, [POINTER(HelmDesc), POINTER(c_void_p)]),
+    "helm_plan_destroy": (c_int32, [c_void_p]),
+    "helm_set_medium": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "helm_set_medium_physical": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
+    "helm_apply": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),
+    "helm_inject": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_int32, c_void_p]),
+    "helm_sample": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                              c_int32, c_void_p]),
+    "helm_solve": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, POINTER(HelmSolveOpts),
+                             POINTER(HelmSolveInfo), c_void_p]),
+    "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
+    "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libhelm.so once; raise if it is missing (no fallback path)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m nsrno_amd.build`")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+class HelmError(RuntimeError):
+    pass
+
+
+def check(status, what):
+    if status != 0:
+        l = lib()
+        raise HelmError(f"{what}: {l.helm_error_string(status).decode()} ({l.helm_last_error().decode()})")

@@ -1,0 +1,156 @@
+// Internal declarations shared by the translation units of libhelm.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/helm.h"
+
+namespace helm {
+
+// ---------------------------------------------------------------- complex
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// a += b * c
+__host__ __device__ inline void cfma(float2& a, float2 b, float2 c) {
+    a.x = fmaf(b.x, c.x, fmaf(-b.y, c.y, a.x));
+    a.y = fmaf(b.x, c.y, fmaf(b.y, c.x, a.y));
+}
+__host__ __device__ inline void cfma(double2& a, double2 b, double2 c) {
+    a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+    a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+template <typename R> struct cplx;
+template <> struct cplx<float> { using type = float2; };
+template <> struct cplx<double> { using type = double2; };
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+#define HELM_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            helm::set_error(std::string(#call) + ": " + hipGetErrorString(e_));           \
+            return HELM_ERR_HIP;                                                          \
+        }                                                                                 \
+    } while (0)
+#define HELM_CHECK(cond, code, msg)                                                       \
+    do {                                                                                  \
+        if (!(cond)) {                                                                    \
+            helm::set_error(msg);                                                         \
+            return code;                                                                  \
+        }                                                                                 \
+    } while (0)
+#define HELM_TRY(call)                                                                    \
+    do {                                                                                  \
+        int32_t s_ = (call);                                                              \
+        if (s_ != HELM_OK) return s_;                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------- geometry
+// One grid level: 1-D stretch profiles and per-node coefficients.
+struct Level {
+    int nz = 0, nx = 0;
+    int64_t n = 0;
+    double h_m = 0;
+    // FP64 and FP32 copies; profile arrays: sx[nx], sz[nz], 1/sxe[nx+1], 1/sze[nz+1]
+    double2 *sx64 = nullptr, *sz64 = nullptr, *isxe64 = nullptr, *isze64 = nullptr;
+    float2 *sx32 = nullptr, *sz32 = nullptr, *isxe32 = nullptr, *isze32 = nullptr;
+    double* eta64 = nullptr;   // eta_rel [n]
+    double2* kap64 = nullptr;  // kappa_c [n]
+    float* eta32 = nullptr;
+    float2* kap32 = nullptr;
+    // multigrid work vectors (c64, nrhs * n each)
+    float2 *mg_u = nullptr, *mg_f = nullptr, *mg_r = nullptr;
+    int mg_cap = 0;  // right-hand sides the work vectors hold
+};
+
+struct Workspace {
+    int nrhs = 0, restart = 0;
+    int64_t n = 0;
+    float2* V = nullptr;     // (m+1) * nrhs * n
+    float2* Z = nullptr;     // m * nrhs * n (flexible directions)
+    float2* w = nullptr;     // nrhs * n
+    double2* r64 = nullptr;  // nrhs * n (true residual)
+    double2* partial = nullptr;  // reduction partials
+    int64_t partial_len = 0;
+    // per-RHS small state (device)
+    double2* H = nullptr;     // nrhs * m * (m+1): rotated Hessenberg columns
+    double* cs = nullptr;     // nrhs * m
+    double2* sn = nullptr;    // nrhs * m
+    double2* g = nullptr;     // nrhs * (m+1)
+    double2* coef = nullptr;  // nrhs * (m+1)
+    double* beta = nullptr;   // nrhs
+    double* bnorm = nullptr;  // nrhs
+    double* rel = nullptr;    // nrhs
+    double* inv = nullptr;    // nrhs (1/h or 1/beta)
+    double* est = nullptr;    // nrhs
+    int* active = nullptr;    // nrhs: not yet converged
+    int* cyc = nullptr;       // nrhs: active in this cycle
+    int* used = nullptr;      // nrhs: steps used in this cycle
+    int* iters = nullptr;     // nrhs: Krylov steps so far
+    // BiCGSTAB vectors (c64, nrhs * n each) and scalars
+    float2 *bp = nullptr, *bv = nullptr, *bs = nullptr, *bt = nullptr, *bphat = nullptr, *bshat = nullptr,
+           *bshadow = nullptr, *br = nullptr;
+    double2* bsc = nullptr;   // nrhs * 8 scalars
+    // host mirrors
+    std::vector<double> h_rel;
+    std::vector<int> h_active;
+    std::vector<int> h_iters;
+};
+
+struct MGConfig {
+    int levels = 0;
+    double shift = 0.5;
+    int pre = 1, post = 1, coarse = 10;
+    double omega = 0.7;
+};
+
+}  // namespace helm
+
+struct helm_plan {
+    helm_desc desc;
+    int device = 0;
+    double omega = 0, k0 = 0;
+    bool medium_set = false;
+    std::vector<helm::Level> levels;  // levels[0] = fine grid
+    helm::MGConfig mg;
+    helm::Workspace ws;
+};
+
+namespace helm {
+// level construction (helm_plan.hip)
+int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm, double h_mm, int nz, int nx);
+// stencil (helm_stencil.hip)
+int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                       const float2* b, float shift_im, hipStream_t s);
+int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                       const double2* b, hipStream_t s);
+// multigrid (helm_mg.hip)
+int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
+int32_t mg_build(helm_plan* p, hipStream_t s);
+// krylov (helm_krylov.hip)
+int32_t ensure_workspace(helm_plan* p, int nrhs, int restart);
+int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const helm_solve_opts* o,
+                    helm_solve_info* info, hipStream_t s);
+int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, const helm_solve_opts* o,
+                       helm_solve_info* info, hipStream_t s);
+int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint,
+                      hipStream_t s);
+}  // namespace helm

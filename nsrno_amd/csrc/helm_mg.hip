@@ -1,0 +1,302 @@
+// Complex-shifted-Laplacian multigrid preconditioner.
+//
+// M = A with kappa_c -> kappa_c (1 + i beta) (the same stretched-PML flux
+// operator, damped by the shift), approximately inverted by one V-cycle:
+//   * vertex-centred coarsening: coarse node J sits on fine node 2J, so an
+//     axis of n nodes coarsens to (n + 1) / 2 nodes at twice the spacing; the
+//     coarse operator is re-discretised from the same 1-D stretch profiles
+//     (evaluated at the coarse nodes and edge midpoints) and from fully
+//     weighted eta_rel / kappa_c;
+//   * damped Jacobi smoothing, full-weighting restriction and bilinear
+//     prolongation with the zero exterior ghosts of the fine operator;
+//   * a fixed number of Jacobi sweeps on the coarsest level.
+// It replaces the reference's FFT shifted-Laplacian multiplier
+// 1 / (1 - q^2 - i shift) (nsrno/wave_operator.py:93-96, applied in
+// wave_solver.py:246-247), which is exact only for a periodic
+// constant-coefficient box and so does not see the PML or the medium.
+// For adjoint solves every level uses conj(M) (M is complex symmetric).
+//
+// All kernels are HBM-bound: per sweep they read u, f (c64), the node
+// coefficients (f32 eta + c64 kappa_c, shared by all right-hand sides) and
+// write u; transfer kernels read one level and write the other.
+#include <cmath>
+
+#include "helm_coef.h"
+#include "helm_internal.h"
+
+namespace helm {
+
+// u_out = u_in + omega (f - M u_in) / dg   (u_in == nullptr: zero start)
+__global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int nrhs, const float2* __restrict__ sx,
+                                                const float2* __restrict__ sz, const float2* __restrict__ isxe,
+                                                const float2* __restrict__ isze, const float* __restrict__ eta,
+                                                const float2* __restrict__ kap, float k0sq, float hh, float shift,
+                                                int conj, float omega, const float2* __restrict__ u,
+                                                const float2* __restrict__ f, float2* __restrict__ out) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nz || j >= nx) return;
+    Coef<float2> c;
+    node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
+    const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
+    const float2 idg = make_float2(omega * c.dg.x / dd, -omega * c.dg.y / dd);
+    const int64_t idx = (int64_t)i * nx + j;
+    const bool hw = j > 0, he = j < nx - 1, hn = i > 0, hs = i < nz - 1;
+    for (int r = 0; r < nrhs; ++r) {
+        const int64_t base = (int64_t)r * n;
+        const float2 fv = f[base + idx];
+        if (u == nullptr) {
+            out[base + idx] = cmul(idg, fv);
+            continue;
+        }
+        const float2* ur = u + base;
+        const float2 uc = ur[idx];
+        float2 acc = cmul(c.dg, uc);
+        if (hw) cfma(acc, c.fw, ur[idx - 1]);
+        if (he) cfma(acc, c.fe, ur[idx + 1]);
+        if (hn) cfma(acc, c.fn, ur[idx - nx]);
+        if (hs) cfma(acc, c.fs, ur[idx + nx]);
+        float2 res = csub(fv, acc);
+        float2 nu = uc;
+        cfma(nu, idg, res);
+        out[base + idx] = nu;
+    }
+}
+
+// full weighting: fc[I,J] = sum_{a,b} w_a w_b rf[2I+a, 2J+b], w = (1/4, 1/2, 1/4)
+__global__ void k_restrict(int nzf, int nxf, int nzc, int nxc, int nrhs, const float2* __restrict__ rf,
+                           float2* __restrict__ fc) {
+    const int J = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = blockIdx.y;
+    const int r = blockIdx.z;
+    if (J >= nxc || I >= nzc || r >= nrhs) return;
+    const float2* src = rf + (int64_t)r * nzf * nxf;
+    const float wz[3] = {0.25f, 0.5f, 0.25f};
+    float2 acc = make_float2(0.f, 0.f);
+    for (int a = -1; a <= 1; ++a) {
+        const int i = 2 * I + a;
+        if (i < 0 || i >= nzf) continue;
+        for (int b = -1; b <= 1; ++b) {
+            const int j = 2 * J + b;
+            if (j < 0 || j >= nxf) continue;
+            const float w = wz[a + 1] * wz[b + 1];
+            const float2 v = src[(int64_t)i * nxf + j];
+            acc.x += w * v.x;
+            acc.y += w * v.y;
+        }
+    }
+    fc[(int64_t)r * nzc * nxc + (int64_t)I * nxc + J] = acc;
+}
+
+// bilinear prolongation, added: uf[i,j] += P uc   (coarse ghosts are zero)
+__global__ void k_prolong_add(int nzf, int nxf, int nzc, int nxc, int nrhs, const float2* __restrict__ uc,
+                              float2* __restrict__ uf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int r = blockIdx.z;
+    if (j >= nxf || i >= nzf || r >= nrhs) return;
+    const float2* src = uc + (int64_t)r * nzc * nxc;
+    const int I0 = i >> 1, J0 = j >> 1;
+    const bool oi = i & 1, oj = j & 1;
+    auto at = [&](int I, int J) -> float2 {
+        if (I >= nzc || J >= nxc) return make_float2(0.f, 0.f);
+        return src[(int64_t)I * nxc + J];
+    };
+    float2 v;
+    if (!oi && !oj) {
+        v = at(I0, J0);
+    } else if (oi && !oj) {
+        v = cscale(cadd(at(I0, J0), at(I0 + 1, J0)), 0.5f);
+    } else if (!oi && oj) {
+        v = cscale(cadd(at(I0, J0), at(I0, J0 + 1)), 0.5f);
+    } else {
+        v = cscale(cadd(cadd(at(I0, J0), at(I0 + 1, J0)), cadd(at(I0, J0 + 1), at(I0 + 1, J0 + 1))), 0.25f);
+    }
+    float2* dst = uf + (int64_t)r * nzf * nxf + (int64_t)i * nxf + j;
+    *dst = cadd(*dst, v);
+}
+
+// coarse coefficients by full weighting (exterior values replicate the edge)
+__global__ void k_coarsen_medium(int nzf, int nxf, int nzc, int nxc, const double* __restrict__ etaf,
+                                 const double2* __restrict__ kapf, double* eta64, double2* kap64, float* eta32,
+                                 float2* kap32) {
+    const int J = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = blockIdx.y;
+    if (J >= nxc || I >= nzc) return;
+    const double wz[3] = {0.25, 0.5, 0.25};
+    double e = 0.0;
+    double2 k = make_double2(0.0, 0.0);
+    for (int a = -1; a <= 1; ++a) {
+        const int i = min(max(2 * I + a, 0), nzf - 1);
+        for (int b = -1; b <= 1; ++b) {
+            const int j = min(max(2 * J + b, 0), nxf - 1);
+            const double w = wz[a + 1] * wz[b + 1];
+            e += w * etaf[(int64_t)i * nxf + j];
+            const double2 kv = kapf[(int64_t)i * nxf + j];
+            k.x += w * kv.x;
+            k.y += w * kv.y;
+        }
+    }
+    const int64_t idx = (int64_t)I * nxc + J;
+    eta64[idx] = e;
+    kap64[idx] = k;
+    eta32[idx] = (float)e;
+    kap32[idx] = make_float2((float)k.x, (float)k.y);
+}
+
+__global__ void k_fill_c64(int64_t count, float2* p) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = make_float2(0.f, 0.f);
+}
+
+static int32_t level_alloc_work(Level& L, int nrhs) {
+    if (L.mg_cap >= nrhs) return HELM_OK;
+    const size_t bytes = (size_t)nrhs * L.n * sizeof(float2) + 256;
+    float2** bufs[3] = {&L.mg_u, &L.mg_f, &L.mg_r};
+    for (float2** b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+        hipError_t e = hipMalloc((void**)b, bytes);
+        if (e != hipSuccess) {
+            *b = nullptr;
+            L.mg_cap = 0;
+            set_error(std::string("hipMalloc (multigrid): ") + hipGetErrorString(e));
+            return HELM_ERR_ALLOC;
+        }
+    }
+    L.mg_cap = nrhs;
+    return HELM_OK;
+}
+
+int32_t mg_build(helm_plan* p, hipStream_t s) {
+    for (size_t l = 1; l < p->levels.size(); ++l) {
+        Level& F = p->levels[l - 1];
+        Level& C = p->levels[l];
+        dim3 grid((C.nx + 63) / 64, C.nz);
+        hipLaunchKernelGGL(k_coarsen_medium, grid, dim3(64), 0, s, F.nz, F.nx, C.nz, C.nx, F.eta64, F.kap64, C.eta64,
+                           C.kap64, C.eta32, C.kap32);
+        HELM_HIP(hipGetLastError());
+    }
+    return HELM_OK;
+}
+
+static int32_t jacobi(helm_plan* p, const Level& L, double shift, bool adj, const float2* u, const float2* f,
+                      float2* out, int nrhs, hipStream_t s) {
+    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+    const double k0sq = p->k0 * p->k0;
+    hipLaunchKernelGGL(k_jacobi, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32, L.isxe32, L.isze32,
+                       L.eta32, L.kap32, (float)k0sq, (float)(0.5 / (L.h_m * L.h_m)), (float)shift, (int)adj,
+                       (float)p->mg.omega, u, f, out);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// Solve M_l u = f approximately; f and u are level-l arrays (u may be the
+// caller's output on level 0).  Uses L.mg_r as scratch and L.mg_u for
+// Jacobi ping-pong.
+static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s) {
+    Level& L = p->levels[l];
+    const MGConfig& c = p->mg;
+    const bool coarsest = l + 1 == (int)p->levels.size();
+    const int sweeps = coarsest ? c.coarse : c.pre;
+    // pre-smoothing from a zero guess; ping-pong between u and L.mg_u so the
+    // result ends in u
+    float2* bufs[2] = {u, L.mg_u};
+    int cur = -1;  // index of the buffer holding the iterate (-1: zero)
+    for (int k = 0; k < sweeps; ++k) {
+        const int dst = (sweeps - k) % 2 == 1 ? 0 : 1;  // last sweep writes u
+        HELM_TRY(jacobi(p, L, c.shift, adj, cur < 0 ? nullptr : bufs[cur], f, bufs[dst], nrhs, s));
+        cur = dst;
+    }
+    if (cur < 0) {  // no sweeps: u = 0
+        const int64_t cnt = (int64_t)nrhs * L.n;
+        hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, cnt, u);
+        HELM_HIP(hipGetLastError());
+    }
+    if (coarsest) return HELM_OK;
+    Level& C = p->levels[l + 1];
+    // residual r = f - M u
+    HELM_TRY(launch_apply32(L, p->k0 * p->k0, u, L.mg_r, nrhs, adj, f, (float)c.shift, s));
+    hipLaunchKernelGGL(k_restrict, dim3((C.nx + 63) / 64, C.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs,
+                       L.mg_r, C.mg_f);
+    HELM_HIP(hipGetLastError());
+    HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_r, nrhs, adj, s));  // coarse solution in C.mg_r
+    hipLaunchKernelGGL(k_prolong_add, dim3((L.nx + 63) / 64, L.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx,
+                       nrhs, C.mg_r, u);
+    HELM_HIP(hipGetLastError());
+    // post-smoothing, ping-pong ending in u
+    const float2* src = u;
+    if (c.post % 2 == 1) {
+        HELM_HIP(hipMemcpyAsync(L.mg_u, u, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+        src = L.mg_u;
+    }
+    for (int k = 0; k < c.post; ++k) {
+        float2* dst = (src == u) ? L.mg_u : u;
+        HELM_TRY(jacobi(p, L, c.shift, adj, src, f, dst, nrhs, s));
+        src = dst;
+    }
+    return HELM_OK;
+}
+
+int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+    HELM_CHECK(p->mg.levels > 0, HELM_ERR_STATE, "multigrid not set up");
+    for (auto& L : p->levels) HELM_TRY(level_alloc_work(L, nrhs));
+    return vcycle(p, 0, v, z, nrhs, adjoint, s);
+}
+
+int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+    if (kind == HELM_PC_MG) return mg_vcycle(p, v, z, nrhs, adjoint, s);
+    set_error("preconditioner kind not available");
+    return HELM_ERR_STATE;
+}
+
+}  // namespace helm
+
+using namespace helm;
+
+extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int32_t pre, int32_t post,
+                                 int32_t coarse_sweeps, double omega, void* stream) {
+    HELM_CHECK(p, HELM_ERR_ARG, "helm_mg_setup: null plan");
+    HELM_CHECK(levels >= 1 && levels <= 12, HELM_ERR_ARG, "helm_mg_setup: levels must be in [1, 12]");
+    HELM_CHECK(std::isfinite(shift) && shift >= 0, HELM_ERR_ARG, "helm_mg_setup: shift must be >= 0");
+    HELM_CHECK(pre >= 0 && post >= 0 && coarse_sweeps >= 0 && pre + post + coarse_sweeps > 0, HELM_ERR_ARG,
+               "helm_mg_setup: sweep counts must be >= 0 and not all zero");
+    HELM_CHECK(std::isfinite(omega) && omega > 0 && omega <= 1.5, HELM_ERR_ARG,
+               "helm_mg_setup: omega must be in (0, 1.5]");
+    hipStream_t s = (hipStream_t)stream;
+    // drop old coarse levels, build new ones down to >= 3 nodes per axis
+    while (p->levels.size() > 1) {
+        Level& L = p->levels.back();
+        void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
+                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        p->levels.pop_back();
+    }
+    const helm_desc& d = p->desc;
+    for (int l = 1; l < levels; ++l) {
+        const Level& F = p->levels.back();
+        const int nzc = (F.nz + 1) / 2, nxc = (F.nx + 1) / 2;
+        if (nzc < 3 || nxc < 3) break;
+        p->levels.emplace_back();
+        const double hc = d.h_mm * (double)(1 << l);
+        HELM_TRY(build_level_profiles(p, p->levels.back(), d.x0_mm, d.z0_mm, hc, nzc, nxc));
+    }
+    p->mg.levels = (int)p->levels.size();
+    p->mg.shift = shift;
+    p->mg.pre = pre;
+    p->mg.post = post;
+    p->mg.coarse = coarse_sweeps;
+    p->mg.omega = omega;
+    if (p->medium_set) HELM_TRY(mg_build(p, s));
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_precond_apply(helm_plan* p, int32_t precond, const void* v, void* z, int32_t nrhs,
+                                      void* stream) {
+    HELM_CHECK(p && v && z, HELM_ERR_ARG, "helm_precond_apply: null argument");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_precond_apply: call helm_set_medium first");
+    HELM_CHECK(nrhs >= 1 && nrhs <= 65535, HELM_ERR_ARG, "helm_precond_apply: nrhs out of range");
+    HELM_CHECK(v != z, HELM_ERR_ARG, "helm_precond_apply: v and z must not alias");
+    return precond_apply(p, precond, (const float2*)v, (float2*)z, nrhs, false, (hipStream_t)stream);
+}

@@ -1,0 +1,214 @@
+"""Device-side Helmholtz plan: operator, ports and Krylov solves on one GPU.
+
+``HelmholtzPlan`` owns one ``helm_plan`` of libhelm.so.  Fields are torch
+tensors on the GPU in [rhs, z, x] layout; every call is enqueued on torch's
+current HIP stream.  The operator is the flux-form stretched-PML Helmholtz
+matrix of nsrno/theranostic/wave.py:174-188 (complex symmetric,
+exp(-i omega t)), extended by a complex kappa_c for attenuation.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+from scipy import sparse
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass
+class SolveInfo:
+    """Per right-hand-side result of a solve (keys of wave_solver.py:65-67)."""
+    reason: str
+    iterations: int
+    applications: int
+    cycles: int
+    relative_residual: float
+
+
+class Ports:
+    """Sparse real port weights W (N x P) laid out for the device.
+
+    Node-major CSR drives injection (rhs = -W a / h^2, wave.py:260), port-major
+    CSC drives sampling (W^T u, wave.py:275-278).
+    """
+
+    def __init__(self, weights, device="cuda"):
+        w = sparse.csc_matrix(weights, dtype=np.float64)
+        if np.iscomplexobj(weights):
+            raise ValueError("port weights must be real")
+        self.n_nodes_total, self.n_ports = w.shape
+        csr = w.tocsr()
+        csr.sort_indices()
+        counts = np.diff(csr.indptr)
+        nodes = np.flatnonzero(counts)
+        ptr = np.concatenate(([0], np.cumsum(counts[nodes]))).astype(np.int32)
+        dev = torch.device(device)
+        self.node_ptr = torch.from_numpy(ptr).to(dev)
+        self.node_idx = torch.from_numpy(nodes.astype(np.int32)).to(dev)
+        self.port_of_tap = torch.from_numpy(csr.indices.astype(np.int32)).to(dev)
+        self.w_node = torch.from_numpy(csr.data.astype(np.float64)).to(dev)
+        self.n_nodes = int(nodes.size)
+        w.sort_indices()
+        self.port_ptr = torch.from_numpy(w.indptr.astype(np.int32)).to(dev)
+        self.node_of_tap = torch.from_numpy(w.indices.astype(np.int32)).to(dev)
+        self.w_port = torch.from_numpy(w.data.astype(np.float64)).to(dev)
+
+
+class HelmholtzPlan:
+    """One grid + frequency + PML on the GPU (helm_plan_create / helm_set_medium)."""
+
+    def __init__(self, nz, nx, h_mm, frequency_mhz, pml_mm, *, x0_mm=0.0, z0_mm=0.0, c0_m_s=1540.0,
+                 pml_strength=12.0, nrhs=1, restart=30, device="cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HelmholtzPlan needs a GPU (no CPU fallback)")
+        self.device = torch.device(device)
+        self.nz, self.nx, self.n = int(nz), int(nx), int(nz) * int(nx)
+        self.h_mm, self.h_m = float(h_mm), float(h_mm) * 1e-3
+        self.frequency_mhz = float(frequency_mhz)
+        self.omega = 2 * np.pi * self.frequency_mhz * 1e6
+        self.c0 = float(c0_m_s)
+        self.k0 = self.omega / self.c0
+        self.nrhs_capacity, self.restart_capacity = int(nrhs), int(restart)
+        desc = _lib.HelmDesc(self.nz, self.nx, int(nrhs), int(restart), float(x0_mm), float(z0_mm),
+                             float(h_mm), float(pml_mm), float(frequency_mhz), float(c0_m_s), float(pml_strength))
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib().helm_plan_create(ctypes.byref(desc), ctypes.byref(handle)), "helm_plan_create")
+        self._h = handle
+        self.mg_levels = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().helm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ medium
+    def _field(self, a, dtype):
+        t = torch.as_tensor(a, device=self.device, dtype=dtype)
+        if t.numel() != self.n:
+            raise ValueError(f"medium map must have {self.n} values, got {t.numel()}")
+        return t.reshape(self.nz, self.nx).contiguous()
+
+    def set_medium(self, eta_rel, kappa_c):
+        """eta_rel = 1 + m_eta (real), kappa_c = (1 + m_kappa) * attenuation factor (complex)."""
+        self._eta = self._field(eta_rel, torch.float64)
+        self._kap = self._field(kappa_c, torch.complex128)
+        check(lib().helm_set_medium(self._h, _ptr(self._eta), _ptr(self._kap), _stream()), "helm_set_medium")
+
+    def set_medium_physical(self, c_m_s, rho_kg_m3, alpha_np_m=None, rho0=1000.0):
+        c = self._field(c_m_s, torch.float64)
+        rho = self._field(rho_kg_m3, torch.float64)
+        alpha = None if alpha_np_m is None else self._field(alpha_np_m, torch.float64)
+        self._phys = (c, rho, alpha)
+        check(lib().helm_set_medium_physical(self._h, _ptr(c), _ptr(rho), None if alpha is None else _ptr(alpha),
+                                              float(rho0), _stream()), "helm_set_medium_physical")
+
+    # ------------------------------------------------------------ operator
+    def _check_field(self, u):
+        if u.device.type != "cuda" or not u.is_contiguous():
+            raise ValueError("fields must be contiguous GPU tensors")
+        if u.dtype not in (torch.complex64, torch.complex128):
+            raise ValueError("fields must be complex64 or complex128")
+        if u.numel() % self.n:
+            raise ValueError(f"field size {u.numel()} is not a multiple of {self.n}")
+        return u.numel() // self.n
+
+    def apply(self, u, adjoint=False, out=None):
+        """A u (or A^H u) for fields [..., nz, nx]: FP32 for complex64, FP64 for complex128."""
+        nrhs = self._check_field(u)
+        out = torch.empty_like(u) if out is None else out
+        prec = 32 if u.dtype == torch.complex64 else 64
+        check(lib().helm_apply(self._h, _ptr(u), _ptr(out), nrhs, prec, int(bool(adjoint)), _stream()),
+              "helm_apply")
+        return out
+
+    # ------------------------------------------------------------ ports
+    def inject(self, ports: Ports, amplitudes):
+        """rhs[r] = -(1/h^2) W @ amplitudes[r]  ->  complex128 [R, nz, nx]."""
+        amp = torch.as_tensor(amplitudes, dtype=torch.complex128, device=self.device)
+        if amp.ndim == 1:
+            amp = amp[None]
+        amp = amp.contiguous()
+        if amp.shape[1] != ports.n_ports:
+            raise ValueError("amplitudes must be [nrhs, n_ports]")
+        nrhs = amp.shape[0]
+        rhs = torch.empty((nrhs, self.nz, self.nx), dtype=torch.complex128, device=self.device)
+        check(lib().helm_inject(self._h, ports.n_ports, ports.n_nodes, _ptr(ports.node_ptr), _ptr(ports.node_idx),
+                                _ptr(ports.port_of_tap), _ptr(ports.w_node), _ptr(amp), _ptr(rhs), nrhs, _stream()),
+              "helm_inject")
+        return rhs
+
+    def sample(self, ports: Ports, field):
+        """W^T u for every field -> complex128 [R, n_ports]."""
+        nrhs = self._check_field(field)
+        traces = torch.empty((nrhs, ports.n_ports), dtype=torch.complex128, device=self.device)
+        prec = 32 if field.dtype == torch.complex64 else 64
+        check(lib().helm_sample(self._h, ports.n_ports, _ptr(ports.port_ptr), _ptr(ports.node_of_tap),
+                                _ptr(ports.w_port), _ptr(field), prec, _ptr(traces), nrhs, _stream()), "helm_sample")
+        return traces
+
+    # ------------------------------------------------------------ preconditioners
+    def setup_multigrid(self, levels=4, shift=0.5, pre=2, post=2, coarse_sweeps=20, omega=0.6):
+        check(lib().helm_mg_setup(self._h, int(levels), float(shift), int(pre), int(post), int(coarse_sweeps),
+                                  float(omega), _stream()), "helm_mg_setup")
+        self.mg_levels = int(levels)
+
+    def precondition(self, v, kind=_lib.PC_MG):
+        nrhs = self._check_field(v)
+        if v.dtype != torch.complex64:
+            raise ValueError("preconditioner input must be complex64")
+        z = torch.empty_like(v)
+        check(lib().helm_precond_apply(self._h, int(kind), _ptr(v), _ptr(z), nrhs, _stream()), "helm_precond_apply")
+        return z
+
+    # ------------------------------------------------------------ solve
+    def solve(self, rhs, x0=None, *, tolerance=1e-6, restart=None, max_iterations=3000, precond="none",
+              method="gmres", adjoint=False, callback=None):
+        """Krylov solve of A x = rhs (A^H x = rhs) for all right-hand sides.
+
+        rhs: complex128 [R, nz, nx] (converted if needed).  Returns
+        (x complex128 [R, nz, nx], [SolveInfo] * R).
+        """
+        rhs = torch.as_tensor(rhs, device=self.device).to(torch.complex128).contiguous()
+        nrhs = self._check_field(rhs)
+        if nrhs > self.nrhs_capacity:
+            raise ValueError(f"nrhs {nrhs} exceeds the plan capacity {self.nrhs_capacity}")
+        restart = self.restart_capacity if restart is None else int(restart)
+        if x0 is None:
+            x = torch.empty_like(rhs)
+        else:
+            x = torch.as_tensor(x0, device=self.device).to(torch.complex128).contiguous().clone()
+        pc = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED}[precond]
+        meth = {"gmres": _lib.HELM_GMRES, "bicgstab": _lib.HELM_BICGSTAB}[method]
+
+        def _cb(user, applications, rel, n):
+            if callback is not None:
+                callback(int(applications), [rel[i] for i in range(n)])
+
+        cfun = _lib.CALLBACK(_cb)
+        opts = _lib.HelmSolveOpts(meth, pc, restart, int(max_iterations), float(tolerance), int(bool(adjoint)),
+                                  int(x0 is not None), cfun, None)
+        infos = (_lib.HelmSolveInfo * nrhs)()
+        check(lib().helm_solve(self._h, _ptr(rhs), _ptr(x), nrhs, ctypes.byref(opts), infos, _stream()),
+              "helm_solve")
+        out = [SolveInfo(_lib.REASONS.get(i.reason, "unknown"), i.iterations, i.applications, i.cycles,
+                         i.relative_residual) for i in infos]
+        return x, out

@@ -1,0 +1,83 @@
+"""GPU: multigrid preconditioner and BiCGSTAB against the CPU oracle."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+from scipy.sparse.linalg import splu
+
+from oracle import helmholtz as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = np.load(Path(__file__).parent / "golden" / "theranostic_small.npz")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def setup(nrhs=4, levels=3):
+    from nsrno_amd.helmholtz import HelmholtzPlan, Ports
+    axis = GOLDEN["axis"]
+    grid = O.Grid(axis, axis, 6.0)
+    f = float(GOLDEN["frequency_mhz"])
+    plan = HelmholtzPlan(grid.shape[0], grid.shape[1], grid.h_mm, f, 6.0, x0_mm=axis[0], z0_mm=axis[0],
+                         nrhs=nrhs, restart=30)
+    plan.set_medium(1 + GOLDEN["m_eta"], (1 + GOLDEN["m_kappa"]).astype(complex))
+    plan.setup_multigrid(levels=levels, shift=0.5, pre=2, post=2, coarse_sweeps=30, omega=0.6)
+    w = O.source_weights(grid, GOLDEN["positions"], GOLDEN["normals"], GOLDEN["widths"])
+    rhs = plan.inject(Ports(w), np.eye(4)[:nrhs])
+    return grid, f, plan, w, rhs
+
+
+def test_vcycle_iteration_solves_shifted_operator():
+    """Richardson iteration with the V-cycle converges to the direct solution
+    of the shifted operator A(kappa_c (1 + 0.5 i)) (oracle splu)."""
+    grid, f, plan, w, rhs = setup(nrhs=1)
+    shifted = O.assemble(grid, f, GOLDEN["m_kappa"], GOLDEN["m_eta"], kappa_factor=np.full(grid.size, 1 + 0.5j))
+    b = rhs.cpu().numpy().ravel()
+    exact = splu(shifted.tocsc()).solve(b)
+    x = np.zeros_like(b)
+    scale = np.linalg.norm(b)
+    for _ in range(60):
+        r = (b - shifted @ x) / scale
+        z = plan.precondition(torch.from_numpy(r.astype(np.complex64).reshape(1, *grid.shape)).cuda())
+        x = x + scale * z.cpu().numpy().ravel().astype(complex)
+    assert rel(x, exact) < 1e-4
+    # linear and deterministic
+    v = torch.randn(2, *grid.shape, dtype=torch.complex64, device="cuda")
+    z1 = plan.precondition(v)
+    z2 = plan.precondition(v)
+    assert torch.equal(z1, z2)
+    z3 = plan.precondition((2 * v).contiguous())
+    assert rel(z3.cpu().numpy(), 2 * z1.cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("method,precond", [("gmres", "mg"), ("bicgstab", "none"), ("bicgstab", "mg")])
+def test_preconditioned_and_bicgstab_solves_match_direct(method, precond):
+    grid, f, plan, w, rhs = setup()
+    x, infos = plan.solve(rhs, tolerance=1e-10, precond=precond, method=method, max_iterations=20000)
+    assert all(i.reason == "converged" for i in infos), infos
+    fields = x.cpu().numpy().reshape(4, -1).T
+    assert rel(fields, GOLDEN["fields"]) < 1e-7
+    a = O.assemble(grid, f, GOLDEN["m_kappa"], GOLDEN["m_eta"])
+    b = rhs.cpu().numpy().reshape(4, -1).T
+    assert rel(a @ fields, b) < 1e-10
+
+
+def test_multigrid_reduces_iterations():
+    grid, f, plan, w, rhs = setup()
+    _, plain = plan.solve(rhs, tolerance=1e-8, precond="none", max_iterations=20000)
+    _, mg = plan.solve(rhs, tolerance=1e-8, precond="mg", max_iterations=20000)
+    assert all(i.reason == "converged" for i in plain + mg)
+    assert max(i.iterations for i in mg) < 0.5 * max(i.iterations for i in plain)
+
+
+def test_adjoint_mg_solve():
+    grid, f, plan, w, rhs = setup(nrhs=2)
+    x, infos = plan.solve(rhs, tolerance=1e-10, precond="mg", adjoint=True, max_iterations=20000)
+    a = O.assemble(grid, f, GOLDEN["m_kappa"], GOLDEN["m_eta"])
+    xs = x.cpu().numpy().reshape(2, -1).T
+    b = rhs.cpu().numpy().reshape(2, -1).T
+    assert all(i.reason == "converged" for i in infos)
+    assert rel(a.conj().T @ xs, b) < 1e-10

@@ -1,0 +1,81 @@
+"""Host-side logic of the drop-in (CPU only): validation, port weights, layouts."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy import sparse
+
+from nsrno_amd import theranostic as T
+from nsrno_amd.helmholtz import Ports
+from oracle import helmholtz as O
+
+GOLDEN = np.load(Path(__file__).parent / "golden" / "theranostic_small.npz")
+
+
+def test_source_weights_match_reference_golden():
+    axis = GOLDEN["axis"]
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    w = solver.source_weights(GOLDEN["positions"], GOLDEN["normals"], GOLDEN["widths"])
+    g = sparse.csc_matrix((GOLDEN["weights_val"], (GOLDEN["weights_row"], GOLDEN["weights_col"])), shape=w.shape)
+    assert abs(w - g).max() < 1e-15
+    # point elements (zero width) as in the Green's-function test
+    pos = np.array([[-3.2, 0.0], [3.2, 0.0], [0.0, 4.8]])
+    w3 = solver.source_weights(pos, np.tile([0.0, 1.0], (3, 1)), np.zeros(3))
+    o3 = O.source_weights(O.Grid(axis, axis, 6.0), pos, np.tile([0.0, 1.0], (3, 1)), np.zeros(3))
+    assert abs(w3 - o3).max() < 1e-15
+
+
+@pytest.mark.parametrize("bad", [
+    dict(positions_mm=np.zeros((2, 3)), normals=np.zeros((2, 3)), widths_mm=np.zeros(2)),
+    dict(positions_mm=np.zeros((1, 2)), normals=np.zeros((1, 2)), widths_mm=np.ones(1)),
+    dict(positions_mm=np.array([[-15.9, 0.0]]), normals=np.array([[0.0, 1.0]]), widths_mm=np.zeros(1)),
+])
+def test_source_weights_validation(bad):
+    axis = GOLDEN["axis"]
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    with pytest.raises(ValueError):
+        solver.source_weights(**bad)
+
+
+def test_grid_and_contrast_validation():
+    with pytest.raises(ValueError):
+        T.Grid(np.array([0.0, 1.0]), np.array([0.0, 1.0, 2.0]), 0.0)
+    with pytest.raises(ValueError):
+        T.Grid(np.linspace(0, 1, 5), np.linspace(0, 2, 5), 0.0)
+    with pytest.raises(ValueError):
+        T.Grid(np.linspace(0, 1, 5), np.linspace(0, 1, 5), 0.5)
+    axis = GOLDEN["axis"]
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    bad = np.zeros(solver.grid.shape)
+    bad[20, 20] = -1.0
+    with pytest.raises(ValueError):
+        solver._contrast(bad, "m_kappa")
+    bad[:] = 0
+    bad[0, 0] = 0.01
+    with pytest.raises(ValueError, match="vanish in the PML"):
+        solver._contrast(bad, "m_eta")
+    np.testing.assert_array_equal(solver.grid.pml_mask, O.Grid(axis, axis, 6.0).pml_mask)
+    with pytest.raises(ValueError):
+        solver.solve(0.15, 1j * np.ones((solver.grid.size, 1)))
+
+
+def test_ports_layout_reproduces_weights():
+    axis = GOLDEN["axis"]
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    w = solver.source_weights(GOLDEN["positions"], GOLDEN["normals"], GOLDEN["widths"])
+    p = Ports(w, device="cpu")
+    rng = np.random.default_rng(0)
+    amp = rng.normal(size=w.shape[1]) + 1j * rng.normal(size=w.shape[1])
+    # node-major CSR gather equals W @ amp
+    out = np.zeros(w.shape[0], complex)
+    ptr, nodes = p.node_ptr.numpy(), p.node_idx.numpy()
+    for q in range(p.n_nodes):
+        s = slice(ptr[q], ptr[q + 1])
+        out[nodes[q]] = np.sum(p.w_node.numpy()[s] * amp[p.port_of_tap.numpy()[s]])
+    np.testing.assert_allclose(out, w @ amp, rtol=1e-15, atol=1e-15)
+    # port-major CSC gather equals W^T u
+    u = rng.normal(size=w.shape[0]) + 1j * rng.normal(size=w.shape[0])
+    pp = p.port_ptr.numpy()
+    tr = [np.sum(p.w_port.numpy()[pp[k]:pp[k + 1]] * u[p.node_of_tap.numpy()[pp[k]:pp[k + 1]]])
+          for k in range(w.shape[1])]
+    np.testing.assert_allclose(tr, w.T @ u, rtol=1e-14)

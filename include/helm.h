@@ -145,8 +145,10 @@ int32_t helm_solve(helm_plan* plan, const void* rhs, void* x, int32_t nrhs,
 
 /* ---- preconditioners --------------------------------------------------- */
 /* Complex-shifted-Laplacian multigrid: kappa_c -> kappa_c (1 + i shift).
- * levels >= 1 (1: smoother only), pre/post damped-Jacobi sweeps with weight
- * omega, coarse sweeps on the coarsest level. */
+ * levels >= 1 (1: smoother only), pre/post Jacobi sweeps, coarse sweeps on
+ * the coarsest level.  omega = 0 picks a complex Jacobi weight per node
+ * from the local stencil's Fourier symbol (convergent on coarse Helmholtz
+ * grids and in the PML); omega in (0, 1.5] forces that real weight. */
 int32_t helm_mg_setup(helm_plan* plan, int32_t levels, double shift, int32_t pre, int32_t post,
                       int32_t coarse_sweeps, double omega, void* stream);
 /* z = M^{-1} v for nrhs c64 fields (the preconditioner the solve applies). */

@@ -79,6 +79,7 @@ struct Level {
     // multigrid work vectors (c64, nrhs * n each)
     float2 *mg_u = nullptr, *mg_f = nullptr, *mg_r = nullptr;
     int mg_cap = 0;  // right-hand sides the work vectors hold
+    float2* wgt = nullptr;  // per-node complex Jacobi weight
 };
 
 struct Workspace {
@@ -119,7 +120,7 @@ struct MGConfig {
     int levels = 0;
     double shift = 0.5;
     int pre = 1, post = 1, coarse = 10;
-    double omega = 0.7;
+    double omega = 0.0;  // 0: per-node complex weights; > 0: this real weight everywhere
 };
 
 }  // namespace helm

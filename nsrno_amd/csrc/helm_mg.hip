@@ -26,12 +26,78 @@
 
 namespace helm {
 
-// u_out = u_in + omega (f - M u_in) / dg   (u_in == nullptr: zero start)
+// Per-node complex Jacobi weight.  Local Fourier analysis of one sweep at a
+// node with stencil (dg, fw, fe, fn, fs): a mode with cos(theta_x) = t1,
+// cos(theta_z) = t2 is multiplied by g = 1 - w z,
+//   z = 1 + t1 a + t2 b,   a = (fw + fe)/dg,  b = (fn + fs)/dg.
+// |g| is convex in z, so its maximum over a rectangle of (t1, t2) sits at a
+// corner.  The weight is picked from 33 x 32 candidates w = r e^{i phi}
+// (phi in [-pi/2, pi/2], r in (0, 2]): on smoothing levels it minimises the
+// worst |g| over the oscillatory modes (t1 <= 0 or t2 <= 0) among weights
+// that amplify no mode by more than 10%; on the coarsest level (and where no
+// weight meets the 10% cap) it minimises the worst |g| over all modes.  A
+// real weight cannot do this for Helmholtz: modes below the wavenumber have
+// Re z < 0, and the PML's complex stretch makes the stencil anisotropic, so
+// plain damped Jacobi diverges there and on coarse grids.
+__global__ __launch_bounds__(256) void k_node_weights(int nz, int nx, const float2* __restrict__ sx,
+                                                      const float2* __restrict__ sz, const float2* __restrict__ isxe,
+                                                      const float2* __restrict__ isze, const float* __restrict__ eta,
+                                                      const float2* __restrict__ kap, float k0sq, float hh,
+                                                      float shift, int coarsest, float2* __restrict__ wgt) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nz || j >= nx) return;
+    Coef<float2> c;
+    node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, false, c);
+    const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
+    const float2 idg = make_float2(c.dg.x / dd, -c.dg.y / dd);
+    const float2 a = cmul(cadd(c.fw, c.fe), idg), b = cmul(cadd(c.fn, c.fs), idg);
+    // corners: 4 of the full square, then the oscillatory L-shaped region
+    const float t1s[11] = {-1, -1, 1, 1, -1, -1, 0, 0, -1, 1, 1};
+    const float t2s[11] = {-1, 1, -1, 1, -1, 1, -1, 1, 0, -1, 0};
+    float2 z[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) z[k] = make_float2(1.f + t1s[k] * a.x + t2s[k] * b.x, t1s[k] * a.y + t2s[k] * b.y);
+    float best = 1e30f;
+    float2 bw = make_float2(0.5f, 0.f);
+    for (int ip = 0; ip <= 32; ++ip) {
+        float sphi, cphi;
+        sincosf(-1.5707963f + 3.1415927f * ip / 32.f, &sphi, &cphi);
+        for (int ir = 1; ir <= 32; ++ir) {
+            const float r = 2.f * ir / 32.f;
+            const float wr = r * cphi, wi = r * sphi;
+            float full = 0.f, hi = 0.f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) {
+                const float gr = 1.f - (wr * z[k].x - wi * z[k].y);
+                const float gi = -(wr * z[k].y + wi * z[k].x);
+                const float g2 = gr * gr + gi * gi;
+                if (k < 4) full = fmaxf(full, g2);
+                else hi = fmaxf(hi, g2);
+            }
+            hi = fmaxf(hi, 0.f);
+            const float score = coarsest ? full : (full <= 1.21f ? hi : 1e3f + full);
+            if (score < best) {
+                best = score;
+                bw = make_float2(wr, wi);
+            }
+        }
+    }
+    wgt[(int64_t)i * nx + j] = bw;
+}
+
+__global__ void k_fill_weight(int64_t n, float omega, float2* __restrict__ wgt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wgt[i] = make_float2(omega, 0.f);
+}
+
+// u_out = u_in + w (f - M u_in) / dg with the node's complex weight w
+// (u_in == nullptr: zero start)
 __global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int nrhs, const float2* __restrict__ sx,
                                                 const float2* __restrict__ sz, const float2* __restrict__ isxe,
                                                 const float2* __restrict__ isze, const float* __restrict__ eta,
                                                 const float2* __restrict__ kap, float k0sq, float hh, float shift,
-                                                int conj, float omega, const float2* __restrict__ u,
+                                                int conj, const float2* __restrict__ wgt, const float2* __restrict__ u,
                                                 const float2* __restrict__ f, float2* __restrict__ out) {
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 4 + threadIdx.y;
@@ -39,8 +105,9 @@ __global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int n
     Coef<float2> c;
     node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
     const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
-    const float2 idg = make_float2(omega * c.dg.x / dd, -omega * c.dg.y / dd);
     const int64_t idx = (int64_t)i * nx + j;
+    const float2 wn = wgt[idx];
+    const float2 idg = cmul(conj ? cconj(wn) : wn, make_float2(c.dg.x / dd, -c.dg.y / dd));
     const bool hw = j > 0, he = j < nx - 1, hn = i > 0, hs = i < nz - 1;
     for (int r = 0; r < nrhs; ++r) {
         const int64_t base = (int64_t)r * n;
@@ -177,16 +244,31 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                            C.kap64, C.eta32, C.kap32);
         HELM_HIP(hipGetLastError());
     }
+    for (size_t l = 0; l < p->levels.size(); ++l) {
+        Level& L = p->levels[l];
+        if (!L.wgt) HELM_HIP(hipMalloc((void**)&L.wgt, L.n * sizeof(float2) + 256));
+        if (p->mg.omega > 0) {
+            hipLaunchKernelGGL(k_fill_weight, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.n,
+                               (float)p->mg.omega, L.wgt);
+        } else {
+            dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+            hipLaunchKernelGGL(k_node_weights, grid, block, 0, s, L.nz, L.nx, L.sx32, L.sz32, L.isxe32, L.isze32,
+                               L.eta32, L.kap32, (float)(p->k0 * p->k0), (float)(0.5 / (L.h_m * L.h_m)),
+                               (float)p->mg.shift, (int)(l + 1 == p->levels.size()), L.wgt);
+        }
+        HELM_HIP(hipGetLastError());
+    }
     return HELM_OK;
 }
 
-static int32_t jacobi(helm_plan* p, const Level& L, double shift, bool adj, const float2* u, const float2* f,
+static int32_t jacobi(helm_plan* p, int l, double shift, bool adj, const float2* u, const float2* f,
                       float2* out, int nrhs, hipStream_t s) {
+    const Level& L = p->levels[l];  // conj(M) uses conj(w): the same contraction
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
     const double k0sq = p->k0 * p->k0;
     hipLaunchKernelGGL(k_jacobi, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32, L.isxe32, L.isze32,
                        L.eta32, L.kap32, (float)k0sq, (float)(0.5 / (L.h_m * L.h_m)), (float)shift, (int)adj,
-                       (float)p->mg.omega, u, f, out);
+                       L.wgt, u, f, out);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
@@ -205,7 +287,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     int cur = -1;  // index of the buffer holding the iterate (-1: zero)
     for (int k = 0; k < sweeps; ++k) {
         const int dst = (sweeps - k) % 2 == 1 ? 0 : 1;  // last sweep writes u
-        HELM_TRY(jacobi(p, L, c.shift, adj, cur < 0 ? nullptr : bufs[cur], f, bufs[dst], nrhs, s));
+        HELM_TRY(jacobi(p, l, c.shift, adj, cur < 0 ? nullptr : bufs[cur], f, bufs[dst], nrhs, s));
         cur = dst;
     }
     if (cur < 0) {  // no sweeps: u = 0
@@ -232,7 +314,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     }
     for (int k = 0; k < c.post; ++k) {
         float2* dst = (src == u) ? L.mg_u : u;
-        HELM_TRY(jacobi(p, L, c.shift, adj, src, f, dst, nrhs, s));
+        HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
         src = dst;
     }
     return HELM_OK;
@@ -261,14 +343,14 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     HELM_CHECK(std::isfinite(shift) && shift >= 0, HELM_ERR_ARG, "helm_mg_setup: shift must be >= 0");
     HELM_CHECK(pre >= 0 && post >= 0 && coarse_sweeps >= 0 && pre + post + coarse_sweeps > 0, HELM_ERR_ARG,
                "helm_mg_setup: sweep counts must be >= 0 and not all zero");
-    HELM_CHECK(std::isfinite(omega) && omega > 0 && omega <= 1.5, HELM_ERR_ARG,
-               "helm_mg_setup: omega must be in (0, 1.5]");
+    HELM_CHECK(std::isfinite(omega) && omega >= 0 && omega <= 1.5, HELM_ERR_ARG,
+               "helm_mg_setup: omega must be 0 (automatic complex weights) or in (0, 1.5]");
     hipStream_t s = (hipStream_t)stream;
     // drop old coarse levels, build new ones down to >= 3 nodes per axis
     while (p->levels.size() > 1) {
         Level& L = p->levels.back();
         void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r};
+                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.wgt};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         p->levels.pop_back();

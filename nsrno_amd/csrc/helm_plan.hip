@@ -82,7 +82,7 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
 
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r};
+                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.wgt};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();

@@ -365,3 +365,34 @@ def gmres(operator, precondition, rhs, x, callback=None, restart=32, max_cycles=
     if callback is not None:
         callback(applications, x, np.linalg.norm(rhs - operator(x)) / scale)
     return x, "iteration_budget"
+
+
+def bicgstab(operator, precondition, rhs, x, tolerance=1e-3, max_iterations=800):
+    """Right-preconditioned BiCGSTAB, restating nsrno/dense_wave.py:118-148
+    (shadow residual r~ = r0, scalars rho, alpha, omega; the 40-step true
+    residual refresh is omitted).  Returns (x, iterations, reason)."""
+    scale = max(np.linalg.norm(rhs), 1e-30)
+    r = rhs - operator(x)
+    shadow = r.copy()
+    p = np.zeros_like(r)
+    v = np.zeros_like(r)
+    rho_old = alpha = omega = 1.0
+    for it in range(max_iterations):
+        if np.linalg.norm(r) / scale < tolerance:
+            return x, it, "converged"
+        rho = np.vdot(shadow, r)
+        if not np.isfinite(rho) or abs(rho) < 1e-30 or abs(omega) < 1e-30:
+            return x, it, "breakdown"
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        phat = precondition(p)
+        v = operator(phat)
+        alpha = rho / np.vdot(shadow, v)
+        s = r - alpha * v
+        shat = precondition(s)
+        t = operator(shat)
+        omega = np.vdot(t, s) / max(np.vdot(t, t).real, 1e-300)
+        x = x + alpha * phat + omega * shat
+        r = s - omega * t
+        rho_old = rho
+    return x, max_iterations, "iteration_budget"

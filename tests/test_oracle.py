@@ -201,3 +201,29 @@ def test_gmres_reasons_and_exact_termination():
     _, reason = H.gmres(lambda v: a @ v, lambda v: v, b, np.linalg.solve(a, b), lambda c, u, r: calls.append(c),
                         restart=4, max_cycles=3, tolerance=1e-8)
     assert reason == "converged" and calls == [0]
+
+
+def test_node_weight_symbol_matches_brute_force_spectrum():
+    """For constant coefficients and no PML the local symbol is exact: the
+    worst |g| the weight rule predicts over all modes equals the spectral
+    radius of the Jacobi iteration matrix I - w D^-1 M (periodic-free Dirichlet
+    grid: eigenvalues sit inside the predicted range)."""
+    from oracle import multigrid as M
+    n = 15
+    axis = np.arange(n) * 0.1
+    grid = H.Grid(axis, axis, 0.0)
+    f = 1.2
+    lev = M.Level(grid, 0.1, n, n, f, 1540.0, 12.0)
+    coef = lev.coef(np.ones((n, n)), np.ones((n, n), complex), 0.5)
+    w = M.node_weights(coef, coarsest=True)
+    wc = w[n // 2, n // 2]
+    a = (coef[1] + coef[2])[n // 2, n // 2] / coef[0][n // 2, n // 2]
+    b = (coef[3] + coef[4])[n // 2, n // 2] / coef[0][n // 2, n // 2]
+    shifted = H.assemble(grid, f, kappa_factor=np.full(grid.size, 1 + 0.5j)).toarray()
+    d = np.diag(shifted)
+    it = np.eye(grid.size) - wc * shifted / d[:, None]
+    rho = np.abs(np.linalg.eigvals(it)).max()
+    t = np.cos(np.pi * np.arange(1, n + 1) / (n + 1))
+    pred = np.abs(1 - wc * (1 + t[:, None] * a + t[None, :] * b)).max()
+    assert abs(rho - pred) < 1e-9
+    assert rho < 1.0

@@ -44,10 +44,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mg-levels", type=int, default=5)
     ap.add_argument("--mg-shift", type=float, default=0.5)
-    ap.add_argument("--mg-omega", type=float, default=0.6)
+    ap.add_argument("--mg-omega", type=float, default=0.0)
     ap.add_argument("--mg-pre", type=int, default=2)
     ap.add_argument("--mg-post", type=int, default=2)
-    ap.add_argument("--mg-coarse", type=int, default=30)
+    ap.add_argument("--mg-coarse", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
     ap.add_argument("--cpu-interior", type=int, default=None,

@@ -338,8 +338,19 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
     const int64_t n = L.n;
     const size_t cnt = (size_t)W.nrhs * n;
     float2** vecs[] = {&W.bp, &W.bv, &W.bs, &W.bt, &W.bphat, &W.bshat, &W.bshadow, &W.br};
+    if (W.bi_cap < cnt) {  // (re)allocate when the workspace grew
+        for (float2** q : vecs)
+            if (*q) {
+                (void)hipFree(*q);
+                *q = nullptr;
+            }
+        if (W.bsc) (void)hipFree(W.bsc);
+        W.bsc = nullptr;
+        W.bi_cap = 0;
+    }
     for (float2** q : vecs) HELM_TRY(bi_alloc(q, cnt));
     if (!W.bsc) HELM_HIP(hipMalloc((void**)&W.bsc, (size_t)W.nrhs * 8 * sizeof(double2) + 256));
+    W.bi_cap = cnt;
     const int nch = (int)((n + BCHUNK - 1) / BCHUNK);
     HELM_CHECK((int64_t)nrhs * 2 * nch <= W.partial_len, HELM_ERR_STATE, "bicgstab: partial buffer too small");
     const double k0sq = p->k0 * p->k0;
@@ -383,7 +394,8 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
         HELM_HIP(hipGetLastError());
         // rho_0 = <shadow, r>
         hipLaunchKernelGGL(k_bi_dots<1>, gch, blk, 0, s, n, nch, W.bshadow, W.br, nullptr, nullptr, W.bsc, W.partial);
-        for (int it = 0; it < o->max_iterations - total_steps; ++it) {
+        const int inner_budget = o->max_iterations - total_steps;
+        for (int it = 0; it < inner_budget; ++it) {
             hipLaunchKernelGGL(k_bi_scalar_rho, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bsc);
             hipLaunchKernelGGL(k_bi_p, gvec, blk, 0, s, n, W.br, W.bp, W.bv, W.bsc);
             const float2* ph = W.bp;

@@ -77,9 +77,10 @@ struct Level {
     float* eta32 = nullptr;
     float2* kap32 = nullptr;
     // multigrid work vectors (c64, nrhs * n each)
-    float2 *mg_u = nullptr, *mg_f = nullptr, *mg_r = nullptr;
+    float2 *mg_u = nullptr, *mg_f = nullptr, *mg_r = nullptr, *mg_t = nullptr;
     int mg_cap = 0;  // right-hand sides the work vectors hold
     float2* wgt = nullptr;  // per-node complex Jacobi weight
+    float2* tcf = nullptr;  // [n][6] precomputed shifted stencil for the LDS tail
 };
 
 struct Workspace {
@@ -106,10 +107,13 @@ struct Workspace {
     int* cyc = nullptr;       // nrhs: active in this cycle
     int* used = nullptr;      // nrhs: steps used in this cycle
     int* iters = nullptr;     // nrhs: Krylov steps so far
+    int* gate = nullptr;      // nrhs: second Gram-Schmidt pass needed
+    double* hn2 = nullptr;    // nrhs: |w|^2 after the first pass
     // BiCGSTAB vectors (c64, nrhs * n each) and scalars
     float2 *bp = nullptr, *bv = nullptr, *bs = nullptr, *bt = nullptr, *bphat = nullptr, *bshat = nullptr,
            *bshadow = nullptr, *br = nullptr;
     double2* bsc = nullptr;   // nrhs * 8 scalars
+    size_t bi_cap = 0;        // elements the BiCGSTAB vectors hold
     // host mirrors
     std::vector<double> h_rel;
     std::vector<int> h_active;
@@ -132,6 +136,8 @@ struct helm_plan {
     bool medium_set = false;
     std::vector<helm::Level> levels;  // levels[0] = fine grid
     helm::MGConfig mg;
+    void* mg_views = nullptr;      // device copy of the per-level views used by k_tail
+    bool mg_views_valid = false;
     helm::Workspace ws;
 };
 

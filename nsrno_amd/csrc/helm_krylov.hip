@@ -5,6 +5,11 @@
 // Gram-Schmidt applied twice, convergence decided on the true residual at
 // each restart, reasons converged / iteration_budget), with these changes
 // made for the GPU:
+//   * classical Gram-Schmidt with the second pass taken only where the first
+//     cancelled heavily (||w'|| < 0.7 ||w||, Kahan-Parlett); the decision is
+//     made on the device per right-hand side, the second-pass kernels are
+//     always launched and return at once where it is not needed, so the
+//     launch sequence stays fixed.  Usually this halves the basis reads;
 //   * every right-hand side has its own Krylov space; all are advanced in
 //     lock step by the same kernels (grid dimension = RHS), so one launch
 //     sequence serves the whole angle batch;
@@ -63,12 +68,17 @@ __device__ __forceinline__ double2 block_sum(double2 v, double* lds) {
 static inline int nchunks(int64_t n) { return (int)((n + CHUNK - 1) / CHUNK); }
 
 // partial[(r*KMAX + k)*nch + chunk] = sum_chunk conj(V_k[r]) * w[r]  for k < kcount
+// Also (wnorm != nullptr, first vector group) wnorm[r*nch+chunk] = |w|^2.
+// A right-hand side is skipped unless cyc[r] and (gate == nullptr or gate[r]).
 __global__ __launch_bounds__(256) void k_dots(int64_t n, int nrhs, int nch, int kmax, int kcount,
                                               const float2* __restrict__ V, const float2* __restrict__ w,
-                                              const int* __restrict__ cyc, double2* __restrict__ partial) {
+                                              const int* __restrict__ cyc, const int* __restrict__ gate,
+                                              double2* __restrict__ partial, double2* __restrict__ wnorm) {
     __shared__ double lds[8];
     const int chunk = blockIdx.x, r = blockIdx.z, k0 = blockIdx.y * KB;
-    if (!cyc[r]) return;
+    if (!cyc[r] || (gate && !gate[r])) return;
+    const bool want_norm = wnorm != nullptr && blockIdx.y == 0;
+    double wn = 0.0;
     const int kc = min(KB, kcount - k0);
     double2 acc[KB];
 #pragma unroll
@@ -79,6 +89,7 @@ __global__ __launch_bounds__(256) void k_dots(int64_t n, int nrhs, int nch, int 
     const float2* vr = V + (int64_t)k0 * vstride + (int64_t)r * n;
     for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
         const float2 wv = wr[p];
+        wn += (double)wv.x * wv.x + (double)wv.y * wv.y;
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             if (q < kc) {
@@ -95,15 +106,19 @@ __global__ __launch_bounds__(256) void k_dots(int64_t n, int nrhs, int nch, int 
             if (threadIdx.x == 0) partial[((int64_t)r * kmax + k0 + q) * nch + chunk] = s;
         }
     }
+    if (want_norm) {
+        double2 s = block_sum(make_double2(wn, 0.0), lds);
+        if (threadIdx.x == 0) wnorm[(int64_t)r * nch + chunk] = s;
+    }
 }
 
 // coef[r][k] = sum over chunks; H column j accumulates it (pass 0 sets).
 __global__ void k_reduce_dots(int nrhs, int nch, int kmax, int kcount, int m, int j, int pass,
                               const double2* __restrict__ partial, const int* __restrict__ cyc,
-                              double2* __restrict__ coef, double2* __restrict__ H) {
+                              const int* __restrict__ gate, double2* __restrict__ coef, double2* __restrict__ H) {
     __shared__ double lds[8];
     const int r = blockIdx.y, k = blockIdx.x;
-    if (!cyc[r] || k >= kcount) return;
+    if (!cyc[r] || k >= kcount || (gate && !gate[r])) return;
     double2 acc = make_double2(0.0, 0.0);
     const double2* src = partial + ((int64_t)r * kmax + k) * nch;
     for (int c = threadIdx.x; c < nch; c += blockDim.x) acc = cadd(acc, src[c]);
@@ -120,11 +135,11 @@ template <bool NORM>
 __global__ __launch_bounds__(256) void k_update(int64_t n, int nrhs, int nch, int kmax, int kcount,
                                                 const float2* __restrict__ V, float2* __restrict__ w,
                                                 const double2* __restrict__ coef, const int* __restrict__ cyc,
-                                                double2* __restrict__ partial) {
+                                                const int* __restrict__ gate, double2* __restrict__ partial) {
     __shared__ double lds[8];
     __shared__ float2 cf[64];
     const int chunk = blockIdx.x, r = blockIdx.y;
-    if (!cyc[r]) return;
+    if (!cyc[r] || (gate && !gate[r])) return;
     for (int k = threadIdx.x; k < kcount; k += 256) {
         double2 c = coef[(int64_t)r * kmax + k];
         cf[k] = make_float2((float)c.x, (float)c.y);
@@ -237,16 +252,33 @@ __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __r
     dst[(int64_t)r * n + p] = make_float2(a.x * s, a.y * s);
 }
 
+// Kahan-Parlett "twice is enough": after one classical Gram-Schmidt pass
+// w' = w - V c, a second pass is needed only if ||w'|| < 0.7 ||w|| (heavy
+// cancellation).  Sets gate[r] for the second pass and stores ||w'||^2.
+__global__ void k_reorth_check(int nch, const double2* __restrict__ wn_before, const double2* __restrict__ wn_after,
+                               const int* __restrict__ cyc, int* __restrict__ gate, double* __restrict__ hn2) {
+    __shared__ double lds[8];
+    const int r = blockIdx.x;
+    if (!cyc[r]) return;
+    const double before = reduce_chunks(wn_before + (int64_t)r * nch, nch, lds);
+    const double after = reduce_chunks(wn_after + (int64_t)r * nch, nch, lds);
+    if (threadIdx.x == 0) {
+        gate[r] = after < 0.49 * before ? 1 : 0;
+        hn2[r] = after;
+    }
+}
+
 // Finish column j: h_{j+1,j} = ||w||, apply old rotations, new rotation,
 // residual estimate, early stop of the cycle.
 __global__ void k_givens(int nch, int m, int j, double tol, const double2* __restrict__ partial,
+                         const int* __restrict__ gate, const double* __restrict__ hn2,
                          const double* __restrict__ bnorm, double2* __restrict__ H, double* __restrict__ cs,
                          double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ est,
                          double* __restrict__ inv, int* __restrict__ cyc, int* __restrict__ used) {
     __shared__ double lds[8];
     const int r = blockIdx.x;
     if (!cyc[r]) return;
-    double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
+    double s = gate[r] ? reduce_chunks(partial + (int64_t)r * nch, nch, lds) : hn2[r];
     if (threadIdx.x != 0) return;
     const double hn = sqrt(s);
     double2* hc = H + ((int64_t)r * m + j) * (m + 1);
@@ -370,7 +402,7 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.Z, (size_t)restart * nrhs * n));
     HELM_TRY(alloc(&w.w, (size_t)nrhs * n));
     HELM_TRY(alloc(&w.r64, (size_t)nrhs * n));
-    w.partial_len = (int64_t)nrhs * kmax * nch;
+    w.partial_len = (int64_t)nrhs * (kmax + 3) * nch;  // dots | |w|^2 | |w'|^2 | |w''|^2
     HELM_TRY(alloc(&w.partial, (size_t)w.partial_len));
     HELM_TRY(alloc(&w.H, (size_t)nrhs * restart * (restart + 1)));
     HELM_TRY(alloc(&w.cs, (size_t)nrhs * restart));
@@ -386,6 +418,8 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.cyc, nrhs));
     HELM_TRY(alloc(&w.used, nrhs));
     HELM_TRY(alloc(&w.iters, nrhs));
+    HELM_TRY(alloc(&w.gate, nrhs));
+    HELM_TRY(alloc(&w.hn2, nrhs));
     w.h_rel.assign(nrhs, 0.0);
     w.h_active.assign(nrhs, 0);
     w.h_iters.assign(nrhs, 0);
@@ -409,6 +443,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const dim3 blk(256);
     const dim3 gvec((unsigned)((n + 255) / 256), nrhs);
     const int64_t vstride = (int64_t)nrhs * n;
+    double2* pw0 = W.partial + (int64_t)nrhs * kmax * nch;  // |w|^2 before projection
+    double2* pw1 = pw0 + (int64_t)nrhs * nch;               // after the first pass
+    double2* pw2 = pw1 + (int64_t)nrhs * nch;               // after the second pass
 
     // ||b|| and initial x
     hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, b, W.partial);
@@ -453,20 +490,24 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
             applications++;
             const int kc = j + 1;
-            for (int pass = 0; pass < 2; ++pass) {
-                hipLaunchKernelGGL(k_dots, dim3(nch, (kc + KB - 1) / KB, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc,
-                                   W.V, W.w, W.cyc, W.partial);
-                hipLaunchKernelGGL(k_reduce_dots, dim3(kc, nrhs), dim3(64), 0, s, nrhs, nch, kmax, kc, mm, j, pass,
-                                   W.partial, W.cyc, W.coef, W.H);
-                if (pass == 0)
-                    hipLaunchKernelGGL(k_update<false>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V,
-                                       W.w, W.coef, W.cyc, W.partial);
-                else
-                    hipLaunchKernelGGL(k_update<true>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V,
-                                       W.w, W.coef, W.cyc, W.partial);
-            }
-            hipLaunchKernelGGL(k_givens, dim3(nrhs), dim3(64), 0, s, nch, mm, j, o->tolerance, W.partial, W.bnorm,
-                               W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used);
+            const dim3 gd(nch, (kc + KB - 1) / KB, nrhs);
+            // pass 1: c = V^H w (and |w|^2), w -= V c (and |w'|^2)
+            hipLaunchKernelGGL(k_dots, gd, blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.cyc, (const int*)nullptr,
+                               W.partial, pw0);
+            hipLaunchKernelGGL(k_reduce_dots, dim3(kc, nrhs), dim3(64), 0, s, nrhs, nch, kmax, kc, mm, j, 0,
+                               W.partial, W.cyc, (const int*)nullptr, W.coef, W.H);
+            hipLaunchKernelGGL(k_update<true>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.coef,
+                               W.cyc, (const int*)nullptr, pw1);
+            hipLaunchKernelGGL(k_reorth_check, dim3(nrhs), dim3(64), 0, s, nch, pw0, pw1, W.cyc, W.gate, W.hn2);
+            // pass 2 only where the first pass cancelled heavily (gate[r])
+            hipLaunchKernelGGL(k_dots, gd, blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.cyc, W.gate, W.partial,
+                               (double2*)nullptr);
+            hipLaunchKernelGGL(k_reduce_dots, dim3(kc, nrhs), dim3(64), 0, s, nrhs, nch, kmax, kc, mm, j, 1,
+                               W.partial, W.cyc, W.gate, W.coef, W.H);
+            hipLaunchKernelGGL(k_update<true>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.coef,
+                               W.cyc, W.gate, pw2);
+            hipLaunchKernelGGL(k_givens, dim3(nrhs), dim3(64), 0, s, nch, mm, j, o->tolerance, pw2, W.gate, W.hn2,
+                               W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used);
             if (j + 1 < m)
                 hipLaunchKernelGGL(k_scale32, gvec, blk, 0, s, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
                                    W.cyc);

@@ -19,6 +19,7 @@
 // All kernels are HBM-bound: per sweep they read u, f (c64), the node
 // coefficients (f32 eta + c64 kappa_c, shared by all right-hand sides) and
 // write u; transfer kernels read one level and write the other.
+#include <algorithm>
 #include <cmath>
 
 #include "helm_coef.h"
@@ -92,16 +93,21 @@ __global__ void k_fill_weight(int64_t n, float omega, float2* __restrict__ wgt) 
 }
 
 // u_out = u_in + w (f - M u_in) / dg with the node's complex weight w
-// (u_in == nullptr: zero start)
-__global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int nrhs, const float2* __restrict__ sx,
-                                                const float2* __restrict__ sz, const float2* __restrict__ isxe,
-                                                const float2* __restrict__ isze, const float* __restrict__ eta,
-                                                const float2* __restrict__ kap, float k0sq, float hh, float shift,
-                                                int conj, const float2* __restrict__ wgt, const float2* __restrict__ u,
+// (u_in == nullptr: zero start).  blockIdx.z selects a group of rb
+// right-hand sides: large levels use few groups (coefficients are computed
+// once and reused across the group), small levels one RHS per group (more
+// workgroups in flight).
+__global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int nrhs, int rb,
+                                                const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                float k0sq, float hh, float shift, int conj,
+                                                const float2* __restrict__ wgt, const float2* __restrict__ u,
                                                 const float2* __restrict__ f, float2* __restrict__ out) {
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 4 + threadIdx.y;
     if (i >= nz || j >= nx) return;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
     Coef<float2> c;
     node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
     const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int n
     const float2 wn = wgt[idx];
     const float2 idg = cmul(conj ? cconj(wn) : wn, make_float2(c.dg.x / dd, -c.dg.y / dd));
     const bool hw = j > 0, he = j < nx - 1, hn = i > 0, hs = i < nz - 1;
-    for (int r = 0; r < nrhs; ++r) {
+    for (int r = r0; r < r1; ++r) {
         const int64_t base = (int64_t)r * n;
         const float2 fv = f[base + idx];
         if (u == nullptr) {
@@ -123,11 +129,177 @@ __global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int n
         if (he) cfma(acc, c.fe, ur[idx + 1]);
         if (hn) cfma(acc, c.fn, ur[idx - nx]);
         if (hs) cfma(acc, c.fs, ur[idx + nx]);
-        float2 res = csub(fv, acc);
         float2 nu = uc;
-        cfma(nu, idg, res);
+        cfma(nu, idg, csub(fv, acc));
         out[base + idx] = nu;
     }
+}
+
+// ---- single-workgroup V-cycle over the small coarse levels ---------------
+// The coarse levels hold too few nodes to fill the GPU, so launching one
+// kernel per sweep is pure latency (tens of launches per V-cycle).  k_tail
+// runs the whole V-cycle from level t down to the coarsest and back inside
+// one kernel: one 1024-thread workgroup per right-hand side.  The iterate and
+// its Jacobi partner of every tail level live in LDS (a stack, 16 B per node
+// per level: 110 KB for 72^2 + 36^2 + 18^2); right-hand sides and residuals
+// stay in global memory (L2); the stencil coefficients of the shifted
+// operator and the Jacobi factor w/dg are precomputed per node at set-up
+// (k_tail_coef), so a sweep is 6 coefficient loads, 5 LDS reads and one f load
+// per node with no dependent chains.
+// first level handled by k_tail: the largest level with at most TAIL_NODES
+// nodes (its LDS stack, 16 B per node over the tail levels, must fit 160 KB)
+constexpr int64_t TAIL_NODES = 80 * 80;
+
+struct LevelView {
+    int nz, nx;
+    int64_t n;
+    const float2* cf;       // [n][6]: w/dg, dg, fw, fe, fn, fs
+    float2 *f, *r;          // [nrhs][n] global work arrays
+};
+
+__global__ void k_tail_coef(int nz, int nx, const float2* __restrict__ sx, const float2* __restrict__ sz,
+                            const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                            const float* __restrict__ eta, const float2* __restrict__ kap, float k0sq, float hh,
+                            float shift, const float2* __restrict__ wgt, float2* __restrict__ cf) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nz * nx) return;
+    const int i = (int)(q / nx), j = (int)(q % nx);
+    Coef<float2> c;
+    node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, false, c);
+    const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
+    float2* o = cf + q * 6;
+    o[0] = cmul(wgt[q], make_float2(c.dg.x / dd, -c.dg.y / dd));
+    o[1] = c.dg;
+    o[2] = c.fw;
+    o[3] = c.fe;
+    o[4] = c.fn;
+    o[5] = c.fs;
+}
+
+__device__ __forceinline__ float2 cc(float2 a, bool conj) { return conj ? cconj(a) : a; }
+
+// M u at node q with u in LDS
+__device__ __forceinline__ float2 tail_mu(const LevelView& V, int64_t q, const float2* cf, const float2* u,
+                                          bool conj) {
+    const int i = (int)(q / V.nx), j = (int)(q % V.nx);
+    float2 acc = cmul(cc(cf[1], conj), u[q]);
+    if (j > 0) cfma(acc, cc(cf[2], conj), u[q - 1]);
+    if (j < V.nx - 1) cfma(acc, cc(cf[3], conj), u[q + 1]);
+    if (i > 0) cfma(acc, cc(cf[4], conj), u[q - V.nx]);
+    if (i < V.nz - 1) cfma(acc, cc(cf[5], conj), u[q + V.nx]);
+    return acc;
+}
+
+// sweeps of damped Jacobi from a zero start (zero_start) or from u, ending
+// in u; t is the LDS partner buffer
+__device__ void tail_sweeps(const LevelView& V, int sweeps, bool zero_start, bool conj, const float2* f, float2* u,
+                            float2* t) {
+    const float2* src = zero_start ? nullptr : u;
+    if (!zero_start && sweeps % 2 == 1) {  // odd count from u: start from a copy in t
+        for (int64_t q = threadIdx.x; q < V.n; q += blockDim.x) t[q] = u[q];
+        __syncthreads();
+        src = t;
+    }
+    for (int k = 0; k < sweeps; ++k) {
+        float2* dst = zero_start ? ((sweeps - k) % 2 == 1 ? u : t) : (src == u ? t : u);
+        for (int64_t q = threadIdx.x; q < V.n; q += blockDim.x) {
+            const float2* cf = V.cf + q * 6;
+            const float2 fv = f[q];
+            if (src == nullptr) {
+                dst[q] = cmul(cc(cf[0], conj), fv);
+            } else {
+                float2 nu = src[q];
+                cfma(nu, cc(cf[0], conj), csub(fv, tail_mu(V, q, cf, src, conj)));
+                dst[q] = nu;
+            }
+        }
+        __syncthreads();
+        src = dst;
+    }
+    if (zero_start && sweeps == 0) {
+        for (int64_t q = threadIdx.x; q < V.n; q += blockDim.x) u[q] = make_float2(0.f, 0.f);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_tail(const LevelView* __restrict__ views, int t, int last, int conj_i,
+                                               int pre, int post, int coarse, const float2* __restrict__ f_top,
+                                               float2* __restrict__ u_top) {
+    extern __shared__ float2 lds[];
+    const int r = blockIdx.x;
+    const bool conj = conj_i != 0;
+    int64_t off[16];
+    {
+        int64_t o = 0;
+        for (int l = t; l <= last; ++l) {
+            off[l - t] = o;
+            o += 2 * views[l].n;
+        }
+    }
+    // down: smooth, residual, restrict
+    for (int l = t; l <= last; ++l) {
+        const LevelView& V = views[l];
+        const float2* f = (l == t ? f_top : V.f) + (int64_t)r * V.n;
+        float2* u = lds + off[l - t];
+        float2* tt = u + V.n;
+        if (l == last) {
+            tail_sweeps(V, coarse, true, conj, f, u, tt);
+            break;
+        }
+        tail_sweeps(V, pre, true, conj, f, u, tt);
+        float2* res = V.r + (int64_t)r * V.n;
+        for (int64_t q = threadIdx.x; q < V.n; q += blockDim.x) res[q] = csub(f[q], tail_mu(V, q, V.cf + q * 6, u, conj));
+        __syncthreads();
+        const LevelView& C = views[l + 1];
+        float2* fc = C.f + (int64_t)r * C.n;
+        for (int64_t q = threadIdx.x; q < C.n; q += blockDim.x) {
+            const int I = (int)(q / C.nx), J = (int)(q % C.nx);
+            float2 acc = make_float2(0.f, 0.f);
+            for (int a = -1; a <= 1; ++a) {
+                const int i = 2 * I + a;
+                if (i < 0 || i >= V.nz) continue;
+                for (int b = -1; b <= 1; ++b) {
+                    const int j = 2 * J + b;
+                    if (j < 0 || j >= V.nx) continue;
+                    const float w = (a == 0 ? 0.5f : 0.25f) * (b == 0 ? 0.5f : 0.25f);
+                    const float2 v = res[(int64_t)i * V.nx + j];
+                    acc.x += w * v.x;
+                    acc.y += w * v.y;
+                }
+            }
+            fc[q] = acc;
+        }
+        __syncthreads();
+    }
+    // up: prolong-add, post-smooth
+    for (int l = last - 1; l >= t; --l) {
+        const LevelView& V = views[l];
+        const LevelView& C = views[l + 1];
+        const float2* f = (l == t ? f_top : V.f) + (int64_t)r * V.n;
+        float2* u = lds + off[l - t];
+        const float2* uc = lds + off[l + 1 - t];
+        for (int64_t q = threadIdx.x; q < V.n; q += blockDim.x) {
+            const int i = (int)(q / V.nx), j = (int)(q % V.nx);
+            const int I0 = i >> 1, J0 = j >> 1;
+            float2 v = make_float2(0.f, 0.f);
+            for (int di = 0; di <= (i & 1); ++di)
+                for (int dj = 0; dj <= (j & 1); ++dj) {
+                    const int I = I0 + di, J = J0 + dj;
+                    if (I < C.nz && J < C.nx) {
+                        const float w = ((i & 1) ? 0.5f : 1.f) * ((j & 1) ? 0.5f : 1.f);
+                        const float2 cv = uc[(int64_t)I * C.nx + J];
+                        v.x += w * cv.x;
+                        v.y += w * cv.y;
+                    }
+                }
+            u[q] = cadd(u[q], v);
+        }
+        __syncthreads();
+        tail_sweeps(V, post, false, conj, f, u, u + V.n);
+    }
+    // result of level t to global memory
+    const LevelView& T = views[t];
+    for (int64_t q = threadIdx.x; q < T.n; q += blockDim.x) u_top[(int64_t)r * T.n + q] = lds[q];
 }
 
 // full weighting: fc[I,J] = sum_{a,b} w_a w_b rf[2I+a, 2J+b], w = (1/4, 1/2, 1/4)
@@ -219,7 +391,7 @@ __global__ void k_fill_c64(int64_t count, float2* p) {
 static int32_t level_alloc_work(Level& L, int nrhs) {
     if (L.mg_cap >= nrhs) return HELM_OK;
     const size_t bytes = (size_t)nrhs * L.n * sizeof(float2) + 256;
-    float2** bufs[3] = {&L.mg_u, &L.mg_f, &L.mg_r};
+    float2** bufs[4] = {&L.mg_u, &L.mg_f, &L.mg_r, &L.mg_t};
     for (float2** b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -257,63 +429,103 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                                (float)p->mg.shift, (int)(l + 1 == p->levels.size()), L.wgt);
         }
         HELM_HIP(hipGetLastError());
+        if (L.n <= TAIL_NODES) {
+            if (!L.tcf) HELM_HIP(hipMalloc((void**)&L.tcf, 6 * L.n * sizeof(float2) + 256));
+            hipLaunchKernelGGL(k_tail_coef, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.nz, L.nx, L.sx32,
+                               L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0),
+                               (float)(0.5 / (L.h_m * L.h_m)), (float)p->mg.shift, L.wgt, L.tcf);
+            HELM_HIP(hipGetLastError());
+        }
     }
     return HELM_OK;
+}
+
+// right-hand sides per workgroup so that a level launches >= ~2048 workgroups
+static int rhs_group(int64_t blocks_xy, int nrhs) {
+    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + blocks_xy - 1) / blocks_xy));
+    return (int)((nrhs + groups - 1) / groups);
 }
 
 static int32_t jacobi(helm_plan* p, int l, double shift, bool adj, const float2* u, const float2* f,
                       float2* out, int nrhs, hipStream_t s) {
     const Level& L = p->levels[l];  // conj(M) uses conj(w): the same contraction
-    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+    const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
+    const int rb = rhs_group(bxy, nrhs);
+    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const double k0sq = p->k0 * p->k0;
-    hipLaunchKernelGGL(k_jacobi, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32, L.isxe32, L.isze32,
-                       L.eta32, L.kap32, (float)k0sq, (float)(0.5 / (L.h_m * L.h_m)), (float)shift, (int)adj,
-                       L.wgt, u, f, out);
+    hipLaunchKernelGGL(k_jacobi, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32, L.isxe32, L.isze32,
+                       L.eta32, L.kap32, (float)k0sq, (float)(0.5 / (L.h_m * L.h_m)), (float)shift, (int)adj, L.wgt,
+                       u, f, out);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
 
-// Solve M_l u = f approximately; f and u are level-l arrays (u may be the
-// caller's output on level 0).  Uses L.mg_r as scratch and L.mg_u for
-// Jacobi ping-pong.
+
+static size_t tail_lds_bytes(const helm_plan* p, int t) {
+    size_t b = 0;
+    for (size_t l = t; l < p->levels.size(); ++l) b += 2 * sizeof(float2) * (size_t)p->levels[l].n;
+    return b;
+}
+
+static int32_t tail_views(helm_plan* p) {
+    const int nl = (int)p->levels.size();
+    std::vector<LevelView> h(nl);
+    for (int l = 0; l < nl; ++l) {
+        const Level& L = p->levels[l];
+        h[l] = LevelView{L.nz, L.nx, L.n, L.tcf, L.mg_f, L.mg_r};
+    }
+    if (!p->mg_views) HELM_HIP(hipMalloc(&p->mg_views, 16 * sizeof(LevelView)));
+    HELM_HIP(hipMemcpy(p->mg_views, h.data(), nl * sizeof(LevelView), hipMemcpyHostToDevice));
+    return HELM_OK;
+}
+
+// Solve M_l u = f approximately.  f and u are level-l arrays that must not
+// alias the level's scratch: on level 0 they are the caller's input and
+// output, on coarser levels L.mg_f and L.mg_u.  L.mg_r holds the residual,
+// L.mg_t is the Jacobi ping-pong partner of u.
 static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s) {
     Level& L = p->levels[l];
     const MGConfig& c = p->mg;
+    if (L.n <= TAIL_NODES) {
+        const size_t lds = tail_lds_bytes(p, l);
+        hipLaunchKernelGGL(k_tail, dim3(nrhs), dim3(1024), lds, s, (const LevelView*)p->mg_views, l,
+                           (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const bool coarsest = l + 1 == (int)p->levels.size();
     const int sweeps = coarsest ? c.coarse : c.pre;
-    // pre-smoothing from a zero guess; ping-pong between u and L.mg_u so the
-    // result ends in u
-    float2* bufs[2] = {u, L.mg_u};
-    int cur = -1;  // index of the buffer holding the iterate (-1: zero)
+    // pre-smoothing from a zero guess; the last sweep writes u
+    const float2* src = nullptr;
     for (int k = 0; k < sweeps; ++k) {
-        const int dst = (sweeps - k) % 2 == 1 ? 0 : 1;  // last sweep writes u
-        HELM_TRY(jacobi(p, l, c.shift, adj, cur < 0 ? nullptr : bufs[cur], f, bufs[dst], nrhs, s));
-        cur = dst;
+        float2* dst = (sweeps - k) % 2 == 1 ? u : L.mg_t;
+        HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
+        src = dst;
     }
-    if (cur < 0) {  // no sweeps: u = 0
+    if (sweeps == 0) {
         const int64_t cnt = (int64_t)nrhs * L.n;
         hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, cnt, u);
         HELM_HIP(hipGetLastError());
     }
     if (coarsest) return HELM_OK;
     Level& C = p->levels[l + 1];
-    // residual r = f - M u
+    // residual r = f - M u, restricted to the coarse right-hand side
     HELM_TRY(launch_apply32(L, p->k0 * p->k0, u, L.mg_r, nrhs, adj, f, (float)c.shift, s));
     hipLaunchKernelGGL(k_restrict, dim3((C.nx + 63) / 64, C.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs,
                        L.mg_r, C.mg_f);
     HELM_HIP(hipGetLastError());
-    HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_r, nrhs, adj, s));  // coarse solution in C.mg_r
+    HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
     hipLaunchKernelGGL(k_prolong_add, dim3((L.nx + 63) / 64, L.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx,
-                       nrhs, C.mg_r, u);
+                       nrhs, C.mg_u, u);
     HELM_HIP(hipGetLastError());
-    // post-smoothing, ping-pong ending in u
-    const float2* src = u;
+    // post-smoothing, ping-pong with L.mg_t ending in u
+    src = u;
     if (c.post % 2 == 1) {
-        HELM_HIP(hipMemcpyAsync(L.mg_u, u, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-        src = L.mg_u;
+        HELM_HIP(hipMemcpyAsync(L.mg_t, u, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+        src = L.mg_t;
     }
     for (int k = 0; k < c.post; ++k) {
-        float2* dst = (src == u) ? L.mg_u : u;
+        float2* dst = (src == u) ? L.mg_t : u;
         HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
         src = dst;
     }
@@ -322,7 +534,15 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
 
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
     HELM_CHECK(p->mg.levels > 0, HELM_ERR_STATE, "multigrid not set up");
-    for (auto& L : p->levels) HELM_TRY(level_alloc_work(L, nrhs));
+    bool grew = false;
+    for (auto& L : p->levels) {
+        grew = grew || L.mg_cap < nrhs;
+        HELM_TRY(level_alloc_work(L, nrhs));
+    }
+    if (grew || !p->mg_views_valid) {
+        HELM_TRY(tail_views(p));
+        p->mg_views_valid = true;
+    }
     return vcycle(p, 0, v, z, nrhs, adjoint, s);
 }
 
@@ -350,7 +570,7 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     while (p->levels.size() > 1) {
         Level& L = p->levels.back();
         void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.wgt};
+                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         p->levels.pop_back();
@@ -365,6 +585,14 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
         HELM_TRY(build_level_profiles(p, p->levels.back(), d.x0_mm, d.z0_mm, hc, nzc, nxc));
     }
     p->mg.levels = (int)p->levels.size();
+    p->mg_views_valid = false;
+    {
+        int t = 0;
+        while (t < (int)p->levels.size() && p->levels[t].n > TAIL_NODES) ++t;
+        const size_t lds = t < (int)p->levels.size() ? tail_lds_bytes(p, t) : 0;
+        HELM_CHECK(lds <= 160 * 1024, HELM_ERR_ARG, "helm_mg_setup: coarse levels exceed the LDS budget");
+        HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     p->mg.shift = shift;
     p->mg.pre = pre;
     p->mg.post = post;

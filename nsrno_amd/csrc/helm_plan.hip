@@ -82,7 +82,7 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
 
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.wgt};
+                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();
@@ -90,7 +90,7 @@ static void free_level(Level& L) {
 
 void free_workspace(Workspace& w) {
     void* ptrs[] = {w.V, w.Z, w.w, w.r64, w.partial, w.H, w.cs, w.sn, w.g, w.coef, w.beta, w.bnorm, w.rel,
-                    w.inv, w.est, w.active, w.cyc, w.used, w.iters, w.bp, w.bv, w.bs, w.bt, w.bphat, w.bshat,
+                    w.inv, w.est, w.active, w.cyc, w.used, w.iters, w.gate, w.hn2, w.bp, w.bv, w.bs, w.bt, w.bphat, w.bshat,
                     w.bshadow, w.br, w.bsc};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -185,6 +185,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     (void)hipDeviceSynchronize();
     for (auto& L : p->levels) free_level(L);
     free_workspace(p->ws);
+    if (p->mg_views) (void)hipFree(p->mg_views);
     delete p;
     return HELM_OK;
 }

@@ -16,6 +16,8 @@
 // per node and reused for every right-hand side in registers; 1-D complex
 // profiles are tiny and stay in cache.  Algorithmic bytes per node per RHS:
 // 8 (read u) + 8 (write Au) + 12/R (coefficients).
+#include <algorithm>
+
 #include "helm_internal.h"
 #include "helm_coef.h"
 
@@ -23,7 +25,7 @@ namespace helm {
 
 // out[r] = A u[r]  (MODE 0)   or   out[r] = b[r] - A u[r]  (MODE 1)
 template <typename R, typename C, int MODE>
-__global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nrhs, const C* __restrict__ sx,
+__global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nrhs, int rb, const C* __restrict__ sx,
                                                const C* __restrict__ sz, const C* __restrict__ isxe,
                                                const C* __restrict__ isze, const R* __restrict__ eta,
                                                const C* __restrict__ kap, R k0sq, R hh, R shift, int conj,
@@ -36,7 +38,8 @@ __global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nr
     node_coef<R, C>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
     const int64_t idx = (int64_t)i * nx + j;
     const bool hw = j > 0, he = j < nx - 1, hn = i > 0, hs = i < nz - 1;
-    for (int r = 0; r < nrhs; ++r) {
+    const int r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
+    for (int r = blockIdx.z * rb; r < r1; ++r) {
         const C* ur = u + (int64_t)r * n;
         C acc = cmul(c.dg, ur[idx]);
         if (hw) cfma(acc, c.fw, ur[idx - 1]);
@@ -48,15 +51,23 @@ __global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nr
     }
 }
 
+// right-hand sides per workgroup: enough groups for >= ~2048 workgroups
+static int rhs_per_group(const Level& L, int nrhs) {
+    const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
+    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + bxy - 1) / bxy));
+    return (int)((nrhs + groups - 1) / groups);
+}
+
 int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                        const float2* b, float shift, hipStream_t s) {
-    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+    const int rb = rhs_per_group(L, nrhs);
+    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const float hh = (float)(0.5 / (L.h_m * L.h_m));
     if (b)
-        hipLaunchKernelGGL((k_apply<float, float2, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32,
+        hipLaunchKernelGGL((k_apply<float, float2, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32,
                            L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out, b);
     else
-        hipLaunchKernelGGL((k_apply<float, float2, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32,
+        hipLaunchKernelGGL((k_apply<float, float2, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32,
                            L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out,
                            (const float2*)nullptr);
     HELM_HIP(hipGetLastError());
@@ -65,13 +76,14 @@ int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out
 
 int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
                        const double2* b, hipStream_t s) {
-    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+    const int rb = rhs_per_group(L, nrhs);
+    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const double hh = 0.5 / (L.h_m * L.h_m);
     if (b)
-        hipLaunchKernelGGL((k_apply<double, double2, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx64,
+        hipLaunchKernelGGL((k_apply<double, double2, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx64,
                            L.sz64, L.isxe64, L.isze64, L.eta64, L.kap64, k0sq, hh, 0.0, (int)adjoint, u, out, b);
     else
-        hipLaunchKernelGGL((k_apply<double, double2, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx64,
+        hipLaunchKernelGGL((k_apply<double, double2, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx64,
                            L.sz64, L.isxe64, L.isze64, L.eta64, L.kap64, k0sq, hh, 0.0, (int)adjoint, u, out,
                            (const double2*)nullptr);
     HELM_HIP(hipGetLastError());

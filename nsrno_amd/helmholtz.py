@@ -166,7 +166,7 @@ class HelmholtzPlan:
         return traces
 
     # ------------------------------------------------------------ preconditioners
-    def setup_multigrid(self, levels=5, shift=0.5, pre=2, post=2, coarse_sweeps=30, omega=0.0):
+    def setup_multigrid(self, levels=5, shift=0.5, pre=2, post=2, coarse_sweeps=10, omega=0.0):
         check(lib().helm_mg_setup(self._h, int(levels), float(shift), int(pre), int(post), int(coarse_sweeps),
                                   float(omega), _stream()), "helm_mg_setup")
         self.mg_levels = int(levels)

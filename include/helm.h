@@ -155,6 +155,31 @@ int32_t helm_mg_setup(helm_plan* plan, int32_t levels, double shift, int32_t pre
 int32_t helm_precond_apply(helm_plan* plan, int32_t precond, const void* v, void* z, int32_t nrhs,
                            void* stream);
 
+/* ---- learned preconditioner network ----------------------------------
+ * A 4-level, 32-channel 3x3-conv U-Net (BF16 MFMA implicit-GEMM kernels, FP32
+ * accumulation).  Replaces the reference's LearnedPreconditioner and its
+ * training harness (nsrno/conditioned_wave.py:94-117,
+ * nsrno/train_preconditioner.py:45-109): with HELM_PC_LEARNED the flexible
+ * GMRES direction is z = MG(v) + N(v).  Tensors are NHWC bf16
+ * [batch][H][W][32] on an H x W canvas (multiples of 8, >= the grid; the
+ * grid sits at the top-left).  Parameters: fp32 [12][9*32*32 + 32]
+ * (weights [tap][out][in], then biases), device memory owned by the caller. */
+typedef struct helm_net helm_net;
+int32_t helm_net_create(int32_t H, int32_t W, int32_t batch, int32_t training, helm_net** net);
+int32_t helm_net_destroy(helm_net* net);
+int32_t helm_net_param_count(void);
+int32_t helm_net_set_params(helm_net* net, const float* params, void* stream);
+int32_t helm_net_forward(helm_net* net, const void* in, void* out, int32_t batch, void* stream);
+int32_t helm_net_backward(helm_net* net, const void* dout, float* grad, int32_t batch, void* stream);
+int32_t helm_adam_step(int64_t count, float* params, const float* grad, float* m, float* v, int32_t step,
+                       double lr, double beta1, double beta2, double eps, double grad_scale, void* stream);
+/* attach (not owned) the net used by HELM_PC_LEARNED; NULL detaches */
+int32_t helm_plan_set_net(helm_plan* plan, helm_net* net);
+/* z = N(v): the network correction alone, c64 [nrhs][N] */
+int32_t helm_net_correction(helm_plan* plan, const void* v, void* z, int32_t nrhs, void* stream);
+/* the net's input features for residuals v (bf16 [nrhs][H][W][32]) */
+int32_t helm_net_features(helm_plan* plan, const void* v, void* features, int32_t nrhs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,17 @@ SIGNATURES = {
                              POINTER(HelmSolveInfo), c_void_p]),
     "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
     "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_net_create": (c_int32, [c_int32, c_int32, c_int32, c_int32, POINTER(c_void_p)]),
+    "helm_net_destroy": (c_int32, [c_void_p]),
+    "helm_net_param_count": (c_int32, []),
+    "helm_net_set_params": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "helm_net_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_net_backward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_adam_step": (c_int32, [ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double,
+                                 c_double, c_double, c_double, c_double, c_void_p]),
+    "helm_plan_set_net": (c_int32, [c_void_p, c_void_p]),
+    "helm_net_correction": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_net_features": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
 }
 
 _LIB = None

@@ -129,6 +129,8 @@ struct MGConfig {
 
 }  // namespace helm
 
+struct helm_net;
+
 struct helm_plan {
     helm_desc desc;
     int device = 0;
@@ -136,6 +138,8 @@ struct helm_plan {
     bool medium_set = false;
     std::vector<helm::Level> levels;  // levels[0] = fine grid
     helm::MGConfig mg;
+    struct helm_net* net = nullptr;  // learned preconditioner (not owned)
+    float* net_scale = nullptr;      // per-RHS residual scale for the net
     void* mg_views = nullptr;      // device copy of the per-level views used by k_tail
     bool mg_views_valid = false;
     helm::Workspace ws;
@@ -160,4 +164,6 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
                        helm_solve_info* info, hipStream_t s);
 int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint,
                       hipStream_t s);
+// learned correction z += N(v) (helm_net.hip)
+int32_t net_correction(helm_plan* p, struct helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s);
 }  // namespace helm

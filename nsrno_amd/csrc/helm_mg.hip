@@ -548,6 +548,12 @@ int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoi
 
 int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
     if (kind == HELM_PC_MG) return mg_vcycle(p, v, z, nrhs, adjoint, s);
+    if (kind == HELM_PC_LEARNED) {
+        HELM_CHECK(p->net, HELM_ERR_STATE, "learned preconditioner: no net attached (helm_plan_set_net)");
+        HELM_CHECK(!adjoint, HELM_ERR_ARG, "learned preconditioner: adjoint solves use HELM_PC_MG");
+        HELM_TRY(mg_vcycle(p, v, z, nrhs, adjoint, s));
+        return net_correction(p, p->net, v, z, nrhs, s);
+    }
     set_error("preconditioner kind not available");
     return HELM_ERR_STATE;
 }

@@ -1,0 +1,422 @@
+// Learned preconditioner network: a 4-level, 32-channel 3x3-conv U-Net with
+// BF16 weights/activations and FP32 accumulation, its backward pass, Adam,
+// and the feature/output maps that connect it to the Helmholtz solve.
+//
+// Graph (H, W divisible by 8; level l is H/2^l x W/2^l; all tensors NHWC x32):
+//   a0 = relu(C0a(in))      e0 = relu(C0b(a0))
+//   a1 = relu(C1a(pool e0)) e1 = relu(C1b(a1))
+//   a2 = relu(C2a(pool e1)) e2 = relu(C2b(a2))
+//   a3 = relu(C3a(pool e2)) e3 = relu(C3b(a3))
+//   d2 = relu(C2c(up e3 + e2)); d1 = relu(C1c(up d2 + e1)); d0 = relu(C0c(up d1 + e0))
+//   out = Cout(d0)                      (channels 0, 1 = Re, Im of the correction)
+// Inputs (channels of `in`): 0, 1 the residual (re, im) divided by its RMS,
+// 2 log(eta_rel) (= log(rho0/rho)), 3 log Re(kappa_c) (compressibility
+// contrast), 4 Im(kappa_c)/Re(kappa_c) (attenuation), 5 PML stretch
+// Im(sx) + Im(sz), 6 k0 h; 7..31 zero.  These are the c, rho, alpha, omega
+// and residual inputs of the learned preconditioner, in the form the
+// operator uses them.  The reference's learned preconditioner
+// (nsrno/conditioned_wave.py:94-117) is a 3-conv context net producing
+// spectral gains; this one predicts a field correction directly.
+//
+// Parameters: 12 convolutions x (9*32*32 weights + 32 biases), FP32 master
+// copy [layer][weights | bias] owned by the caller; BF16 copies and the
+// flipped copies for the backward pass live in the net.
+#include <cmath>
+#include <vector>
+
+#include "helm_coef.h"
+#include "helm_internal.h"
+
+namespace helm {
+
+typedef __bf16 bf16;
+
+}  // namespace helm
+
+#include "helm_conv.inc"
+
+namespace helm {
+
+constexpr int CONV_ROWS_HOST = CONV_ROWS;
+constexpr int NL = 12;                 // convolutions
+constexpr int WSZ = 9 * 32 * 32;       // weights per convolution
+constexpr int PSZ = WSZ + 32;          // parameters per convolution
+enum { C0A, C0B, C1A, C1B, C2A, C2B, C3A, C3B, C2C, C1C, C0C, COUT };
+constexpr int WGRAD_GROUPS = 512;
+
+}  // namespace helm
+
+struct helm_net {
+    int H = 0, W = 0, batch = 0;
+    bool training = false;
+    helm::bf16* wb = nullptr;   // [NL][WSZ] bf16
+    helm::bf16* wf = nullptr;   // [NL][WSZ] flipped
+    float* bias = nullptr;      // [NL][32]
+    // activations (bf16, batch x level size x 32)
+    helm::bf16 *in = nullptr, *a[4] = {}, *e[4] = {}, *p[4] = {}, *s[3] = {}, *d[3] = {}, *out = nullptr;
+    // backward scratch
+    helm::bf16 *g0[4] = {}, *g1[4] = {}, *g2[4] = {};
+    float* wpart = nullptr;     // [WGRAD_GROUPS][9*1024]
+    float* bpart = nullptr;     // [WGRAD_GROUPS][32]
+    std::vector<void*> owned;
+};
+
+namespace helm {
+
+static int64_t lvl_elems(const helm_net* n, int l) {
+    return (int64_t)n->batch * (n->H >> l) * (n->W >> l) * 32;
+}
+
+static int32_t net_alloc(helm_net* n, void** ptr, size_t bytes) {
+    hipError_t e = hipMalloc(ptr, bytes + 256);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc (net): ") + hipGetErrorString(e));
+        return HELM_ERR_ALLOC;
+    }
+    n->owned.push_back(*ptr);
+    return HELM_OK;
+}
+
+static inline dim3 g1d(int64_t count) { return dim3((unsigned)((count + 255) / 256)); }
+
+static int32_t conv_fwd(helm_net* n, int layer, int l, const bf16* x, bf16* y, bool relu, hipStream_t s) {
+    const int H = n->H >> l, W = n->W >> l;
+    dim3 grid((W + 31) / 32, (H + CONV_ROWS_HOST - 1) / CONV_ROWS_HOST, n->batch);
+    const bf16* w = n->wb + (size_t)layer * WSZ;
+    const float* b = n->bias + layer * 32;
+    if (relu)
+        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
+    else
+        hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// dx = conv^T(dy) (no bias); optionally masked by relu'(act)
+static int32_t conv_bwd_data(helm_net* n, int layer, int l, const bf16* dy, bf16* dx, const bf16* act,
+                             hipStream_t s) {
+    const int H = n->H >> l, W = n->W >> l;
+    dim3 grid((W + 31) / 32, (H + CONV_ROWS_HOST - 1) / CONV_ROWS_HOST, n->batch);
+    hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, dy, n->wf + (size_t)layer * WSZ,
+                       (const float*)nullptr, dx);
+    if (act) hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), act, dx);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// grad[layer] = (dW, dbias) from input x and output gradient dy
+static int32_t conv_wgrad(helm_net* n, int layer, int l, const bf16* x, const bf16* dy, float* grad,
+                          hipStream_t s) {
+    const int H = n->H >> l, W = n->W >> l;
+    const int tx = (W + 31) / 32, ty = (H + 3) / 4;
+    const int64_t tiles = (int64_t)tx * ty * n->batch;
+    const int groups = (int)std::min<int64_t>(WGRAD_GROUPS, tiles);
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(groups), dim3(256), 0, s, n->batch, H, W, x, dy, tx, ty, tiles, n->wpart,
+                       n->bpart);
+    hipLaunchKernelGGL(k_sum_partials, g1d(WSZ), dim3(256), 0, s, groups, WSZ, n->wpart, grad + (size_t)layer * PSZ);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(32), 0, s, groups, 32, n->bpart,
+                       grad + (size_t)layer * PSZ + WSZ);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t net_forward(helm_net* n, hipStream_t s) {
+    HELM_TRY(conv_fwd(n, C0A, 0, n->in, n->a[0], true, s));
+    HELM_TRY(conv_fwd(n, C0B, 0, n->a[0], n->e[0], true, s));
+    for (int l = 1; l < 4; ++l) {
+        hipLaunchKernelGGL(k_pool2, g1d(lvl_elems(n, l)), dim3(256), 0, s, n->batch, n->H >> (l - 1), n->W >> (l - 1),
+                           n->e[l - 1], n->p[l]);
+        HELM_TRY(conv_fwd(n, C0A + 2 * l, l, n->p[l], n->a[l], true, s));
+        HELM_TRY(conv_fwd(n, C0B + 2 * l, l, n->a[l], n->e[l], true, s));
+    }
+    const int up_layers[3] = {C0C, C1C, C2C};
+    const bf16* coarse = n->e[3];
+    for (int l = 2; l >= 0; --l) {
+        hipLaunchKernelGGL(k_up2_add, g1d(lvl_elems(n, l)), dim3(256), 0, s, n->batch, n->H >> l, n->W >> l, coarse,
+                           n->e[l], n->s[l]);
+        HELM_TRY(conv_fwd(n, up_layers[l], l, n->s[l], n->d[l], true, s));
+        coarse = n->d[l];
+    }
+    HELM_TRY(conv_fwd(n, COUT, 0, n->d[0], n->out, false, s));
+    return HELM_OK;
+}
+
+// dout -> parameter gradients (grad [NL][PSZ] fp32, overwritten)
+int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) {
+    // g0[l], g1[l], g2[l]: level-l scratch
+    HELM_TRY(conv_wgrad(n, COUT, 0, n->d[0], dout, grad, s));
+    HELM_TRY(conv_bwd_data(n, COUT, 0, dout, n->g0[0], n->d[0], s));        // dd0
+    const int up_layers[3] = {C0C, C1C, C2C};
+    // decoder: level l: dd_l in g0[l] -> ds_l in g1[l] (also the skip gradient de_l)
+    for (int l = 0; l <= 2; ++l) {
+        HELM_TRY(conv_wgrad(n, up_layers[l], l, n->s[l], n->g0[l], grad, s));
+        HELM_TRY(conv_bwd_data(n, up_layers[l], l, n->g0[l], n->g1[l], nullptr, s));   // ds_l
+        // gradient to the coarser decoder output (or e3): up_bwd(ds_l)
+        hipLaunchKernelGGL(k_up2_bwd, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, n->batch, n->H >> l, n->W >> l,
+                           n->g1[l], n->g0[l + 1]);
+        const bf16* act = (l < 2) ? n->d[l + 1] : n->e[3];
+        hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, lvl_elems(n, l + 1), act,
+                           n->g0[l + 1]);
+    }
+    // g0[3] = de3 (masked).  encoder from the bottom up; g1[l] holds the skip gradient de_l (l <= 2)
+    for (int l = 3; l >= 0; --l) {
+        bf16* de = n->g0[l];
+        if (l < 3) {
+            // de_l = skip gradient (g1[l]) + pool_bwd(dp_{l+1}) (in g2[l]) , then relu mask
+            hipLaunchKernelGGL(k_add_bf16, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->g2[l], n->g1[l]);
+            hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->e[l], n->g1[l]);
+            de = n->g1[l];
+        }
+        const bf16* xin = (l == 0) ? n->in : n->p[l];
+        HELM_TRY(conv_wgrad(n, C0B + 2 * l, l, n->a[l], de, grad, s));
+        HELM_TRY(conv_bwd_data(n, C0B + 2 * l, l, de, n->g2[l], n->a[l], s));    // da_l (masked)
+        HELM_TRY(conv_wgrad(n, C0A + 2 * l, l, xin, n->g2[l], grad, s));
+        if (l > 0) {
+            HELM_TRY(conv_bwd_data(n, C0A + 2 * l, l, n->g2[l], n->g0[l], nullptr, s));  // dp_l (reuse g0[l])
+            hipLaunchKernelGGL(k_pool2_bwd, g1d(lvl_elems(n, l - 1)), dim3(256), 0, s, n->batch, n->H >> (l - 1),
+                               n->W >> (l - 1), n->g0[l], n->g2[l - 1]);
+        }
+    }
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// ---- feature / output maps between the solver's fields and the net --------
+// features for RHS r on a canvas H x W (grid nz x nx at the top-left, zero
+// elsewhere).  scale[r] = RMS of the residual (computed by k_rms).
+__global__ void k_features(int nz, int nx, int H, int W, int nrhs, const float2* __restrict__ v,
+                           const float* __restrict__ inv_scale, const float* __restrict__ eta,
+                           const float2* __restrict__ kap, const float2* __restrict__ sx,
+                           const float2* __restrict__ sz, float kh, bf16* __restrict__ in) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nrhs * H * W) return;
+    const int j = (int)(q % W), i = (int)((q / W) % H), r = (int)(q / ((int64_t)W * H));
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < nz && j < nx) {
+        const int64_t idx = (int64_t)i * nx + j;
+        const float2 u = v[(int64_t)r * nz * nx + idx];
+        const float is = inv_scale[r];
+        const float2 k = kap[idx];
+        f[0] = u.x * is;
+        f[1] = u.y * is;
+        f[2] = logf(eta[idx]);
+        f[3] = logf(fmaxf(k.x, 1e-6f));
+        f[4] = k.y / fmaxf(k.x, 1e-6f);
+        f[5] = sx[j].y + sz[i].y;
+        f[6] = kh;
+        f[7] = 1.f;
+    }
+    bf16* o = in + q * 32;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o[c] = (bf16)f[c];
+#pragma unroll
+    for (int c = 8; c < 32; ++c) o[c] = (bf16)0.f;
+}
+
+// inv_scale[r] = 1 / RMS(v_r)  (one workgroup per RHS)
+__global__ void k_rms(int64_t n, const float2* __restrict__ v, float* __restrict__ inv_scale) {
+    __shared__ double red[256];
+    const int r = blockIdx.x;
+    double s = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+        const float2 a = v[(int64_t)r * n + q];
+        s += (double)a.x * a.x + (double)a.y * a.y;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(red[0] / (double)n);
+        inv_scale[r] = rms > 0 ? (float)(1.0 / rms) : 0.f;
+    }
+}
+
+// z[r] += (out[...,0] + i out[...,1]) / inv_scale[r]   (crop the canvas)
+__global__ void k_net_output(int nz, int nx, int H, int W, int nrhs, const bf16* __restrict__ out,
+                             const float* __restrict__ inv_scale, float2* __restrict__ z, int accumulate) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nrhs * nz * nx) return;
+    const int j = (int)(q % nx), i = (int)((q / nx) % nz), r = (int)(q / ((int64_t)nx * nz));
+    const bf16* o = out + (((int64_t)r * H + i) * W + j) * 32;
+    const float is = inv_scale[r];
+    const float sc = is > 0 ? 1.f / is : 0.f;
+    float2 v = make_float2((float)o[0] * sc, (float)o[1] * sc);
+    if (accumulate) v = cadd(v, z[q]);
+    z[q] = v;
+}
+
+int32_t net_correction(helm_plan* p, helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s) {
+    const Level& L = p->levels[0];
+    HELM_CHECK(n->H >= L.nz && n->W >= L.nx && n->batch >= nrhs, HELM_ERR_ARG,
+               "learned preconditioner: net canvas or batch too small for this plan");
+    if (!p->net_scale) HELM_HIP(hipMalloc((void**)&p->net_scale, 4096 * sizeof(float)));
+    hipLaunchKernelGGL(k_rms, dim3(nrhs), dim3(256), 0, s, L.n, v, p->net_scale);
+    const int saved = n->batch;
+    n->batch = nrhs;
+    const int64_t cnt = (int64_t)nrhs * n->H * n->W;
+    hipLaunchKernelGGL(k_features, g1d(cnt), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, v, p->net_scale,
+                       L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), n->in);
+    int32_t st = net_forward(n, s);
+    n->batch = saved;
+    HELM_TRY(st);
+    hipLaunchKernelGGL(k_net_output, g1d((int64_t)nrhs * L.n), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, n->out,
+                       p->net_scale, z, 1);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+}  // namespace helm
+
+using namespace helm;
+
+extern "C" {
+
+int32_t helm_net_create(int32_t H, int32_t W, int32_t batch, int32_t training, helm_net** out) {
+    HELM_CHECK(out, HELM_ERR_ARG, "helm_net_create: null argument");
+    *out = nullptr;
+    HELM_CHECK(H >= 8 && W >= 8 && H % 8 == 0 && W % 8 == 0, HELM_ERR_ARG,
+               "helm_net_create: H and W must be positive multiples of 8");
+    HELM_CHECK(batch >= 1 && batch <= 4096, HELM_ERR_ARG, "helm_net_create: batch must be in [1, 4096]");
+    HELM_CHECK((int64_t)batch * H * W * 32 < ((int64_t)1 << 40), HELM_ERR_ARG, "helm_net_create: too large");
+    helm_net* n = new helm_net();
+    n->H = H;
+    n->W = W;
+    n->batch = batch;
+    n->training = training != 0;
+    auto fail = [&](int32_t st) {
+        helm_net_destroy(n);
+        return st;
+    };
+    int32_t st;
+#define NET_ALLOC(ptr, bytes)                                          \
+    if ((st = net_alloc(n, (void**)&(ptr), (size_t)(bytes))) != HELM_OK) \
+    return fail(st)
+    NET_ALLOC(n->wb, NL * WSZ * sizeof(bf16));
+    NET_ALLOC(n->wf, NL * WSZ * sizeof(bf16));
+    NET_ALLOC(n->bias, NL * 32 * sizeof(float));
+    NET_ALLOC(n->in, lvl_elems(n, 0) * sizeof(bf16));
+    NET_ALLOC(n->out, lvl_elems(n, 0) * sizeof(bf16));
+    for (int l = 0; l < 4; ++l) {
+        NET_ALLOC(n->a[l], lvl_elems(n, l) * sizeof(bf16));
+        NET_ALLOC(n->e[l], lvl_elems(n, l) * sizeof(bf16));
+        if (l > 0) NET_ALLOC(n->p[l], lvl_elems(n, l) * sizeof(bf16));
+        if (l < 3) {
+            NET_ALLOC(n->s[l], lvl_elems(n, l) * sizeof(bf16));
+            NET_ALLOC(n->d[l], lvl_elems(n, l) * sizeof(bf16));
+        }
+        if (n->training) {
+            NET_ALLOC(n->g0[l], lvl_elems(n, l) * sizeof(bf16));
+            NET_ALLOC(n->g1[l], lvl_elems(n, l) * sizeof(bf16));
+            NET_ALLOC(n->g2[l], lvl_elems(n, l) * sizeof(bf16));
+        }
+    }
+    if (n->training) {
+        NET_ALLOC(n->wpart, (size_t)WGRAD_GROUPS * 9 * 1024 * sizeof(float));
+        NET_ALLOC(n->bpart, (size_t)WGRAD_GROUPS * 32 * sizeof(float));
+    }
+#undef NET_ALLOC
+    *out = n;
+    return HELM_OK;
+}
+
+int32_t helm_net_destroy(helm_net* n) {
+    if (!n) return HELM_OK;
+    (void)hipDeviceSynchronize();
+    for (void* q : n->owned) (void)hipFree(q);
+    delete n;
+    return HELM_OK;
+}
+
+int32_t helm_net_param_count(void) { return NL * PSZ; }
+
+// params: device fp32 [NL][9*32*32 weights (t, out, in) | 32 biases]
+int32_t helm_net_set_params(helm_net* n, const float* params, void* stream) {
+    HELM_CHECK(n && params, HELM_ERR_ARG, "helm_net_set_params: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    for (int l = 0; l < NL; ++l) {
+        hipLaunchKernelGGL(k_to_bf16, g1d(WSZ), dim3(256), 0, s, (int64_t)WSZ, params + (size_t)l * PSZ,
+                           n->wb + (size_t)l * WSZ);
+        hipLaunchKernelGGL(k_flip_weights, g1d(WSZ), dim3(256), 0, s, n->wb + (size_t)l * WSZ, n->wf + (size_t)l * WSZ);
+        HELM_HIP(hipMemcpyAsync(n->bias + l * 32, params + (size_t)l * PSZ + WSZ, 32 * sizeof(float),
+                                hipMemcpyDeviceToDevice, s));
+    }
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// in: device bf16 [batch][H][W][32] (copied into the net); out: bf16 [batch][H][W][32]
+int32_t helm_net_forward(helm_net* n, const void* in, void* out, int32_t batch, void* stream) {
+    HELM_CHECK(n && in, HELM_ERR_ARG, "helm_net_forward: null argument");
+    HELM_CHECK(batch >= 1 && batch <= n->batch, HELM_ERR_ARG, "helm_net_forward: batch out of range");
+    hipStream_t s = (hipStream_t)stream;
+    const int saved = n->batch;
+    n->batch = batch;
+    HELM_HIP(hipMemcpyAsync(n->in, in, lvl_elems(n, 0) * sizeof(bf16), hipMemcpyDeviceToDevice, s));
+    int32_t st = net_forward(n, s);
+    if (st == HELM_OK && out)
+        st = hipMemcpyAsync(out, n->out, lvl_elems(n, 0) * sizeof(bf16), hipMemcpyDeviceToDevice, s) == hipSuccess
+                 ? HELM_OK
+                 : HELM_ERR_HIP;
+    n->batch = saved;
+    return st;
+}
+
+// dout: bf16 [batch][H][W][32]; grad: fp32 [NL][PSZ] (overwritten).  Uses the
+// activations of the last helm_net_forward with the same batch.
+int32_t helm_net_backward(helm_net* n, const void* dout, float* grad, int32_t batch, void* stream) {
+    HELM_CHECK(n && dout && grad, HELM_ERR_ARG, "helm_net_backward: null argument");
+    HELM_CHECK(n->training, HELM_ERR_STATE, "helm_net_backward: net created without training buffers");
+    HELM_CHECK(batch >= 1 && batch <= n->batch, HELM_ERR_ARG, "helm_net_backward: batch out of range");
+    const int saved = n->batch;
+    n->batch = batch;
+    int32_t st = net_backward(n, (const bf16*)dout, grad, (hipStream_t)stream);
+    n->batch = saved;
+    return st;
+}
+
+int32_t helm_adam_step(int64_t count, float* params, const float* grad, float* m, float* v, int32_t step, double lr,
+                       double beta1, double beta2, double eps, double grad_scale, void* stream) {
+    HELM_CHECK(params && grad && m && v && count > 0 && step >= 1, HELM_ERR_ARG, "helm_adam_step: bad argument");
+    const double c1 = 1.0 - pow(beta1, step), c2 = 1.0 - pow(beta2, step);
+    hipLaunchKernelGGL(k_adam, g1d(count), dim3(256), 0, (hipStream_t)stream, count, params, grad, m, v, (float)lr,
+                       (float)beta1, (float)beta2, (float)eps, (float)c1, (float)c2, (float)grad_scale);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t helm_plan_set_net(helm_plan* p, helm_net* n) {
+    HELM_CHECK(p, HELM_ERR_ARG, "helm_plan_set_net: null plan");
+    p->net = n;
+    return HELM_OK;
+}
+
+// z = N(v) (the net's correction alone) for nrhs c64 fields of the plan's grid
+int32_t helm_net_correction(helm_plan* p, const void* v, void* z, int32_t nrhs, void* stream) {
+    HELM_CHECK(p && v && z, HELM_ERR_ARG, "helm_net_correction: null argument");
+    HELM_CHECK(p->net, HELM_ERR_STATE, "helm_net_correction: no net attached (helm_plan_set_net)");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_net_correction: call helm_set_medium first");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t cnt = (int64_t)nrhs * p->levels[0].n;
+    HELM_HIP(hipMemsetAsync(z, 0, cnt * sizeof(float2), s));
+    return net_correction(p, p->net, (const float2*)v, (float2*)z, nrhs, s);
+}
+
+// the feature tensor the net sees for these residuals (bf16 [nrhs][H][W][32])
+int32_t helm_net_features(helm_plan* p, const void* v, void* features, int32_t nrhs, void* stream) {
+    HELM_CHECK(p && v && features && p->net, HELM_ERR_ARG, "helm_net_features: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    const Level& L = p->levels[0];
+    helm_net* n = p->net;
+    if (!p->net_scale) HELM_HIP(hipMalloc((void**)&p->net_scale, 4096 * sizeof(float)));
+    hipLaunchKernelGGL(k_rms, dim3(nrhs), dim3(256), 0, s, L.n, (const float2*)v, p->net_scale);
+    const int64_t cnt = (int64_t)nrhs * n->H * n->W;
+    hipLaunchKernelGGL(k_features, g1d(cnt), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, (const float2*)v,
+                       p->net_scale, L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), (bf16*)features);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+}  // extern "C"

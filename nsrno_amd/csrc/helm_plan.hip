@@ -186,6 +186,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     for (auto& L : p->levels) free_level(L);
     free_workspace(p->ws);
     if (p->mg_views) (void)hipFree(p->mg_views);
+    if (p->net_scale) (void)hipFree(p->net_scale);
     delete p;
     return HELM_OK;
 }

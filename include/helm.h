@@ -168,6 +168,9 @@ typedef struct helm_net helm_net;
 int32_t helm_net_create(int32_t H, int32_t W, int32_t batch, int32_t training, helm_net** net);
 int32_t helm_net_destroy(helm_net* net);
 int32_t helm_net_param_count(void);
+/* linear != 0: identity activations and only the residual channels as input,
+ * so N(v) is a linear operator of v (keep the biases at zero) */
+int32_t helm_net_set_linear(helm_net* net, int32_t linear);
 int32_t helm_net_set_params(helm_net* net, const float* params, void* stream);
 int32_t helm_net_forward(helm_net* net, const void* in, void* out, int32_t batch, void* stream);
 int32_t helm_net_backward(helm_net* net, const void* dout, float* grad, int32_t batch, void* stream);

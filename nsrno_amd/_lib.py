@@ -57,6 +57,7 @@ SIGNATURES = {
     "helm_net_create": (c_int32, [c_int32, c_int32, c_int32, c_int32, POINTER(c_void_p)]),
     "helm_net_destroy": (c_int32, [c_void_p]),
     "helm_net_param_count": (c_int32, []),
+    "helm_net_set_linear": (c_int32, [c_void_p, c_int32]),
     "helm_net_set_params": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "helm_net_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_net_backward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),

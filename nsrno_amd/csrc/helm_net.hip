@@ -49,6 +49,7 @@ constexpr int WGRAD_GROUPS = 512;
 struct helm_net {
     int H = 0, W = 0, batch = 0;
     bool training = false;
+    bool linear = false;        // identity activations, residual channels only: N is linear in v
     helm::bf16* wb = nullptr;   // [NL][WSZ] bf16
     helm::bf16* wf = nullptr;   // [NL][WSZ] flipped
     float* bias = nullptr;      // [NL][32]
@@ -84,7 +85,7 @@ static int32_t conv_fwd(helm_net* n, int layer, int l, const bf16* x, bf16* y, b
     dim3 grid((W + 31) / 32, (H + CONV_ROWS_HOST - 1) / CONV_ROWS_HOST, n->batch);
     const bf16* w = n->wb + (size_t)layer * WSZ;
     const float* b = n->bias + layer * 32;
-    if (relu)
+    if (relu && !n->linear)
         hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
     else
         hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
@@ -99,7 +100,8 @@ static int32_t conv_bwd_data(helm_net* n, int layer, int l, const bf16* dy, bf16
     dim3 grid((W + 31) / 32, (H + CONV_ROWS_HOST - 1) / CONV_ROWS_HOST, n->batch);
     hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, dy, n->wf + (size_t)layer * WSZ,
                        (const float*)nullptr, dx);
-    if (act) hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), act, dx);
+    if (act && !n->linear)
+        hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), act, dx);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
@@ -155,8 +157,9 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
         hipLaunchKernelGGL(k_up2_bwd, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, n->batch, n->H >> l, n->W >> l,
                            n->g1[l], n->g0[l + 1]);
         const bf16* act = (l < 2) ? n->d[l + 1] : n->e[3];
-        hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, lvl_elems(n, l + 1), act,
-                           n->g0[l + 1]);
+        if (!n->linear)
+            hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, lvl_elems(n, l + 1), act,
+                               n->g0[l + 1]);
     }
     // g0[3] = de3 (masked).  encoder from the bottom up; g1[l] holds the skip gradient de_l (l <= 2)
     for (int l = 3; l >= 0; --l) {
@@ -164,7 +167,9 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
         if (l < 3) {
             // de_l = skip gradient (g1[l]) + pool_bwd(dp_{l+1}) (in g2[l]) , then relu mask
             hipLaunchKernelGGL(k_add_bf16, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->g2[l], n->g1[l]);
-            hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->e[l], n->g1[l]);
+            if (!n->linear)
+                hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->e[l],
+                                   n->g1[l]);
             de = n->g1[l];
         }
         const bf16* xin = (l == 0) ? n->in : n->p[l];
@@ -187,7 +192,7 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
 __global__ void k_features(int nz, int nx, int H, int W, int nrhs, const float2* __restrict__ v,
                            const float* __restrict__ inv_scale, const float* __restrict__ eta,
                            const float2* __restrict__ kap, const float2* __restrict__ sx,
-                           const float2* __restrict__ sz, float kh, bf16* __restrict__ in) {
+                           const float2* __restrict__ sz, float kh, int linear, bf16* __restrict__ in) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)nrhs * H * W) return;
     const int j = (int)(q % W), i = (int)((q / W) % H), r = (int)(q / ((int64_t)W * H));
@@ -196,9 +201,12 @@ __global__ void k_features(int nz, int nx, int H, int W, int nrhs, const float2*
         const int64_t idx = (int64_t)i * nx + j;
         const float2 u = v[(int64_t)r * nz * nx + idx];
         const float is = inv_scale[r];
-        const float2 k = kap[idx];
         f[0] = u.x * is;
         f[1] = u.y * is;
+    }
+    if (i < nz && j < nx && !linear) {
+        const int64_t idx = (int64_t)i * nx + j;
+        const float2 k = kap[idx];
         f[2] = logf(eta[idx]);
         f[3] = logf(fmaxf(k.x, 1e-6f));
         f[4] = k.y / fmaxf(k.x, 1e-6f);
@@ -234,15 +242,18 @@ __global__ void k_rms(int64_t n, const float2* __restrict__ v, float* __restrict
     }
 }
 
-// z[r] += (out[...,0] + i out[...,1]) / inv_scale[r]   (crop the canvas)
+// z[r] += (out[...,0] + i out[...,1]) * h^2 / inv_scale[r]   (crop the canvas).
+// The output unit h^2 RMS(v) is the size of the local inverse of the stencil
+// (its diagonal is about -4/h^2), so O(1) network outputs are O(1) corrections.
 __global__ void k_net_output(int nz, int nx, int H, int W, int nrhs, const bf16* __restrict__ out,
-                             const float* __restrict__ inv_scale, float2* __restrict__ z, int accumulate) {
+                             const float* __restrict__ inv_scale, float h2, float2* __restrict__ z,
+                             int accumulate) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)nrhs * nz * nx) return;
     const int j = (int)(q % nx), i = (int)((q / nx) % nz), r = (int)(q / ((int64_t)nx * nz));
     const bf16* o = out + (((int64_t)r * H + i) * W + j) * 32;
     const float is = inv_scale[r];
-    const float sc = is > 0 ? 1.f / is : 0.f;
+    const float sc = is > 0 ? h2 / is : 0.f;
     float2 v = make_float2((float)o[0] * sc, (float)o[1] * sc);
     if (accumulate) v = cadd(v, z[q]);
     z[q] = v;
@@ -258,12 +269,12 @@ int32_t net_correction(helm_plan* p, helm_net* n, const float2* v, float2* z, in
     n->batch = nrhs;
     const int64_t cnt = (int64_t)nrhs * n->H * n->W;
     hipLaunchKernelGGL(k_features, g1d(cnt), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, v, p->net_scale,
-                       L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), n->in);
+                       L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), (int)n->linear, n->in);
     int32_t st = net_forward(n, s);
     n->batch = saved;
     HELM_TRY(st);
     hipLaunchKernelGGL(k_net_output, g1d((int64_t)nrhs * L.n), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, n->out,
-                       p->net_scale, z, 1);
+                       p->net_scale, (float)(L.h_m * L.h_m), z, 1);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
@@ -331,6 +342,12 @@ int32_t helm_net_destroy(helm_net* n) {
 }
 
 int32_t helm_net_param_count(void) { return NL * PSZ; }
+
+int32_t helm_net_set_linear(helm_net* n, int32_t linear) {
+    HELM_CHECK(n, HELM_ERR_ARG, "helm_net_set_linear: null net");
+    n->linear = linear != 0;
+    return HELM_OK;
+}
 
 // params: device fp32 [NL][9*32*32 weights (t, out, in) | 32 biases]
 int32_t helm_net_set_params(helm_net* n, const float* params, void* stream) {
@@ -414,7 +431,8 @@ int32_t helm_net_features(helm_plan* p, const void* v, void* features, int32_t n
     hipLaunchKernelGGL(k_rms, dim3(nrhs), dim3(256), 0, s, L.n, (const float2*)v, p->net_scale);
     const int64_t cnt = (int64_t)nrhs * n->H * n->W;
     hipLaunchKernelGGL(k_features, g1d(cnt), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, (const float2*)v,
-                       p->net_scale, L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), (bf16*)features);
+                       p->net_scale, L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), (int)n->linear,
+                       (bf16*)features);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

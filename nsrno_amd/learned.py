@@ -48,7 +48,7 @@ def init_params(seed=0, out_scale=0.1):
 
 
 class LearnedNet:
-    def __init__(self, H, W, batch, training=False, params=None, device="cuda"):
+    def __init__(self, H, W, batch, training=False, params=None, device="cuda", linear=False):
         if not torch.cuda.is_available():
             raise RuntimeError("LearnedNet needs a GPU")
         self.H, self.W, self.batch = int(H), int(W), int(batch)
@@ -57,6 +57,8 @@ class LearnedNet:
         check(lib().helm_net_create(self.H, self.W, self.batch, int(bool(training)), ctypes.byref(h)),
               "helm_net_create")
         self._h = h
+        self.linear = bool(linear)
+        check(lib().helm_net_set_linear(h, int(self.linear)), "helm_net_set_linear")
         assert lib().helm_net_param_count() == NL * PSZ
         p = init_params() if params is None else params
         self.params = torch.as_tensor(np.asarray(p, dtype=np.float32), device=self.device).contiguous()
@@ -96,6 +98,8 @@ class LearnedNet:
     def adam(self, lr=2e-3, beta1=0.9, beta2=0.999, eps=1e-8, clip=None):
         """One Adam step on self.grad (global-norm clipping as in train_preconditioner.py:75)."""
         scale = 1.0
+        if self.linear:  # biases stay zero so that N is linear
+            self.grad.view(NL, PSZ)[:, WSZ:] = 0
         if clip is not None:
             gn = float(torch.linalg.vector_norm(self.grad))
             if gn > clip:
@@ -106,12 +110,17 @@ class LearnedNet:
         self.sync_params()
 
     def save(self, path):
-        np.savez_compressed(path, params=self.params.cpu().numpy(), H=self.H, W=self.W)
+        np.savez_compressed(path, params=self.params.cpu().numpy(), H=self.H, W=self.W, linear=self.linear)
 
     @staticmethod
     def load_params(path):
         with np.load(path, allow_pickle=False) as z:
             return z["params"].astype(np.float32)
+
+    @staticmethod
+    def load_linear(path):
+        with np.load(path, allow_pickle=False) as z:
+            return bool(z["linear"]) if "linear" in z.files else False
 
 
 def attach(plan, net: LearnedNet | None):

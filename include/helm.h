@@ -143,6 +143,16 @@ int32_t helm_sample(helm_plan* plan, int32_t n_ports, const int32_t* port_ptr,
 int32_t helm_solve(helm_plan* plan, const void* rhs, void* x, int32_t nrhs,
                    const helm_solve_opts* opts, helm_solve_info* info, void* stream);
 
+/* ---- adjoint gradient: replaces the material-gradient step of
+ * dense_wave.ImplicitChannels.backward (nsrno/dense_wave.py:240-256).
+ * u, lam: c128 [nrhs][N] forward and adjoint (A^H lam = P^T g) fields;
+ * s_eta, s_kap: c128 [N] outputs,
+ *   s_kap[q] = sum_r conj(lam_rq) k0^2 sz sx u_rq,
+ *   s_eta[q] = sum_r lam_r^H (dA/d eta_rel[q]) u_r,
+ * so dL/dm = -Re(sum_q s_eta[q] d eta_rel[q]/dm + s_kap[q] d kappa_c[q]/dm). */
+int32_t helm_material_sensitivity(helm_plan* plan, const void* u, const void* lam, int32_t nrhs,
+                                  void* s_eta, void* s_kap, void* stream);
+
 /* ---- preconditioners --------------------------------------------------- */
 /* Complex-shifted-Laplacian multigrid: kappa_c -> kappa_c (1 + i shift).
  * levels >= 1 (1: smoother only), pre/post Jacobi sweeps, coarse sweeps on

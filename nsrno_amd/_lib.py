@@ -54,6 +54,7 @@ SIGNATURES = {
                              POINTER(HelmSolveInfo), c_void_p]),
     "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
     "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_material_sensitivity": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "helm_net_create": (c_int32, [c_int32, c_int32, c_int32, c_int32, POINTER(c_void_p)]),
     "helm_net_destroy": (c_int32, [c_void_p]),
     "helm_net_param_count": (c_int32, []),

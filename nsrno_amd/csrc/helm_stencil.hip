@@ -197,3 +197,63 @@ int32_t helm_sample(helm_plan* p, int32_t n_ports, const int32_t* port_ptr, cons
 }
 
 }  // extern "C"
+
+// ---- material sensitivities for the adjoint gradient ---------------------
+// For fields u_r and adjoint fields lam_r (A^H lam_r = P^T g_r):
+//   s_kap[q] = sum_r conj(lam_rq) (dA/dkappa_q u_r)  = sum_r conj(lam_rq) k0^2 sz_i sx_j u_rq
+//   s_eta[q] = sum_r lam_r^H (dA/deta_q) u_r
+//            = sum_r sum_{edges e at q} stretch_e/(2h^2) conj(lam_q - lam_nb)(u_nb - u_q)
+// (ghost neighbours: lam = u = 0).  The gradient of L(u(m)) is then
+// dL/dm = -Re(sum_q s_eta[q] deta_q/dm + s_kap[q] dkappa_q/dm), the discrete
+// adjoint-state formula of nsrno/dense_wave.py:240-256 (there obtained by
+// autograd of -Re<lam, A u - s>) and of the Born Jacobian products of
+// nsrno/theranostic/wave.py:372-388.
+namespace helm {
+__global__ void k_material_sens(int nz, int nx, int64_t n, int nrhs, const double2* __restrict__ sx,
+                                const double2* __restrict__ sz, const double2* __restrict__ isxe,
+                                const double2* __restrict__ isze, double k0sq, double hh,
+                                const double2* __restrict__ u, const double2* __restrict__ lam,
+                                double2* __restrict__ s_eta, double2* __restrict__ s_kap) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nz || j >= nx) return;
+    const int64_t q = (int64_t)i * nx + j;
+    const double2 szi = sz[i], sxj = sx[j];
+    const double2 sw = cscale(cmul(szi, isxe[j]), hh), se = cscale(cmul(szi, isxe[j + 1]), hh);
+    const double2 sn = cscale(cmul(sxj, isze[i]), hh), ss = cscale(cmul(sxj, isze[i + 1]), hh);
+    const double2 mass = cscale(cmul(szi, sxj), k0sq);
+    double2 ae = make_double2(0.0, 0.0), ak = make_double2(0.0, 0.0);
+    for (int r = 0; r < nrhs; ++r) {
+        const double2* ur = u + (int64_t)r * n;
+        const double2* lr = lam + (int64_t)r * n;
+        const double2 uq = ur[q], lq = cconj(lr[q]);
+        cfma(ak, lq, cmul(mass, uq));
+        auto edge = [&](bool inside, int64_t nb, double2 st) {
+            const double2 un = inside ? ur[nb] : make_double2(0.0, 0.0);
+            const double2 ln = inside ? cconj(lr[nb]) : make_double2(0.0, 0.0);
+            cfma(ae, cmul(st, csub(lq, ln)), csub(un, uq));
+        };
+        edge(j > 0, q - 1, sw);
+        edge(j < nx - 1, q + 1, se);
+        edge(i > 0, q - nx, sn);
+        edge(i < nz - 1, q + nx, ss);
+    }
+    s_eta[q] = ae;
+    s_kap[q] = ak;
+}
+}  // namespace helm
+
+extern "C" int32_t helm_material_sensitivity(helm_plan* p, const void* u, const void* lam, int32_t nrhs,
+                                             void* s_eta, void* s_kap, void* stream) {
+    HELM_CHECK(p && u && lam && s_eta && s_kap, HELM_ERR_ARG, "helm_material_sensitivity: null argument");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_material_sensitivity: call helm_set_medium first");
+    HELM_CHECK(nrhs >= 1 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_material_sensitivity: nrhs out of range");
+    const Level& L = p->levels[0];
+    dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+    hipLaunchKernelGGL(k_material_sens, grid, block, 0, (hipStream_t)stream, L.nz, L.nx, L.n, nrhs, L.sx64, L.sz64,
+                       L.isxe64, L.isze64, p->k0 * p->k0, 0.5 / (L.h_m * L.h_m), (const double2*)u,
+                       (const double2*)lam, (double2*)s_eta, (double2*)s_kap);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}

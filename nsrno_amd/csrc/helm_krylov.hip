@@ -5,11 +5,11 @@
 // Gram-Schmidt applied twice, convergence decided on the true residual at
 // each restart, reasons converged / iteration_budget), with these changes
 // made for the GPU:
-//   * classical Gram-Schmidt with the second pass taken only where the first
-//     cancelled heavily (||w'|| < 0.7 ||w||, Kahan-Parlett); the decision is
-//     made on the device per right-hand side, the second-pass kernels are
-//     always launched and return at once where it is not needed, so the
-//     launch sequence stays fixed.  Usually this halves the basis reads;
+//   * delayed classical Gram-Schmidt (DCGS2): the reorthogonalisation of V_j
+//     is folded into the projection of the next vector, so each step reads
+//     the basis twice (one fused dot pass, one fused update pass) instead of
+//     four times, with CGS2's orthogonality; the Hessenberg column of a step
+//     is finalised (corrected, then rotated) one step later;
 //   * every right-hand side has its own Krylov space; all are advanced in
 //     lock step by the same kernels (grid dimension = RHS), so one launch
 //     sequence serves the whole angle batch;
@@ -67,104 +67,255 @@ __device__ __forceinline__ double2 block_sum(double2 v, double* lds) {
 
 static inline int nchunks(int64_t n) { return (int)((n + CHUNK - 1) / CHUNK); }
 
-// partial[(r*KMAX + k)*nch + chunk] = sum_chunk conj(V_k[r]) * w[r]  for k < kcount
-// Also (wnorm != nullptr, first vector group) wnorm[r*nch+chunk] = |w|^2.
-// A right-hand side is skipped unless cyc[r] and (gate == nullptr or gate[r]).
-__global__ __launch_bounds__(256) void k_dots(int64_t n, int nrhs, int nch, int kmax, int kcount,
-                                              const float2* __restrict__ V, const float2* __restrict__ w,
-                                              const int* __restrict__ cyc, const int* __restrict__ gate,
-                                              double2* __restrict__ partial, double2* __restrict__ wnorm) {
+__device__ double reduce_chunks(const double2* src, int nch, double* lds) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int c = threadIdx.x; c < nch; c += blockDim.x) acc = cadd(acc, src[c]);
+    acc = block_sum(acc, lds);
+    return acc.x;
+}
+
+// ---- delayed classical Gram-Schmidt (DCGS2) --------------------------------
+// Step j holds V_0..V_{j-1} final and V_j orthogonalised once (provisional).
+// One read of the basis computes both the reorthogonalisation coefficients
+// of V_j, c_k = <V_k, V_j> (k < j, and <V_j, V_j>), and the projections of
+// w = A M V_j, d_k = <V_k, w> (k <= j).  A second read updates both vectors:
+//   V_j <- (V_j - sum c_k V_k) / alpha,          alpha^2 = |V_j|^2 - sum |c_k|^2
+//   w   <- w - sum_{k<j} d_k V_k - h_jj V_j,     h_jj = (d_j - sum conj(c_k) d_k) / alpha
+// and the previous Hessenberg column is corrected for the change of V_j,
+//   H[k][j-1] += H[j][j-1] c_k,   H[j][j-1] *= alpha,
+// after which its Givens rotation is applied (one step later than plain
+// Arnoldi).  Two basis reads per step instead of four, numerically a CGS2.
+
+// pc[(r*kmax + k)*nch + chunk] = <V_k, V_j>, pd[...] = <V_k, w>, k <= j
+__global__ __launch_bounds__(256) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax, int j,
+                                                   const float2* __restrict__ V, const float2* __restrict__ w,
+                                                   const int* __restrict__ cyc, double2* __restrict__ pc,
+                                                   double2* __restrict__ pd) {
     __shared__ double lds[8];
     const int chunk = blockIdx.x, r = blockIdx.z, k0 = blockIdx.y * KB;
-    if (!cyc[r] || (gate && !gate[r])) return;
-    const bool want_norm = wnorm != nullptr && blockIdx.y == 0;
-    double wn = 0.0;
-    const int kc = min(KB, kcount - k0);
-    double2 acc[KB];
+    if (!cyc[r]) return;
+    const int kc = min(KB, j + 1 - k0);
+    double2 ac[KB], ad[KB];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < KB; ++q) {
+        ac[q] = make_double2(0.0, 0.0);
+        ad[q] = make_double2(0.0, 0.0);
+    }
     const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
     const int64_t vstride = (int64_t)nrhs * n;
     const float2* wr = w + (int64_t)r * n;
+    const float2* vj = V + (int64_t)j * vstride + (int64_t)r * n;
     const float2* vr = V + (int64_t)k0 * vstride + (int64_t)r * n;
     for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
-        const float2 wv = wr[p];
-        wn += (double)wv.x * wv.x + (double)wv.y * wv.y;
+        const float2 wv = wr[p], jv = vj[p];
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             if (q < kc) {
                 const float2 v = vr[(int64_t)q * vstride + p];
-                acc[q].x += (double)v.x * wv.x + (double)v.y * wv.y;
-                acc[q].y += (double)v.x * wv.y - (double)v.y * wv.x;
+                ac[q].x += (double)v.x * jv.x + (double)v.y * jv.y;
+                ac[q].y += (double)v.x * jv.y - (double)v.y * jv.x;
+                ad[q].x += (double)v.x * wv.x + (double)v.y * wv.y;
+                ad[q].y += (double)v.x * wv.y - (double)v.y * wv.x;
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
         if (q < kc) {
-            double2 s = block_sum(acc[q], lds);
-            if (threadIdx.x == 0) partial[((int64_t)r * kmax + k0 + q) * nch + chunk] = s;
+            const double2 a = block_sum(ac[q], lds);
+            const double2 b = block_sum(ad[q], lds);
+            if (threadIdx.x == 0) {
+                pc[((int64_t)r * kmax + k0 + q) * nch + chunk] = a;
+                pd[((int64_t)r * kmax + k0 + q) * nch + chunk] = b;
+            }
         }
     }
-    if (want_norm) {
-        double2 s = block_sum(make_double2(wn, 0.0), lds);
-        if (threadIdx.x == 0) wnorm[(int64_t)r * nch + chunk] = s;
-    }
 }
 
-// coef[r][k] = sum over chunks; H column j accumulates it (pass 0 sets).
-__global__ void k_reduce_dots(int nrhs, int nch, int kmax, int kcount, int m, int j, int pass,
-                              const double2* __restrict__ partial, const int* __restrict__ cyc,
-                              const int* __restrict__ gate, double2* __restrict__ coef, double2* __restrict__ H) {
-    __shared__ double lds[8];
-    const int r = blockIdx.y, k = blockIdx.x;
-    if (!cyc[r] || k >= kcount || (gate && !gate[r])) return;
-    double2 acc = make_double2(0.0, 0.0);
-    const double2* src = partial + ((int64_t)r * kmax + k) * nch;
-    for (int c = threadIdx.x; c < nch; c += blockDim.x) acc = cadd(acc, src[c]);
-    acc = block_sum(acc, lds);
-    if (threadIdx.x == 0) {
-        coef[(int64_t)r * kmax + k] = acc;
-        double2* hc = H + ((int64_t)r * m + j) * (m + 1);
-        hc[k] = pass == 0 ? acc : cadd(hc[k], acc);
+// apply rotations 0..c-1 to column c, create rotation c, update g; returns |g_{c+1}|
+__device__ double givens_column(double2* hc, double* cs, double2* sn, double2* g, int c) {
+    for (int k = 0; k < c; ++k) {
+        const double2 a = hc[k], b = hc[k + 1];
+        hc[k] = cadd(cscale(a, cs[k]), cmul(sn[k], b));
+        hc[k + 1] = csub(cscale(b, cs[k]), cmul(cconj(sn[k]), a));
     }
+    const double2 a = hc[c];
+    const double b = hc[c + 1].x;  // real, nonnegative (a norm)
+    const double aa = sqrt(a.x * a.x + a.y * a.y);
+    const double rr = sqrt(aa * aa + b * b);
+    double ck;
+    double2 sk, rho;
+    if (aa == 0.0) {
+        ck = 0.0;
+        sk = make_double2(1.0, 0.0);
+        rho = make_double2(b, 0.0);
+    } else {
+        const double2 ph = make_double2(a.x / aa, a.y / aa);
+        ck = aa / rr;
+        sk = cscale(ph, b / rr);
+        rho = cscale(ph, rr);
+    }
+    cs[c] = ck;
+    sn[c] = sk;
+    hc[c] = rho;
+    hc[c + 1] = make_double2(0.0, 0.0);
+    const double2 gc = g[c];
+    g[c] = cscale(gc, ck);
+    g[c + 1] = cscale(cmul(cconj(sk), gc), -1.0);
+    return sqrt(g[c + 1].x * g[c + 1].x + g[c + 1].y * g[c + 1].y);
 }
 
-// w[r] -= sum_k coef[r][k] V_k[r]; optionally partial[r*nch+chunk] = |w|^2
-template <bool NORM>
-__global__ __launch_bounds__(256) void k_update(int64_t n, int nrhs, int nch, int kmax, int kcount,
-                                                const float2* __restrict__ V, float2* __restrict__ w,
-                                                const double2* __restrict__ coef, const int* __restrict__ cyc,
-                                                const int* __restrict__ gate, double2* __restrict__ partial) {
+// Per RHS: reduce the partials, correct and finalise column j-1 (Givens,
+// residual estimate, early stop), build provisional column j and the update
+// coefficients cf[r][0..j-1] = c_k, cf[r][kmax + k] = d_k, cf[r][2kmax] =
+// (h_jj, 1/alpha).
+__global__ void k_dcgs_reduce(int nch, int m, int kmax, int j, double tol, const double2* __restrict__ pc,
+                              const double2* __restrict__ pd, const double* __restrict__ bnorm,
+                              double2* __restrict__ H, double* __restrict__ cs, double2* __restrict__ sn,
+                              double2* __restrict__ g, double* __restrict__ est, int* __restrict__ cyc,
+                              int* __restrict__ used, double2* __restrict__ cf) {
     __shared__ double lds[8];
-    __shared__ float2 cf[64];
+    __shared__ double2 sc[64], sd[64];
+    const int r = blockIdx.x;
+    if (!cyc[r]) return;
+    for (int k = 0; k <= j; ++k) {
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        const double2* srcc = pc + ((int64_t)r * kmax + k) * nch;
+        const double2* srcd = pd + ((int64_t)r * kmax + k) * nch;
+        for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+            a = cadd(a, srcc[c]);
+            b = cadd(b, srcd[c]);
+        }
+        a = block_sum(a, lds);
+        b = block_sum(b, lds);
+        if (threadIdx.x == 0) {
+            sc[k] = a;
+            sd[k] = b;
+        }
+    }
+    if (threadIdx.x != 0) return;
+    double2* Hr = H + (int64_t)r * m * (m + 1);
+    double* csr = cs + (int64_t)r * m;
+    double2* snr = sn + (int64_t)r * m;
+    double2* gr = g + (int64_t)r * (m + 1);
+    double2* cfr = cf + (int64_t)r * (2 * kmax + 1);
+    double alpha = 1.0;
+    if (j > 0) {
+        double c2 = 0.0;
+        for (int k = 0; k < j; ++k) c2 += sc[k].x * sc[k].x + sc[k].y * sc[k].y;
+        alpha = sqrt(fmax(sc[j].x - c2, 1e-300));
+        // correct column j-1 for V_j = alpha V_j'' + sum c_k V_k, then finalise it
+        double2* hp = Hr + (int64_t)(j - 1) * (m + 1);
+        const double hn = hp[j].x;
+        for (int k = 0; k < j; ++k) hp[k] = cadd(hp[k], cscale(sc[k], hn));
+        hp[j] = make_double2(hn * alpha, 0.0);
+        const double e = givens_column(hp, csr, snr, gr, j - 1);
+        est[r] = e / bnorm[r];
+        used[r] = j;
+        if (e <= 0.5 * tol * bnorm[r] || !isfinite(e)) {
+            cyc[r] = 0;
+            return;
+        }
+    }
+    // provisional column j w.r.t. the final V_0..V_{j-1} and V_j''
+    double2* hc = Hr + (int64_t)j * (m + 1);
+    double2 hjj = sd[j];
+    for (int k = 0; k < j; ++k) {
+        hc[k] = sd[k];
+        hjj = csub(hjj, cmul(cconj(sc[k]), sd[k]));
+        cfr[k] = sc[k];
+        cfr[kmax + k] = sd[k];
+    }
+    hjj = cscale(hjj, 1.0 / alpha);
+    hc[j] = hjj;
+    cfr[2 * kmax] = make_double2(1.0 / alpha, 0.0);
+    cfr[2 * kmax - 1] = hjj;  // slot kmax - 1 + kmax: h_jj (j < kmax - 1 always)
+}
+
+// V_j <- (V_j - sum c_k V_k)/alpha ; w <- w - sum d_k V_k - h_jj V_j ; |w|^2 partials
+__global__ __launch_bounds__(256) void k_dcgs_update(int64_t n, int nrhs, int nch, int kmax, int j,
+                                                     float2* __restrict__ V, float2* __restrict__ w,
+                                                     const double2* __restrict__ cf, const int* __restrict__ cyc,
+                                                     double2* __restrict__ partial) {
+    __shared__ double lds[8];
+    __shared__ float2 c_s[64], d_s[64];
+    __shared__ float2 hjj_s;
+    __shared__ float ialpha_s;
     const int chunk = blockIdx.x, r = blockIdx.y;
-    if (!cyc[r] || (gate && !gate[r])) return;
-    for (int k = threadIdx.x; k < kcount; k += 256) {
-        double2 c = coef[(int64_t)r * kmax + k];
-        cf[k] = make_float2((float)c.x, (float)c.y);
+    if (!cyc[r]) return;
+    const double2* cfr = cf + (int64_t)r * (2 * kmax + 1);
+    for (int k = threadIdx.x; k < j; k += 256) {
+        c_s[k] = make_float2((float)cfr[k].x, (float)cfr[k].y);
+        d_s[k] = make_float2((float)cfr[kmax + k].x, (float)cfr[kmax + k].y);
+    }
+    if (threadIdx.x == 0) {
+        hjj_s = make_float2((float)cfr[2 * kmax - 1].x, (float)cfr[2 * kmax - 1].y);
+        ialpha_s = (float)cfr[2 * kmax].x;
     }
     __syncthreads();
     const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
     const int64_t vstride = (int64_t)nrhs * n;
     float2* wr = w + (int64_t)r * n;
+    float2* vj = V + (int64_t)j * vstride + (int64_t)r * n;
     const float2* vr = V + (int64_t)r * n;
+    const float2 hjj = hjj_s;
+    const float ia = ialpha_s;
     double nrm = 0.0;
     for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
-        float2 acc = wr[p];
-        for (int k = 0; k < kcount; ++k) {
+        float2 sc = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+        for (int k = 0; k < j; ++k) {
             const float2 v = vr[(int64_t)k * vstride + p];
-            const float2 c = cf[k];
-            acc.x -= c.x * v.x - c.y * v.y;
-            acc.y -= c.x * v.y + c.y * v.x;
+            cfma(sc, c_s[k], v);
+            cfma(sd, d_s[k], v);
         }
-        wr[p] = acc;
-        if (NORM) nrm += (double)acc.x * acc.x + (double)acc.y * acc.y;
+        float2 jv = csub(vj[p], sc);
+        jv = make_float2(jv.x * ia, jv.y * ia);
+        vj[p] = jv;
+        float2 wv = csub(wr[p], sd);
+        wv.x -= hjj.x * jv.x - hjj.y * jv.y;
+        wv.y -= hjj.x * jv.y + hjj.y * jv.x;
+        wr[p] = wv;
+        nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
     }
-    if (NORM) {
-        double2 s = block_sum(make_double2(nrm, 0.0), lds);
-        if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = s;
+    const double2 tot = block_sum(make_double2(nrm, 0.0), lds);
+    if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = tot;
+}
+
+// provisional H[j+1][j] = ||w||, inv = 1/||w||; a happy breakdown (w = 0)
+// finalises column j at once and stops the cycle
+__global__ void k_dcgs_norm(int nch, int m, int j, const double2* __restrict__ partial,
+                            const double* __restrict__ bnorm, double2* __restrict__ H, double* __restrict__ cs,
+                            double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ est,
+                            double* __restrict__ inv, int* __restrict__ cyc, int* __restrict__ used) {
+    __shared__ double lds[8];
+    const int r = blockIdx.x;
+    if (!cyc[r]) return;
+    const double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
+    if (threadIdx.x != 0) return;
+    const double hn = sqrt(s);
+    double2* hc = H + ((int64_t)r * m + j) * (m + 1);
+    hc[j + 1] = make_double2(hn, 0.0);
+    inv[r] = hn > 0 ? 1.0 / hn : 0.0;
+    if (hn < 1e-12 * bnorm[r]) {
+        const double e = givens_column(hc, cs + (int64_t)r * m, sn + (int64_t)r * m, g + (int64_t)r * (m + 1), j);
+        est[r] = e / bnorm[r];
+        used[r] = j + 1;
+        cyc[r] = 0;
     }
+}
+
+// end of a cycle: finalise the last provisional column (no reorthogonalisation
+// of the unused V_{j+1}) for every RHS still in the cycle
+__global__ void k_dcgs_finish(int nrhs, int m, int j, double2* __restrict__ H, double* __restrict__ cs,
+                              double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ est,
+                              const double* __restrict__ bnorm, int* __restrict__ cyc, int* __restrict__ used) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrhs || !cyc[r]) return;
+    double2* hc = H + ((int64_t)r * m + j) * (m + 1);
+    const double e = givens_column(hc, cs + (int64_t)r * m, sn + (int64_t)r * m, g + (int64_t)r * (m + 1), j);
+    est[r] = e / bnorm[r];
+    used[r] = j + 1;
+    cyc[r] = 0;
 }
 
 // partial[r*nch+chunk] = |v[r]|^2 over the chunk (c128 or c64 vector)
@@ -184,12 +335,6 @@ __global__ __launch_bounds__(256) void k_norm_partial(int64_t n, int nch, const 
     if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = s;
 }
 
-__device__ double reduce_chunks(const double2* src, int nch, double* lds) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int c = threadIdx.x; c < nch; c += blockDim.x) acc = cadd(acc, src[c]);
-    acc = block_sum(acc, lds);
-    return acc.x;
-}
 
 // ||b_r|| (clamped like wave_solver.py:10)
 __global__ void k_bnorm(int nch, const double2* __restrict__ partial, double* __restrict__ bnorm,
@@ -250,79 +395,6 @@ __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __r
     float s = (float)inv[r];
     float2 a = src[(int64_t)r * n + p];
     dst[(int64_t)r * n + p] = make_float2(a.x * s, a.y * s);
-}
-
-// Kahan-Parlett "twice is enough": after one classical Gram-Schmidt pass
-// w' = w - V c, a second pass is needed only if ||w'|| < 0.7 ||w|| (heavy
-// cancellation).  Sets gate[r] for the second pass and stores ||w'||^2.
-__global__ void k_reorth_check(int nch, const double2* __restrict__ wn_before, const double2* __restrict__ wn_after,
-                               const int* __restrict__ cyc, int* __restrict__ gate, double* __restrict__ hn2) {
-    __shared__ double lds[8];
-    const int r = blockIdx.x;
-    if (!cyc[r]) return;
-    const double before = reduce_chunks(wn_before + (int64_t)r * nch, nch, lds);
-    const double after = reduce_chunks(wn_after + (int64_t)r * nch, nch, lds);
-    if (threadIdx.x == 0) {
-        gate[r] = after < 0.49 * before ? 1 : 0;
-        hn2[r] = after;
-    }
-}
-
-// Finish column j: h_{j+1,j} = ||w||, apply old rotations, new rotation,
-// residual estimate, early stop of the cycle.
-__global__ void k_givens(int nch, int m, int j, double tol, const double2* __restrict__ partial,
-                         const int* __restrict__ gate, const double* __restrict__ hn2,
-                         const double* __restrict__ bnorm, double2* __restrict__ H, double* __restrict__ cs,
-                         double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ est,
-                         double* __restrict__ inv, int* __restrict__ cyc, int* __restrict__ used) {
-    __shared__ double lds[8];
-    const int r = blockIdx.x;
-    if (!cyc[r]) return;
-    double s = gate[r] ? reduce_chunks(partial + (int64_t)r * nch, nch, lds) : hn2[r];
-    if (threadIdx.x != 0) return;
-    const double hn = sqrt(s);
-    double2* hc = H + ((int64_t)r * m + j) * (m + 1);
-    double* c = cs + (int64_t)r * m;
-    double2* sv = sn + (int64_t)r * m;
-    double2* gr = g + (int64_t)r * (m + 1);
-    hc[j + 1] = make_double2(hn, 0.0);
-    for (int k = 0; k < j; ++k) {  // previous rotations
-        double2 a = hc[k], b = hc[k + 1];
-        double2 na = cadd(cscale(a, c[k]), cmul(sv[k], b));
-        double2 nb = csub(cscale(b, c[k]), cmul(cconj(sv[k]), a));
-        hc[k] = na;
-        hc[k + 1] = nb;
-    }
-    double2 a = hc[j];
-    double b = hn;
-    double aa = sqrt(a.x * a.x + a.y * a.y);
-    double rr = sqrt(aa * aa + b * b);
-    double ck;
-    double2 sk, rho;
-    if (aa == 0.0) {
-        ck = 0.0;
-        sk = make_double2(1.0, 0.0);
-        rho = make_double2(b, 0.0);
-    } else {
-        double2 ph = make_double2(a.x / aa, a.y / aa);
-        ck = aa / rr;
-        sk = cscale(ph, b / rr);
-        rho = cscale(ph, rr);
-    }
-    c[j] = ck;
-    sv[j] = sk;
-    hc[j] = rho;
-    hc[j + 1] = make_double2(0.0, 0.0);
-    double2 gj = gr[j];
-    gr[j] = cscale(gj, ck);
-    gr[j + 1] = cscale(cmul(cconj(sk), gj), -1.0);
-    double e = sqrt(gr[j + 1].x * gr[j + 1].x + gr[j + 1].y * gr[j + 1].y);
-    est[r] = e / bnorm[r];
-    used[r] = j + 1;
-    inv[r] = hn > 0 ? 1.0 / hn : 0.0;
-    // stop this right-hand side's cycle: converged estimate, happy breakdown
-    // (wave_solver.py:29-30), or the basis is full
-    if (e <= 0.5 * tol * bnorm[r] || hn < 1e-12 * bnorm[r] || !isfinite(e)) cyc[r] = 0;
 }
 
 // y = R^{-1} g (upper triangular, used[r] columns), stored in coef[r][k]
@@ -402,13 +474,13 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.Z, (size_t)restart * nrhs * n));
     HELM_TRY(alloc(&w.w, (size_t)nrhs * n));
     HELM_TRY(alloc(&w.r64, (size_t)nrhs * n));
-    w.partial_len = (int64_t)nrhs * (kmax + 3) * nch;  // dots | |w|^2 | |w'|^2 | |w''|^2
+    w.partial_len = (int64_t)nrhs * (2 * kmax + 1) * nch;  // <V_k,V_j> | <V_k,w> | |w|^2
     HELM_TRY(alloc(&w.partial, (size_t)w.partial_len));
     HELM_TRY(alloc(&w.H, (size_t)nrhs * restart * (restart + 1)));
     HELM_TRY(alloc(&w.cs, (size_t)nrhs * restart));
     HELM_TRY(alloc(&w.sn, (size_t)nrhs * restart));
     HELM_TRY(alloc(&w.g, (size_t)nrhs * (restart + 1)));
-    HELM_TRY(alloc(&w.coef, (size_t)nrhs * kmax));
+    HELM_TRY(alloc(&w.coef, (size_t)nrhs * (2 * kmax + 1)));
     HELM_TRY(alloc(&w.beta, nrhs));
     HELM_TRY(alloc(&w.bnorm, nrhs));
     HELM_TRY(alloc(&w.rel, nrhs));
@@ -443,9 +515,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const dim3 blk(256);
     const dim3 gvec((unsigned)((n + 255) / 256), nrhs);
     const int64_t vstride = (int64_t)nrhs * n;
-    double2* pw0 = W.partial + (int64_t)nrhs * kmax * nch;  // |w|^2 before projection
-    double2* pw1 = pw0 + (int64_t)nrhs * nch;               // after the first pass
-    double2* pw2 = pw1 + (int64_t)nrhs * nch;               // after the second pass
+    double2* pw2 = W.partial + (int64_t)2 * nrhs * kmax * nch;  // |w|^2 after the update
 
     // ||b|| and initial x
     hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, b, W.partial);
@@ -485,38 +555,36 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         const int msteps = std::min(m, o->max_iterations - steps);
         for (int j = 0; j < msteps; ++j) {
             float2* vj = W.V + (int64_t)j * vstride;
-            float2* zj = (pc == HELM_PC_NONE) ? vj : W.Z + (int64_t)j * vstride;
-            if (pc != HELM_PC_NONE) HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, s));
+            // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
+            float2* zj = W.Z + (int64_t)j * vstride;
+            if (pc != HELM_PC_NONE)
+                HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, s));
+            else
+                HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, s));
             HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
             applications++;
-            const int kc = j + 1;
-            const dim3 gd(nch, (kc + KB - 1) / KB, nrhs);
-            // pass 1: c = V^H w (and |w|^2), w -= V c (and |w'|^2)
-            hipLaunchKernelGGL(k_dots, gd, blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.cyc, (const int*)nullptr,
-                               W.partial, pw0);
-            hipLaunchKernelGGL(k_reduce_dots, dim3(kc, nrhs), dim3(64), 0, s, nrhs, nch, kmax, kc, mm, j, 0,
-                               W.partial, W.cyc, (const int*)nullptr, W.coef, W.H);
-            hipLaunchKernelGGL(k_update<true>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.coef,
-                               W.cyc, (const int*)nullptr, pw1);
-            hipLaunchKernelGGL(k_reorth_check, dim3(nrhs), dim3(64), 0, s, nch, pw0, pw1, W.cyc, W.gate, W.hn2);
-            // pass 2 only where the first pass cancelled heavily (gate[r])
-            hipLaunchKernelGGL(k_dots, gd, blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.cyc, W.gate, W.partial,
-                               (double2*)nullptr);
-            hipLaunchKernelGGL(k_reduce_dots, dim3(kc, nrhs), dim3(64), 0, s, nrhs, nch, kmax, kc, mm, j, 1,
-                               W.partial, W.cyc, W.gate, W.coef, W.H);
-            hipLaunchKernelGGL(k_update<true>, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, kc, W.V, W.w, W.coef,
-                               W.cyc, W.gate, pw2);
-            hipLaunchKernelGGL(k_givens, dim3(nrhs), dim3(64), 0, s, nch, mm, j, o->tolerance, pw2, W.gate, W.hn2,
-                               W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used);
-            if (j + 1 < m)
+            const dim3 gd(nch, (j + 1 + KB - 1) / KB, nrhs);
+            hipLaunchKernelGGL(k_dcgs_dots, gd, blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.cyc, W.partial,
+                               W.partial + (int64_t)nrhs * kmax * nch);
+            hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(64), 0, s, nch, mm, kmax, j, o->tolerance, W.partial,
+                               W.partial + (int64_t)nrhs * kmax * nch, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
+                               W.used, W.coef);
+            hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
+                               W.cyc, pw2);
+            hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, s, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
+                               W.g, W.est, W.inv, W.cyc, W.used);
+            if (j + 1 < msteps)
                 hipLaunchKernelGGL(k_scale32, gvec, blk, 0, s, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
                                    W.cyc);
+            else
+                hipLaunchKernelGGL(k_dcgs_finish, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, mm, j, W.H, W.cs,
+                                   W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
             HELM_HIP(hipGetLastError());
             steps++;
         }
         hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, mm, kmax, W.H, W.g, W.used,
                            W.coef);
-        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, s, n, nrhs, kmax, pc == HELM_PC_NONE ? W.V : W.Z, W.coef,
+        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, s, n, nrhs, kmax, W.Z, W.coef,
                            W.used, x);
         HELM_HIP(hipGetLastError());
     }

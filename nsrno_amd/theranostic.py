@@ -118,6 +118,46 @@ class Solver:
             raise ValueError("frequency_mhz must be finite and positive")
         return float(frequency_mhz)
 
+    # ------------------------------------------------------------ coefficients
+    def _stretch(self, coordinates, lower, upper, omega):
+        """s = 1 + i (sigma_max/omega) clip(frac, 0, 1)^2 (wave.py:127-135)."""
+        coords = np.asarray(coordinates, dtype=np.float64)
+        if self.grid.pml_mm == 0:
+            return np.ones_like(coords, dtype=np.complex128)
+        width = self.grid.pml_mm
+        frac = np.maximum((lower + width - coords) / width, (coords - upper + width) / width)
+        sigma_max = self.pml_strength * self.c0_m_s / (width * 1e-3)
+        return 1.0 + 1j * (sigma_max / omega) * np.clip(frac, 0.0, 1.0) ** 2
+
+    def _edges(self):
+        """Edge topology (wave.py:107-125): horizontal, vertical, then the four
+        exterior ghost groups; edge_j = -1 marks a ghost end."""
+        if not hasattr(self, "_edge_cache"):
+            nodes = np.arange(self.grid.size).reshape(self.grid.shape)
+            bnd = np.concatenate((nodes[:, 0], nodes[:, -1], nodes[0], nodes[-1]))
+            ei = np.concatenate((nodes[:, :-1].ravel(), nodes[:-1, :].ravel(), bnd))
+            ej = np.concatenate((nodes[:, 1:].ravel(), nodes[1:, :].ravel(), -np.ones(bnd.size, dtype=int)))
+            valid = ej >= 0
+            rows = np.concatenate((np.arange(ei.size), np.flatnonzero(valid)))
+            cols = np.concatenate((ei, ej[valid]))
+            avg = sparse.csr_matrix((np.full(rows.size, 0.5), (rows, cols)), shape=(ei.size, self.grid.size))
+            self._edge_cache = (ei, ej, avg)
+        return self._edge_cache
+
+    def coefficients(self, frequency_mhz):
+        """(k0, stretched cell mass, stretched edge flux weights), wave.py:137-156."""
+        f = self._frequency(frequency_mhz)
+        omega = 2 * np.pi * f * 1e6
+        x, z, h = self.grid.x_mm, self.grid.z_mm, self.grid.h_mm
+        st = lambda c, a: self._stretch(c, a[0], a[-1], omega)
+        sx, sz = st(x, x), st(z, z)
+        sxh, szh = st((x[:-1] + x[1:]) / 2, x), st((z[:-1] + z[1:]) / 2, z)
+        sxo = st(np.array([x[0] - h / 2, x[-1] + h / 2]), x)
+        szo = st(np.array([z[0] - h / 2, z[-1] + h / 2]), z)
+        edge = np.concatenate(((sz[:, None] / sxh[None, :]).ravel(), (sx[None, :] / szh[:, None]).ravel(),
+                               sz / sxo[0], sz / sxo[1], sx / szo[0], sx / szo[1]))
+        return omega / self.c0_m_s, (sz[:, None] * sx[None, :]).ravel(), edge
+
     def plan(self, frequency_mhz, m_kappa=None, m_eta=None, *, nrhs=1, restart=30):
         """A HelmholtzPlan holding this grid, frequency and contrast on the GPU."""
         f = self._frequency(frequency_mhz)
@@ -251,3 +291,126 @@ class Solver:
                            lu_nonzeros=None,
                            krylov_iterations=[i.iterations for i in infos], reasons=[i.reason for i in infos])
         return fields, diagnostics
+
+    # ------------------------------------------------------------ Born
+    def born_data(self, frequency_mhz, source_weights, m_kappa, m_eta, background_fields=None, *,
+                  return_diagnostics=False, source_chunk=8, tolerance=1e-10, restart=30, max_iterations=200000):
+        """Born data delta_u = -A0^{-1} delta_A u0 at the ports, [tx, rx] (wave.py:280-329).
+
+        delta_A u0 is formed matrix-free on the GPU as A(m) u0 - A0 u0 (FP64
+        stencil applies); both solves use the GPU Krylov solver."""
+        if np.iscomplexobj(source_weights):
+            raise ValueError("Reciprocal aperture weights must be real; apply transmit encoding to solved port fields")
+        weights = sparse.csc_matrix(source_weights, dtype=np.float64)
+        if weights.shape[0] != self.grid.size or weights.shape[1] == 0:
+            raise ValueError("source_weights has incompatible shape")
+        if not isinstance(source_chunk, (int, np.integer)) or source_chunk < 1:
+            raise ValueError("source_chunk must be a positive integer")
+        if not np.isfinite(weights.data).all():
+            raise ValueError("source_weights must be finite")
+        started = perf_counter()
+        n_src = weights.shape[1]
+        chunk = min(int(source_chunk), n_src)
+        p0 = self.plan(frequency_mhz, nrhs=chunk, restart=restart)
+        p1 = self.plan(frequency_mhz, m_kappa, m_eta, nrhs=chunk, restart=restart)
+        use_mg = min(self.grid.shape) >= 64
+        if use_mg:
+            p0.setup_multigrid()
+        pc = "mg" if use_mg else "none"
+        ports = Ports(weights, device=self.device)
+        if background_fields is not None:
+            background_fields = np.asarray(background_fields, dtype=np.complex128)
+            if background_fields.shape != weights.shape or not np.isfinite(background_fields).all():
+                raise ValueError("background_fields has incompatible shape or nonfinite values")
+        torch.cuda.synchronize()
+        setup = perf_counter() - started
+        result = np.empty((n_src, n_src), dtype=np.complex128)
+        err2 = rhs2 = 0.0
+        shape = (-1,) + self.grid.shape
+        for first in range(0, n_src, chunk):
+            end = min(first + chunk, n_src)
+            if background_fields is None:
+                amp = np.zeros((end - first, n_src), dtype=np.complex128)
+                amp[np.arange(end - first), np.arange(first, end)] = 1.0
+                u0, _ = p0.solve(p0.inject(ports, amp), tolerance=tolerance, restart=restart,
+                                 max_iterations=max_iterations, precond=pc)
+            else:
+                u0 = torch.from_numpy(np.ascontiguousarray(background_fields[:, first:end].T)).to(p0.device)
+                u0 = u0.reshape(shape).contiguous()
+            rhs = (p0.apply(u0) - p1.apply(u0)).contiguous()
+            sc, info = p0.solve(rhs, tolerance=tolerance, restart=restart, max_iterations=max_iterations,
+                                precond=pc)
+            result[first:end] = p0.sample(ports, sc).cpu().numpy()
+            b2 = torch.linalg.vector_norm(rhs.reshape(end - first, -1), dim=1).square().cpu().numpy()
+            for i, bb in zip(info, b2):
+                err2 += (i.relative_residual ** 2) * bb
+                rhs2 += bb
+        if not return_diagnostics:
+            return result
+        return result, dict(grid_shape=list(self.grid.shape), frequency_mhz=float(frequency_mhz), sources=n_src,
+                            relative_linear_residual=float(np.sqrt(err2 / max(rhs2, np.finfo(float).tiny))),
+                            factor_seconds=float(setup), elapsed_seconds=float(perf_counter() - started))
+
+    def born_jacobian(self, frequency_mhz, background_fields, source_indices, receiver_indices, basis, *,
+                      pair_chunk=16):
+        """Exact derivative of this discretisation at zero contrast (wave.py:331-389).
+
+        Rows [tx, rx] flattened, columns [m_kappa basis | m_eta basis].  The
+        reciprocal field products over the basis support and the contraction
+        with the (stretch- and mass-weighted) basis run on the GPU as complex
+        GEMMs; no conjugate enters the products (inversion of real parameters
+        uses Re(J^H r))."""
+        fields = np.asarray(background_fields, dtype=np.complex128)
+        if fields.ndim != 2 or fields.shape[0] != self.grid.size:
+            raise ValueError("background_fields has incompatible shape")
+        indices = []
+        for value in (source_indices, receiver_indices):
+            index = np.asarray(value)
+            if index.ndim != 1 or not np.issubdtype(index.dtype, np.integer) or index.size == 0:
+                raise ValueError("Source/receiver indices must be nonempty integer vectors")
+            if np.any(index < 0) or np.any(index >= fields.shape[1]):
+                raise ValueError("Source/receiver index out of bounds")
+            indices.append(index.astype(int))
+        transmitters, receivers = indices
+        if not isinstance(pair_chunk, (int, np.integer)) or pair_chunk < 1:
+            raise ValueError("pair_chunk must be a positive integer")
+        if np.iscomplexobj(basis):
+            raise ValueError("Material basis must be real")
+        basis = sparse.csr_matrix(basis, dtype=np.float64)
+        basis.eliminate_zeros()
+        if basis.shape[0] != self.grid.size or basis.shape[1] == 0 or not np.isfinite(basis.data).all():
+            raise ValueError("Material basis has incompatible shape or nonfinite entries")
+        active_nodes = np.flatnonzero(np.diff(basis.indptr))
+        if np.any(self.grid.pml_mask.ravel()[active_nodes]):
+            raise ValueError("Material basis must vanish in the PML")
+        ei, ej, avg = self._edges()
+        edge_basis = (avg @ basis).tocsr()
+        edge_basis.eliminate_zeros()
+        active_edges = np.flatnonzero(np.diff(edge_basis.indptr))
+        k0, mass, stretch = self.coefficients(frequency_mhz)
+        h2 = self.grid.h_m ** 2
+        dev = torch.device(self.device)
+        mass_b = torch.from_numpy((sparse.diags(k0 ** 2 * mass[active_nodes]) @ basis[active_nodes]).toarray()
+                                  .astype(np.complex128)).to(dev)
+        flux_b = torch.from_numpy((sparse.diags(-stretch[active_edges]) @ edge_basis[active_edges]).toarray()
+                                  .astype(np.complex128)).to(dev)
+        f_t = torch.from_numpy(fields).to(dev)
+        u_act = f_t[torch.from_numpy(active_nodes).to(dev)]
+        i_e = torch.from_numpy(ei[active_edges]).to(dev)
+        j_e = ej[active_edges]
+        valid = j_e >= 0
+        grads = -f_t[i_e]
+        vi = torch.from_numpy(np.flatnonzero(valid)).to(dev)
+        grads[vi] += f_t[torch.from_numpy(j_e[valid]).to(dev)]
+        grads /= self.grid.h_m
+        tx = np.repeat(transmitters, receivers.size)
+        rx = np.tile(receivers, transmitters.size)
+        nb = basis.shape[1]
+        out = torch.empty((tx.size, 2 * nb), dtype=torch.complex128, device=dev)
+        for first in range(0, tx.size, pair_chunk):
+            end = min(first + pair_chunk, tx.size)
+            t_i = torch.from_numpy(tx[first:end]).to(dev)
+            r_i = torch.from_numpy(rx[first:end]).to(dev)
+            out[first:end, :nb] = (mass_b.T @ (u_act[:, r_i] * u_act[:, t_i])).T * h2
+            out[first:end, nb:] = (flux_b.T @ (grads[:, r_i] * grads[:, t_i])).T * h2
+        return out.cpu().numpy()

@@ -396,3 +396,43 @@ def bicgstab(operator, precondition, rhs, x, tolerance=1e-3, max_iterations=800)
         r = s - omega * t
         rho_old = rho
     return x, max_iterations, "iteration_budget"
+
+
+# --------------------------------------------------------------------------
+# Born data and Jacobian (wave.py:280-389)
+# --------------------------------------------------------------------------
+def born_data(grid, frequency_mhz, weights, m_kappa, m_eta, background_fields=None, c0=1540.0, pml_strength=12.0):
+    """delta_u = -A0^{-1} (A(m) - A0) u0 sampled at the ports -> [tx, rx] (wave.py:280-329)."""
+    a0 = assemble(grid, frequency_mhz, c0=c0, pml_strength=pml_strength)
+    da = assemble(grid, frequency_mhz, m_kappa, m_eta, c0=c0, pml_strength=pml_strength) - a0
+    lu = splu(a0)
+    w = weights.tocsc()
+    if background_fields is None:
+        u0 = lu.solve(-w.toarray().astype(np.complex128) / grid.h_m ** 2)
+    else:
+        u0 = np.asarray(background_fields, dtype=np.complex128)
+    scattered = lu.solve(-(da @ u0))
+    return sample(scattered, w)
+
+
+def born_jacobian(grid, frequency_mhz, background_fields, source_indices, receiver_indices, basis, c0=1540.0,
+                  pml_strength=12.0):
+    """Exact derivative of the port data at zero contrast (wave.py:331-389):
+    rows [tx, rx] flattened, columns [m_kappa basis | m_eta basis]."""
+    fields = np.asarray(background_fields, dtype=np.complex128)
+    tx = np.repeat(np.asarray(source_indices), len(receiver_indices))
+    rx = np.tile(np.asarray(receiver_indices), len(source_indices))
+    basis = sparse.csr_matrix(basis, dtype=np.float64)
+    ei, ej, avg = edges(grid)
+    k0, mass, stretch_e = coefficients(grid, frequency_mhz, c0, pml_strength)
+    edge_basis = avg @ basis
+    grads = -fields[ei].copy()
+    valid = ej >= 0
+    grads[valid] += fields[ej[valid]]
+    grads /= grid.h_m
+    h2 = grid.h_m ** 2
+    mass_b = sparse.diags(k0 ** 2 * mass) @ basis
+    flux_b = sparse.diags(-stretch_e) @ edge_basis
+    jk = (mass_b.T @ (fields[:, rx] * fields[:, tx])).T * h2
+    je = (flux_b.T @ (grads[:, rx] * grads[:, tx])).T * h2
+    return np.hstack((jk, je))

@@ -154,3 +154,28 @@ def test_theranostic_dropin_matches_golden():
     a = O.assemble(O.Grid(axis, axis, 6.0), float(GOLDEN["frequency_mhz"]), GOLDEN["m_kappa"], GOLDEN["m_eta"])
     v = np.random.default_rng(0).normal(size=a.shape[0]) + 0j
     assert rel(op @ v, a @ v) < 1e-13
+
+
+def test_born_data_and_jacobian_match_oracle():
+    from scipy import sparse as sp
+    from nsrno_amd import theranostic as T
+    axis = GOLDEN["axis"]
+    grid = O.Grid(axis, axis, 6.0)
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    f = float(GOLDEN["frequency_mhz"])
+    w = solver.source_weights(GOLDEN["positions"], GOLDEN["normals"], GOLDEN["widths"])
+    xx, zz = np.meshgrid(axis, axis)
+
+    def bump(x0, z0, a, b):
+        return np.maximum(1 - ((xx - x0) / a) ** 2 - ((zz - z0) / b) ** 2, 0) ** 2
+
+    basis = sp.csr_matrix(np.column_stack([bump(-2.0, 1.0, 2.2, 1.7).ravel(), bump(2.0, 4.0, 1.6, 2.1).ravel()]))
+    u0 = solver.solve(f, w)
+    jac = solver.born_jacobian(f, u0, np.array([3, 0]), np.array([2, 1, 0]), basis, pair_chunk=2)
+    ref = O.born_jacobian(grid, f, u0, np.array([3, 0]), np.array([2, 1, 0]), basis)
+    assert rel(jac, ref) < 1e-12
+    mk, me = basis @ np.array([0.05, -0.03]), basis @ np.array([-0.02, 0.04])
+    bd = solver.born_data(f, w, mk, me, source_chunk=3)
+    assert rel(bd, O.born_data(grid, f, w, mk, me)) < 1e-8
+    bd2 = solver.born_data(f, w, mk, me, background_fields=u0)
+    assert rel(bd2, bd) < 1e-8

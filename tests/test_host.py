@@ -79,3 +79,12 @@ def test_ports_layout_reproduces_weights():
     tr = [np.sum(p.w_port.numpy()[pp[k]:pp[k + 1]] * u[p.node_of_tap.numpy()[pp[k]:pp[k + 1]]])
           for k in range(w.shape[1])]
     np.testing.assert_allclose(tr, w.T @ u, rtol=1e-14)
+
+
+def test_coefficients_match_reference_golden():
+    axis = GOLDEN["axis"]
+    solver = T.Solver(T.Grid(axis, axis, 6.0))
+    k0, mass, edge = solver.coefficients(float(GOLDEN["frequency_mhz"]))
+    assert abs(k0 - GOLDEN["k0"]) / GOLDEN["k0"] < 1e-15
+    np.testing.assert_allclose(mass, GOLDEN["cell_mass"], rtol=1e-15)
+    np.testing.assert_allclose(edge, GOLDEN["edge_stretch"], rtol=1e-14)

@@ -1,0 +1,60 @@
+"""Write golden vectors of the reference's spectral (S) path (run here only).
+
+Imports nsrno.wave_operator / nsrno.wave_solver from /root/reference (never
+shipped) with a small default-initialised model (no checkpoint exists: the
+zero-initialised last conv layer makes the learned gain 1 and the spectral
+modulation 0), and stores setup coefficients, the incident
+field, the operator applied to a seeded field, the source factor and a full
+solve_frequency prediction.  These functions have no closed form that pins
+them (discrete-dispersion Newton, tissue mask, sponge, source spectrum).
+
+Usage:  python tests/golden/make_golden_s.py  ->  tests/golden/spectral_small.npz
+"""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import torch
+
+sys.path.insert(0, "/root/reference")
+from nsrno.wave_operator import WaveResidualOperator  # noqa: E402
+from nsrno.wave_solver import solve_frequency  # noqa: E402
+
+
+def main():
+    torch.manual_seed(0)
+    h, w = 40, 200
+    dx = 2.0e-4
+    spec = dict(medium_shape=[h, w], dx_m=dx, x0_m=-0.0191 - 2 * dx, z0_m=-3 * dx,
+                angles_deg=[-8.0, 0.0, 8.0], source_tref_s=[1e-6, 0.0, 2e-6])
+    model = WaveResidualOperator(spec, iterations=4).eval()
+    rng = np.random.default_rng(7)
+    zz, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    blob = np.exp(-((zz - 20) ** 2 + (xx - 100) ** 2) / 40.0)
+    c = 1540.0 + 25.0 * blob
+    rho = 1000.0 - 30.0 * blob
+    medium = torch.tensor(np.stack((c, rho))[None], dtype=torch.float32)
+    f = 2.5e6
+    with torch.no_grad():
+        (ax, az, kc, k0), gain, mult = model.setup(medium, f)
+        inc = model.incident(f, 1)
+        u = torch.tensor(rng.normal(size=(model.H, model.W)) + 1j * rng.normal(size=(model.H, model.W)),
+                         dtype=torch.complex64)
+        lu = model.apply_operator(u, (ax, az, kc, k0))
+        sf = model.source_factor(f, 2)
+    pred, info = solve_frequency(model, medium, f, 1, tolerance=1e-6, max_iterations=4096, restart=32)
+    out = dict(h=h, w=w, dx=dx, x0=spec["x0_m"], z0=spec["z0_m"], angles=np.array(spec["angles_deg"]),
+               trefs=np.array(spec["source_tref_s"]), c=c, rho=rho, f=f, H=model.H, W=model.W, pz=model.pz,
+               px=model.px, face=model.face, rx_cols=model.rx_cols.numpy(),
+               ax=ax.numpy(), az=az.numpy(), kc=kc.numpy(), k0=float(k0), gain=np.asarray(gain.numpy()),
+               inc=inc.numpy(), u=u.numpy(), lu=lu.numpy(), sf=complex(sf), pred=pred.numpy(),
+               pred_residual=info["relative_residual"], pred_reason=info["reason"])
+    path = Path(__file__).with_name("spectral_small.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, path.stat().st_size, "bytes", info["reason"], info["relative_residual"],
+          info["solver_iterations"])
+
+
+if __name__ == "__main__":
+    main()

@@ -11,7 +11,22 @@
  *
  * eta_rel = 1 + m_eta = rho0/rho,  kappa_c = (1 + m_kappa) (1 + i alpha c/omega)^2.
  * This is the operator of nsrno/theranostic/wave.py:127-188 (Solver.matrix),
- * extended by the complex kappa_c for attenuation.
+ * extended by the complex kappa_c for attenuation (op_kind HELM_OP_FLUX5).
+ *
+ * op_kind HELM_OP_STAG8 holds the spectral operator of nsrno/wave_operator.py
+ * (:105-109) on a periodic box with an absorbing sponge instead of a PML:
+ *
+ *   L u = [D-x(ax D+x u) + D-z(az D+z u)] / k0^2 + kappa_S u,
+ *   (D+ u)_j = sum_m a_m (u_{j+m+1} - u_{j-m}) / h,
+ *   (D- g)_j = sum_m a_m (g_{j+m} - g_{j-m-1}) / h,
+ *   a = (1225/1024, -245/3072, 49/5120, -5/7168),  indices mod the box size,
+ *
+ * whose Fourier symbol is the reference's (wave_operator.py:22-28).  That
+ * operator uses the opposite sign convention (outgoing exp(-i k r), damping
+ * as -i terms), so the plan stores it in this library's convention,
+ * A = k0^2 conj(L) (real ax, az; kappa_c = conj(kappa_S)); the adjoint
+ * A^H = k0^2 L is the reference's operator, and the reference's solve
+ * L u = f is helm_solve(adjoint = 1) with right-hand side k0^2 f.
  *
  * Conventions: arrays are [rhs][z][x] row-major, complex numbers are
  * interleaved (re, im); c64 = 2 x float32, c128 = 2 x float64.  All data
@@ -54,6 +69,9 @@ extern "C" {
 #define HELM_PC_MG 1       /* complex-shifted-Laplacian multigrid V-cycle   */
 #define HELM_PC_LEARNED 2  /* learned conv-net correction + MG V-cycle      */
 
+#define HELM_OP_FLUX5 0    /* 5-point flux form, stretched PML (wave.py)   */
+#define HELM_OP_STAG8 1    /* 8th-order staggered, periodic + sponge       */
+
 typedef struct helm_plan helm_plan;
 
 /* Grid, frequency and PML: the arguments of Grid/Solver/coefficients,
@@ -68,6 +86,7 @@ typedef struct {
     double frequency_mhz;
     double c0_m_s;             /* k0 = omega/c0 and sigma_max reference      */
     double pml_strength;       /* sigma_max = pml_strength*c0/pml_width      */
+    int32_t op_kind;           /* HELM_OP_FLUX5 (0) or HELM_OP_STAG8         */
 } helm_desc;
 
 typedef struct {
@@ -192,6 +211,59 @@ int32_t helm_plan_set_net(helm_plan* plan, helm_net* net);
 int32_t helm_net_correction(helm_plan* plan, const void* v, void* z, int32_t nrhs, void* stream);
 /* the net's input features for residuals v (bf16 [nrhs][H][W][32]) */
 int32_t helm_net_features(helm_plan* plan, const void* v, void* features, int32_t nrhs, void* stream);
+
+/* ---- spectral (S) operator: replaces WaveResidualOperator.setup /
+ * incident / apply_operator / _iteration (nsrno/wave_operator.py:46-135) and
+ * the warm start of solve_frequency (nsrno/wave_solver.py:50-55).  Plans of
+ * op_kind HELM_OP_STAG8 only (pml_mm = 0; nz, nx = the padded box H, W;
+ * h_mm = 1e3 dx; c0_m_s = the reference speed 1540). */
+typedef struct {
+    int32_t h, w;              /* medium size inside the box                 */
+    int32_t pz, px;            /* offset of the medium in the box            */
+    double x0_m, z0_m;         /* coordinates of medium node [0,0] [m]; box
+                                  node [i,j] sits at ((j-px) h, (i-pz) h)
+                                  + (x0_m, z0_m)                             */
+    double rho0;               /* reference density (1000)                   */
+    double alpha_scale;        /* clamp(exp(alpha_log), .01, 3): alpha [Np/m]
+                                  = alpha_scale*100*ln(10)/20*f_MHz^1.1     */
+    int32_t relaxation;        /* 0: power law; 1: discrete relaxation
+                                  modulus (wave_operator.py:59-65)          */
+    int32_t sponge_cells;      /* sponge width (48)                          */
+    double sponge_amplitude;   /* sponge peak (2)                            */
+    double eta_per_unit_alpha[2];  /* relaxation only                        */
+    double taus_s[2];              /* relaxation only                        */
+    double relax_dt_s;             /* relaxation time step (2.5e-9)          */
+} helm_spectral_medium;
+
+/* c, rho: device float32 [h][w] (the float medium of wave_operator.py:48);
+ * replicate-padded into the box, a = rho0/rho, b = a (c0/c)^2, harmonic face
+ * means ax, az (periodic), tissue mask, kappa_S with attenuation and sponge
+ * (wave_operator.py:46-70), stored as A = k0^2 conj(L).  k0 = omega/c0, or
+ * the relaxation form of wave_operator.py:55.  Rebuilds the multigrid
+ * preconditioner (a 5-point shifted operator of the same coefficients, zero
+ * Dirichlet ghosts) when one is set up. */
+int32_t helm_set_medium_spectral(helm_plan* plan, const float* c, const float* rho,
+                                 const helm_spectral_medium* medium, void* stream);
+/* Plane-wave incident field (wave_operator.py:111-127), c128 [H][W]: a Hann
+ * aperture of half width aperture_m times exp(-i k0 x sin(theta)) on row
+ * z = 0, propagated to every row with the discrete dispersion k_z of the
+ * staggered stencil (6 Newton steps, evanescent branch imaginary).
+ * Needs helm_set_medium_spectral first (geometry and k0). */
+int32_t helm_spectral_incident(helm_plan* plan, double sin_theta, double aperture_m, void* field,
+                               void* stream);
+/* Scattered-field right-hand side of the reference's solve, scaled for
+ * helm_solve(adjoint = 1): rhs = -k0^2 (L - L0) u_inc, where L0 has
+ * ax = az = 1 and kappa = 1 - i sponge (wave_operator.py:142-143,
+ * wave_solver.py:53).  inc, rhs: c128 [nrhs][H][W]. */
+int32_t helm_spectral_rhs(helm_plan* plan, const void* inc, void* rhs, int32_t nrhs, void* stream);
+
+/* ---- warm start: `steps` minimal-residual steps per right-hand side,
+ * d = M^{-1} r, alpha = <A d, r>/|A d|^2, x += alpha d (the residual never
+ * grows), as WaveResidualOperator._iteration (wave_operator.py:129-135) with
+ * this library's preconditioner in place of the spectral one.  b, x: c128
+ * [nrhs][N]; x holds the initial guess.  adjoint as in helm_solve. */
+int32_t helm_warm_start(helm_plan* plan, int32_t precond, const void* b, void* x, int32_t nrhs,
+                        int32_t steps, int32_t adjoint, void* stream);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,18 @@ PC_NONE, PC_MG, PC_LEARNED = 0, 1, 2
 class HelmDesc(ctypes.Structure):
     _fields_ = [("nz", c_int32), ("nx", c_int32), ("nrhs", c_int32), ("restart", c_int32),
                 ("x0_mm", c_double), ("z0_mm", c_double), ("h_mm", c_double), ("pml_mm", c_double),
-                ("frequency_mhz", c_double), ("c0_m_s", c_double), ("pml_strength", c_double)]
+                ("frequency_mhz", c_double), ("c0_m_s", c_double), ("pml_strength", c_double),
+                ("op_kind", c_int32)]
+
+
+HELM_OP_FLUX5, HELM_OP_STAG8 = 0, 1
+
+
+class HelmSpectralMedium(ctypes.Structure):
+    _fields_ = [("h", c_int32), ("w", c_int32), ("pz", c_int32), ("px", c_int32),
+                ("x0_m", c_double), ("z0_m", c_double), ("rho0", c_double), ("alpha_scale", c_double),
+                ("relaxation", c_int32), ("sponge_cells", c_int32), ("sponge_amplitude", c_double),
+                ("eta_per_unit_alpha", c_double * 2), ("taus_s", c_double * 2), ("relax_dt_s", c_double)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), c_int32)
@@ -67,6 +78,10 @@ SIGNATURES = {
     "helm_plan_set_net": (c_int32, [c_void_p, c_void_p]),
     "helm_net_correction": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_net_features": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_set_medium_spectral": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HelmSpectralMedium), c_void_p]),
+    "helm_spectral_incident": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
+    "helm_spectral_rhs": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_warm_start": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),
 }
 
 _LIB = None

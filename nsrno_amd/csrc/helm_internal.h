@@ -81,6 +81,14 @@ struct Level {
     int mg_cap = 0;  // right-hand sides the work vectors hold
     float2* wgt = nullptr;  // per-node complex Jacobi weight
     float2* tcf = nullptr;  // [n][6] precomputed shifted stencil for the LDS tail
+    // HELM_OP_STAG8 (level 0 of a spectral plan only): periodic 8-tap operator
+    // with face coefficients ax[i][j] (face j+1/2), az[i][j] (face i+1/2); the
+    // eta/kap arrays above then hold the 5-point approximation the multigrid uses
+    int op = HELM_OP_FLUX5;
+    float *ax32 = nullptr, *az32 = nullptr;
+    double *ax64 = nullptr, *az64 = nullptr;
+    int sp_cells = 0;      // sponge width
+    double sp_amp = 0.0;   // sponge peak
 };
 
 struct Workspace {
@@ -143,16 +151,28 @@ struct helm_plan {
     void* mg_views = nullptr;      // device copy of the per-level views used by k_tail
     bool mg_views_valid = false;
     helm::Workspace ws;
+    helm_spectral_medium spectral{};  // geometry of the last helm_set_medium_spectral
+    double2* sp_buf = nullptr;        // incident-field scratch: 4 * nx c128
 };
 
 namespace helm {
 // level construction (helm_plan.hip)
 int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm, double h_mm, int nz, int nx);
-// stencil (helm_stencil.hip)
+// stencil (helm_stencil.hip): the level's operator (5-point or, for STAG8, 8-tap)
 int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                        const float2* b, float shift_im, hipStream_t s);
 int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
                        const double2* b, hipStream_t s);
+// the 5-point flux operator of the level's eta/kap (the multigrid's operator)
+int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                      const float2* b, float shift_im, hipStream_t s);
+int32_t launch_flux64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                      const double2* b, hipStream_t s);
+// 8-tap periodic operator (helm_spectral.hip); mode 0: A u, 1: b - A u, 2: -(A - A0) u
+int32_t launch_stag8_32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                        const float2* b, hipStream_t s);
+int32_t launch_stag8_64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                        const double2* b, int mode, hipStream_t s);
 // multigrid (helm_mg.hip)
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
 int32_t mg_build(helm_plan* p, hipStream_t s);

@@ -498,6 +498,98 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     return HELM_OK;
 }
 
+// ---- warm start (minimal-residual steps, wave_operator.py:129-135) --------
+// inv[r] = 1 / ||r64_r|| (0 for a zero residual), cyc[r] = 1
+__global__ void k_ws_scale(int nch, const double2* __restrict__ partial, double* __restrict__ inv,
+                           int* __restrict__ cyc) {
+    __shared__ double lds[8];
+    const int r = blockIdx.x;
+    double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
+    if (threadIdx.x == 0) {
+        inv[r] = s > 0 ? 1.0 / sqrt(s) : 0.0;
+        cyc[r] = 1;
+    }
+}
+
+// part[(2r)nch + c] = sum conj(w) res, part[(2r+1)nch + c] = sum |w|^2 over chunk c
+__global__ __launch_bounds__(256) void k_ws_partial(int64_t n, int nch, const float2* __restrict__ w,
+                                                    const double2* __restrict__ res, double2* __restrict__ part) {
+    __shared__ double lds[8];
+    const int chunk = blockIdx.x, r = blockIdx.y;
+    const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
+    double2 d = make_double2(0.0, 0.0);
+    double q = 0.0;
+    for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
+        const float2 a = w[(int64_t)r * n + p];
+        const double2 b = res[(int64_t)r * n + p];
+        d.x += (double)a.x * b.x + (double)a.y * b.y;
+        d.y += (double)a.x * b.y - (double)a.y * b.x;
+        q += (double)a.x * a.x + (double)a.y * a.y;
+    }
+    const double2 sd = block_sum(d, lds);
+    const double2 sq = block_sum(make_double2(q, 0.0), lds);
+    if (threadIdx.x == 0) {
+        part[(int64_t)(2 * r) * nch + chunk] = sd;
+        part[(int64_t)(2 * r + 1) * nch + chunk] = sq;
+    }
+}
+
+// alpha_r = <A d, r> / |A d|^2 (0 when A d vanishes)
+__global__ __launch_bounds__(256) void k_ws_alpha(int nch, const double2* __restrict__ part, double2* __restrict__ alpha) {
+    __shared__ double lds[8];
+    const int r = blockIdx.x;
+    double2 d = make_double2(0.0, 0.0), q = make_double2(0.0, 0.0);
+    for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+        d = cadd(d, part[(int64_t)(2 * r) * nch + c]);
+        q = cadd(q, part[(int64_t)(2 * r + 1) * nch + c]);
+    }
+    d = block_sum(d, lds);
+    q = block_sum(q, lds);
+    if (threadIdx.x == 0) alpha[r] = q.x > 0 ? cscale(d, 1.0 / q.x) : make_double2(0.0, 0.0);
+}
+
+// x[r] += alpha_r z[r]
+__global__ void k_ws_update(int64_t n, const float2* __restrict__ z, const double2* __restrict__ alpha,
+                            double2* __restrict__ x) {
+    const int r = blockIdx.y;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double2 a = alpha[r];
+    const float2 v = z[(int64_t)r * n + p];
+    double2& o = x[(int64_t)r * n + p];
+    o.x += a.x * v.x - a.y * v.y;
+    o.y += a.x * v.y + a.y * v.x;
+}
+
+int32_t warm_start(helm_plan* p, int pc, const double2* b, double2* x, int nrhs, int steps, bool adj,
+                   hipStream_t s) {
+    HELM_TRY(ensure_workspace(p, nrhs, std::max(1, p->ws.restart)));
+    Workspace& W = p->ws;
+    const Level& L = p->levels[0];
+    const int64_t n = L.n;
+    const int nch = nchunks(n);
+    const double k0sq = p->k0 * p->k0;
+    const dim3 blk(256), gvec((unsigned)((n + 255) / 256), nrhs);
+    float2* r32 = W.V;
+    float2* z = W.Z;
+    for (int it = 0; it < steps; ++it) {
+        HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
+        hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
+        hipLaunchKernelGGL(k_ws_scale, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.inv, W.cyc);
+        hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, s, n, W.r64, r32, W.inv, W.cyc);
+        if (pc != HELM_PC_NONE)
+            HELM_TRY(precond_apply(p, pc, r32, z, nrhs, adj, s));
+        else
+            HELM_HIP(hipMemcpyAsync(z, r32, (size_t)nrhs * n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+        HELM_TRY(launch_apply32(L, k0sq, z, W.w, nrhs, adj, nullptr, 0.0f, s));
+        hipLaunchKernelGGL(k_ws_partial, dim3(nch, nrhs), blk, 0, s, n, nch, W.w, W.r64, W.partial);
+        hipLaunchKernelGGL(k_ws_alpha, dim3(nrhs), blk, 0, s, nch, W.partial, W.coef);
+        hipLaunchKernelGGL(k_ws_update, gvec, blk, 0, s, n, z, W.coef, x);
+        HELM_HIP(hipGetLastError());
+    }
+    return HELM_OK;
+}
+
 // ---- host driver -----------------------------------------------------------
 int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const helm_solve_opts* o,
                     helm_solve_info* info, hipStream_t s) {
@@ -623,4 +715,19 @@ extern "C" int32_t helm_solve(helm_plan* p, const void* rhs, void* x, int32_t nr
     if (o->method == HELM_BICGSTAB) return solve_bicgstab(p, (const double2*)rhs, (double2*)x, nrhs, o, info, s);
     set_error("helm_solve: unknown method");
     return HELM_ERR_ARG;
+}
+
+extern "C" int32_t helm_warm_start(helm_plan* p, int32_t precond, const void* b, void* x, int32_t nrhs,
+                                   int32_t steps, int32_t adjoint, void* stream) {
+    HELM_CHECK(p && b && x, HELM_ERR_ARG, "helm_warm_start: null argument");
+    HELM_CHECK(b != x, HELM_ERR_ARG, "helm_warm_start: b and x must not alias");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_warm_start: set the medium first");
+    HELM_CHECK(nrhs >= 1 && nrhs <= 65535 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_warm_start: nrhs out of range");
+    HELM_CHECK(steps >= 0 && steps <= 1000, HELM_ERR_ARG, "helm_warm_start: steps must be in [0, 1000]");
+    HELM_CHECK(precond >= HELM_PC_NONE && precond <= HELM_PC_LEARNED, HELM_ERR_ARG,
+               "helm_warm_start: unknown preconditioner");
+    HELM_CHECK(precond == HELM_PC_NONE || p->mg.levels > 0, HELM_ERR_STATE,
+               "helm_warm_start: call helm_mg_setup before using the multigrid preconditioner");
+    return warm_start(p, precond, (const double2*)b, (double2*)x, nrhs, steps, adjoint != 0, (hipStream_t)stream);
 }

@@ -510,7 +510,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     if (coarsest) return HELM_OK;
     Level& C = p->levels[l + 1];
     // residual r = f - M u, restricted to the coarse right-hand side
-    HELM_TRY(launch_apply32(L, p->k0 * p->k0, u, L.mg_r, nrhs, adj, f, (float)c.shift, s));
+    HELM_TRY(launch_flux32(L, p->k0 * p->k0, u, L.mg_r, nrhs, adj, f, (float)c.shift, s));
     hipLaunchKernelGGL(k_restrict, dim3((C.nx + 63) / 64, C.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs,
                        L.mg_r, C.mg_f);
     HELM_HIP(hipGetLastError());

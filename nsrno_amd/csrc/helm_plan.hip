@@ -82,7 +82,8 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
 
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf};
+                    L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf,
+                    L.ax32, L.az32, L.ax64, L.az64};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();
@@ -165,6 +166,9 @@ int32_t helm_plan_create(const helm_desc* desc, helm_plan** out) {
                "PML leaves no physical domain");
     HELM_CHECK((int64_t)d.nz * d.nx * d.nrhs < (int64_t)1 << 31, HELM_ERR_ARG,
                "nz*nx*nrhs must stay below 2^31");
+    HELM_CHECK(d.op_kind == HELM_OP_FLUX5 || d.op_kind == HELM_OP_STAG8, HELM_ERR_ARG, "unknown op_kind");
+    HELM_CHECK(d.op_kind != HELM_OP_STAG8 || (d.pml_mm == 0.0 && d.nz >= 16 && d.nx >= 16), HELM_ERR_ARG,
+               "op_kind HELM_OP_STAG8 needs pml_mm = 0 (the sponge absorbs) and a box of at least 16 x 16");
     helm_plan* p = new helm_plan();
     p->desc = d;
     (void)hipGetDevice(&p->device);
@@ -187,12 +191,15 @@ int32_t helm_plan_destroy(helm_plan* p) {
     free_workspace(p->ws);
     if (p->mg_views) (void)hipFree(p->mg_views);
     if (p->net_scale) (void)hipFree(p->net_scale);
+    if (p->sp_buf) (void)hipFree(p->sp_buf);
     delete p;
     return HELM_OK;
 }
 
 int32_t helm_set_medium(helm_plan* p, const double* eta_rel, const double* kappa_c, void* stream) {
     HELM_CHECK(p && eta_rel && kappa_c, HELM_ERR_ARG, "helm_set_medium: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_FLUX5, HELM_ERR_STATE,
+               "helm_set_medium: spectral plans take helm_set_medium_spectral");
     hipStream_t s = (hipStream_t)stream;
     Level& L = p->levels[0];
     int64_t n = L.n;
@@ -207,6 +214,8 @@ int32_t helm_set_medium(helm_plan* p, const double* eta_rel, const double* kappa
 int32_t helm_set_medium_physical(helm_plan* p, const double* c, const double* rho, const double* alpha,
                                  double rho0, void* stream) {
     HELM_CHECK(p && c && rho, HELM_ERR_ARG, "helm_set_medium_physical: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_FLUX5, HELM_ERR_STATE,
+               "helm_set_medium_physical: spectral plans take helm_set_medium_spectral");
     HELM_CHECK(std::isfinite(rho0) && rho0 > 0, HELM_ERR_ARG, "rho0 must be finite and positive");
     hipStream_t s = (hipStream_t)stream;
     Level& L = p->levels[0];

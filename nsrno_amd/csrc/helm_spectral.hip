@@ -1,0 +1,533 @@
+// Spectral (S) operator: 8th-order staggered periodic stencil, its medium
+// set-up, the plane-wave incident field and the scattered-field right-hand
+// side.
+//
+// Operator (nsrno/wave_operator.py:105-109, applied there with six FFTs per
+// call; here as the equivalent real-space stencil, whose Fourier symbol is
+// the reference's D = 2i sum_m a_m sin((m+1/2) q) e^{iq/2} / h):
+//   gx_j = ax_j (D+ u)_j,  (D+ u)_j = sum_m a_m (u_{j+m+1} - u_{j-m}) / h
+//   (D- g)_j = sum_m a_m (g_{j+m} - g_{j-m-1}) / h       (D- = -(D+)^T)
+//   L u = [D-x gx + D-z gz] / k0^2 + kappa_S u,  all indices periodic.
+// The plan stores A = k0^2 conj(L): real face coefficients, kappa_c =
+// conj(kappa_S); the adjoint apply (conj of the mass term) is k0^2 L.
+//
+// Kernel layout: a workgroup owns a TZ x 64 output tile and loops over its
+// group of right-hand sides.  Per RHS it stages u on the tile plus a 7-node
+// periodic halo (x halo on the tile rows, z halo on the tile columns; the
+// corners are never needed) in LDS, forms the x- and z-face fluxes of the
+// tile (64+7 and TZ+7 faces) in LDS, then the divergence.  Face and node
+// coefficients are loaded once per workgroup into registers and reused for
+// every right-hand side.  Algorithmic bytes per node per RHS:
+// 8 (read u) + 8 (write) + (4 + 4 + 8)/R (ax, az, kappa).
+#include <algorithm>
+#include <cmath>
+
+#include "helm_internal.h"
+
+namespace helm {
+
+constexpr double FD0 = 1225.0 / 1024.0, FD1 = -245.0 / 3072.0, FD2 = 49.0 / 5120.0, FD3 = -5.0 / 7168.0;
+constexpr int S8_TX = 64;
+constexpr int S8_HALO = 7;
+
+__host__ __device__ __forceinline__ int wrap_index(int v, int m) {
+    if (v < 0 || v >= m) {
+        v %= m;
+        if (v < 0) v += m;
+    }
+    return v;
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ R sponge_at(int i, int j, int H, int W, int cells, R amp) {
+    const int iz = min(i, H - 1 - i), ix = min(j, W - 1 - j);
+    const int d = cells - min(iz, ix);
+    if (d <= 0 || cells <= 0) return R(0);
+    const R t = R(d) / R(cells);
+    return amp * t * t;
+}
+
+template <typename C, typename R>
+__device__ __forceinline__ C dstag(R c0, R c1, R c2, R c3, C a0, C b0, C a1, C b1, C a2, C b2, C a3, C b3) {
+    // c0 (a0 - b0) + c1 (a1 - b1) + c2 (a2 - b2) + c3 (a3 - b3)
+    C d;
+    d.x = c0 * (a0.x - b0.x) + c1 * (a1.x - b1.x) + c2 * (a2.x - b2.x) + c3 * (a3.x - b3.x);
+    d.y = c0 * (a0.y - b0.y) + c1 * (a1.y - b1.y) + c2 * (a2.y - b2.y) + c3 * (a3.y - b3.y);
+    return d;
+}
+
+// MODE 0: out = A u;  1: out = b - A u;  2: out = -(A - A0) u with A0 the
+// background operator (faces 1, kappa_c 1 + i sponge).  conj: conjugate the
+// mass term (A^H; the face terms are real).
+template <typename R, typename C, int TZ, int MODE>
+__global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs, int rb, const R* __restrict__ ax,
+                                               const R* __restrict__ az, const C* __restrict__ kap, R k0sq, R c0,
+                                               R c1, R c2, R c3, int sp_cells, R sp_amp, int conj,
+                                               const C* __restrict__ u, C* __restrict__ out,
+                                               const C* __restrict__ b) {
+    constexpr int TX = S8_TX, HL = S8_HALO;
+    constexpr int SU = TX + 2 * HL;  // staged row length
+    constexpr int NFX = TX + 7;      // x faces per tile row: -4 .. TX+2
+    constexpr int NFZ = TZ + 7;      // z face rows: -4 .. TZ+2
+    constexpr int RPT = TZ / 4;      // output rows per thread
+    constexpr int FXT = (TZ * NFX + 255) / 256;
+    constexpr int FZT = (NFZ * TX + 255) / 256;
+    __shared__ C su[(TZ + 2 * HL) * SU];
+    __shared__ C sgx[TZ * NFX];
+    __shared__ C sgz[NFZ * TX];
+
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+
+    R cfx[FXT], cfz[FZT];
+#pragma unroll
+    for (int q = 0; q < FXT; ++q) {
+        const int e = tid + q * 256;
+        R v = R(0);
+        if (e < TZ * NFX) {
+            const int r = e / NFX, f = e - r * NFX - 4;
+            v = ax[(int64_t)wrap_index(z0 + r, H) * W + wrap_index(x0 + f, W)];
+            if (MODE == 2) v -= R(1);
+        }
+        cfx[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < FZT; ++q) {
+        const int e = tid + q * 256;
+        R v = R(0);
+        if (e < NFZ * TX) {
+            const int fr = e / TX - 4, c = e % TX;
+            v = az[(int64_t)wrap_index(z0 + fr, H) * W + wrap_index(x0 + c, W)];
+            if (MODE == 2) v -= R(1);
+        }
+        cfz[q] = v;
+    }
+    C kc[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = z0 + ty + 4 * k, j = x0 + tx;
+        C v;
+        v.x = R(0);
+        v.y = R(0);
+        if (i < H && j < W) {
+            v = kap[(int64_t)i * W + j];
+            if (MODE == 2) {
+                v.x -= R(1);
+                v.y -= sponge_at<R>(i, j, H, W, sp_cells, sp_amp);
+            }
+            if (conj) v.y = -v.y;
+            v.x *= k0sq;
+            v.y *= k0sq;
+        }
+        kc[k] = v;
+    }
+
+    const int r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
+    for (int r = blockIdx.z * rb; r < r1; ++r) {
+        const C* ur = u + (int64_t)r * n;
+        __syncthreads();  // the previous right-hand side's readers are done
+        for (int rr = ty; rr < TZ + 2 * HL; rr += 4) {
+            const bool zh = rr < HL || rr >= TZ + HL;
+            const int64_t gz = wrap_index(z0 + rr - HL, H);
+            for (int cc = tx; cc < SU; cc += 64) {
+                const bool xh = cc < HL || cc >= TX + HL;
+                if (zh && xh) continue;
+                su[rr * SU + cc] = ur[gz * W + wrap_index(x0 + cc - HL, W)];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < FXT; ++q) {
+            const int e = tid + q * 256;
+            if (e < TZ * NFX) {
+                const int rw = e / NFX, f = e - rw * NFX - 4;
+                const C* row = su + (rw + HL) * SU + HL;
+                C d = dstag<C, R>(c0, c1, c2, c3, row[f + 1], row[f], row[f + 2], row[f - 1], row[f + 3], row[f - 2],
+                                  row[f + 4], row[f - 3]);
+                d.x *= cfx[q];
+                d.y *= cfx[q];
+                sgx[e] = d;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < FZT; ++q) {
+            const int e = tid + q * 256;
+            if (e < NFZ * TX) {
+                const int fr = e / TX - 4, c = e % TX;
+                const C* col = su + (fr + HL) * SU + HL + c;  // col[k * SU] = u at tile row fr + k
+                C d = dstag<C, R>(c0, c1, c2, c3, col[SU], col[0], col[2 * SU], col[-SU], col[3 * SU], col[-2 * SU],
+                                  col[4 * SU], col[-3 * SU]);
+                d.x *= cfz[q];
+                d.y *= cfz[q];
+                sgz[e] = d;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int i = ty + 4 * k, c = tx;
+            const int gi = z0 + i, gj = x0 + c;
+            if (gi < H && gj < W) {
+                const C* gxr = sgx + i * NFX + 4 + c;   // gxr[m] = face c + m
+                const C* gzc = sgz + (i + 4) * TX + c;  // gzc[m * TX] = face row i + m
+                C acc = dstag<C, R>(c0, c1, c2, c3, gxr[0], gxr[-1], gxr[1], gxr[-2], gxr[2], gxr[-3], gxr[3],
+                                    gxr[-4]);
+                const C dz = dstag<C, R>(c0, c1, c2, c3, gzc[0], gzc[-TX], gzc[TX], gzc[-2 * TX], gzc[2 * TX],
+                                         gzc[-3 * TX], gzc[3 * TX], gzc[-4 * TX]);
+                acc.x += dz.x;
+                acc.y += dz.y;
+                cfma(acc, kc[k], su[(i + HL) * SU + HL + c]);
+                const int64_t o = (int64_t)r * n + (int64_t)gi * W + gj;
+                if (MODE == 1) acc = csub(b[o], acc);
+                if (MODE == 2) {
+                    acc.x = -acc.x;
+                    acc.y = -acc.y;
+                }
+                out[o] = acc;
+            }
+        }
+    }
+}
+
+template <int TZ>
+static int stag8_rhs_per_group(const Level& L, int nrhs) {
+    const int64_t tiles = (int64_t)((L.nx + S8_TX - 1) / S8_TX) * ((L.nz + TZ - 1) / TZ);
+    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + tiles - 1) / tiles));
+    return (int)((nrhs + groups - 1) / groups);
+}
+
+int32_t launch_stag8_32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                        const float2* b, hipStream_t s) {
+    constexpr int TZ = 16;
+    HELM_CHECK(L.ax32 && L.az32, HELM_ERR_STATE, "8-tap operator: coefficients not set");
+    const int rb = stag8_rhs_per_group<TZ>(L, nrhs);
+    const dim3 block(64, 4), grid((L.nx + S8_TX - 1) / S8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
+    const float ih = (float)(1.0 / L.h_m);
+    const float c0 = (float)FD0 * ih, c1 = (float)FD1 * ih, c2 = (float)FD2 * ih, c3 = (float)FD3 * ih;
+    if (b)
+        hipLaunchKernelGGL((k_stag8<float, float2, TZ, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.ax32,
+                           L.az32, L.kap32, (float)k0sq, c0, c1, c2, c3, L.sp_cells, (float)L.sp_amp, (int)adjoint,
+                           u, out, b);
+    else
+        hipLaunchKernelGGL((k_stag8<float, float2, TZ, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.ax32,
+                           L.az32, L.kap32, (float)k0sq, c0, c1, c2, c3, L.sp_cells, (float)L.sp_amp, (int)adjoint,
+                           u, out, (const float2*)nullptr);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t launch_stag8_64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                        const double2* b, int mode, hipStream_t s) {
+    constexpr int TZ = 8;
+    HELM_CHECK(L.ax64 && L.az64, HELM_ERR_STATE, "8-tap operator: coefficients not set");
+    const int rb = stag8_rhs_per_group<TZ>(L, nrhs);
+    const dim3 block(64, 4), grid((L.nx + S8_TX - 1) / S8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
+    const double ih = 1.0 / L.h_m;
+    const double c0 = FD0 * ih, c1 = FD1 * ih, c2 = FD2 * ih, c3 = FD3 * ih;
+#define HELM_S8_LAUNCH(M, BP)                                                                                    \
+    hipLaunchKernelGGL((k_stag8<double, double2, TZ, M>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.ax64, \
+                       L.az64, L.kap64, k0sq, c0, c1, c2, c3, L.sp_cells, L.sp_amp, (int)adjoint, u, out, BP)
+    if (mode == 1) {
+        HELM_CHECK(b, HELM_ERR_ARG, "8-tap residual needs b");
+        HELM_S8_LAUNCH(1, b);
+    } else if (mode == 2) {
+        HELM_S8_LAUNCH(2, (const double2*)nullptr);
+    } else {
+        HELM_S8_LAUNCH(0, (const double2*)nullptr);
+    }
+#undef HELM_S8_LAUNCH
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// ---- medium (wave_operator.py:46-70) -------------------------------------
+struct SpGeom {
+    int H, W, h, w, pz, px;
+    double dx, z0, rho0, c0, omega, alpha_np, alpha_scale, sp_amp, relax_dt;
+    int sp_cells, relaxation;
+    double epa0, epa1, tau0, tau1;
+};
+
+__device__ __forceinline__ double sp_a(const float* __restrict__ rho, const SpGeom& g, int bi, int bj) {
+    bi = wrap_index(bi, g.H);
+    bj = wrap_index(bj, g.W);
+    const int oi = min(max(bi - g.pz, 0), g.h - 1), oj = min(max(bj - g.px, 0), g.w - 1);
+    return g.rho0 / (double)rho[(int64_t)oi * g.w + oj];
+}
+
+__device__ __forceinline__ double harmonic(double a, double b) { return 2.0 * a * b / (a + b); }
+
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+__global__ void k_sp_medium(SpGeom g, const float* __restrict__ c, const float* __restrict__ rho, double* ax64,
+                            double* az64, float* ax32, float* az32, double2* kap64, float2* kap32, double* eta64,
+                            float* eta32) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= g.W || i >= g.H) return;
+    const int oi = min(max(i - g.pz, 0), g.h - 1), oj = min(max(j - g.px, 0), g.w - 1);
+    const double a = sp_a(rho, g, i, j);
+    const double cp = (double)c[(int64_t)oi * g.w + oj];
+    const double bb = a * (g.c0 / cp) * (g.c0 / cp);
+    const double axe = harmonic(a, sp_a(rho, g, i, j + 1)), axw = harmonic(a, sp_a(rho, g, i, j - 1));
+    const double azs = harmonic(a, sp_a(rho, g, i + 1, j)), azn = harmonic(a, sp_a(rho, g, i - 1, j));
+    const double z = (double)(i - g.pz) * g.dx + g.z0;
+    double tissue = 0.0;
+    if (z >= -1e-6 * g.dx) {  // z >= 0 up to rounding (the row at z = 0 is tissue)
+        const double lc = log(cp / g.c0) / 0.03, la = log(a) / 0.05;
+        tissue = 1.0 - exp(-lc * lc - la * la);
+    }
+    const double sp = sponge_at<double>(i, j, g.H, g.W, g.sp_cells, g.sp_amp);
+    double2 ks;
+    if (g.relaxation) {
+        const double e0 = g.alpha_scale * tissue * cp / g.c0 * g.epa0;
+        const double e1 = g.alpha_scale * tissue * cp / g.c0 * g.epa1;
+        const double s1 = 1.0 + 0.5 * (e0 + e1);
+        const double amp = s1 * s1;
+        double sn, cs;
+        sincos(g.omega * g.relax_dt, &sn, &cs);
+        const double2 step = make_double2(cs, sn);
+        const double2 sm1 = make_double2(cs - 1.0, sn);
+        // h = sum_k e_k / (amp (1 + tau_k (step - 1)/dt))
+        double2 hs = make_double2(0.0, 0.0);
+        const double ek[2] = {e0, e1}, tk[2] = {g.tau0, g.tau1};
+        for (int q = 0; q < 2; ++q) {
+            const double2 den = make_double2(amp * (1.0 + tk[q] * sm1.x / g.relax_dt), amp * tk[q] * sm1.y / g.relax_dt);
+            hs = cadd(hs, cdiv(make_double2(ek[q], 0.0), den));
+        }
+        // modulus = amp (1 - h) / (1 - (1 - 1/step) h)
+        const double2 inv_step = cdiv(make_double2(1.0, 0.0), step);
+        const double2 one_m = make_double2(1.0 - inv_step.x, -inv_step.y);
+        const double2 num = make_double2(amp * (1.0 - hs.x), -amp * hs.y);
+        const double2 den = csub(make_double2(1.0, 0.0), cmul(one_m, hs));
+        const double2 modulus = cdiv(num, den);
+        ks = cdiv(make_double2(bb, 0.0), modulus);
+        ks.y -= sp;
+    } else {
+        const double t = g.alpha_np * tissue * cp / g.omega;
+        ks = make_double2(bb * (1.0 - t * t), -2.0 * bb * t - sp);  // b (1 - i t)^2 - i sponge
+    }
+    const int64_t o = (int64_t)i * g.W + j;
+    ax64[o] = axe;
+    az64[o] = azs;
+    ax32[o] = (float)axe;
+    az32[o] = (float)azs;
+    const double2 kt = cconj(ks);  // stored in this library's convention
+    kap64[o] = kt;
+    kap32[o] = make_float2((float)kt.x, (float)kt.y);
+    const double e = 0.25 * (axe + axw + azs + azn);
+    eta64[o] = e;
+    eta32[o] = (float)e;
+}
+
+// ---- incident field (wave_operator.py:111-127) -----------------------------
+// sp_buf layout: [0, W) twiddles e^{2 pi i m / W}; [W, 2W) boundary trace;
+// [2W, 3W) k_z; [3W, 4W) boundary spectrum.
+__device__ __forceinline__ double stag_sum(double q) {
+    return FD0 * sin(0.5 * q) + FD1 * sin(1.5 * q) + FD2 * sin(2.5 * q) + FD3 * sin(3.5 * q);
+}
+__device__ __forceinline__ double stag_dsum(double q) {
+    return FD0 * 0.5 * cos(0.5 * q) + FD1 * 1.5 * cos(1.5 * q) + FD2 * 2.5 * cos(2.5 * q) + FD3 * 3.5 * cos(3.5 * q);
+}
+
+__global__ void k_inc_prepare(int W, int px, double dx, double x0, double k0, double sin_theta, double aperture,
+                              double2* buf) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= W) return;
+    double sn, cs;
+    sincospi(2.0 * m / W, &sn, &cs);
+    buf[m] = make_double2(cs, sn);
+    const double x = (double)(m - px) * dx + x0;
+    double ap = 0.0;
+    if (fabs(x) <= aperture) ap = 0.5 * (1.0 + cospi(x / aperture));
+    sincos(-k0 * x * sin_theta, &sn, &cs);
+    buf[W + m] = make_double2(ap * cs, ap * sn);
+    // discrete dispersion: solve 2 S(q) = sqrt(k0^2 - sx) h for the propagating branch
+    const int kk = m < (W + 1) / 2 ? m : m - W;  // fftfreq order
+    const double qx = 2.0 * M_PI * (double)kk / W;
+    const double dxs = 2.0 * stag_sum(qx) / dx;
+    const double sx = dxs * dxs;
+    const double desired = sqrt(fmax(k0 * k0 - sx, 0.0)) * dx;
+    double q = fmin(fmax(desired, 0.0), M_PI);
+    for (int it = 0; it < 6; ++it) {
+        const double value = 2.0 * stag_sum(q), deriv = 2.0 * stag_dsum(q);
+        q = fmin(fmax(q - (value - desired) / fmax(deriv, 0.05), 0.0), M_PI);
+    }
+    buf[2 * W + m] = make_double2(q / dx, -sqrt(fmax(sx - k0 * k0, 0.0)));
+}
+
+// spectrum[k] = sum_j trace[j] e^{-2 pi i j k / W}
+__global__ void k_inc_spectrum(int W, double2* buf) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= W) return;
+    const double2* tw = buf;
+    const double2* tr = buf + W;
+    double2 acc = make_double2(0.0, 0.0);
+    int idx = 0;
+    for (int j = 0; j < W; ++j) {
+        cfma(acc, tr[j], cconj(tw[idx]));
+        idx += k;
+        if (idx >= W) idx -= W;
+    }
+    buf[3 * W + k] = acc;
+}
+
+// field[i][x] = (1/W) sum_k spectrum[k] e^{-i |z_i| kz_k} e^{2 pi i k x / W}; one workgroup per row
+constexpr int INC_CHUNK = 1024;
+constexpr int INC_MAXX = 16;  // x values per thread: W <= 4096
+__global__ __launch_bounds__(256) void k_inc_field(int H, int W, int pz, double dx, double z0,
+                                                   const double2* __restrict__ buf, double2* __restrict__ field) {
+    __shared__ double2 row[INC_CHUNK];
+    const int i = blockIdx.x;
+    const double az = fabs((double)(i - pz) * dx + z0);
+    const double2* tw = buf;
+    const double2* kz = buf + 2 * W;
+    const double2* spec = buf + 3 * W;
+    double2 acc[INC_MAXX];
+#pragma unroll
+    for (int q = 0; q < INC_MAXX; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < W; k0 += INC_CHUNK) {
+        const int kn = min(INC_CHUNK, W - k0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < kn; k += blockDim.x) {
+            const double2 kk = kz[k0 + k];
+            // e^{-i |z| kz} = e^{|z| Im kz} e^{-i |z| Re kz}
+            const double mag = exp(az * kk.y);
+            double sn, cs;
+            sincos(-az * kk.x, &sn, &cs);
+            row[k] = cmul(spec[k0 + k], make_double2(mag * cs, mag * sn));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < INC_MAXX; ++q) {
+            const int x = threadIdx.x + q * 256;
+            if (x < W) {
+                int idx = (int)(((int64_t)k0 * x) % W);
+                double2 a = acc[q];
+                for (int k = 0; k < kn; ++k) {
+                    cfma(a, row[k], tw[idx]);
+                    idx += x;
+                    if (idx >= W) idx -= W;
+                }
+                acc[q] = a;
+            }
+        }
+    }
+    const double s = 1.0 / W;
+#pragma unroll
+    for (int q = 0; q < INC_MAXX; ++q) {
+        const int x = threadIdx.x + q * 256;
+        if (x < W) field[(int64_t)i * W + x] = cscale(acc[q], s);
+    }
+}
+
+template <typename T>
+static int32_t sp_alloc(T** ptr, size_t count) {
+    if (*ptr) return HELM_OK;
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T) + 16);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        set_error(std::string("hipMalloc (spectral operator): ") + hipGetErrorString(e));
+        return HELM_ERR_ALLOC;
+    }
+    return HELM_OK;
+}
+
+}  // namespace helm
+
+using namespace helm;
+
+extern "C" {
+
+int32_t helm_set_medium_spectral(helm_plan* p, const float* c, const float* rho, const helm_spectral_medium* m,
+                                 void* stream) {
+    HELM_CHECK(p && c && rho && m, HELM_ERR_ARG, "helm_set_medium_spectral: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_STAG8, HELM_ERR_STATE,
+               "helm_set_medium_spectral: the plan was not created with op_kind HELM_OP_STAG8");
+    Level& L = p->levels[0];
+    HELM_CHECK(m->h >= 1 && m->w >= 1 && m->pz >= 0 && m->px >= 0 && m->pz + m->h <= L.nz && m->px + m->w <= L.nx,
+               HELM_ERR_ARG, "helm_set_medium_spectral: the medium does not fit the box");
+    HELM_CHECK(std::isfinite(m->rho0) && m->rho0 > 0, HELM_ERR_ARG, "rho0 must be finite and positive");
+    HELM_CHECK(std::isfinite(m->alpha_scale) && m->alpha_scale >= 0, HELM_ERR_ARG,
+               "alpha_scale must be finite and nonnegative");
+    HELM_CHECK(m->sponge_cells >= 0 && std::isfinite(m->sponge_amplitude) && m->sponge_amplitude >= 0, HELM_ERR_ARG,
+               "invalid sponge");
+    HELM_CHECK(!m->relaxation || (std::isfinite(m->relax_dt_s) && m->relax_dt_s > 0), HELM_ERR_ARG,
+               "relaxation needs relax_dt_s > 0");
+    hipStream_t s = (hipStream_t)stream;
+    HELM_TRY(sp_alloc(&L.ax64, L.n));
+    HELM_TRY(sp_alloc(&L.az64, L.n));
+    HELM_TRY(sp_alloc(&L.ax32, L.n));
+    HELM_TRY(sp_alloc(&L.az32, L.n));
+    SpGeom g{};
+    g.H = L.nz;
+    g.W = L.nx;
+    g.h = m->h;
+    g.w = m->w;
+    g.pz = m->pz;
+    g.px = m->px;
+    g.dx = L.h_m;
+    g.z0 = m->z0_m;
+    g.rho0 = m->rho0;
+    g.c0 = p->desc.c0_m_s;
+    g.omega = p->omega;
+    g.alpha_scale = m->alpha_scale;
+    g.alpha_np = m->alpha_scale * 100.0 * std::log(10.0) / 20.0 * std::pow(p->desc.frequency_mhz, 1.1);
+    g.sp_cells = m->sponge_cells;
+    g.sp_amp = m->sponge_amplitude;
+    g.relaxation = m->relaxation;
+    g.relax_dt = m->relax_dt_s;
+    g.epa0 = m->eta_per_unit_alpha[0];
+    g.epa1 = m->eta_per_unit_alpha[1];
+    g.tau0 = m->taus_s[0];
+    g.tau1 = m->taus_s[1];
+    hipLaunchKernelGGL(k_sp_medium, dim3((L.nx + 255) / 256, L.nz), dim3(256), 0, s, g, c, rho, L.ax64, L.az64,
+                       L.ax32, L.az32, L.kap64, L.kap32, L.eta64, L.eta32);
+    HELM_HIP(hipGetLastError());
+    L.op = HELM_OP_STAG8;
+    L.sp_cells = m->sponge_cells;
+    L.sp_amp = m->sponge_amplitude;
+    // k0 of wave_operator.py:55 (the relaxation form matches the 2.5 ns time stepper)
+    p->k0 = m->relaxation ? 2.0 * std::sin(p->omega * m->relax_dt_s / 2.0) / m->relax_dt_s / p->desc.c0_m_s
+                          : p->omega / p->desc.c0_m_s;
+    p->spectral = *m;
+    p->medium_set = true;
+    if (p->mg.levels > 0) HELM_TRY(mg_build(p, s));
+    return HELM_OK;
+}
+
+int32_t helm_spectral_incident(helm_plan* p, double sin_theta, double aperture_m, void* field, void* stream) {
+    HELM_CHECK(p && field, HELM_ERR_ARG, "helm_spectral_incident: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_STAG8 && p->medium_set, HELM_ERR_STATE,
+               "helm_spectral_incident: call helm_set_medium_spectral first");
+    HELM_CHECK(std::isfinite(sin_theta) && std::isfinite(aperture_m) && aperture_m > 0, HELM_ERR_ARG,
+               "helm_spectral_incident: invalid angle or aperture");
+    const Level& L = p->levels[0];
+    const int W = L.nx, H = L.nz;
+    HELM_CHECK(W <= 256 * INC_MAXX, HELM_ERR_ARG, "helm_spectral_incident: box wider than 4096 nodes");
+    hipStream_t s = (hipStream_t)stream;
+    HELM_TRY(sp_alloc(&p->sp_buf, (size_t)4 * W));
+    const helm_spectral_medium& m = p->spectral;
+    hipLaunchKernelGGL(k_inc_prepare, dim3((W + 255) / 256), dim3(256), 0, s, W, m.px, L.h_m, m.x0_m, p->k0,
+                       sin_theta, aperture_m, p->sp_buf);
+    hipLaunchKernelGGL(k_inc_spectrum, dim3((W + 255) / 256), dim3(256), 0, s, W, p->sp_buf);
+    hipLaunchKernelGGL(k_inc_field, dim3(H), dim3(256), 0, s, H, W, m.pz, L.h_m, m.z0_m, p->sp_buf,
+                       (double2*)field);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t helm_spectral_rhs(helm_plan* p, const void* inc, void* rhs, int32_t nrhs, void* stream) {
+    HELM_CHECK(p && inc && rhs, HELM_ERR_ARG, "helm_spectral_rhs: null argument");
+    HELM_CHECK(inc != rhs, HELM_ERR_ARG, "helm_spectral_rhs: inc and rhs must not alias");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_STAG8 && p->medium_set, HELM_ERR_STATE,
+               "helm_spectral_rhs: call helm_set_medium_spectral first");
+    HELM_CHECK(nrhs >= 1 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_spectral_rhs: nrhs out of range");
+    return launch_stag8_64(p->levels[0], p->k0 * p->k0, (const double2*)inc, (double2*)rhs, nrhs, true, nullptr, 2,
+                           (hipStream_t)stream);
+}
+
+}  // extern "C"

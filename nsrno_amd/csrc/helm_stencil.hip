@@ -60,6 +60,21 @@ static int rhs_per_group(const Level& L, int nrhs) {
 
 int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                        const float2* b, float shift, hipStream_t s) {
+    if (L.op == HELM_OP_STAG8) {
+        HELM_CHECK(shift == 0.0f, HELM_ERR_ARG, "the 8-tap operator has no shifted form");
+        return launch_stag8_32(L, k0sq, u, out, nrhs, adjoint, b, s);
+    }
+    return launch_flux32(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+}
+
+int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                       const double2* b, hipStream_t s) {
+    if (L.op == HELM_OP_STAG8) return launch_stag8_64(L, k0sq, u, out, nrhs, adjoint, b, b ? 1 : 0, s);
+    return launch_flux64(L, k0sq, u, out, nrhs, adjoint, b, s);
+}
+
+int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                      const float2* b, float shift, hipStream_t s) {
     const int rb = rhs_per_group(L, nrhs);
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const float hh = (float)(0.5 / (L.h_m * L.h_m));
@@ -74,8 +89,8 @@ int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out
     return HELM_OK;
 }
 
-int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
-                       const double2* b, hipStream_t s) {
+int32_t launch_flux64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
+                      const double2* b, hipStream_t s) {
     const int rb = rhs_per_group(L, nrhs);
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const double hh = 0.5 / (L.h_m * L.h_m);
@@ -246,6 +261,8 @@ __global__ void k_material_sens(int nz, int nx, int64_t n, int nrhs, const doubl
 extern "C" int32_t helm_material_sensitivity(helm_plan* p, const void* u, const void* lam, int32_t nrhs,
                                              void* s_eta, void* s_kap, void* stream) {
     HELM_CHECK(p && u && lam && s_eta && s_kap, HELM_ERR_ARG, "helm_material_sensitivity: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_FLUX5, HELM_ERR_STATE,
+               "helm_material_sensitivity: flux-form (HELM_OP_FLUX5) plans only");
     HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_material_sensitivity: call helm_set_medium first");
     HELM_CHECK(nrhs >= 1 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
                "helm_material_sensitivity: nrhs out of range");

@@ -70,7 +70,7 @@ class HelmholtzPlan:
     """One grid + frequency + PML on the GPU (helm_plan_create / helm_set_medium)."""
 
     def __init__(self, nz, nx, h_mm, frequency_mhz, pml_mm, *, x0_mm=0.0, z0_mm=0.0, c0_m_s=1540.0,
-                 pml_strength=12.0, nrhs=1, restart=30, device="cuda"):
+                 pml_strength=12.0, nrhs=1, restart=30, device="cuda", op_kind=_lib.HELM_OP_FLUX5):
         if not torch.cuda.is_available():
             raise RuntimeError("HelmholtzPlan needs a GPU (no CPU fallback)")
         self.device = torch.device(device)
@@ -82,7 +82,8 @@ class HelmholtzPlan:
         self.k0 = self.omega / self.c0
         self.nrhs_capacity, self.restart_capacity = int(nrhs), int(restart)
         desc = _lib.HelmDesc(self.nz, self.nx, int(nrhs), int(restart), float(x0_mm), float(z0_mm),
-                             float(h_mm), float(pml_mm), float(frequency_mhz), float(c0_m_s), float(pml_strength))
+                             float(h_mm), float(pml_mm), float(frequency_mhz), float(c0_m_s), float(pml_strength),
+                             int(op_kind))
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             check(lib().helm_plan_create(ctypes.byref(desc), ctypes.byref(handle)), "helm_plan_create")
@@ -180,6 +181,18 @@ class HelmholtzPlan:
         return z
 
     # ------------------------------------------------------------ solve
+    def warm_start(self, rhs, x, steps, *, precond="mg", adjoint=False):
+        """`steps` minimal-residual preconditioned steps on x in place (complex128 [R, nz, nx])."""
+        if x.dtype != torch.complex128 or rhs.dtype != torch.complex128 or not x.is_contiguous():
+            raise ValueError("warm_start takes contiguous complex128 rhs and x")
+        nrhs = self._check_field(rhs)
+        if x.shape != rhs.shape:
+            raise ValueError("x and rhs must have the same shape")
+        pc = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED}[precond]
+        check(lib().helm_warm_start(self._h, pc, _ptr(rhs), _ptr(x), nrhs, int(steps), int(bool(adjoint)),
+                                    _stream()), "helm_warm_start")
+        return x
+
     def solve(self, rhs, x0=None, *, tolerance=1e-6, restart=None, max_iterations=3000, precond="none",
               method="gmres", adjoint=False, callback=None):
         """Krylov solve of A x = rhs (A^H x = rhs) for all right-hand sides.

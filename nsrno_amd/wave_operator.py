@@ -1,0 +1,260 @@
+"""Spectral residual operator on the GPU: drop-in for nsrno/wave_operator.py.
+
+``WaveResidualOperator`` keeps the reference's constructor, geometry
+attributes (H, W, pz, px, pad, dx, x, z, sponge, rx_cols, face, angles, trefs)
+and methods (setup, apply_operator, incident, source_factor, forward) with
+the same arguments and meaning; the arithmetic runs in libhelm.so:
+
+* setup            helm_set_medium_spectral  (wave_operator.py:46-70)
+* apply_operator   helm_apply on the 8-tap periodic stencil whose Fourier
+                   symbol is the reference's D (wave_operator.py:105-109)
+* incident         helm_spectral_incident    (wave_operator.py:111-127)
+* the scattered-field right-hand side        helm_spectral_rhs (:142-143)
+* _iteration / the warm start of forward     helm_warm_start   (:129-135)
+
+Differences, by design:
+* the preconditioner of the warm start and of the solve is the library's
+  complex-shifted-Laplacian multigrid on a 5-point discretisation of the same
+  coefficients, not the Fourier-space shifted-Laplacian multiplier
+  (wave_operator.py:93-102); the learned context gain (a small conv net of the
+  trained model, :71-75) is not applied, so with a trained checkpoint the warm
+  start differs while the solution the solve converges to does not;
+* ``setup`` returns an opaque ``SpectralData`` (plan handle and k0) in place of
+  the (ax, az, kappa_complex, k0) tensor tuple; ``coefficients`` gives the
+  tensors when they are wanted.
+
+The trained scalars that change the solution are honoured: ``alpha_log``
+(attenuation) and ``response`` (receiver scale), as loaded by
+``load_wave_operator``.
+"""
+from __future__ import annotations
+
+import math
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .helmholtz import HelmholtzPlan, _ptr, _stream
+
+C0 = 1540.0
+RHO0 = 1000.0
+APERTURE_M = 0.0191
+SPONGE_CELLS = 48
+SPONGE_AMPLITUDE = 2.0
+RELAX_DT = 2.5e-9
+
+
+def _mg_levels(nz, nx, coarsest=40):
+    levels, n = 1, max(nz, nx)
+    while n > coarsest and levels < 7:
+        n = (n + 1) // 2
+        levels += 1
+    return max(levels, 2)
+
+
+class SpectralData:
+    """One medium at one frequency on the GPU (the reference's ``data`` tuple)."""
+
+    def __init__(self, plan, k0):
+        self.plan = plan
+        self.k0 = k0
+
+
+class WaveResidualOperator:
+    """GPU mirror of nsrno/wave_operator.py:14-167 (see the module docstring)."""
+
+    def __init__(self, spec, iterations=8, preconditioner="shifted", *, restart=32, nrhs=16, device="cuda"):
+        if preconditioner not in ("shifted", "shifted_tapered", "free_space", "free_space_tapered"):
+            raise ValueError(preconditioner)
+        self.spec = spec
+        self.iterations = int(iterations)
+        self.preconditioner = preconditioner
+        self.device = torch.device(device)
+        h, w = spec["medium_shape"][-2:]
+        self.h, self.w = int(h), int(w)
+        self.H = math.ceil((self.h + 64) / 32) * 32
+        self.W = math.ceil((self.w + 64) / 32) * 32
+        self.pz = (self.H - self.h) // 2
+        self.px = (self.W - self.w) // 2
+        self.pad = (self.px, self.W - self.w - self.px, self.pz, self.H - self.h - self.pz)
+        self.dx = float(spec["dx_m"])
+        self.x = torch.arange(self.W, dtype=torch.float64) * self.dx + spec["x0_m"] - self.px * self.dx
+        self.z = torch.arange(self.H, dtype=torch.float64) * self.dx + spec["z0_m"] - self.pz * self.dx
+        iz = torch.minimum(torch.arange(self.H), torch.arange(self.H).flip(0))[:, None]
+        ix = torch.minimum(torch.arange(self.W), torch.arange(self.W).flip(0))[None, :]
+        self.sponge = SPONGE_AMPLITUDE * ((SPONGE_CELLS - torch.minimum(iz, ix)).clamp_min(0) / SPONGE_CELLS) ** 2
+        self.rx_cols = torch.round((torch.arange(192, dtype=torch.float64) * 0.0002 - APERTURE_M - spec["x0_m"])
+                                   / self.dx).long() + self.px
+        self.face = int(round(-spec["z0_m"] / self.dx)) + self.pz
+        self.angles = torch.tensor(spec["angles_deg"], dtype=torch.float64) * math.pi / 180
+        self.trefs = torch.tensor(spec["source_tref_s"], dtype=torch.float64)
+        # trained scalars (defaults = the reference's initial values)
+        self.alpha_log = math.log(0.5)
+        self.log_shift = math.log(0.15)
+        self.response = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+        self.training = False
+        self.restart_capacity = int(restart)
+        self.nrhs_capacity = int(nrhs)
+        self._plans = OrderedDict()
+
+    # ------------------------------------------------------------ module-like API
+    def eval(self):
+        self.training = False
+        return self
+
+    def to(self, device):
+        self.device = torch.device(device)
+        self._plans.clear()
+        return self
+
+    def load_state_dict(self, state):
+        """Take the scalars that change the solution from a reference state dict."""
+        def val(key):
+            v = state[key]
+            return v.detach().cpu().double().numpy() if torch.is_tensor(v) else np.asarray(v, dtype=np.float64)
+        if "alpha_log" in state:
+            self.alpha_log = float(val("alpha_log"))
+        if "log_shift" in state:
+            self.log_shift = float(val("log_shift"))
+        if "response" in state:
+            self.response = val("response").reshape(2, 3)
+        return self
+
+    @property
+    def alpha_scale(self):
+        return min(max(math.exp(self.alpha_log), 0.01), 3.0)
+
+    def wavenumber(self, frequency):
+        omega = 2 * math.pi * frequency
+        if self.spec.get("attenuation") == "relaxation":
+            return 2 * math.sin(omega * RELAX_DT / 2) / RELAX_DT / C0
+        return omega / C0
+
+    # ------------------------------------------------------------ plans
+    def _plan(self, frequency):
+        key = float(frequency)
+        if key in self._plans:
+            self._plans.move_to_end(key)
+            return self._plans[key]
+        plan = HelmholtzPlan(self.H, self.W, self.dx * 1e3, key / 1e6, 0.0, c0_m_s=C0, nrhs=self.nrhs_capacity,
+                             restart=self.restart_capacity, device=self.device, op_kind=_lib.HELM_OP_STAG8)
+        plan.medium_key = None
+        self._plans[key] = plan
+        while len(self._plans) > 4:
+            self._plans.popitem(last=False)[1].close()
+        return plan
+
+    def _medium_struct(self):
+        m = _lib.HelmSpectralMedium()
+        m.h, m.w, m.pz, m.px = self.h, self.w, self.pz, self.px
+        m.x0_m, m.z0_m = float(self.spec["x0_m"]), float(self.spec["z0_m"])
+        m.rho0 = RHO0
+        m.alpha_scale = self.alpha_scale
+        m.relaxation = int(self.spec.get("attenuation") == "relaxation")
+        m.sponge_cells, m.sponge_amplitude = SPONGE_CELLS, SPONGE_AMPLITUDE
+        if m.relaxation:
+            epa, taus = self.spec["eta_per_unit_alpha"], self.spec["taus_s"]
+            m.eta_per_unit_alpha[0], m.eta_per_unit_alpha[1] = float(epa[0]), float(epa[1])
+            m.taus_s[0], m.taus_s[1] = float(taus[0]), float(taus[1])
+        m.relax_dt_s = RELAX_DT
+        return m
+
+    def _set_medium(self, plan, c, rho):
+        c = c.to(self.device, torch.float32).contiguous()
+        rho = rho.to(self.device, torch.float32).contiguous()
+        m = self._medium_struct()
+        check(lib().helm_set_medium_spectral(plan._h, _ptr(c), _ptr(rho), m, _stream()), "helm_set_medium_spectral")
+        plan._medium = (c, rho)
+        if plan.mg_levels == 0:
+            plan.setup_multigrid(levels=_mg_levels(self.H, self.W), shift=0.5, pre=2, post=2, coarse_sweeps=10,
+                                 omega=0.0)
+
+    def setup(self, medium, frequency, learned=True):
+        """(data, gain, multiplier) as wave_operator.py:46-103; data is a SpectralData."""
+        if medium.shape[0] != 1:
+            raise ValueError("One native medium per frequency solve")
+        if tuple(medium.shape[-2:]) != (self.h, self.w):
+            raise ValueError(f"medium must be [1, 2, {self.h}, {self.w}]")
+        c, rho = medium.detach().float().unbind(1)
+        plan = self._plan(frequency)
+        self._set_medium(plan, c[0], rho[0])
+        return SpectralData(plan, self.wavenumber(frequency)), 1.0, None
+
+    def coefficients(self, data):
+        """(kappa_S, L applied to ones) read back through the FP64 operator: kappa_S = L 1."""
+        one = torch.ones((self.H, self.W), dtype=torch.complex128, device=self.device)
+        return data.plan.apply(one, adjoint=True) / data.k0 ** 2
+
+    def apply_operator(self, u, data):
+        """L u for complex64 (FP32 stencil) or complex128 (FP64) fields [..., H, W]."""
+        u = u.to(self.device).contiguous()
+        return data.plan.apply(u, adjoint=True) / data.k0 ** 2
+
+    def incident(self, frequency, angle, data=None):
+        """Incident plane wave [H, W] complex64 for spec angle index ``angle``."""
+        return self.incident64(frequency, angle, data).to(torch.complex64)
+
+    def incident64(self, frequency, angle, data=None):
+        plan = data.plan if data is not None else self._plan(frequency)
+        if getattr(plan, "_medium", None) is None:
+            water = torch.full((self.h, self.w), C0, dtype=torch.float32)
+            self._set_medium(plan, water, torch.full((self.h, self.w), RHO0, dtype=torch.float32))
+        out = torch.empty((self.H, self.W), dtype=torch.complex128, device=self.device)
+        theta = float(self.angles[angle])
+        check(lib().helm_spectral_incident(plan._h, math.sin(theta), APERTURE_M, _ptr(out), _stream()),
+              "helm_spectral_incident")
+        return out
+
+    def scattered_rhs(self, incident, data):
+        """-k0^2 (L - L0) u_inc (complex128), the right-hand side helm_solve(adjoint) takes."""
+        inc = incident.to(self.device, torch.complex128).contiguous()
+        nrhs = inc.numel() // (self.H * self.W)
+        rhs = torch.empty_like(inc)
+        check(lib().helm_spectral_rhs(data.plan._h, _ptr(inc), _ptr(rhs), nrhs, _stream()), "helm_spectral_rhs")
+        return rhs
+
+    def source_factor(self, frequency, angle):
+        """Gaussian-sine source coefficient x exp(-i w t_ref) x response (wave_operator.py:156-167)."""
+        f0 = self.spec.get("source_f0_hz", 7.5e6)
+        sigma = 1 / f0
+        duration = 2401 / 40e6
+        omega = 2 * math.pi * frequency
+        src = -0.5j * 1e5 * sigma * math.sqrt(2 * math.pi) / duration * (
+            math.exp(-0.5 * (sigma * (omega - 2 * math.pi * f0)) ** 2)
+            - math.exp(-0.5 * (sigma * (omega + 2 * math.pi * f0)) ** 2))
+        src = src * complex(np.exp(-1j * omega * float(self.trefs[angle])))
+        f = frequency / 6e6 - 1
+        coef = self.response @ np.array([1.0, f, f * f])
+        return torch.tensor(src * complex(coef[0], coef[1]), dtype=torch.complex64)
+
+    def forward(self, medium, frequency, angle, learned=True, return_trace=False):
+        """Warm-start-only forward (wave_operator.py:137-154): prediction, residual ratio, trace."""
+        data, _, _ = self.setup(medium, frequency, learned)
+        rhs = self.scattered_rhs(self.incident64(frequency, angle, data), data)
+        x = torch.zeros_like(rhs)
+        den = rhs.abs().square().mean().clamp_min(1e-20)
+        trace = []
+        for _ in range(self.iterations if return_trace else 0):
+            data.plan.warm_start(rhs, x, 1, precond="mg", adjoint=True)
+            r = rhs - data.plan.apply(x, adjoint=True)
+            trace.append(float((r.abs().square().mean() / den).sqrt()))
+        if not return_trace:
+            data.plan.warm_start(rhs, x, self.iterations, precond="mg", adjoint=True)
+        r = rhs - data.plan.apply(x, adjoint=True)
+        loss = r.abs().square().mean() / den
+        pred = x[self.face, self.rx_cols.to(self.device)] * self.source_factor(frequency, angle).to(self.device)
+        return pred.to(torch.complex64), loss, trace
+
+    __call__ = forward
+
+
+def load_wave_operator(path, device="cuda", iterations=None):
+    """Build the mirror from a reference checkpoint (weights_only load, nsrno/wave_operator.py:182-186)."""
+    saved = torch.load(path, map_location="cpu", weights_only=True)
+    model = WaveResidualOperator(saved["spec"], saved["iterations"] if iterations is None else iterations,
+                                 saved.get("preconditioner", "shifted"), device=device)
+    model.load_state_dict(saved["model"])
+    return model.eval()

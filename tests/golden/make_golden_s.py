@@ -3,10 +3,11 @@
 Imports nsrno.wave_operator / nsrno.wave_solver from /root/reference (never
 shipped) with a small default-initialised model (no checkpoint exists: the
 zero-initialised last conv layer makes the learned gain 1 and the spectral
-modulation 0), and stores setup coefficients, the incident
-field, the operator applied to a seeded field, the source factor and a full
-solve_frequency prediction.  These functions have no closed form that pins
-them (discrete-dispersion Newton, tissue mask, sponge, source spectrum).
+modulation 0), and stores setup coefficients (power-law and relaxation
+attenuation), the incident field, the operator applied to a seeded field,
+the source factor and a full solve_frequency prediction.  These functions
+have no closed form that pins them (discrete-dispersion Newton, tissue mask,
+sponge, relaxation modulus, source spectrum).
 
 Usage:  python tests/golden/make_golden_s.py  ->  tests/golden/spectral_small.npz
 """
@@ -43,13 +44,23 @@ def main():
                          dtype=torch.complex64)
         lu = model.apply_operator(u, (ax, az, kc, k0))
         sf = model.source_factor(f, 2)
+    # discrete relaxation modulus (wave_operator.py:55,59-65) on a tissue medium
+    rspec = dict(spec, attenuation="relaxation", eta_per_unit_alpha=[0.03, 0.015], taus_s=[8e-8, 2e-8])
+    rmodel = WaveResidualOperator(rspec, iterations=4).eval()
+    tissue = torch.tensor(np.stack((c - 60.0 * blob, rho + 40.0 * blob))[None], dtype=torch.float32)
+    with torch.no_grad():
+        (rax, raz, rkc, rk0), _, _ = rmodel.setup(tissue, f)
+        pkc = model.setup(tissue, f)[0][2]
     pred, info = solve_frequency(model, medium, f, 1, tolerance=1e-6, max_iterations=4096, restart=32)
     out = dict(h=h, w=w, dx=dx, x0=spec["x0_m"], z0=spec["z0_m"], angles=np.array(spec["angles_deg"]),
                trefs=np.array(spec["source_tref_s"]), c=c, rho=rho, f=f, H=model.H, W=model.W, pz=model.pz,
                px=model.px, face=model.face, rx_cols=model.rx_cols.numpy(),
                ax=ax.numpy(), az=az.numpy(), kc=kc.numpy(), k0=float(k0), gain=np.asarray(gain.numpy()),
                inc=inc.numpy(), u=u.numpy(), lu=lu.numpy(), sf=complex(sf), pred=pred.numpy(),
-               pred_residual=info["relative_residual"], pred_reason=info["reason"])
+               pred_residual=info["relative_residual"], pred_reason=info["reason"],
+               tissue_c=tissue[0, 0].numpy().astype(np.float64), tissue_rho=tissue[0, 1].numpy().astype(np.float64),
+               relax_epa=np.array(rspec["eta_per_unit_alpha"]), relax_taus=np.array(rspec["taus_s"]),
+               relax_kc=rkc.numpy(), relax_k0=float(rk0), power_kc_tissue=pkc.numpy())
     path = Path(__file__).with_name("spectral_small.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, path.stat().st_size, "bytes", info["reason"], info["relative_residual"],

@@ -265,6 +265,24 @@ int32_t helm_spectral_rhs(helm_plan* plan, const void* inc, void* rhs, int32_t n
 int32_t helm_warm_start(helm_plan* plan, int32_t precond, const void* b, void* x, int32_t nrhs,
                         int32_t steps, int32_t adjoint, void* stream);
 
+/* ---- in-solve kernel timing: no counterpart in the reference (its path
+ * has no device kernels); for benchmarks that need one kernel's duration
+ * inside a running solve.  With timing on, helm_solve brackets every launch
+ * of the operator apply, the preconditioner, the two Arnoldi passes and the
+ * FP64 true-residual apply with HIP events on the solve's stream, and
+ * accumulates per kernel the launches, the summed event time and the
+ * algorithmic bytes (each input and output vector counted once). */
+typedef struct {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+    double bytes;
+} helm_kernel_stat;
+/* enable != 0 turns timing on and clears the totals; 0 turns it off */
+int32_t helm_profile(helm_plan* plan, int32_t enable);
+/* copies up to capacity entries (host array); *count = entries available */
+int32_t helm_profile_read(helm_plan* plan, helm_kernel_stat* stats, int32_t capacity, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,11 @@ class HelmSolveInfo(ctypes.Structure):
                 ("relative_residual", c_double)]
 
 
+class HelmKernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", c_double),
+                ("bytes", c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "helm_version": (ctypes.c_char_p, []),
@@ -81,6 +86,8 @@ SIGNATURES = {
     "helm_set_medium_spectral": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HelmSpectralMedium), c_void_p]),
     "helm_spectral_incident": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "helm_spectral_rhs": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_profile": (c_int32, [c_void_p, c_int32]),
+    "helm_profile_read": (c_int32, [c_void_p, POINTER(HelmKernelStat), c_int32, POINTER(c_int32)]),
     "helm_warm_start": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),
 }
 

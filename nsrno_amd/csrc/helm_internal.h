@@ -128,6 +128,25 @@ struct Workspace {
     std::vector<int> h_iters;
 };
 
+// HIP-event timing of the kernels inside a solve (helm_profile)
+enum ProfKernel { PK_APPLY = 0, PK_PRECOND, PK_DOTS, PK_UPDATE, PK_RESIDUAL, PK_COUNT };
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int id;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> pending;
+    int64_t launches[PK_COUNT] = {};
+    double ms[PK_COUNT] = {};
+    double bytes[PK_COUNT] = {};
+};
+hipEvent_t prof_begin(Profiler* pf, hipStream_t s);
+void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s);
+void prof_flush(Profiler* pf);  // after the stream has been synchronised
+
 struct MGConfig {
     int levels = 0;
     double shift = 0.5;
@@ -151,6 +170,7 @@ struct helm_plan {
     void* mg_views = nullptr;      // device copy of the per-level views used by k_tail
     bool mg_views_valid = false;
     helm::Workspace ws;
+    helm::Profiler* prof = nullptr;   // kernel timing inside solves (helm_profile)
     helm_spectral_medium spectral{};  // geometry of the last helm_set_medium_spectral
     double2* sp_buf = nullptr;        // incident-field scratch: 4 * nx c128
 };

@@ -37,7 +37,6 @@
 namespace helm {
 
 constexpr int CHUNK = 4096;  // nodes per reduction chunk (256 threads x 16)
-constexpr int KB = 8;        // basis vectors per dot-product workgroup
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -86,49 +85,79 @@ __device__ double reduce_chunks(const double2* src, int nch, double* lds) {
 // after which its Givens rotation is applied (one step later than plain
 // Arnoldi).  Two basis reads per step instead of four, numerically a CGS2.
 
-// pc[(r*kmax + k)*nch + chunk] = <V_k, V_j>, pd[...] = <V_k, w>, k <= j
+// pc[(r*kmax + k)*nch + chunk] = <V_k, V_j>, pd[...] = <V_k, w>, k <= j.
+// One workgroup per (chunk of DOT_CHUNK nodes, RHS): its w and V_j are read
+// once into registers (8 nodes per thread), then V_0..V_{j-1} stream through with the
+// next vector prefetched while the current one is reduced, so a launch reads
+// exactly j + 2 vectors.  Per k the four sums are reduced within each wave
+// (shuffles); the four waves' results meet in LDS once, after the k loop.
+constexpr int DOT_CHUNK = 2048;           // nodes per dot-product workgroup
+constexpr int DOT_NPT = DOT_CHUNK / 256;  // nodes per thread
+static inline int ndotchunks(int64_t n) { return (int)((n + DOT_CHUNK - 1) / DOT_CHUNK); }
 __global__ __launch_bounds__(256) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax, int j,
                                                    const float2* __restrict__ V, const float2* __restrict__ w,
                                                    const int* __restrict__ cyc, double2* __restrict__ pc,
                                                    double2* __restrict__ pd) {
-    __shared__ double lds[8];
-    const int chunk = blockIdx.x, r = blockIdx.z, k0 = blockIdx.y * KB;
+    __shared__ double4 red[64][4];  // [k][wave] = (c.re, c.im, d.re, d.im)
+    const int chunk = blockIdx.x, r = blockIdx.y;
     if (!cyc[r]) return;
-    const int kc = min(KB, j + 1 - k0);
-    double2 ac[KB], ad[KB];
-#pragma unroll
-    for (int q = 0; q < KB; ++q) {
-        ac[q] = make_double2(0.0, 0.0);
-        ad[q] = make_double2(0.0, 0.0);
-    }
-    const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t beg = (int64_t)chunk * DOT_CHUNK;
     const int64_t vstride = (int64_t)nrhs * n;
     const float2* wr = w + (int64_t)r * n;
-    const float2* vj = V + (int64_t)j * vstride + (int64_t)r * n;
-    const float2* vr = V + (int64_t)k0 * vstride + (int64_t)r * n;
-    for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
-        const float2 wv = wr[p], jv = vj[p];
+    const float2* vr = V + (int64_t)r * n;
+    const float2* vj = vr + (int64_t)j * vstride;
+    float2 wv[DOT_NPT], jv[DOT_NPT], cur[DOT_NPT], nxt[DOT_NPT];
+    const float2 zero = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int q = 0; q < KB; ++q) {
-            if (q < kc) {
-                const float2 v = vr[(int64_t)q * vstride + p];
-                ac[q].x += (double)v.x * jv.x + (double)v.y * jv.y;
-                ac[q].y += (double)v.x * jv.y - (double)v.y * jv.x;
-                ad[q].x += (double)v.x * wv.x + (double)v.y * wv.y;
-                ad[q].y += (double)v.x * wv.y - (double)v.y * wv.x;
-            }
-        }
+    for (int i = 0; i < DOT_NPT; ++i) {
+        const int64_t p = beg + threadIdx.x + 256 * i;
+        const bool in = p < n;
+        wv[i] = in ? wr[p] : zero;
+        jv[i] = in ? vj[p] : zero;
+        cur[i] = (in && j > 0) ? vr[p] : jv[i];
     }
+    for (int k = 0; k <= j; ++k) {
+        if (k + 1 < j) {
+            const float2* vn = vr + (int64_t)(k + 1) * vstride;
 #pragma unroll
-    for (int q = 0; q < KB; ++q) {
-        if (q < kc) {
-            const double2 a = block_sum(ac[q], lds);
-            const double2 b = block_sum(ad[q], lds);
-            if (threadIdx.x == 0) {
-                pc[((int64_t)r * kmax + k0 + q) * nch + chunk] = a;
-                pd[((int64_t)r * kmax + k0 + q) * nch + chunk] = b;
+            for (int i = 0; i < DOT_NPT; ++i) {
+                const int64_t p = beg + threadIdx.x + 256 * i;
+                nxt[i] = p < n ? vn[p] : zero;
             }
+        } else {
+#pragma unroll
+            for (int i = 0; i < DOT_NPT; ++i) nxt[i] = jv[i];
         }
+        double cr = 0.0, ci = 0.0, dr = 0.0, di = 0.0;
+#pragma unroll
+        for (int i = 0; i < DOT_NPT; ++i) {
+            const double vx = cur[i].x, vy = cur[i].y;
+            cr += vx * jv[i].x + vy * jv[i].y;
+            ci += vx * jv[i].y - vy * jv[i].x;
+            dr += vx * wv[i].x + vy * wv[i].y;
+            di += vx * wv[i].y - vy * wv[i].x;
+        }
+        cr = wave_sum(cr);
+        ci = wave_sum(ci);
+        dr = wave_sum(dr);
+        di = wave_sum(di);
+        if (lane == 0) red[k][wid] = make_double4(cr, ci, dr, di);
+#pragma unroll
+        for (int i = 0; i < DOT_NPT; ++i) cur[i] = nxt[i];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= j; k += blockDim.x) {
+        double4 t = red[k][0];
+        for (int q = 1; q < 4; ++q) {
+            const double4 u = red[k][q];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        pc[((int64_t)r * kmax + k) * nch + chunk] = make_double2(t.x, t.y);
+        pd[((int64_t)r * kmax + k) * nch + chunk] = make_double2(t.z, t.w);
     }
 }
 
@@ -178,21 +207,27 @@ __global__ void k_dcgs_reduce(int nch, int m, int kmax, int j, double tol, const
     __shared__ double2 sc[64], sd[64];
     const int r = blockIdx.x;
     if (!cyc[r]) return;
-    for (int k = 0; k <= j; ++k) {
+    (void)lds;
+    // one wave per k: sum the chunk partials in a fixed order (deterministic)
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int k = threadIdx.x >> 6; k <= j; k += nw) {
         double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
         const double2* srcc = pc + ((int64_t)r * kmax + k) * nch;
         const double2* srcd = pd + ((int64_t)r * kmax + k) * nch;
-        for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+        for (int c = lane; c < nch; c += 64) {
             a = cadd(a, srcc[c]);
             b = cadd(b, srcd[c]);
         }
-        a = block_sum(a, lds);
-        b = block_sum(b, lds);
-        if (threadIdx.x == 0) {
+        a.x = wave_sum(a.x);
+        a.y = wave_sum(a.y);
+        b.x = wave_sum(b.x);
+        b.y = wave_sum(b.y);
+        if (lane == 0) {
             sc[k] = a;
             sd[k] = b;
         }
     }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     double2* Hr = H + (int64_t)r * m * (m + 1);
     double* csr = cs + (int64_t)r * m;
@@ -474,7 +509,8 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.Z, (size_t)restart * nrhs * n));
     HELM_TRY(alloc(&w.w, (size_t)nrhs * n));
     HELM_TRY(alloc(&w.r64, (size_t)nrhs * n));
-    w.partial_len = (int64_t)nrhs * (2 * kmax + 1) * nch;  // <V_k,V_j> | <V_k,w> | |w|^2
+    // <V_k,V_j> | <V_k,w> (per dot chunk) | |w|^2 (per chunk)
+    w.partial_len = (int64_t)nrhs * (2 * kmax * ndotchunks(n) + nch);
     HELM_TRY(alloc(&w.partial, (size_t)w.partial_len));
     HELM_TRY(alloc(&w.H, (size_t)nrhs * restart * (restart + 1)));
     HELM_TRY(alloc(&w.cs, (size_t)nrhs * restart));
@@ -607,7 +643,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const dim3 blk(256);
     const dim3 gvec((unsigned)((n + 255) / 256), nrhs);
     const int64_t vstride = (int64_t)nrhs * n;
-    double2* pw2 = W.partial + (int64_t)2 * nrhs * kmax * nch;  // |w|^2 after the update
+    const int nchd = ndotchunks(n);
+    double2* pd_part = W.partial + (int64_t)nrhs * kmax * nchd;      // <V_k, w>
+    double2* pw2 = W.partial + (int64_t)2 * nrhs * kmax * nchd;      // |w|^2 after the update
 
     // ||b|| and initial x
     hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, b, W.partial);
@@ -622,9 +660,16 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
 
     int steps = 0, cycles = 0, applications = 0;
     bool budget = false;
+    Profiler* pf = (p->prof && p->prof->on) ? p->prof : nullptr;
+    const double nn = (double)n, rr = (double)nrhs;
+    const double coef_bytes32 = L.op == HELM_OP_STAG8 ? 16.0 : 12.0;  // per node, FP32 coefficients
+    const double apply_bytes = 16.0 * nn * rr + coef_bytes32 * nn;
+    const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
     for (;;) {
         // true residual in FP64
+        hipEvent_t ev = pf ? prof_begin(pf, s) : nullptr;
         HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
+        if (pf) prof_end(pf, PK_RESIDUAL, ev, resid_bytes, s);
         applications++;
         hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
         hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, nch, mm, o->tolerance, W.partial, W.bnorm,
@@ -634,9 +679,11 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         HELM_HIP(hipMemcpyAsync(W.h_active.data(), W.active, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipMemcpyAsync(W.h_iters.data(), W.iters, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipStreamSynchronize(s));
+        if (pf) prof_flush(pf);
         if (o->callback) o->callback(o->user, applications, W.h_rel.data(), nrhs);
         int n_active = 0;
         for (int r = 0; r < nrhs; ++r) n_active += W.h_active[r];
+        const double vec_bytes = 8.0 * nn * n_active;  // one c64 vector over the active RHS
         if (n_active == 0) break;
         if (steps >= o->max_iterations) {
             budget = true;
@@ -649,20 +696,28 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             float2* vj = W.V + (int64_t)j * vstride;
             // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
             float2* zj = W.Z + (int64_t)j * vstride;
-            if (pc != HELM_PC_NONE)
+            if (pc != HELM_PC_NONE) {
+                ev = pf ? prof_begin(pf, s) : nullptr;
                 HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, s));
-            else
+                if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, s);
+            } else {
                 HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, s));
+            }
+            ev = pf ? prof_begin(pf, s) : nullptr;
             HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
+            if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, s);
             applications++;
-            const dim3 gd(nch, (j + 1 + KB - 1) / KB, nrhs);
-            hipLaunchKernelGGL(k_dcgs_dots, gd, blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.cyc, W.partial,
-                               W.partial + (int64_t)nrhs * kmax * nch);
-            hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(64), 0, s, nch, mm, kmax, j, o->tolerance, W.partial,
-                               W.partial + (int64_t)nrhs * kmax * nch, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
+            ev = pf ? prof_begin(pf, s) : nullptr;
+            hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd, kmax, j, W.V, W.w, W.cyc,
+                               W.partial, pd_part);
+            if (pf) prof_end(pf, PK_DOTS, ev, (j + 2) * vec_bytes, s);
+            hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, s, nchd, mm, kmax, j, o->tolerance, W.partial,
+                               pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
                                W.used, W.coef);
+            ev = pf ? prof_begin(pf, s) : nullptr;
             hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
                                W.cyc, pw2);
+            if (pf) prof_end(pf, PK_UPDATE, ev, (j + 4) * vec_bytes, s);
             hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, s, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
                                W.g, W.est, W.inv, W.cyc, W.used);
             if (j + 1 < msteps)

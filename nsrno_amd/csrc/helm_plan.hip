@@ -127,11 +127,93 @@ __global__ void k_medium_physical(int64_t n, const double* __restrict__ c, const
     kap32[i] = make_float2((float)k.x, (float)k.y);
 }
 
+// ---- kernel timing --------------------------------------------------------
+hipEvent_t prof_begin(Profiler* pf, hipStream_t s) {
+    hipEvent_t e = nullptr;
+    if (!pf->pool.empty()) {
+        e = pf->pool.back();
+        pf->pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    (void)hipEventRecord(e, s);
+    return e;
+}
+
+void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s) {
+    if (!a) return;
+    hipEvent_t b = nullptr;
+    if (!pf->pool.empty()) {
+        b = pf->pool.back();
+        pf->pool.pop_back();
+    } else if (hipEventCreate(&b) != hipSuccess) {
+        pf->pool.push_back(a);
+        return;
+    }
+    (void)hipEventRecord(b, s);
+    pf->pending.push_back({id, a, b, bytes});
+}
+
+void prof_flush(Profiler* pf) {
+    for (auto& r : pf->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            pf->launches[r.id]++;
+            pf->ms[r.id] += ms;
+            pf->bytes[r.id] += r.bytes;
+        }
+        pf->pool.push_back(r.a);
+        pf->pool.push_back(r.b);
+    }
+    pf->pending.clear();
+}
+
+static void prof_destroy(Profiler* pf) {
+    if (!pf) return;
+    prof_flush(pf);
+    for (hipEvent_t e : pf->pool) (void)hipEventDestroy(e);
+    delete pf;
+}
+
 }  // namespace helm
 
 using namespace helm;
 
 extern "C" {
+
+int32_t helm_profile(helm_plan* p, int32_t enable) {
+    HELM_CHECK(p, HELM_ERR_ARG, "helm_profile: null plan");
+    if (!p->prof) p->prof = new Profiler();
+    Profiler* pf = p->prof;
+    prof_flush(pf);
+    pf->on = enable != 0;
+    if (pf->on) {
+        for (int k = 0; k < PK_COUNT; ++k) {
+            pf->launches[k] = 0;
+            pf->ms[k] = 0.0;
+            pf->bytes[k] = 0.0;
+        }
+    }
+    return HELM_OK;
+}
+
+int32_t helm_profile_read(helm_plan* p, helm_kernel_stat* stats, int32_t capacity, int32_t* count) {
+    HELM_CHECK(p && count && (stats || capacity == 0), HELM_ERR_ARG, "helm_profile_read: null argument");
+    static const char* names[PK_COUNT] = {"operator_apply", "preconditioner", "dcgs_dots", "dcgs_update",
+                                          "true_residual"};
+    *count = PK_COUNT;
+    Profiler* pf = p->prof;
+    for (int k = 0; k < PK_COUNT && k < capacity; ++k) {
+        std::memset(&stats[k], 0, sizeof(helm_kernel_stat));
+        std::strncpy(stats[k].name, names[k], sizeof(stats[k].name) - 1);
+        if (pf) {
+            stats[k].launches = pf->launches[k];
+            stats[k].total_ms = pf->ms[k];
+            stats[k].bytes = pf->bytes[k];
+        }
+    }
+    return HELM_OK;
+}
 
 const char* helm_version(void) { return "helm 0.1 (gfx950)"; }
 
@@ -192,6 +274,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     if (p->mg_views) (void)hipFree(p->mg_views);
     if (p->net_scale) (void)hipFree(p->net_scale);
     if (p->sp_buf) (void)hipFree(p->sp_buf);
+    prof_destroy(p->prof);
     delete p;
     return HELM_OK;
 }

@@ -180,6 +180,24 @@ class HelmholtzPlan:
         check(lib().helm_precond_apply(self._h, int(kind), _ptr(v), _ptr(z), nrhs, _stream()), "helm_precond_apply")
         return z
 
+    # ------------------------------------------------------------ kernel timing
+    def profile(self, enable=True):
+        """Turn HIP-event timing of the kernels inside solves on (clearing totals) or off."""
+        check(lib().helm_profile(self._h, int(bool(enable))), "helm_profile")
+
+    def profile_read(self):
+        """{kernel: dict(launches, total_ms, avg_us, bytes, gb_s)} accumulated since profile(True)."""
+        stats = (_lib.HelmKernelStat * 16)()
+        count = ctypes.c_int32()
+        check(lib().helm_profile_read(self._h, stats, 16, ctypes.byref(count)), "helm_profile_read")
+        out = {}
+        for st in stats[:count.value]:
+            n = int(st.launches)
+            out[st.name.decode()] = dict(launches=n, total_ms=st.total_ms,
+                                         avg_us=1e3 * st.total_ms / n if n else 0.0, bytes=st.bytes,
+                                         gb_s=st.bytes / (st.total_ms * 1e6) if st.total_ms > 0 else 0.0)
+        return out
+
     # ------------------------------------------------------------ solve
     def warm_start(self, rhs, x, steps, *, precond="mg", adjoint=False):
         """`steps` minimal-residual preconditioned steps on x in place (complex128 [R, nz, nx])."""

@@ -278,8 +278,12 @@ typedef struct {
     double total_ms;
     double bytes;
 } helm_kernel_stat;
-/* enable != 0 turns timing on and clears the totals; 0 turns it off */
-int32_t helm_profile(helm_plan* plan, int32_t enable);
+/* mask != 0 turns timing on for the kernels whose bit is set (bit k =
+ * entry k of helm_profile_read: 0 operator apply, 1 preconditioner, 2 DCGS2
+ * dots, 3 DCGS2 update, 4 true residual) and clears the totals; 0 turns it
+ * off.  Each timed launch adds an event pair to the stream, so time only the
+ * kernels needed. */
+int32_t helm_profile(helm_plan* plan, int32_t mask);
 /* copies up to capacity entries (host array); *count = entries available */
 int32_t helm_profile_read(helm_plan* plan, helm_kernel_stat* stats, int32_t capacity, int32_t* count);
 

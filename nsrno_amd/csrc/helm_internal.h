@@ -80,6 +80,7 @@ struct Level {
     float2 *mg_u = nullptr, *mg_f = nullptr, *mg_r = nullptr, *mg_t = nullptr;
     int mg_cap = 0;  // right-hand sides the work vectors hold
     float2* wgt = nullptr;  // per-node complex Jacobi weight
+    float2* wid = nullptr;  // per-node Jacobi factor w / dg of the shifted operator
     float2* tcf = nullptr;  // [n][6] precomputed shifted stencil for the LDS tail
     // HELM_OP_STAG8 (level 0 of a spectral plan only): periodic 8-tap operator
     // with face coefficients ax[i][j] (face j+1/2), az[i][j] (face i+1/2); the
@@ -132,6 +133,7 @@ struct Workspace {
 enum ProfKernel { PK_APPLY = 0, PK_PRECOND, PK_DOTS, PK_UPDATE, PK_RESIDUAL, PK_COUNT };
 struct Profiler {
     bool on = false;
+    int mask = 0;  // bit k: time kernel k
     struct Rec {
         int id;
         hipEvent_t a, b;
@@ -143,7 +145,7 @@ struct Profiler {
     double ms[PK_COUNT] = {};
     double bytes[PK_COUNT] = {};
 };
-hipEvent_t prof_begin(Profiler* pf, hipStream_t s);
+hipEvent_t prof_begin(Profiler* pf, int id, hipStream_t s);  // nullptr when id is not timed
 void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s);
 void prof_flush(Profiler* pf);  // after the stream has been synchronised
 

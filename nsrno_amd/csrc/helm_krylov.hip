@@ -667,7 +667,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
     for (;;) {
         // true residual in FP64
-        hipEvent_t ev = pf ? prof_begin(pf, s) : nullptr;
+        hipEvent_t ev = pf ? prof_begin(pf, PK_RESIDUAL, s) : nullptr;
         HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
         if (pf) prof_end(pf, PK_RESIDUAL, ev, resid_bytes, s);
         applications++;
@@ -697,24 +697,24 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
             float2* zj = W.Z + (int64_t)j * vstride;
             if (pc != HELM_PC_NONE) {
-                ev = pf ? prof_begin(pf, s) : nullptr;
+                ev = pf ? prof_begin(pf, PK_PRECOND, s) : nullptr;
                 HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, s));
                 if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, s);
             } else {
                 HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, s));
             }
-            ev = pf ? prof_begin(pf, s) : nullptr;
+            ev = pf ? prof_begin(pf, PK_APPLY, s) : nullptr;
             HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
             if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, s);
             applications++;
-            ev = pf ? prof_begin(pf, s) : nullptr;
+            ev = pf ? prof_begin(pf, PK_DOTS, s) : nullptr;
             hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd, kmax, j, W.V, W.w, W.cyc,
                                W.partial, pd_part);
             if (pf) prof_end(pf, PK_DOTS, ev, (j + 2) * vec_bytes, s);
             hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, s, nchd, mm, kmax, j, o->tolerance, W.partial,
                                pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
                                W.used, W.coef);
-            ev = pf ? prof_begin(pf, s) : nullptr;
+            ev = pf ? prof_begin(pf, PK_UPDATE, s) : nullptr;
             hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
                                W.cyc, pw2);
             if (pf) prof_end(pf, PK_UPDATE, ev, (j + 4) * vec_bytes, s);

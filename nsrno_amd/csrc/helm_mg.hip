@@ -135,6 +135,8 @@ __global__ __launch_bounds__(256) void k_jacobi(int nz, int nx, int64_t n, int n
     }
 }
 
+__device__ __forceinline__ float2 cc(float2 a, bool conj) { return conj ? cconj(a) : a; }
+
 // ---- single-workgroup V-cycle over the small coarse levels ---------------
 // The coarse levels hold too few nodes to fill the GPU, so launching one
 // kernel per sweep is pure latency (tens of launches per V-cycle).  k_tail
@@ -175,8 +177,6 @@ __global__ void k_tail_coef(int nz, int nx, const float2* __restrict__ sx, const
     o[4] = c.fn;
     o[5] = c.fs;
 }
-
-__device__ __forceinline__ float2 cc(float2 a, bool conj) { return conj ? cconj(a) : a; }
 
 // M u at node q with u in LDS
 __device__ __forceinline__ float2 tail_mu(const LevelView& V, int64_t q, const float2* cf, const float2* u,
@@ -302,57 +302,338 @@ __global__ __launch_bounds__(1024) void k_tail(const LevelView* __restrict__ vie
     for (int64_t q = threadIdx.x; q < T.n; q += blockDim.x) u_top[(int64_t)r * T.n + q] = lds[q];
 }
 
-// full weighting: fc[I,J] = sum_{a,b} w_a w_b rf[2I+a, 2J+b], w = (1/4, 1/2, 1/4)
-__global__ void k_restrict(int nzf, int nxf, int nzc, int nxc, int nrhs, const float2* __restrict__ rf,
-                           float2* __restrict__ fc) {
-    const int J = blockIdx.x * blockDim.x + threadIdx.x;
-    const int I = blockIdx.y;
-    const int r = blockIdx.z;
-    if (J >= nxc || I >= nzc || r >= nrhs) return;
-    const float2* src = rf + (int64_t)r * nzf * nxf;
-    const float wz[3] = {0.25f, 0.5f, 0.25f};
-    float2 acc = make_float2(0.f, 0.f);
-    for (int a = -1; a <= 1; ++a) {
-        const int i = 2 * I + a;
-        if (i < 0 || i >= nzf) continue;
-        for (int b = -1; b <= 1; ++b) {
-            const int j = 2 * J + b;
-            if (j < 0 || j >= nxf) continue;
-            const float w = wz[a + 1] * wz[b + 1];
-            const float2 v = src[(int64_t)i * nxf + j];
-            acc.x += w * v.x;
-            acc.y += w * v.y;
-        }
-    }
-    fc[(int64_t)r * nzc * nxc + (int64_t)I * nxc + J] = acc;
+// ---- fused grid-level V-cycle kernels ----------------------------------
+// A level above the LDS tail costs four passes over its fields per V-cycle:
+//   pre     u = S(S(0))         one pass: the first sweep from zero is the
+//                               pointwise u1 = (w/dg) f, so the second sweep
+//                               needs only f and w/dg at the 5 stencil nodes
+//   down    f_c = R (f - M u)   residual on a fine tile in LDS, then full
+//                               weighting; the fine residual never reaches HBM
+//   up      t = S(u + P u_c)    bilinear prolongation evaluated at the 5
+//                               stencil nodes inside the first post-sweep
+//   post    u = S(t)            the remaining sweeps
+// Bytes per node per RHS: 16 + 18 + 26 + 24 = 84 (the unfused sequence of
+// 2 + 1 + 1 + 1 + 2 kernels moves 140).
+
+// per-node Jacobi factor of the shifted operator: wid = w / dg
+__global__ void k_node_wid(int nz, int nx, const float2* __restrict__ sx, const float2* __restrict__ sz,
+                           const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                           const float* __restrict__ eta, const float2* __restrict__ kap, float k0sq, float hh,
+                           float shift, const float2* __restrict__ wgt, float2* __restrict__ wid) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= nz || j >= nx) return;
+    Coef<float2> c;
+    node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, false, c);
+    const float dd = c.dg.x * c.dg.x + c.dg.y * c.dg.y;
+    const int64_t q = (int64_t)i * nx + j;
+    wid[q] = cmul(wgt[q], make_float2(c.dg.x / dd, -c.dg.y / dd));
 }
 
-// bilinear prolongation, added: uf[i,j] += P uc   (coarse ghosts are zero)
-__global__ void k_prolong_add(int nzf, int nxf, int nzc, int nxc, int nrhs, const float2* __restrict__ uc,
-                              float2* __restrict__ uf) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    const int r = blockIdx.z;
-    if (j >= nxf || i >= nzf || r >= nrhs) return;
-    const float2* src = uc + (int64_t)r * nzc * nxc;
+// The grid-level kernels work on 16 x 64 tiles (block 64 x 4, four rows per
+// thread).  The field a sweep reads is staged in LDS on the tile plus a
+// one-node halo with the operator's zero ghosts already in place, so the
+// stencil needs no bounds tests and every global load is independent and
+// coalesced; each thread keeps its four nodes' stencil in registers for all
+// right-hand sides of its group.
+constexpr int MT_TX = 64, MT_TZ = 16, MT_RPT = MT_TZ / 4;
+constexpr int MT_SW = MT_TX + 2, MT_SH = MT_TZ + 2, MT_NS = MT_SW * MT_SH;
+constexpr int MT_SPT = (MT_NS + 255) / 256;  // staged nodes per thread
+
+struct TileCoef {
+    Coef<float2> c[MT_RPT];
+    float2 w[MT_RPT];  // Jacobi factor w/dg
+    bool in[MT_RPT];
+};
+
+__device__ __forceinline__ void tile_coef(int z0, int x0, int nz, int nx, const float2* __restrict__ sx,
+                                          const float2* __restrict__ sz, const float2* __restrict__ isxe,
+                                          const float2* __restrict__ isze, const float* __restrict__ eta,
+                                          const float2* __restrict__ kap, float k0sq, float hh, float shift, bool cj,
+                                          const float2* __restrict__ wid, TileCoef& t) {
+#pragma unroll
+    for (int k = 0; k < MT_RPT; ++k) {
+        const int i = z0 + threadIdx.y + 4 * k, j = x0 + threadIdx.x;
+        t.in[k] = i < nz && j < nx;
+        if (t.in[k]) {
+            node_coef<float, float2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, t.c[k]);
+            t.w[k] = cc(wid[(int64_t)i * nx + j], cj);
+        }
+    }
+}
+
+// LDS index of tile node (row, col) of the staged block
+__device__ __forceinline__ int st_idx(int row, int col) { return (row + 1) * MT_SW + col + 1; }
+
+// M v at the thread's k-th node from the staged field
+__device__ __forceinline__ float2 tile_mv(const Coef<float2>& c, const float2* sv, int row, int col) {
+    const int q = st_idx(row, col);
+    float2 acc = cmul(c.dg, sv[q]);
+    cfma(acc, c.fw, sv[q - 1]);
+    cfma(acc, c.fe, sv[q + 1]);
+    cfma(acc, c.fn, sv[q - MT_SW]);
+    cfma(acc, c.fs, sv[q + MT_SW]);
+    return acc;
+}
+
+// one damped-Jacobi sweep out = u + w (f - M u)
+__global__ __launch_bounds__(256) void k_mgt_jacobi(int nz, int nx, int64_t n, int nrhs, int rb,
+                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                    const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                    const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                    float k0sq, float hh, float shift, int conj,
+                                                    const float2* __restrict__ wid, const float2* __restrict__ u,
+                                                    const float2* __restrict__ f, float2* __restrict__ out) {
+    __shared__ float2 sv[MT_NS];
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    TileCoef t;
+    tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, wid, t);
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    for (int r = r0; r < r1; ++r) {
+        const float2* ur = u + (int64_t)r * n;
+        const float2* fr = f + (int64_t)r * n;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < MT_NS) {
+                const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+                sv[e] = (gi >= 0 && gi < nz && gj >= 0 && gj < nx) ? ur[(int64_t)gi * nx + gj] : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MT_RPT; ++k) {
+            if (!t.in[k]) continue;
+            const int row = threadIdx.y + 4 * k, col = threadIdx.x;
+            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
+            float2 v = sv[st_idx(row, col)];
+            cfma(v, t.w[k], csub(fr[o], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + o] = v;
+        }
+    }
+}
+
+// out = S(S(0)) = u1 + w (f - M u1) with u1 = (w/dg) f staged on the halo
+__global__ __launch_bounds__(256) void k_mgt_pre2(int nz, int nx, int64_t n, int nrhs, int rb,
+                                                  const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                  const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                  const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                  float k0sq, float hh, float shift, int conj,
+                                                  const float2* __restrict__ wid, const float2* __restrict__ f,
+                                                  float2* __restrict__ out) {
+    __shared__ float2 sv[MT_NS];
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    const bool cj = conj != 0;
+    TileCoef t;
+    tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, t);
+    float2 ws[MT_SPT];  // w/dg at the staged nodes (0 outside the grid)
+    int64_t gs[MT_SPT];
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) {
+        const int e = tid + 256 * q;
+        ws[q] = make_float2(0.f, 0.f);
+        gs[q] = -1;
+        if (e < MT_NS) {
+            const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
+                gs[q] = (int64_t)gi * nx + gj;
+                ws[q] = cc(wid[gs[q]], cj);
+            }
+        }
+    }
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    for (int r = r0; r < r1; ++r) {
+        const float2* fr = f + (int64_t)r * n;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < MT_NS) sv[e] = gs[q] >= 0 ? cmul(ws[q], fr[gs[q]]) : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MT_RPT; ++k) {
+            if (!t.in[k]) continue;
+            const int row = threadIdx.y + 4 * k, col = threadIdx.x;
+            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
+            float2 v = sv[st_idx(row, col)];
+            cfma(v, t.w[k], csub(fr[o], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + o] = v;
+        }
+    }
+}
+
+// out = S(u + P u_c): the coarse patch under the tile and its halo is staged,
+// the corrected field v = u + P u_c is formed on the halo in LDS (bilinear
+// weights without branches), then one sweep; out must not alias u
+constexpr int MP_CH = MT_TZ / 2 + 3, MP_CW = MT_TX / 2 + 3, MP_NC = MP_CH * MP_CW;
+__global__ __launch_bounds__(256) void k_mgt_prolong_smooth(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
+                                                            int rb, const float2* __restrict__ sx,
+                                                            const float2* __restrict__ sz,
+                                                            const float2* __restrict__ isxe,
+                                                            const float2* __restrict__ isze,
+                                                            const float* __restrict__ eta,
+                                                            const float2* __restrict__ kap, float k0sq, float hh,
+                                                            float shift, int conj, const float2* __restrict__ wid,
+                                                            const float2* __restrict__ uc,
+                                                            const float2* __restrict__ u,
+                                                            const float2* __restrict__ f, float2* __restrict__ out) {
+    __shared__ float2 sv[MT_NS];
+    __shared__ float2 scv[MP_NC];
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
+    TileCoef t;
+    tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, wid, t);
+    const int64_t ncn = (int64_t)nzc * nxc;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    for (int r = r0; r < r1; ++r) {
+        const float2* ur = u + (int64_t)r * n;
+        const float2* ucr = uc + (int64_t)r * ncn;
+        __syncthreads();
+        for (int e = tid; e < MP_NC; e += 256) {
+            const int I = cz0 + e / MP_CW, J = cx0 + e % MP_CW;
+            scv[e] = (I >= 0 && I < nzc && J >= 0 && J < nxc) ? ucr[(int64_t)I * nxc + J] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < MT_NS) {
+                const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+                float2 v = make_float2(0.f, 0.f);
+                if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
+                    const int I = (gi >> 1) - cz0, J = (gj >> 1) - cx0;
+                    const float wz1 = (gi & 1) ? 0.5f : 0.f, wz0 = 1.f - wz1;
+                    const float wx1 = (gj & 1) ? 0.5f : 0.f, wx0 = 1.f - wx1;
+                    const float2* c0 = scv + I * MP_CW + J;
+                    v = ur[(int64_t)gi * nx + gj];
+                    v.x += wz0 * (wx0 * c0[0].x + wx1 * c0[1].x) + wz1 * (wx0 * c0[MP_CW].x + wx1 * c0[MP_CW + 1].x);
+                    v.y += wz0 * (wx0 * c0[0].y + wx1 * c0[1].y) + wz1 * (wx0 * c0[MP_CW].y + wx1 * c0[MP_CW + 1].y);
+                }
+                sv[e] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MT_RPT; ++k) {
+            if (!t.in[k]) continue;
+            const int row = threadIdx.y + 4 * k, col = threadIdx.x;
+            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
+            float2 v = sv[st_idx(row, col)];
+            cfma(v, t.w[k], csub(f[(int64_t)r * n + o], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + o] = v;
+        }
+    }
+}
+
+// f_c = R (f - M u): one workgroup per 8 x 32 coarse tile; u is staged on the
+// (2*8+3) x (2*32+3) fine nodes around it, the residual on the
+// (2*8+1) x (2*32+1) nodes under it is formed in LDS and fully weighted
+constexpr int RR_TZ = 8, RR_TX = 32;
+constexpr int RR_FZ = 2 * RR_TZ + 1, RR_FX = 2 * RR_TX + 1;
+constexpr int RR_NF = RR_FZ * RR_FX;
+constexpr int RR_PT = (RR_NF + 255) / 256;
+constexpr int RR_UZ = RR_FZ + 2, RR_UX = RR_FX + 2, RR_NU = RR_UZ * RR_UX;
+__global__ __launch_bounds__(256) void k_mgt_restrict_res(int nzf, int nxf, int nzc, int nxc, int nrhs, int rb,
+                                                          const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                          const float2* __restrict__ isxe,
+                                                          const float2* __restrict__ isze,
+                                                          const float* __restrict__ eta,
+                                                          const float2* __restrict__ kap, float k0sq, float hh,
+                                                          float shift, int conj, const float2* __restrict__ u,
+                                                          const float2* __restrict__ f, float2* __restrict__ fcoarse) {
+    __shared__ float2 su[RR_NU];
+    __shared__ float2 rs[RR_NF];
+    const int tid = threadIdx.y * RR_TX + threadIdx.x;
+    const int I0 = blockIdx.y * RR_TZ, J0 = blockIdx.x * RR_TX;
+    const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
+    const int64_t nf = (int64_t)nzf * nxf, nc = (int64_t)nzc * nxc;
+    const bool cj = conj != 0;
+    Coef<float2> c[RR_PT];
+    int64_t node[RR_PT];
+#pragma unroll
+    for (int q = 0; q < RR_PT; ++q) {
+        const int e = tid + 256 * q;
+        node[q] = -1;
+        if (e < RR_NF) {
+            const int fi = fi0 + e / RR_FX, fj = fj0 + e % RR_FX;
+            if (fi >= 0 && fi < nzf && fj >= 0 && fj < nxf) {
+                node[q] = (int64_t)fi * nxf + fj;
+                node_coef<float, float2>(fi, fj, nzf, nxf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c[q]);
+            }
+        }
+    }
+    const int I = I0 + threadIdx.y, J = J0 + threadIdx.x;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    for (int r = r0; r < r1; ++r) {
+        const float2* ur = u + (int64_t)r * nf;
+        const float2* fr = f + (int64_t)r * nf;
+        __syncthreads();
+        for (int e = tid; e < RR_NU; e += 256) {
+            const int gi = fi0 - 1 + e / RR_UX, gj = fj0 - 1 + e % RR_UX;
+            su[e] = (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) ? ur[(int64_t)gi * nxf + gj] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RR_PT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < RR_NF) {
+                float2 v = make_float2(0.f, 0.f);
+                if (node[q] >= 0) {
+                    const int qu = (e / RR_FX + 1) * RR_UX + e % RR_FX + 1;
+                    float2 acc = cmul(c[q].dg, su[qu]);
+                    cfma(acc, c[q].fw, su[qu - 1]);
+                    cfma(acc, c[q].fe, su[qu + 1]);
+                    cfma(acc, c[q].fn, su[qu - RR_UX]);
+                    cfma(acc, c[q].fs, su[qu + RR_UX]);
+                    v = csub(fr[node[q]], acc);
+                }
+                rs[e] = v;
+            }
+        }
+        __syncthreads();
+        if (I < nzc && J < nxc) {
+            const float2* base = rs + (2 * threadIdx.y + 1) * RR_FX + 2 * threadIdx.x + 1;
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int a = -1; a <= 1; ++a)
+#pragma unroll
+                for (int b = -1; b <= 1; ++b) {
+                    const float w = (a == 0 ? 0.5f : 0.25f) * (b == 0 ? 0.5f : 0.25f);
+                    const float2 v = base[a * RR_FX + b];
+                    acc.x += w * v.x;
+                    acc.y += w * v.y;
+                }
+            fcoarse[(int64_t)r * nc + (int64_t)I * nxc + J] = acc;
+        }
+    }
+}
+
+// bilinear prolongation of the coarse correction at fine node (i, j) (zero
+// coarse ghosts)
+__device__ __forceinline__ float2 prolong_at(const float2* __restrict__ uc, int nzc, int nxc, int i, int j) {
     const int I0 = i >> 1, J0 = j >> 1;
     const bool oi = i & 1, oj = j & 1;
     auto at = [&](int I, int J) -> float2 {
         if (I >= nzc || J >= nxc) return make_float2(0.f, 0.f);
-        return src[(int64_t)I * nxc + J];
+        return uc[(int64_t)I * nxc + J];
     };
-    float2 v;
-    if (!oi && !oj) {
-        v = at(I0, J0);
-    } else if (oi && !oj) {
-        v = cscale(cadd(at(I0, J0), at(I0 + 1, J0)), 0.5f);
-    } else if (!oi && oj) {
-        v = cscale(cadd(at(I0, J0), at(I0, J0 + 1)), 0.5f);
-    } else {
-        v = cscale(cadd(cadd(at(I0, J0), at(I0 + 1, J0)), cadd(at(I0, J0 + 1), at(I0 + 1, J0 + 1))), 0.25f);
-    }
-    float2* dst = uf + (int64_t)r * nzf * nxf + (int64_t)i * nxf + j;
-    *dst = cadd(*dst, v);
+    if (!oi && !oj) return at(I0, J0);
+    if (oi && !oj) return cscale(cadd(at(I0, J0), at(I0 + 1, J0)), 0.5f);
+    if (!oi && oj) return cscale(cadd(at(I0, J0), at(I0, J0 + 1)), 0.5f);
+    return cscale(cadd(cadd(at(I0, J0), at(I0 + 1, J0)), cadd(at(I0, J0 + 1), at(I0 + 1, J0 + 1))), 0.25f);
+}
+
+// u += P u_c (only when no post-smoothing follows)
+__global__ void k_prolong_add(int nz, int nx, int nzc, int nxc, int nrhs, const float2* __restrict__ uc,
+                              float2* __restrict__ u) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int r = blockIdx.z;
+    if (j >= nx || i >= nz || r >= nrhs) return;
+    float2* dst = u + (int64_t)r * nz * nx + (int64_t)i * nx + j;
+    *dst = cadd(*dst, prolong_at(uc + (int64_t)r * nzc * nxc, nzc, nxc, i, j));
 }
 
 // coarse coefficients by full weighting (exterior values replicate the edge)
@@ -429,6 +710,14 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                                (float)p->mg.shift, (int)(l + 1 == p->levels.size()), L.wgt);
         }
         HELM_HIP(hipGetLastError());
+        if (!L.wid) HELM_HIP(hipMalloc((void**)&L.wid, L.n * sizeof(float2) + 256));
+        {
+            dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4);
+            hipLaunchKernelGGL(k_node_wid, grid, block, 0, s, L.nz, L.nx, L.sx32, L.sz32, L.isxe32, L.isze32, L.eta32,
+                               L.kap32, (float)(p->k0 * p->k0), (float)(0.5 / (L.h_m * L.h_m)), (float)p->mg.shift,
+                               L.wgt, L.wid);
+            HELM_HIP(hipGetLastError());
+        }
         if (L.n <= TAIL_NODES) {
             if (!L.tcf) HELM_HIP(hipMalloc((void**)&L.tcf, 6 * L.n * sizeof(float2) + 256));
             hipLaunchKernelGGL(k_tail_coef, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.nz, L.nx, L.sx32,
@@ -449,6 +738,16 @@ static int rhs_group(int64_t blocks_xy, int nrhs) {
 static int32_t jacobi(helm_plan* p, int l, double shift, bool adj, const float2* u, const float2* f,
                       float2* out, int nrhs, hipStream_t s) {
     const Level& L = p->levels[l];  // conj(M) uses conj(w): the same contraction
+    if (u != nullptr && L.wid) {  // sweep from a start vector: LDS-tiled kernel
+        const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
+        const int rb = rhs_group(tiles, nrhs);
+        const dim3 grid((L.nx + MT_TX - 1) / MT_TX, (L.nz + MT_TZ - 1) / MT_TZ, (nrhs + rb - 1) / rb);
+        hipLaunchKernelGGL(k_mgt_jacobi, grid, dim3(64, 4), 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32, L.isxe32,
+                           L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0), (float)(0.5 / (L.h_m * L.h_m)),
+                           (float)shift, (int)adj, L.wid, u, f, out);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
     const int rb = rhs_group(bxy, nrhs);
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
@@ -495,40 +794,60 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     }
     const bool coarsest = l + 1 == (int)p->levels.size();
     const int sweeps = coarsest ? c.coarse : c.pre;
-    // pre-smoothing from a zero guess; the last sweep writes u
-    const float2* src = nullptr;
-    for (int k = 0; k < sweeps; ++k) {
-        float2* dst = (sweeps - k) % 2 == 1 ? u : L.mg_t;
-        HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
-        src = dst;
-    }
+    const float k0sq = (float)(p->k0 * p->k0), hh = (float)(0.5 / (L.h_m * L.h_m)), sh = (float)c.shift;
+    const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
+    const int rb = rhs_group(tiles, nrhs);
+    const dim3 blk(64, 4), grd((L.nx + MT_TX - 1) / MT_TX, (L.nz + MT_TZ - 1) / MT_TZ, (nrhs + rb - 1) / rb);
+    // pre-smoothing from a zero guess, ending in u
     if (sweeps == 0) {
         const int64_t cnt = (int64_t)nrhs * L.n;
         hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, cnt, u);
         HELM_HIP(hipGetLastError());
+    } else if (sweeps == 1) {
+        HELM_TRY(jacobi(p, l, c.shift, adj, nullptr, f, u, nrhs, s));
+    } else {
+        const int rem = sweeps - 2;
+        float2* dst = rem % 2 == 0 ? u : L.mg_t;
+        hipLaunchKernelGGL(k_mgt_pre2, grd, blk, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32, L.isxe32, L.isze32,
+                           L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, f, dst);
+        HELM_HIP(hipGetLastError());
+        const float2* src = dst;
+        for (int k = 0; k < rem; ++k) {
+            dst = src == u ? L.mg_t : u;
+            HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
+            src = dst;
+        }
     }
     if (coarsest) return HELM_OK;
     Level& C = p->levels[l + 1];
-    // residual r = f - M u, restricted to the coarse right-hand side
-    HELM_TRY(launch_flux32(L, p->k0 * p->k0, u, L.mg_r, nrhs, adj, f, (float)c.shift, s));
-    hipLaunchKernelGGL(k_restrict, dim3((C.nx + 63) / 64, C.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs,
-                       L.mg_r, C.mg_f);
-    HELM_HIP(hipGetLastError());
-    HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
-    hipLaunchKernelGGL(k_prolong_add, dim3((L.nx + 63) / 64, L.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx,
-                       nrhs, C.mg_u, u);
-    HELM_HIP(hipGetLastError());
-    // post-smoothing, ping-pong with L.mg_t ending in u
-    src = u;
-    if (c.post % 2 == 1) {
-        HELM_HIP(hipMemcpyAsync(L.mg_t, u, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-        src = L.mg_t;
+    // coarse right-hand side R (f - M u), the fine residual kept in LDS
+    {
+        const int64_t bc = (int64_t)((C.nx + RR_TX - 1) / RR_TX) * ((C.nz + RR_TZ - 1) / RR_TZ);
+        const int rbc = rhs_group(bc, nrhs);
+        const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
+        hipLaunchKernelGGL(k_mgt_restrict_res, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
+                           L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, u, f, C.mg_f);
+        HELM_HIP(hipGetLastError());
     }
-    for (int k = 0; k < c.post; ++k) {
-        float2* dst = (src == u) ? L.mg_t : u;
+    HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
+    if (c.post == 0) {
+        hipLaunchKernelGGL(k_prolong_add, dim3((L.nx + 63) / 64, L.nz, nrhs), dim3(64), 0, s, L.nz, L.nx, C.nz, C.nx,
+                           nrhs, C.mg_u, u);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
+    // first post-sweep on u + P u_c into mg_t, the rest ping-pong, ending in u
+    hipLaunchKernelGGL(k_mgt_prolong_smooth, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
+                       L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, C.mg_u, u, f, L.mg_t);
+    HELM_HIP(hipGetLastError());
+    const float2* src = L.mg_t;
+    for (int k = 1; k < c.post; ++k) {
+        float2* dst = src == u ? L.mg_t : u;
         HELM_TRY(jacobi(p, l, c.shift, adj, src, f, dst, nrhs, s));
         src = dst;
     }
+    if (src != u)
+        HELM_HIP(hipMemcpyAsync(u, src, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
     return HELM_OK;
 }
 
@@ -576,7 +895,7 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     while (p->levels.size() > 1) {
         Level& L = p->levels.back();
         void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
-                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf};
+                        L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf, L.wid};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         p->levels.pop_back();

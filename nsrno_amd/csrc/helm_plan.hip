@@ -83,7 +83,7 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
                     L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf,
-                    L.ax32, L.az32, L.ax64, L.az64};
+                    L.wid, L.ax32, L.az32, L.ax64, L.az64};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();
@@ -128,7 +128,8 @@ __global__ void k_medium_physical(int64_t n, const double* __restrict__ c, const
 }
 
 // ---- kernel timing --------------------------------------------------------
-hipEvent_t prof_begin(Profiler* pf, hipStream_t s) {
+hipEvent_t prof_begin(Profiler* pf, int id, hipStream_t s) {
+    if (!(pf->mask >> id & 1)) return nullptr;
     hipEvent_t e = nullptr;
     if (!pf->pool.empty()) {
         e = pf->pool.back();
@@ -181,12 +182,13 @@ using namespace helm;
 
 extern "C" {
 
-int32_t helm_profile(helm_plan* p, int32_t enable) {
+int32_t helm_profile(helm_plan* p, int32_t mask) {
     HELM_CHECK(p, HELM_ERR_ARG, "helm_profile: null plan");
     if (!p->prof) p->prof = new Profiler();
     Profiler* pf = p->prof;
     prof_flush(pf);
-    pf->on = enable != 0;
+    pf->on = mask != 0;
+    pf->mask = mask;
     if (pf->on) {
         for (int k = 0; k < PK_COUNT; ++k) {
             pf->launches[k] = 0;

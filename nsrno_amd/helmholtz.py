@@ -181,9 +181,19 @@ class HelmholtzPlan:
         return z
 
     # ------------------------------------------------------------ kernel timing
-    def profile(self, enable=True):
-        """Turn HIP-event timing of the kernels inside solves on (clearing totals) or off."""
-        check(lib().helm_profile(self._h, int(bool(enable))), "helm_profile")
+    PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual")
+
+    def profile(self, kernels=None, enable=True):
+        """Turn HIP-event timing of kernels inside solves on (clearing totals) or off.
+
+        kernels: names from PROFILE_KERNELS (None = all).  Every timed launch
+        adds an event pair to the stream, so time only what is needed."""
+        mask = 0
+        if enable:
+            names = self.PROFILE_KERNELS if kernels is None else kernels
+            for k in names:
+                mask |= 1 << self.PROFILE_KERNELS.index(k)
+        check(lib().helm_profile(self._h, mask), "helm_profile")
 
     def profile_read(self):
         """{kernel: dict(launches, total_ms, avg_us, bytes, gb_s)} accumulated since profile(True)."""

@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--roofline-kernel", default="dcgs_dots",
                     choices=["dcgs_dots", "dcgs_update", "operator_apply", "true_residual"],
                     help="kernel timed with HIP events inside the timed solves for the roofline object")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of known size (PMC counter calibration) and exit")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="summarise rocprofv3 --pmc runs under DIR (fetch_bench, write_bench, fetch_copy, "
+                         "write_copy) into profiles/pmc_traffic.json and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
     ap.add_argument("--cpu-interior", type=int, default=None,
@@ -139,6 +144,75 @@ class Bench:
         return t, bytes_alg
 
 
+COPY_BYTES = 1 << 28  # bytes read (and written) by one calibration copy
+COPY_REPS = 5
+
+
+def copy_calibration():
+    """A device-to-device copy of COPY_BYTES, COPY_REPS times: the PMC counters
+    of these dispatches give the counter-to-bytes factor."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(COPY_REPS):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: {COPY_REPS} copies of {COPY_BYTES} B", flush=True)
+
+
+def _pmc_rows(directory, counter):
+    """(kernel name, counter value) per dispatch from a rocprofv3 --pmc run."""
+    import csv
+    import glob
+    out = []
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    out.append((r["Kernel_Name"], float(r["Counter_Value"])))
+    return out
+
+
+def pmc_summary(directory):
+    """FETCH_SIZE/WRITE_SIZE (KiB per dispatch) of the bench kernels, each
+    scaled by the factor that maps the same counter to the known bytes of the
+    calibration copy -> profiles/pmc_traffic.json."""
+    import collections
+    scale = {}
+    for counter, sub in (("FETCH_SIZE", "fetch_copy"), ("WRITE_SIZE", "write_copy")):
+        rows = _pmc_rows(os.path.join(directory, sub), counter)
+        vals = sorted(v for _, v in rows)
+        if not vals:
+            raise SystemExit(f"no {counter} rows under {directory}/{sub}")
+        big = vals[-COPY_REPS:]  # the calibration copies are the largest dispatches
+        scale[counter] = COPY_BYTES / (sum(big) / len(big))
+    short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_update": "k_dcgs_update", "k_apply<float": "k_apply<float>",
+             "k_mgt_jacobi": "k_mgt_jacobi", "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
+             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail"}
+    totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
+    for counter, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench")):
+        for name, v in _pmc_rows(os.path.join(directory, sub), counter):
+            for key, label in short.items():
+                if key in name:
+                    t = totals[label][counter]
+                    t[0] += v * scale[counter]
+                    t[1] += 1
+                    break
+    out = {"unit": "bytes per launch (mean over the launches of one 11-angle solve)",
+           "scale_fetch_bytes_per_count": scale["FETCH_SIZE"], "scale_write_bytes_per_count": scale["WRITE_SIZE"],
+           "kernels": {}}
+    for label, t in totals.items():
+        f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
+        if f[1] and w[1]:
+            fb, wb = f[0] / f[1], w[0] / w[1]
+            out["kernels"][label] = {"launches": f[1], "fetch_bytes_per_launch": round(fb),
+                                     "write_bytes_per_launch": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out), flush=True)
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -176,6 +250,9 @@ def cpu_baseline(bench, a):
 
 def main():
     a = parse()
+    if a.pmc_summary:
+        pmc_summary(a.pmc_summary)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
@@ -184,6 +261,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)
     # the roofline kernel is timed inside the timed steps; the other kernels'

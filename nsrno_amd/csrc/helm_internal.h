@@ -148,6 +148,11 @@ struct Profiler {
 hipEvent_t prof_begin(Profiler* pf, int id, hipStream_t s);  // nullptr when id is not timed
 void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s);
 void prof_flush(Profiler* pf);  // after the stream has been synchronised
+// event pair for a kernel launched with hipExtLaunchKernelGGL (the dispatch
+// itself records the events, so no marker packets are added); false when id
+// is not timed
+bool prof_pair(Profiler* pf, int id, hipEvent_t* a, hipEvent_t* b);
+void prof_push(Profiler* pf, int id, hipEvent_t a, hipEvent_t b, double bytes);
 
 struct MGConfig {
     int levels = 0;

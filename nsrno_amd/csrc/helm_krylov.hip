@@ -32,6 +32,8 @@
 // fixed-order sum over chunks; no atomics, so results are bit reproducible.
 #include <cmath>
 
+#include <hip/hip_ext.h>
+
 #include "helm_internal.h"
 
 namespace helm {
@@ -707,17 +709,26 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
             if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, s);
             applications++;
-            ev = pf ? prof_begin(pf, PK_DOTS, s) : nullptr;
-            hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd, kmax, j, W.V, W.w, W.cyc,
-                               W.partial, pd_part);
-            if (pf) prof_end(pf, PK_DOTS, ev, (j + 2) * vec_bytes, s);
+            hipEvent_t ea, eb;
+            if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
+                hipExtLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, ea, eb, 0, n, nrhs, nchd, kmax, j,
+                                      (const float2*)W.V, (const float2*)W.w, (const int*)W.cyc, W.partial, pd_part);
+                prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
+            } else {
+                hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd, kmax, j, W.V, W.w, W.cyc,
+                                   W.partial, pd_part);
+            }
             hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, s, nchd, mm, kmax, j, o->tolerance, W.partial,
                                pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
                                W.used, W.coef);
-            ev = pf ? prof_begin(pf, PK_UPDATE, s) : nullptr;
-            hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
-                               W.cyc, pw2);
-            if (pf) prof_end(pf, PK_UPDATE, ev, (j + 4) * vec_bytes, s);
+            if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
+                hipExtLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, ea, eb, 0, n, nrhs, nch, kmax, j,
+                                      W.V, W.w, (const double2*)W.coef, (const int*)W.cyc, pw2);
+                prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
+            } else {
+                hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
+                                   W.cyc, pw2);
+            }
             hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, s, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
                                W.g, W.est, W.inv, W.cyc, W.used);
             if (j + 1 < msteps)

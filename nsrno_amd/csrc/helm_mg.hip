@@ -20,6 +20,7 @@
 // coefficients (f32 eta + c64 kappa_c, shared by all right-hand sides) and
 // write u; transfer kernels read one level and write the other.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 #include "helm_coef.h"
@@ -150,7 +151,27 @@ __device__ __forceinline__ float2 cc(float2 a, bool conj) { return conj ? cconj(
 // per node with no dependent chains.
 // first level handled by k_tail: the largest level with at most TAIL_NODES
 // nodes (its LDS stack, 16 B per node over the tail levels, must fit 160 KB)
-constexpr int64_t TAIL_NODES = 80 * 80;
+constexpr int64_t TAIL_NODES = 80 * 80;  // LDS capacity bound
+constexpr int64_t TAIL_NODES_DEFAULT = 40 * 40;  // measured best on 576^2 (11 RHS)
+
+// tuning knobs, read once from the environment: HELM_MG_WG_TARGET (workgroups
+// a grid-level launch aims for) and HELM_MG_TAIL_NODES (largest level the
+// LDS-resident tail takes)
+static int64_t env_int(const char* name, int64_t dflt, int64_t lo, int64_t hi) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    const long long x = std::atoll(v);
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+static int64_t wg_target() {
+    static const int64_t t = env_int("HELM_MG_WG_TARGET", 2048, 64, 1 << 20);
+    return t;
+}
+static int64_t tail_nodes() {
+    static const int64_t t = env_int("HELM_MG_TAIL_NODES", TAIL_NODES_DEFAULT, 16, TAIL_NODES);
+    return t;
+}
+
 
 struct LevelView {
     int nz, nx;
@@ -718,7 +739,7 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                                L.wgt, L.wid);
             HELM_HIP(hipGetLastError());
         }
-        if (L.n <= TAIL_NODES) {
+        if (L.n <= tail_nodes()) {
             if (!L.tcf) HELM_HIP(hipMalloc((void**)&L.tcf, 6 * L.n * sizeof(float2) + 256));
             hipLaunchKernelGGL(k_tail_coef, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.nz, L.nx, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0),
@@ -729,9 +750,10 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
     return HELM_OK;
 }
 
-// right-hand sides per workgroup so that a level launches >= ~2048 workgroups
+// right-hand sides per workgroup so that a level launches >= wg_target() workgroups
 static int rhs_group(int64_t blocks_xy, int nrhs) {
-    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + blocks_xy - 1) / blocks_xy));
+    const int64_t target = wg_target();
+    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (target + blocks_xy - 1) / blocks_xy));
     return (int)((nrhs + groups - 1) / groups);
 }
 
@@ -785,7 +807,7 @@ static int32_t tail_views(helm_plan* p) {
 static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s) {
     Level& L = p->levels[l];
     const MGConfig& c = p->mg;
-    if (L.n <= TAIL_NODES) {
+    if (L.n <= tail_nodes()) {
         const size_t lds = tail_lds_bytes(p, l);
         hipLaunchKernelGGL(k_tail, dim3(nrhs), dim3(1024), lds, s, (const LevelView*)p->mg_views, l,
                            (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u);
@@ -913,7 +935,7 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     p->mg_views_valid = false;
     {
         int t = 0;
-        while (t < (int)p->levels.size() && p->levels[t].n > TAIL_NODES) ++t;
+        while (t < (int)p->levels.size() && p->levels[t].n > tail_nodes()) ++t;
         const size_t lds = t < (int)p->levels.size() ? tail_lds_bytes(p, t) : 0;
         HELM_CHECK(lds <= 160 * 1024, HELM_ERR_ARG, "helm_mg_setup: coarse levels exceed the LDS budget");
         HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

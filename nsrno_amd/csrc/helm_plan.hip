@@ -155,6 +155,35 @@ void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s) {
     pf->pending.push_back({id, a, b, bytes});
 }
 
+static hipEvent_t prof_event(Profiler* pf) {
+    hipEvent_t e = nullptr;
+    if (!pf->pool.empty()) {
+        e = pf->pool.back();
+        pf->pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+bool prof_pair(Profiler* pf, int id, hipEvent_t* a, hipEvent_t* b) {
+    *a = *b = nullptr;
+    if (!pf || !(pf->mask >> id & 1)) return false;
+    *a = prof_event(pf);
+    *b = prof_event(pf);
+    if (!*a || !*b) {
+        if (*a) pf->pool.push_back(*a);
+        if (*b) pf->pool.push_back(*b);
+        *a = *b = nullptr;
+        return false;
+    }
+    return true;
+}
+
+void prof_push(Profiler* pf, int id, hipEvent_t a, hipEvent_t b, double bytes) {
+    if (a && b) pf->pending.push_back({id, a, b, bytes});
+}
+
 void prof_flush(Profiler* pf) {
     for (auto& r : pf->pending) {
         float ms = 0.f;

@@ -17,6 +17,7 @@
 // profiles are tiny and stay in cache.  Algorithmic bytes per node per RHS:
 // 8 (read u) + 8 (write Au) + 12/R (coefficients).
 #include <algorithm>
+#include <cstdlib>
 
 #include "helm_internal.h"
 #include "helm_coef.h"
@@ -51,6 +52,120 @@ __global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nr
     }
 }
 
+// Streaming form of the same operator (FP32): one wavefront per (64-column
+// strip, run of tz rows, group of RB right-hand sides).  Each lane owns one
+// column and walks down the run holding rows i-1, i, i+1 of every RHS of its
+// group in registers, with row i+2 already in flight; the west/east
+// neighbours come from the adjacent lanes (cross-lane shuffles), and only
+// the two edge lanes load one extra value per row.  The node coefficients
+// are formed once per node for the whole group.  Each u value is read from
+// HBM once, plus (tz+2)/tz for the run's halo rows.  Each row step waits for
+// one memory latency, so short runs (8 rows) with many wavefronts in flight
+// do best.
+__device__ __forceinline__ float2 shfl_up1(float2 v) {
+    return make_float2(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64));
+}
+__device__ __forceinline__ float2 shfl_down1(float2 v) {
+    return make_float2(__shfl_down(v.x, 1, 64), __shfl_down(v.y, 1, 64));
+}
+
+template <int RB, int MODE>
+__global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n, int nrhs, int tz,
+                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                      const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                      const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                      float k0sq, float hh, float shift, int conj,
+                                                      const float2* __restrict__ u, float2* __restrict__ out,
+                                                      const float2* __restrict__ b) {
+    const int lane = threadIdx.x;
+    const int x0 = blockIdx.x * 64, j = x0 + lane;
+    const int i0 = (blockIdx.y * 4 + threadIdx.y) * tz;
+    if (i0 >= nz) return;  // uniform over the wavefront
+    const int i1 = min(nz, i0 + tz);
+    const int r0 = blockIdx.z * RB;
+    const bool colok = j < nx;
+    const int jj = colok ? j : nx - 1;  // lanes past the grid mirror the last column (results dropped)
+    const bool hw = jj > 0, he = jj < nx - 1;
+    const float2 zero = make_float2(0.f, 0.f);
+    auto ld = [&](int r, int i, int col) -> float2 {
+        return (r < nrhs && i >= 0 && i < nz) ? u[(int64_t)r * n + (int64_t)i * nx + col] : zero;
+    };
+    float2 un[RB], uc[RB], us[RB], up[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        un[q] = ld(r0 + q, i0 - 1, jj);
+        uc[q] = ld(r0 + q, i0, jj);
+        us[q] = ld(r0 + q, i0 + 1, jj);
+    }
+    for (int i = i0; i < i1; ++i) {
+        float2 ew[RB], ee[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            up[q] = ld(r0 + q, i + 2, jj);
+            ew[q] = (lane == 0 && x0 > 0) ? ld(r0 + q, i, x0 - 1) : zero;
+            ee[q] = (lane == 63 && x0 + 64 < nx) ? ld(r0 + q, i, x0 + 64) : zero;
+        }
+        Coef<float2> c;
+        node_coef<float, float2>(i, jj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
+        const bool hn = i > 0, hs = i < nz - 1;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            float2 w = shfl_up1(uc[q]), e = shfl_down1(uc[q]);
+            if (lane == 0) w = ew[q];
+            if (lane == 63) e = ee[q];
+            float2 acc = cmul(c.dg, uc[q]);
+            if (hw) cfma(acc, c.fw, w);
+            if (he) cfma(acc, c.fe, e);
+            if (hn) cfma(acc, c.fn, un[q]);
+            if (hs) cfma(acc, c.fs, us[q]);
+            const int r = r0 + q;
+            if (colok && r < nrhs) {
+                const int64_t o = (int64_t)r * n + (int64_t)i * nx + j;
+                if (MODE == 1) acc = csub(b[o], acc);
+                out[o] = acc;
+            }
+            un[q] = uc[q];
+            uc[q] = us[q];
+            us[q] = up[q];
+        }
+    }
+}
+
+// streaming-kernel shape (rows per run, RHS per group): tuning knobs read
+// once from HELM_STENCIL_TZ and HELM_STENCIL_RB
+static int stream_tz() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_STENCIL_TZ");
+        const int x = v ? std::atoi(v) : 0;
+        return x >= 2 && x <= 1024 ? x : 8;  // measured best on 576^2 x 11 (8 rows, 4 RHS)
+    }();
+    return t;
+}
+static int stream_rb() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_STENCIL_RB");
+        const int x = v ? std::atoi(v) : 0;
+        return (x == 1 || x == 2 || x == 4) ? x : 4;
+    }();
+    return t;
+}
+
+template <int RB>
+static void launch_stream(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                          const float2* b, float shift, hipStream_t s) {
+    const int tz = stream_tz();
+    const int runs = (L.nz + tz - 1) / tz;
+    const dim3 block(64, 4), grid((L.nx + 63) / 64, (runs + 3) / 4, (nrhs + RB - 1) / RB);
+    const float hh = (float)(0.5 / (L.h_m * L.h_m));
+    if (b)
+        hipLaunchKernelGGL((k_apply_stream<RB, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tz, L.sx32, L.sz32,
+                           L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out, b);
+    else
+        hipLaunchKernelGGL((k_apply_stream<RB, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tz, L.sx32, L.sz32,
+                           L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out,
+                           (const float2*)nullptr);
+}
+
 // right-hand sides per workgroup: enough groups for >= ~2048 workgroups
 static int rhs_per_group(const Level& L, int nrhs) {
     const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
@@ -75,6 +190,15 @@ int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* o
 
 int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                       const float2* b, float shift, hipStream_t s) {
+    if (L.nx >= 64) {  // streaming kernel (a strip needs a full wavefront of columns to pay off)
+        switch (stream_rb()) {
+            case 1: launch_stream<1>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+            case 2: launch_stream<2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+            default: launch_stream<4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+        }
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const int rb = rhs_per_group(L, nrhs);
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const float hh = (float)(0.5 / (L.h_m * L.h_m));

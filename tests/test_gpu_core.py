@@ -179,3 +179,24 @@ def test_born_data_and_jacobian_match_oracle():
     assert rel(bd, O.born_data(grid, f, w, mk, me)) < 1e-8
     bd2 = solver.born_data(f, w, mk, me, background_fields=u0)
     assert rel(bd2, bd) < 1e-8
+
+
+@pytest.mark.parametrize("nrhs", [1, 5])
+def test_apply_fp32_streaming_shapes(nrhs):
+    """The streaming FP32 stencil on a grid that is no multiple of its strip
+    (64 columns) or its row runs, with a RHS count that leaves a partial group."""
+    from nsrno_amd.helmholtz import HelmholtzPlan
+    grid = O.grid_with_pml(170, 131, 12, 0.05)
+    nz, nx = grid.shape
+    rng = np.random.default_rng(5)
+    eta = 1 + 0.1 * rng.random((nz, nx))
+    kap = (1 + 0.2 * rng.random((nz, nx))) * (1 + 0.05j * rng.random((nz, nx)))
+    plan = HelmholtzPlan(nz, nx, grid.h_mm, 2.0, grid.pml_mm, x0_mm=grid.x_mm[0], z0_mm=grid.z_mm[0], nrhs=nrhs,
+                         restart=4)
+    plan.set_medium(eta, kap)
+    u = random_fields(grid.shape, nrhs, 7)
+    ref = O.apply(grid, 2.0, u, eta, kap)
+    out = plan.apply(torch.from_numpy(u.astype(np.complex64)).cuda()).cpu().numpy()
+    assert rel(out, ref) < 2e-6
+    outa = plan.apply(torch.from_numpy(u.astype(np.complex64)).cuda(), adjoint=True).cpu().numpy()
+    assert rel(outa, np.conj(O.apply(grid, 2.0, np.conj(u), eta, kap))) < 2e-6

@@ -3,12 +3,18 @@
 Workload (default): 512^2 interior + 32-cell stretched PML (576^2 nodes),
 3 MHz, dx = lambda_min/10, seeded smooth random medium (c, rho, power-law
 alpha), 11 plane-wave angles in [-15, +15] deg from a 192-element line array,
-192 receivers; complex64 (F)GMRES(30) with FP64 inner products and FP64
-true-residual refinement to relative residual 1e-6 on every angle.
+192 receivers; complex64 flexible GMRES(20) with FP64 inner products and
+FP64 true-residual refinement to relative residual 1e-6 on every angle.
+
+"learned FGMRES" (the timed leg): a warm start of 8 minimal-residual steps
+along the learned directions MG(r) + N(r) (the BF16 U-Net trained by
+nsrno_amd/train_preconditioner.py, weights in nsrno_amd/data/), then
+multigrid-preconditioned FGMRES.  Multigrid FGMRES without the net and plain
+GMRES are timed beside it.
 
 A "step" = one full batched solve of the 11 angles from a zero start:
-coefficients + multigrid set-up on the device, source injection, the Krylov
-solve to 1e-6, receiver sampling.  Inputs (c, rho, alpha maps) are resident
+coefficients + multigrid set-up on the device, source injection, warm start,
+the Krylov solve to 1e-6, receiver sampling.  Inputs (c, rho, alpha maps) are resident
 in HBM before the timed region.  ``value`` = solves/s = 11 * K / seconds.
 
 Prints ONE JSON line (rank 0, N=1).  --gpus N>1 exits with a message: this
@@ -35,7 +41,12 @@ def parse():
     ap.add_argument("--restart", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=6000)
-    ap.add_argument("--plain-max-iterations", type=int, default=1500,
+    ap.add_argument("--warm", default="learned", choices=["learned", "mg", "none"],
+                    help="warm-start directions before the FGMRES solve")
+    ap.add_argument("--warm-steps", type=int, default=8)
+    ap.add_argument("--learned-params", default=os.path.join(ROOT, "nsrno_amd", "data",
+                                                             "learned_warm_576x576_3mhz.npz"))
+    ap.add_argument("--plain-max-iterations", type=int, default=6000,
                     help="iteration cap of the plain (unpreconditioned) GMRES comparison leg")
     ap.add_argument("--interior", type=int, default=512)
     ap.add_argument("--pml", type=int, default=32)
@@ -90,31 +101,42 @@ class Bench:
                                   nrhs=a.angles, restart=a.restart)
         self.R = a.angles
         self.n = w.nz * w.nx
+        if a.warm == "learned":
+            from nsrno_amd.learned import LearnedNet, attach, canvas
+            self.net = LearnedNet(canvas(w.nz), canvas(w.nx), a.angles,
+                                  params=LearnedNet.load_params(a.learned_params),
+                                  linear=LearnedNet.load_linear(a.learned_params))
+            attach(self.plan, self.net)
 
-    def step(self, precond, max_iterations):
+    def step(self, precond, max_iterations, warm="none"):
         a, p = self.a, self.plan
         p.set_medium_physical(self.c, self.rho, self.alpha, rho0=self.work.rho0)
         if precond != "none":
             p.setup_multigrid(levels=a.mg_levels, shift=a.mg_shift, pre=a.mg_pre, post=a.mg_post,
                               coarse_sweeps=a.mg_coarse, omega=a.mg_omega)
         rhs = p.inject(self.ports, self.amp_d)
-        x, infos = p.solve(rhs, tolerance=a.tol, restart=a.restart, max_iterations=max_iterations,
+        x0 = None
+        if warm != "none":
+            x0 = self.torch.zeros_like(rhs)
+            p.warm_start(rhs, x0, a.warm_steps, precond=warm)
+        x, infos = p.solve(rhs, x0=x0, tolerance=a.tol, restart=a.restart, max_iterations=max_iterations,
                            precond=precond)
         traces = p.sample(self.ports, x)
+        self.warm_applications = 2 * a.warm_steps if warm != "none" else 0
         return x, infos, traces
 
-    def timed(self, precond, steps, warmup, max_iterations, profile=None):
+    def timed(self, precond, steps, warmup, max_iterations, profile=None, warm="none"):
         """Time `steps` solves; `profile` names kernels whose launches the library
         times with HIP events over the same timed steps (helm_profile)."""
         torch = self.torch
         for _ in range(warmup):
-            self.step(precond, max_iterations)
+            self.step(precond, max_iterations, warm)
         torch.cuda.synchronize()
         if profile:
             self.plan.profile(profile)
         t0 = time.perf_counter()
         for _ in range(steps):
-            x, infos, traces = self.step(precond, max_iterations)
+            x, infos, traces = self.step(precond, max_iterations, warm)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         self.kernel_stats = self.plan.profile_read() if profile else None
@@ -268,9 +290,11 @@ def main():
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)
     # the roofline kernel is timed inside the timed steps; the other kernels'
     # split comes from one extra profiled solve outside the timed region
-    dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations, profile=[a.roofline_kernel])
+    dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations, profile=[a.roofline_kernel],
+                                   warm=a.warm)
     kroof = b.kernel_stats[a.roofline_kernel]
-    b.timed(a.precond, 1, 0, a.max_iterations, profile=list(b.plan.PROFILE_KERNELS))
+    applications = [i.applications + b.warm_applications for i in infos]
+    b.timed(a.precond, 1, 0, a.max_iterations, profile=list(b.plan.PROFILE_KERNELS), warm=a.warm)
     kstats = b.kernel_stats
     solves_per_s = b.R * a.steps / dt
     iters = [i.iterations for i in infos]
@@ -278,10 +302,19 @@ def main():
     print(f"[bench] {a.precond}: {dt / a.steps * 1e3:.1f} ms/step, iterations {iters}, max rel {max(rel):.2e}",
           flush=True)
     result = {}
+    if a.warm != "none" and a.precond != "none":
+        mdt, _, minf, _ = b.timed(a.precond, a.steps, 1, a.max_iterations)
+        result["mg_fgmres_without_net"] = dict(
+            seconds_per_step=round(mdt / a.steps, 5), solves_per_s=round(b.R * a.steps / mdt, 4),
+            iterations=[i.iterations for i in minf], applications=[i.applications for i in minf],
+            max_relative_residual=max(i.relative_residual for i in minf))
+        print(f"[bench] mg-fgmres without the net: {mdt / a.steps * 1e3:.1f} ms/step, applications "
+              f"{max(result['mg_fgmres_without_net']['applications'])}", flush=True)
     if not a.no_plain and a.precond != "none":
         pdt, _, pinf, _ = b.timed("none", 1, 0, a.plain_max_iterations)
         result["plain_gmres"] = dict(
             seconds_per_step=pdt, solves_per_s=b.R / pdt, iterations=[i.iterations for i in pinf],
+            applications=[i.applications for i in pinf],
             max_relative_residual=max(i.relative_residual for i in pinf),
             reasons=sorted(set(i.reason for i in pinf)), max_iterations=a.plain_max_iterations)
         print(f"[bench] plain: {pdt * 1e3:.1f} ms, iterations {result['plain_gmres']['iterations'][:3]}..., "
@@ -327,8 +360,9 @@ def main():
         "data": "synthetic (seeded smooth random medium)",
         "config": {"workload": f"helmholtz {a.interior}^2+{a.pml}PML {a.freq}MHz x{a.angles} angles",
                    "nodes": b.n, "rhs": b.R, "tolerance": a.tol, "method": f"fgmres({a.restart})",
-                   "precond": a.precond, "parallelism": "single GPU"},
-        "iterations": iters, "max_relative_residual": max(rel),
+                   "precond": a.precond, "warm_start": f"{a.warm} x{a.warm_steps}" if a.warm != "none" else "none",
+                   "parallelism": "single GPU"},
+        "iterations": iters, "applications": applications, "max_relative_residual": max(rel),
         "reasons": sorted(set(i.reason for i in infos)),
         "roofline": roofline, "cpu_baseline": cpu,
     }

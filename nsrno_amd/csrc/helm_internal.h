@@ -174,6 +174,7 @@ struct helm_plan {
     helm::MGConfig mg;
     struct helm_net* net = nullptr;  // learned preconditioner (not owned)
     float* net_scale = nullptr;      // per-RHS residual scale for the net
+    double* net_part = nullptr;      // partial sums of that scale (4096 x 128)
     void* mg_views = nullptr;      // device copy of the per-level views used by k_tail
     bool mg_views_valid = false;
     helm::Workspace ws;

@@ -188,7 +188,7 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
 
 // ---- feature / output maps between the solver's fields and the net --------
 // features for RHS r on a canvas H x W (grid nz x nx at the top-left, zero
-// elsewhere).  scale[r] = RMS of the residual (computed by k_rms).
+// elsewhere).  scale[r] = RMS of the residual (computed by launch_rms).
 __global__ void k_features(int nz, int nx, int H, int W, int nrhs, const float2* __restrict__ v,
                            const float* __restrict__ inv_scale, const float* __restrict__ eta,
                            const float2* __restrict__ kap, const float2* __restrict__ sx,
@@ -221,12 +221,15 @@ __global__ void k_features(int nz, int nx, int H, int W, int nrhs, const float2*
     for (int c = 8; c < 32; ++c) o[c] = (bf16)0.f;
 }
 
-// inv_scale[r] = 1 / RMS(v_r)  (one workgroup per RHS)
-__global__ void k_rms(int64_t n, const float2* __restrict__ v, float* __restrict__ inv_scale) {
+// inv_scale[r] = 1 / RMS(v_r) in two deterministic stages: RMS_BLOCKS
+// workgroups per RHS sum strided slices, then one workgroup per RHS adds the
+// slices in a fixed order
+constexpr int RMS_BLOCKS = 128;
+__global__ void k_rms_partial(int64_t n, const float2* __restrict__ v, double* __restrict__ part) {
     __shared__ double red[256];
-    const int r = blockIdx.x;
+    const int r = blockIdx.y;
     double s = 0.0;
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)RMS_BLOCKS * blockDim.x) {
         const float2 a = v[(int64_t)r * n + q];
         s += (double)a.x * a.x + (double)a.y * a.y;
     }
@@ -236,10 +239,31 @@ __global__ void k_rms(int64_t n, const float2* __restrict__ v, float* __restrict
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) part[(int64_t)r * RMS_BLOCKS + blockIdx.x] = red[0];
+}
+
+__global__ void k_rms_final(int64_t n, const double* __restrict__ part, float* __restrict__ inv_scale) {
+    __shared__ double red[RMS_BLOCKS];
+    const int r = blockIdx.x;
+    red[threadIdx.x] = part[(int64_t)r * RMS_BLOCKS + threadIdx.x];
+    __syncthreads();
+    for (int w = RMS_BLOCKS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         const double rms = sqrt(red[0] / (double)n);
         inv_scale[r] = rms > 0 ? (float)(1.0 / rms) : 0.f;
     }
+}
+
+static int32_t launch_rms(helm_plan* p, int64_t n, const float2* v, int nrhs, hipStream_t s) {
+    if (!p->net_scale) HELM_HIP(hipMalloc((void**)&p->net_scale, 4096 * sizeof(float)));
+    if (!p->net_part) HELM_HIP(hipMalloc((void**)&p->net_part, (size_t)4096 * RMS_BLOCKS * sizeof(double)));
+    hipLaunchKernelGGL(k_rms_partial, dim3(RMS_BLOCKS, nrhs), dim3(256), 0, s, n, v, p->net_part);
+    hipLaunchKernelGGL(k_rms_final, dim3(nrhs), dim3(RMS_BLOCKS), 0, s, n, (const double*)p->net_part, p->net_scale);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
 }
 
 // z[r] += (out[...,0] + i out[...,1]) * h^2 / inv_scale[r]   (crop the canvas).
@@ -263,8 +287,7 @@ int32_t net_correction(helm_plan* p, helm_net* n, const float2* v, float2* z, in
     const Level& L = p->levels[0];
     HELM_CHECK(n->H >= L.nz && n->W >= L.nx && n->batch >= nrhs, HELM_ERR_ARG,
                "learned preconditioner: net canvas or batch too small for this plan");
-    if (!p->net_scale) HELM_HIP(hipMalloc((void**)&p->net_scale, 4096 * sizeof(float)));
-    hipLaunchKernelGGL(k_rms, dim3(nrhs), dim3(256), 0, s, L.n, v, p->net_scale);
+    HELM_TRY(launch_rms(p, L.n, v, nrhs, s));
     const int saved = n->batch;
     n->batch = nrhs;
     const int64_t cnt = (int64_t)nrhs * n->H * n->W;
@@ -427,8 +450,8 @@ int32_t helm_net_features(helm_plan* p, const void* v, void* features, int32_t n
     hipStream_t s = (hipStream_t)stream;
     const Level& L = p->levels[0];
     helm_net* n = p->net;
-    if (!p->net_scale) HELM_HIP(hipMalloc((void**)&p->net_scale, 4096 * sizeof(float)));
-    hipLaunchKernelGGL(k_rms, dim3(nrhs), dim3(256), 0, s, L.n, (const float2*)v, p->net_scale);
+    HELM_CHECK(nrhs >= 1 && nrhs <= 4096, HELM_ERR_ARG, "helm_net_features: nrhs out of range");
+    HELM_TRY(launch_rms(p, L.n, (const float2*)v, nrhs, s));
     const int64_t cnt = (int64_t)nrhs * n->H * n->W;
     hipLaunchKernelGGL(k_features, g1d(cnt), dim3(256), 0, s, L.nz, L.nx, n->H, n->W, nrhs, (const float2*)v,
                        p->net_scale, L.eta32, L.kap32, L.sx32, L.sz32, (float)(p->k0 * L.h_m), (int)n->linear,

@@ -304,6 +304,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     free_workspace(p->ws);
     if (p->mg_views) (void)hipFree(p->mg_views);
     if (p->net_scale) (void)hipFree(p->net_scale);
+    if (p->net_part) (void)hipFree(p->net_part);
     if (p->sp_buf) (void)hipFree(p->sp_buf);
     prof_destroy(p->prof);
     delete p;

@@ -62,6 +62,11 @@ def parse():
     ap.add_argument("--roofline-kernel", default="dcgs_dots",
                     choices=["dcgs_dots", "dcgs_update", "operator_apply", "true_residual"],
                     help="kernel timed with HIP events inside the timed solves for the roofline object")
+    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies"],
+                    help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
+                         "64 frequencies x 11 angles on uneven grids")
+    ap.add_argument("--frequencies", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=4, help="frequencies in flight (frequencies workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (PMC counter calibration) and exit")
     ap.add_argument("--pmc-summary", metavar="DIR",
@@ -235,6 +240,39 @@ def pmc_summary(directory):
     print(json.dumps(out), flush=True)
 
 
+def frequency_sweep(a):
+    """Batched uneven solves: `a.frequencies` frequencies evenly spaced in
+    4.0-8.9 MHz x 11 angles; each grid sized for 10 points per wavelength at
+    1440 m/s (interior 384^2 at 4 MHz to 854^2 at 8.9 MHz, + 32-cell PML) on
+    one inclusion medium resampled; frequencies in flight on `a.workers` HIP
+    streams.  One JSON line."""
+    import numpy as np
+    from nsrno_amd.batch import run_sweep
+    freqs = np.linspace(4.0, 8.9, a.frequencies)
+    angles = np.linspace(-15.0, 15.0, a.angles)
+    for _ in range(a.warmup):  # one small frequency warms the kernels and allocator
+        run_sweep(freqs[:1], angles, tol=a.tol, restart=a.restart, workers=1)
+    cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers,
+                            log=lambda m: print("[bench] " + m, flush=True))
+    solves = len(freqs) * len(angles)
+    kbytes = sum(c.krylov_bytes for c in cases)
+    line = {
+        "metric": f"solves/s, {len(freqs)} frequencies (4.0-8.9 MHz) x {len(angles)} angles, uneven grids",
+        "value": round(solves / wall, 3), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (FP64 reductions/refinement)", "data": "synthetic (seeded inclusion medium, resampled)",
+        "config": {"workload": "frequency sweep", "frequencies": len(freqs), "angles": len(angles),
+                   "interiors": [cases[0].interior, cases[-1].interior], "pml": 32, "ppw": 10,
+                   "tolerance": a.tol, "method": f"fgmres({a.restart}) + multigrid",
+                   "streams": a.workers, "parallelism": "single GPU"},
+        "krylov_operator_gb_s": round(kbytes / wall / 1e9, 1),
+        "all_converged": all(c.reasons == ("converged",) for c in cases),
+        "max_relative_residual": max(c.relative_residual for c in cases),
+        "steps_range": [min(max(c.iterations) for c in cases), max(max(c.iterations) for c in cases)],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -285,6 +323,9 @@ def main():
         sys.exit(1)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.workload == "frequencies":
+        frequency_sweep(a)
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)

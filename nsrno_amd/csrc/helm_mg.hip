@@ -938,7 +938,9 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
         while (t < (int)p->levels.size() && p->levels[t].n > tail_nodes()) ++t;
         const size_t lds = t < (int)p->levels.size() ? tail_lds_bytes(p, t) : 0;
         HELM_CHECK(lds <= 160 * 1024, HELM_ERR_ARG, "helm_mg_setup: coarse levels exceed the LDS budget");
-        HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        // the attribute is per kernel, not per plan: allow the full 160 KB so
+        // plans set up on other threads never lower it under each other
+        HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
     p->mg.shift = shift;
     p->mg.pre = pre;

@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--restart", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=6000)
+    ap.add_argument("--graph", type=int, default=1, help="1: replay restart cycles as captured HIP graphs")
+    ap.add_argument("--autotune", type=int, default=1, help="1: time launch shapes on the plan first (helm_autotune)")
     ap.add_argument("--warm", default="learned", choices=["learned", "mg", "none"],
                     help="warm-start directions before the FGMRES solve")
     ap.add_argument("--warm-steps", type=int, default=8)
@@ -125,7 +127,7 @@ class Bench:
             x0 = self.torch.zeros_like(rhs)
             p.warm_start(rhs, x0, a.warm_steps, precond=warm)
         x, infos = p.solve(rhs, x0=x0, tolerance=a.tol, restart=a.restart, max_iterations=max_iterations,
-                           precond=precond)
+                           precond=precond, graph=bool(a.graph))
         traces = p.sample(self.ports, x)
         self.warm_applications = 2 * a.warm_steps if warm != "none" else 0
         return x, infos, traces
@@ -253,7 +255,8 @@ def frequency_sweep(a):
     for _ in range(a.warmup):  # one small frequency warms the kernels and allocator
         run_sweep(freqs[:1], angles, tol=a.tol, restart=a.restart, workers=1)
     cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers,
-                            log=lambda m: print("[bench] " + m, flush=True))
+                            log=lambda m: print("[bench] " + m, flush=True), autotune=bool(a.autotune),
+                            graph=bool(a.graph))
     solves = len(freqs) * len(angles)
     kbytes = sum(c.krylov_bytes for c in cases)
     line = {
@@ -264,7 +267,8 @@ def frequency_sweep(a):
         "config": {"workload": "frequency sweep", "frequencies": len(freqs), "angles": len(angles),
                    "interiors": [cases[0].interior, cases[-1].interior], "pml": 32, "ppw": 10,
                    "tolerance": a.tol, "method": f"fgmres({a.restart}) + multigrid",
-                   "streams": a.workers, "parallelism": "single GPU"},
+                   "streams": a.workers, "autotune": bool(a.autotune), "graph": bool(a.graph),
+                   "parallelism": "single GPU"},
         "krylov_operator_gb_s": round(kbytes / wall / 1e9, 1),
         "all_converged": all(c.reasons == ("converged",) for c in cases),
         "max_relative_residual": max(c.relative_residual for c in cases),
@@ -329,6 +333,13 @@ def main():
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)
+    tuning = None
+    if a.autotune:
+        b.plan.set_medium_physical(b.c, b.rho, b.alpha, rho0=b.work.rho0)
+        b.plan.setup_multigrid(levels=a.mg_levels, shift=a.mg_shift, pre=a.mg_pre, post=a.mg_post,
+                               coarse_sweeps=a.mg_coarse, omega=a.mg_omega)
+        tuning = b.plan.autotune(b.R)
+        print(f"[bench] autotune {tuning}", flush=True)
     # the roofline kernel is timed inside the timed steps; the other kernels'
     # split comes from one extra profiled solve outside the timed region
     dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations, profile=[a.roofline_kernel],
@@ -405,7 +416,7 @@ def main():
                    "parallelism": "single GPU"},
         "iterations": iters, "applications": applications, "max_relative_residual": max(rel),
         "reasons": sorted(set(i.reason for i in infos)),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "autotune": tuning,
     }
     line.update(result)
     print(json.dumps(line), flush=True)

@@ -102,6 +102,9 @@ typedef struct {
      * nsrno/wave_solver.py:9-17; may be NULL */
     void (*callback)(void* user, int32_t applications, const double* relative, int32_t nrhs);
     void* user;
+    /* nonzero: GMRES replays every full restart cycle after the first as one
+     * captured HIP graph (the cycle's kernels, preconditioner included) */
+    int32_t use_graph;
 } helm_solve_opts;
 
 typedef struct {
@@ -264,6 +267,21 @@ int32_t helm_spectral_rhs(helm_plan* plan, const void* inc, void* rhs, int32_t n
  * [nrhs][N]; x holds the initial guess.  adjoint as in helm_solve. */
 int32_t helm_warm_start(helm_plan* plan, int32_t precond, const void* b, void* x, int32_t nrhs,
                         int32_t steps, int32_t adjoint, void* stream);
+
+/* ---- autotuning of launch shapes: no counterpart in the reference.  Times,
+ * with HIP events on the plan's own grid and nrhs right-hand sides, the
+ * FP32 stencil's streaming shapes (rows per run x right-hand sides per group)
+ * and the multigrid grid-level workgroup target, and keeps the fastest in the
+ * plan (later launches use them).  Call after the medium and the multigrid
+ * are set; allocates two scratch fields for the duration of the call. */
+typedef struct {
+    int32_t stencil_rows;      /* rows per wavefront run of the FP32 stencil */
+    int32_t stencil_rhs;       /* right-hand sides per group                 */
+    int64_t mg_workgroups;     /* workgroups a multigrid launch aims for     */
+    double stencil_us;         /* the chosen stencil's time per apply        */
+    double vcycle_us;          /* the chosen V-cycle's time                  */
+} helm_tuning;
+int32_t helm_autotune(helm_plan* plan, int32_t nrhs, helm_tuning* result, void* stream);
 
 /* ---- in-solve kernel timing: no counterpart in the reference (its path
  * has no device kernels); for benchmarks that need one kernel's duration

@@ -39,12 +39,17 @@ CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), c_int32)
 class HelmSolveOpts(ctypes.Structure):
     _fields_ = [("method", c_int32), ("precond", c_int32), ("restart", c_int32), ("max_iterations", c_int32),
                 ("tolerance", c_double), ("adjoint", c_int32), ("use_x0", c_int32),
-                ("callback", CALLBACK), ("user", c_void_p)]
+                ("callback", CALLBACK), ("user", c_void_p), ("use_graph", c_int32)]
 
 
 class HelmSolveInfo(ctypes.Structure):
     _fields_ = [("reason", c_int32), ("iterations", c_int32), ("applications", c_int32), ("cycles", c_int32),
                 ("relative_residual", c_double)]
+
+
+class HelmTuning(ctypes.Structure):
+    _fields_ = [("stencil_rows", c_int32), ("stencil_rhs", c_int32), ("mg_workgroups", ctypes.c_int64),
+                ("stencil_us", c_double), ("vcycle_us", c_double)]
 
 
 class HelmKernelStat(ctypes.Structure):
@@ -86,6 +91,7 @@ SIGNATURES = {
     "helm_set_medium_spectral": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HelmSpectralMedium), c_void_p]),
     "helm_spectral_incident": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "helm_spectral_rhs": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_autotune": (c_int32, [c_void_p, c_int32, POINTER(HelmTuning), c_void_p]),
     "helm_profile": (c_int32, [c_void_p, c_int32]),
     "helm_profile_read": (c_int32, [c_void_p, POINTER(HelmKernelStat), c_int32, POINTER(c_int32)]),
     "helm_warm_start": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),

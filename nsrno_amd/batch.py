@@ -37,6 +37,7 @@ class FrequencyCase:
     seconds: float = 0.0
     traces: np.ndarray | None = None
     krylov_bytes: float = 0.0
+    tuning: dict | None = None
 
 
 class PhysicalMedium:
@@ -82,7 +83,7 @@ def _mg_levels(n, coarsest=40):
     return levels
 
 
-def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None):
+def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False):
     """One frequency: all angles as one batched multigrid-FGMRES solve."""
     p = medium.pml
     n = case.interior + 2 * p
@@ -97,8 +98,10 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None):
     ports = Ports(weights)
     plan.set_medium_physical(torch.from_numpy(c).cuda(), torch.from_numpy(rho).cuda(), None, rho0=work.rho0)
     plan.setup_multigrid(levels=_mg_levels(n), shift=0.5, pre=2, post=2, coarse_sweeps=10)
+    if autotune:
+        case.tuning = plan.autotune(len(angles))
     rhs = plan.inject(ports, torch.from_numpy(amp).cuda())
-    x, info = plan.solve(rhs, tolerance=tol, restart=restart, max_iterations=6000, precond="mg")
+    x, info = plan.solve(rhs, tolerance=tol, restart=restart, max_iterations=6000, precond="mg", graph=graph)
     traces = plan.sample(ports, x)
     if stream is not None:
         stream.synchronize()
@@ -117,7 +120,7 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None):
     return case
 
 
-def run_sweep(frequencies, angles, tol=1e-6, restart=20, workers=4, seed=0, log=None):
+def run_sweep(frequencies, angles, tol=1e-6, restart=20, workers=4, seed=0, log=None, autotune=True, graph=False):
     """Solve every frequency (largest grids first) on `workers` concurrent
     streams.  Returns (cases in input order, wall seconds)."""
     medium = PhysicalMedium(seed=seed)
@@ -137,7 +140,7 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=20, workers=4, seed=0, log=
                 except queue.Empty:
                     return
                 try:
-                    solve_case(medium, cases[k], angles, tol, restart, stream=stream)
+                    solve_case(medium, cases[k], angles, tol, restart, stream=stream, autotune=autotune, graph=graph)
                     if log:
                         c = cases[k]
                         log(f"{c.frequency_mhz:.3f} MHz {c.interior}^2 steps {max(c.iterations)} "

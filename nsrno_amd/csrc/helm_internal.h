@@ -90,6 +90,8 @@ struct Level {
     double *ax64 = nullptr, *az64 = nullptr;
     int sp_cells = 0;      // sponge width
     double sp_amp = 0.0;   // sponge peak
+    // streaming-stencil launch shape chosen by helm_autotune (0: default)
+    int st_tz = 0, st_rb = 0;
 };
 
 struct Workspace {
@@ -123,6 +125,10 @@ struct Workspace {
            *bshadow = nullptr, *br = nullptr;
     double2* bsc = nullptr;   // nrhs * 8 scalars
     size_t bi_cap = 0;        // elements the BiCGSTAB vectors hold
+    // captured restart cycle (helm_solve_opts.use_graph) and what it was captured for
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gkey[8] = {};
+    hipStream_t gstream = nullptr;  // capture stream (capture is not allowed on the null stream)
     // host mirrors
     std::vector<double> h_rel;
     std::vector<int> h_active;
@@ -179,6 +185,8 @@ struct helm_plan {
     bool mg_views_valid = false;
     helm::Workspace ws;
     helm::Profiler* prof = nullptr;   // kernel timing inside solves (helm_profile)
+    uint64_t generation = 0;          // bumped whenever buffers a captured cycle uses are reallocated
+    int64_t mg_wg_target = 0;         // multigrid workgroup target chosen by helm_autotune (0: default)
     helm_spectral_medium spectral{};  // geometry of the last helm_set_medium_spectral
     double2* sp_buf = nullptr;        // incident-field scratch: 4 * nx c128
 };
@@ -212,6 +220,9 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
                        helm_solve_info* info, hipStream_t s);
 int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint,
                       hipStream_t s);
+// launch-shape tuning (helm_stencil.hip, helm_mg.hip)
+int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
+int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 // learned correction z += N(v) (helm_net.hip)
 int32_t net_correction(helm_plan* p, struct helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s);
 }  // namespace helm

@@ -31,6 +31,7 @@
 // sums (one workgroup per 4096-node chunk, per vector group, per RHS) then a
 // fixed-order sum over chunks; no atomics, so results are bit reproducible.
 #include <cmath>
+#include <cstring>
 
 #include <hip/hip_ext.h>
 
@@ -502,6 +503,7 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     if (w.V && w.nrhs >= nrhs && w.restart >= restart && w.n == n) return HELM_OK;
     nrhs = std::max(nrhs, w.nrhs);
     restart = std::max(restart, w.restart);
+    p->generation++;  // captured cycles refer to the old buffers
     const int nch = nchunks(n);
     const int kmax = restart + 1;
     w.nrhs = nrhs;
@@ -628,6 +630,39 @@ int32_t warm_start(helm_plan* p, int pc, const double2* b, double2* x, int nrhs,
     return HELM_OK;
 }
 
+// Capture (when the key changed) and launch the restart cycle as a HIP graph.
+// Capture runs on the workspace's own stream (the null stream cannot be
+// captured); the graph is launched on the caller's stream.
+template <typename F>
+static int32_t cycle_graph(Workspace& W, const uint64_t key[8], F&& enqueue, int m, hipStream_t s) {
+    if (!W.gexec || std::memcmp(W.gkey, key, sizeof(W.gkey)) != 0) {
+        if (W.gexec) {
+            (void)hipGraphExecDestroy(W.gexec);
+            W.gexec = nullptr;
+        }
+        if (!W.gstream) HELM_HIP(hipStreamCreateWithFlags(&W.gstream, hipStreamNonBlocking));
+        HELM_HIP(hipStreamBeginCapture(W.gstream, hipStreamCaptureModeThreadLocal));
+        const int32_t st = enqueue(m, W.gstream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(W.gstream, &g);
+        if (st != HELM_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HELM_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&W.gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            W.gexec = nullptr;
+            set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            return HELM_ERR_HIP;
+        }
+        std::memcpy(W.gkey, key, sizeof(W.gkey));
+    }
+    HELM_HIP(hipGraphLaunch(W.gexec, s));
+    return HELM_OK;
+}
+
 // ---- host driver -----------------------------------------------------------
 int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const helm_solve_opts* o,
                     helm_solve_info* info, hipStream_t s) {
@@ -692,59 +727,71 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             break;
         }
         cycles++;
-        hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, s, n, W.r64, W.V, W.inv, W.cyc);
         const int msteps = std::min(m, o->max_iterations - steps);
-        for (int j = 0; j < msteps; ++j) {
-            float2* vj = W.V + (int64_t)j * vstride;
-            // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
-            float2* zj = W.Z + (int64_t)j * vstride;
-            if (pc != HELM_PC_NONE) {
-                ev = pf ? prof_begin(pf, PK_PRECOND, s) : nullptr;
-                HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, s));
-                if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, s);
-            } else {
-                HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, s));
+        // one restart cycle: V_0 = r/beta, msteps flexible DCGS2 steps, x += Z y
+        auto enqueue_cycle = [&](int nsteps, hipStream_t cs) -> int32_t {
+            hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
+            for (int j = 0; j < nsteps; ++j) {
+                float2* vj = W.V + (int64_t)j * vstride;
+                // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
+                float2* zj = W.Z + (int64_t)j * vstride;
+                hipEvent_t ev;
+                if (pc != HELM_PC_NONE) {
+                    ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
+                    HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));
+                    if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, cs);
+                } else {
+                    HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, cs));
+                }
+                ev = pf ? prof_begin(pf, PK_APPLY, cs) : nullptr;
+                HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, cs));
+                if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, cs);
+                hipEvent_t ea, eb;
+                if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
+                    hipExtLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs, nchd, kmax, j,
+                                          (const float2*)W.V, (const float2*)W.w, (const int*)W.cyc, W.partial,
+                                          pd_part);
+                    prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
+                } else {
+                    hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, cs, n, nrhs, nchd, kmax, j, W.V, W.w,
+                                       W.cyc, W.partial, pd_part);
+                }
+                hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, cs, nchd, mm, kmax, j, o->tolerance,
+                                   W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc, W.used, W.coef);
+                if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
+                    hipExtLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs, nch, kmax, j,
+                                          W.V, W.w, (const double2*)W.coef, (const int*)W.cyc, pw2);
+                    prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
+                } else {
+                    hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, kmax, j, W.V, W.w,
+                                       W.coef, W.cyc, pw2);
+                }
+                hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, cs, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
+                                   W.g, W.est, W.inv, W.cyc, W.used);
+                if (j + 1 < nsteps)
+                    hipLaunchKernelGGL(k_scale32, gvec, blk, 0, cs, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
+                                       W.cyc);
+                else
+                    hipLaunchKernelGGL(k_dcgs_finish, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, j, W.H,
+                                       W.cs, W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
+                HELM_HIP(hipGetLastError());
             }
-            ev = pf ? prof_begin(pf, PK_APPLY, s) : nullptr;
-            HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, s));
-            if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, s);
-            applications++;
-            hipEvent_t ea, eb;
-            if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
-                hipExtLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, ea, eb, 0, n, nrhs, nchd, kmax, j,
-                                      (const float2*)W.V, (const float2*)W.w, (const int*)W.cyc, W.partial, pd_part);
-                prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
-            } else {
-                hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd, kmax, j, W.V, W.w, W.cyc,
-                                   W.partial, pd_part);
-            }
-            hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, s, nchd, mm, kmax, j, o->tolerance, W.partial,
-                               pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc,
-                               W.used, W.coef);
-            if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
-                hipExtLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, ea, eb, 0, n, nrhs, nch, kmax, j,
-                                      W.V, W.w, (const double2*)W.coef, (const int*)W.cyc, pw2);
-                prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
-            } else {
-                hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, kmax, j, W.V, W.w, W.coef,
-                                   W.cyc, pw2);
-            }
-            hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, s, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
-                               W.g, W.est, W.inv, W.cyc, W.used);
-            if (j + 1 < msteps)
-                hipLaunchKernelGGL(k_scale32, gvec, blk, 0, s, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
-                                   W.cyc);
-            else
-                hipLaunchKernelGGL(k_dcgs_finish, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, mm, j, W.H, W.cs,
-                                   W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
+            hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, kmax, W.H, W.g, W.used,
+                               W.coef);
+            hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, cs, n, nrhs, kmax, W.Z, W.coef, W.used, x);
             HELM_HIP(hipGetLastError());
-            steps++;
+            return HELM_OK;
+        };
+        // full cycles after the first (which performs any lazy allocation) replay a captured graph
+        if (o->use_graph && !pf && msteps == m && cycles > 1) {
+            const uint64_t key[8] = {(uint64_t)m, (uint64_t)nrhs, (uint64_t)adj, (uint64_t)pc, (uint64_t)(uintptr_t)x,
+                                     (uint64_t)(uintptr_t)p->net, p->generation, (uint64_t)(o->tolerance * 1e18)};
+            HELM_TRY(cycle_graph(W, key, enqueue_cycle, m, s));
+        } else {
+            HELM_TRY(enqueue_cycle(msteps, s));
         }
-        hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, mm, kmax, W.H, W.g, W.used,
-                           W.coef);
-        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, s, n, nrhs, kmax, W.Z, W.coef,
-                           W.used, x);
-        HELM_HIP(hipGetLastError());
+        steps += msteps;
+        applications += msteps;
     }
     for (int r = 0; r < nrhs; ++r) {
         info[r].reason = W.h_active[r] ? HELM_ITERATION_BUDGET : HELM_CONVERGED;

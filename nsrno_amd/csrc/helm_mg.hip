@@ -751,8 +751,8 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
 }
 
 // right-hand sides per workgroup so that a level launches >= wg_target() workgroups
-static int rhs_group(int64_t blocks_xy, int nrhs) {
-    const int64_t target = wg_target();
+static int rhs_group(const helm_plan* p, int64_t blocks_xy, int nrhs) {
+    const int64_t target = p->mg_wg_target ? p->mg_wg_target : wg_target();
     const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (target + blocks_xy - 1) / blocks_xy));
     return (int)((nrhs + groups - 1) / groups);
 }
@@ -762,7 +762,7 @@ static int32_t jacobi(helm_plan* p, int l, double shift, bool adj, const float2*
     const Level& L = p->levels[l];  // conj(M) uses conj(w): the same contraction
     if (u != nullptr && L.wid) {  // sweep from a start vector: LDS-tiled kernel
         const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
-        const int rb = rhs_group(tiles, nrhs);
+        const int rb = rhs_group(p, tiles, nrhs);
         const dim3 grid((L.nx + MT_TX - 1) / MT_TX, (L.nz + MT_TZ - 1) / MT_TZ, (nrhs + rb - 1) / rb);
         hipLaunchKernelGGL(k_mgt_jacobi, grid, dim3(64, 4), 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32, L.isxe32,
                            L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0), (float)(0.5 / (L.h_m * L.h_m)),
@@ -771,7 +771,7 @@ static int32_t jacobi(helm_plan* p, int l, double shift, bool adj, const float2*
         return HELM_OK;
     }
     const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
-    const int rb = rhs_group(bxy, nrhs);
+    const int rb = rhs_group(p, bxy, nrhs);
     dim3 block(64, 4), grid((L.nx + 63) / 64, (L.nz + 3) / 4, (nrhs + rb - 1) / rb);
     const double k0sq = p->k0 * p->k0;
     hipLaunchKernelGGL(k_jacobi, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.sx32, L.sz32, L.isxe32, L.isze32,
@@ -818,7 +818,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     const int sweeps = coarsest ? c.coarse : c.pre;
     const float k0sq = (float)(p->k0 * p->k0), hh = (float)(0.5 / (L.h_m * L.h_m)), sh = (float)c.shift;
     const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
-    const int rb = rhs_group(tiles, nrhs);
+    const int rb = rhs_group(p, tiles, nrhs);
     const dim3 blk(64, 4), grd((L.nx + MT_TX - 1) / MT_TX, (L.nz + MT_TZ - 1) / MT_TZ, (nrhs + rb - 1) / rb);
     // pre-smoothing from a zero guess, ending in u
     if (sweeps == 0) {
@@ -845,7 +845,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     // coarse right-hand side R (f - M u), the fine residual kept in LDS
     {
         const int64_t bc = (int64_t)((C.nx + RR_TX - 1) / RR_TX) * ((C.nz + RR_TZ - 1) / RR_TZ);
-        const int rbc = rhs_group(bc, nrhs);
+        const int rbc = rhs_group(p, bc, nrhs);
         const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
         hipLaunchKernelGGL(k_mgt_restrict_res, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                            L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, u, f, C.mg_f);
@@ -883,8 +883,50 @@ int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoi
     if (grew || !p->mg_views_valid) {
         HELM_TRY(tail_views(p));
         p->mg_views_valid = true;
+        p->generation++;
     }
     return vcycle(p, 0, v, z, nrhs, adjoint, s);
+}
+
+// Time one V-cycle for each workgroup target and keep the fastest.
+int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
+    *best_us = 0.0;
+    if (p->mg.levels == 0) return HELM_OK;
+    const size_t bytes = (size_t)nrhs * p->levels[0].n * sizeof(float2);
+    float2 *v = nullptr, *z = nullptr;
+    HELM_HIP(hipMalloc((void**)&v, bytes));
+    if (hipMalloc((void**)&z, bytes) != hipSuccess) {
+        (void)hipFree(v);
+        set_error("helm_autotune: scratch allocation failed");
+        return HELM_ERR_ALLOC;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int32_t st = hipMemsetAsync(v, 0, bytes, s) == hipSuccess ? HELM_OK : HELM_ERR_HIP;
+    const int64_t targets[3] = {1024, 2048, 4096};
+    double best = 1e30;
+    int64_t bt = 2048;
+    for (int a = 0; a < 3 && st == HELM_OK; ++a) {
+        p->mg_wg_target = targets[a];
+        st = mg_vcycle(p, v, z, nrhs, false, s);
+        (void)hipEventRecord(e0, s);
+        for (int w = 0; w < 3 && st == HELM_OK; ++w) st = mg_vcycle(p, v, z, nrhs, false, s);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (st == HELM_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            ms / 3.0 < best) {
+            best = ms / 3.0;
+            bt = targets[a];
+        }
+    }
+    p->mg_wg_target = bt;
+    *best_us = best * 1e3;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(v);
+    (void)hipFree(z);
+    return st;
 }
 
 int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
@@ -933,6 +975,7 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     }
     p->mg.levels = (int)p->levels.size();
     p->mg_views_valid = false;
+    p->generation++;
     {
         int t = 0;
         while (t < (int)p->levels.size() && p->levels[t].n > tail_nodes()) ++t;

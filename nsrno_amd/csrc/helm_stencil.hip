@@ -153,7 +153,7 @@ static int stream_rb() {
 template <int RB>
 static void launch_stream(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                           const float2* b, float shift, hipStream_t s) {
-    const int tz = stream_tz();
+    const int tz = L.st_tz ? L.st_tz : stream_tz();
     const int runs = (L.nz + tz - 1) / tz;
     const dim3 block(64, 4), grid((L.nx + 63) / 64, (runs + 3) / 4, (nrhs + RB - 1) / RB);
     const float hh = (float)(0.5 / (L.h_m * L.h_m));
@@ -191,7 +191,7 @@ int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* o
 int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                       const float2* b, float shift, hipStream_t s) {
     if (L.nx >= 64) {  // streaming kernel (a strip needs a full wavefront of columns to pay off)
-        switch (stream_rb()) {
+        switch (L.st_rb ? L.st_rb : stream_rb()) {
             case 1: launch_stream<1>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
             case 2: launch_stream<2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
             default: launch_stream<4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
@@ -266,6 +266,54 @@ __global__ void k_sample(int n_ports, int64_t n, int nrhs, const int* __restrict
         acc.y += w_port[t] * (double)v.y;
     }
     traces[(int64_t)r * n_ports + p] = acc;
+}
+
+// Time every streaming shape on the plan's grid and keep the fastest.
+int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
+    Level& L = p->levels[0];
+    *best_us = 0.0;
+    if (L.op != HELM_OP_FLUX5 || L.nx < 64) return HELM_OK;  // the streaming kernel is not used there
+    const size_t bytes = (size_t)nrhs * L.n * sizeof(float2);
+    float2 *u = nullptr, *out = nullptr;
+    HELM_HIP(hipMalloc((void**)&u, bytes));
+    if (hipMalloc((void**)&out, bytes) != hipSuccess) {
+        (void)hipFree(u);
+        set_error("helm_autotune: scratch allocation failed");
+        return HELM_ERR_ALLOC;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int32_t st = HELM_OK;
+    double best = 1e30;
+    int btz = 8, brb = 4;
+    if (hipMemsetAsync(u, 0, bytes, s) != hipSuccess) st = HELM_ERR_HIP;
+    const int tzs[3] = {4, 8, 16}, rbs[3] = {1, 2, 4};
+    const double k0sq = p->k0 * p->k0;
+    for (int a = 0; a < 3 && st == HELM_OK; ++a)
+        for (int b = 0; b < 3 && st == HELM_OK; ++b) {
+            L.st_tz = tzs[a];
+            L.st_rb = rbs[b];
+            for (int w = 0; w < 2 && st == HELM_OK; ++w) st = launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+            (void)hipEventRecord(e0, s);
+            for (int w = 0; w < 5 && st == HELM_OK; ++w) st = launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+            (void)hipEventRecord(e1, s);
+            float ms = 0.f;
+            if (st == HELM_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+                ms / 5.0 < best) {
+                best = ms / 5.0;
+                btz = tzs[a];
+                brb = rbs[b];
+            }
+        }
+    L.st_tz = btz;
+    L.st_rb = brb;
+    *best_us = best * 1e3;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(u);
+    (void)hipFree(out);
+    return st;
 }
 
 }  // namespace helm
@@ -396,5 +444,26 @@ extern "C" int32_t helm_material_sensitivity(helm_plan* p, const void* u, const 
                        L.isxe64, L.isze64, p->k0 * p->k0, 0.5 / (L.h_m * L.h_m), (const double2*)u,
                        (const double2*)lam, (double2*)s_eta, (double2*)s_kap);
     HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_autotune(helm_plan* p, int32_t nrhs, helm_tuning* result, void* stream) {
+    HELM_CHECK(p, HELM_ERR_ARG, "helm_autotune: null plan");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_autotune: set the medium first");
+    HELM_CHECK(nrhs >= 1 && nrhs <= 4096 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_autotune: nrhs out of range");
+    hipStream_t s = (hipStream_t)stream;
+    double st_us = 0.0, mg_us = 0.0;
+    HELM_TRY(tune_stencil(p, nrhs, s, &st_us));
+    HELM_TRY(tune_mg(p, nrhs, s, &mg_us));
+    p->generation++;  // captured cycles hold the old launch shapes
+    if (result) {
+        const Level& L = p->levels[0];
+        result->stencil_rows = L.st_tz;
+        result->stencil_rhs = L.st_rb;
+        result->mg_workgroups = p->mg_wg_target;
+        result->stencil_us = st_us;
+        result->vcycle_us = mg_us;
+    }
     return HELM_OK;
 }

@@ -181,6 +181,15 @@ class HelmholtzPlan:
         return z
 
     # ------------------------------------------------------------ kernel timing
+    def autotune(self, nrhs=None):
+        """Pick the stencil and multigrid launch shapes by timing them on this
+        plan (helm_autotune); returns the choice and the timings."""
+        t = _lib.HelmTuning()
+        n = self.nrhs_capacity if nrhs is None else int(nrhs)
+        check(lib().helm_autotune(self._h, n, ctypes.byref(t), _stream()), "helm_autotune")
+        return dict(stencil_rows=t.stencil_rows, stencil_rhs=t.stencil_rhs, mg_workgroups=t.mg_workgroups,
+                    stencil_us=t.stencil_us, vcycle_us=t.vcycle_us)
+
     PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual")
 
     def profile(self, kernels=None, enable=True):
@@ -222,11 +231,12 @@ class HelmholtzPlan:
         return x
 
     def solve(self, rhs, x0=None, *, tolerance=1e-6, restart=None, max_iterations=3000, precond="none",
-              method="gmres", adjoint=False, callback=None):
+              method="gmres", adjoint=False, callback=None, graph=False):
         """Krylov solve of A x = rhs (A^H x = rhs) for all right-hand sides.
 
         rhs: complex128 [R, nz, nx] (converted if needed).  Returns
-        (x complex128 [R, nz, nx], [SolveInfo] * R).
+        (x complex128 [R, nz, nx], [SolveInfo] * R).  graph: replay full restart
+        cycles after the first as one captured HIP graph (GMRES).
         """
         rhs = torch.as_tensor(rhs, device=self.device).to(torch.complex128).contiguous()
         nrhs = self._check_field(rhs)
@@ -246,7 +256,7 @@ class HelmholtzPlan:
 
         cfun = _lib.CALLBACK(_cb)
         opts = _lib.HelmSolveOpts(meth, pc, restart, int(max_iterations), float(tolerance), int(bool(adjoint)),
-                                  int(x0 is not None), cfun, None)
+                                  int(x0 is not None), cfun, None, int(bool(graph)))
         infos = (_lib.HelmSolveInfo * nrhs)()
         check(lib().helm_solve(self._h, _ptr(rhs), _ptr(x), nrhs, ctypes.byref(opts), infos, _stream()),
               "helm_solve")

@@ -131,3 +131,30 @@ def test_vcycle_matches_oracle_restatement():
     for _ in range(30):
         xx = xx + mg2.vcycle((b - shifted @ xx).reshape(1, *grid.shape)).ravel()
     assert rel(xx, exact) < 1e-3
+
+
+def test_graph_replayed_cycles_match_eager():
+    """Restart cycles replayed as a captured HIP graph give bit-identical
+    solutions to the eagerly launched cycles, replay after replay, with other
+    work and synchronisations in between."""
+    from nsrno_amd.helmholtz import HelmholtzPlan
+    grid = O.grid_with_pml(120, 100, 12, 0.05)
+    nz, nx = grid.shape
+    rng = np.random.default_rng(3)
+    eta = 1 + 0.1 * rng.random((nz, nx))
+    kap = (1 + 0.2 * rng.random((nz, nx))).astype(complex)
+    plan = HelmholtzPlan(nz, nx, grid.h_mm, 2.0, grid.pml_mm, x0_mm=grid.x_mm[0], z0_mm=grid.z_mm[0], nrhs=3,
+                         restart=8)
+    plan.set_medium(eta, kap)
+    plan.setup_multigrid(levels=3, shift=0.5, pre=2, post=2, coarse_sweeps=10)
+    b = torch.from_numpy(rng.normal(size=(3, nz, nx)) + 1j * rng.normal(size=(3, nz, nx))).cuda()
+    x_e, i_e = plan.solve(b, tolerance=1e-9, restart=8, max_iterations=800, precond="mg")
+    assert all(i.reason == "converged" for i in i_e) and i_e[0].cycles > 3
+    for rep in range(3):
+        noise = plan.apply(torch.randn(3, nz, nx, dtype=torch.complex64, device="cuda"))  # eager work between
+        torch.cuda.synchronize()
+        x_g, i_g = plan.solve(b, tolerance=1e-9, restart=8, max_iterations=800, precond="mg", graph=True)
+        torch.cuda.synchronize()
+        assert [i.iterations for i in i_g] == [i.iterations for i in i_e], rep
+        assert torch.equal(x_g, x_e), rep
+        del noise

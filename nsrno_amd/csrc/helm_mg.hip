@@ -397,6 +397,53 @@ __device__ __forceinline__ float2 tile_mv(const Coef<float2>& c, const float2* s
     return acc;
 }
 
+// Pipelining shared by the tile kernels: the loads of right-hand side r+1
+// (its staged halo values and the f values of the thread's own nodes) are
+// issued into registers before RHS r is computed from LDS, and the first
+// RHS's loads before the stencil coefficients are formed, so each workgroup
+// waits for about one memory round trip instead of two per RHS.
+struct TileIdx {  // node indices within one field (a level has < 2^31 nodes)
+    int st[MT_SPT];   // each staged node (-1: ghost or not staged)
+    int own[MT_RPT];  // each own node (-1: outside the grid)
+};
+
+__device__ __forceinline__ void tile_idx(int z0, int x0, int nz, int nx, TileIdx& t) {
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) {
+        const int e = tid + 256 * q;
+        t.st[q] = -1;
+        if (e < MT_NS) {
+            const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) t.st[q] = gi * nx + gj;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MT_RPT; ++k) {
+        const int i = z0 + threadIdx.y + 4 * k, j = x0 + threadIdx.x;
+        t.own[k] = (i < nz && j < nx) ? i * nx + j : -1;
+    }
+}
+
+__device__ __forceinline__ void tile_fetch(const TileIdx& t, const float2* __restrict__ src,
+                                           const float2* __restrict__ fr, float2 (&st)[MT_SPT],
+                                           float2 (&fv)[MT_RPT]) {
+    const float2 zero = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) st[q] = t.st[q] >= 0 ? src[t.st[q]] : zero;
+#pragma unroll
+    for (int k = 0; k < MT_RPT; ++k) fv[k] = t.own[k] >= 0 ? fr[t.own[k]] : zero;
+}
+
+__device__ __forceinline__ void tile_store_lds(const float2 (&st)[MT_SPT], float2* sv) {
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) {
+        const int e = tid + 256 * q;
+        if (e < MT_NS) sv[e] = st[q];
+    }
+}
+
 // one damped-Jacobi sweep out = u + w (f - M u)
 __global__ __launch_bounds__(256) void k_mgt_jacobi(int nz, int nx, int64_t n, int nrhs, int rb,
                                                     const float2* __restrict__ sx, const float2* __restrict__ sz,
@@ -406,31 +453,29 @@ __global__ __launch_bounds__(256) void k_mgt_jacobi(int nz, int nx, int64_t n, i
                                                     const float2* __restrict__ wid, const float2* __restrict__ u,
                                                     const float2* __restrict__ f, float2* __restrict__ out) {
     __shared__ float2 sv[MT_NS];
-    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    TileIdx ix;
+    tile_idx(z0, x0, nz, nx, ix);
+    float2 st[MT_SPT], fv[MT_RPT];
+    tile_fetch(ix, u + (int64_t)r0 * n, f + (int64_t)r0 * n, st, fv);
     TileCoef t;
     tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, wid, t);
-    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
     for (int r = r0; r < r1; ++r) {
-        const float2* ur = u + (int64_t)r * n;
-        const float2* fr = f + (int64_t)r * n;
-        __syncthreads();
+        __syncthreads();  // the previous RHS is done with sv
+        tile_store_lds(st, sv);
+        float2 fc[MT_RPT];
 #pragma unroll
-        for (int q = 0; q < MT_SPT; ++q) {
-            const int e = tid + 256 * q;
-            if (e < MT_NS) {
-                const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
-                sv[e] = (gi >= 0 && gi < nz && gj >= 0 && gj < nx) ? ur[(int64_t)gi * nx + gj] : make_float2(0.f, 0.f);
-            }
-        }
+        for (int k = 0; k < MT_RPT; ++k) fc[k] = fv[k];
+        if (r + 1 < r1) tile_fetch(ix, u + (int64_t)(r + 1) * n, f + (int64_t)(r + 1) * n, st, fv);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < MT_RPT; ++k) {
             if (!t.in[k]) continue;
             const int row = threadIdx.y + 4 * k, col = threadIdx.x;
-            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
             float2 v = sv[st_idx(row, col)];
-            cfma(v, t.w[k], csub(fr[o], tile_mv(t.c[k], sv, row, col)));
-            out[(int64_t)r * n + o] = v;
+            cfma(v, t.w[k], csub(fc[k], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + ix.own[k]] = v;
         }
     }
 }
@@ -444,43 +489,35 @@ __global__ __launch_bounds__(256) void k_mgt_pre2(int nz, int nx, int64_t n, int
                                                   const float2* __restrict__ wid, const float2* __restrict__ f,
                                                   float2* __restrict__ out) {
     __shared__ float2 sv[MT_NS];
-    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX;
     const bool cj = conj != 0;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    TileIdx ix;
+    tile_idx(z0, x0, nz, nx, ix);
+    float2 st[MT_SPT], fv[MT_RPT];
+    tile_fetch(ix, f + (int64_t)r0 * n, f + (int64_t)r0 * n, st, fv);
+    float2 ws[MT_SPT];  // w/dg at the staged nodes (0 outside the grid)
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) ws[q] = ix.st[q] >= 0 ? cc(wid[ix.st[q]], cj) : make_float2(0.f, 0.f);
     TileCoef t;
     tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, t);
-    float2 ws[MT_SPT];  // w/dg at the staged nodes (0 outside the grid)
-    int64_t gs[MT_SPT];
-#pragma unroll
-    for (int q = 0; q < MT_SPT; ++q) {
-        const int e = tid + 256 * q;
-        ws[q] = make_float2(0.f, 0.f);
-        gs[q] = -1;
-        if (e < MT_NS) {
-            const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
-            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
-                gs[q] = (int64_t)gi * nx + gj;
-                ws[q] = cc(wid[gs[q]], cj);
-            }
-        }
-    }
-    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
     for (int r = r0; r < r1; ++r) {
-        const float2* fr = f + (int64_t)r * n;
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < MT_SPT; ++q) {
-            const int e = tid + 256 * q;
-            if (e < MT_NS) sv[e] = gs[q] >= 0 ? cmul(ws[q], fr[gs[q]]) : make_float2(0.f, 0.f);
-        }
+        for (int q = 0; q < MT_SPT; ++q) st[q] = cmul(ws[q], st[q]);  // u1 = (w/dg) f
+        tile_store_lds(st, sv);
+        float2 fc[MT_RPT];
+#pragma unroll
+        for (int k = 0; k < MT_RPT; ++k) fc[k] = fv[k];
+        if (r + 1 < r1) tile_fetch(ix, f + (int64_t)(r + 1) * n, f + (int64_t)(r + 1) * n, st, fv);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < MT_RPT; ++k) {
             if (!t.in[k]) continue;
             const int row = threadIdx.y + 4 * k, col = threadIdx.x;
-            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
             float2 v = sv[st_idx(row, col)];
-            cfma(v, t.w[k], csub(fr[o], tile_mv(t.c[k], sv, row, col)));
-            out[(int64_t)r * n + o] = v;
+            cfma(v, t.w[k], csub(fc[k], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + ix.own[k]] = v;
         }
     }
 }
@@ -489,6 +526,7 @@ __global__ __launch_bounds__(256) void k_mgt_pre2(int nz, int nx, int64_t n, int
 // the corrected field v = u + P u_c is formed on the halo in LDS (bilinear
 // weights without branches), then one sweep; out must not alias u
 constexpr int MP_CH = MT_TZ / 2 + 3, MP_CW = MT_TX / 2 + 3, MP_NC = MP_CH * MP_CW;
+constexpr int MP_CPT = (MP_NC + 255) / 256;
 __global__ __launch_bounds__(256) void k_mgt_prolong_smooth(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
                                                             int rb, const float2* __restrict__ sx,
                                                             const float2* __restrict__ sz,
@@ -504,46 +542,68 @@ __global__ __launch_bounds__(256) void k_mgt_prolong_smooth(int nz, int nx, int6
     __shared__ float2 scv[MP_NC];
     const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
     const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
-    TileCoef t;
-    tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, wid, t);
     const int64_t ncn = (int64_t)nzc * nxc;
     const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
-    for (int r = r0; r < r1; ++r) {
-        const float2* ur = u + (int64_t)r * n;
-        const float2* ucr = uc + (int64_t)r * ncn;
-        __syncthreads();
-        for (int e = tid; e < MP_NC; e += 256) {
+    TileIdx ix;
+    tile_idx(z0, x0, nz, nx, ix);
+    int cix[MP_CPT];
+#pragma unroll
+    for (int q = 0; q < MP_CPT; ++q) {
+        const int e = tid + 256 * q;
+        cix[q] = -1;
+        if (e < MP_NC) {
             const int I = cz0 + e / MP_CW, J = cx0 + e % MP_CW;
-            scv[e] = (I >= 0 && I < nzc && J >= 0 && J < nxc) ? ucr[(int64_t)I * nxc + J] : make_float2(0.f, 0.f);
+            if (I >= 0 && I < nzc && J >= 0 && J < nxc) cix[q] = I * nxc + J;
+        }
+    }
+    float2 st[MT_SPT], fv[MT_RPT], cs[MP_CPT];
+    auto fetch = [&](int r) {
+        tile_fetch(ix, u + (int64_t)r * n, f + (int64_t)r * n, st, fv);
+#pragma unroll
+        for (int q = 0; q < MP_CPT; ++q) cs[q] = cix[q] >= 0 ? uc[(int64_t)r * ncn + cix[q]] : make_float2(0.f, 0.f);
+    };
+    fetch(r0);
+    TileCoef t;
+    tile_coef(z0, x0, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, wid, t);
+    for (int r = r0; r < r1; ++r) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MP_CPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < MP_NC) scv[e] = cs[q];
         }
         __syncthreads();
+        // v = u + P u_c on the staged halo
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) {
             const int e = tid + 256 * q;
             if (e < MT_NS) {
-                const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
                 float2 v = make_float2(0.f, 0.f);
-                if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
+                if (ix.st[q] >= 0) {
+                    const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
                     const int I = (gi >> 1) - cz0, J = (gj >> 1) - cx0;
                     const float wz1 = (gi & 1) ? 0.5f : 0.f, wz0 = 1.f - wz1;
                     const float wx1 = (gj & 1) ? 0.5f : 0.f, wx0 = 1.f - wx1;
                     const float2* c0 = scv + I * MP_CW + J;
-                    v = ur[(int64_t)gi * nx + gj];
+                    v = st[q];
                     v.x += wz0 * (wx0 * c0[0].x + wx1 * c0[1].x) + wz1 * (wx0 * c0[MP_CW].x + wx1 * c0[MP_CW + 1].x);
                     v.y += wz0 * (wx0 * c0[0].y + wx1 * c0[1].y) + wz1 * (wx0 * c0[MP_CW].y + wx1 * c0[MP_CW + 1].y);
                 }
                 sv[e] = v;
             }
         }
+        float2 fc[MT_RPT];
+#pragma unroll
+        for (int k = 0; k < MT_RPT; ++k) fc[k] = fv[k];
+        if (r + 1 < r1) fetch(r + 1);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < MT_RPT; ++k) {
             if (!t.in[k]) continue;
             const int row = threadIdx.y + 4 * k, col = threadIdx.x;
-            const int64_t o = (int64_t)(z0 + row) * nx + x0 + col;
             float2 v = sv[st_idx(row, col)];
-            cfma(v, t.w[k], csub(f[(int64_t)r * n + o], tile_mv(t.c[k], sv, row, col)));
-            out[(int64_t)r * n + o] = v;
+            cfma(v, t.w[k], csub(fc[k], tile_mv(t.c[k], sv, row, col)));
+            out[(int64_t)r * n + ix.own[k]] = v;
         }
     }
 }

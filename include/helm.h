@@ -71,6 +71,7 @@ extern "C" {
 
 #define HELM_OP_FLUX5 0    /* 5-point flux form, stretched PML (wave.py)   */
 #define HELM_OP_STAG8 1    /* 8th-order staggered, periodic + sponge       */
+#define HELM_OP_COND8 2    /* 8th-order staggered, split PML, masked box   */
 
 typedef struct helm_plan helm_plan;
 
@@ -86,7 +87,7 @@ typedef struct {
     double frequency_mhz;
     double c0_m_s;             /* k0 = omega/c0 and sigma_max reference      */
     double pml_strength;       /* sigma_max = pml_strength*c0/pml_width      */
-    int32_t op_kind;           /* HELM_OP_FLUX5 (0) or HELM_OP_STAG8         */
+    int32_t op_kind;           /* HELM_OP_FLUX5, HELM_OP_STAG8, HELM_OP_COND8 */
 } helm_desc;
 
 typedef struct {
@@ -259,6 +260,50 @@ int32_t helm_spectral_incident(helm_plan* plan, double sin_theta, double apertur
  * ax = az = 1 and kappa = 1 - i sponge (wave_operator.py:142-143,
  * wave_solver.py:53).  inc, rhs: c128 [nrhs][H][W]. */
 int32_t helm_spectral_rhs(helm_plan* plan, const void* inc, void* rhs, int32_t nrhs, void* stream);
+
+/* ---- conditioned split-PML operator: replaces FrequencyGeometry,
+ * ConditionedFrequencySystem (nsrno/conditioned_wave.py:12-92) and
+ * dense_wave.projected_operator / adjoint_operator / source_rhs
+ * (nsrno/dense_wave.py:59-82).  Plans of op_kind HELM_OP_COND8 (pml_mm = 0;
+ * nz, nx = the box H, W; h_mm = 1e3 dx; c0_m_s = 1540).  The plan holds the
+ * reference's operator itself, in its convention (D = exp(+i omega dt)):
+ *
+ *   A p = M [ scalar sum_axes ( left D-(F) + right D-(g F) ) + mass M p ] + (1 - M) p,
+ *   F = a_face sv D+(M p),  scalar = (D-1)/(D k0^2),  k0 = 2 sin(pi f dt)/(dt c0),
+ *
+ * with D+/D- the 8-tap staggered differences of the spectral operator above,
+ * M the native-medium mask and the split-PML factors sv, g, left, right of
+ * conditioned_wave.py:33-41,67-69.  helm_apply(adjoint = 0) is
+ * projected_operator, adjoint = 1 adjoint_operator; helm_solve(adjoint = 0)
+ * is solve_system's forward solve, adjoint = 1 its adjoint solve. */
+typedef struct {
+    int32_t h, w;              /* native medium size                         */
+    int32_t pz, px;            /* offset of the medium in the box (>= 8,
+                                  and >= 8 nodes of box beyond it)            */
+    int32_t pml;               /* split-PML width [cells]; also the rim that
+                                  is cropped and replicated (:46-49)          */
+    double dt_s;               /* native time step                           */
+    double rho0;               /* reference density (1000)                   */
+    double eta_per_unit_alpha[2];
+    double taus_s[2];
+} helm_conditioned_medium;
+
+/* c, rho, alpha: device float64 [h][w] native maps (alpha may be NULL: 0).
+ * Crops and replicates the pml rim, pads into the box (:45-49), forms the
+ * masked harmonic face means, the relaxation modulus and the mass (:56-66)
+ * and the per-column/row split-PML factors.  Rebuilds the multigrid
+ * preconditioner when one is set up: a 5-point shifted operator of the same
+ * coefficients whose stretch is taken from the split-PML factors, applied to
+ * the input scaled by k0^2 sx sz (identity outside the medium). */
+int32_t helm_set_medium_conditioned(helm_plan* plan, const double* c, const double* rho, const double* alpha,
+                                    const helm_conditioned_medium* medium, void* stream);
+/* Source term of dense_wave.source_rhs (:59-62): rhs[r] (c128 [nrhs][H][W],
+ * written everywhere) = line[r] (c128 [nrhs][w]) on box row `face`, times
+ * a c0^2 2 dt / (c_src dx (D - 1)), c_src = c at native (source_z, source_x)
+ * after cropping.  c, rho: the maps given to helm_set_medium_conditioned. */
+int32_t helm_conditioned_source(helm_plan* plan, const double* c, const double* rho, const void* line,
+                                int32_t face, int32_t source_z, int32_t source_x, void* rhs, int32_t nrhs,
+                                void* stream);
 
 /* ---- warm start: `steps` minimal-residual steps per right-hand side,
  * d = M^{-1} r, alpha = <A d, r>/|A d|^2, x += alpha d (the residual never

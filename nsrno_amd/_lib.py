@@ -23,7 +23,7 @@ class HelmDesc(ctypes.Structure):
                 ("op_kind", c_int32)]
 
 
-HELM_OP_FLUX5, HELM_OP_STAG8 = 0, 1
+HELM_OP_FLUX5, HELM_OP_STAG8, HELM_OP_COND8 = 0, 1, 2
 
 
 class HelmSpectralMedium(ctypes.Structure):
@@ -31,6 +31,12 @@ class HelmSpectralMedium(ctypes.Structure):
                 ("x0_m", c_double), ("z0_m", c_double), ("rho0", c_double), ("alpha_scale", c_double),
                 ("relaxation", c_int32), ("sponge_cells", c_int32), ("sponge_amplitude", c_double),
                 ("eta_per_unit_alpha", c_double * 2), ("taus_s", c_double * 2), ("relax_dt_s", c_double)]
+
+
+class HelmConditionedMedium(ctypes.Structure):
+    _fields_ = [("h", c_int32), ("w", c_int32), ("pz", c_int32), ("px", c_int32), ("pml", c_int32),
+                ("dt_s", c_double), ("rho0", c_double), ("eta_per_unit_alpha", c_double * 2),
+                ("taus_s", c_double * 2)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), c_int32)
@@ -95,6 +101,10 @@ SIGNATURES = {
     "helm_profile": (c_int32, [c_void_p, c_int32]),
     "helm_profile_read": (c_int32, [c_void_p, POINTER(HelmKernelStat), c_int32, POINTER(c_int32)]),
     "helm_warm_start": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),
+    "helm_set_medium_conditioned": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                              POINTER(HelmConditionedMedium), c_void_p]),
+    "helm_conditioned_source": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                          c_void_p, c_int32, c_void_p]),
 }
 
 _LIB = None

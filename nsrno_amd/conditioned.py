@@ -1,0 +1,205 @@
+"""Conditioned split-PML frequency system on the GPU (the dense path).
+
+Drop-in for nsrno/conditioned_wave.py:12-92 (FrequencyGeometry,
+ConditionedFrequencySystem) and the operator functions of
+nsrno/dense_wave.py:59-225 (source_rhs, channels, projected_operator,
+adjoint_operator, solve_system).  The operator is applied by the HIP 8-tap
+kernels of libhelm.so (op_kind HELM_OP_COND8) instead of eleven FFTs; solves
+run multigrid-preconditioned FGMRES or BiCGSTAB with FP32 vectors and FP64
+true-residual refinement.  There is no CPU path: a missing libhelm.so or GPU
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import _lib
+from ._lib import check, lib
+from .helmholtz import HelmholtzPlan, _ptr, _stream
+
+C0 = 1540.0
+RHO0 = 1000.0
+
+
+class FrequencyGeometry:
+    """FrequencyGeometry (conditioned_wave.py:12-50): the padded box, native
+    mask, receivers and source row.  Split-PML factors live in the plan."""
+
+    def __init__(self, spec, device="cuda"):
+        self.spec = spec
+        self.device = torch.device(device)
+        self.h, self.w = spec["medium_shape"][-2:]
+        self.dx = spec["dx_m"]
+        self.H = math.ceil((self.h + 64) / 32) * 32
+        self.W = math.ceil((self.w + 64) / 32) * 32
+        if "fft_shape" in spec:
+            self.H, self.W = spec["fft_shape"]
+            if self.H - self.h < 16 or self.W - self.w < 16:
+                raise ValueError("FFT padding must separate the8th-order finite-stencil support")
+        self.pz = (self.H - self.h) // 2
+        self.px = (self.W - self.w) // 2
+        self.pad = (self.px, self.W - self.w - self.px, self.pz, self.H - self.h - self.pz)
+        self.mask = F.pad(torch.ones(self.h, self.w, dtype=torch.float64, device=self.device), self.pad)
+        self.rx_cols = (torch.round((torch.arange(192, dtype=torch.float64) * .0002 - .0191 - spec["x0_m"])
+                                    / self.dx).long() + self.px).to(self.device)
+        self.face = round(-spec["z0_m"] / self.dx) + self.pz
+        self.dt = spec["native_dt_s"]
+        self.pml = spec["pml"]
+
+    def pad_field(self, p):
+        return F.pad(p, self.pad)
+
+    def native(self, p):
+        return p[..., self.pz:self.pz + self.h, self.px:self.px + self.w]
+
+    def coefficients(self, m):
+        """Crop the pml rim, replicate it back, replicate into the box (:45-49)."""
+        p = self.pml
+        m = F.pad(m[..., p:-p, p:-p][None], (p, p, p, p), mode="replicate")
+        return F.pad(m, self.pad, mode="replicate")[0]
+
+    def channels(self, p):
+        return p[self.face, self.rx_cols]
+
+
+def _mg_levels(n, coarsest=40):
+    levels = 1
+    while n > coarsest and levels < 8:
+        n = (n + 1) // 2
+        levels += 1
+    return levels
+
+
+class ConditionedFrequencySystem:
+    """ConditionedFrequencySystem (conditioned_wave.py:52-92) on a HIP plan.
+
+    ``system(p)`` is the projected operator M A(M p) + (1-M) p of
+    dense_wave.projected_operator; it equals the reference's ``__call__`` on
+    every field that vanishes outside the medium, which is what all of the
+    reference's call sites pass (pad_field outputs and masked fields,
+    dense_wave.py:56,253).  nrhs / restart size the plan's Krylov workspace.
+    """
+
+    def __init__(self, geometry, medium, alpha, frequency, *, nrhs=16, restart=24, multigrid=None):
+        g = geometry
+        if tuple(medium.shape) != (2, g.h, g.w):
+            raise ValueError("Expected native [2,z,x] medium")
+        self.g = g
+        self.frequency = float(frequency)
+        dev = g.device
+        medium = torch.as_tensor(medium, device=dev).to(torch.float64)
+        alpha = torch.as_tensor(alpha, device=dev).to(torch.float64)
+        self._c = medium[0].contiguous()
+        self._rho = medium[1].contiguous()
+        self._alpha = alpha.contiguous()
+        self.D = complex(math.cos(2 * math.pi * self.frequency * g.dt), math.sin(2 * math.pi * self.frequency * g.dt))
+        self.q = 1 / self.D
+        self.k0 = 2 * math.sin(math.pi * self.frequency * g.dt) / (g.dt * C0)
+        self.plan = HelmholtzPlan(g.H, g.W, g.dx * 1e3, self.frequency / 1e6, 0.0, c0_m_s=C0, nrhs=nrhs,
+                                  restart=restart, device=dev, op_kind=_lib.HELM_OP_COND8)
+        spec = g.spec
+        self._med = _lib.HelmConditionedMedium(g.h, g.w, g.pz, g.px, int(g.pml), float(g.dt), RHO0,
+                                               (ctypes.c_double * 2)(*spec["eta_per_unit_alpha"]),
+                                               (ctypes.c_double * 2)(*spec["taus_s"]))
+        check(lib().helm_set_medium_conditioned(self.plan._h, _ptr(self._c), _ptr(self._rho), _ptr(self._alpha),
+                                                ctypes.byref(self._med), _stream()), "helm_set_medium_conditioned")
+        mg = dict(levels=_mg_levels(max(g.H, g.W)), shift=0.5, pre=2, post=2, coarse_sweeps=10)
+        mg.update(multigrid or {})
+        self.plan.setup_multigrid(**mg)
+        box = g.coefficients(medium)
+        self.c, self.rho = box[0], box[1]
+        self.alpha = g.coefficients(alpha[None])[0]
+        self.a = RHO0 / self.rho
+
+    def close(self):
+        self.plan.close()
+
+    def __call__(self, p):
+        return self.plan.apply(p.contiguous())
+
+    def adjoint(self, p):
+        return self.plan.apply(p.contiguous(), adjoint=True)
+
+    def source(self, line):
+        """conditioned_wave.py:82-88: one line [w] -> [H, W]."""
+        return source_rhs(self, line[None])[0]
+
+
+def source_rhs(system, line):
+    """dense_wave.source_rhs (:59-62): lines [R, w] -> c128 [R, H, W]."""
+    g = system.g
+    line = torch.as_tensor(line, device=g.device).to(torch.complex128).contiguous()
+    if line.dim() != 2 or line.shape[1] != g.w:
+        raise ValueError(f"source lines must be [R, {g.w}]")
+    out = torch.empty((line.shape[0], g.H, g.W), dtype=torch.complex128, device=g.device)
+    zi, xi = g.spec["source_c_lookup_zx"]
+    check(lib().helm_conditioned_source(system.plan._h, _ptr(system._c), _ptr(system._rho), _ptr(line), int(g.face),
+                                        int(zi), int(xi), _ptr(out), int(line.shape[0]), _stream()),
+          "helm_conditioned_source")
+    return out
+
+
+def channels(g, p):
+    return p[..., g.face, g.rx_cols]
+
+
+def projected_operator(system, p):
+    """dense_wave.projected_operator (:66-68)."""
+    return system(p)
+
+
+def adjoint_operator(system, p):
+    """dense_wave.adjoint_operator (:70-82)."""
+    return system.adjoint(p)
+
+
+def scattered_rhs(system, reference_system, pref, line):
+    """The right-hand side DenseData.assemble forms (dense_wave.py:56):
+    source(system) - source(reference) - A pref + A_ref pref."""
+    pref = pref.to(torch.complex128).contiguous()
+    return (source_rhs(system, line) - source_rhs(reference_system, line) - system(pref)
+            + reference_system(pref))
+
+
+def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5, max_cycles=100, warm_steps=8,
+                 mixed=False, method="gmres", initial=None, restart=24):
+    """dense_wave.solve_system (:167-225) on the GPU.
+
+    rhs [R, H, W] (complex64 or complex128).  Every right-hand side runs its
+    own Krylov space to ``tolerance`` on its true FP64 residual, so the batch
+    residual the reference tests (all angles flattened into one vector,
+    dense_wave.py:88-89) is below ``tolerance`` as well.  The preconditioner
+    is the plan's multigrid V-cycle; the reference's learned network is not
+    accepted here (``preconditioner`` must be None, and ``warm_steps`` is then
+    unused, as in the reference).  ``mixed`` only changes the reported dtype:
+    the solve always runs FP32 vectors with FP64 refinement.  Returns
+    (value, info) with the reference's info keys.
+    """
+    if preconditioner is not None:
+        raise ValueError("conditioned solves take the multigrid preconditioner; pass preconditioner=None")
+    if method not in ("gmres", "bicgstab"):
+        raise ValueError(f"unknown method {method!r}")
+    start = time.perf_counter()
+    dtype = rhs.dtype
+    b = rhs.to(torch.complex128).contiguous()
+    R = b.shape[0] if b.dim() == 3 else 1
+    b = b.reshape(R, system.g.H, system.g.W)
+    x0 = None if initial is None else initial.to(torch.complex128).reshape(R, system.g.H, system.g.W)
+    x, infos = system.plan.solve(b, x0, tolerance=tolerance, restart=restart, max_iterations=restart * max_cycles,
+                                 precond="mg", method=method, adjoint=adjoint)
+    residual = system.plan.apply(x, adjoint=adjoint) - b
+    relative = float(residual.norm() / b.norm().clamp_min(1e-30))
+    individual = residual.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1).clamp_min(1e-30)
+    reason = "converged" if all(i.reason == "converged" for i in infos) and relative < tolerance else \
+        ("converged" if relative < tolerance else "iteration_budget")
+    info = {"reason": reason, "steps": max(i.iterations for i in infos), "residual": relative,
+            "max_angle_residual": float(individual.max()), "seconds": time.perf_counter() - start,
+            "dtype": "mixed FP32 inverse / FP64 residual" if mixed else str(dtype), "method": method,
+            "refinements": max(i.cycles for i in infos)}
+    return x.reshape(rhs.shape).to(dtype), info

@@ -1,0 +1,715 @@
+// Conditioned split-PML operator (op_kind HELM_OP_COND8): the masked,
+// projected frequency operator of the dense path, its analytic adjoint, its
+// medium set-up and its source term.
+//
+// Operator (nsrno/conditioned_wave.py:52-80 applied through
+// nsrno/dense_wave.py:66-68; the reference applies it with eleven FFTs; here
+// it is the equivalent real-space stencil, whose Fourier symbol is the
+// reference's D = 2i sum_m a_m sin((m+1/2) q) e^{iq/2} / h, on a box whose
+// padding (>= 16 nodes) keeps the periodic wrap out of the native domain):
+//   p_m = M p (M = 1 inside the native medium, 0 outside)
+//   Fx  = ax sv_x (D+x p_m)                          (faces j+1/2)
+//   (A p) = M [ cA_x D-x(Fx) + cB_x D-x((g_x - 1) Fx) + (same in z) + mass p_m ]
+//         + (1 - M) p
+// with, per column j (rows likewise), from the split-PML factors of the
+// reference (D = e^{i omega dt}, scalar = (D-1)/(D k0^2)):
+//   sv = (D-1)/(D-av), g = 1 + (1-av)/D, left = m/(D-ar), right = 1 + (1-m)/(D-ar),
+//   cA = scalar (left + right), cB = scalar right        (m = [ar != 1])
+// which is scalar [left D-(F) + right D-(g F)] rearranged.  The adjoint
+// (dense_wave.py:70-82):
+//   (A^H y) = M [ D-x( conj(ax sv_x) [D+x(conj(cA_x) y_m) + conj(g_x - 1) D+x(conj(cB_x) y_m)] )
+//               + (same in z) + conj(mass) y_m ] + (1 - M) y.
+// Away from the PML g = 1 exactly, so tiles whose faces all have g = 1 skip
+// the second divergence, and tiles whose columns (rows) share one cA skip the
+// per-node scaling of the adjoint's differences.
+//
+// Kernel layout as the spectral 8-tap kernel: a workgroup owns a TZ x 64
+// output tile, stages the masked field with a 7-node halo in LDS, forms the
+// face fluxes in LDS, then the divergences; coefficients are loaded once per
+// workgroup and reused for its group of right-hand sides.  Algorithmic bytes
+// per node per RHS: 8 (read) + 8 (write) for FP32, plus (4 + 4 + 8)/R for
+// ax, az, mass.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+#include "helm_internal.h"
+
+namespace helm {
+
+constexpr double CD0 = 1225.0 / 1024.0, CD1 = -245.0 / 3072.0, CD2 = 49.0 / 5120.0, CD3 = -5.0 / 7168.0;
+constexpr int C8_TX = 64;
+constexpr int C8_HL = 7;
+
+__device__ __forceinline__ int cwrap(int v, int m) {
+    if (v < 0 || v >= m) {
+        v %= m;
+        if (v < 0) v += m;
+    }
+    return v;
+}
+
+template <typename C, typename R>
+__device__ __forceinline__ C c8diff(R c0, R c1, R c2, R c3, C a0, C b0, C a1, C b1, C a2, C b2, C a3, C b3) {
+    C d;
+    d.x = c0 * (a0.x - b0.x) + c1 * (a1.x - b1.x) + c2 * (a2.x - b2.x) + c3 * (a3.x - b3.x);
+    d.y = c0 * (a0.y - b0.y) + c1 * (a1.y - b1.y) + c2 * (a2.y - b2.y) + c3 * (a3.y - b3.y);
+    return d;
+}
+
+template <typename C>
+__device__ __forceinline__ C cm(C a, C b) {
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+// conj(a) * b
+template <typename C>
+__device__ __forceinline__ C cjm(C a, C b) {
+    C r;
+    r.x = a.x * b.x + a.y * b.y;
+    r.y = a.x * b.y - a.y * b.x;
+    return r;
+}
+template <typename C>
+__device__ __forceinline__ bool ceq(C a, C b) {
+    return a.x == b.x && a.y == b.y;
+}
+
+// Table layout (per axis, length n = W for columns, H for rows):
+// t[0 n + k] = sv, t[1 n + k] = g - 1, t[2 n + k] = cA, t[3 n + k] = cB.
+// MODE 0: out = A u (or A^H u with ADJ); 1: out = b - A u.
+template <typename R, typename C, int TZ, int MODE, bool ADJ>
+__global__ __launch_bounds__(256) void k_cond8(int H, int W, int64_t n, int nrhs, int rb, const R* __restrict__ ax,
+                                               const R* __restrict__ az, const C* __restrict__ mass,
+                                               const C* __restrict__ tx, const C* __restrict__ tz, int bz0, int bx0,
+                                               int bh, int bw, R c0, R c1, R c2, R c3, const C* __restrict__ u,
+                                               C* __restrict__ out, const C* __restrict__ b) {
+    constexpr int TX = C8_TX, HL = C8_HL;
+    constexpr int SU = TX + 2 * HL;
+    constexpr int NFX = TX + 7;  // x faces -4 .. TX+2 of each tile row
+    constexpr int NFZ = TZ + 7;  // z face rows -4 .. TZ+2
+    constexpr int NZT = TZ + 2 * HL;
+    constexpr int RPT = TZ / 4;
+    constexpr int FXT = (TZ * NFX + 255) / 256;
+    constexpr int FZT = (NFZ * TX + 255) / 256;
+    __shared__ C su[NZT * SU];
+    __shared__ C sgx[TZ * NFX];
+    __shared__ C sgz[NFZ * TX];
+    __shared__ C xsv[SU], xg[SU], xa[SU], xb[SU];      // columns x0-7 .. x0+TX+6
+    __shared__ C zsv[NZT], zg[NZT], za[NZT], zb[NZT];  // rows z0-7 .. z0+TZ+6
+
+    const int tx_ = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx_;
+    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+
+    for (int k = tid; k < SU; k += 256) {
+        const int j = cwrap(x0 + k - HL, W);
+        xsv[k] = tx[j];
+        xg[k] = tx[W + j];
+        xa[k] = tx[2 * W + j];
+        xb[k] = tx[3 * W + j];
+    }
+    for (int k = tid; k < NZT; k += 256) {
+        const int i = cwrap(z0 + k - HL, H);
+        zsv[k] = tz[i];
+        zg[k] = tz[H + i];
+        za[k] = tz[2 * H + i];
+        zb[k] = tz[3 * H + i];
+    }
+    __syncthreads();
+    // second divergence needed: some face of the tile window has g != 1
+    bool px_ = false, pz_ = false, nux = false, nuz = false;
+    if (tid < NFX) px_ = xg[tid - 4 + HL].x != R(0) || xg[tid - 4 + HL].y != R(0);
+    if (tid < NFZ) pz_ = zg[tid - 4 + HL].x != R(0) || zg[tid - 4 + HL].y != R(0);
+    if (ADJ) {  // node scalings of the window not all equal
+        if (tid < SU) nux = !ceq(xa[tid], xa[HL]);
+        if (tid < NZT) nuz = !ceq(za[tid], za[HL]);
+    }
+    const bool pmlx = __syncthreads_or(px_);
+    const bool pmlz = __syncthreads_or(pz_);
+    const bool varx = ADJ ? (__syncthreads_or(nux) || pmlx) : false;
+    const bool varz = ADJ ? (__syncthreads_or(nuz) || pmlz) : false;
+
+    // face coefficients ax sv (x faces of the tile rows) and az sv (z faces of the tile columns)
+    C cfx[FXT], cfz[FZT];
+#pragma unroll
+    for (int q = 0; q < FXT; ++q) {
+        const int e = tid + q * 256;
+        C v;
+        v.x = R(0);
+        v.y = R(0);
+        if (e < TZ * NFX) {
+            const int r = e / NFX, f = e - r * NFX - 4;
+            const R a = ax[(int64_t)cwrap(z0 + r, H) * W + cwrap(x0 + f, W)];
+            v = xsv[f + HL];
+            v.x *= a;
+            v.y *= a;
+        }
+        cfx[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < FZT; ++q) {
+        const int e = tid + q * 256;
+        C v;
+        v.x = R(0);
+        v.y = R(0);
+        if (e < NFZ * TX) {
+            const int fr = e / TX - 4, c = e % TX;
+            const R a = az[(int64_t)cwrap(z0 + fr, H) * W + cwrap(x0 + c, W)];
+            v = zsv[fr + HL];
+            v.x *= a;
+            v.y *= a;
+        }
+        cfz[q] = v;
+    }
+    C mk[RPT];
+    bool inside[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = z0 + ty + 4 * k, j = x0 + tx_;
+        inside[k] = i < H && j < W && i >= bz0 && i < bz0 + bh && j >= bx0 && j < bx0 + bw;
+        C v;
+        v.x = R(0);
+        v.y = R(0);
+        if (inside[k]) v = mass[(int64_t)i * W + j];
+        if (ADJ) v.y = -v.y;
+        mk[k] = v;
+    }
+
+    const int r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
+    for (int r = blockIdx.z * rb; r < r1; ++r) {
+        const C* ur = u + (int64_t)r * n;
+        __syncthreads();
+        for (int rr = ty; rr < NZT; rr += 4) {
+            const bool zh = rr < HL || rr >= TZ + HL;
+            const int gz = cwrap(z0 + rr - HL, H);
+            const bool zin = gz >= bz0 && gz < bz0 + bh;
+            for (int cc = tx_; cc < SU; cc += 64) {
+                const bool xh = cc < HL || cc >= TX + HL;
+                if (zh && xh) continue;
+                const int gx = cwrap(x0 + cc - HL, W);
+                C v;
+                v.x = R(0);
+                v.y = R(0);
+                if (zin && gx >= bx0 && gx < bx0 + bw) v = ur[(int64_t)gz * W + gx];
+                su[rr * SU + cc] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < FXT; ++q) {
+            const int e = tid + q * 256;
+            if (e < TZ * NFX) {
+                const int rw = e / NFX, f = e - rw * NFX - 4;
+                const C* row = su + (rw + HL) * SU + HL;
+                C flux;
+                if (!ADJ) {
+                    flux = cm(cfx[q], c8diff<C, R>(c0, c1, c2, c3, row[f + 1], row[f], row[f + 2], row[f - 1],
+                                                   row[f + 3], row[f - 2], row[f + 4], row[f - 3]));
+                } else if (!varx) {
+                    const C d = c8diff<C, R>(c0, c1, c2, c3, row[f + 1], row[f], row[f + 2], row[f - 1], row[f + 3],
+                                             row[f - 2], row[f + 4], row[f - 3]);
+                    flux = cjm(cfx[q], cjm(xa[HL], d));
+                } else {
+                    const C* sa = xa + HL;
+                    const C d = c8diff<C, R>(c0, c1, c2, c3, cjm(sa[f + 1], row[f + 1]), cjm(sa[f], row[f]),
+                                             cjm(sa[f + 2], row[f + 2]), cjm(sa[f - 1], row[f - 1]),
+                                             cjm(sa[f + 3], row[f + 3]), cjm(sa[f - 2], row[f - 2]),
+                                             cjm(sa[f + 4], row[f + 4]), cjm(sa[f - 3], row[f - 3]));
+                    C t = d;
+                    if (pmlx) {
+                        const C* sb = xb + HL;
+                        const C d2 = c8diff<C, R>(c0, c1, c2, c3, cjm(sb[f + 1], row[f + 1]), cjm(sb[f], row[f]),
+                                                  cjm(sb[f + 2], row[f + 2]), cjm(sb[f - 1], row[f - 1]),
+                                                  cjm(sb[f + 3], row[f + 3]), cjm(sb[f - 2], row[f - 2]),
+                                                  cjm(sb[f + 4], row[f + 4]), cjm(sb[f - 3], row[f - 3]));
+                        const C gm = xg[f + HL];
+                        t = cadd(t, cjm(gm, d2));
+                    }
+                    flux = cjm(cfx[q], t);
+                }
+                sgx[e] = flux;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < FZT; ++q) {
+            const int e = tid + q * 256;
+            if (e < NFZ * TX) {
+                const int fr = e / TX - 4, c = e % TX;
+                const C* col = su + (fr + HL) * SU + HL + c;  // col[k * SU] = field at tile row fr + k
+                C flux;
+                if (!ADJ) {
+                    flux = cm(cfz[q], c8diff<C, R>(c0, c1, c2, c3, col[SU], col[0], col[2 * SU], col[-SU],
+                                                   col[3 * SU], col[-2 * SU], col[4 * SU], col[-3 * SU]));
+                } else if (!varz) {
+                    const C d = c8diff<C, R>(c0, c1, c2, c3, col[SU], col[0], col[2 * SU], col[-SU], col[3 * SU],
+                                             col[-2 * SU], col[4 * SU], col[-3 * SU]);
+                    flux = cjm(cfz[q], cjm(za[HL], d));
+                } else {
+                    const C* sa = za + HL + fr;  // sa[k] = row fr + k
+                    const C d = c8diff<C, R>(c0, c1, c2, c3, cjm(sa[1], col[SU]), cjm(sa[0], col[0]),
+                                             cjm(sa[2], col[2 * SU]), cjm(sa[-1], col[-SU]),
+                                             cjm(sa[3], col[3 * SU]), cjm(sa[-2], col[-2 * SU]),
+                                             cjm(sa[4], col[4 * SU]), cjm(sa[-3], col[-3 * SU]));
+                    C t = d;
+                    if (pmlz) {
+                        const C* sb = zb + HL + fr;
+                        const C d2 = c8diff<C, R>(c0, c1, c2, c3, cjm(sb[1], col[SU]), cjm(sb[0], col[0]),
+                                                  cjm(sb[2], col[2 * SU]), cjm(sb[-1], col[-SU]),
+                                                  cjm(sb[3], col[3 * SU]), cjm(sb[-2], col[-2 * SU]),
+                                                  cjm(sb[4], col[4 * SU]), cjm(sb[-3], col[-3 * SU]));
+                        t = cadd(t, cjm(zg[fr + HL], d2));
+                    }
+                    flux = cjm(cfz[q], t);
+                }
+                sgz[e] = flux;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int i = ty + 4 * k, c = tx_;
+            const int gi = z0 + i, gj = x0 + c;
+            if (gi < H && gj < W) {
+                const int64_t o = (int64_t)r * n + (int64_t)gi * W + gj;
+                C acc;
+                if (inside[k]) {
+                    const C* gxr = sgx + i * NFX + 4 + c;   // gxr[m] = face c + m
+                    const C* gzc = sgz + (i + 4) * TX + c;  // gzc[m * TX] = face row i + m
+                    const C dx = c8diff<C, R>(c0, c1, c2, c3, gxr[0], gxr[-1], gxr[1], gxr[-2], gxr[2], gxr[-3],
+                                              gxr[3], gxr[-4]);
+                    const C dz = c8diff<C, R>(c0, c1, c2, c3, gzc[0], gzc[-TX], gzc[TX], gzc[-2 * TX], gzc[2 * TX],
+                                              gzc[-3 * TX], gzc[3 * TX], gzc[-4 * TX]);
+                    if (!ADJ) {
+                        acc = cm(xa[c + HL], dx);
+                        cfma(acc, za[i + HL], dz);
+                        if (pmlx) {
+                            const C* gg = xg + HL + c;  // gg[m] = g - 1 at face c + m
+                            const C d2 = c8diff<C, R>(c0, c1, c2, c3, cm(gg[0], gxr[0]), cm(gg[-1], gxr[-1]),
+                                                      cm(gg[1], gxr[1]), cm(gg[-2], gxr[-2]), cm(gg[2], gxr[2]),
+                                                      cm(gg[-3], gxr[-3]), cm(gg[3], gxr[3]), cm(gg[-4], gxr[-4]));
+                            cfma(acc, xb[c + HL], d2);
+                        }
+                        if (pmlz) {
+                            const C* gg = zg + HL + i;  // gg[m] = g - 1 at face row i + m
+                            const C d2 = c8diff<C, R>(c0, c1, c2, c3, cm(gg[0], gzc[0]), cm(gg[-1], gzc[-TX]),
+                                                      cm(gg[1], gzc[TX]), cm(gg[-2], gzc[-2 * TX]),
+                                                      cm(gg[2], gzc[2 * TX]), cm(gg[-3], gzc[-3 * TX]),
+                                                      cm(gg[3], gzc[3 * TX]), cm(gg[-4], gzc[-4 * TX]));
+                            cfma(acc, zb[i + HL], d2);
+                        }
+                    } else {
+                        acc = cadd(dx, dz);
+                    }
+                    cfma(acc, mk[k], su[(i + HL) * SU + HL + c]);
+                } else {
+                    acc = u[o];
+                }
+                if (MODE == 1) acc = csub(b[o], acc);
+                out[o] = acc;
+            }
+        }
+    }
+}
+
+template <int TZ>
+static int cond8_rhs_per_group(const Level& L, int nrhs) {
+    const int64_t tiles = (int64_t)((L.nx + C8_TX - 1) / C8_TX) * ((L.nz + TZ - 1) / TZ);
+    const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + tiles - 1) / tiles));
+    return (int)((nrhs + groups - 1) / groups);
+}
+
+int32_t launch_cond8_32(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
+                        hipStream_t s) {
+    constexpr int TZ = 16;
+    HELM_CHECK(L.ax32 && L.az32 && L.cmass32 && L.ctab32, HELM_ERR_STATE, "conditioned operator: medium not set");
+    const int rb = cond8_rhs_per_group<TZ>(L, nrhs);
+    const dim3 block(64, 4), grid((L.nx + C8_TX - 1) / C8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
+    const float ih = (float)(1.0 / L.h_m);
+    const float c0 = (float)CD0 * ih, c1 = (float)CD1 * ih, c2 = (float)CD2 * ih, c3 = (float)CD3 * ih;
+    const float2* tx = L.ctab32;
+    const float2* tz = L.ctab32 + 4 * L.nx;
+#define HELM_C8_32(M, A, BP)                                                                                   \
+    hipLaunchKernelGGL((k_cond8<float, float2, TZ, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.ax32, \
+                       L.az32, L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3, u,  \
+                       out, BP)
+    if (b) {
+        if (adjoint) HELM_C8_32(1, true, b);
+        else HELM_C8_32(1, false, b);
+    } else {
+        if (adjoint) HELM_C8_32(0, true, (const float2*)nullptr);
+        else HELM_C8_32(0, false, (const float2*)nullptr);
+    }
+#undef HELM_C8_32
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t launch_cond8_64(const Level& L, const double2* u, double2* out, int nrhs, bool adjoint, const double2* b,
+                        hipStream_t s) {
+    constexpr int TZ = 8;
+    HELM_CHECK(L.ax64 && L.az64 && L.cmass64 && L.ctab64, HELM_ERR_STATE, "conditioned operator: medium not set");
+    const int rb = cond8_rhs_per_group<TZ>(L, nrhs);
+    const dim3 block(64, 4), grid((L.nx + C8_TX - 1) / C8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
+    const double ih = 1.0 / L.h_m;
+    const double c0 = CD0 * ih, c1 = CD1 * ih, c2 = CD2 * ih, c3 = CD3 * ih;
+    const double2* tx = L.ctab64;
+    const double2* tz = L.ctab64 + 4 * L.nx;
+#define HELM_C8_64(M, A, BP)                                                                                     \
+    hipLaunchKernelGGL((k_cond8<double, double2, TZ, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, rb, L.ax64, \
+                       L.az64, L.cmass64, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3, u,    \
+                       out, BP)
+    if (b) {
+        if (adjoint) HELM_C8_64(1, true, b);
+        else HELM_C8_64(1, false, b);
+    } else {
+        if (adjoint) HELM_C8_64(0, true, (const double2*)nullptr);
+        else HELM_C8_64(0, false, (const double2*)nullptr);
+    }
+#undef HELM_C8_64
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+// ---- host-side factors (conditioned_wave.py:33-41, 58-69) ---------------
+using cd = std::complex<double>;
+
+struct CondAxis {
+    int size, offset, pml;
+    double dt, dx, c0;
+    // damping of FrequencyGeometry (conditioned_wave.py:34-37) at box position t (node index)
+    double damping(double t, double stagger) const {
+        const double pos = t - offset;
+        const double left = pos < pml ? std::pow(1.0 - (pos + stagger) / pml, 4) : 0.0;
+        const double right = pos >= size - pml ? std::pow(1.0 - (size - 1 - pos - stagger) / pml, 4) : 0.0;
+        return (left + right) * 2.0 * c0 / dx;
+    }
+    double ar(double t) const { return 1.0 - dt * damping(t, 0.0); }
+    double av(double t) const { return 1.0 - dt * damping(t, 0.5); }  // face t + 1/2
+};
+
+struct CondFactors {
+    cd sv, gm1, cA, cB, left, right;
+};
+
+static CondFactors cond_factors(const CondAxis& a, double t, cd D, double k0) {
+    const cd q = 1.0 / D, scalar = (D - 1.0) / D / (k0 * k0);
+    const double ar = a.ar(t), av = a.av(t);
+    const double m = ar != 1.0 ? 1.0 : 0.0;
+    CondFactors f;
+    f.left = m / (D - ar);
+    f.right = 1.0 + (1.0 - m) / (D - ar);
+    f.sv = (D - 1.0) / (D - av);
+    f.gm1 = (1.0 - av) * q;
+    f.cA = scalar * (f.left + f.right);
+    f.cB = scalar * f.right;
+    return f;
+}
+
+// Multigrid stretch of the 5-point approximation at box position t (fine
+// node units): node factor s with 1/s = k0^2 scalar (left + right g_mean),
+// g_mean the mean gain of the node's two faces, and face factor 1/s_face = sv;
+// returned in this library's convention (conjugated).
+static double2 cond_node_stretch(const CondAxis& a, double t, cd D, double k0) {
+    const CondFactors f = cond_factors(a, t, D, k0);
+    const cd gl = 1.0 + (1.0 - a.av(t - 1.0)) / D, gr = 1.0 + f.gm1;
+    const cd inv = (D - 1.0) / D * (f.left + f.right * 0.5 * (gl + gr));
+    const cd s = std::conj(1.0 / inv);
+    return make_double2(s.real(), s.imag());
+}
+static double2 cond_face_istretch(const CondAxis& a, double tface, cd D) {
+    // face at box position tface = t + 1/2 of node t
+    const cd sv = (D - 1.0) / (D - a.av(tface - 0.5));
+    const cd c = std::conj(sv);
+    return make_double2(c.real(), c.imag());
+}
+
+static CondAxis cond_axis(const helm_plan* p, bool xaxis) {
+    const helm_conditioned_medium& m = p->cond;
+    CondAxis a;
+    a.size = xaxis ? m.w : m.h;
+    a.offset = xaxis ? m.px : m.pz;
+    a.pml = m.pml;
+    a.dt = m.dt_s;
+    a.dx = p->levels[0].h_m;
+    a.c0 = p->desc.c0_m_s;
+    return a;
+}
+
+int32_t cond_level_profiles(const helm_plan* p, int level, std::vector<double2>& sx, std::vector<double2>& sz,
+                            std::vector<double2>& isxe, std::vector<double2>& isze) {
+    const cd D = std::exp(cd(0.0, 2.0 * M_PI * p->desc.frequency_mhz * 1e6 * p->cond.dt_s));
+    const double k0 = p->k0, scale = (double)(1 << level);
+    const CondAxis ax = cond_axis(p, true), az = cond_axis(p, false);
+    for (size_t j = 0; j < sx.size(); ++j) sx[j] = cond_node_stretch(ax, j * scale, D, k0);
+    for (size_t i = 0; i < sz.size(); ++i) sz[i] = cond_node_stretch(az, i * scale, D, k0);
+    for (size_t k = 0; k < isxe.size(); ++k) isxe[k] = cond_face_istretch(ax, (k - 0.5) * scale, D);
+    for (size_t k = 0; k < isze.size(); ++k) isze[k] = cond_face_istretch(az, (k - 0.5) * scale, D);
+    return HELM_OK;
+}
+
+// ---- medium (conditioned_wave.py:45-66) ----------------------------------
+struct CondGeom {
+    int H, W, h, w, pz, px, pml;
+    double rho0, c0, dt, epa0, epa1, tau0, tau1;
+    double2 D;
+};
+
+__device__ __forceinline__ int cond_src(int box, int off, int size, int pml) {
+    // box index -> native index of the cropped-and-replicated coefficients
+    return min(max(box - off, pml), size - 1 - pml);
+}
+
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+__global__ void k_cond_medium(CondGeom g, const double* __restrict__ c, const double* __restrict__ rho,
+                              const double* __restrict__ alpha, double* ax64, double* az64, float* ax32, float* az32,
+                              double2* mass64, float2* mass32, double* eta64, float* eta32, double2* kap64,
+                              float2* kap32) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= g.W) return;
+    auto at = [&](const double* f, int bi, int bj) {
+        bi = cwrap(bi, g.H);
+        bj = cwrap(bj, g.W);
+        return f[(int64_t)cond_src(bi, g.pz, g.h, g.pml) * g.w + cond_src(bj, g.px, g.w, g.pml)];
+    };
+    const bool in = i >= g.pz && i < g.pz + g.h && j >= g.px && j < g.px + g.w;
+    const double a = g.rho0 / at(rho, i, j);
+    const double ae = g.rho0 / at(rho, i, j + 1), as = g.rho0 / at(rho, i + 1, j);
+    const double axv = in ? 2.0 * a * ae / (a + ae) : 0.0;
+    const double azv = in ? 2.0 * a * as / (a + as) : 0.0;
+    const double ci = at(c, i, j), al = alpha ? at(alpha, i, j) : 0.0;
+    const double e0 = al * ci / g.c0 * g.epa0, e1 = al * ci / g.c0 * g.epa1;
+    const double amp = (1.0 + (e0 + e1) / 2.0) * (1.0 + (e0 + e1) / 2.0);
+    const double2 dm1 = make_double2(g.D.x - 1.0, g.D.y);
+    const double2 one = make_double2(1.0, 0.0);
+    // h = sum_k e_k / (amp (1 + tau_k (D - 1)/dt))
+    double2 hs = make_double2(0.0, 0.0);
+    {
+        const double2 d0 = make_double2(amp * (1.0 + g.tau0 * dm1.x / g.dt), amp * (g.tau0 * dm1.y / g.dt));
+        const double2 d1 = make_double2(amp * (1.0 + g.tau1 * dm1.x / g.dt), amp * (g.tau1 * dm1.y / g.dt));
+        hs = cadd(zdiv(make_double2(e0, 0.0), d0), zdiv(make_double2(e1, 0.0), d1));
+    }
+    // modulus = amp (1 - h) / (1 - (1 - 1/D) h)
+    const double2 qd = zdiv(one, g.D);
+    const double2 omq = make_double2(1.0 - qd.x, -qd.y);
+    const double2 num = cscale(csub(one, hs), amp);
+    const double2 den = csub(one, cmul(omq, hs));
+    const double2 modulus = zdiv(num, den);
+    const double r = g.c0 / ci;
+    const double2 mk = zdiv(make_double2(a * (r * r), 0.0), modulus);
+    const double2 ms = in ? mk : make_double2(0.0, 0.0);
+    const int64_t o = (int64_t)i * g.W + j;
+    ax64[o] = axv;
+    az64[o] = azv;
+    ax32[o] = (float)axv;
+    az32[o] = (float)azv;
+    mass64[o] = ms;
+    mass32[o] = make_float2((float)ms.x, (float)ms.y);
+    // the multigrid's 5-point approximation, in this library's convention: eta = a, kappa = conj(mass)
+    eta64[o] = a;
+    eta32[o] = (float)a;
+    kap64[o] = make_double2(mk.x, -mk.y);
+    kap32[o] = make_float2((float)mk.x, (float)-mk.y);
+}
+
+// dense_wave.source_rhs (dense_wave.py:59-62): line [nrhs][w] on box row `face`
+__global__ void k_cond_source(CondGeom g, int face, int zi, int xi, double dx, const double* __restrict__ c,
+                              const double* __restrict__ rho, const double2* __restrict__ line, double2* rhs,
+                              int nrhs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, r = blockIdx.z;
+    if (j >= g.W || r >= nrhs) return;
+    double2 v = make_double2(0.0, 0.0);
+    if (i == face && j >= g.px && j < g.px + g.w) {
+        const double sc = c[(int64_t)cond_src(zi + g.pz, g.pz, g.h, g.pml) * g.w + cond_src(xi + g.px, g.px, g.w, g.pml)];
+        const double a = g.rho0 / rho[(int64_t)cond_src(i, g.pz, g.h, g.pml) * g.w + cond_src(j, g.px, g.w, g.pml)];
+        const double2 l = line[(int64_t)r * g.w + (j - g.px)];
+        const double2 scale = zdiv(make_double2(a * g.c0 * g.c0 * 2.0 * g.dt, 0.0),
+                                   make_double2(sc * dx * (g.D.x - 1.0), sc * dx * g.D.y));
+        v = cmul(l, scale);
+    }
+    rhs[(int64_t)r * g.H * g.W + (int64_t)i * g.W + j] = v;
+}
+
+// preconditioner wrapper: forward z = MG^{-1}(k0^2 sx sz v), adjoint
+// z = conj(k0^2 sx sz) MG^{-H}(v); outside the native box z = v (the operator
+// is the identity there and decoupled from the box).
+__global__ void k_cond_prescale(int H, int W, int64_t n, int nrhs, int bz0, int bx0, int bh, int bw, float k0sq,
+                                const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                const float2* __restrict__ v, float2* out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nrhs) return;
+    const int64_t node = e % n;
+    const int i = (int)(node / W), j = (int)(node % W);
+    float2 r = make_float2(0.f, 0.f);
+    if (i >= bz0 && i < bz0 + bh && j >= bx0 && j < bx0 + bw) {
+        const float2 f = cscale(cconj(cmul(sx[j], sz[i])), k0sq);  // the profiles hold conj(s)
+        r = cmul(f, v[e]);
+    }
+    out[e] = r;
+}
+
+__global__ void k_cond_post(int H, int W, int64_t n, int nrhs, int bz0, int bx0, int bh, int bw, float k0sq,
+                            int adjoint, const float2* __restrict__ sx, const float2* __restrict__ sz,
+                            const float2* __restrict__ v, float2* z) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nrhs) return;
+    const int64_t node = e % n;
+    const int i = (int)(node / W), j = (int)(node % W);
+    if (i >= bz0 && i < bz0 + bh && j >= bx0 && j < bx0 + bw) {
+        if (adjoint) z[e] = cmul(cscale(cmul(sx[j], sz[i]), k0sq), z[e]);
+    } else {
+        z[e] = v[e];
+    }
+}
+
+int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+    const Level& L = p->levels[0];
+    const int64_t cnt = (int64_t)nrhs * L.n;
+    const dim3 grid((unsigned)((cnt + 255) / 256)), block(256);
+    const float k0sq = (float)(p->k0 * p->k0);
+    if (!adjoint) {
+        if (p->cond_cap < cnt) {
+            if (p->cond_scr) (void)hipFree(p->cond_scr);
+            p->cond_scr = nullptr;
+            p->cond_cap = 0;
+            HELM_HIP(hipMalloc((void**)&p->cond_scr, cnt * sizeof(float2) + 256));
+            p->cond_cap = cnt;
+            p->generation++;
+        }
+        hipLaunchKernelGGL(k_cond_prescale, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1],
+                           L.cbox[2], L.cbox[3], k0sq, L.sx32, L.sz32, v, p->cond_scr);
+        HELM_HIP(hipGetLastError());
+        HELM_TRY(mg_vcycle(p, p->cond_scr, z, nrhs, true, s));
+    } else {
+        HELM_TRY(mg_vcycle(p, v, z, nrhs, false, s));
+    }
+    hipLaunchKernelGGL(k_cond_post, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1], L.cbox[2],
+                       L.cbox[3], k0sq, (int)adjoint, L.sx32, L.sz32, v, z);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+template <typename T>
+static int32_t c_alloc(T** ptr, size_t count) {
+    if (*ptr) return HELM_OK;
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T) + 256);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return HELM_ERR_ALLOC;
+    }
+    return HELM_OK;
+}
+
+static CondGeom cond_geom(const helm_plan* p) {
+    const helm_conditioned_medium& m = p->cond;
+    const Level& L = p->levels[0];
+    CondGeom g{};
+    g.H = L.nz;
+    g.W = L.nx;
+    g.h = m.h;
+    g.w = m.w;
+    g.pz = m.pz;
+    g.px = m.px;
+    g.pml = m.pml;
+    g.rho0 = m.rho0;
+    g.c0 = p->desc.c0_m_s;
+    g.dt = m.dt_s;
+    g.epa0 = m.eta_per_unit_alpha[0];
+    g.epa1 = m.eta_per_unit_alpha[1];
+    g.tau0 = m.taus_s[0];
+    g.tau1 = m.taus_s[1];
+    const double ph = 2.0 * M_PI * p->desc.frequency_mhz * 1e6 * m.dt_s;
+    g.D = make_double2(std::cos(ph), std::sin(ph));
+    return g;
+}
+
+}  // namespace helm
+
+using namespace helm;
+
+extern "C" {
+
+int32_t helm_set_medium_conditioned(helm_plan* p, const double* c, const double* rho, const double* alpha,
+                                    const helm_conditioned_medium* m, void* stream) {
+    HELM_CHECK(p && c && rho && m, HELM_ERR_ARG, "helm_set_medium_conditioned: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_COND8, HELM_ERR_STATE,
+               "helm_set_medium_conditioned: the plan was not created with op_kind HELM_OP_COND8");
+    Level& L = p->levels[0];
+    HELM_CHECK(m->h >= 1 && m->w >= 1 && m->pz >= 0 && m->px >= 0 && m->pz + m->h <= L.nz && m->px + m->w <= L.nx,
+               HELM_ERR_ARG, "helm_set_medium_conditioned: the medium does not fit the box");
+    HELM_CHECK(m->pz >= 8 && m->px >= 8 && L.nz - m->pz - m->h >= 8 && L.nx - m->px - m->w >= 8, HELM_ERR_ARG,
+               "helm_set_medium_conditioned: the box must pad the medium by >= 8 nodes on every side");
+    HELM_CHECK(m->pml >= 1 && 2 * m->pml < m->h && 2 * m->pml < m->w, HELM_ERR_ARG,
+               "helm_set_medium_conditioned: pml must be >= 1 and leave a medium core");
+    HELM_CHECK(std::isfinite(m->dt_s) && m->dt_s > 0 && std::isfinite(m->rho0) && m->rho0 > 0, HELM_ERR_ARG,
+               "helm_set_medium_conditioned: dt_s and rho0 must be finite and positive");
+    HELM_CHECK(std::sin(M_PI * p->desc.frequency_mhz * 1e6 * m->dt_s) > 0, HELM_ERR_ARG,
+               "helm_set_medium_conditioned: frequency * dt_s must lie in (0, 1)");
+    hipStream_t s = (hipStream_t)stream;
+    p->cond = *m;
+    p->k0 = 2.0 * std::sin(M_PI * p->desc.frequency_mhz * 1e6 * m->dt_s) / (m->dt_s * p->desc.c0_m_s);
+    HELM_TRY(c_alloc(&L.ax64, L.n));
+    HELM_TRY(c_alloc(&L.az64, L.n));
+    HELM_TRY(c_alloc(&L.ax32, L.n));
+    HELM_TRY(c_alloc(&L.az32, L.n));
+    HELM_TRY(c_alloc(&L.cmass64, L.n));
+    HELM_TRY(c_alloc(&L.cmass32, L.n));
+    HELM_TRY(c_alloc(&L.ctab64, (size_t)4 * (L.nx + L.nz)));
+    HELM_TRY(c_alloc(&L.ctab32, (size_t)4 * (L.nx + L.nz)));
+    const CondGeom g = cond_geom(p);
+    hipLaunchKernelGGL(k_cond_medium, dim3((L.nx + 255) / 256, L.nz), dim3(256), 0, s, g, c, rho, alpha, L.ax64,
+                       L.az64, L.ax32, L.az32, L.cmass64, L.cmass32, L.eta64, L.eta32, L.kap64, L.kap32);
+    HELM_HIP(hipGetLastError());
+    // per-column and per-row factors
+    const cd D(g.D.x, g.D.y);
+    std::vector<double2> t64((size_t)4 * (L.nx + L.nz));
+    std::vector<float2> t32(t64.size());
+    auto fill = [&](const CondAxis& a, int n, size_t base) {
+        for (int k = 0; k < n; ++k) {
+            const CondFactors f = cond_factors(a, k, D, p->k0);
+            const cd v[4] = {f.sv, f.gm1, f.cA, f.cB};
+            for (int t = 0; t < 4; ++t) t64[base + (size_t)t * n + k] = make_double2(v[t].real(), v[t].imag());
+        }
+    };
+    fill(cond_axis(p, true), L.nx, 0);
+    fill(cond_axis(p, false), L.nz, (size_t)4 * L.nx);
+    for (size_t k = 0; k < t64.size(); ++k) t32[k] = make_float2((float)t64[k].x, (float)t64[k].y);
+    HELM_HIP(hipMemcpyAsync(L.ctab64, t64.data(), t64.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    HELM_HIP(hipMemcpyAsync(L.ctab32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+    HELM_HIP(hipStreamSynchronize(s));  // the host tables go out of scope
+    L.op = HELM_OP_COND8;
+    L.cbox[0] = m->pz;
+    L.cbox[1] = m->px;
+    L.cbox[2] = m->h;
+    L.cbox[3] = m->w;
+    // multigrid stretch profiles of every level from the split-PML factors
+    for (size_t l = 0; l < p->levels.size(); ++l) HELM_TRY(upload_cond_profiles(p, (int)l));
+    p->medium_set = true;
+    p->generation++;
+    if (p->mg.levels > 0) HELM_TRY(mg_build(p, s));
+    return HELM_OK;
+}
+
+int32_t helm_conditioned_source(helm_plan* p, const double* c, const double* rho, const void* line, int32_t face,
+                                int32_t source_z, int32_t source_x, void* rhs, int32_t nrhs, void* stream) {
+    HELM_CHECK(p && c && rho && line && rhs, HELM_ERR_ARG, "helm_conditioned_source: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_COND8 && p->medium_set, HELM_ERR_STATE,
+               "helm_conditioned_source: call helm_set_medium_conditioned first");
+    const Level& L = p->levels[0];
+    HELM_CHECK(face >= 0 && face < L.nz, HELM_ERR_ARG, "helm_conditioned_source: face row outside the box");
+    HELM_CHECK(nrhs >= 1 && nrhs <= 65535 && (int64_t)nrhs * L.n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_conditioned_source: nrhs out of range");
+    const CondGeom g = cond_geom(p);
+    hipLaunchKernelGGL(k_cond_source, dim3((L.nx + 255) / 256, L.nz, nrhs), dim3(256), 0, (hipStream_t)stream, g,
+                       face, source_z, source_x, L.h_m, c, rho, (const double2*)line, (double2*)rhs, nrhs);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+}  // extern "C"

@@ -90,6 +90,15 @@ struct Level {
     double *ax64 = nullptr, *az64 = nullptr;
     int sp_cells = 0;      // sponge width
     double sp_amp = 0.0;   // sponge peak
+    // HELM_OP_COND8 (level 0 only): masked split-PML operator; ax/az above hold
+    // the masked face means, cmass the node mass, ctab the per-column [4][nx]
+    // then per-row [4][nz] factors (sv, g - 1, cA, cB), cbox the native box
+    // (z0, x0, h, w); eta/kap/sx/.. hold the multigrid's 5-point approximation
+    double2* cmass64 = nullptr;
+    float2* cmass32 = nullptr;
+    double2* ctab64 = nullptr;
+    float2* ctab32 = nullptr;
+    int cbox[4] = {0, 0, 0, 0};
     // streaming-stencil launch shape chosen by helm_autotune (0: default)
     int st_tz = 0, st_rb = 0;
 };
@@ -189,6 +198,9 @@ struct helm_plan {
     int64_t mg_wg_target = 0;         // multigrid workgroup target chosen by helm_autotune (0: default)
     helm_spectral_medium spectral{};  // geometry of the last helm_set_medium_spectral
     double2* sp_buf = nullptr;        // incident-field scratch: 4 * nx c128
+    helm_conditioned_medium cond{};   // geometry of the last helm_set_medium_conditioned
+    float2* cond_scr = nullptr;       // scaled preconditioner input (HELM_OP_COND8)
+    int64_t cond_cap = 0;             // elements cond_scr holds
 };
 
 namespace helm {
@@ -209,6 +221,17 @@ int32_t launch_stag8_32(const Level& L, double k0sq, const float2* u, float2* ou
                         const float2* b, hipStream_t s);
 int32_t launch_stag8_64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
                         const double2* b, int mode, hipStream_t s);
+// conditioned split-PML operator (helm_conditioned.hip): A u, or b - A u when b is given
+int32_t launch_cond8_32(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
+                        hipStream_t s);
+int32_t launch_cond8_64(const Level& L, const double2* u, double2* out, int nrhs, bool adjoint, const double2* b,
+                        hipStream_t s);
+// multigrid stretch profiles of level `level` from the split-PML factors
+int32_t cond_level_profiles(const helm_plan* p, int level, std::vector<double2>& sx, std::vector<double2>& sz,
+                            std::vector<double2>& isxe, std::vector<double2>& isze);
+int32_t upload_cond_profiles(helm_plan* p, int level);
+// the multigrid preconditioner of a conditioned plan (node scaling + V-cycle)
+int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
 // multigrid (helm_mg.hip)
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
 int32_t mg_build(helm_plan* p, hipStream_t s);

@@ -699,7 +699,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     bool budget = false;
     Profiler* pf = (p->prof && p->prof->on) ? p->prof : nullptr;
     const double nn = (double)n, rr = (double)nrhs;
-    const double coef_bytes32 = L.op == HELM_OP_STAG8 ? 16.0 : 12.0;  // per node, FP32 coefficients
+    const double coef_bytes32 = L.op != HELM_OP_FLUX5 ? 16.0 : 12.0;  // per node, FP32 coefficients
     const double apply_bytes = 16.0 * nn * rr + coef_bytes32 * nn;
     const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
     for (;;) {

@@ -65,10 +65,16 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
         return stretch_at(c, lo, hi, d.pml_mm, p->omega, d.c0_m_s, d.pml_strength);
     };
     std::vector<double2> sx(nx), sz(nz), isxe(nx + 1), isze(nz + 1);
-    for (int j = 0; j < nx; ++j) sx[j] = s(x0_mm + j * h_mm, lower_x, upper_x);
-    for (int i = 0; i < nz; ++i) sz[i] = s(z0_mm + i * h_mm, lower_z, upper_z);
-    for (int k = 0; k <= nx; ++k) isxe[k] = crecip(s(x0_mm + (k - 0.5) * h_mm, lower_x, upper_x));
-    for (int k = 0; k <= nz; ++k) isze[k] = crecip(s(z0_mm + (k - 0.5) * h_mm, lower_z, upper_z));
+    if (d.op_kind == HELM_OP_COND8 && p->cond.h > 0) {  // split-PML factors of the conditioned medium
+        int level = 0;
+        while (level < 30 && std::fabs(h_mm - d.h_mm * (double)(1 << level)) > 1e-9 * d.h_mm) ++level;
+        HELM_TRY(cond_level_profiles(p, level, sx, sz, isxe, isze));
+    } else {
+        for (int j = 0; j < nx; ++j) sx[j] = s(x0_mm + j * h_mm, lower_x, upper_x);
+        for (int i = 0; i < nz; ++i) sz[i] = s(z0_mm + i * h_mm, lower_z, upper_z);
+        for (int k = 0; k <= nx; ++k) isxe[k] = crecip(s(x0_mm + (k - 0.5) * h_mm, lower_x, upper_x));
+        for (int k = 0; k <= nz; ++k) isze[k] = crecip(s(z0_mm + (k - 0.5) * h_mm, lower_z, upper_z));
+    }
     HELM_TRY(upload_profile(sx, &L.sx64, &L.sx32));
     HELM_TRY(upload_profile(sz, &L.sz64, &L.sz32));
     HELM_TRY(upload_profile(isxe, &L.isxe64, &L.isxe32));
@@ -80,10 +86,21 @@ int32_t build_level_profiles(helm_plan* p, Level& L, double x0_mm, double z0_mm,
     return HELM_OK;
 }
 
+int32_t upload_cond_profiles(helm_plan* p, int level) {
+    Level& L = p->levels[level];
+    std::vector<double2> sx(L.nx), sz(L.nz), isxe(L.nx + 1), isze(L.nz + 1);
+    HELM_TRY(cond_level_profiles(p, level, sx, sz, isxe, isze));
+    HELM_TRY(upload_profile(sx, &L.sx64, &L.sx32));
+    HELM_TRY(upload_profile(sz, &L.sz64, &L.sz32));
+    HELM_TRY(upload_profile(isxe, &L.isxe64, &L.isxe32));
+    HELM_TRY(upload_profile(isze, &L.isze64, &L.isze32));
+    return HELM_OK;
+}
+
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
                     L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf,
-                    L.wid, L.ax32, L.az32, L.ax64, L.az64};
+                    L.wid, L.ax32, L.az32, L.ax64, L.az64, L.cmass64, L.cmass32, L.ctab64, L.ctab32};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();
@@ -281,9 +298,13 @@ int32_t helm_plan_create(const helm_desc* desc, helm_plan** out) {
                "PML leaves no physical domain");
     HELM_CHECK((int64_t)d.nz * d.nx * d.nrhs < (int64_t)1 << 31, HELM_ERR_ARG,
                "nz*nx*nrhs must stay below 2^31");
-    HELM_CHECK(d.op_kind == HELM_OP_FLUX5 || d.op_kind == HELM_OP_STAG8, HELM_ERR_ARG, "unknown op_kind");
+    HELM_CHECK(d.op_kind == HELM_OP_FLUX5 || d.op_kind == HELM_OP_STAG8 || d.op_kind == HELM_OP_COND8,
+               HELM_ERR_ARG, "unknown op_kind");
     HELM_CHECK(d.op_kind != HELM_OP_STAG8 || (d.pml_mm == 0.0 && d.nz >= 16 && d.nx >= 16), HELM_ERR_ARG,
                "op_kind HELM_OP_STAG8 needs pml_mm = 0 (the sponge absorbs) and a box of at least 16 x 16");
+    HELM_CHECK(d.op_kind != HELM_OP_COND8 || (d.pml_mm == 0.0 && d.nz >= 24 && d.nx >= 24), HELM_ERR_ARG,
+               "op_kind HELM_OP_COND8 needs pml_mm = 0 (the split PML lies inside the medium) and a box of at "
+               "least 24 x 24");
     helm_plan* p = new helm_plan();
     p->desc = d;
     (void)hipGetDevice(&p->device);
@@ -308,6 +329,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     if (p->net_scale) (void)hipFree(p->net_scale);
     if (p->net_part) (void)hipFree(p->net_part);
     if (p->sp_buf) (void)hipFree(p->sp_buf);
+    if (p->cond_scr) (void)hipFree(p->cond_scr);
     prof_destroy(p->prof);
     delete p;
     return HELM_OK;
@@ -316,7 +338,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
 int32_t helm_set_medium(helm_plan* p, const double* eta_rel, const double* kappa_c, void* stream) {
     HELM_CHECK(p && eta_rel && kappa_c, HELM_ERR_ARG, "helm_set_medium: null argument");
     HELM_CHECK(p->desc.op_kind == HELM_OP_FLUX5, HELM_ERR_STATE,
-               "helm_set_medium: spectral plans take helm_set_medium_spectral");
+               "helm_set_medium: spectral and conditioned plans take their own medium call");
     hipStream_t s = (hipStream_t)stream;
     Level& L = p->levels[0];
     int64_t n = L.n;
@@ -332,7 +354,7 @@ int32_t helm_set_medium_physical(helm_plan* p, const double* c, const double* rh
                                  double rho0, void* stream) {
     HELM_CHECK(p && c && rho, HELM_ERR_ARG, "helm_set_medium_physical: null argument");
     HELM_CHECK(p->desc.op_kind == HELM_OP_FLUX5, HELM_ERR_STATE,
-               "helm_set_medium_physical: spectral plans take helm_set_medium_spectral");
+               "helm_set_medium_physical: spectral and conditioned plans take their own medium call");
     HELM_CHECK(std::isfinite(rho0) && rho0 > 0, HELM_ERR_ARG, "rho0 must be finite and positive");
     hipStream_t s = (hipStream_t)stream;
     Level& L = p->levels[0];

@@ -179,12 +179,17 @@ int32_t launch_apply32(const Level& L, double k0sq, const float2* u, float2* out
         HELM_CHECK(shift == 0.0f, HELM_ERR_ARG, "the 8-tap operator has no shifted form");
         return launch_stag8_32(L, k0sq, u, out, nrhs, adjoint, b, s);
     }
+    if (L.op == HELM_OP_COND8) {
+        HELM_CHECK(shift == 0.0f, HELM_ERR_ARG, "the conditioned operator has no shifted form");
+        return launch_cond8_32(L, u, out, nrhs, adjoint, b, s);
+    }
     return launch_flux32(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
 }
 
 int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* out, int nrhs, bool adjoint,
                        const double2* b, hipStream_t s) {
     if (L.op == HELM_OP_STAG8) return launch_stag8_64(L, k0sq, u, out, nrhs, adjoint, b, b ? 1 : 0, s);
+    if (L.op == HELM_OP_COND8) return launch_cond8_64(L, u, out, nrhs, adjoint, b, s);
     return launch_flux64(L, k0sq, u, out, nrhs, adjoint, b, s);
 }
 

@@ -1,0 +1,208 @@
+"""CPU oracle for the conditioned split-PML frequency operator (dense path).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, never by ``nsrno_amd``.
+NumPy in float64/complex128.
+
+Restates nsrno/conditioned_wave.py:12-92 and nsrno/dense_wave.py:59-82:
+
+* ``Geometry``         FrequencyGeometry (:12-50): padded box, 8th-order
+                       symbols, domain mask, receiver row/columns, split-PML
+                       damping factors arx, arz (nodes) and avx, avz (faces)
+* ``System``           ConditionedFrequencySystem.__init__ (:52-69): cropped
+                       and edge-padded coefficients, D = e^{i omega dt}, k0 of
+                       the leapfrog, harmonic face means ax, az (masked),
+                       relaxation modulus, mass, svx/svz, gvx/gvz, mx/mz
+* ``System.apply_fft`` __call__ (:71-80) in the Fourier form
+* ``System.apply``     the same operator as a real-space 8-tap stencil
+* ``projected`` / ``adjoint``  dense_wave.projected_operator / adjoint_operator
+                       (:66-82), stencil forms; ``adjoint_fft`` the reference's
+                       Fourier form of the adjoint
+* ``source_rhs``       dense_wave.source_rhs (:59-62)
+* ``fixed_multiplier`` the fixed shifted-Laplacian preconditioner (:90-92)
+
+Pinned by (tests/test_oracle_conditioned.py): golden vectors written by
+tests/golden/make_golden_c.py from the reference's own classes (geometry
+factors, system coefficients, the operator, the projected operator, the
+adjoint and the source term on seeded inputs); stencil == Fourier forms;
+<y, A x> = <A^H y, x> for the adjoint pair.
+"""
+from __future__ import annotations
+
+import cmath
+import math
+
+import numpy as np
+
+FD = (1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168)
+C0 = 1540.0
+RHO0 = 1000.0
+
+
+def symbol(n, dx):
+    q = 2 * np.pi * np.fft.fftfreq(n)
+    return 2j * sum(a * np.sin((j + 0.5) * q) for j, a in enumerate(FD)) * np.exp(0.5j * q) / dx
+
+
+class Geometry:
+    """FrequencyGeometry (conditioned_wave.py:12-50)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.h, self.w = spec["medium_shape"][-2:]
+        self.dx = spec["dx_m"]
+        self.H = math.ceil((self.h + 64) / 32) * 32
+        self.W = math.ceil((self.w + 64) / 32) * 32
+        if "fft_shape" in spec:
+            self.H, self.W = spec["fft_shape"]
+        self.pz = (self.H - self.h) // 2
+        self.px = (self.W - self.w) // 2
+        self.Dx = symbol(self.W, self.dx)[None, :]
+        self.Dz = symbol(self.H, self.dx)[:, None]
+        self.lap = -(np.abs(self.Dx) ** 2 + np.abs(self.Dz) ** 2)
+        self.mask = np.zeros((self.H, self.W))
+        self.mask[self.pz:self.pz + self.h, self.px:self.px + self.w] = 1.0
+        self.rx_cols = np.round((np.arange(192) * 0.0002 - 0.0191 - spec["x0_m"]) / self.dx).astype(np.int64) + self.px
+        self.face = round(-spec["z0_m"] / self.dx) + self.pz
+        self.dt = spec["native_dt_s"]
+        self.pml = spec["pml"]
+        xx = np.arange(self.W) - self.px
+        zz = np.arange(self.H) - self.pz
+
+        def damping(pos, size, stagger):
+            left = np.where(pos < self.pml, (1 - (pos + stagger) / self.pml) ** 4, 0.0)
+            right = np.where(pos >= size - self.pml, (1 - (size - 1 - pos - stagger) / self.pml) ** 4, 0.0)
+            return (left + right) * 2 * C0 / self.dx
+
+        self.arx = (1 - self.dt * damping(xx, self.w, 0.0))[None, :]
+        self.arz = (1 - self.dt * damping(zz, self.h, 0.0))[:, None]
+        self.avx = (1 - self.dt * damping(xx, self.w, 0.5))[None, :]
+        self.avz = (1 - self.dt * damping(zz, self.h, 0.5))[:, None]
+
+    def coefficients(self, m):
+        """Crop the pml rim, edge-pad it back, then edge-pad into the box (:46-50)."""
+        p = self.pml
+        m = np.asarray(m, dtype=np.float64)
+        core = m[..., p:-p, p:-p]
+        pad = [(0, 0)] * (m.ndim - 2) + [(p, p), (p, p)]
+        m = np.pad(core, pad, mode="edge")
+        pad = [(0, 0)] * (m.ndim - 2) + [(self.pz, self.H - self.h - self.pz), (self.px, self.W - self.w - self.px)]
+        return np.pad(m, pad, mode="edge")
+
+
+class System:
+    """ConditionedFrequencySystem (conditioned_wave.py:52-92)."""
+
+    def __init__(self, g, medium, alpha, frequency):
+        self.g = g
+        self.frequency = float(frequency)
+        cf = g.coefficients(medium)
+        self.c, self.rho = cf[0], cf[1]
+        self.alpha = g.coefficients(np.asarray(alpha)[None])[0]
+        self.a = RHO0 / self.rho
+        self.D = cmath.exp(2j * math.pi * frequency * g.dt)
+        self.q = 1 / self.D
+        self.k0 = 2 * math.sin(math.pi * frequency * g.dt) / (g.dt * C0)
+        self.ax = 2 * self.a * np.roll(self.a, -1, -1) / (self.a + np.roll(self.a, -1, -1)) * g.mask
+        self.az = 2 * self.a * np.roll(self.a, -1, -2) / (self.a + np.roll(self.a, -1, -2)) * g.mask
+        eta = [self.alpha * self.c / C0 * v for v in g.spec["eta_per_unit_alpha"]]
+        amplitude = (1 + (eta[0] + eta[1]) / 2) ** 2
+        hsum = sum(e / (amplitude * (1 + t * (self.D - 1) / g.dt)) for e, t in zip(eta, g.spec["taus_s"]))
+        modulus = amplitude * (1 - hsum) / (1 - (1 - self.q) * hsum)
+        self.mass = self.a * (C0 / self.c) ** 2 / modulus
+        self.svx = (self.D - 1) / (self.D - g.avx)
+        self.svz = (self.D - 1) / (self.D - g.avz)
+        self.gvx = 1 + (1 - g.avx) * self.q
+        self.gvz = 1 + (1 - g.avz) * self.q
+        self.mx = (g.arx != 1).astype(np.float64)
+        self.mz = (g.arz != 1).astype(np.float64)
+        self.scalar = (self.D - 1) / self.D / self.k0 ** 2
+
+    # ---- forward -----------------------------------------------------------
+    def apply_fft(self, p):
+        """__call__ (conditioned_wave.py:71-80)."""
+        g = self.g
+        spectrum = np.fft.fft2(p)
+        x = self.ax * self.svx * np.fft.ifft2(g.Dx * spectrum)
+        z = self.az * self.svz * np.fft.ifft2(g.Dz * spectrum)
+
+        def part(flux, gain, mask, ar, sym):
+            regular = np.fft.ifft2(-np.conj(sym) * np.fft.fft2(flux))
+            provisional = np.fft.ifft2(-np.conj(sym) * np.fft.fft2(gain * flux))
+            return mask / (self.D - ar) * regular + (1 + (1 - mask) / (self.D - ar)) * provisional
+
+        spatial = self.scalar * (part(x, self.gvx, self.mx, g.arx, g.Dx) + part(z, self.gvz, self.mz, g.arz, g.Dz))
+        return g.mask * (spatial + self.mass * p) + (1 - g.mask) * p
+
+    def _axes(self):
+        g = self.g
+        return ((-1, self.ax * self.svx, self.gvx, self.mx / (self.D - g.arx), 1 + (1 - self.mx) / (self.D - g.arx)),
+                (-2, self.az * self.svz, self.gvz, self.mz / (self.D - g.arz), 1 + (1 - self.mz) / (self.D - g.arz)))
+
+    def apply(self, p):
+        """The same operator with real-space 8-tap staggered differences."""
+        g = self.g
+        spatial = 0
+        for axis, f, gain, left, right in self._axes():
+            flux = f * _dplus(p, axis, g.dx)
+            spatial = spatial + left * _dminus(flux, axis, g.dx) + right * _dminus(gain * flux, axis, g.dx)
+        return g.mask * (self.scalar * spatial + self.mass * p) + (1 - g.mask) * p
+
+    def projected(self, p):
+        """dense_wave.projected_operator (:66-68)."""
+        m = self.g.mask
+        return m * self.apply(m * p) + (1 - m) * p
+
+    def adjoint(self, p):
+        """dense_wave.adjoint_operator (:70-82) with real-space differences:
+        per axis D-( conj(ax sv) [D+(conj(left) w) + conj(gain) D+(conj(right) w)] ),
+        w = conj(scalar) mask p."""
+        g = self.g
+        y = g.mask * p
+        w = np.conj(self.scalar) * y
+        value = np.conj(self.mass) * y
+        for axis, f, gain, left, right in self._axes():
+            flux = np.conj(f) * (_dplus(np.conj(left) * w, axis, g.dx) + np.conj(gain) * _dplus(np.conj(right) * w,
+                                                                                                 axis, g.dx))
+            value = value + _dminus(flux, axis, g.dx)
+        return g.mask * value + (1 - g.mask) * p
+
+    def adjoint_fft(self, p):
+        """The reference's Fourier form of the adjoint (dense_wave.py:70-82)."""
+        g = self.g
+        y = g.mask * p
+        weighted = np.conj(self.scalar) * y
+
+        def axis_term(ax, sv, gain, mask, ar, sym):
+            left = mask / (self.D - ar)
+            right = 1 + (1 - mask) / (self.D - ar)
+            first = np.fft.ifft2(-sym * np.fft.fft2(np.conj(left) * weighted))
+            second = np.fft.ifft2(-sym * np.fft.fft2(np.conj(right) * weighted))
+            flux = np.conj(ax * sv) * (first + np.conj(gain) * second)
+            return np.fft.ifft2(np.conj(sym) * np.fft.fft2(flux))
+
+        value = (axis_term(self.ax, self.svx, self.gvx, self.mx, g.arx, g.Dx)
+                 + axis_term(self.az, self.svz, self.gvz, self.mz, g.arz, g.Dz) + np.conj(self.mass) * y)
+        return g.mask * value + (1 - g.mask) * p
+
+    def fixed_multiplier(self, shift=0.15):
+        """Fourier multiplier of the fixed preconditioner (conditioned_wave.py:90-92)."""
+        return 1 / (1 + self.g.lap / self.k0 ** 2 - 1j * shift)
+
+
+def source_rhs(system, line):
+    """dense_wave.source_rhs (:59-62): line [..., w] -> [..., H, W]."""
+    g = system.g
+    zi, xi = g.spec["source_c_lookup_zx"]
+    source_c = system.c[zi + g.pz, xi + g.px]
+    line = np.asarray(line)
+    profile = np.zeros(line.shape[:-1] + (g.H, g.W), dtype=np.complex128)
+    profile[..., g.face, g.px:g.px + g.w] = line
+    return profile * system.a * C0 ** 2 * 2 * g.dt / (source_c * g.dx * (system.D - 1))
+
+
+def _dplus(u, axis, dx):
+    return sum(a * (np.roll(u, -(m + 1), axis) - np.roll(u, m, axis)) for m, a in enumerate(FD)) / dx
+
+
+def _dminus(g, axis, dx):
+    return sum(a * (np.roll(g, -m, axis) - np.roll(g, m + 1, axis)) for m, a in enumerate(FD)) / dx

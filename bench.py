@@ -64,7 +64,10 @@ def parse():
     ap.add_argument("--roofline-kernel", default="dcgs_dots",
                     choices=["dcgs_dots", "dcgs_update", "operator_apply", "true_residual"],
                     help="kernel timed with HIP events inside the timed solves for the roofline object")
-    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies"],
+    ap.add_argument("--cond-shape", type=int, nargs=2, default=[960, 1216],
+                    help="native medium of the conditioned workload (box 1024 x 1280)")
+    ap.add_argument("--cond-tol", type=float, default=1e-5)
+    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
                          "64 frequencies x 11 angles on uneven grids")
     ap.add_argument("--frequencies", type=int, default=64)
@@ -217,7 +220,7 @@ def pmc_summary(directory):
         scale[counter] = COPY_BYTES / (sum(big) / len(big))
     short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_update": "k_dcgs_update", "k_apply<float": "k_apply<float>",
              "k_mgt_jacobi": "k_mgt_jacobi", "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
-             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail"}
+             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail", "k_cond8<float": "k_cond8<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
     for counter, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench")):
         for name, v in _pmc_rows(os.path.join(directory, sub), counter):
@@ -277,6 +280,117 @@ def frequency_sweep(a):
     print(json.dumps(line), flush=True)
 
 
+def conditioned_workload(a):
+    """The dense path's conditioned split-PML system (conditioned_wave.py,
+    dense_wave.solve_system): native medium 960 x 1216 at dx = 50 um in a
+    1024 x 1280 box, 24-cell split PML, dt 2.5 ns, bin 360 (5.997 MHz),
+    seeded smooth random c, rho and alpha, 11 plane-wave source lines;
+    multigrid FGMRES(24) with FP64 refinement to 1e-5 on every angle.  A step
+    = coefficients + multigrid set-up + source term + solve.  One JSON line,
+    with the FP32 forward/adjoint operator kernel against the HBM roofline and
+    the oracle's Fourier-form operator (the reference's algorithm, 11 FFTs per
+    apply, numpy complex128, 1 thread) timed beside it."""
+    import numpy as np
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from nsrno_amd.conditioned import ConditionedFrequencySystem, FrequencyGeometry, solve_system, source_rhs
+    h, w = a.cond_shape
+    dx = 50e-6
+    spec = {"medium_shape": [2, h, w], "dx_m": dx, "x0_m": -w * dx / 2, "z0_m": -30 * dx, "native_dt_s": 2.5e-9,
+            "pml": 24, "eta_per_unit_alpha": [.015, .025], "taus_s": [12e-9, 35e-9], "source_c_lookup_zx": [30, 24]}
+    rng = np.random.default_rng(a.seed)
+
+    def smooth():
+        f = gaussian_filter(rng.normal(size=(h, w)), 8.0)
+        return f / np.abs(f).max()
+
+    c = 1540.0 + 40.0 * smooth()
+    rho = 1000.0 + 50.0 * smooth()
+    alpha = 0.5 + 0.2 * smooth()
+    f = 360 * 40e6 / 2401
+    R = a.angles
+    g = FrequencyGeometry(spec)
+    medium = torch.tensor(np.stack((c, rho)), dtype=torch.float64, device="cuda")
+    alpha_d = torch.tensor(alpha, dtype=torch.float64, device="cuda")
+    system = ConditionedFrequencySystem(g, medium, alpha_d, f, nrhs=R, restart=24)
+    x = (np.arange(w) - (w - 1) / 2) * dx
+    taper = np.hanning(w)
+    lines = np.stack([taper * np.exp(-2j * np.pi * f * x * np.sin(np.deg2rad(t)) / 1540.0)
+                      for t in np.linspace(-15.0, 15.0, R)])
+    lines_d = torch.from_numpy(lines).cuda()
+    print(f"[bench] conditioned box {g.H}x{g.W} medium {h}x{w} R={R} f={f / 1e6:.3f} MHz", flush=True)
+
+    def step():
+        system.update(medium, alpha_d)
+        b = source_rhs(system, lines_d)
+        return solve_system(system, b, tolerance=a.cond_tol, mixed=True, max_cycles=400)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        u, info = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"[bench] conditioned solve {dt / a.steps * 1e3:.1f} ms/step, {info}", flush=True)
+    # operator kernel alone: FP32 forward and adjoint over the R fields
+    v = torch.randn(R, g.H, g.W, dtype=torch.complex64, device="cuda")
+    out = torch.empty_like(v)
+    n = g.H * g.W
+    alg = (16.0 + 16.0 / R) * n * R
+    kern = {}
+    for name, adj in (("forward", False), ("adjoint", True)):
+        for _ in range(3):
+            system.plan.apply(v, adjoint=adj, out=out)
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        reps = 30
+        for _ in range(reps):
+            system.plan.apply(v, adjoint=adj, out=out)
+        e1.record(s)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / reps
+        kern[name] = {"avg_launch_us": round(t * 1e6, 2), "gb_s": round(alg / t / 1e9, 1),
+                      "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4)}
+    cpu = None
+    if not a.no_cpu:
+        from oracle import conditioned as oc
+        o = oc.System(oc.Geometry(spec), np.stack((c, rho)), alpha, f)
+        field = (rng.normal(size=(g.H, g.W)) + 1j * rng.normal(size=(g.H, g.W))) * o.g.mask
+        o.apply_fft(field)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 10.0:
+            o.apply_fft(field)
+            k += 1
+        tc = (time.perf_counter() - t0) / k
+        cpu = {"value": round(1.0 / tc, 4), "unit": "operator applies/s (one field)", "cores": 1, "kind": "port",
+               "sample": f"{k} applies of the Fourier-form operator (conditioned_wave.py:71-80, 11 FFTs, numpy "
+                         f"complex128) on the same {g.H}x{g.W} box, {tc * 1e3:.0f} ms each",
+               "gpu_same_unit": round(R / (kern["forward"]["avg_launch_us"] * 1e-6), 1)}
+    line = {
+        "metric": f"solves/s, conditioned split-PML system {g.H}x{g.W} x {R} angles to rel-res {a.cond_tol:g}",
+        "value": round(R * a.steps / dt, 4), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (FP64 reductions/refinement)",
+        "data": "synthetic (seeded smooth random c, rho, alpha)",
+        "config": {"workload": f"conditioned {h}x{w} medium, box {g.H}x{g.W}, 24-cell split PML, "
+                               f"{f / 1e6:.3f} MHz x{R} angles", "tolerance": a.cond_tol,
+                   "method": "fgmres(24) + multigrid", "parallelism": "single GPU"},
+        "steps_per_solve": info["steps"], "relative_residual": info["residual"],
+        "max_angle_residual": info["max_angle_residual"], "reason": info["reason"],
+        "roofline": {"bound": "hbm", "kernel": "k_cond8<float> forward (all RHS)",
+                     "achieved": kern["forward"]["gb_s"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": kern["forward"]["frac"], "traffic": traffic_from_profile("k_cond8<float>"),
+                     "avg_launch_us": kern["forward"]["avg_launch_us"], "algorithmic_bytes_per_launch": int(alg)},
+        "adjoint_kernel": kern["adjoint"], "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    system.close()
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -330,6 +444,9 @@ def main():
         return
     if a.workload == "frequencies":
         frequency_sweep(a)
+        return
+    if a.workload == "conditioned":
+        conditioned_workload(a)
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)

@@ -304,6 +304,20 @@ int32_t helm_set_medium_conditioned(helm_plan* plan, const double* c, const doub
 int32_t helm_conditioned_source(helm_plan* plan, const double* c, const double* rho, const void* line,
                                 int32_t face, int32_t source_z, int32_t source_x, void* rhs, int32_t nrhs,
                                 void* stream);
+/* Material gradient of the implicit adjoint (replaces the autograd step of
+ * dense_wave.ImplicitChannels.backward, nsrno/dense_wave.py:250-255):
+ *   G(c, rho) = -Re sum_r <lam_r, M (A(c, rho) v_r - s_r(c, rho))>,
+ * v = M (u + pref) (the library applies M), lam the adjoint fields, s the
+ * source term of helm_conditioned_source for `line` (NULL: no source term);
+ * v, lam c128 [nrhs][H][W], line c128 [nrhs][w].  grad_c, grad_rho: device
+ * float64 [h][w], overwritten with dG/dc and dG/drho of the native maps
+ * (through the face means, the relaxation modulus, the source term and the
+ * crop/replicate of the rim); alpha is held fixed.  c, rho, alpha: the maps
+ * given to helm_set_medium_conditioned. */
+int32_t helm_conditioned_gradient(helm_plan* plan, const double* c, const double* rho, const double* alpha,
+                                  const void* v, const void* lam, const void* line, int32_t nrhs, int32_t face,
+                                  int32_t source_z, int32_t source_x, double* grad_c, double* grad_rho,
+                                  void* stream);
 
 /* ---- warm start: `steps` minimal-residual steps per right-hand side,
  * d = M^{-1} r, alpha = <A d, r>/|A d|^2, x += alpha d (the residual never

@@ -105,6 +105,8 @@ SIGNATURES = {
                                               POINTER(HelmConditionedMedium), c_void_p]),
     "helm_conditioned_source": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
                                           c_void_p, c_int32, c_void_p]),
+    "helm_conditioned_gradient": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
 }
 
 _LIB = None

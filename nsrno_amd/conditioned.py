@@ -93,11 +93,6 @@ class ConditionedFrequencySystem:
         self.g = g
         self.frequency = float(frequency)
         dev = g.device
-        medium = torch.as_tensor(medium, device=dev).to(torch.float64)
-        alpha = torch.as_tensor(alpha, device=dev).to(torch.float64)
-        self._c = medium[0].contiguous()
-        self._rho = medium[1].contiguous()
-        self._alpha = alpha.contiguous()
         self.D = complex(math.cos(2 * math.pi * self.frequency * g.dt), math.sin(2 * math.pi * self.frequency * g.dt))
         self.q = 1 / self.D
         self.k0 = 2 * math.sin(math.pi * self.frequency * g.dt) / (g.dt * C0)
@@ -107,11 +102,24 @@ class ConditionedFrequencySystem:
         self._med = _lib.HelmConditionedMedium(g.h, g.w, g.pz, g.px, int(g.pml), float(g.dt), RHO0,
                                                (ctypes.c_double * 2)(*spec["eta_per_unit_alpha"]),
                                                (ctypes.c_double * 2)(*spec["taus_s"]))
-        check(lib().helm_set_medium_conditioned(self.plan._h, _ptr(self._c), _ptr(self._rho), _ptr(self._alpha),
-                                                ctypes.byref(self._med), _stream()), "helm_set_medium_conditioned")
+        self.update(medium, alpha)
         mg = dict(levels=_mg_levels(max(g.H, g.W)), shift=0.5, pre=2, post=2, coarse_sweeps=10)
         mg.update(multigrid or {})
         self.plan.setup_multigrid(**mg)
+
+    def update(self, medium, alpha):
+        """New c, rho, alpha maps on the same plan (coefficients and multigrid
+        rebuilt on the device; no reallocation)."""
+        g = self.g
+        if tuple(medium.shape) != (2, g.h, g.w):
+            raise ValueError("Expected native [2,z,x] medium")
+        medium = torch.as_tensor(medium, device=g.device).to(torch.float64)
+        alpha = torch.as_tensor(alpha, device=g.device).to(torch.float64)
+        self._c = medium[0].contiguous()
+        self._rho = medium[1].contiguous()
+        self._alpha = alpha.contiguous()
+        check(lib().helm_set_medium_conditioned(self.plan._h, _ptr(self._c), _ptr(self._rho), _ptr(self._alpha),
+                                                ctypes.byref(self._med), _stream()), "helm_set_medium_conditioned")
         box = g.coefficients(medium)
         self.c, self.rho = box[0], box[1]
         self.alpha = g.coefficients(alpha[None])[0]
@@ -129,6 +137,22 @@ class ConditionedFrequencySystem:
     def source(self, line):
         """conditioned_wave.py:82-88: one line [w] -> [H, W]."""
         return source_rhs(self, line[None])[0]
+
+    def fixed_preconditioner(self, shift=.15):
+        """conditioned_wave.py:90-92: r -> M ifft2(fft2(r) / (1 + lap/k0^2 - i shift)),
+        lap = -(|Dx|^2 + |Dz|^2) of the 8-tap symbols.  The Fourier transforms
+        are the platform FFT library's (torch.fft on the device); the solves
+        here use the multigrid preconditioner instead."""
+        g = self.g
+        coeff = (1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168)
+
+        def symbol(n):
+            q = 2 * math.pi * torch.fft.fftfreq(n, dtype=torch.float64, device=g.device)
+            return 2j * sum(a * torch.sin((j + .5) * q) for j, a in enumerate(coeff)) * torch.exp(.5j * q) / g.dx
+
+        lap = -(symbol(g.W).abs().square()[None, :] + symbol(g.H).abs().square()[:, None])
+        multiplier = 1 / (1 + lap / self.k0 ** 2 - 1j * shift)
+        return lambda r: g.mask * torch.fft.ifft2(multiplier * torch.fft.fft2(r))
 
 
 def source_rhs(system, line):
@@ -196,10 +220,118 @@ def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5
     residual = system.plan.apply(x, adjoint=adjoint) - b
     relative = float(residual.norm() / b.norm().clamp_min(1e-30))
     individual = residual.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1).clamp_min(1e-30)
-    reason = "converged" if all(i.reason == "converged" for i in infos) and relative < tolerance else \
-        ("converged" if relative < tolerance else "iteration_budget")
+    reason = "converged" if relative < tolerance else next(
+        (i.reason for i in infos if i.reason != "converged"), "iteration_budget")
     info = {"reason": reason, "steps": max(i.iterations for i in infos), "residual": relative,
             "max_angle_residual": float(individual.max()), "seconds": time.perf_counter() - start,
             "dtype": "mixed FP32 inverse / FP64 residual" if mixed else str(dtype), "method": method,
             "refinements": max(i.cycles for i in infos)}
     return x.reshape(rhs.shape).to(dtype), info
+
+
+def receiver_response(frequency, dt=2.5e-9):
+    """conditioned_data.receiver_response (conditioned_data.py:10-24): the
+    squared magnitude of a 4th-order Butterworth band-pass (4-7.5 MHz, zero
+    phase) at `frequency` times the real, delay-compensated response of the
+    201-tap Kaiser(5) low-pass FIR (cutoff 1/10) of the 10x resampler."""
+    from scipy.signal import butter, firwin, freqz, sosfreqz
+    band = butter(4, [4e6, 7.5e6], fs=1 / dt, btype="bandpass", output="sos")
+    _, hb = sosfreqz(band, worN=[frequency], fs=1 / dt)
+    taps = firwin(201, 0.1, window=("kaiser", 5.0))
+    w = 2 * np.pi * frequency * dt
+    _, hf = freqz(taps, worN=[w])
+    return float(abs(hb[0]) ** 2 * (hf[0] * np.exp(1j * w * 100)).real)
+
+
+class DenseCase:
+    """One frequency and angle set of the dense path: what DenseData.assemble
+    supplies (dense_wave.py:46-57) -- geometry, frequency, source lines
+    [R, w], reference fields pref [R, H, W] (zero when None) and the uniform
+    reference system (c 1540, rho 1000, alpha .002).  The system's plan is
+    created once and updated with each new medium."""
+
+    def __init__(self, geometry, frequency, lines, pref=None, *, restart=24, multigrid=None):
+        g = geometry
+        self.g = g
+        self.frequency = float(frequency)
+        self.lines = torch.as_tensor(lines, device=g.device).to(torch.complex128).contiguous()
+        R = self.lines.shape[0]
+        self.pref = (torch.zeros((R, g.H, g.W), dtype=torch.complex128, device=g.device) if pref is None
+                     else torch.as_tensor(pref, device=g.device).to(torch.complex128).contiguous())
+        uniform = torch.empty((2, g.h, g.w), dtype=torch.float64, device=g.device)
+        uniform[0] = C0
+        uniform[1] = RHO0
+        self.reference_system = ConditionedFrequencySystem(g, uniform, torch.full((g.h, g.w), .002, dtype=torch.float64,
+                                                                                  device=g.device),
+                                                           self.frequency, nrhs=R, restart=restart,
+                                                           multigrid=multigrid)
+        self.restart, self.multigrid, self.system = restart, multigrid, None
+
+    def assemble(self, medium, alpha):
+        if self.system is None:
+            self.system = ConditionedFrequencySystem(self.g, medium, alpha, self.frequency, nrhs=self.lines.shape[0],
+                                                     restart=self.restart, multigrid=self.multigrid)
+        else:
+            self.system.update(medium, alpha)
+        rhs = scattered_rhs(self.system, self.reference_system, self.pref, self.lines)
+        return self.system, rhs
+
+    def close(self):
+        for s in (self.system, self.reference_system):
+            if s is not None:
+                s.close()
+
+
+def material_gradient(system, v, lam, line=None):
+    """d/d(c, rho) of -Re sum_r <lam_r, M (A v_r - s_r)> on the native maps
+    (helm_conditioned_gradient); v = u + pref, lam: c128 [R, H, W]."""
+    g = system.g
+    v = v.to(torch.complex128).contiguous()
+    lam = lam.to(torch.complex128).contiguous()
+    R = v.shape[0]
+    if lam.shape != v.shape:
+        raise ValueError("v and lam must have the same shape")
+    ln = None if line is None else torch.as_tensor(line, device=g.device).to(torch.complex128).contiguous()
+    grad = torch.empty((2, g.h, g.w), dtype=torch.float64, device=g.device)
+    zi, xi = g.spec["source_c_lookup_zx"]
+    check(lib().helm_conditioned_gradient(system.plan._h, _ptr(system._c), _ptr(system._rho), _ptr(system._alpha),
+                                          _ptr(v), _ptr(lam), None if ln is None else _ptr(ln), int(R), int(g.face),
+                                          int(zi), int(xi), _ptr(grad[0]), _ptr(grad[1]), _stream()),
+          "helm_conditioned_gradient")
+    return grad
+
+
+class ImplicitChannels(torch.autograd.Function):
+    """dense_wave.ImplicitChannels (:227-256): receiver channels of the
+    converged forward solve times the receiver response; backward solves the
+    adjoint system with the channel gradient injected at the receivers and
+    forms the material gradient of -Re <lam, M (A (u + pref) - s)> on the GPU
+    (alpha fixed).  Non-convergence raises RuntimeError."""
+
+    @staticmethod
+    def forward(ctx, medium, alpha, case, tolerance):
+        system, rhs = case.assemble(medium.detach().double(), alpha.detach().double())
+        u, info = solve_system(system, rhs, tolerance=tolerance, mixed=True)
+        if info["reason"] != "converged":
+            raise RuntimeError("Forward failed to converge: " + str(info))
+        ctx.case, ctx.tolerance, ctx.info = case, tolerance, info
+        ctx.save_for_backward(medium, alpha, u)
+        return channels(case.g, u) * receiver_response(case.frequency)
+
+    @staticmethod
+    def backward(ctx, gradient):
+        medium, alpha, u = ctx.saved_tensors
+        case = ctx.case
+        g = case.g
+        system, _ = case.assemble(medium.detach().double(), alpha.detach().double())
+        rhs = torch.zeros_like(u)
+        rhs[..., g.face, g.rx_cols] = gradient.to(torch.complex128) * receiver_response(case.frequency)
+        lam, info = solve_system(system, rhs, adjoint=True, tolerance=ctx.tolerance, mixed=True)
+        if info["reason"] != "converged":
+            raise RuntimeError("Adjoint failed to converge: " + str(info))
+        grad = material_gradient(system, u + case.pref, lam, case.lines)
+        return grad.to(medium.dtype), None, None, None
+
+
+def implicit_channels(medium, alpha, case, tolerance=1e-5):
+    return ImplicitChannels.apply(medium, alpha, case, tolerance)

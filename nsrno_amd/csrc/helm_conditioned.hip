@@ -594,6 +594,140 @@ int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool ad
     return HELM_OK;
 }
 
+// ---- material gradient (dense_wave.py:250-255) ---------------------------
+// For G = -Re sum_r sum_i conj(lam_ri) M_i ((A v_r)_i - s_ri), v = M (u + pref):
+//   S_ax[q]  = dG-part / d ax_q = -sum_r sv_x (D+x v_r)_q [(D+x yA_r)_q + (g_x-1)_q (D+x yB_r)_q],
+//   yA = conj(lam) M cA, yB = conj(lam) M cB   (D-^T = -D+), likewise S_az,
+//   S_m[q]   = sum_r conj(lam_rq) M_q v_rq   (coefficient of mass_q),
+//   T[q]     = sum_r conj(lam_rq) M_q line_r K / c_src on the source row (coefficient of a_q in s).
+__global__ void k_cond_sens(int H, int W, int64_t n, int nrhs, const double2* __restrict__ tx,
+                            const double2* __restrict__ tz, int bz0, int bx0, int bh, int bw, double c0, double c1,
+                            double c2, double c3, const double2* __restrict__ v, const double2* __restrict__ lam,
+                            double2* s_ax, double2* s_az, double2* s_m) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= W) return;
+    auto in = [&](int zi, int xj) { return zi >= bz0 && zi < bz0 + bh && xj >= bx0 && xj < bx0 + bw; };
+    const double cf[4] = {c0, c1, c2, c3};
+    double2 sax = make_double2(0, 0), saz = make_double2(0, 0), sm = make_double2(0, 0);
+    const double2 svx = tx[j], gx = tx[W + j], svz = tz[i], gz = tz[H + i];
+    for (int r = 0; r < nrhs; ++r) {
+        const double2* vr = v + (int64_t)r * n;
+        const double2* lr = lam + (int64_t)r * n;
+        auto vm = [&](int zi, int xj) {
+            zi = cwrap(zi, H);
+            xj = cwrap(xj, W);
+            return in(zi, xj) ? vr[(int64_t)zi * W + xj] : make_double2(0, 0);
+        };
+        // y = conj(lam) M coef at a wrapped node
+        auto yl = [&](int zi, int xj, double2 coef) {
+            zi = cwrap(zi, H);
+            xj = cwrap(xj, W);
+            if (!in(zi, xj)) return make_double2(0, 0);
+            return cmul(cconj(lr[(int64_t)zi * W + xj]), coef);
+        };
+        double2 dvx = make_double2(0, 0), dvz = dvx, dax = dvx, dbx = dvx, daz = dvx, dbz = dvx;
+        for (int m = 0; m < 4; ++m) {
+            const int jp = cwrap(j + m + 1, W), jm = cwrap(j - m, W);
+            const int ip = cwrap(i + m + 1, H), im = cwrap(i - m, H);
+            dvx = cadd(dvx, cscale(csub(vm(i, jp), vm(i, jm)), cf[m]));
+            dvz = cadd(dvz, cscale(csub(vm(ip, j), vm(im, j)), cf[m]));
+            dax = cadd(dax, cscale(csub(yl(i, jp, tx[2 * W + jp]), yl(i, jm, tx[2 * W + jm])), cf[m]));
+            dbx = cadd(dbx, cscale(csub(yl(i, jp, tx[3 * W + jp]), yl(i, jm, tx[3 * W + jm])), cf[m]));
+            daz = cadd(daz, cscale(csub(yl(ip, j, tz[2 * H + ip]), yl(im, j, tz[2 * H + im])), cf[m]));
+            dbz = cadd(dbz, cscale(csub(yl(ip, j, tz[3 * H + ip]), yl(im, j, tz[3 * H + im])), cf[m]));
+        }
+        sax = csub(sax, cmul(cmul(svx, dvx), cadd(dax, cmul(gx, dbx))));
+        saz = csub(saz, cmul(cmul(svz, dvz), cadd(daz, cmul(gz, dbz))));
+        if (in(i, j)) sm = cadd(sm, cmul(cconj(lr[(int64_t)i * W + j]), vr[(int64_t)i * W + j]));
+    }
+    const int64_t o = (int64_t)i * W + j;
+    s_ax[o] = sax;
+    s_az[o] = saz;
+    s_m[o] = sm;
+}
+
+// chain rule to the native maps: box node q takes native node map(q) (crop +
+// replicate), a = rho0/rho; grad_a is accumulated into grad_rho and turned into
+// d/d rho by k_cond_grad_rho.
+__global__ void k_cond_grad(CondGeom g, int nrhs, int face, int zsrc, int xsrc, double dx,
+                            const double* __restrict__ c, const double* __restrict__ rho,
+                            const double* __restrict__ alpha, const double2* __restrict__ s_ax,
+                            const double2* __restrict__ s_az, const double2* __restrict__ s_m,
+                            const double2* __restrict__ lam, const double2* __restrict__ line, double* grad_c,
+                            double* grad_a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= g.W) return;
+    auto in = [&](int zi, int xj) { return zi >= g.pz && zi < g.pz + g.h && xj >= g.px && xj < g.px + g.w; };
+    auto nat = [&](int zi, int xj) {
+        zi = cwrap(zi, g.H);
+        xj = cwrap(xj, g.W);
+        return (int64_t)cond_src(zi, g.pz, g.h, g.pml) * g.w + cond_src(xj, g.px, g.w, g.pml);
+    };
+    auto aat = [&](int zi, int xj) { return g.rho0 / rho[nat(zi, xj)]; };
+    auto d1 = [](double a, double b) { return 2.0 * b * b / ((a + b) * (a + b)); };  // d/da 2ab/(a+b)
+    auto d2 = [](double a, double b) { return 2.0 * a * a / ((a + b) * (a + b)); };  // d/db
+    const int64_t q = (int64_t)i * g.W + j;
+    const int jw = cwrap(j - 1, g.W), iw = cwrap(i - 1, g.H);
+    const double a = aat(i, j);
+    double ga = 0.0;
+    // faces q+1/2 (a_q first) and q-1/2 (a_q second); the face coefficients carry M of their node
+    if (in(i, j)) {
+        ga -= s_ax[q].x * d1(a, aat(i, j + 1)) + s_az[q].x * d1(a, aat(i + 1, j));
+    }
+    if (in(i, jw)) ga -= s_ax[(int64_t)i * g.W + jw].x * d2(aat(i, jw), a);
+    if (in(iw, j)) ga -= s_az[(int64_t)iw * g.W + j].x * d2(aat(iw, j), a);
+    double gc = 0.0;
+    if (in(i, j)) {
+        // mass = a c0^2 / (c^2 K(c)), K the relaxation modulus
+        const double cv = c[nat(i, j)], al = alpha ? alpha[nat(i, j)] : 0.0;
+        const double e0 = al * cv / g.c0 * g.epa0, e1 = al * cv / g.c0 * g.epa1;
+        const double de0 = al / g.c0 * g.epa0, de1 = al / g.c0 * g.epa1;
+        const double ah = 1.0 + (e0 + e1) / 2.0, A = ah * ah, dA = ah * (de0 + de1);
+        const double2 dm1 = make_double2(g.D.x - 1.0, g.D.y);
+        const double2 dk0 = make_double2(1.0 + g.tau0 * dm1.x / g.dt, g.tau0 * dm1.y / g.dt);
+        const double2 dk1 = make_double2(1.0 + g.tau1 * dm1.x / g.dt, g.tau1 * dm1.y / g.dt);
+        const double2 i0 = zdiv(make_double2(1.0, 0.0), dk0), i1 = zdiv(make_double2(1.0, 0.0), dk1);
+        const double2 hs = cadd(cscale(i0, e0 / A), cscale(i1, e1 / A));
+        const double2 dh = cadd(cscale(i0, (de0 * A - e0 * dA) / (A * A)), cscale(i1, (de1 * A - e1 * dA) / (A * A)));
+        const double2 qd = zdiv(make_double2(1.0, 0.0), g.D);
+        const double2 one = make_double2(1.0, 0.0);
+        const double2 den = csub(one, cmul(make_double2(1.0 - qd.x, -qd.y), hs));
+        const double2 omh = csub(one, hs);
+        const double2 K = zdiv(cscale(omh, A), den);
+        const double2 dK = csub(zdiv(cscale(omh, dA), den), zdiv(cscale(cmul(qd, dh), A), cmul(den, den)));
+        const double2 iK = zdiv(one, K);
+        const double c2 = g.c0 * g.c0;
+        const double2 dm_da = cscale(iK, c2 / (cv * cv));
+        // d mass / dc = a c0^2 (-2/(c^3 K) - dK/(c^2 K^2))
+        const double2 dm_dc = cscale(csub(cscale(iK, -2.0 / (cv * cv * cv)), cscale(cmul(dK, cmul(iK, iK)), 1.0 / (cv * cv))),
+                                     a * c2);
+        const double2 sm = s_m[q];
+        ga -= sm.x * dm_da.x - sm.y * dm_da.y;  // Re(s_m dm/da)
+        gc -= sm.x * dm_dc.x - sm.y * dm_dc.y;
+        if (line && i == face) {
+            // T_q = sum_r conj(lam) line K / c_src,  K = c0^2 2 dt / (dx (D - 1))
+            const double csrc = c[(int64_t)cond_src(zsrc + g.pz, g.pz, g.h, g.pml) * g.w +
+                                  cond_src(xsrc + g.px, g.px, g.w, g.pml)];
+            const double2 Kc = zdiv(make_double2(c2 * 2.0 * g.dt, 0.0), make_double2(dx * dm1.x, dx * dm1.y));
+            double2 t = make_double2(0, 0);
+            for (int r = 0; r < nrhs; ++r)
+                t = cadd(t, cmul(cconj(lam[(int64_t)r * g.H * g.W + q]), line[(int64_t)r * g.w + (j - g.px)]));
+            t = cscale(cmul(t, Kc), 1.0 / csrc);
+            ga += t.x;
+            atomicAdd(&grad_c[(int64_t)cond_src(zsrc + g.pz, g.pz, g.h, g.pml) * g.w +
+                              cond_src(xsrc + g.px, g.px, g.w, g.pml)],
+                      -a * t.x / csrc);
+        }
+    }
+    if (ga != 0.0) atomicAdd(&grad_a[nat(i, j)], ga);
+    if (gc != 0.0) atomicAdd(&grad_c[nat(i, j)], gc);
+}
+
+__global__ void k_cond_grad_rho(int64_t n, double rho0, const double* __restrict__ rho, double* grad) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) grad[k] *= -rho0 / (rho[k] * rho[k]);
+}
+
 template <typename T>
 static int32_t c_alloc(T** ptr, size_t count) {
     if (*ptr) return HELM_OK;
@@ -708,6 +842,40 @@ int32_t helm_conditioned_source(helm_plan* p, const double* c, const double* rho
     const CondGeom g = cond_geom(p);
     hipLaunchKernelGGL(k_cond_source, dim3((L.nx + 255) / 256, L.nz, nrhs), dim3(256), 0, (hipStream_t)stream, g,
                        face, source_z, source_x, L.h_m, c, rho, (const double2*)line, (double2*)rhs, nrhs);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+int32_t helm_conditioned_gradient(helm_plan* p, const double* c, const double* rho, const double* alpha,
+                                  const void* v, const void* lam, const void* line, int32_t nrhs, int32_t face,
+                                  int32_t source_z, int32_t source_x, double* grad_c, double* grad_rho,
+                                  void* stream) {
+    HELM_CHECK(p && c && rho && v && lam && grad_c && grad_rho, HELM_ERR_ARG,
+               "helm_conditioned_gradient: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_COND8 && p->medium_set, HELM_ERR_STATE,
+               "helm_conditioned_gradient: call helm_set_medium_conditioned first");
+    const Level& L = p->levels[0];
+    HELM_CHECK(nrhs >= 1 && (int64_t)nrhs * L.n < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_conditioned_gradient: nrhs out of range");
+    HELM_CHECK(!line || (face >= 0 && face < L.nz), HELM_ERR_ARG, "helm_conditioned_gradient: face outside the box");
+    hipStream_t s = (hipStream_t)stream;
+    const CondGeom g = cond_geom(p);
+    if (!p->cond_grad) HELM_HIP(hipMalloc((void**)&p->cond_grad, 3 * L.n * sizeof(double2) + 256));
+    double2* sax = p->cond_grad;
+    double2* saz = sax + L.n;
+    double2* sm = saz + L.n;
+    const size_t nn = (size_t)g.h * g.w;
+    HELM_HIP(hipMemsetAsync(grad_c, 0, nn * sizeof(double), s));
+    HELM_HIP(hipMemsetAsync(grad_rho, 0, nn * sizeof(double), s));
+    const double ih = 1.0 / L.h_m;
+    const dim3 grid((L.nx + 255) / 256, L.nz), block(256);
+    hipLaunchKernelGGL(k_cond_sens, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, (const double2*)L.ctab64,
+                       (const double2*)(L.ctab64 + 4 * L.nx), L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], CD0 * ih,
+                       CD1 * ih, CD2 * ih, CD3 * ih, (const double2*)v, (const double2*)lam, sax, saz, sm);
+    hipLaunchKernelGGL(k_cond_grad, grid, block, 0, s, g, nrhs, face, source_z, source_x, L.h_m, c, rho, alpha, sax,
+                       saz, sm, (const double2*)lam, (const double2*)line, grad_c, grad_rho);
+    hipLaunchKernelGGL(k_cond_grad_rho, dim3((unsigned)((nn + 255) / 256)), block, 0, s, (int64_t)nn, g.rho0, rho,
+                       grad_rho);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

@@ -201,6 +201,7 @@ struct helm_plan {
     helm_conditioned_medium cond{};   // geometry of the last helm_set_medium_conditioned
     float2* cond_scr = nullptr;       // scaled preconditioner input (HELM_OP_COND8)
     int64_t cond_cap = 0;             // elements cond_scr holds
+    double2* cond_grad = nullptr;     // material-gradient scratch: 3 * n c128
 };
 
 namespace helm {

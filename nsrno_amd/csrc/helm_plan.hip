@@ -330,6 +330,7 @@ int32_t helm_plan_destroy(helm_plan* p) {
     if (p->net_part) (void)hipFree(p->net_part);
     if (p->sp_buf) (void)hipFree(p->sp_buf);
     if (p->cond_scr) (void)hipFree(p->cond_scr);
+    if (p->cond_grad) (void)hipFree(p->cond_grad);
     prof_destroy(p->prof);
     delete p;
     return HELM_OK;

@@ -19,12 +19,16 @@ Restates nsrno/conditioned_wave.py:12-92 and nsrno/dense_wave.py:59-82:
                        Fourier form of the adjoint
 * ``source_rhs``       dense_wave.source_rhs (:59-62)
 * ``fixed_multiplier`` the fixed shifted-Laplacian preconditioner (:90-92)
+* ``matrix`` / ``solve_direct``  the projected operator as a sparse matrix (by
+                       probing its radius-7 cross stencil) and its complex128
+                       sparse LU solve: the direct reference for solves
 
 Pinned by (tests/test_oracle_conditioned.py): golden vectors written by
 tests/golden/make_golden_c.py from the reference's own classes (geometry
 factors, system coefficients, the operator, the projected operator, the
 adjoint and the source term on seeded inputs); stencil == Fourier forms;
-<y, A x> = <A^H y, x> for the adjoint pair.
+<y, A x> = <A^H y, x> for the adjoint pair; the adjoint stencil equals the
+conjugate transpose of the probed matrix.
 """
 from __future__ import annotations
 
@@ -187,6 +191,49 @@ class System:
     def fixed_multiplier(self, shift=0.15):
         """Fourier multiplier of the fixed preconditioner (conditioned_wave.py:90-92)."""
         return 1 / (1 + self.g.lap / self.k0 ** 2 - 1j * shift)
+
+
+def matrix(system):
+    """Sparse (CSR) matrix of the projected operator, by probing: the stencil
+    is a cross of radius 7, so 16 x 16 colour classes of unit vectors give
+    every column (box sides must be multiples of 16)."""
+    import scipy.sparse as sp
+    g = system.g
+    H, W = g.H, g.W
+    if H % 16 or W % 16:
+        raise ValueError("probing needs box sides that are multiples of 16")
+    zi, xi = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    rows, cols, vals = [], [], []
+    for cz in range(16):
+        for cx in range(16):
+            e = np.zeros((H, W))
+            e[cz::16, cx::16] = 1.0
+            y = system.projected(e)
+            dx = (cx - xi + 8) % 16 - 8
+            dz = (cz - zi + 8) % 16 - 8
+            in_row = zi % 16 == cz
+            in_col = ~in_row & (xi % 16 == cx)
+            jz = np.where(in_row, zi, (zi + dz) % H)
+            jx = np.where(in_row, (xi + dx) % W, xi)
+            ok = (in_row & (np.abs(dx) <= 7)) | (in_col & (np.abs(dz) <= 7))
+            ok &= y != 0
+            rows.append((zi * W + xi)[ok])
+            cols.append((jz * W + jx)[ok])
+            vals.append(y[ok])
+    n = H * W
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def solve_direct(system, rhs, adjoint=False):
+    """complex128 sparse LU solve of A u = rhs (A^H u = rhs), rhs [R, H, W]."""
+    import scipy.sparse.linalg as sla
+    A = matrix(system)
+    if adjoint:
+        A = A.conj().T
+    lu = sla.splu(A.tocsc())
+    rhs = np.asarray(rhs, dtype=np.complex128)
+    flat = rhs.reshape(rhs.shape[0], -1)
+    return np.stack([lu.solve(b) for b in flat]).reshape(rhs.shape)
 
 
 def source_rhs(system, line):

@@ -53,6 +53,8 @@ def test_golden_apply_adjoint_source():
     assert rel(src, GOLD["src"]) < 1e-13
     # box coefficients (crop + replicate) as the reference's
     np.testing.assert_array_equal(s.c.cpu().numpy(), GOLD["sc"])
+    fixed = s.fixed_preconditioner()(torch.from_numpy(GOLD["r"]).cuda()).cpu().numpy()
+    assert rel(fixed, GOLD["fixed"]) < 1e-12
     s.close()
 
 
@@ -155,3 +157,90 @@ def test_scattered_rhs_matches_oracle():
     assert rel(got, want) < 1e-12
     s.close()
     ref.close()
+
+
+def test_direct_solve_field_accuracy():
+    """Fields of the GPU solve within 1e-8 of the oracle's complex128 sparse LU."""
+    from nsrno_amd.conditioned import solve_system, source_rhs
+    s = golden_system(nrhs=2)
+    o = oc.System(oc.Geometry(SPEC), np.stack((GOLD["c"], GOLD["rho"])), GOLD["alpha"], float(GOLD["f"]))
+    b = source_rhs(s, torch.from_numpy(GOLD["line"]).cuda())
+    u, info = solve_system(s, b, tolerance=1e-11, max_cycles=400)
+    assert info["reason"] == "converged", info
+    ref = oc.solve_direct(o, b.cpu().numpy())
+    assert rel(u.cpu().numpy(), ref) < 1e-8
+    lam, ainfo = solve_system(s, b, adjoint=True, tolerance=1e-11, max_cycles=400)
+    assert rel(lam.cpu().numpy(), oc.solve_direct(o, b.cpu().numpy(), adjoint=True)) < 1e-8
+    s.close()
+
+
+def test_material_gradient_matches_oracle_fd():
+    """helm_conditioned_gradient = d/d(c, rho) of G = -Re sum_r <lam_r, M (A v_r - s_r)> at fixed
+    v, lam (the step of dense_wave.py:250-255), against central differences of the oracle's G."""
+    from nsrno_amd.conditioned import material_gradient
+    s = golden_system(nrhs=2)
+    spec = SPEC
+    g = oc.Geometry(spec)
+    rng = np.random.default_rng(21)
+    v = (rng.normal(size=(2, g.H, g.W)) + 1j * rng.normal(size=(2, g.H, g.W)))
+    lam = (rng.normal(size=(2, g.H, g.W)) + 1j * rng.normal(size=(2, g.H, g.W)))
+    line = GOLD["line"]
+    grad = material_gradient(s, torch.from_numpy(v).cuda(), torch.from_numpy(lam).cuda(),
+                             torch.from_numpy(line).cuda()).cpu().numpy()
+
+    def G(c, rho):
+        o = oc.System(g, np.stack((c, rho)), GOLD["alpha"], float(GOLD["f"]))
+        src = oc.source_rhs(o, line)
+        return -sum(np.vdot(lam[r], g.mask * (o.apply(g.mask * v[r]) - src[r])).real for r in range(2))
+
+    from scipy.ndimage import gaussian_filter
+    for k in range(3):
+        dc = gaussian_filter(rng.normal(size=GOLD["c"].shape), 2.0) * 20.0
+        dr = gaussian_filter(rng.normal(size=GOLD["c"].shape), 2.0) * 20.0
+        if k == 1:
+            dr[:] = 0.0
+        if k == 2:
+            dc[:] = 0.0
+        eps = 1e-4
+        fd = (G(GOLD["c"] + eps * dc, GOLD["rho"] + eps * dr) - G(GOLD["c"] - eps * dc, GOLD["rho"] - eps * dr)) / (2 * eps)
+        an = float(np.sum(grad[0] * dc) + np.sum(grad[1] * dr))
+        assert abs(an - fd) / abs(fd) < 1e-6, (k, an, fd)
+    s.close()
+
+
+def test_implicit_channels_gradient_fd():
+    """dense_wave.ImplicitChannels semantics: autograd through the converged forward solve,
+    adjoint solve and material gradient, against central differences of converged solves."""
+    from nsrno_amd.conditioned import DenseCase, FrequencyGeometry, implicit_channels
+    h, w, dx = 64, 800, 50e-6
+    spec = dict(SPEC, medium_shape=[2, h, w], dx_m=dx, x0_m=-0.0191 - 10 * dx, z0_m=-30 * dx, native_dt_s=2.5e-9,
+                pml=24, source_c_lookup_zx=[30, 12])
+    g = FrequencyGeometry(spec)
+    assert int(g.rx_cols.max()) < g.px + w and int(g.rx_cols.min()) >= g.px
+    rng = np.random.default_rng(31)
+    zz, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    blob = np.exp(-((zz - 40) ** 2 + (xx - 400) ** 2) / (2 * 8.0 ** 2))
+    f = 5.5e6
+    lines = np.stack([np.hanning(w) * np.exp(-2j * np.pi * f * (np.arange(w) - w / 2) * dx * np.sin(t) / 1540.0)
+                      for t in (-0.1, 0.15)])
+    case = DenseCase(g, f, torch.from_numpy(lines).cuda())
+    med = torch.tensor(np.stack((1540.0 + 30.0 * blob, 1000.0 + 40.0 * blob)), dtype=torch.float64, device="cuda")
+    alpha = torch.full((h, w), 0.5, dtype=torch.float64, device="cuda")
+    wts = torch.from_numpy(rng.normal(size=(2, 192)) + 1j * rng.normal(size=(2, 192))).cuda()
+
+    def J(m):
+        ch = implicit_channels(m, alpha, case, tolerance=1e-11)
+        return (wts.conj() * ch).real.sum()
+
+    m = med.clone().requires_grad_(True)
+    J(m).backward()
+    grad = m.grad.detach()
+    dm = torch.zeros_like(med)
+    dm[0] = torch.from_numpy(20.0 * blob).cuda()
+    dm[1] = torch.from_numpy(-15.0 * blob).cuda()
+    eps = 1e-3
+    with torch.no_grad():
+        fd = float((J(med + eps * dm) - J(med - eps * dm)) / (2 * eps))
+    an = float((grad * dm).sum())
+    assert abs(an - fd) / abs(fd) < 1e-4, (an, fd)
+    case.close()

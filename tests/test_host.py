@@ -88,3 +88,23 @@ def test_coefficients_match_reference_golden():
     assert abs(k0 - GOLDEN["k0"]) / GOLDEN["k0"] < 1e-15
     np.testing.assert_allclose(mass, GOLDEN["cell_mass"], rtol=1e-15)
     np.testing.assert_allclose(edge, GOLDEN["edge_stretch"], rtol=1e-14)
+
+
+def test_receiver_response_closed_forms():
+    """conditioned_data.receiver_response restated: |H_bp|^2 of the bilinear Butterworth band-pass
+    (closed form 1/(1 + r^8) in the prewarped frequency: zero-phase filtering squares the magnitude
+    of the order-4 design) times the delay-compensated FIR response."""
+    import numpy as np
+    from scipy.signal import firwin
+    from nsrno_amd.conditioned import receiver_response
+    dt = 2.5e-9
+    fs = 1 / dt
+    taps = firwin(201, 0.1, window=("kaiser", 5.0))
+    for f in (3e6, 4.5e6, 6e6, 7.4e6, 9e6):
+        pre = lambda x: 2 * fs * np.tan(np.pi * x / fs)  # noqa: E731
+        lo, hi, om = pre(4e6), pre(7.5e6), pre(f)
+        ratio = (om ** 2 - lo * hi) / (om * (hi - lo))
+        band2 = 1.0 / (1.0 + ratio ** 8)
+        w = 2 * np.pi * f * dt
+        fir = float(np.sum(taps * np.cos(w * (np.arange(201) - 100))))
+        assert abs(receiver_response(f, dt) - band2 * fir) < 1e-9

@@ -131,3 +131,19 @@ def test_uniform_interior_closed_form():
     expected = (1 - (abs(dx) ** 2 + abs(dz) ** 2) / s.k0 ** 2) * wave
     assert rel(s.apply(wave), expected) < 1e-12
     assert s.mass[5, 5] == pytest.approx(1.0, abs=1e-15)
+
+
+def test_probed_matrix_and_direct_solve(system):
+    g = system.g
+    A = oc.matrix(system)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(g.H, g.W)) + 1j * rng.normal(size=(g.H, g.W))
+    assert rel((A @ x.ravel()).reshape(g.H, g.W), system.projected(x)) < 1e-13
+    # the adjoint stencil is the conjugate transpose of the probed matrix
+    assert rel((A.conj().T @ x.ravel()).reshape(g.H, g.W), system.adjoint(x)) < 1e-13
+    b = rng.normal(size=(2, g.H, g.W)) + 1j * rng.normal(size=(2, g.H, g.W))
+    u = oc.solve_direct(system, b)
+    for r in range(2):
+        assert rel(system.projected(u[r]), b[r]) < 1e-10
+    lam = oc.solve_direct(system, b, adjoint=True)
+    assert rel(system.adjoint(lam[0]), b[0]) < 1e-10

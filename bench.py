@@ -220,7 +220,7 @@ def pmc_summary(directory):
         scale[counter] = COPY_BYTES / (sum(big) / len(big))
     short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_update": "k_dcgs_update", "k_apply<float": "k_apply<float>",
              "k_mgt_jacobi": "k_mgt_jacobi", "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
-             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail", "k_cond8<float": "k_cond8<float>"}
+             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail", "k_cond8s": "k_cond8s<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
     for counter, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench")):
         for name, v in _pmc_rows(os.path.join(directory, sub), counter):
@@ -319,6 +319,8 @@ def conditioned_workload(a):
                       for t in np.linspace(-15.0, 15.0, R)])
     lines_d = torch.from_numpy(lines).cuda()
     print(f"[bench] conditioned box {g.H}x{g.W} medium {h}x{w} R={R} f={f / 1e6:.3f} MHz", flush=True)
+    tuning = system.autotune(R) if a.autotune else None
+    print(f"[bench] autotune {tuning}", flush=True)
 
     def step():
         system.update(medium, alpha_d)
@@ -381,11 +383,11 @@ def conditioned_workload(a):
                    "method": "fgmres(24) + multigrid", "parallelism": "single GPU"},
         "steps_per_solve": info["steps"], "relative_residual": info["residual"],
         "max_angle_residual": info["max_angle_residual"], "reason": info["reason"],
-        "roofline": {"bound": "hbm", "kernel": "k_cond8<float> forward (all RHS)",
+        "roofline": {"bound": "hbm", "kernel": "k_cond8s (FP32 streaming) forward, all RHS",
                      "achieved": kern["forward"]["gb_s"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": kern["forward"]["frac"], "traffic": traffic_from_profile("k_cond8<float>"),
+                     "frac": kern["forward"]["frac"], "traffic": traffic_from_profile("k_cond8s<float>"),
                      "avg_launch_us": kern["forward"]["avg_launch_us"], "algorithmic_bytes_per_launch": int(alg)},
-        "adjoint_kernel": kern["adjoint"], "cpu_baseline": cpu,
+        "adjoint_kernel": kern["adjoint"], "cpu_baseline": cpu, "autotune": tuning,
     }
     print(json.dumps(line), flush=True)
     system.close()

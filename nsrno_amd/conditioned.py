@@ -128,6 +128,11 @@ class ConditionedFrequencySystem:
     def close(self):
         self.plan.close()
 
+    def autotune(self, nrhs=None):
+        """Time the streaming operator's launch shapes and the multigrid's
+        workgroup target on this plan and keep the fastest (helm_autotune)."""
+        return self.plan.autotune(nrhs)
+
     def __call__(self, p):
         return self.plan.apply(p.contiguous())
 

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 
 #include "helm_internal.h"
 
@@ -313,6 +314,268 @@ __global__ __launch_bounds__(256) void k_cond8(int H, int W, int64_t n, int nrhs
     }
 }
 
+// ---- streaming FP32 form ------------------------------------------------
+// One wavefront per workgroup owns a strip of 64 columns and a run of tz
+// output rows for RB right-hand sides, marching down the rows.  The z
+// differences use register windows (8 staged rows of the field, 8 z-face
+// fluxes); the x differences go through a per-wavefront LDS row (78 nodes,
+// 71 faces).  Iteration t stages row t+7, forms z face t+3 and, once t has
+// reached the run, writes output row t.  Forward: the staged field is M u;
+// adjoint: M y, scaled per column (x) and per row (z) by conj(cA), conj(cB)
+// as the differences are formed.
+template <int RB, int MODE, bool ADJ>
+__global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs, int tzr,
+                                               const float* __restrict__ ax, const float* __restrict__ az,
+                                               const float2* __restrict__ mass, const float2* __restrict__ tx,
+                                               const float2* __restrict__ tz, int bz0, int bx0, int bh, int bw,
+                                               float c0, float c1, float c2, float c3,
+                                               const float2* __restrict__ u, float2* __restrict__ out,
+                                               const float2* __restrict__ b) {
+    constexpr int NS = 78, NF = 71;
+    __shared__ float2 sa[RB][NS], sb[RB][NS], sf[RB][NF], sg[RB][NF];
+    const int l = threadIdx.x;
+    const int x0 = blockIdx.x * 64, j = x0 + l;
+    const int i0 = blockIdx.y * tzr;
+    if (i0 >= H) return;
+    const int i1 = min(H, i0 + tzr);
+    const int r0 = blockIdx.z * RB;
+    const float2 zero = make_float2(0.f, 0.f);
+    const bool jok = j < W;
+    const int jc = cwrap(j, W);
+    const bool xin = jc >= bx0 && jc < bx0 + bw;
+    auto zin = [&](int r) { return r >= bz0 && r < bz0 + bh; };
+    // per-lane column data: own node column, halo node columns (lanes < 7), faces l-4 and (lanes < 7) 60+l
+    const int ch0 = cwrap(x0 - 7 + l, W), ch1 = cwrap(x0 + 64 + l, W);
+    const bool hin0 = ch0 >= bx0 && ch0 < bx0 + bw, hin1 = ch1 >= bx0 && ch1 < bx0 + bw;
+    const int fa = cwrap(x0 - 4 + l, W), fb = cwrap(x0 + 60 + l, W);
+    const float2 svA = tx[fa], gA = tx[W + fa];
+    const float2 svB = l < 7 ? tx[fb] : zero, gB = l < 7 ? tx[W + fb] : zero;
+    const float2 cAj = tx[2 * W + jc], cBj = tx[3 * W + jc];
+    float2 cA0 = zero, cB0 = zero, cA1 = zero, cB1 = zero;
+    if (ADJ && l < 7) {
+        cA0 = tx[2 * W + ch0];
+        cB0 = tx[3 * W + ch0];
+        cA1 = tx[2 * W + ch1];
+        cB1 = tx[3 * W + ch1];
+    }
+    const bool pmlx = __any((gA.x != 0.f || gA.y != 0.f) || (gB.x != 0.f || gB.y != 0.f));
+
+    float2 U[RB][8], G[RB][8];
+    auto ldu = [=](int q, int row, int col, bool colin) -> float2 {
+        const int r = r0 + q;
+        if (r >= nrhs || !colin || row < bz0 || row >= bz0 + bh) return make_float2(0.f, 0.f);
+        return u[(int64_t)r * n + (int64_t)row * W + col];
+    };
+    // initial window: rows i0-8 .. i0-1
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int row = cwrap(i0 - 8 + k, H);
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            U[q][k] = ldu(q, row, jc, xin);
+            G[q][k] = zero;
+        }
+    }
+    for (int t = i0 - 7; t < i1; ++t) {
+        const int rn = cwrap(t + 7, H);  // row staged now
+        const int fz = cwrap(t + 3, H);  // z face formed now (face fz + 1/2)
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) U[q][k] = U[q][k + 1];
+            U[q][7] = ldu(q, rn, jc, xin);
+        }
+        // z face t+3: rows t .. t+7 are U[.][0..7]
+        {
+            const float2 sv = tz[fz], gm = tz[H + fz];
+            const float a = az[(int64_t)fz * W + jc];
+            float2 cz[8], czb[8];
+            bool pz = false;
+            if (ADJ) {
+                pz = gm.x != 0.f || gm.y != 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int row = cwrap(t + k, H);
+                    cz[k] = tz[2 * H + row];
+                    czb[k] = pz ? tz[3 * H + row] : zero;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                float2 d;
+                if (!ADJ) {
+                    d = c8diff<float2, float>(c0, c1, c2, c3, U[q][4], U[q][3], U[q][5], U[q][2], U[q][6], U[q][1],
+                                              U[q][7], U[q][0]);
+                    d = cscale(cm(sv, d), a);
+                } else {
+                    d = c8diff<float2, float>(c0, c1, c2, c3, cjm(cz[4], U[q][4]), cjm(cz[3], U[q][3]),
+                                              cjm(cz[5], U[q][5]), cjm(cz[2], U[q][2]), cjm(cz[6], U[q][6]),
+                                              cjm(cz[1], U[q][1]), cjm(cz[7], U[q][7]), cjm(cz[0], U[q][0]));
+                    if (pz) {
+                        const float2 d2 = c8diff<float2, float>(
+                            c0, c1, c2, c3, cjm(czb[4], U[q][4]), cjm(czb[3], U[q][3]), cjm(czb[5], U[q][5]),
+                            cjm(czb[2], U[q][2]), cjm(czb[6], U[q][6]), cjm(czb[1], U[q][1]), cjm(czb[7], U[q][7]),
+                            cjm(czb[0], U[q][0]));
+                        d = cadd(d, cjm(gm, d2));
+                    }
+                    d = cscale(cjm(sv, d), a);  // conj(az sv), az real
+                }
+#pragma unroll
+                for (int k = 0; k < 7; ++k) G[q][k] = G[q][k + 1];
+                G[q][7] = d;  // G[.][k] = face t-4+k
+            }
+        }
+        if (t < i0) continue;
+        // ---- output row t ----
+        const int i = t;
+        const bool rin = zin(i);
+        // x: stage row i (and the halo columns) in LDS
+        __syncthreads();  // previous row's LDS readers are done
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const float2 v = U[q][0];  // U[q][k] holds row t + k
+            if (!ADJ) {
+                sa[q][l + 7] = v;
+            } else {
+                sa[q][l + 7] = cjm(cAj, v);
+                if (pmlx) sb[q][l + 7] = cjm(cBj, v);
+            }
+            if (l < 7) {
+                const float2 h0 = ldu(q, i, ch0, hin0), h1 = ldu(q, i, ch1, hin1);
+                if (!ADJ) {
+                    sa[q][l] = h0;
+                    sa[q][71 + l] = h1;
+                } else {
+                    sa[q][l] = cjm(cA0, h0);
+                    sa[q][71 + l] = cjm(cA1, h1);
+                    if (pmlx) {
+                        sb[q][l] = cjm(cB0, h0);
+                        sb[q][71 + l] = cjm(cB1, h1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const float axa = ax[(int64_t)i * W + fa];
+        const float axb = l < 7 ? ax[(int64_t)i * W + fb] : 0.f;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+                if (pass == 1 && l >= 7) continue;
+                const int f = pass ? 64 + l : l;  // face index in the strip window (0 .. 70 = faces -4 .. 66)
+                const float2* s = sa[q] + f + 3;  // s[m] = node f - 4 + m  (node index - (x0-7) = f + 3 + m)
+                // D+ at face f-4: nodes f-4+m+1 - (f-4-m) -> s[m+1] - s[-m] with s at node f-4
+                float2 d = c8diff<float2, float>(c0, c1, c2, c3, s[1], s[0], s[2], s[-1], s[3], s[-2], s[4], s[-3]);
+                const float2 sv = pass ? svB : svA;
+                const float a = pass ? axb : axa;
+                if (ADJ && pmlx) {
+                    const float2* sB = sb[q] + f + 3;
+                    const float2 d2 = c8diff<float2, float>(c0, c1, c2, c3, sB[1], sB[0], sB[2], sB[-1], sB[3], sB[-2],
+                                                            sB[4], sB[-3]);
+                    d = cadd(d, cjm(pass ? gB : gA, d2));
+                }
+                const float2 F = ADJ ? cscale(cjm(sv, d), a) : cscale(cm(sv, d), a);
+                sf[q][f] = F;
+                if (!ADJ && pmlx) sg[q][f] = cm(pass ? gB : gA, F);
+            }
+        }
+        __syncthreads();
+        // z PML terms of the forward output: g - 1 of faces i-4 .. i+3
+        float2 gz[8];
+        bool pzo = false;
+        if (!ADJ) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                gz[k] = tz[H + cwrap(i - 4 + k, H)];
+                pzo = pzo || gz[k].x != 0.f || gz[k].y != 0.f;
+            }
+        }
+        const float2 cAz = tz[2 * H + cwrap(i, H)], cBz = tz[3 * H + cwrap(i, H)];
+        const bool inside = jok && xin && rin;
+        float2 mkc = zero;
+        if (inside) mkc = mass[(int64_t)i * W + jc];
+        if (ADJ) mkc.y = -mkc.y;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int r = r0 + q;
+            if (!jok || r >= nrhs) continue;
+            const int64_t o = (int64_t)r * n + (int64_t)i * W + j;
+            float2 acc = zero;
+            if (inside) {
+                const float2* fx = sf[q] + l + 4;  // fx[m] = face l + m
+                const float2 dx = c8diff<float2, float>(c0, c1, c2, c3, fx[0], fx[-1], fx[1], fx[-2], fx[2], fx[-3],
+                                                        fx[3], fx[-4]);
+                const float2 dz = c8diff<float2, float>(c0, c1, c2, c3, G[q][4], G[q][3], G[q][5], G[q][2], G[q][6],
+                                                        G[q][1], G[q][7], G[q][0]);
+                if (!ADJ) {
+                    acc = cm(cAj, dx);
+                    cfma(acc, cAz, dz);
+                    if (pmlx) {
+                        const float2* gx = sg[q] + l + 4;
+                        const float2 d2 = c8diff<float2, float>(c0, c1, c2, c3, gx[0], gx[-1], gx[1], gx[-2], gx[2],
+                                                                gx[-3], gx[3], gx[-4]);
+                        cfma(acc, cBj, d2);
+                    }
+                    if (pzo) {
+                        const float2 d2 = c8diff<float2, float>(
+                            c0, c1, c2, c3, cm(gz[4], G[q][4]), cm(gz[3], G[q][3]), cm(gz[5], G[q][5]),
+                            cm(gz[2], G[q][2]), cm(gz[6], G[q][6]), cm(gz[1], G[q][1]), cm(gz[7], G[q][7]),
+                            cm(gz[0], G[q][0]));
+                        cfma(acc, cBz, d2);
+                    }
+                } else {
+                    acc = cadd(dx, dz);
+                }
+                cfma(acc, mkc, U[q][0]);
+            } else {
+                acc = u[o];
+            }
+            if (MODE == 1) acc = csub(b[o], acc);
+            out[o] = acc;
+        }
+    }
+}
+
+static int cond_stream_tz() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_COND_TZ");
+        const int x = v ? std::atoi(v) : 0;
+        return x >= 8 && x <= 4096 ? x : 64;
+    }();
+    return t;
+}
+static int cond_stream_rb() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_COND_RB");
+        const int x = v ? std::atoi(v) : 0;
+        return (x == 1 || x == 2 || x == 4) ? x : 2;
+    }();
+    return t;
+}
+
+template <int RB>
+static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
+                          hipStream_t s) {
+    const int tzr = L.st_tz ? L.st_tz : cond_stream_tz();
+    const dim3 block(64), grid((L.nx + 63) / 64, (L.nz + tzr - 1) / tzr, (nrhs + RB - 1) / RB);
+    const float ih = (float)(1.0 / L.h_m);
+    const float c0 = (float)CD0 * ih, c1 = (float)CD1 * ih, c2 = (float)CD2 * ih, c3 = (float)CD3 * ih;
+    const float2* tx = L.ctab32;
+    const float2* tz = L.ctab32 + 4 * L.nx;
+#define HELM_C8S(M, A, BP)                                                                                       \
+    hipLaunchKernelGGL((k_cond8s<RB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr, L.ax32, L.az32,     \
+                       L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3, u, out, BP)
+    if (b) {
+        if (adjoint) HELM_C8S(1, true, b);
+        else HELM_C8S(1, false, b);
+    } else {
+        if (adjoint) HELM_C8S(0, true, (const float2*)nullptr);
+        else HELM_C8S(0, false, (const float2*)nullptr);
+    }
+#undef HELM_C8S
+}
+
 template <int TZ>
 static int cond8_rhs_per_group(const Level& L, int nrhs) {
     const int64_t tiles = (int64_t)((L.nx + C8_TX - 1) / C8_TX) * ((L.nz + TZ - 1) / TZ);
@@ -324,6 +587,15 @@ int32_t launch_cond8_32(const Level& L, const float2* u, float2* out, int nrhs, 
                         hipStream_t s) {
     constexpr int TZ = 16;
     HELM_CHECK(L.ax32 && L.az32 && L.cmass32 && L.ctab32, HELM_ERR_STATE, "conditioned operator: medium not set");
+    if (cond_stream_tz() > 0 && !std::getenv("HELM_COND_TILED")) {
+        switch (L.st_rb ? L.st_rb : cond_stream_rb()) {
+            case 1: launch_cond8s<1>(L, u, out, nrhs, adjoint, b, s); break;
+            case 4: launch_cond8s<4>(L, u, out, nrhs, adjoint, b, s); break;
+            default: launch_cond8s<2>(L, u, out, nrhs, adjoint, b, s); break;
+        }
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const int rb = cond8_rhs_per_group<TZ>(L, nrhs);
     const dim3 block(64, 4), grid((L.nx + C8_TX - 1) / C8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
     const float ih = (float)(1.0 / L.h_m);

@@ -277,7 +277,8 @@ __global__ void k_sample(int n_ports, int64_t n, int nrhs, const int* __restrict
 int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     Level& L = p->levels[0];
     *best_us = 0.0;
-    if (L.op != HELM_OP_FLUX5 || L.nx < 64) return HELM_OK;  // the streaming kernel is not used there
+    const bool cond = L.op == HELM_OP_COND8;
+    if (!cond && (L.op != HELM_OP_FLUX5 || L.nx < 64)) return HELM_OK;  // no streaming kernel there
     const size_t bytes = (size_t)nrhs * L.n * sizeof(float2);
     float2 *u = nullptr, *out = nullptr;
     HELM_HIP(hipMalloc((void**)&u, bytes));
@@ -291,17 +292,24 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     (void)hipEventCreate(&e1);
     int32_t st = HELM_OK;
     double best = 1e30;
-    int btz = 8, brb = 4;
+    int btz = cond ? 64 : 8, brb = cond ? 2 : 4;
     if (hipMemsetAsync(u, 0, bytes, s) != hipSuccess) st = HELM_ERR_HIP;
-    const int tzs[3] = {4, 8, 16}, rbs[3] = {1, 2, 4};
+    // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up)
+    const int ntz = cond ? 4 : 3;
+    const int tzs_flux[4] = {4, 8, 16, 16}, tzs_cond[4] = {32, 64, 128, 256}, rbs[3] = {1, 2, 4};
+    const int* tzs = cond ? tzs_cond : tzs_flux;
     const double k0sq = p->k0 * p->k0;
-    for (int a = 0; a < 3 && st == HELM_OK; ++a)
+    auto run = [&]() {
+        return cond ? launch_cond8_32(L, u, out, nrhs, false, nullptr, s)
+                    : launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+    };
+    for (int a = 0; a < ntz && st == HELM_OK; ++a)
         for (int b = 0; b < 3 && st == HELM_OK; ++b) {
             L.st_tz = tzs[a];
             L.st_rb = rbs[b];
-            for (int w = 0; w < 2 && st == HELM_OK; ++w) st = launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+            for (int w = 0; w < 2 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e0, s);
-            for (int w = 0; w < 5 && st == HELM_OK; ++w) st = launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+            for (int w = 0; w < 5 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e1, s);
             float ms = 0.f;
             if (st == HELM_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&

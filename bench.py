@@ -230,16 +230,21 @@ def pmc_summary(directory):
                     t[0] += v * scale[counter]
                     t[1] += 1
                     break
-    out = {"unit": "bytes per launch (mean over the launches of one 11-angle solve)",
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:  # kernels measured by earlier runs (other workloads) are kept
+        with open(path) as fh:
+            kept = json.load(fh).get("kernels", {})
+    except (OSError, ValueError):
+        kept = {}
+    out = {"unit": "bytes per launch (mean over the launches of one solve of the workload that ran the kernel)",
            "scale_fetch_bytes_per_count": scale["FETCH_SIZE"], "scale_write_bytes_per_count": scale["WRITE_SIZE"],
-           "kernels": {}}
+           "kernels": kept}
     for label, t in totals.items():
         f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
         if f[1] and w[1]:
             fb, wb = f[0] / f[1], w[0] / w[1]
             out["kernels"][label] = {"launches": f[1], "fetch_bytes_per_launch": round(fb),
                                      "write_bytes_per_launch": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out), flush=True)

@@ -167,6 +167,11 @@ static int64_t wg_target() {
     static const int64_t t = env_int("HELM_MG_WG_TARGET", 2048, 64, 1 << 20);
     return t;
 }
+// fused level kernels for pre = post = 2 (HELM_MG_FUSED=0 selects the four-kernel path)
+static bool fused_levels() {
+    static const bool f = env_int("HELM_MG_FUSED", 1, 0, 1) != 0;
+    return f;
+}
 static int64_t tail_nodes() {
     static const int64_t t = env_int("HELM_MG_TAIL_NODES", TAIL_NODES_DEFAULT, 16, TAIL_NODES);
     return t;
@@ -691,6 +696,289 @@ __global__ __launch_bounds__(256) void k_mgt_restrict_res(int nzf, int nxf, int 
     }
 }
 
+// ---- fused level kernels (pre = post = 2) ---------------------------------
+// k_mgt_pre2_rr: u2 = S(S(0)) and f_c = R (f - M u2) in one pass.  Works on
+// the 8 x 32 coarse tile of k_mgt_restrict_res: f is staged on the fine
+// region that the residual's full weighting needs plus a 2-node halo
+// (21 x 69), u1 = (w/dg) f there, u2 on the region plus a 1-node halo
+// (19 x 67), the residual on the 17 x 65 region; each fine node of the 16 x 64
+// block [2 I0, 2 I0 + 16) x [2 J0, 2 J0 + 64) writes its u2 (these blocks
+// partition the fine grid).  Each thread keeps the stencil of its u2-region
+// nodes in registers and uses it for both the sweep and the residual.
+constexpr int PR_RZ = 2 * RR_TZ + 1, PR_RX = 2 * RR_TX + 1;  // residual region 17 x 65
+constexpr int PR_2Z = PR_RZ + 2, PR_2X = PR_RX + 2, PR_2N = PR_2Z * PR_2X;  // u2 region 19 x 67
+constexpr int PR_1Z = PR_RZ + 4, PR_1X = PR_RX + 4, PR_1N = PR_1Z * PR_1X;  // f, u1 region 21 x 69
+constexpr int PR_1PT = (PR_1N + 255) / 256, PR_2PT = (PR_2N + 255) / 256;
+__global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs, int rb,
+                                                     const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                     const float2* __restrict__ isxe,
+                                                     const float2* __restrict__ isze, const float* __restrict__ eta,
+                                                     const float2* __restrict__ kap, float k0sq, float hh,
+                                                     float shift, int conj, const float2* __restrict__ wid,
+                                                     const float2* __restrict__ f, float2* __restrict__ u,
+                                                     float2* __restrict__ fcoarse) {
+    __shared__ float2 sf[PR_1N];
+    __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
+    __shared__ float2 s2[PR_2N];
+    const int tid = threadIdx.y * RR_TX + threadIdx.x;
+    const int I0 = blockIdx.y * RR_TZ, J0 = blockIdx.x * RR_TX;
+    const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
+    const int64_t nf = (int64_t)nzf * nxf, nc = (int64_t)nzc * nxc;
+    const bool cj = conj != 0;
+    // f / u1 region nodes staged by this thread
+    int i1[PR_1PT];
+    float2 w1[PR_1PT];
+#pragma unroll
+    for (int q = 0; q < PR_1PT; ++q) {
+        const int e = tid + 256 * q;
+        i1[q] = -1;
+        w1[q] = make_float2(0.f, 0.f);
+        if (e < PR_1N) {
+            const int gi = fi0 - 2 + e / PR_1X, gj = fj0 - 2 + e % PR_1X;
+            if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
+                i1[q] = gi * nxf + gj;
+                w1[q] = cc(wid[i1[q]], cj);
+            }
+        }
+    }
+    // u2 region nodes computed by this thread: stencil, Jacobi factor, roles
+    int i2[PR_2PT];
+    Coef<float2> c2[PR_2PT];
+    float2 w2[PR_2PT];
+#pragma unroll
+    for (int q = 0; q < PR_2PT; ++q) {
+        const int e = tid + 256 * q;
+        i2[q] = -1;
+        if (e < PR_2N) {
+            const int gi = fi0 - 1 + e / PR_2X, gj = fj0 - 1 + e % PR_2X;
+            if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
+                i2[q] = gi * nxf + gj;
+                node_coef<float, float2>(gi, gj, nzf, nxf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c2[q]);
+                w2[q] = cc(wid[i2[q]], cj);
+            }
+        }
+    }
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    float2 fq[PR_1PT];
+#pragma unroll
+    for (int q = 0; q < PR_1PT; ++q) fq[q] = i1[q] >= 0 ? f[(int64_t)r0 * nf + i1[q]] : make_float2(0.f, 0.f);
+    for (int r = r0; r < r1; ++r) {
+        __syncthreads();  // the previous RHS is done with the LDS arrays
+#pragma unroll
+        for (int q = 0; q < PR_1PT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < PR_1N) {
+                sf[e] = fq[q];
+                s1[e] = cmul(w1[q], fq[q]);  // u1 = (w/dg) f (0 on ghosts)
+            }
+        }
+        if (r + 1 < r1) {
+#pragma unroll
+            for (int q = 0; q < PR_1PT; ++q)
+                fq[q] = i1[q] >= 0 ? f[(int64_t)(r + 1) * nf + i1[q]] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        // u2 = u1 + (w/dg) (f - M u1) on the u2 region
+        float2 v2[PR_2PT];
+#pragma unroll
+        for (int q = 0; q < PR_2PT; ++q) {
+            const int e = tid + 256 * q;
+            v2[q] = make_float2(0.f, 0.f);
+            if (e < PR_2N && i2[q] >= 0) {
+                const int k = (e / PR_2X + 1) * PR_1X + e % PR_2X + 1;  // same node in the 21 x 69 layout
+                float2 acc = cmul(c2[q].dg, s1[k]);
+                cfma(acc, c2[q].fw, s1[k - 1]);
+                cfma(acc, c2[q].fe, s1[k + 1]);
+                cfma(acc, c2[q].fn, s1[k - PR_1X]);
+                cfma(acc, c2[q].fs, s1[k + PR_1X]);
+                float2 v = s1[k];
+                cfma(v, w2[q], csub(sf[k], acc));
+                v2[q] = v;
+            }
+            if (e < PR_2N) s2[e] = v2[q];
+        }
+        __syncthreads();
+        // residual f - M u2 on the 17 x 65 region (into s1) and the owned u2 outputs
+#pragma unroll
+        for (int q = 0; q < PR_2PT; ++q) {
+            const int e = tid + 256 * q;
+            if (e >= PR_2N) continue;
+            const int row = e / PR_2X, col = e % PR_2X;  // region-2 coordinates
+            const int gi = fi0 - 1 + row, gj = fj0 - 1 + col;
+            if (row >= 1 && row <= PR_RZ && col >= 1 && col <= PR_RX) {
+                float2 res = make_float2(0.f, 0.f);
+                if (i2[q] >= 0) {
+                    float2 acc = cmul(c2[q].dg, s2[e]);
+                    cfma(acc, c2[q].fw, s2[e - 1]);
+                    cfma(acc, c2[q].fe, s2[e + 1]);
+                    cfma(acc, c2[q].fn, s2[e - PR_2X]);
+                    cfma(acc, c2[q].fs, s2[e + PR_2X]);
+                    res = csub(sf[(row + 1) * PR_1X + col + 1], acc);
+                }
+                s1[(row - 1) * PR_RX + col - 1] = res;
+            }
+            if (i2[q] >= 0 && gi >= 2 * I0 && gi < 2 * I0 + 2 * RR_TZ && gj >= 2 * J0 && gj < 2 * J0 + 2 * RR_TX)
+                u[(int64_t)r * nf + i2[q]] = v2[q];
+        }
+        __syncthreads();
+        const int I = I0 + threadIdx.y, J = J0 + threadIdx.x;
+        if (I < nzc && J < nxc) {
+            const float2* base = s1 + (2 * threadIdx.y + 1) * PR_RX + 2 * threadIdx.x + 1;
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int a = -1; a <= 1; ++a)
+#pragma unroll
+                for (int b = -1; b <= 1; ++b) {
+                    const float w = (a == 0 ? 0.5f : 0.25f) * (b == 0 ? 0.5f : 0.25f);
+                    const float2 v = base[a * PR_RX + b];
+                    acc.x += w * v.x;
+                    acc.y += w * v.y;
+                }
+            fcoarse[(int64_t)r * nc + (int64_t)I * nxc + J] = acc;
+        }
+    }
+}
+
+// k_mgt_post2: out = S(S(u + P u_c)) in one pass on the 16 x 64 fine tile:
+// v = u + P u_c is formed on the tile plus a 2-node halo (20 x 68), the first
+// sweep on the tile plus a 1-node halo (the staged layout of the other tile
+// kernels, 18 x 66), the second on the tile.  Each thread keeps the stencil of
+// its halo-1 nodes and uses it for both sweeps.  out must not alias u.
+constexpr int P2_X = MT_TX + 4, P2_Z = MT_TZ + 4, P2_N = P2_X * P2_Z;
+constexpr int P2_PT = (P2_N + 255) / 256;
+__global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs, int rb,
+                                                   const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                   const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                   const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                   float k0sq, float hh, float shift, int conj,
+                                                   const float2* __restrict__ wid, const float2* __restrict__ uc,
+                                                   const float2* __restrict__ u, const float2* __restrict__ f,
+                                                   float2* __restrict__ out) {
+    __shared__ float2 sv2[P2_N];
+    __shared__ float2 sv1[MT_NS];
+    __shared__ float2 scv[MP_NC];
+    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
+    const int64_t ncn = (int64_t)nzc * nxc;
+    const bool cj = conj != 0;
+    int ih[P2_PT];  // halo-2 nodes staged by this thread
+#pragma unroll
+    for (int q = 0; q < P2_PT; ++q) {
+        const int e = tid + 256 * q;
+        ih[q] = -1;
+        if (e < P2_N) {
+            const int gi = z0 - 2 + e / P2_X, gj = x0 - 2 + e % P2_X;
+            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) ih[q] = gi * nx + gj;
+        }
+    }
+    int cix[MP_CPT];
+#pragma unroll
+    for (int q = 0; q < MP_CPT; ++q) {
+        const int e = tid + 256 * q;
+        cix[q] = -1;
+        if (e < MP_NC) {
+            const int I = cz0 + e / MP_CW, J = cx0 + e % MP_CW;
+            if (I >= 0 && I < nzc && J >= 0 && J < nxc) cix[q] = I * nxc + J;
+        }
+    }
+    int i1[MT_SPT];  // halo-1 nodes swept by this thread
+    Coef<float2> c1[MT_SPT];
+    float2 w1[MT_SPT];
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) {
+        const int e = tid + 256 * q;
+        i1[q] = -1;
+        if (e < MT_NS) {
+            const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
+                i1[q] = gi * nx + gj;
+                node_coef<float, float2>(gi, gj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c1[q]);
+                w1[q] = cc(wid[i1[q]], cj);
+            }
+        }
+    }
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    float2 su[P2_PT], cs[MP_CPT], fv[MT_SPT];
+    auto fetch = [&](int r) {
+#pragma unroll
+        for (int q = 0; q < P2_PT; ++q) su[q] = ih[q] >= 0 ? u[(int64_t)r * n + ih[q]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < MP_CPT; ++q) cs[q] = cix[q] >= 0 ? uc[(int64_t)r * ncn + cix[q]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) fv[q] = i1[q] >= 0 ? f[(int64_t)r * n + i1[q]] : make_float2(0.f, 0.f);
+    };
+    fetch(r0);
+    for (int r = r0; r < r1; ++r) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MP_CPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < MP_NC) scv[e] = cs[q];
+        }
+        __syncthreads();
+        // v = u + P u_c on the halo-2 block (bilinear weights without branches)
+#pragma unroll
+        for (int q = 0; q < P2_PT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < P2_N) {
+                float2 v = make_float2(0.f, 0.f);
+                if (ih[q] >= 0) {
+                    const int gi = z0 - 2 + e / P2_X, gj = x0 - 2 + e % P2_X;
+                    const int I = (gi >> 1) - cz0, J = (gj >> 1) - cx0;
+                    const float wz1 = (gi & 1) ? 0.5f : 0.f, wz0 = 1.f - wz1;
+                    const float wx1 = (gj & 1) ? 0.5f : 0.f, wx0 = 1.f - wx1;
+                    const float2* c0 = scv + I * MP_CW + J;
+                    v = su[q];
+                    v.x += wz0 * (wx0 * c0[0].x + wx1 * c0[1].x) + wz1 * (wx0 * c0[MP_CW].x + wx1 * c0[MP_CW + 1].x);
+                    v.y += wz0 * (wx0 * c0[0].y + wx1 * c0[1].y) + wz1 * (wx0 * c0[MP_CW].y + wx1 * c0[MP_CW + 1].y);
+                }
+                sv2[e] = v;
+            }
+        }
+        float2 fc[MT_SPT];
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) fc[q] = fv[q];
+        if (r + 1 < r1) fetch(r + 1);
+        __syncthreads();
+        // first sweep on the halo-1 block
+        float2 v1[MT_SPT];
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) {
+            const int e = tid + 256 * q;
+            v1[q] = make_float2(0.f, 0.f);
+            if (e < MT_NS && i1[q] >= 0) {
+                const int k = (e / MT_SW + 1) * P2_X + e % MT_SW + 1;  // same node in the halo-2 layout
+                float2 acc = cmul(c1[q].dg, sv2[k]);
+                cfma(acc, c1[q].fw, sv2[k - 1]);
+                cfma(acc, c1[q].fe, sv2[k + 1]);
+                cfma(acc, c1[q].fn, sv2[k - P2_X]);
+                cfma(acc, c1[q].fs, sv2[k + P2_X]);
+                float2 v = sv2[k];
+                cfma(v, w1[q], csub(fc[q], acc));
+                v1[q] = v;
+            }
+            if (e < MT_NS) sv1[e] = v1[q];
+        }
+        __syncthreads();
+        // second sweep on the tile
+#pragma unroll
+        for (int q = 0; q < MT_SPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e >= MT_NS || i1[q] < 0) continue;
+            const int row = e / MT_SW - 1, col = e % MT_SW - 1;
+            if (row < 0 || row >= MT_TZ || col < 0 || col >= MT_TX) continue;
+            float2 acc = cmul(c1[q].dg, sv1[e]);
+            cfma(acc, c1[q].fw, sv1[e - 1]);
+            cfma(acc, c1[q].fe, sv1[e + 1]);
+            cfma(acc, c1[q].fn, sv1[e - MT_SW]);
+            cfma(acc, c1[q].fs, sv1[e + MT_SW]);
+            float2 v = v1[q];
+            cfma(v, w1[q], csub(fc[q], acc));
+            out[(int64_t)r * n + i1[q]] = v;
+        }
+    }
+}
+
 // bilinear prolongation of the coarse correction at fine node (i, j) (zero
 // coarse ghosts)
 __device__ __forceinline__ float2 prolong_at(const float2* __restrict__ uc, int nzc, int nxc, int i, int j) {
@@ -880,6 +1168,23 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
     const int rb = rhs_group(p, tiles, nrhs);
     const dim3 blk(64, 4), grd((L.nx + MT_TX - 1) / MT_TX, (L.nz + MT_TZ - 1) / MT_TZ, (nrhs + rb - 1) / rb);
+    if (!coarsest && c.pre == 2 && c.post == 2 && fused_levels()) {
+        // two launches per level: smoothing + residual + restriction, then
+        // prolongation + smoothing (the smoothed field lives in mg_t between)
+        Level& C = p->levels[l + 1];
+        const int64_t bc = (int64_t)((C.nx + RR_TX - 1) / RR_TX) * ((C.nz + RR_TZ - 1) / RR_TZ);
+        const int rbc = rhs_group(p, bc, nrhs);
+        const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
+        hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
+                           L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, f, L.mg_t,
+                           C.mg_f);
+        HELM_HIP(hipGetLastError());
+        HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
+        hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
+                           L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, C.mg_u, L.mg_t, f, u);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     // pre-smoothing from a zero guess, ending in u
     if (sweeps == 0) {
         const int64_t cnt = (int64_t)nrhs * L.n;

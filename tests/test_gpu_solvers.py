@@ -109,18 +109,22 @@ def test_bicgstab_recurrence_matches_oracle():
         assert rel(x[r].cpu().numpy().ravel(), ref) < 1e-4
 
 
-def test_vcycle_matches_oracle_restatement():
+@pytest.mark.parametrize("pre,post", [(2, 1), (2, 2), (3, 2)])
+def test_vcycle_matches_oracle_restatement(pre, post):
     """One V-cycle (fixed real weight so no arg-min ties) equals the CPU
-    restatement in oracle/multigrid.py."""
+    restatement in oracle/multigrid.py; (2, 2) runs the fused level kernels
+    (smoothing + residual + restriction, prolongation + two sweeps)."""
     from oracle import multigrid as M
-    grid, f, plan, w, rhs = setup(nrhs=2, levels=3)
-    plan.setup_multigrid(levels=3, shift=0.5, pre=2, post=1, coarse_sweeps=7, omega=0.6)
+    grid, f, plan, w, rhs = setup(nrhs=3, levels=3)
+    plan.setup_multigrid(levels=3, shift=0.5, pre=pre, post=post, coarse_sweeps=7, omega=0.6)
     mg = M.Multigrid(grid, f, 1 + GOLDEN["m_eta"], (1 + GOLDEN["m_kappa"]).astype(complex), levels=3, shift=0.5,
-                     pre=2, post=1, coarse=7, omega=0.6)
-    v = np.random.default_rng(3).normal(size=(2,) + grid.shape) + 1j * np.random.default_rng(4).normal(
-        size=(2,) + grid.shape)
+                     pre=pre, post=post, coarse=7, omega=0.6)
+    v = np.random.default_rng(3).normal(size=(3,) + grid.shape) + 1j * np.random.default_rng(4).normal(
+        size=(3,) + grid.shape)
     z = plan.precondition(torch.from_numpy(v.astype(np.complex64)).cuda()).cpu().numpy()
     assert rel(z, mg.vcycle(v)) < 2e-5
+    if (pre, post) != (2, 1):
+        return
     # per-node weights: the oracle's V-cycle with its own weights converges
     mg2 = M.Multigrid(grid, f, 1 + GOLDEN["m_eta"], (1 + GOLDEN["m_kappa"]).astype(complex), levels=3, shift=0.5,
                       pre=2, post=2, coarse=30)

@@ -174,6 +174,7 @@ struct MGConfig {
     double shift = 0.5;
     int pre = 1, post = 1, coarse = 10;
     double omega = 0.0;  // 0: per-node complex weights; > 0: this real weight everywhere
+    int requested = 0;   // levels asked for by the last helm_mg_setup (the grid may allow fewer)
 };
 
 }  // namespace helm

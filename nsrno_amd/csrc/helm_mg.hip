@@ -941,11 +941,10 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
         if (r + 1 < r1) fetch(r + 1);
         __syncthreads();
         // first sweep on the halo-1 block
-        float2 v1[MT_SPT];
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) {
             const int e = tid + 256 * q;
-            v1[q] = make_float2(0.f, 0.f);
+            float2 v1 = make_float2(0.f, 0.f);
             if (e < MT_NS && i1[q] >= 0) {
                 const int k = (e / MT_SW + 1) * P2_X + e % MT_SW + 1;  // same node in the halo-2 layout
                 float2 acc = cmul(c1[q].dg, sv2[k]);
@@ -953,11 +952,10 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
                 cfma(acc, c1[q].fe, sv2[k + 1]);
                 cfma(acc, c1[q].fn, sv2[k - P2_X]);
                 cfma(acc, c1[q].fs, sv2[k + P2_X]);
-                float2 v = sv2[k];
-                cfma(v, w1[q], csub(fc[q], acc));
-                v1[q] = v;
+                v1 = sv2[k];
+                cfma(v1, w1[q], csub(fc[q], acc));
             }
-            if (e < MT_NS) sv1[e] = v1[q];
+            if (e < MT_NS) sv1[e] = v1;
         }
         __syncthreads();
         // second sweep on the tile
@@ -972,7 +970,7 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
             cfma(acc, c1[q].fe, sv1[e + 1]);
             cfma(acc, c1[q].fn, sv1[e - MT_SW]);
             cfma(acc, c1[q].fs, sv1[e + MT_SW]);
-            float2 v = v1[q];
+            float2 v = sv1[e];
             cfma(v, w1[q], csub(fc[q], acc));
             out[(int64_t)r * n + i1[q]] = v;
         }
@@ -1269,10 +1267,10 @@ int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     int32_t st = hipMemsetAsync(v, 0, bytes, s) == hipSuccess ? HELM_OK : HELM_ERR_HIP;
-    const int64_t targets[3] = {1024, 2048, 4096};
+    const int64_t targets[5] = {256, 512, 1024, 2048, 4096};
     double best = 1e30;
     int64_t bt = 2048;
-    for (int a = 0; a < 3 && st == HELM_OK; ++a) {
+    for (int a = 0; a < 5 && st == HELM_OK; ++a) {
         p->mg_wg_target = targets[a];
         st = mg_vcycle(p, v, z, nrhs, false, s);
         (void)hipEventRecord(e0, s);
@@ -1324,6 +1322,11 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
     HELM_CHECK(std::isfinite(omega) && omega >= 0 && omega <= 1.5, HELM_ERR_ARG,
                "helm_mg_setup: omega must be 0 (automatic complex weights) or in (0, 1.5]");
     hipStream_t s = (hipStream_t)stream;
+    // the same configuration again: the levels exist and helm_set_medium* has
+    // already rebuilt their coefficients and weights for the current medium
+    if (p->mg.levels > 0 && p->mg.requested == levels && p->mg.shift == shift && p->mg.pre == pre &&
+        p->mg.post == post && p->mg.coarse == coarse_sweeps && p->mg.omega == omega && p->medium_set)
+        return HELM_OK;
     // drop old coarse levels, build new ones down to >= 3 nodes per axis
     while (p->levels.size() > 1) {
         Level& L = p->levels.back();
@@ -1354,6 +1357,7 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
         // plans set up on other threads never lower it under each other
         HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
+    p->mg.requested = levels;
     p->mg.shift = shift;
     p->mg.pre = pre;
     p->mg.post = post;

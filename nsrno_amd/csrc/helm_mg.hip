@@ -696,6 +696,30 @@ __global__ __launch_bounds__(256) void k_mgt_restrict_res(int nzf, int nxf, int 
     }
 }
 
+// stencil and Jacobi factor of fine node `node`: from the level's
+// precomputed table (levels >= 1: [w/dg, dg, fw, fe, fn, fs] per node) or
+// formed from the medium (level 0, where the table would cost more bytes than
+// the formation saves)
+__device__ __forceinline__ void level_coef(int gi, int gj, int node, int nz, int nx, const float2* __restrict__ tcf,
+                                           const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                           const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                           const float* __restrict__ eta, const float2* __restrict__ kap, float k0sq,
+                                           float hh, float shift, bool cj, const float2* __restrict__ wid,
+                                           Coef<float2>& c, float2& w) {
+    if (tcf) {
+        const float2* t = tcf + (int64_t)node * 6;
+        w = cc(t[0], cj);
+        c.dg = cc(t[1], cj);
+        c.fw = cc(t[2], cj);
+        c.fe = cc(t[3], cj);
+        c.fn = cc(t[4], cj);
+        c.fs = cc(t[5], cj);
+    } else {
+        node_coef<float, float2>(gi, gj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c);
+        w = cc(wid[node], cj);
+    }
+}
+
 // ---- fused level kernels (pre = post = 2) ---------------------------------
 // k_mgt_pre2_rr: u2 = S(S(0)) and f_c = R (f - M u2) in one pass.  Works on
 // the 8 x 32 coarse tile of k_mgt_restrict_res: f is staged on the fine
@@ -715,8 +739,8 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
                                                      const float2* __restrict__ isze, const float* __restrict__ eta,
                                                      const float2* __restrict__ kap, float k0sq, float hh,
                                                      float shift, int conj, const float2* __restrict__ wid,
-                                                     const float2* __restrict__ f, float2* __restrict__ u,
-                                                     float2* __restrict__ fcoarse) {
+                                                     const float2* __restrict__ tcf, const float2* __restrict__ f,
+                                                     float2* __restrict__ u, float2* __restrict__ fcoarse) {
     __shared__ float2 sf[PR_1N];
     __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
     __shared__ float2 s2[PR_2N];
@@ -737,7 +761,7 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
             const int gi = fi0 - 2 + e / PR_1X, gj = fj0 - 2 + e % PR_1X;
             if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
                 i1[q] = gi * nxf + gj;
-                w1[q] = cc(wid[i1[q]], cj);
+                w1[q] = tcf ? cc(tcf[(int64_t)i1[q] * 6], cj) : cc(wid[i1[q]], cj);
             }
         }
     }
@@ -753,8 +777,8 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
             const int gi = fi0 - 1 + e / PR_2X, gj = fj0 - 1 + e % PR_2X;
             if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
                 i2[q] = gi * nxf + gj;
-                node_coef<float, float2>(gi, gj, nzf, nxf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c2[q]);
-                w2[q] = cc(wid[i2[q]], cj);
+                level_coef(gi, gj, i2[q], nzf, nxf, tcf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, c2[q],
+                           w2[q]);
             }
         }
     }
@@ -851,9 +875,9 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
                                                    const float2* __restrict__ isxe, const float2* __restrict__ isze,
                                                    const float* __restrict__ eta, const float2* __restrict__ kap,
                                                    float k0sq, float hh, float shift, int conj,
-                                                   const float2* __restrict__ wid, const float2* __restrict__ uc,
-                                                   const float2* __restrict__ u, const float2* __restrict__ f,
-                                                   float2* __restrict__ out) {
+                                                   const float2* __restrict__ wid, const float2* __restrict__ tcf,
+                                                   const float2* __restrict__ uc, const float2* __restrict__ u,
+                                                   const float2* __restrict__ f, float2* __restrict__ out) {
     __shared__ float2 sv2[P2_N];
     __shared__ float2 sv1[MT_NS];
     __shared__ float2 scv[MP_NC];
@@ -892,8 +916,8 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
             const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
             if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
                 i1[q] = gi * nx + gj;
-                node_coef<float, float2>(gi, gj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c1[q]);
-                w1[q] = cc(wid[i1[q]], cj);
+                level_coef(gi, gj, i1[q], nz, nx, tcf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, c1[q],
+                           w1[q]);
             }
         }
     }
@@ -1085,7 +1109,7 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                                L.wgt, L.wid);
             HELM_HIP(hipGetLastError());
         }
-        if (L.n <= tail_nodes()) {
+        if (l > 0 || L.n <= tail_nodes()) {  // the tail and the fused kernels of coarse grid levels read it
             if (!L.tcf) HELM_HIP(hipMalloc((void**)&L.tcf, 6 * L.n * sizeof(float2) + 256));
             hipLaunchKernelGGL(k_tail_coef, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.nz, L.nx, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0),
@@ -1174,12 +1198,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const int rbc = rhs_group(p, bc, nrhs);
         const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
         hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
-                           L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, f, L.mg_t,
-                           C.mg_f);
+                           L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
+                           l > 0 ? (const float2*)L.tcf : nullptr, f, L.mg_t, C.mg_f);
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
         hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
-                           L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, C.mg_u, L.mg_t, f, u);
+                           L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
+                           l > 0 ? (const float2*)L.tcf : nullptr, C.mg_u, L.mg_t, f, u);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }

@@ -172,6 +172,11 @@ static bool fused_levels() {
     static const bool f = env_int("HELM_MG_FUSED", 1, 0, 1) != 0;
     return f;
 }
+// stencil table on the finest grid level too (HELM_MG_TABLE0=1)
+static bool table0() {
+    static const bool t = env_int("HELM_MG_TABLE0", 0, 0, 1) != 0;
+    return t;
+}
 static int64_t tail_nodes() {
     static const int64_t t = env_int("HELM_MG_TAIL_NODES", TAIL_NODES_DEFAULT, 16, TAIL_NODES);
     return t;
@@ -1109,7 +1114,7 @@ int32_t mg_build(helm_plan* p, hipStream_t s) {
                                L.wgt, L.wid);
             HELM_HIP(hipGetLastError());
         }
-        if (l > 0 || L.n <= tail_nodes()) {  // the tail and the fused kernels of coarse grid levels read it
+        if (l > 0 || table0() || L.n <= tail_nodes()) {  // read by the tail and the fused kernels
             if (!L.tcf) HELM_HIP(hipMalloc((void**)&L.tcf, 6 * L.n * sizeof(float2) + 256));
             hipLaunchKernelGGL(k_tail_coef, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.nz, L.nx, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, (float)(p->k0 * p->k0),
@@ -1199,12 +1204,12 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
         hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                            L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
-                           l > 0 ? (const float2*)L.tcf : nullptr, f, L.mg_t, C.mg_f);
+                           (l > 0 || table0()) ? (const float2*)L.tcf : nullptr, f, L.mg_t, C.mg_f);
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
         hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                            L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
-                           l > 0 ? (const float2*)L.tcf : nullptr, C.mg_u, L.mg_t, f, u);
+                           (l > 0 || table0()) ? (const float2*)L.tcf : nullptr, C.mg_u, L.mg_t, f, u);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }

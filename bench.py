@@ -3,14 +3,15 @@
 Workload (default): 512^2 interior + 32-cell stretched PML (576^2 nodes),
 3 MHz, dx = lambda_min/10, seeded smooth random medium (c, rho, power-law
 alpha), 11 plane-wave angles in [-15, +15] deg from a 192-element line array,
-192 receivers; complex64 flexible GMRES(20) with FP64 inner products and
-FP64 true-residual refinement to relative residual 1e-6 on every angle.
+192 receivers; complex64 flexible GMRES(6) with FP64 inner products and
+FP64 true-residual refinement at every restart to relative residual 1e-6 on
+every angle; multigrid shift 0.4.
 
-"learned FGMRES" (the timed leg): a warm start of 8 minimal-residual steps
+"learned FGMRES" (the timed leg): a warm start of 4 minimal-residual steps
 along the learned directions MG(r) + N(r) (the BF16 U-Net trained by
 nsrno_amd/train_preconditioner.py, weights in nsrno_amd/data/), then
 multigrid-preconditioned FGMRES.  Multigrid FGMRES without the net and plain
-GMRES are timed beside it.
+GMRES(30) (no preconditioner) are timed beside it.
 
 A "step" = one full batched solve of the 11 angles from a zero start:
 coefficients + multigrid set-up on the device, source injection, warm start,
@@ -31,6 +32,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
+# profiled single kernels of the headline solve -> kernel names (the
+# "preconditioner" entry is the whole V-cycle, a group, so it is not one)
+names = {"dcgs_dots": "k_dcgs_dots", "dcgs_update": "k_dcgs_update", "operator_apply": "k_apply<float>",
+         "true_residual": "k_apply<double> (residual)", "mg_pre2_restrict": "k_mgt_pre2_rr",
+         "mg_prolong_post2": "k_mgt_post2"}
+# the multigrid kernels are timed on the finest level only inside the timed
+# steps (one event pair per V-cycle instead of one per level)
+finest = {"mg_pre2_restrict": "mg_pre2_restrict_finest", "mg_prolong_post2": "mg_prolong_post2_finest"}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -38,32 +48,37 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precond", default="mg", choices=["none", "mg", "learned"])
-    ap.add_argument("--restart", type=int, default=20)
+    ap.add_argument("--restart", type=int, default=6,
+                    help="FGMRES restart length (short cycles: each restart is an FP64 refinement step)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=6000)
     ap.add_argument("--graph", type=int, default=1, help="1: replay restart cycles as captured HIP graphs")
     ap.add_argument("--autotune", type=int, default=1, help="1: time launch shapes on the plan first (helm_autotune)")
     ap.add_argument("--warm", default="learned", choices=["learned", "mg", "none"],
                     help="warm-start directions before the FGMRES solve")
-    ap.add_argument("--warm-steps", type=int, default=8)
+    ap.add_argument("--warm-steps", type=int, default=4)
     ap.add_argument("--learned-params", default=os.path.join(ROOT, "nsrno_amd", "data",
                                                              "learned_warm_576x576_3mhz.npz"))
     ap.add_argument("--plain-max-iterations", type=int, default=6000,
                     help="iteration cap of the plain (unpreconditioned) GMRES comparison leg")
+    ap.add_argument("--plain-restart", type=int, default=30,
+                    help="restart length of the plain GMRES comparison leg (GMRES(30))")
     ap.add_argument("--interior", type=int, default=512)
     ap.add_argument("--pml", type=int, default=32)
     ap.add_argument("--freq", type=float, default=3.0)
     ap.add_argument("--angles", type=int, default=11)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mg-levels", type=int, default=5)
-    ap.add_argument("--mg-shift", type=float, default=0.5)
+    ap.add_argument("--mg-shift", type=float, default=0.4)
     ap.add_argument("--mg-omega", type=float, default=0.0)
     ap.add_argument("--mg-pre", type=int, default=2)
     ap.add_argument("--mg-post", type=int, default=2)
     ap.add_argument("--mg-coarse", type=int, default=10)
-    ap.add_argument("--roofline-kernel", default="dcgs_dots",
-                    choices=["dcgs_dots", "dcgs_update", "operator_apply", "true_residual"],
-                    help="kernel timed with HIP events inside the timed solves for the roofline object")
+    ap.add_argument("--roofline-kernel", default="auto",
+                    choices=["auto", "dcgs_dots", "dcgs_update", "operator_apply", "true_residual",
+                             "mg_pre2_restrict", "mg_prolong_post2"],
+                    help="kernel timed with HIP events inside the timed solves for the roofline object "
+                         "(auto: the single kernel with the most time in an all-kernel profiled solve)")
     ap.add_argument("--cond-shape", type=int, nargs=2, default=[960, 1216],
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
@@ -108,7 +123,7 @@ class Bench:
         self.rho = torch.from_numpy(w.rho).to(dev)
         self.alpha = torch.from_numpy(w.alpha_np_m).to(dev)
         self.plan = HelmholtzPlan(w.nz, w.nx, w.h_mm, w.frequency_mhz, w.pml_mm, c0_m_s=w.c0,
-                                  nrhs=a.angles, restart=a.restart)
+                                  nrhs=a.angles, restart=max(a.restart, a.plain_restart))
         self.R = a.angles
         self.n = w.nz * w.nx
         if a.warm == "learned":
@@ -120,6 +135,7 @@ class Bench:
 
     def step(self, precond, max_iterations, warm="none"):
         a, p = self.a, self.plan
+        restart = a.restart if precond != "none" else a.plain_restart
         p.set_medium_physical(self.c, self.rho, self.alpha, rho0=self.work.rho0)
         if precond != "none":
             p.setup_multigrid(levels=a.mg_levels, shift=a.mg_shift, pre=a.mg_pre, post=a.mg_post,
@@ -129,7 +145,7 @@ class Bench:
         if warm != "none":
             x0 = self.torch.zeros_like(rhs)
             p.warm_start(rhs, x0, a.warm_steps, precond=warm)
-        x, infos = p.solve(rhs, x0=x0, tolerance=a.tol, restart=a.restart, max_iterations=max_iterations,
+        x, infos = p.solve(rhs, x0=x0, tolerance=a.tol, restart=restart, max_iterations=max_iterations,
                            precond=precond, graph=bool(a.graph))
         traces = p.sample(self.ports, x)
         self.warm_applications = 2 * a.warm_steps if warm != "none" else 0
@@ -201,7 +217,7 @@ def _pmc_rows(directory, counter):
         with open(path) as fh:
             for r in csv.DictReader(fh):
                 if r.get("Counter_Name") == counter:
-                    out.append((r["Kernel_Name"], float(r["Counter_Value"])))
+                    out.append((r["Kernel_Name"], float(r["Counter_Value"]), int(r.get("Grid_Size") or 0)))
     return out
 
 
@@ -213,22 +229,34 @@ def pmc_summary(directory):
     scale = {}
     for counter, sub in (("FETCH_SIZE", "fetch_copy"), ("WRITE_SIZE", "write_copy")):
         rows = _pmc_rows(os.path.join(directory, sub), counter)
-        vals = sorted(v for _, v in rows)
+        vals = sorted(v for _, v, _ in rows)
         if not vals:
             raise SystemExit(f"no {counter} rows under {directory}/{sub}")
         big = vals[-COPY_REPS:]  # the calibration copies are the largest dispatches
         scale[counter] = COPY_BYTES / (sum(big) / len(big))
+    # matched in this order (a name that contains another comes first)
     short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_update": "k_dcgs_update", "k_apply<float": "k_apply<float>",
-             "k_mgt_jacobi": "k_mgt_jacobi", "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
+             "k_apply_stream": "k_apply_stream<float>",
+             "k_mgt_pre2_rr": "k_mgt_pre2_rr", "k_mgt_post2": "k_mgt_post2", "k_mgt_jacobi": "k_mgt_jacobi",
+             "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
              "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail", "k_cond8s": "k_cond8s<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
-    for counter, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench")):
-        for name, v in _pmc_rows(os.path.join(directory, sub), counter):
+    # multigrid kernels run on every level: their finest-level launches (the
+    # largest grid) are also summarised on their own ("<name> (finest level)")
+    biggest = collections.defaultdict(int)
+    rows_bench = {c: _pmc_rows(os.path.join(directory, sub), c)
+                  for c, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench"))}
+    for name, _, grid in rows_bench["FETCH_SIZE"]:
+        biggest[name] = max(biggest[name], grid)
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for name, v, grid in rows_bench[counter]:
             for key, label in short.items():
                 if key in name:
-                    t = totals[label][counter]
-                    t[0] += v * scale[counter]
-                    t[1] += 1
+                    for lab in ([label, label + " (finest level)"] if key.startswith("k_mgt") and grid == biggest[name]
+                                else [label]):
+                        t = totals[lab][counter]
+                        t[0] += v * scale[counter]
+                        t[1] += 1
                     break
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:  # kernels measured by earlier runs (other workloads) are kept
@@ -464,14 +492,19 @@ def main():
                                coarse_sweeps=a.mg_coarse, omega=a.mg_omega)
         tuning = b.plan.autotune(b.R)
         print(f"[bench] autotune {tuning}", flush=True)
-    # the roofline kernel is timed inside the timed steps; the other kernels'
-    # split comes from one extra profiled solve outside the timed region
-    dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations, profile=[a.roofline_kernel],
-                                   warm=a.warm)
-    kroof = b.kernel_stats[a.roofline_kernel]
-    applications = [i.applications + b.warm_applications for i in infos]
-    b.timed(a.precond, 1, 0, a.max_iterations, profile=list(b.plan.PROFILE_KERNELS), warm=a.warm)
+    # one all-kernel profiled solve (outside the timed region) gives the split
+    # and names the dominant single kernel; that kernel is then timed with HIP
+    # events inside the timed steps
+    b.timed(a.precond, 1, 0, a.max_iterations,
+            profile=[k for k in b.plan.PROFILE_KERNELS if not k.endswith("_finest")], warm=a.warm)
     kstats = b.kernel_stats
+    singles = [k for k in names if kstats.get(k, {}).get("launches")]
+    roof_name = max(singles, key=lambda k: kstats[k]["total_ms"]) if a.roofline_kernel == "auto" else a.roofline_kernel
+    timed_name = finest.get(roof_name, roof_name)
+    dt, x, infos, traces = b.timed(a.precond, a.steps, a.warmup, a.max_iterations, profile=[timed_name],
+                                   warm=a.warm)
+    kroof = b.kernel_stats[timed_name]
+    applications = [i.applications + b.warm_applications for i in infos]
     solves_per_s = b.R * a.steps / dt
     iters = [i.iterations for i in infos]
     rel = [i.relative_residual for i in infos]
@@ -492,22 +525,20 @@ def main():
             seconds_per_step=pdt, solves_per_s=b.R / pdt, iterations=[i.iterations for i in pinf],
             applications=[i.applications for i in pinf],
             max_relative_residual=max(i.relative_residual for i in pinf),
-            reasons=sorted(set(i.reason for i in pinf)), max_iterations=a.plain_max_iterations)
+            reasons=sorted(set(i.reason for i in pinf)), max_iterations=a.plain_max_iterations,
+            restart=a.plain_restart)
         print(f"[bench] plain: {pdt * 1e3:.1f} ms, iterations {result['plain_gmres']['iterations'][:3]}..., "
               f"max rel {result['plain_gmres']['max_relative_residual']:.2e}", flush=True)
-    # the dominant single kernel (the preconditioner entry is a group of
-    # multigrid kernels, so it is not a candidate)
-    names = {"dcgs_dots": "k_dcgs_dots", "dcgs_update": "k_dcgs_update", "operator_apply": "k_apply<float>",
-             "true_residual": "k_apply<double> (residual)"}
-    dom = max((k for k in kstats if k in names and kstats[k]["launches"]), key=lambda k: kstats[k]["total_ms"])
     ks = kroof
     achieved = ks["gb_s"]
-    roofline = {"bound": "hbm", "kernel": names[a.roofline_kernel], "achieved": round(achieved, 1),
+    kname = names[roof_name] + (" (finest level)" if timed_name != roof_name else "")
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": traffic_from_profile(names[a.roofline_kernel]),
+                "traffic": traffic_from_profile(kname),
                 "avg_launch_us": round(ks["avg_us"], 2),
                 "algorithmic_bytes_per_launch": int(ks["bytes"] / ks["launches"]), "launches": ks["launches"],
-                "dominant_single_kernel": names[dom]}
+                "share_of_kernel_time": round(kstats[roof_name]["total_ms"] / sum(
+                    v["total_ms"] for k, v in kstats.items() if k in names), 3)}
     result["kernels"] = {k: dict(launches=v["launches"], total_ms=round(v["total_ms"], 3),
                                  avg_us=round(v["avg_us"], 2), gb_s=round(v["gb_s"], 1))
                          for k, v in kstats.items() if v["launches"]}

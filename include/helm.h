@@ -357,8 +357,10 @@ typedef struct {
 } helm_kernel_stat;
 /* mask != 0 turns timing on for the kernels whose bit is set (bit k =
  * entry k of helm_profile_read: 0 operator apply, 1 preconditioner, 2 DCGS2
- * dots, 3 DCGS2 update, 4 true residual) and clears the totals; 0 turns it
- * off.  Each timed launch adds an event pair to the stream, so time only the
+ * dots, 3 DCGS2 update, 4 true residual, 5 multigrid smoothing + residual +
+ * restriction, 6 multigrid prolongation + smoothing, both on every level,
+ * 7 and 8 the same two on the finest level only) and clears the totals; 0
+ * turns it off.  Each timed launch adds an event pair to the stream, so time only the
  * kernels needed. */
 int32_t helm_profile(helm_plan* plan, int32_t mask);
 /* copies up to capacity entries (host array); *count = entries available */

@@ -36,6 +36,29 @@ __host__ __device__ inline void cfma(double2& a, double2 b, double2 c) {
     a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
     a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
 }
+// XCD-aware tile order: workgroups are handed to the 8 XCDs of the device in
+// turn (workgroup b to XCD b mod 8), each XCD with its own L2.  Remapping the
+// linear workgroup id so that each XCD works on one contiguous run of tiles
+// keeps the halos that neighbouring tiles share in one L2.  A bijection of
+// [0, total): XCD k takes ids [start_k, start_k + count_k).
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = true) {
+    if (!on) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+        bz = blockIdx.z;
+        return;
+    }
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int total = gx * gy * gridDim.z;
+    const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int q = total / 8, r = total % 8;
+    const int k = b % 8, slot = b / 8;
+    const int t = k < r ? k * (q + 1) + slot : r * (q + 1) + (k - r) * q + slot;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+}
+
 template <typename R> struct cplx;
 template <> struct cplx<float> { using type = float2; };
 template <> struct cplx<double> { using type = double2; };
@@ -145,7 +168,12 @@ struct Workspace {
 };
 
 // HIP-event timing of the kernels inside a solve (helm_profile)
-enum ProfKernel { PK_APPLY = 0, PK_PRECOND, PK_DOTS, PK_UPDATE, PK_RESIDUAL, PK_COUNT };
+enum ProfKernel {
+    PK_APPLY = 0, PK_PRECOND, PK_DOTS, PK_UPDATE, PK_RESIDUAL,
+    PK_MG_PRE, PK_MG_POST,    // fused multigrid kernels, every level
+    PK_MG_PRE0, PK_MG_POST0,  // the same kernels, finest level only
+    PK_COUNT
+};
 struct Profiler {
     bool on = false;
     int mask = 0;  // bit k: time kernel k

@@ -23,6 +23,8 @@
 #include <cstdlib>
 #include <cmath>
 
+#include <hip/hip_ext.h>
+
 #include "helm_coef.h"
 #include "helm_internal.h"
 
@@ -171,6 +173,14 @@ static int64_t wg_target() {
 static bool fused_levels() {
     static const bool f = env_int("HELM_MG_FUSED", 1, 0, 1) != 0;
     return f;
+}
+// XCD-aware tile order in the fused kernels (HELM_XCD_ORDER=1).  Off by
+// default: at 576^2 x 11 it cuts the finest level's post-smoothing traffic
+// from 208 to 136 MB per launch but the launch takes 3% longer (these
+// launches are bound by latency and occupancy, not by HBM bytes).
+static int xcd_order() {
+    static const int t = (int)env_int("HELM_XCD_ORDER", 0, 0, 1);
+    return t;
 }
 // stencil table on the finest grid level too (HELM_MG_TABLE0=1)
 static bool table0() {
@@ -745,12 +755,14 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
                                                      const float2* __restrict__ kap, float k0sq, float hh,
                                                      float shift, int conj, const float2* __restrict__ wid,
                                                      const float2* __restrict__ tcf, const float2* __restrict__ f,
-                                                     float2* __restrict__ u, float2* __restrict__ fcoarse) {
+                                                     float2* __restrict__ u, float2* __restrict__ fcoarse, int xcd) {
     __shared__ float2 sf[PR_1N];
     __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
     __shared__ float2 s2[PR_2N];
     const int tid = threadIdx.y * RR_TX + threadIdx.x;
-    const int I0 = blockIdx.y * RR_TZ, J0 = blockIdx.x * RR_TX;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz, xcd != 0);
+    const int I0 = by * RR_TZ, J0 = bx * RR_TX;
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
     const int64_t nf = (int64_t)nzf * nxf, nc = (int64_t)nzc * nxc;
     const bool cj = conj != 0;
@@ -787,7 +799,7 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
             }
         }
     }
-    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    const int r0 = bz * rb, r1 = min(nrhs, r0 + rb);
     float2 fq[PR_1PT];
 #pragma unroll
     for (int q = 0; q < PR_1PT; ++q) fq[q] = i1[q] >= 0 ? f[(int64_t)r0 * nf + i1[q]] : make_float2(0.f, 0.f);
@@ -882,11 +894,13 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
                                                    float k0sq, float hh, float shift, int conj,
                                                    const float2* __restrict__ wid, const float2* __restrict__ tcf,
                                                    const float2* __restrict__ uc, const float2* __restrict__ u,
-                                                   const float2* __restrict__ f, float2* __restrict__ out) {
+                                                   const float2* __restrict__ f, float2* __restrict__ out, int xcd) {
     __shared__ float2 sv2[P2_N];
     __shared__ float2 sv1[MT_NS];
     __shared__ float2 scv[MP_NC];
-    const int z0 = blockIdx.y * MT_TZ, x0 = blockIdx.x * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz, xcd != 0);
+    const int z0 = by * MT_TZ, x0 = bx * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
     const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
     const int64_t ncn = (int64_t)nzc * nxc;
     const bool cj = conj != 0;
@@ -926,7 +940,7 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
             }
         }
     }
-    const int r0 = blockIdx.z * rb, r1 = min(nrhs, r0 + rb);
+    const int r0 = bz * rb, r1 = min(nrhs, r0 + rb);
     float2 su[P2_PT], cs[MP_CPT], fv[MT_SPT];
     auto fetch = [&](int r) {
 #pragma unroll
@@ -1202,14 +1216,40 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const int64_t bc = (int64_t)((C.nx + RR_TX - 1) / RR_TX) * ((C.nz + RR_TZ - 1) / RR_TZ);
         const int rbc = rhs_group(p, bc, nrhs);
         const dim3 g((C.nx + RR_TX - 1) / RR_TX, (C.nz + RR_TZ - 1) / RR_TZ, (nrhs + rbc - 1) / rbc);
-        hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
-                           L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
-                           (l > 0 || table0()) ? (const float2*)L.tcf : nullptr, f, L.mg_t, C.mg_f);
+        const float2* tab = (l > 0 || table0()) ? (const float2*)L.tcf : nullptr;
+        // algorithmic bytes per launch (timing inside solves): each field read
+        // or written once, the stencil (table or eta, kappa, w/dg) once
+        Profiler* pf = (p->prof && p->prof->on) ? p->prof : nullptr;
+        const double nf = (double)L.n, ncn = (double)C.n, coef = (tab ? 48.0 : 20.0) * nf;
+        hipEvent_t ea, eb;
+        // timed as every level's launch, or as the finest level's only
+        const int kpre = (pf && (pf->mask >> PK_MG_PRE & 1)) ? PK_MG_PRE : (l == 0 ? PK_MG_PRE0 : -1);
+        const int kpost = (pf && (pf->mask >> PK_MG_POST & 1)) ? PK_MG_POST : (l == 0 ? PK_MG_POST0 : -1);
+        if (kpre >= 0 && prof_pair(pf, kpre, &ea, &eb)) {
+            hipExtLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
+                                  rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
+                                  (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
+                                  sh, (int)adj, (const float2*)L.wid, tab, f, L.mg_t, C.mg_f, xcd_order());
+            prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
+        } else {
+            hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
+                               L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
+                               L.mg_t, C.mg_f, xcd_order());
+        }
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
-        hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
-                           L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid,
-                           (l > 0 || table0()) ? (const float2*)L.tcf : nullptr, C.mg_u, L.mg_t, f, u);
+        if (kpost >= 0 && prof_pair(pf, kpost, &ea, &eb)) {
+            hipExtLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, ea, eb, 0, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb,
+                                  (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
+                                  (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
+                                  sh, (int)adj, (const float2*)L.wid, tab, (const float2*)C.mg_u,
+                                  (const float2*)L.mg_t, f, u, xcd_order());
+            prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
+        } else {
+            hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
+                               L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
+                               L.mg_t, f, u, xcd_order());
+        }
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }

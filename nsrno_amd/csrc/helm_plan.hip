@@ -250,7 +250,8 @@ int32_t helm_profile(helm_plan* p, int32_t mask) {
 int32_t helm_profile_read(helm_plan* p, helm_kernel_stat* stats, int32_t capacity, int32_t* count) {
     HELM_CHECK(p && count && (stats || capacity == 0), HELM_ERR_ARG, "helm_profile_read: null argument");
     static const char* names[PK_COUNT] = {"operator_apply", "preconditioner", "dcgs_dots", "dcgs_update",
-                                          "true_residual"};
+                                          "true_residual", "mg_pre2_restrict", "mg_prolong_post2",
+                                          "mg_pre2_restrict_finest", "mg_prolong_post2_finest"};
     *count = PK_COUNT;
     Profiler* pf = p->prof;
     for (int k = 0; k < PK_COUNT && k < capacity; ++k) {

@@ -190,7 +190,8 @@ class HelmholtzPlan:
         return dict(stencil_rows=t.stencil_rows, stencil_rhs=t.stencil_rhs, mg_workgroups=t.mg_workgroups,
                     stencil_us=t.stencil_us, vcycle_us=t.vcycle_us)
 
-    PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual")
+    PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual",
+                       "mg_pre2_restrict", "mg_prolong_post2", "mg_pre2_restrict_finest", "mg_prolong_post2_finest")
 
     def profile(self, kernels=None, enable=True):
         """Turn HIP-event timing of kernels inside solves on (clearing totals) or off.

@@ -289,10 +289,10 @@ def frequency_sweep(a):
     freqs = np.linspace(4.0, 8.9, a.frequencies)
     angles = np.linspace(-15.0, 15.0, a.angles)
     for _ in range(a.warmup):  # one small frequency warms the kernels and allocator
-        run_sweep(freqs[:1], angles, tol=a.tol, restart=a.restart, workers=1)
+        run_sweep(freqs[:1], angles, tol=a.tol, restart=a.restart, workers=1, mg_shift=a.mg_shift)
     cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers,
                             log=lambda m: print("[bench] " + m, flush=True), autotune=bool(a.autotune),
-                            graph=bool(a.graph))
+                            graph=bool(a.graph), mg_shift=a.mg_shift)
     solves = len(freqs) * len(angles)
     kbytes = sum(c.krylov_bytes for c in cases)
     line = {
@@ -318,7 +318,7 @@ def conditioned_workload(a):
     dense_wave.solve_system): native medium 960 x 1216 at dx = 50 um in a
     1024 x 1280 box, 24-cell split PML, dt 2.5 ns, bin 360 (5.997 MHz),
     seeded smooth random c, rho and alpha, 11 plane-wave source lines;
-    multigrid FGMRES(24) with FP64 refinement to 1e-5 on every angle.  A step
+    multigrid FGMRES(8) with FP64 refinement to 1e-5 on every angle.  A step
     = coefficients + multigrid set-up + source term + solve.  One JSON line,
     with the FP32 forward/adjoint operator kernel against the HBM roofline and
     the oracle's Fourier-form operator (the reference's algorithm, 11 FFTs per
@@ -345,7 +345,7 @@ def conditioned_workload(a):
     g = FrequencyGeometry(spec)
     medium = torch.tensor(np.stack((c, rho)), dtype=torch.float64, device="cuda")
     alpha_d = torch.tensor(alpha, dtype=torch.float64, device="cuda")
-    system = ConditionedFrequencySystem(g, medium, alpha_d, f, nrhs=R, restart=24)
+    system = ConditionedFrequencySystem(g, medium, alpha_d, f, nrhs=R, restart=8)
     x = (np.arange(w) - (w - 1) / 2) * dx
     taper = np.hanning(w)
     lines = np.stack([taper * np.exp(-2j * np.pi * f * x * np.sin(np.deg2rad(t)) / 1540.0)
@@ -358,7 +358,7 @@ def conditioned_workload(a):
     def step():
         system.update(medium, alpha_d)
         b = source_rhs(system, lines_d)
-        return solve_system(system, b, tolerance=a.cond_tol, mixed=True, max_cycles=400)
+        return solve_system(system, b, tolerance=a.cond_tol, mixed=True, max_cycles=1200)
 
     for _ in range(a.warmup):
         step()
@@ -413,7 +413,7 @@ def conditioned_workload(a):
         "data": "synthetic (seeded smooth random c, rho, alpha)",
         "config": {"workload": f"conditioned {h}x{w} medium, box {g.H}x{g.W}, 24-cell split PML, "
                                f"{f / 1e6:.3f} MHz x{R} angles", "tolerance": a.cond_tol,
-                   "method": "fgmres(24) + multigrid", "parallelism": "single GPU"},
+                   "method": "fgmres(8) + multigrid", "parallelism": "single GPU"},
         "steps_per_solve": info["steps"], "relative_residual": info["residual"],
         "max_angle_residual": info["max_angle_residual"], "reason": info["reason"],
         "roofline": {"bound": "hbm", "kernel": "k_cond8s (FP32 streaming) forward, all RHS",

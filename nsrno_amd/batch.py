@@ -83,7 +83,8 @@ def _mg_levels(n, coarsest=40):
     return levels
 
 
-def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False):
+def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False,
+               mg_shift=0.4):
     """One frequency: all angles as one batched multigrid-FGMRES solve."""
     p = medium.pml
     n = case.interior + 2 * p
@@ -97,7 +98,7 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
                          restart=restart)
     ports = Ports(weights)
     plan.set_medium_physical(torch.from_numpy(c).cuda(), torch.from_numpy(rho).cuda(), None, rho0=work.rho0)
-    plan.setup_multigrid(levels=_mg_levels(n), shift=0.5, pre=2, post=2, coarse_sweeps=10)
+    plan.setup_multigrid(levels=_mg_levels(n), shift=mg_shift, pre=2, post=2, coarse_sweeps=10)
     if autotune:
         case.tuning = plan.autotune(len(angles))
     rhs = plan.inject(ports, torch.from_numpy(amp).cuda())
@@ -120,7 +121,8 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     return case
 
 
-def run_sweep(frequencies, angles, tol=1e-6, restart=20, workers=4, seed=0, log=None, autotune=True, graph=False):
+def run_sweep(frequencies, angles, tol=1e-6, restart=6, workers=4, seed=0, log=None, autotune=True, graph=False,
+              mg_shift=0.4):
     """Solve every frequency (largest grids first) on `workers` concurrent
     streams.  Returns (cases in input order, wall seconds)."""
     medium = PhysicalMedium(seed=seed)
@@ -140,7 +142,8 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=20, workers=4, seed=0, log=
                 except queue.Empty:
                     return
                 try:
-                    solve_case(medium, cases[k], angles, tol, restart, stream=stream, autotune=autotune, graph=graph)
+                    solve_case(medium, cases[k], angles, tol, restart, stream=stream, autotune=autotune, graph=graph,
+                               mg_shift=mg_shift)
                     if log:
                         c = cases[k]
                         log(f"{c.frequency_mhz:.3f} MHz {c.interior}^2 steps {max(c.iterations)} "

@@ -86,7 +86,7 @@ class ConditionedFrequencySystem:
     dense_wave.py:56,253).  nrhs / restart size the plan's Krylov workspace.
     """
 
-    def __init__(self, geometry, medium, alpha, frequency, *, nrhs=16, restart=24, multigrid=None):
+    def __init__(self, geometry, medium, alpha, frequency, *, nrhs=16, restart=8, multigrid=None):
         g = geometry
         if tuple(medium.shape) != (2, g.h, g.w):
             raise ValueError("Expected native [2,z,x] medium")
@@ -196,8 +196,8 @@ def scattered_rhs(system, reference_system, pref, line):
             + reference_system(pref))
 
 
-def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5, max_cycles=100, warm_steps=8,
-                 mixed=False, method="gmres", initial=None, restart=24):
+def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5, max_cycles=300, warm_steps=8,
+                 mixed=False, method="gmres", initial=None, restart=8):
     """dense_wave.solve_system (:167-225) on the GPU.
 
     rhs [R, H, W] (complex64 or complex128).  Every right-hand side runs its
@@ -207,8 +207,10 @@ def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5
     is the plan's multigrid V-cycle; the reference's learned network is not
     accepted here (``preconditioner`` must be None, and ``warm_steps`` is then
     unused, as in the reference).  ``mixed`` only changes the reported dtype:
-    the solve always runs FP32 vectors with FP64 refinement.  Returns
-    (value, info) with the reference's info keys.
+    the solve always runs FP32 vectors with FP64 refinement.  FGMRES(8) by
+    default (the reference's GMRES(24) budget ``max_cycles`` x 24 steps is
+    kept: ``max_cycles`` counts cycles of ``restart`` steps, default 300).
+    Returns (value, info) with the reference's info keys.
     """
     if preconditioner is not None:
         raise ValueError("conditioned solves take the multigrid preconditioner; pass preconditioner=None")
@@ -255,7 +257,7 @@ class DenseCase:
     reference system (c 1540, rho 1000, alpha .002).  The system's plan is
     created once and updated with each new medium."""
 
-    def __init__(self, geometry, frequency, lines, pref=None, *, restart=24, multigrid=None):
+    def __init__(self, geometry, frequency, lines, pref=None, *, restart=8, multigrid=None):
         g = geometry
         self.g = g
         self.frequency = float(frequency)

@@ -182,6 +182,18 @@ static int xcd_order() {
     static const int t = (int)env_int("HELM_XCD_ORDER", 0, 0, 1);
     return t;
 }
+// post-smoothing kernel variant: without prefetching (default) it fits three
+// waves per SIMD, which measured 25% faster at 576^2 x 11 than the prefetching
+// variant's two (HELM_MG_POST_PREFETCH=1)
+static bool post2_prefetch() {
+    static const bool t = env_int("HELM_MG_POST_PREFETCH", 0, 0, 1) != 0;
+    return t;
+}
+// pre-smoothing kernel variant (HELM_MG_PRE_PREFETCH=1: next right-hand side prefetched)
+static bool pre2_prefetch() {
+    static const bool t = env_int("HELM_MG_PRE_PREFETCH", 0, 0, 1) != 0;
+    return t;
+}
 // stencil table on the finest grid level too (HELM_MG_TABLE0=1)
 static bool table0() {
     static const bool t = env_int("HELM_MG_TABLE0", 0, 0, 1) != 0;
@@ -748,7 +760,9 @@ constexpr int PR_RZ = 2 * RR_TZ + 1, PR_RX = 2 * RR_TX + 1;  // residual region 
 constexpr int PR_2Z = PR_RZ + 2, PR_2X = PR_RX + 2, PR_2N = PR_2Z * PR_2X;  // u2 region 19 x 67
 constexpr int PR_1Z = PR_RZ + 4, PR_1X = PR_RX + 4, PR_1N = PR_1Z * PR_1X;  // f, u1 region 21 x 69
 constexpr int PR_1PT = (PR_1N + 255) / 256, PR_2PT = (PR_2N + 255) / 256;
-__global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs, int rb,
+template <bool PF>
+__global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs,
+                                                               int rb,
                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
                                                      const float2* __restrict__ isxe,
                                                      const float2* __restrict__ isze, const float* __restrict__ eta,
@@ -801,9 +815,13 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
     }
     const int r0 = bz * rb, r1 = min(nrhs, r0 + rb);
     float2 fq[PR_1PT];
+    auto fetch = [&](int r) {
 #pragma unroll
-    for (int q = 0; q < PR_1PT; ++q) fq[q] = i1[q] >= 0 ? f[(int64_t)r0 * nf + i1[q]] : make_float2(0.f, 0.f);
+        for (int q = 0; q < PR_1PT; ++q) fq[q] = i1[q] >= 0 ? f[(int64_t)r * nf + i1[q]] : make_float2(0.f, 0.f);
+    };
+    if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
+        if (!PF) fetch(r);
         __syncthreads();  // the previous RHS is done with the LDS arrays
 #pragma unroll
         for (int q = 0; q < PR_1PT; ++q) {
@@ -813,11 +831,7 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
                 s1[e] = cmul(w1[q], fq[q]);  // u1 = (w/dg) f (0 on ghosts)
             }
         }
-        if (r + 1 < r1) {
-#pragma unroll
-            for (int q = 0; q < PR_1PT; ++q)
-                fq[q] = i1[q] >= 0 ? f[(int64_t)(r + 1) * nf + i1[q]] : make_float2(0.f, 0.f);
-        }
+        if (PF && r + 1 < r1) fetch(r + 1);
         __syncthreads();
         // u2 = u1 + (w/dg) (f - M u1) on the u2 region
         float2 v2[PR_2PT];
@@ -886,8 +900,13 @@ __global__ __launch_bounds__(256) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, 
 // kernels, 18 x 66), the second on the tile.  Each thread keeps the stencil of
 // its halo-1 nodes and uses it for both sweeps.  out must not alias u.
 constexpr int P2_X = MT_TX + 4, P2_Z = MT_TZ + 4, P2_N = P2_X * P2_Z;
+
 constexpr int P2_PT = (P2_N + 255) / 256;
-__global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs, int rb,
+// PF: the loads of the next right-hand side are issued before this one is
+// computed (more registers, two waves per SIMD); without, three waves per SIMD
+template <bool PF>
+__global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
+                                                             int rb,
                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
                                                    const float2* __restrict__ isxe, const float2* __restrict__ isze,
                                                    const float* __restrict__ eta, const float2* __restrict__ kap,
@@ -950,8 +969,9 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) fv[q] = i1[q] >= 0 ? f[(int64_t)r * n + i1[q]] : make_float2(0.f, 0.f);
     };
-    fetch(r0);
+    if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
+        if (!PF) fetch(r);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < MP_CPT; ++q) {
@@ -981,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_mgt_post2(int nz, int nx, int64_t n, in
         float2 fc[MT_SPT];
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) fc[q] = fv[q];
-        if (r + 1 < r1) fetch(r + 1);
+        if (PF && r + 1 < r1) fetch(r + 1);
         __syncthreads();
         // first sweep on the halo-1 block
 #pragma unroll
@@ -1226,27 +1246,31 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const int kpre = (pf && (pf->mask >> PK_MG_PRE & 1)) ? PK_MG_PRE : (l == 0 ? PK_MG_PRE0 : -1);
         const int kpost = (pf && (pf->mask >> PK_MG_POST & 1)) ? PK_MG_POST : (l == 0 ? PK_MG_POST0 : -1);
         if (kpre >= 0 && prof_pair(pf, kpre, &ea, &eb)) {
-            hipExtLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
+            hipExtLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ),
+                                  0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
                                   rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, f, L.mg_t, C.mg_f, xcd_order());
             prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
         } else {
-            hipLaunchKernelGGL(k_mgt_pre2_rr, g, dim3(RR_TX, RR_TZ), 0, s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
+            hipLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ), 0,
+                               s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
                                L.mg_t, C.mg_f, xcd_order());
         }
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
         if (kpost >= 0 && prof_pair(pf, kpost, &ea, &eb)) {
-            hipExtLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, ea, eb, 0, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb,
+            hipExtLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, ea, eb, 0,
+                                  L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb,
                                   (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, (const float2*)C.mg_u,
                                   (const float2*)L.mg_t, f, u, xcd_order());
             prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
         } else {
-            hipLaunchKernelGGL(k_mgt_post2, grd, blk, 0, s, L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
+            hipLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, L.nz, L.nx,
+                               L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                                L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
                                L.mg_t, f, u, xcd_order());
         }

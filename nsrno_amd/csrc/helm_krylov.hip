@@ -31,6 +31,7 @@
 // sums (one workgroup per 4096-node chunk, per vector group, per RHS) then a
 // fixed-order sum over chunks; no atomics, so results are bit reproducible.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include <hip/hip_ext.h>
@@ -94,10 +95,22 @@ __device__ double reduce_chunks(const double2* src, int nch, double* lds) {
 // next vector prefetched while the current one is reduced, so a launch reads
 // exactly j + 2 vectors.  Per k the four sums are reduced within each wave
 // (shuffles); the four waves' results meet in LDS once, after the k loop.
-constexpr int DOT_CHUNK = 2048;           // nodes per dot-product workgroup
-constexpr int DOT_NPT = DOT_CHUNK / 256;  // nodes per thread
-static inline int ndotchunks(int64_t n) { return (int)((n + DOT_CHUNK - 1) / DOT_CHUNK); }
-__global__ __launch_bounds__(256) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax, int j,
+// nodes per thread: 8 (2048-node chunks, 3 waves per SIMD) or 4 (1024-node
+// chunks, twice the workgroups and partial sums, more waves); HELM_DOT_NPT
+static int dot_npt() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_DOT_NPT");
+        const int x = v ? std::atoi(v) : 0;
+        return x == 4 ? 4 : 8;
+    }();
+    return t;
+}
+static inline int ndotchunks(int64_t n) { return (int)((n + 256 * dot_npt() - 1) / (256 * dot_npt())); }
+// partial-sum capacity: enough chunks for the smaller chunk size
+static inline int ndotchunks_max(int64_t n) { return (int)((n + 1023) / 1024); }
+template <int DOT_NPT>
+__global__ __launch_bounds__(256, DOT_NPT == 8 ? 3 : 4) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax,
+                                                                          int j,
                                                    const float2* __restrict__ V, const float2* __restrict__ w,
                                                    const int* __restrict__ cyc, double2* __restrict__ pc,
                                                    double2* __restrict__ pd) {
@@ -105,7 +118,7 @@ __global__ __launch_bounds__(256) void k_dcgs_dots(int64_t n, int nrhs, int nch,
     const int chunk = blockIdx.x, r = blockIdx.y;
     if (!cyc[r]) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t beg = (int64_t)chunk * DOT_CHUNK;
+    const int64_t beg = (int64_t)chunk * (256 * DOT_NPT);
     const int64_t vstride = (int64_t)nrhs * n;
     const float2* wr = w + (int64_t)r * n;
     const float2* vr = V + (int64_t)r * n;
@@ -514,7 +527,7 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.w, (size_t)nrhs * n));
     HELM_TRY(alloc(&w.r64, (size_t)nrhs * n));
     // <V_k,V_j> | <V_k,w> (per dot chunk) | |w|^2 (per chunk)
-    w.partial_len = (int64_t)nrhs * (2 * kmax * ndotchunks(n) + nch);
+    w.partial_len = (int64_t)nrhs * (2 * kmax * ndotchunks_max(n) + nch);
     HELM_TRY(alloc(&w.partial, (size_t)w.partial_len));
     HELM_TRY(alloc(&w.H, (size_t)nrhs * restart * (restart + 1)));
     HELM_TRY(alloc(&w.cs, (size_t)nrhs * restart));
@@ -748,12 +761,14 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                 if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, cs);
                 hipEvent_t ea, eb;
                 if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
-                    hipExtLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs, nchd, kmax, j,
+                    hipExtLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs,
+                                          ea, eb, 0, n, nrhs, nchd, kmax, j,
                                           (const float2*)W.V, (const float2*)W.w, (const int*)W.cyc, W.partial,
                                           pd_part);
                     prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
                 } else {
-                    hipLaunchKernelGGL(k_dcgs_dots, dim3(nchd, nrhs), blk, 0, cs, n, nrhs, nchd, kmax, j, W.V, W.w,
+                    hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs, n,
+                                       nrhs, nchd, kmax, j, W.V, W.w,
                                        W.cyc, W.partial, pd_part);
                 }
                 hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, cs, nchd, mm, kmax, j, o->tolerance,

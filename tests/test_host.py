@@ -108,3 +108,30 @@ def test_receiver_response_closed_forms():
         w = 2 * np.pi * f * dt
         fir = float(np.sum(taps * np.cos(w * (np.arange(201) - 100))))
         assert abs(receiver_response(f, dt) - band2 * fir) < 1e-9
+
+
+def test_conditioned_geometry_and_validation_on_cpu():
+    """Host logic of the dense-path mirror (no GPU needed): the padded box,
+    mask, face row and receiver columns of FrequencyGeometry
+    (conditioned_wave.py:12-31), its padding guard, and the medium-shape
+    ValueError of ConditionedFrequencySystem (:54), raised before any plan."""
+    import json
+    import torch
+    from nsrno_amd.conditioned import ConditionedFrequencySystem, FrequencyGeometry
+    from oracle import conditioned as oc
+    gold = np.load(Path(__file__).parent / "golden" / "conditioned_small.npz")
+    spec = json.loads(str(gold["spec_json"]))
+    g = FrequencyGeometry(spec, device="cpu")
+    o = oc.Geometry(spec)
+    assert (g.H, g.W, g.pz, g.px, g.face) == (o.H, o.W, o.pz, o.px, o.face)
+    np.testing.assert_array_equal(g.mask.numpy(), gold["mask"])
+    np.testing.assert_array_equal(g.rx_cols.numpy(), gold["rx_cols"])
+    assert g.pad == (o.px, o.W - o.w - o.px, o.pz, o.H - o.h - o.pz)
+    with pytest.raises(ValueError, match="FFT padding"):
+        FrequencyGeometry({**spec, "fft_shape": [70, 128]}, device="cpu")
+    with pytest.raises(ValueError, match="Expected native"):
+        ConditionedFrequencySystem(g, torch.zeros(2, 10, 10, dtype=torch.float64), torch.zeros(10, 10), 5e6)
+    # crop + replicate of the coefficients (conditioned_wave.py:45-49)
+    box = g.coefficients(torch.tensor(np.stack((gold["c"], gold["rho"]))))
+    np.testing.assert_array_equal(box[0].numpy(), gold["sc"])
+    np.testing.assert_array_equal(box[1].numpy(), gold["srho"])

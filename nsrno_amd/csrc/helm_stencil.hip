@@ -61,7 +61,11 @@ __global__ __launch_bounds__(256) void k_apply(int nz, int nx, int64_t n, int nr
 // are formed once per node for the whole group.  Each u value is read from
 // HBM once, plus (tz+2)/tz for the run's halo rows.  Each row step waits for
 // one memory latency, so short runs (8 rows) with many wavefronts in flight
-// do best.
+// do best.  eta is carried down the column in a register window and its
+// west/east values come from the neighbouring lanes, so a node's stencil
+// costs two loads (eta of row i+2, kappa) instead of the seven of node_coef.
+// (Loading the edge lanes' halo values two rows ahead as well costs more
+// registers than it saves: measured 10% slower.)
 __device__ __forceinline__ float2 shfl_up1(float2 v) {
     return make_float2(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64));
 }
@@ -86,10 +90,14 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
     const bool colok = j < nx;
     const int jj = colok ? j : nx - 1;  // lanes past the grid mirror the last column (results dropped)
     const bool hw = jj > 0, he = jj < nx - 1;
-    const float2 zero = make_float2(0.f, 0.f);
-    auto ld = [&](int r, int i, int col) -> float2 {
-        return (r < nrhs && i >= 0 && i < nz) ? u[(int64_t)r * n + (int64_t)i * nx + col] : zero;
+    const bool cj = conj != 0;
+    auto ld = [=](int r, int i, int col) -> float2 {
+        return (r < nrhs && i >= 0 && i < nz) ? u[(int64_t)r * n + (int64_t)i * nx + col] : make_float2(0.f, 0.f);
     };
+    // eta with the operator's ghost value 1 outside the grid
+    auto le = [=](int i, int col) -> float { return (i >= 0 && i < nz) ? eta[(int64_t)i * nx + col] : 1.f; };
+    // per-column stretch factors of this lane (constant down the run)
+    const float2 sxj = sx[jj], iw = isxe[jj], ie = isxe[jj + 1];
     float2 un[RB], uc[RB], us[RB], up[RB];
 #pragma unroll
     for (int q = 0; q < RB; ++q) {
@@ -97,16 +105,44 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
         uc[q] = ld(r0 + q, i0, jj);
         us[q] = ld(r0 + q, i0 + 1, jj);
     }
+    // eta window down the column: rows i-1, i, i+1
+    float en = le(i0 - 1, jj), ec = le(i0, jj), es = le(i0 + 1, jj);
     for (int i = i0; i < i1; ++i) {
         float2 ew[RB], ee[RB];
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             up[q] = ld(r0 + q, i + 2, jj);
-            ew[q] = (lane == 0 && x0 > 0) ? ld(r0 + q, i, x0 - 1) : zero;
-            ee[q] = (lane == 63 && x0 + 64 < nx) ? ld(r0 + q, i, x0 + 64) : zero;
+            ew[q] = (lane == 0 && x0 > 0) ? ld(r0 + q, i, x0 - 1) : make_float2(0.f, 0.f);
+            ee[q] = (lane == 63 && x0 + 64 < nx) ? ld(r0 + q, i, x0 + 64) : make_float2(0.f, 0.f);
         }
+        const float eup = le(i + 2, jj);
+        const float2 kc = kap[(int64_t)i * nx + jj];
+        // west/east eta from the neighbouring lanes (edge lanes load theirs)
+        float eww = __shfl_up(ec, 1, 64), eee = __shfl_down(ec, 1, 64);
+        if (lane == 0) eww = hw ? eta[(int64_t)i * nx + jj - 1] : 1.f;
+        if (lane == 63) eee = he ? eta[(int64_t)i * nx + jj + 1] : 1.f;
+        if (!hw) eww = 1.f;
+        if (!he) eee = 1.f;
+        // the node's stencil, as node_coef forms it
+        const float2 szi = sz[i], in_ = isze[i], is_ = isze[i + 1];
         Coef<float2> c;
-        node_coef<float, float2>(i, jj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj != 0, c);
+        c.fw = cscale(cmul(szi, iw), hh * (ec + eww));
+        c.fe = cscale(cmul(szi, ie), hh * (ec + eee));
+        c.fn = cscale(cmul(sxj, in_), hh * (ec + en));
+        c.fs = cscale(cmul(sxj, is_), hh * (ec + es));
+        float2 mass = cscale(cmul(cmul(szi, sxj), kc), k0sq);
+        if (shift != 0.f) {
+            const float2 m2 = make_float2(mass.x - shift * mass.y, mass.y + shift * mass.x);
+            mass = m2;
+        }
+        c.dg = csub(mass, cadd(cadd(c.fw, c.fe), cadd(c.fn, c.fs)));
+        if (cj) {
+            c.dg = cconj(c.dg);
+            c.fw = cconj(c.fw);
+            c.fe = cconj(c.fe);
+            c.fn = cconj(c.fn);
+            c.fs = cconj(c.fs);
+        }
         const bool hn = i > 0, hs = i < nz - 1;
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
@@ -128,6 +164,9 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
             uc[q] = us[q];
             us[q] = up[q];
         }
+        en = ec;
+        ec = es;
+        es = eup;
     }
 }
 

@@ -329,7 +329,8 @@ int32_t helm_warm_start(helm_plan* plan, int32_t precond, const void* b, void* x
 
 /* ---- autotuning of launch shapes: no counterpart in the reference.  Times,
  * with HIP events on the plan's own grid and nrhs right-hand sides, the
- * FP32 stencil's streaming shapes (rows per run x right-hand sides per group)
+ * FP32 stencil's streaming shapes (rows per run x right-hand sides per group,
+ * streaming or block form)
  * and the multigrid grid-level workgroup target, and keeps the fastest in the
  * plan (later launches use them).  Call after the medium and the multigrid
  * are set; allocates two scratch fields for the duration of the call. */
@@ -339,8 +340,17 @@ typedef struct {
     int64_t mg_workgroups;     /* workgroups a multigrid launch aims for     */
     double stencil_us;         /* the chosen stencil's time per apply        */
     double vcycle_us;          /* the chosen V-cycle's time                  */
+    int32_t stencil_block;     /* 1: the block form (whole run loaded first),
+                                  2: the same in XCD-contiguous order        */
 } helm_tuning;
 int32_t helm_autotune(helm_plan* plan, int32_t nrhs, helm_tuning* result, void* stream);
+
+/* ---- fixed launch shape of the FP32 flux stencil: no counterpart in the
+ * reference.  form 0: streaming (rows 2..1024 per run), 1: block (rows 2, 4,
+ * 8 or 16 loaded up front), 2: block in XCD-contiguous workgroup order; rhs
+ * 1, 2 or 4 per group.  rows = 0 returns to the built-in default.  For tests
+ * and for callers that tuned once and reuse the result. */
+int32_t helm_set_stencil_shape(helm_plan* plan, int32_t rows, int32_t rhs, int32_t form);
 
 /* ---- in-solve kernel timing: no counterpart in the reference (its path
  * has no device kernels); for benchmarks that need one kernel's duration

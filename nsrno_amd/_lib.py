@@ -55,7 +55,7 @@ class HelmSolveInfo(ctypes.Structure):
 
 class HelmTuning(ctypes.Structure):
     _fields_ = [("stencil_rows", c_int32), ("stencil_rhs", c_int32), ("mg_workgroups", ctypes.c_int64),
-                ("stencil_us", c_double), ("vcycle_us", c_double)]
+                ("stencil_us", c_double), ("vcycle_us", c_double), ("stencil_block", c_int32)]
 
 
 class HelmKernelStat(ctypes.Structure):
@@ -98,6 +98,7 @@ SIGNATURES = {
     "helm_spectral_incident": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "helm_spectral_rhs": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_autotune": (c_int32, [c_void_p, c_int32, POINTER(HelmTuning), c_void_p]),
+    "helm_set_stencil_shape": (c_int32, [c_void_p, c_int32, c_int32, c_int32]),
     "helm_profile": (c_int32, [c_void_p, c_int32]),
     "helm_profile_read": (c_int32, [c_void_p, POINTER(HelmKernelStat), c_int32, POINTER(c_int32)]),
     "helm_warm_start": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p]),

@@ -124,6 +124,8 @@ struct Level {
     int cbox[4] = {0, 0, 0, 0};
     // streaming-stencil launch shape chosen by helm_autotune (0: default)
     int st_tz = 0, st_rb = 0;
+    bool st_block = false;  // FP32 flux stencil: block form (whole run loaded up front)
+    bool st_xcd = false;    // block form in XCD-contiguous workgroup order
 };
 
 struct Workspace {

@@ -355,6 +355,163 @@ __global__ __launch_bounds__(1024) void k_tail(const LevelView* __restrict__ vie
     for (int64_t q = threadIdx.x; q < T.n; q += blockDim.x) u_top[(int64_t)r * T.n + q] = lds[q];
 }
 
+// ---- the same V-cycle with every operand in LDS --------------------------
+// k_tail's sweeps each wait on global (L2) loads of the six coefficients and
+// f per node, so at 37^2 with 10 coarse sweeps the launch is ~20 us of load
+// latency for under 1 MB of data.  This form copies the coefficients of every
+// tail level (as six planes, so a wavefront reads consecutive words) and the
+// top level's f into LDS once, keeps the residual and the restricted
+// right-hand sides there too, and touches global memory only for that copy
+// and the final u: 72 B of LDS per node over the tail levels (u, partner, f,
+// six coefficients), used whenever that fits 160 KB.
+// row of node q on a level at most TAIL_NODES large: floor((q + 1/2) / nx) in
+// FP32 (the rounding error, under 1e-3, stays below the 1/(2 nx) margin)
+__device__ __forceinline__ int tail_row(int q, float inx) { return (int)(((float)q + 0.5f) * inx); }
+
+__device__ __forceinline__ float2 tail_mu_p(int nz, int nx, int n, int q, int i, const float2* cf, const float2* u,
+                                            bool conj) {
+    const int j = q - i * nx;
+    float2 acc = cmul(cc(cf[n + q], conj), u[q]);
+    if (j > 0) cfma(acc, cc(cf[2 * n + q], conj), u[q - 1]);
+    if (j < nx - 1) cfma(acc, cc(cf[3 * n + q], conj), u[q + 1]);
+    if (i > 0) cfma(acc, cc(cf[4 * n + q], conj), u[q - nx]);
+    if (i < nz - 1) cfma(acc, cc(cf[5 * n + q], conj), u[q + nx]);
+    return acc;
+}
+
+// k_tail's sweep sequence (same ping-pong order) on plane-layout coefficients
+__device__ void tail_sweeps_p(int nz, int nx, int n, const float2* cf, int sweeps, bool zero_start, bool conj,
+                              const float2* f, float2* u, float2* t) {
+    const float inx = 1.f / (float)nx;
+    const float2* src = zero_start ? nullptr : u;
+    if (!zero_start && sweeps % 2 == 1) {
+        for (int q = threadIdx.x; q < n; q += blockDim.x) t[q] = u[q];
+        __syncthreads();
+        src = t;
+    }
+    for (int k = 0; k < sweeps; ++k) {
+        float2* dst = zero_start ? ((sweeps - k) % 2 == 1 ? u : t) : (src == u ? t : u);
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            const float2 fv = f[q];
+            if (src == nullptr) {
+                dst[q] = cmul(cc(cf[q], conj), fv);
+            } else {
+                float2 nu = src[q];
+                cfma(nu, cc(cf[q], conj), csub(fv, tail_mu_p(nz, nx, n, q, tail_row(q, inx), cf, src, conj)));
+                dst[q] = nu;
+            }
+        }
+        __syncthreads();
+        src = dst;
+    }
+    if (zero_start && sweeps == 0) {
+        for (int q = threadIdx.x; q < n; q += blockDim.x) u[q] = make_float2(0.f, 0.f);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_tail_lds(const LevelView* __restrict__ views, int t, int last, int conj_i,
+                                                   int pre, int post, int coarse, const float2* __restrict__ f_top,
+                                                   float2* __restrict__ u_top) {
+    extern __shared__ float2 lds[];
+    const int r = blockIdx.x;
+    const bool conj = conj_i != 0;
+    int off[16];  // level l's block: u, partner, f, 6 coefficient planes (9 n float2)
+    {
+        int o = 0;
+        for (int l = t; l <= last; ++l) {
+            off[l - t] = o;
+            o += 9 * (int)views[l].n;
+        }
+    }
+    for (int l = t; l <= last; ++l) {
+        const LevelView& V = views[l];
+        const int n = (int)V.n;
+        float2* cf = lds + off[l - t] + 3 * n;
+        for (int k = threadIdx.x; k < 6 * n; k += blockDim.x) {
+            const int q = k / 6, c = k - 6 * q;
+            cf[c * n + q] = V.cf[k];
+        }
+    }
+    {
+        const int n = (int)views[t].n;
+        float2* f = lds + off[0] + 2 * n;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) f[q] = f_top[(int64_t)r * n + q];
+    }
+    __syncthreads();
+    // down: smooth, residual (into the partner buffer), restrict into the coarse level's f
+    for (int l = t; l <= last; ++l) {
+        const LevelView& V = views[l];
+        const int n = (int)V.n;
+        float2* u = lds + off[l - t];
+        float2* tt = u + n;
+        const float2* f = u + 2 * n;
+        const float2* cf = u + 3 * n;
+        if (l == last) {
+            tail_sweeps_p(V.nz, V.nx, n, cf, coarse, true, conj, f, u, tt);
+            break;
+        }
+        tail_sweeps_p(V.nz, V.nx, n, cf, pre, true, conj, f, u, tt);
+        const float inx = 1.f / (float)V.nx;
+        for (int q = threadIdx.x; q < n; q += blockDim.x)
+            tt[q] = csub(f[q], tail_mu_p(V.nz, V.nx, n, q, tail_row(q, inx), cf, u, conj));
+        __syncthreads();
+        const LevelView& C = views[l + 1];
+        const int nc = (int)C.n;
+        const float inxc = 1.f / (float)C.nx;
+        float2* fc = lds + off[l + 1 - t] + 2 * nc;
+        for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+            const int I = tail_row(q, inxc), J = q - I * C.nx;
+            float2 acc = make_float2(0.f, 0.f);
+            for (int a = -1; a <= 1; ++a) {
+                const int i = 2 * I + a;
+                if (i < 0 || i >= V.nz) continue;
+                for (int b = -1; b <= 1; ++b) {
+                    const int j = 2 * J + b;
+                    if (j < 0 || j >= V.nx) continue;
+                    const float w = (a == 0 ? 0.5f : 0.25f) * (b == 0 ? 0.5f : 0.25f);
+                    const float2 v = tt[i * V.nx + j];
+                    acc.x += w * v.x;
+                    acc.y += w * v.y;
+                }
+            }
+            fc[q] = acc;
+        }
+        __syncthreads();
+    }
+    // up: prolong-add, post-smooth
+    for (int l = last - 1; l >= t; --l) {
+        const LevelView& V = views[l];
+        const LevelView& C = views[l + 1];
+        const int n = (int)V.n;
+        float2* u = lds + off[l - t];
+        const float2* f = u + 2 * n;
+        const float2* cf = u + 3 * n;
+        const float2* uc = lds + off[l + 1 - t];
+        const float inx = 1.f / (float)V.nx;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            const int i = tail_row(q, inx), j = q - i * V.nx;
+            const int I0 = i >> 1, J0 = j >> 1;
+            float2 v = make_float2(0.f, 0.f);
+            for (int di = 0; di <= (i & 1); ++di)
+                for (int dj = 0; dj <= (j & 1); ++dj) {
+                    const int I = I0 + di, J = J0 + dj;
+                    if (I < C.nz && J < C.nx) {
+                        const float w = ((i & 1) ? 0.5f : 1.f) * ((j & 1) ? 0.5f : 1.f);
+                        const float2 cv = uc[I * C.nx + J];
+                        v.x += w * cv.x;
+                        v.y += w * cv.y;
+                    }
+                }
+            u[q] = cadd(u[q], v);
+        }
+        __syncthreads();
+        tail_sweeps_p(V.nz, V.nx, n, cf, post, false, conj, f, u, u + n);
+    }
+    const int n = (int)views[t].n;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) u_top[(int64_t)r * n + q] = lds[off[0] + q];
+}
+
 // ---- fused grid-level V-cycle kernels ----------------------------------
 // A level above the LDS tail costs four passes over its fields per V-cycle:
 //   pre     u = S(S(0))         one pass: the first sweep from zero is the
@@ -1196,6 +1353,17 @@ static size_t tail_lds_bytes(const helm_plan* p, int t) {
     for (size_t l = t; l < p->levels.size(); ++l) b += 2 * sizeof(float2) * (size_t)p->levels[l].n;
     return b;
 }
+// LDS of the all-in-LDS tail (k_tail_lds): 9 float2 per node over the tail levels
+static size_t tail_lds_bytes_full(const helm_plan* p, int t) {
+    size_t b = 0;
+    for (size_t l = t; l < p->levels.size(); ++l) b += 9 * sizeof(float2) * (size_t)p->levels[l].n;
+    return b;
+}
+// HELM_MG_TAIL_LDS=0 keeps the coefficients and right-hand sides of the tail in global memory
+static bool tail_all_lds() {
+    static const bool t = env_int("HELM_MG_TAIL_LDS", 1, 0, 1) != 0;
+    return t;
+}
 
 static int32_t tail_views(helm_plan* p) {
     const int nl = (int)p->levels.size();
@@ -1217,9 +1385,14 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     Level& L = p->levels[l];
     const MGConfig& c = p->mg;
     if (L.n <= tail_nodes()) {
-        const size_t lds = tail_lds_bytes(p, l);
-        hipLaunchKernelGGL(k_tail, dim3(nrhs), dim3(1024), lds, s, (const LevelView*)p->mg_views, l,
-                           (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u);
+        const size_t full = tail_lds_bytes_full(p, l);
+        if (tail_all_lds() && full <= 160 * 1024)
+            hipLaunchKernelGGL(k_tail_lds, dim3(nrhs), dim3(1024), full, s, (const LevelView*)p->mg_views, l,
+                               (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u);
+        else
+            hipLaunchKernelGGL(k_tail, dim3(nrhs), dim3(1024), tail_lds_bytes(p, l), s,
+                               (const LevelView*)p->mg_views, l, (int)p->levels.size() - 1, (int)adj, c.pre, c.post,
+                               c.coarse, f, u);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }
@@ -1450,6 +1623,8 @@ extern "C" int32_t helm_mg_setup(helm_plan* p, int32_t levels, double shift, int
         // the attribute is per kernel, not per plan: allow the full 160 KB so
         // plans set up on other threads never lower it under each other
         HELM_HIP(hipFuncSetAttribute((const void*)k_tail, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HELM_HIP(
+            hipFuncSetAttribute((const void*)k_tail_lds, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     }
     p->mg.requested = levels;
     p->mg.shift = shift;

@@ -170,6 +170,128 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
     }
 }
 
+// Block form of the streaming kernel: the run length TZ is a compile-time
+// constant and every value the run needs (TZ + 2 field rows of each RHS of
+// the group, eta, kappa, and the strip's west/east halo columns) is loaded
+// before the first node is formed, so a wavefront waits for one memory
+// latency per run instead of one per row.  The halo columns are packed one
+// value per lane (lane q * TZ + t holds row t of RHS q) and handed to the
+// edge lane with a readlane, so they cost two registers instead of 2 RB TZ.
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <int RB, int TZ, int MODE>
+__global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, int nrhs,
+                                                     const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                     const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                     const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                     float k0sq, float hh, float shift, int conj,
+                                                     const float2* __restrict__ u, float2* __restrict__ out,
+                                                     const float2* __restrict__ b, int xcd) {
+    static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
+    int bx, by, bz;
+    xcd_tile(bx, by, bz, xcd != 0);
+    const int lane = threadIdx.x;
+    const int x0 = bx * 64, j = x0 + lane;
+    // the run's first row, uniform over the wavefront (so the row profiles below are scalar loads)
+    const int i0 = __builtin_amdgcn_readfirstlane((by * 4 + threadIdx.y) * TZ);
+    if (i0 >= nz) return;
+    const int r0 = bz * RB;
+    const bool colok = j < nx;
+    const int jj = colok ? j : nx - 1;  // lanes past the grid mirror the last column (results dropped)
+    const bool hw = jj > 0, he = jj < nx - 1;
+    const bool wstrip = x0 > 0, estrip = x0 + 64 < nx;
+    // row profiles of the run (rows past the grid clamped; their results are dropped)
+    float2 SZ[TZ], IZ[TZ + 1];
+#pragma unroll
+    for (int t = 0; t < TZ; ++t) SZ[t] = sz[min(i0 + t, nz - 1)];
+#pragma unroll
+    for (int t = 0; t <= TZ; ++t) IZ[t] = isze[min(i0 + t, nz)];
+    float2 U[RB][TZ + 2];
+    float E[TZ + 2];
+    float2 K[TZ];
+#pragma unroll
+    for (int t = 0; t < TZ + 2; ++t) {
+        const int i = i0 - 1 + t;
+        const bool rok = i >= 0 && i < nz;
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+            U[q][t] = (rok && r0 + q < nrhs) ? u[(int64_t)(r0 + q) * n + (int64_t)i * nx + jj] : make_float2(0.f, 0.f);
+        E[t] = rok ? eta[(int64_t)i * nx + jj] : 1.f;  // the operator's ghost value outside the grid
+    }
+#pragma unroll
+    for (int t = 0; t < TZ; ++t)
+        K[t] = i0 + t < nz ? kap[(int64_t)(i0 + t) * nx + jj] : make_float2(0.f, 0.f);
+    // halo columns x0 - 1 and x0 + 64: lane q * TZ + t holds row i0 + t of RHS r0 + q
+    float2 HW = make_float2(0.f, 0.f), HE = make_float2(0.f, 0.f);
+    float EH = 1.f;  // lanes 0..TZ-1: west eta of row t; lanes 32..32+TZ-1: east eta
+    {
+        const int q = lane / TZ, t = lane % TZ, i = i0 + t;
+        if (lane < RB * TZ && i < nz && r0 + q < nrhs) {
+            const int64_t o = (int64_t)(r0 + q) * n + (int64_t)i * nx;
+            if (wstrip) HW = u[o + x0 - 1];
+            if (estrip) HE = u[o + x0 + 64];
+        }
+        if (lane < TZ && i0 + lane < nz && wstrip) EH = eta[(int64_t)(i0 + lane) * nx + x0 - 1];
+        if (lane >= 32 && lane < 32 + TZ && i0 + lane - 32 < nz && estrip)
+            EH = eta[(int64_t)(i0 + lane - 32) * nx + x0 + 64];
+    }
+    const float2 sxj = sx[jj], iw = isxe[jj], ie = isxe[jj + 1];
+    const bool cj = conj != 0;
+#pragma unroll
+    for (int t = 0; t < TZ; ++t) {
+        const int i = i0 + t;
+        const float en = E[t], ec = E[t + 1], es = E[t + 2];
+        float eww = __shfl_up(ec, 1, 64), eee = __shfl_down(ec, 1, 64);
+        const float ehw = readlane_f(EH, t), ehe = readlane_f(EH, 32 + t);
+        if (lane == 0) eww = ehw;
+        if (lane == 63) eee = ehe;
+        if (!hw) eww = 1.f;
+        if (!he) eee = 1.f;
+        const float2 szi = SZ[t], in_ = IZ[t], is_ = IZ[t + 1];
+        Coef<float2> c;
+        c.fw = cscale(cmul(szi, iw), hh * (ec + eww));
+        c.fe = cscale(cmul(szi, ie), hh * (ec + eee));
+        c.fn = cscale(cmul(sxj, in_), hh * (ec + en));
+        c.fs = cscale(cmul(sxj, is_), hh * (ec + es));
+        float2 mass = cscale(cmul(cmul(szi, sxj), K[t]), k0sq);
+        if (shift != 0.f) {
+            const float2 m2 = make_float2(mass.x - shift * mass.y, mass.y + shift * mass.x);
+            mass = m2;
+        }
+        c.dg = csub(mass, cadd(cadd(c.fw, c.fe), cadd(c.fn, c.fs)));
+        if (cj) {
+            c.dg = cconj(c.dg);
+            c.fw = cconj(c.fw);
+            c.fe = cconj(c.fe);
+            c.fn = cconj(c.fn);
+            c.fs = cconj(c.fs);
+        }
+        const bool hn = i > 0, hs = i < nz - 1;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const float2 uc = U[q][t + 1];
+            float2 w = shfl_up1(uc), e = shfl_down1(uc);
+            const float2 hwv = make_float2(readlane_f(HW.x, q * TZ + t), readlane_f(HW.y, q * TZ + t));
+            const float2 hev = make_float2(readlane_f(HE.x, q * TZ + t), readlane_f(HE.y, q * TZ + t));
+            if (lane == 0) w = hwv;
+            if (lane == 63) e = hev;
+            float2 acc = cmul(c.dg, uc);
+            if (hw) cfma(acc, c.fw, w);
+            if (he) cfma(acc, c.fe, e);
+            if (hn) cfma(acc, c.fn, U[q][t]);
+            if (hs) cfma(acc, c.fs, U[q][t + 2]);
+            const int r = r0 + q;
+            if (colok && r < nrhs && i < nz) {
+                const int64_t o = (int64_t)r * n + (int64_t)i * nx + j;
+                if (MODE == 1) acc = csub(b[o], acc);
+                out[o] = acc;
+            }
+        }
+    }
+}
+
 // streaming-kernel shape (rows per run, RHS per group): tuning knobs read
 // once from HELM_STENCIL_TZ and HELM_STENCIL_RB
 static int stream_tz() {
@@ -205,6 +327,62 @@ static void launch_stream(const Level& L, double k0sq, const float2* u, float2* 
                            (const float2*)nullptr);
 }
 
+// form before tuning: the block form (4-row runs unless HELM_STENCIL_TZ says
+// otherwise); HELM_STENCIL_BLOCK=0 selects the streaming form
+static bool stream_block() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_STENCIL_BLOCK");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
+static int block_tz() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_STENCIL_TZ");
+        const int x = v ? std::atoi(v) : 0;
+        return x >= 2 && x <= 16 ? x : 4;  // measured best or near it on 576^2 .. 2048^2 x 11
+    }();
+    return t;
+}
+// block form's workgroup order: HELM_STENCIL_XCD=1 gives each XCD a contiguous
+// run of workgroups (neighbouring runs share their halo rows in one L2)
+static int stencil_xcd() {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_STENCIL_XCD");
+        return v && std::atoi(v) == 1 ? 1 : 0;
+    }();
+    return t;
+}
+
+template <int RB, int TZ>
+static void launch_block_tz(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                            const float2* b, float shift, hipStream_t s) {
+    const int runs = (L.nz + TZ - 1) / TZ;
+    const dim3 block(64, 4), grid((L.nx + 63) / 64, (runs + 3) / 4, (nrhs + RB - 1) / RB);
+    const float hh = (float)(0.5 / (L.h_m * L.h_m));
+    const int xcd = L.st_tz ? (int)L.st_xcd : stencil_xcd();
+    if (b)
+        hipLaunchKernelGGL((k_apply_block<RB, TZ, 1>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32,
+                           L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out, b, xcd);
+    else
+        hipLaunchKernelGGL((k_apply_block<RB, TZ, 0>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32,
+                           L.isxe32, L.isze32, L.eta32, L.kap32, (float)k0sq, hh, shift, (int)adjoint, u, out,
+                           (const float2*)nullptr, xcd);
+}
+
+template <int RB>
+static void launch_block(const Level& L, int tz, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                         const float2* b, float shift, hipStream_t s) {
+    if (tz <= 2)
+        launch_block_tz<RB, 2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+    else if (tz <= 4)
+        launch_block_tz<RB, 4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+    else if (tz <= 8)
+        launch_block_tz<RB, 8>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+    else
+        launch_block_tz<RB, 16>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+}
+
 // right-hand sides per workgroup: enough groups for >= ~2048 workgroups
 static int rhs_per_group(const Level& L, int nrhs) {
     const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
@@ -235,10 +413,20 @@ int32_t launch_apply64(const Level& L, double k0sq, const double2* u, double2* o
 int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
                       const float2* b, float shift, hipStream_t s) {
     if (L.nx >= 64) {  // streaming kernel (a strip needs a full wavefront of columns to pay off)
-        switch (L.st_rb ? L.st_rb : stream_rb()) {
-            case 1: launch_stream<1>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
-            case 2: launch_stream<2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
-            default: launch_stream<4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+        const int rb = L.st_rb ? L.st_rb : stream_rb();
+        if (L.st_tz ? L.st_block : stream_block()) {
+            const int tz = L.st_tz ? L.st_tz : block_tz();
+            switch (rb) {
+                case 1: launch_block<1>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+                case 2: launch_block<2>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+                default: launch_block<4>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+            }
+        } else {
+            switch (rb) {
+                case 1: launch_stream<1>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+                case 2: launch_stream<2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+                default: launch_stream<4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+            }
         }
         HELM_HIP(hipGetLastError());
         return HELM_OK;
@@ -342,24 +530,36 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
         return cond ? launch_cond8_32(L, u, out, nrhs, false, nullptr, s)
                     : launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
     };
-    for (int a = 0; a < ntz && st == HELM_OK; ++a)
+    // the flux stencil also tries its block form (runs of 2, 4, 8 or 16 rows loaded up front), in the
+    // hardware's workgroup order and in the XCD-contiguous one
+    const int nblk = cond ? 0 : 8, tzs_blk[4] = {2, 4, 8, 16};
+    bool bblk = false, bxcd = false;
+    const int reps = 20;  // launches timed per candidate (shapes differ by a few percent)
+    for (int a = 0; a < ntz + nblk && st == HELM_OK; ++a)
         for (int b = 0; b < 3 && st == HELM_OK; ++b) {
-            L.st_tz = tzs[a];
+            const bool blk = a >= ntz;
+            L.st_tz = blk ? tzs_blk[(a - ntz) % 4] : tzs[a];
             L.st_rb = rbs[b];
-            for (int w = 0; w < 2 && st == HELM_OK; ++w) st = run();
+            L.st_block = blk;
+            L.st_xcd = blk && a - ntz >= 4;
+            for (int w = 0; w < 3 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e0, s);
-            for (int w = 0; w < 5 && st == HELM_OK; ++w) st = run();
+            for (int w = 0; w < reps && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e1, s);
             float ms = 0.f;
             if (st == HELM_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
-                ms / 5.0 < best) {
-                best = ms / 5.0;
-                btz = tzs[a];
+                ms / reps < best) {
+                best = ms / reps;
+                btz = L.st_tz;
                 brb = rbs[b];
+                bblk = blk;
+                bxcd = L.st_xcd;
             }
         }
     L.st_tz = btz;
     L.st_rb = brb;
+    L.st_block = bblk;
+    L.st_xcd = bxcd;
     *best_us = best * 1e3;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -513,9 +713,30 @@ extern "C" int32_t helm_autotune(helm_plan* p, int32_t nrhs, helm_tuning* result
         const Level& L = p->levels[0];
         result->stencil_rows = L.st_tz;
         result->stencil_rhs = L.st_rb;
+        result->stencil_block = L.st_block ? (L.st_xcd ? 2 : 1) : 0;
         result->mg_workgroups = p->mg_wg_target;
         result->stencil_us = st_us;
         result->vcycle_us = mg_us;
     }
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_set_stencil_shape(helm_plan* p, int32_t rows, int32_t rhs, int32_t form) {
+    HELM_CHECK(p, HELM_ERR_ARG, "helm_set_stencil_shape: null plan");
+    Level& L = p->levels[0];
+    if (rows == 0) {  // back to the built-in defaults
+        L.st_tz = L.st_rb = 0;
+        L.st_block = L.st_xcd = false;
+    } else {
+        HELM_CHECK(rhs == 1 || rhs == 2 || rhs == 4, HELM_ERR_ARG, "helm_set_stencil_shape: rhs must be 1, 2 or 4");
+        HELM_CHECK(form >= 0 && form <= 2, HELM_ERR_ARG, "helm_set_stencil_shape: form must be 0, 1 or 2");
+        HELM_CHECK(form == 0 ? rows >= 2 && rows <= 1024 : rows == 2 || rows == 4 || rows == 8 || rows == 16,
+                   HELM_ERR_ARG, "helm_set_stencil_shape: rows out of range for the form");
+        L.st_tz = rows;
+        L.st_rb = rhs;
+        L.st_block = form != 0;
+        L.st_xcd = form == 2;
+    }
+    p->generation++;  // captured cycles hold the old launch shapes
     return HELM_OK;
 }

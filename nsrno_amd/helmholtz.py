@@ -188,7 +188,13 @@ class HelmholtzPlan:
         n = self.nrhs_capacity if nrhs is None else int(nrhs)
         check(lib().helm_autotune(self._h, n, ctypes.byref(t), _stream()), "helm_autotune")
         return dict(stencil_rows=t.stencil_rows, stencil_rhs=t.stencil_rhs, mg_workgroups=t.mg_workgroups,
-                    stencil_us=t.stencil_us, vcycle_us=t.vcycle_us)
+                    stencil_us=t.stencil_us, vcycle_us=t.vcycle_us, stencil_block=t.stencil_block)
+
+    def set_stencil_shape(self, rows=0, rhs=4, form=1):
+        """Fix the FP32 stencil's launch shape (helm_set_stencil_shape): form 0
+        streaming, 1 block, 2 block in XCD-contiguous order; rows=0 restores
+        the default."""
+        check(lib().helm_set_stencil_shape(self._h, int(rows), int(rhs), int(form)), "helm_set_stencil_shape")
 
     PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual",
                        "mg_pre2_restrict", "mg_prolong_post2", "mg_pre2_restrict_finest", "mg_prolong_post2_finest")

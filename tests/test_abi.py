@@ -42,6 +42,8 @@ def test_version_and_errors_without_gpu():
     bad = _lib.HelmDesc(2, 2, 1, 0, 0.0, 0.0, 0.1, 0.0, 1.0, 1500.0, 12.0)
     assert lib.helm_plan_create(ctypes.byref(bad), ctypes.byref(h)) == 1
     assert "3 nodes" in lib.helm_last_error().decode()
+    assert lib.helm_set_stencil_shape(None, 4, 4, 1) == 1
+    assert "null plan" in lib.helm_last_error().decode()
 
 
 def test_struct_layout_matches_c_compiler(tmp_path):
@@ -49,6 +51,7 @@ def test_struct_layout_matches_c_compiler(tmp_path):
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "helm.h"\n'
                    'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(helm_desc), sizeof(helm_solve_opts),'
                    ' sizeof(helm_solve_info), offsetof(helm_solve_opts, callback), offsetof(helm_desc, x0_mm));'
+                   'printf("%zu %zu\\n", sizeof(helm_tuning), offsetof(helm_tuning, stencil_block));'
                    'return 0;}\n')
     exe = tmp_path / "sizes"
     try:
@@ -58,5 +61,6 @@ def test_struct_layout_matches_c_compiler(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     got = [int(v) for v in out]
     want = [ctypes.sizeof(_lib.HelmDesc), ctypes.sizeof(_lib.HelmSolveOpts), ctypes.sizeof(_lib.HelmSolveInfo),
-            _lib.HelmSolveOpts.callback.offset, _lib.HelmDesc.x0_mm.offset]
+            _lib.HelmSolveOpts.callback.offset, _lib.HelmDesc.x0_mm.offset, ctypes.sizeof(_lib.HelmTuning),
+            _lib.HelmTuning.stencil_block.offset]
     assert got == want

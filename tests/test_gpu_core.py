@@ -181,10 +181,13 @@ def test_born_data_and_jacobian_match_oracle():
     assert rel(bd2, bd) < 1e-8
 
 
+@pytest.mark.parametrize("shape", [None, (8, 4, 0), (3, 1, 0), (2, 2, 1), (4, 4, 1), (8, 1, 1), (16, 4, 1),
+                                   (4, 2, 2), (16, 1, 2)])
 @pytest.mark.parametrize("nrhs", [1, 5])
-def test_apply_fp32_streaming_shapes(nrhs):
-    """The streaming FP32 stencil on a grid that is no multiple of its strip
-    (64 columns) or its row runs, with a RHS count that leaves a partial group."""
+def test_apply_fp32_streaming_shapes(nrhs, shape):
+    """The FP32 stencil's streaming and block forms (rows per run, RHS per
+    group, workgroup order) on a grid that is no multiple of the strip (64
+    columns) or of the row runs, with a RHS count that leaves a partial group."""
     from nsrno_amd.helmholtz import HelmholtzPlan
     grid = O.grid_with_pml(170, 131, 12, 0.05)
     nz, nx = grid.shape
@@ -194,6 +197,8 @@ def test_apply_fp32_streaming_shapes(nrhs):
     plan = HelmholtzPlan(nz, nx, grid.h_mm, 2.0, grid.pml_mm, x0_mm=grid.x_mm[0], z0_mm=grid.z_mm[0], nrhs=nrhs,
                          restart=4)
     plan.set_medium(eta, kap)
+    if shape is not None:
+        plan.set_stencil_shape(*shape)
     u = random_fields(grid.shape, nrhs, 7)
     ref = O.apply(grid, 2.0, u, eta, kap)
     out = plan.apply(torch.from_numpy(u.astype(np.complex64)).cuda()).cpu().numpy()

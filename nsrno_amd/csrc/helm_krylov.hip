@@ -105,6 +105,14 @@ static int dot_npt() {
     }();
     return t;
 }
+// fused reduce+update and norm+scale launches (HELM_DCGS_FUSED=0: the separate kernels)
+static bool dcgs_fused() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_DCGS_FUSED");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
 static inline int ndotchunks(int64_t n) { return (int)((n + 256 * dot_npt() - 1) / (256 * dot_npt())); }
 // partial-sum capacity: enough chunks for the smaller chunk size
 static inline int ndotchunks_max(int64_t n) { return (int)((n + 1023) / 1024); }
@@ -367,6 +375,162 @@ __global__ void k_dcgs_finish(int nrhs, int m, int j, double2* __restrict__ H, d
     est[r] = e / bnorm[r];
     used[r] = j + 1;
     cyc[r] = 0;
+}
+
+// ---- fused forms: two launches fewer per step ------------------------------
+// k_dcgs_reduce_update = k_dcgs_reduce + k_dcgs_update: every workgroup sums
+// the dots partials of its right-hand side itself, in k_dcgs_reduce's fixed
+// order (so every workgroup derives bit-identical coefficients), and the
+// chunk-0 workgroup alone does the Hessenberg bookkeeping (correct and rotate
+// column j-1, residual estimate, early stop, provisional column j).  A
+// right-hand side that stops here is still updated by workgroups that read
+// cyc before the stop: harmless, as its V_j and w are never read again (the
+// x update uses Z_0..Z_{j-1}).  Extra reads per workgroup: 2 (j + 1) nchd
+// partials (31 KB at j = 5, from L2).
+__global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs, int nch, int nchd, int m, int kmax,
+                                                            int j, double tol, const double2* __restrict__ pc,
+                                                            const double2* __restrict__ pd,
+                                                            const double* __restrict__ bnorm, double2* __restrict__ H,
+                                                            double* __restrict__ cs, double2* __restrict__ sn,
+                                                            double2* __restrict__ g, double* __restrict__ est,
+                                                            int* __restrict__ cyc, int* __restrict__ used,
+                                                            float2* __restrict__ V, float2* __restrict__ w,
+                                                            double2* __restrict__ partial) {
+    __shared__ double lds[8];
+    __shared__ double2 sc[64], sd[64];
+    __shared__ float2 c_s[64], d_s[64];
+    __shared__ float2 hjj_s;
+    __shared__ float ialpha_s;
+    const int chunk = blockIdx.x, r = blockIdx.y;
+    if (!cyc[r]) return;
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int k = threadIdx.x >> 6; k <= j; k += nw) {
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        const double2* srcc = pc + ((int64_t)r * kmax + k) * nchd;
+        const double2* srcd = pd + ((int64_t)r * kmax + k) * nchd;
+        for (int c = lane; c < nchd; c += 64) {
+            a = cadd(a, srcc[c]);
+            b = cadd(b, srcd[c]);
+        }
+        a.x = wave_sum(a.x);
+        a.y = wave_sum(a.y);
+        b.x = wave_sum(b.x);
+        b.y = wave_sum(b.y);
+        if (lane == 0) {
+            sc[k] = a;
+            sd[k] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double alpha = 1.0;
+        bool stop = false;
+        double2* Hr = H + (int64_t)r * m * (m + 1);
+        if (j > 0) {
+            double c2 = 0.0;
+            for (int k = 0; k < j; ++k) c2 += sc[k].x * sc[k].x + sc[k].y * sc[k].y;
+            alpha = sqrt(fmax(sc[j].x - c2, 1e-300));
+            if (chunk == 0) {
+                double2* hp = Hr + (int64_t)(j - 1) * (m + 1);
+                const double hn = hp[j].x;
+                for (int k = 0; k < j; ++k) hp[k] = cadd(hp[k], cscale(sc[k], hn));
+                hp[j] = make_double2(hn * alpha, 0.0);
+                const double e = givens_column(hp, cs + (int64_t)r * m, sn + (int64_t)r * m, g + (int64_t)r * (m + 1),
+                                               j - 1);
+                est[r] = e / bnorm[r];
+                used[r] = j;
+                if (e <= 0.5 * tol * bnorm[r] || !isfinite(e)) {
+                    cyc[r] = 0;
+                    stop = true;
+                }
+            }
+        }
+        double2 hjj = sd[j];
+        for (int k = 0; k < j; ++k) hjj = csub(hjj, cmul(cconj(sc[k]), sd[k]));
+        hjj = cscale(hjj, 1.0 / alpha);
+        if (chunk == 0 && !stop) {
+            double2* hc = Hr + (int64_t)j * (m + 1);
+            for (int k = 0; k < j; ++k) hc[k] = sd[k];
+            hc[j] = hjj;
+        }
+        for (int k = 0; k < j; ++k) {
+            c_s[k] = make_float2((float)sc[k].x, (float)sc[k].y);
+            d_s[k] = make_float2((float)sd[k].x, (float)sd[k].y);
+        }
+        hjj_s = make_float2((float)hjj.x, (float)hjj.y);
+        ialpha_s = (float)(1.0 / alpha);
+    }
+    __syncthreads();
+    const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
+    const int64_t vstride = (int64_t)nrhs * n;
+    float2* wr = w + (int64_t)r * n;
+    float2* vj = V + (int64_t)j * vstride + (int64_t)r * n;
+    const float2* vr = V + (int64_t)r * n;
+    const float2 hjj = hjj_s;
+    const float ia = ialpha_s;
+    double nrm = 0.0;
+    for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
+        float2 sc2 = make_float2(0.f, 0.f), sd2 = make_float2(0.f, 0.f);
+        for (int k = 0; k < j; ++k) {
+            const float2 v = vr[(int64_t)k * vstride + p];
+            cfma(sc2, c_s[k], v);
+            cfma(sd2, d_s[k], v);
+        }
+        float2 jv = csub(vj[p], sc2);
+        jv = make_float2(jv.x * ia, jv.y * ia);
+        vj[p] = jv;
+        float2 wv = csub(wr[p], sd2);
+        wv.x -= hjj.x * jv.x - hjj.y * jv.y;
+        wv.y -= hjj.x * jv.y + hjj.y * jv.x;
+        wr[p] = wv;
+        nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+    }
+    const double2 tot = block_sum(make_double2(nrm, 0.0), lds);
+    if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = tot;
+}
+
+// k_dcgs_norm + k_scale32 (last = 0) or k_dcgs_norm + k_dcgs_finish (last = 1,
+// one workgroup per RHS): every workgroup sums the |w|^2 partials of its RHS
+// in k_dcgs_norm's order; the chunk-0 workgroup writes H[j+1][j] and 1/||w||
+// and finalises the column on a happy breakdown or at the end of the cycle;
+// otherwise each workgroup writes its chunk of V_{j+1} = w / ||w||.
+__global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int m, int j, int last,
+                                                         const double2* __restrict__ partial,
+                                                         const double* __restrict__ bnorm, double2* __restrict__ H,
+                                                         double* __restrict__ cs, double2* __restrict__ sn,
+                                                         double2* __restrict__ g, double* __restrict__ est,
+                                                         double* __restrict__ inv, int* __restrict__ cyc,
+                                                         int* __restrict__ used, const float2* __restrict__ w,
+                                                         float2* __restrict__ vnext) {
+    __shared__ double lds[8];
+    __shared__ double hn_s;
+    const int chunk = blockIdx.x, r = blockIdx.y;
+    if (!cyc[r]) return;
+    const double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
+    if (threadIdx.x == 0) hn_s = sqrt(s);
+    __syncthreads();
+    const double hn = hn_s;
+    const bool fin = last || hn < 1e-12 * bnorm[r];
+    if (chunk == 0 && threadIdx.x == 0) {
+        double2* hc = H + ((int64_t)r * m + j) * (m + 1);
+        hc[j + 1] = make_double2(hn, 0.0);
+        inv[r] = hn > 0 ? 1.0 / hn : 0.0;
+        if (fin) {
+            const double e = givens_column(hc, cs + (int64_t)r * m, sn + (int64_t)r * m, g + (int64_t)r * (m + 1), j);
+            est[r] = e / bnorm[r];
+            used[r] = j + 1;
+            cyc[r] = 0;
+        }
+    }
+    if (fin) return;
+    const float sc = (float)(hn > 0 ? 1.0 / hn : 0.0);
+    const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
+    const float2* wr = w + (int64_t)r * n;
+    float2* vr = vnext + (int64_t)r * n;
+    for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
+        const float2 a = wr[p];
+        vr[p] = make_float2(a.x * sc, a.y * sc);
+    }
 }
 
 // partial[r*nch+chunk] = |v[r]|^2 over the chunk (c128 or c64 vector)
@@ -770,6 +934,25 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                     hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs, n,
                                        nrhs, nchd, kmax, j, W.V, W.w,
                                        W.cyc, W.partial, pd_part);
+                }
+                if (dcgs_fused()) {
+                    if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
+                        hipExtLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs,
+                                              nch, nchd, mm, kmax, j, o->tolerance, (const double2*)W.partial,
+                                              (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g,
+                                              W.est, W.cyc, W.used, W.V, W.w, pw2);
+                        prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
+                    } else {
+                        hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, nchd, mm,
+                                           kmax, j, o->tolerance, W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g,
+                                           W.est, W.cyc, W.used, W.V, W.w, pw2);
+                    }
+                    const bool last = j + 1 >= nsteps;
+                    hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm, j,
+                                       (int)last, pw2, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used,
+                                       W.w, W.V + (int64_t)(last ? j : j + 1) * vstride);
+                    HELM_HIP(hipGetLastError());
+                    continue;
                 }
                 hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, cs, nchd, mm, kmax, j, o->tolerance,
                                    W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc, W.used, W.coef);

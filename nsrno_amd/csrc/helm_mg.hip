@@ -174,14 +174,17 @@ static bool fused_levels() {
     static const bool f = env_int("HELM_MG_FUSED", 1, 0, 1) != 0;
     return f;
 }
-// XCD-aware tile order in the fused kernels (HELM_XCD_ORDER=1).  Off by
-// default: at 576^2 x 11 it cuts the finest level's post-smoothing traffic
-// from 208 to 136 MB per launch but the launch takes 3% longer (these
-// launches are bound by latency and occupancy, not by HBM bytes).
+// XCD-aware tile order in the fused kernels, per kernel and level
+// (HELM_XCD_ORDER bits: 1 pre-smoothing on the finest level, 2 pre-smoothing
+// on coarser levels, 4 post-smoothing on the finest level, 8 post-smoothing
+// on coarser levels; 15: all).  At 576^2 x 11 it cuts the finest level's
+// post-smoothing traffic from 208 to 136 MB per launch.
 static int xcd_order() {
-    static const int t = (int)env_int("HELM_XCD_ORDER", 0, 0, 1);
+    static const int t = (int)env_int("HELM_XCD_ORDER", 0, 0, 15);
     return t;
 }
+static int xcd_pre(int l) { return (xcd_order() >> (l == 0 ? 0 : 1)) & 1; }
+static int xcd_post(int l) { return (xcd_order() >> (l == 0 ? 2 : 3)) & 1; }
 // post-smoothing kernel variant: without prefetching (default) it fits three
 // waves per SIMD, which measured 25% faster at 576^2 x 11 than the prefetching
 // variant's two (HELM_MG_POST_PREFETCH=1)
@@ -1423,13 +1426,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
                                   rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
-                                  sh, (int)adj, (const float2*)L.wid, tab, f, L.mg_t, C.mg_f, xcd_order());
+                                  sh, (int)adj, (const float2*)L.wid, tab, f, L.mg_t, C.mg_f, xcd_pre(l));
             prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ), 0,
                                s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
-                               L.mg_t, C.mg_f, xcd_order());
+                               L.mg_t, C.mg_f, xcd_pre(l));
         }
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
@@ -1439,13 +1442,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, (const float2*)C.mg_u,
-                                  (const float2*)L.mg_t, f, u, xcd_order());
+                                  (const float2*)L.mg_t, f, u, xcd_post(l));
             prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, L.nz, L.nx,
                                L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                                L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
-                               L.mg_t, f, u, xcd_order());
+                               L.mg_t, f, u, xcd_post(l));
         }
         HELM_HIP(hipGetLastError());
         return HELM_OK;

@@ -519,10 +519,12 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     (void)hipEventCreate(&e1);
     int32_t st = HELM_OK;
     double best = 1e30;
-    int btz = cond ? 64 : 8, brb = cond ? 2 : 4;
+    int btz = cond ? 64 : 4, brb = cond ? 2 : 4;
     if (hipMemsetAsync(u, 0, bytes, s) != hipSuccess) st = HELM_ERR_HIP;
-    // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up)
-    const int ntz = cond ? 4 : 3;
+    // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up);
+    // the flux stencil's streaming form is no longer a candidate (its block form measured faster on
+    // every grid from 576^2 to 2048^2) but stays selectable with helm_set_stencil_shape
+    const int ntz = cond ? 4 : 0;
     const int tzs_flux[4] = {4, 8, 16, 16}, tzs_cond[4] = {32, 64, 128, 256}, rbs[3] = {1, 2, 4};
     const int* tzs = cond ? tzs_cond : tzs_flux;
     const double k0sq = p->k0 * p->k0;
@@ -530,18 +532,18 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
         return cond ? launch_cond8_32(L, u, out, nrhs, false, nullptr, s)
                     : launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
     };
-    // the flux stencil also tries its block form (runs of 2, 4, 8 or 16 rows loaded up front), in the
-    // hardware's workgroup order and in the XCD-contiguous one
-    const int nblk = cond ? 0 : 8, tzs_blk[4] = {2, 4, 8, 16};
-    bool bblk = false, bxcd = false;
-    const int reps = 20;  // launches timed per candidate (shapes differ by a few percent)
+    // the flux stencil's block form: runs of 2, 4 or 8 rows loaded up front (16 was never the fastest),
+    // in the hardware's workgroup order and in the XCD-contiguous one
+    const int nblk = cond ? 0 : 6, tzs_blk[3] = {2, 4, 8};
+    bool bblk = !cond, bxcd = false;
+    const int reps = 10;  // launches timed per candidate
     for (int a = 0; a < ntz + nblk && st == HELM_OK; ++a)
         for (int b = 0; b < 3 && st == HELM_OK; ++b) {
             const bool blk = a >= ntz;
-            L.st_tz = blk ? tzs_blk[(a - ntz) % 4] : tzs[a];
+            L.st_tz = blk ? tzs_blk[(a - ntz) % 3] : tzs[a];
             L.st_rb = rbs[b];
             L.st_block = blk;
-            L.st_xcd = blk && a - ntz >= 4;
+            L.st_xcd = blk && a - ntz >= 3;
             for (int w = 0; w < 3 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e0, s);
             for (int w = 0; w < reps && st == HELM_OK; ++w) st = run();

@@ -59,23 +59,29 @@ __global__ __launch_bounds__(256) void k_node_weights(int nz, int nx, const floa
     // corners: 4 of the full square, then the oscillatory L-shaped region
     const float t1s[11] = {-1, -1, 1, 1, -1, -1, 0, 0, -1, 1, 1};
     const float t2s[11] = {-1, 1, -1, 1, -1, 1, -1, 1, 0, -1, 0};
+    // |1 - w z|^2 = 1 + r (r |z|^2 - 2 Re(e^{i phi} z)) for w = r e^{i phi}: per angle the 11 projections
+    // are formed once, and each radius costs two FMAs per mode
     float2 z[11];
+    float m2[11];
 #pragma unroll
-    for (int k = 0; k < 11; ++k) z[k] = make_float2(1.f + t1s[k] * a.x + t2s[k] * b.x, t1s[k] * a.y + t2s[k] * b.y);
+    for (int k = 0; k < 11; ++k) {
+        z[k] = make_float2(1.f + t1s[k] * a.x + t2s[k] * b.x, t1s[k] * a.y + t2s[k] * b.y);
+        m2[k] = z[k].x * z[k].x + z[k].y * z[k].y;
+    }
     float best = 1e30f;
     float2 bw = make_float2(0.5f, 0.f);
     for (int ip = 0; ip <= 32; ++ip) {
         float sphi, cphi;
         sincosf(-1.5707963f + 3.1415927f * ip / 32.f, &sphi, &cphi);
+        float q[11];  // -2 Re(e^{i phi} z_k)
+#pragma unroll
+        for (int k = 0; k < 11; ++k) q[k] = -2.f * (cphi * z[k].x - sphi * z[k].y);
         for (int ir = 1; ir <= 32; ++ir) {
             const float r = 2.f * ir / 32.f;
-            const float wr = r * cphi, wi = r * sphi;
             float full = 0.f, hi = 0.f;
 #pragma unroll
             for (int k = 0; k < 11; ++k) {
-                const float gr = 1.f - (wr * z[k].x - wi * z[k].y);
-                const float gi = -(wr * z[k].y + wi * z[k].x);
-                const float g2 = gr * gr + gi * gi;
+                const float g2 = fmaf(r, fmaf(r, m2[k], q[k]), 1.f);
                 if (k < 4) full = fmaxf(full, g2);
                 else hi = fmaxf(hi, g2);
             }
@@ -83,7 +89,7 @@ __global__ __launch_bounds__(256) void k_node_weights(int nz, int nx, const floa
             const float score = coarsest ? full : (full <= 1.21f ? hi : 1e3f + full);
             if (score < best) {
                 best = score;
-                bw = make_float2(wr, wi);
+                bw = make_float2(r * cphi, r * sphi);
             }
         }
     }

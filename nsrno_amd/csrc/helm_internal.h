@@ -147,6 +147,7 @@ struct Workspace {
     double* bnorm = nullptr;  // nrhs
     double* rel = nullptr;    // nrhs
     double* inv = nullptr;    // nrhs (1/h or 1/beta)
+    double* pscale = nullptr; // nrhs: scale the preconditioner applies to an unnormalised basis vector
     double* est = nullptr;    // nrhs
     int* active = nullptr;    // nrhs: not yet converged
     int* cyc = nullptr;       // nrhs: active in this cycle
@@ -265,7 +266,11 @@ int32_t upload_cond_profiles(helm_plan* p, int level);
 // the multigrid preconditioner of a conditioned plan (node scaling + V-cycle)
 int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
 // multigrid (helm_mg.hip)
-int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
+// fscale (nullable, per RHS): the V-cycle solves for fscale[r] * v[r] (applied as the
+// finest level's kernels load v); only where mg_scaled_input_ok holds
+int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                  const double* fscale = nullptr);
+bool mg_scaled_input_ok(const helm_plan* p);
 int32_t mg_build(helm_plan* p, hipStream_t s);
 // krylov (helm_krylov.hip)
 int32_t ensure_workspace(helm_plan* p, int nrhs, int restart);

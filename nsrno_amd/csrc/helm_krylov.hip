@@ -105,6 +105,15 @@ static int dot_npt() {
     }();
     return t;
 }
+// V_{j+1} left unnormalised and scaled by the preconditioner as it loads it
+// (multigrid on the fused path only; HELM_LAZY_BASIS=0: a scaling pass per step)
+static bool lazy_basis() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_LAZY_BASIS");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
 // fused reduce+update and norm+scale launches (HELM_DCGS_FUSED=0: the separate kernels)
 static bool dcgs_fused() {
     static const bool t = [] {
@@ -494,6 +503,13 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
 // in k_dcgs_norm's order; the chunk-0 workgroup writes H[j+1][j] and 1/||w||
 // and finalises the column on a happy breakdown or at the end of the cycle;
 // otherwise each workgroup writes its chunk of V_{j+1} = w / ||w||.
+// Lazily scaled basis (pscale != nullptr, one workgroup per RHS): w already
+// sits in V_{j+1}'s slot and stays unnormalised.  The preconditioner applies
+// pscale = 1/||w|| as it loads V_{j+1}, so Z_{j+1} and the next w come out at
+// the normalised scale; the next step's reorthogonalisation divides V_{j+1}
+// by alpha' = ||w|| alpha and its coefficients c_k carry the factor ||w||, so
+// H[j+1][j] = 1 here reproduces H[j+1][j] = ||w|| alpha and
+// H[k][j] += ||w|| c_k exactly as with a normalised V_{j+1}.
 __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int m, int j, int last,
                                                          const double2* __restrict__ partial,
                                                          const double* __restrict__ bnorm, double2* __restrict__ H,
@@ -501,7 +517,7 @@ __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int
                                                          double2* __restrict__ g, double* __restrict__ est,
                                                          double* __restrict__ inv, int* __restrict__ cyc,
                                                          int* __restrict__ used, const float2* __restrict__ w,
-                                                         float2* __restrict__ vnext) {
+                                                         float2* __restrict__ vnext, double* __restrict__ pscale) {
     __shared__ double lds[8];
     __shared__ double hn_s;
     const int chunk = blockIdx.x, r = blockIdx.y;
@@ -513,8 +529,9 @@ __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int
     const bool fin = last || hn < 1e-12 * bnorm[r];
     if (chunk == 0 && threadIdx.x == 0) {
         double2* hc = H + ((int64_t)r * m + j) * (m + 1);
-        hc[j + 1] = make_double2(hn, 0.0);
+        hc[j + 1] = make_double2(pscale && !fin ? 1.0 : hn, 0.0);
         inv[r] = hn > 0 ? 1.0 / hn : 0.0;
+        if (pscale) pscale[r] = inv[r];
         if (fin) {
             const double e = givens_column(hc, cs + (int64_t)r * m, sn + (int64_t)r * m, g + (int64_t)r * (m + 1), j);
             est[r] = e / bnorm[r];
@@ -522,7 +539,7 @@ __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int
             cyc[r] = 0;
         }
     }
-    if (fin) return;
+    if (fin || pscale) return;
     const float sc = (float)(hn > 0 ? 1.0 / hn : 0.0);
     const int64_t beg = (int64_t)chunk * CHUNK, end = min(beg + CHUNK, n);
     const float2* wr = w + (int64_t)r * n;
@@ -568,7 +585,8 @@ __global__ void k_bnorm(int nch, const double2* __restrict__ partial, double* __
 __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restrict__ partial,
                              const double* __restrict__ bnorm, double* __restrict__ beta, double* __restrict__ rel,
                              double* __restrict__ inv, int* __restrict__ active, int* __restrict__ cyc,
-                             int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g) {
+                             int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g,
+                             double* __restrict__ pscale) {
     __shared__ double lds[8];
     const int r = blockIdx.x;
     double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
@@ -583,6 +601,7 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
         if (!isfinite(rr)) active[r] = 0;
         cyc[r] = active[r];
         inv[r] = b > 0 ? 1.0 / b : 0.0;
+        if (pscale) pscale[r] = 1.0;  // V_0 is stored normalised
         g[(int64_t)r * (m + 1)] = make_double2(b, 0.0);
         for (int k = 1; k <= m; ++k) g[(int64_t)r * (m + 1) + k] = make_double2(0.0, 0.0);
     }
@@ -702,6 +721,7 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.bnorm, nrhs));
     HELM_TRY(alloc(&w.rel, nrhs));
     HELM_TRY(alloc(&w.inv, nrhs));
+    HELM_TRY(alloc(&w.pscale, nrhs));
     HELM_TRY(alloc(&w.est, nrhs));
     HELM_TRY(alloc(&w.active, nrhs));
     HELM_TRY(alloc(&w.cyc, nrhs));
@@ -887,7 +907,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         applications++;
         hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
         hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, nch, mm, o->tolerance, W.partial, W.bnorm,
-                           W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g);
+                           W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale);
         HELM_HIP(hipGetLastError());
         HELM_HIP(hipMemcpyAsync(W.h_rel.data(), W.rel, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipMemcpyAsync(W.h_active.data(), W.active, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -905,6 +925,8 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         }
         cycles++;
         const int msteps = std::min(m, o->max_iterations - steps);
+        // lazily scaled basis: the multigrid's finest-level kernels normalise V_j as they load it
+        const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG && mg_scaled_input_ok(p);
         // one restart cycle: V_0 = r/beta, msteps flexible DCGS2 steps, x += Z y
         auto enqueue_cycle = [&](int nsteps, hipStream_t cs) -> int32_t {
             hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
@@ -912,27 +934,32 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                 float2* vj = W.V + (int64_t)j * vstride;
                 // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
                 float2* zj = W.Z + (int64_t)j * vstride;
+                // w = A z_j: in V_{j+1}'s slot when the basis is scaled lazily
+                float2* wj = lazy ? W.V + (int64_t)(j + 1) * vstride : W.w;
                 hipEvent_t ev;
                 if (pc != HELM_PC_NONE) {
                     ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
-                    HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));
+                    if (lazy)
+                        HELM_TRY(mg_vcycle(p, vj, zj, nrhs, adj, cs, W.pscale));
+                    else
+                        HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));
                     if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, cs);
                 } else {
                     HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, cs));
                 }
                 ev = pf ? prof_begin(pf, PK_APPLY, cs) : nullptr;
-                HELM_TRY(launch_apply32(L, k0sq, zj, W.w, nrhs, adj, nullptr, 0.0f, cs));
+                HELM_TRY(launch_apply32(L, k0sq, zj, wj, nrhs, adj, nullptr, 0.0f, cs));
                 if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, cs);
                 hipEvent_t ea, eb;
                 if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
                     hipExtLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs,
                                           ea, eb, 0, n, nrhs, nchd, kmax, j,
-                                          (const float2*)W.V, (const float2*)W.w, (const int*)W.cyc, W.partial,
+                                          (const float2*)W.V, (const float2*)wj, (const int*)W.cyc, W.partial,
                                           pd_part);
                     prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
                 } else {
                     hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs, n,
-                                       nrhs, nchd, kmax, j, W.V, W.w,
+                                       nrhs, nchd, kmax, j, W.V, wj,
                                        W.cyc, W.partial, pd_part);
                 }
                 if (dcgs_fused()) {
@@ -940,17 +967,18 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                         hipExtLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs,
                                               nch, nchd, mm, kmax, j, o->tolerance, (const double2*)W.partial,
                                               (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g,
-                                              W.est, W.cyc, W.used, W.V, W.w, pw2);
+                                              W.est, W.cyc, W.used, W.V, wj, pw2);
                         prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
                     } else {
                         hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, nchd, mm,
                                            kmax, j, o->tolerance, W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g,
-                                           W.est, W.cyc, W.used, W.V, W.w, pw2);
+                                           W.est, W.cyc, W.used, W.V, wj, pw2);
                     }
                     const bool last = j + 1 >= nsteps;
-                    hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm, j,
-                                       (int)last, pw2, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used,
-                                       W.w, W.V + (int64_t)(last ? j : j + 1) * vstride);
+                    hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last || lazy ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm,
+                                       j, (int)last, pw2, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used,
+                                       wj, lazy ? wj : W.V + (int64_t)(last ? j : j + 1) * vstride,
+                                       lazy ? W.pscale : nullptr);
                     HELM_HIP(hipGetLastError());
                     continue;
                 }

@@ -935,7 +935,8 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
                                                      const float2* __restrict__ kap, float k0sq, float hh,
                                                      float shift, int conj, const float2* __restrict__ wid,
                                                      const float2* __restrict__ tcf, const float2* __restrict__ f,
-                                                     float2* __restrict__ u, float2* __restrict__ fcoarse, int xcd) {
+                                                     const double* __restrict__ fscale, float2* __restrict__ u,
+                                                     float2* __restrict__ fcoarse, int xcd) {
     __shared__ float2 sf[PR_1N];
     __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
     __shared__ float2 s2[PR_2N];
@@ -984,6 +985,11 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
     auto fetch = [&](int r) {
 #pragma unroll
         for (int q = 0; q < PR_1PT; ++q) fq[q] = i1[q] >= 0 ? f[(int64_t)r * nf + i1[q]] : make_float2(0.f, 0.f);
+        if (fscale) {
+            const float fs = (float)fscale[r];
+#pragma unroll
+            for (int q = 0; q < PR_1PT; ++q) fq[q] = make_float2(fq[q].x * fs, fq[q].y * fs);
+        }
     };
     if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
@@ -1079,7 +1085,8 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
                                                    float k0sq, float hh, float shift, int conj,
                                                    const float2* __restrict__ wid, const float2* __restrict__ tcf,
                                                    const float2* __restrict__ uc, const float2* __restrict__ u,
-                                                   const float2* __restrict__ f, float2* __restrict__ out, int xcd) {
+                                                   const float2* __restrict__ f, const double* __restrict__ fscale,
+                                                   float2* __restrict__ out, int xcd) {
     __shared__ float2 sv2[P2_N];
     __shared__ float2 sv1[MT_NS];
     __shared__ float2 scv[MP_NC];
@@ -1134,6 +1141,11 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
         for (int q = 0; q < MP_CPT; ++q) cs[q] = cix[q] >= 0 ? uc[(int64_t)r * ncn + cix[q]] : make_float2(0.f, 0.f);
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) fv[q] = i1[q] >= 0 ? f[(int64_t)r * n + i1[q]] : make_float2(0.f, 0.f);
+        if (fscale) {
+            const float fs = (float)fscale[r];
+#pragma unroll
+            for (int q = 0; q < MT_SPT; ++q) fv[q] = make_float2(fv[q].x * fs, fv[q].y * fs);
+        }
     };
     if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
@@ -1390,9 +1402,12 @@ static int32_t tail_views(helm_plan* p) {
 // alias the level's scratch: on level 0 they are the caller's input and
 // output, on coarser levels L.mg_f and L.mg_u.  L.mg_r holds the residual,
 // L.mg_t is the Jacobi ping-pong partner of u.
-static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s) {
+static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s,
+                      const double* fscale = nullptr) {
     Level& L = p->levels[l];
     const MGConfig& c = p->mg;
+    HELM_CHECK(!fscale || (l == 0 && mg_scaled_input_ok(p)), HELM_ERR_STATE,
+               "multigrid: scaled input needs the fused finest-level kernels");
     if (L.n <= tail_nodes()) {
         const size_t full = tail_lds_bytes_full(p, l);
         if (tail_all_lds() && full <= 160 * 1024)
@@ -1432,13 +1447,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
                                   rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
-                                  sh, (int)adj, (const float2*)L.wid, tab, f, L.mg_t, C.mg_f, xcd_pre(l));
+                                  sh, (int)adj, (const float2*)L.wid, tab, f, fscale, L.mg_t, C.mg_f, xcd_pre(l));
             prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ), 0,
                                s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
-                               L.mg_t, C.mg_f, xcd_pre(l));
+                               fscale, L.mg_t, C.mg_f, xcd_pre(l));
         }
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
@@ -1448,13 +1463,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, (const float2*)C.mg_u,
-                                  (const float2*)L.mg_t, f, u, xcd_post(l));
+                                  (const float2*)L.mg_t, f, fscale, u, xcd_post(l));
             prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, L.nz, L.nx,
                                L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                                L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
-                               L.mg_t, f, u, xcd_post(l));
+                               L.mg_t, f, fscale, u, xcd_post(l));
         }
         HELM_HIP(hipGetLastError());
         return HELM_OK;
@@ -1512,7 +1527,13 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     return HELM_OK;
 }
 
-int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+bool mg_scaled_input_ok(const helm_plan* p) {
+    return p->levels.size() > 1 && fused_levels() && p->mg.pre == 2 && p->mg.post == 2 &&
+           p->levels[0].n > tail_nodes() && p->levels[0].op != HELM_OP_COND8;
+}
+
+int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                  const double* fscale) {
     HELM_CHECK(p->mg.levels > 0, HELM_ERR_STATE, "multigrid not set up");
     bool grew = false;
     for (auto& L : p->levels) {
@@ -1524,7 +1545,7 @@ int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoi
         p->mg_views_valid = true;
         p->generation++;
     }
-    return vcycle(p, 0, v, z, nrhs, adjoint, s);
+    return vcycle(p, 0, v, z, nrhs, adjoint, s, fscale);
 }
 
 // Time one V-cycle for each workgroup target and keep the fastest.

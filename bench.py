@@ -82,6 +82,7 @@ def parse():
     ap.add_argument("--cond-shape", type=int, nargs=2, default=[960, 1216],
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
+    ap.add_argument("--cond-restart", type=int, default=8, help="FGMRES restart length of the conditioned workload")
     ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
                          "64 frequencies x 11 angles on uneven grids")
@@ -356,7 +357,7 @@ def conditioned_workload(a):
     g = FrequencyGeometry(spec)
     medium = torch.tensor(np.stack((c, rho)), dtype=torch.float64, device="cuda")
     alpha_d = torch.tensor(alpha, dtype=torch.float64, device="cuda")
-    system = ConditionedFrequencySystem(g, medium, alpha_d, f, nrhs=R, restart=8)
+    system = ConditionedFrequencySystem(g, medium, alpha_d, f, nrhs=R, restart=a.cond_restart)
     x = (np.arange(w) - (w - 1) / 2) * dx
     taper = np.hanning(w)
     lines = np.stack([taper * np.exp(-2j * np.pi * f * x * np.sin(np.deg2rad(t)) / 1540.0)
@@ -369,7 +370,7 @@ def conditioned_workload(a):
     def step():
         system.update(medium, alpha_d)
         b = source_rhs(system, lines_d)
-        return solve_system(system, b, tolerance=a.cond_tol, mixed=True, max_cycles=1200)
+        return solve_system(system, b, tolerance=a.cond_tol, mixed=True, max_cycles=1200, restart=a.cond_restart)
 
     for _ in range(a.warmup):
         step()
@@ -424,7 +425,7 @@ def conditioned_workload(a):
         "data": "synthetic (seeded smooth random c, rho, alpha)",
         "config": {"workload": f"conditioned {h}x{w} medium, box {g.H}x{g.W}, 24-cell split PML, "
                                f"{f / 1e6:.3f} MHz x{R} angles", "tolerance": a.cond_tol,
-                   "method": "fgmres(8) + multigrid", "parallelism": "single GPU"},
+                   "method": f"fgmres({a.cond_restart}) + multigrid", "parallelism": "single GPU"},
         "steps_per_solve": info["steps"], "relative_residual": info["residual"],
         "max_angle_residual": info["max_angle_residual"], "reason": info["reason"],
         "roofline": {"bound": "hbm", "kernel": "k_cond8s (FP32 streaming) forward, all RHS",

@@ -168,6 +168,12 @@ struct Workspace {
     std::vector<double> h_rel;
     std::vector<int> h_active;
     std::vector<int> h_iters;
+    // restart status written by the device straight into pinned, mapped host
+    // memory (rel, active, iters per RHS): no copies per restart
+    void* hstat = nullptr;      // host block (hipHostMalloc)
+    double* dstat_rel = nullptr;  // its device-side addresses
+    int* dstat_active = nullptr;
+    int* dstat_iters = nullptr;
 };
 
 // HIP-event timing of the kernels inside a solve (helm_profile)

@@ -586,7 +586,8 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
                              const double* __restrict__ bnorm, double* __restrict__ beta, double* __restrict__ rel,
                              double* __restrict__ inv, int* __restrict__ active, int* __restrict__ cyc,
                              int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g,
-                             double* __restrict__ pscale) {
+                             double* __restrict__ pscale, double* __restrict__ hrel, int* __restrict__ hact,
+                             int* __restrict__ hit) {
     __shared__ double lds[8];
     const int r = blockIdx.x;
     double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
@@ -602,6 +603,11 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
         cyc[r] = active[r];
         inv[r] = b > 0 ? 1.0 / b : 0.0;
         if (pscale) pscale[r] = 1.0;  // V_0 is stored normalised
+        if (hrel) {  // the host's copy (pinned, mapped): read after the stream synchronises
+            hrel[r] = rr;
+            hact[r] = active[r];
+            hit[r] = iters[r];
+        }
         g[(int64_t)r * (m + 1)] = make_double2(b, 0.0);
         for (int k = 1; k <= m; ++k) g[(int64_t)r * (m + 1) + k] = make_double2(0.0, 0.0);
     }
@@ -732,6 +738,17 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     w.h_rel.assign(nrhs, 0.0);
     w.h_active.assign(nrhs, 0);
     w.h_iters.assign(nrhs, 0);
+    if (w.hstat) (void)hipHostFree(w.hstat);
+    w.hstat = nullptr;
+    HELM_HIP(hipHostMalloc(&w.hstat, (size_t)nrhs * (sizeof(double) + 2 * sizeof(int)),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    {
+        char* base = nullptr;
+        HELM_HIP(hipHostGetDevicePointer((void**)&base, w.hstat, 0));
+        w.dstat_rel = (double*)base;
+        w.dstat_active = (int*)(base + (size_t)nrhs * sizeof(double));
+        w.dstat_iters = w.dstat_active + nrhs;
+    }
     return HELM_OK;
 }
 
@@ -907,12 +924,16 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         applications++;
         hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
         hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, nch, mm, o->tolerance, W.partial, W.bnorm,
-                           W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale);
+                           W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, W.dstat_rel,
+                           W.dstat_active, W.dstat_iters);
         HELM_HIP(hipGetLastError());
-        HELM_HIP(hipMemcpyAsync(W.h_rel.data(), W.rel, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
-        HELM_HIP(hipMemcpyAsync(W.h_active.data(), W.active, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
-        HELM_HIP(hipMemcpyAsync(W.h_iters.data(), W.iters, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipStreamSynchronize(s));
+        {
+            const char* hs = (const char*)W.hstat;
+            std::memcpy(W.h_rel.data(), hs, nrhs * sizeof(double));
+            std::memcpy(W.h_active.data(), hs + (size_t)W.nrhs * sizeof(double), nrhs * sizeof(int));
+            std::memcpy(W.h_iters.data(), hs + (size_t)W.nrhs * (sizeof(double) + sizeof(int)), nrhs * sizeof(int));
+        }
         if (pf) prof_flush(pf);
         if (o->callback) o->callback(o->user, applications, W.h_rel.data(), nrhs);
         int n_active = 0;

@@ -114,6 +114,7 @@ void free_workspace(Workspace& w) {
                     w.bshadow, w.br, w.bsc};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    if (w.hstat) (void)hipHostFree(w.hstat);
     w = Workspace();
 }
 

@@ -524,8 +524,8 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up);
     // the flux stencil's streaming form is no longer a candidate (its block form measured faster on
     // every grid from 576^2 to 2048^2) but stays selectable with helm_set_stencil_shape
-    const int ntz = cond ? 4 : 0;
-    const int tzs_flux[4] = {4, 8, 16, 16}, tzs_cond[4] = {32, 64, 128, 256}, rbs[3] = {1, 2, 4};
+    const int ntz = cond ? 5 : 0;
+    const int tzs_flux[5] = {4, 8, 16, 16, 16}, tzs_cond[5] = {8, 16, 32, 64, 128}, rbs[3] = {1, 2, 4};
     const int* tzs = cond ? tzs_cond : tzs_flux;
     const double k0sq = p->k0 * p->k0;
     auto run = [&]() {

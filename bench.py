@@ -34,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 
 # profiled single kernels of the headline solve -> kernel names (the
 # "preconditioner" entry is the whole V-cycle, a group, so it is not one)
-names = {"dcgs_dots": "k_dcgs_dots", "dcgs_update": "k_dcgs_update", "operator_apply": "k_apply<float>",
+names = {"dcgs_dots": "k_dcgs_dots", "dcgs_update": "k_dcgs_reduce_update", "operator_apply": "k_apply_block<float>",
          "true_residual": "k_apply<double> (residual)", "mg_pre2_restrict": "k_mgt_pre2_rr",
          "mg_prolong_post2": "k_mgt_post2"}
 # the multigrid kernels are timed on the finest level only inside the timed
@@ -247,11 +247,13 @@ def pmc_summary(directory):
         big = vals[-COPY_REPS:]  # the calibration copies are the largest dispatches
         scale[counter] = COPY_BYTES / (sum(big) / len(big))
     # matched in this order (a name that contains another comes first)
-    short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_update": "k_dcgs_update", "k_apply<float": "k_apply<float>",
+    short = {"k_dcgs_dots": "k_dcgs_dots", "k_dcgs_reduce_update": "k_dcgs_reduce_update",
+             "k_dcgs_norm_scale": "k_dcgs_norm_scale", "k_dcgs_update": "k_dcgs_update", "k_tail_lds": "k_tail_lds",
+             "k_apply<float": "k_apply<float>",
              "k_apply_stream": "k_apply_stream<float>", "k_apply_block": "k_apply_block<float>",
              "k_mgt_pre2_rr": "k_mgt_pre2_rr", "k_mgt_post2": "k_mgt_post2", "k_mgt_jacobi": "k_mgt_jacobi",
              "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
-             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail": "k_tail", "k_cond8s": "k_cond8s<float>"}
+             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail(": "k_tail", "k_cond8s": "k_cond8s<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
     # multigrid kernels run on every level: their finest-level launches (the
     # largest grid) are also summarised on their own ("<name> (finest level)")

@@ -36,6 +36,7 @@
 
 #include <hip/hip_ext.h>
 
+#include "helm_coef.h"
 #include "helm_internal.h"
 
 namespace helm {
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
                                                             double2* __restrict__ g, double* __restrict__ est,
                                                             int* __restrict__ cyc, int* __restrict__ used,
                                                             float2* __restrict__ V, float2* __restrict__ w,
-                                                            double2* __restrict__ partial) {
+                                                            double2* __restrict__ partial, int v0_raw) {
     __shared__ double lds[8];
     __shared__ double2 sc[64], sd[64];
     __shared__ float2 c_s[64], d_s[64];
@@ -435,6 +436,7 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
         double alpha = 1.0;
         bool stop = false;
         double2* Hr = H + (int64_t)r * m * (m + 1);
+        if (j == 0 && v0_raw) alpha = sqrt(fmax(sc[0].x, 1e-300));  // V_0 = r, not r / beta
         if (j > 0) {
             double c2 = 0.0;
             for (int k = 0; k < j; ++k) c2 += sc[k].x * sc[k].x + sc[k].y * sc[k].y;
@@ -587,7 +589,7 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
                              double* __restrict__ inv, int* __restrict__ active, int* __restrict__ cyc,
                              int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g,
                              double* __restrict__ pscale, double* __restrict__ hrel, int* __restrict__ hact,
-                             int* __restrict__ hit) {
+                             int* __restrict__ hit, int pscale_v0) {
     __shared__ double lds[8];
     const int r = blockIdx.x;
     double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
@@ -602,7 +604,7 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
         if (!isfinite(rr)) active[r] = 0;
         cyc[r] = active[r];
         inv[r] = b > 0 ? 1.0 / b : 0.0;
-        if (pscale) pscale[r] = 1.0;  // V_0 is stored normalised
+        if (pscale) pscale[r] = pscale_v0 ? inv[r] : 1.0;  // V_0 stored unnormalised / normalised
         if (hrel) {  // the host's copy (pinned, mapped): read after the stream synchronises
             hrel[r] = rr;
             hact[r] = active[r];
@@ -623,6 +625,49 @@ __global__ void k_scale_to32(int64_t n, const double2* __restrict__ src, float2*
     double2 a = src[(int64_t)r * n + p];
     double s = inv[r];
     dst[(int64_t)r * n + p] = make_float2((float)(a.x * s), (float)(a.y * s));
+}
+
+// Restart residual of the flux-form operator in one pass (lazily scaled
+// basis): r = b - A x in FP64 (x c128), written straight into V_0 as c64 and
+// left unnormalised (the multigrid applies 1/beta as it loads V_0, and the
+// first step's reorthogonalisation divides V_0 by its norm), with |r|^2 per
+// workgroup and RHS for k_cycle_init (partial[r * nchr + workgroup]).  This
+// replaces the FP64 apply, the norm pass over r and the c128 -> c64
+// conversion of r/beta.  One workgroup = 64 columns x 4 rows x rb RHS.
+__global__ __launch_bounds__(256) void k_residual_v0(int nz, int nx, int64_t n, int nrhs, int rb,
+                                                     const double2* __restrict__ sx, const double2* __restrict__ sz,
+                                                     const double2* __restrict__ isxe,
+                                                     const double2* __restrict__ isze,
+                                                     const double* __restrict__ eta, const double2* __restrict__ kap,
+                                                     double k0sq, double hh, int conj, const double2* __restrict__ x,
+                                                     const double2* __restrict__ b, float2* __restrict__ v0,
+                                                     double2* __restrict__ partial) {
+    __shared__ double lds[8];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * 64 + (tid & 63), i = blockIdx.y * 4 + (tid >> 6);
+    const bool inb = i < nz && j < nx;
+    Coef<double2> c;
+    if (inb) node_coef<double, double2>(i, j, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, 0.0, conj != 0, c);
+    const int64_t idx = (int64_t)i * nx + j;
+    const bool hw = j > 0, he = j < nx - 1, hn = i > 0, hs = i < nz - 1;
+    const int nchr = gridDim.x * gridDim.y, wg = blockIdx.y * gridDim.x + blockIdx.x;
+    const int r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
+    for (int r = blockIdx.z * rb; r < r1; ++r) {
+        double nrm = 0.0;
+        if (inb) {
+            const double2* xr = x + (int64_t)r * n;
+            double2 acc = cmul(c.dg, xr[idx]);
+            if (hw) cfma(acc, c.fw, xr[idx - 1]);
+            if (he) cfma(acc, c.fe, xr[idx + 1]);
+            if (hn) cfma(acc, c.fn, xr[idx - nx]);
+            if (hs) cfma(acc, c.fs, xr[idx + nx]);
+            const double2 res = csub(b[(int64_t)r * n + idx], acc);
+            v0[(int64_t)r * n + idx] = make_float2((float)res.x, (float)res.y);
+            nrm = res.x * res.x + res.y * res.y;
+        }
+        const double2 tot = block_sum(make_double2(nrm, 0.0), lds);  // every thread of the block takes part
+        if (tid == 0) partial[(int64_t)r * nchr + wg] = tot;
+    }
 }
 
 // v_{j+1}[r] = w[r] / h_{j+1,j}
@@ -916,16 +961,31 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const double coef_bytes32 = L.op != HELM_OP_FLUX5 ? 16.0 : 12.0;  // per node, FP32 coefficients
     const double apply_bytes = 16.0 * nn * rr + coef_bytes32 * nn;
     const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
+    // lazily scaled basis: the multigrid's finest-level kernels normalise V_j as they load it
+    const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG && mg_scaled_input_ok(p);
+    // ... and for the flux operator the restart residual is written straight into V_0 (k_residual_v0)
+    const dim3 rgrid((unsigned)((L.nx + 63) / 64), (unsigned)((L.nz + 3) / 4), 1);
+    const int nchr = (int)(rgrid.x * rgrid.y);
+    const int rgroups = (int)std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + nchr - 1) / nchr));
+    const int rrb = (nrhs + rgroups - 1) / rgroups;
+    const bool v0raw = lazy && L.op == HELM_OP_FLUX5 && (int64_t)nrhs * nchr <= W.partial_len;
     for (;;) {
         // true residual in FP64
         hipEvent_t ev = pf ? prof_begin(pf, PK_RESIDUAL, s) : nullptr;
-        HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
+        if (v0raw) {
+            hipLaunchKernelGGL(k_residual_v0, dim3(rgrid.x, rgrid.y, (nrhs + rrb - 1) / rrb), dim3(256), 0, s, L.nz,
+                               L.nx, n, nrhs, rrb, L.sx64, L.sz64, L.isxe64, L.isze64, L.eta64, L.kap64, k0sq,
+                               0.5 / (L.h_m * L.h_m), (int)adj, (const double2*)x, b, W.V, W.partial);
+        } else {
+            HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
+        }
         if (pf) prof_end(pf, PK_RESIDUAL, ev, resid_bytes, s);
         applications++;
-        hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
-        hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, nch, mm, o->tolerance, W.partial, W.bnorm,
-                           W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, W.dstat_rel,
-                           W.dstat_active, W.dstat_iters);
+        if (!v0raw)
+            hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
+        hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, v0raw ? nchr : nch, mm, o->tolerance, W.partial,
+                           W.bnorm, W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, W.dstat_rel,
+                           W.dstat_active, W.dstat_iters, (int)v0raw);
         HELM_HIP(hipGetLastError());
         HELM_HIP(hipStreamSynchronize(s));
         {
@@ -946,11 +1006,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         }
         cycles++;
         const int msteps = std::min(m, o->max_iterations - steps);
-        // lazily scaled basis: the multigrid's finest-level kernels normalise V_j as they load it
-        const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG && mg_scaled_input_ok(p);
         // one restart cycle: V_0 = r/beta, msteps flexible DCGS2 steps, x += Z y
         auto enqueue_cycle = [&](int nsteps, hipStream_t cs) -> int32_t {
-            hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
+            if (!v0raw) hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
             for (int j = 0; j < nsteps; ++j) {
                 float2* vj = W.V + (int64_t)j * vstride;
                 // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
@@ -988,12 +1046,12 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                         hipExtLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs,
                                               nch, nchd, mm, kmax, j, o->tolerance, (const double2*)W.partial,
                                               (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g,
-                                              W.est, W.cyc, W.used, W.V, wj, pw2);
+                                              W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
                         prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
                     } else {
                         hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, nchd, mm,
                                            kmax, j, o->tolerance, W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g,
-                                           W.est, W.cyc, W.used, W.V, wj, pw2);
+                                           W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
                     }
                     const bool last = j + 1 >= nsteps;
                     hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last || lazy ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm,

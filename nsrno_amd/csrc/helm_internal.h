@@ -153,8 +153,6 @@ struct Workspace {
     int* cyc = nullptr;       // nrhs: active in this cycle
     int* used = nullptr;      // nrhs: steps used in this cycle
     int* iters = nullptr;     // nrhs: Krylov steps so far
-    int* gate = nullptr;      // nrhs: second Gram-Schmidt pass needed
-    double* hn2 = nullptr;    // nrhs: |w|^2 after the first pass
     // BiCGSTAB vectors (c64, nrhs * n each) and scalars
     float2 *bp = nullptr, *bv = nullptr, *bs = nullptr, *bt = nullptr, *bphat = nullptr, *bshat = nullptr,
            *bshadow = nullptr, *br = nullptr;

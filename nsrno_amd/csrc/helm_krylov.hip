@@ -778,8 +778,6 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     HELM_TRY(alloc(&w.cyc, nrhs));
     HELM_TRY(alloc(&w.used, nrhs));
     HELM_TRY(alloc(&w.iters, nrhs));
-    HELM_TRY(alloc(&w.gate, nrhs));
-    HELM_TRY(alloc(&w.hn2, nrhs));
     w.h_rel.assign(nrhs, 0.0);
     w.h_active.assign(nrhs, 0);
     w.h_iters.assign(nrhs, 0);

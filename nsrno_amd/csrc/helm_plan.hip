@@ -110,7 +110,7 @@ void free_workspace(Workspace& w) {
     if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
     if (w.gstream) (void)hipStreamDestroy(w.gstream);
     void* ptrs[] = {w.V, w.Z, w.w, w.r64, w.partial, w.H, w.cs, w.sn, w.g, w.coef, w.beta, w.bnorm, w.rel,
-                    w.inv, w.pscale, w.est, w.active, w.cyc, w.used, w.iters, w.gate, w.hn2, w.bp, w.bv, w.bs, w.bt, w.bphat, w.bshat,
+                    w.inv, w.pscale, w.est, w.active, w.cyc, w.used, w.iters, w.bp, w.bv, w.bs, w.bt, w.bphat, w.bshat,
                     w.bshadow, w.br, w.bsc};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);

@@ -812,7 +812,7 @@ __global__ void k_cond_source(CondGeom g, int face, int zi, int xi, double dx, c
 // is the identity there and decoupled from the box).
 __global__ void k_cond_prescale(int H, int W, int64_t n, int nrhs, int bz0, int bx0, int bh, int bw, float k0sq,
                                 const float2* __restrict__ sx, const float2* __restrict__ sz,
-                                const float2* __restrict__ v, float2* out) {
+                                const float2* __restrict__ v, float2* out, const double* __restrict__ fscale) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * nrhs) return;
     const int64_t node = e % n;
@@ -821,13 +821,14 @@ __global__ void k_cond_prescale(int H, int W, int64_t n, int nrhs, int bz0, int 
     if (i >= bz0 && i < bz0 + bh && j >= bx0 && j < bx0 + bw) {
         const float2 f = cscale(cconj(cmul(sx[j], sz[i])), k0sq);  // the profiles hold conj(s)
         r = cmul(f, v[e]);
+        if (fscale) r = cscale(r, (float)fscale[e / n]);
     }
     out[e] = r;
 }
 
 __global__ void k_cond_post(int H, int W, int64_t n, int nrhs, int bz0, int bx0, int bh, int bw, float k0sq,
                             int adjoint, const float2* __restrict__ sx, const float2* __restrict__ sz,
-                            const float2* __restrict__ v, float2* z) {
+                            const float2* __restrict__ v, float2* z, const double* __restrict__ fscale) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * nrhs) return;
     const int64_t node = e % n;
@@ -835,11 +836,12 @@ __global__ void k_cond_post(int H, int W, int64_t n, int nrhs, int bz0, int bx0,
     if (i >= bz0 && i < bz0 + bh && j >= bx0 && j < bx0 + bw) {
         if (adjoint) z[e] = cmul(cscale(cmul(sx[j], sz[i]), k0sq), z[e]);
     } else {
-        z[e] = v[e];
+        z[e] = fscale ? cscale(v[e], (float)fscale[e / n]) : v[e];
     }
 }
 
-int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                     const double* fscale) {
     const Level& L = p->levels[0];
     const int64_t cnt = (int64_t)nrhs * L.n;
     const dim3 grid((unsigned)((cnt + 255) / 256)), block(256);
@@ -854,14 +856,14 @@ int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool ad
             p->generation++;
         }
         hipLaunchKernelGGL(k_cond_prescale, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1],
-                           L.cbox[2], L.cbox[3], k0sq, L.sx32, L.sz32, v, p->cond_scr);
+                           L.cbox[2], L.cbox[3], k0sq, L.sx32, L.sz32, v, p->cond_scr, fscale);
         HELM_HIP(hipGetLastError());
         HELM_TRY(mg_vcycle(p, p->cond_scr, z, nrhs, true, s));
     } else {
-        HELM_TRY(mg_vcycle(p, v, z, nrhs, false, s));
+        HELM_TRY(mg_vcycle(p, v, z, nrhs, false, s, fscale));
     }
     hipLaunchKernelGGL(k_cond_post, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1], L.cbox[2],
-                       L.cbox[3], k0sq, (int)adjoint, L.sx32, L.sz32, v, z);
+                       L.cbox[3], k0sq, (int)adjoint, L.sx32, L.sz32, v, z, fscale);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

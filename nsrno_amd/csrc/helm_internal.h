@@ -268,13 +268,16 @@ int32_t cond_level_profiles(const helm_plan* p, int level, std::vector<double2>&
                             std::vector<double2>& isxe, std::vector<double2>& isze);
 int32_t upload_cond_profiles(helm_plan* p, int level);
 // the multigrid preconditioner of a conditioned plan (node scaling + V-cycle)
-int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
+// fscale (nullable, per RHS): precondition fscale[r] * v[r]; with adjoint = true it
+// needs mg_scaled_input_ok (the multigrid then applies it)
+int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                     const double* fscale = nullptr);
 // multigrid (helm_mg.hip)
 // fscale (nullable, per RHS): the V-cycle solves for fscale[r] * v[r] (applied as the
 // finest level's kernels load v); only where mg_scaled_input_ok holds
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
                   const double* fscale = nullptr);
-bool mg_scaled_input_ok(const helm_plan* p);
+bool mg_scaled_input_ok(const helm_plan* p);  // the finest level runs the fused kernels
 int32_t mg_build(helm_plan* p, hipStream_t s);
 // krylov (helm_krylov.hip)
 int32_t ensure_workspace(helm_plan* p, int nrhs, int restart);

@@ -960,7 +960,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const double apply_bytes = 16.0 * nn * rr + coef_bytes32 * nn;
     const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
     // lazily scaled basis: the multigrid's finest-level kernels normalise V_j as they load it
-    const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG && mg_scaled_input_ok(p);
+    // (conditioned plans: the prescale kernel applies it, or on adjoint solves the multigrid)
+    const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG &&
+                      ((L.op == HELM_OP_COND8 && !adj) || mg_scaled_input_ok(p));
     // ... and for the flux operator the restart residual is written straight into V_0 (k_residual_v0)
     const dim3 rgrid((unsigned)((L.nx + 63) / 64), (unsigned)((L.nz + 3) / 4), 1);
     const int nchr = (int)(rgrid.x * rgrid.y);
@@ -1016,7 +1018,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                 hipEvent_t ev;
                 if (pc != HELM_PC_NONE) {
                     ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
-                    if (lazy)
+                    if (lazy && L.op == HELM_OP_COND8)
+                        HELM_TRY(cond_precond(p, vj, zj, nrhs, adj, cs, W.pscale));
+                    else if (lazy)
                         HELM_TRY(mg_vcycle(p, vj, zj, nrhs, adj, cs, W.pscale));
                     else
                         HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));

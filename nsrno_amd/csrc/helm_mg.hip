@@ -1529,7 +1529,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
 
 bool mg_scaled_input_ok(const helm_plan* p) {
     return p->levels.size() > 1 && fused_levels() && p->mg.pre == 2 && p->mg.post == 2 &&
-           p->levels[0].n > tail_nodes() && p->levels[0].op != HELM_OP_COND8;
+           p->levels[0].n > tail_nodes();
 }
 
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,

@@ -126,16 +126,16 @@ int32_t net_forward(helm_net* n, hipStream_t s) {
     HELM_TRY(conv_fwd(n, C0A, 0, n->in, n->a[0], true, s));
     HELM_TRY(conv_fwd(n, C0B, 0, n->a[0], n->e[0], true, s));
     for (int l = 1; l < 4; ++l) {
-        hipLaunchKernelGGL(k_pool2, g1d(lvl_elems(n, l)), dim3(256), 0, s, n->batch, n->H >> (l - 1), n->W >> (l - 1),
-                           n->e[l - 1], n->p[l]);
+        hipLaunchKernelGGL(k_pool2, dim3((unsigned)(((n->W >> l) + 63) / 64), (unsigned)(n->H >> l), n->batch),
+                           dim3(256), 0, s, n->batch, n->H >> (l - 1), n->W >> (l - 1), n->e[l - 1], n->p[l]);
         HELM_TRY(conv_fwd(n, C0A + 2 * l, l, n->p[l], n->a[l], true, s));
         HELM_TRY(conv_fwd(n, C0B + 2 * l, l, n->a[l], n->e[l], true, s));
     }
     const int up_layers[3] = {C0C, C1C, C2C};
     const bf16* coarse = n->e[3];
     for (int l = 2; l >= 0; --l) {
-        hipLaunchKernelGGL(k_up2_add, g1d(lvl_elems(n, l)), dim3(256), 0, s, n->batch, n->H >> l, n->W >> l, coarse,
-                           n->e[l], n->s[l]);
+        hipLaunchKernelGGL(k_up2_add, dim3((unsigned)(((n->W >> l) + 63) / 64), (unsigned)(n->H >> l), n->batch),
+                           dim3(256), 0, s, n->batch, n->H >> l, n->W >> l, coarse, n->e[l], n->s[l]);
         HELM_TRY(conv_fwd(n, up_layers[l], l, n->s[l], n->d[l], true, s));
         coarse = n->d[l];
     }

@@ -106,6 +106,15 @@ static int dot_npt() {
     }();
     return t;
 }
+// restart residual of the flux operator in its block form (HELM_RESID_BLOCK=0: the per-node kernel)
+constexpr int RES_RB = 2, RES_TZ = 2;
+static bool resid_block() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_RESID_BLOCK");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
 // V_{j+1} left unnormalised and scaled by the preconditioner as it loads it
 // (multigrid on the fused path only; HELM_LAZY_BASIS=0: a scaling pass per step)
 static bool lazy_basis() {
@@ -670,6 +679,154 @@ __global__ __launch_bounds__(256) void k_residual_v0(int nz, int nx, int64_t n, 
     }
 }
 
+// Block form of k_residual_v0 (the block form of the FP32 stencil in FP64):
+// one wavefront per (64-column strip, run of TZ rows, RB right-hand sides)
+// loads every value of its run before forming the first node -- TZ + 2 rows
+// of x per RHS, eta, kappa, b, and the strip's west/east halo columns packed
+// one value per lane and handed to the edge lanes with readlane -- so a
+// wavefront waits for one memory latency; |r|^2 is reduced per wavefront by
+// shuffles and per workgroup (4 wavefronts) in LDS.  partial[r * nchr + wg]
+// with nchr = gridDim.x * gridDim.y as in k_residual_v0.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double2 shfl_up1d(double2 v) {
+    return make_double2(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64));
+}
+__device__ __forceinline__ double2 shfl_down1d(double2 v) {
+    return make_double2(__shfl_down(v.x, 1, 64), __shfl_down(v.y, 1, 64));
+}
+
+template <int RB, int TZ>
+__global__ __launch_bounds__(256) void k_residual_block(int nz, int nx, int64_t n, int nrhs,
+                                                        const double2* __restrict__ sx,
+                                                        const double2* __restrict__ sz,
+                                                        const double2* __restrict__ isxe,
+                                                        const double2* __restrict__ isze,
+                                                        const double* __restrict__ eta,
+                                                        const double2* __restrict__ kap, double k0sq, double hh,
+                                                        int conj, const double2* __restrict__ x,
+                                                        const double2* __restrict__ b, float2* __restrict__ v0,
+                                                        double2* __restrict__ partial) {
+    static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
+    __shared__ double red[4][RB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 64, j = x0 + lane;
+    const int i0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * TZ);
+    const int r0 = blockIdx.z * RB;
+    const int nchr = gridDim.x * gridDim.y, wg = blockIdx.y * gridDim.x + blockIdx.x;
+    double nrm[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) nrm[q] = 0.0;
+    if (i0 < nz) {  // uniform over the wavefront; every wavefront still takes part in the reduction below
+        const bool colok = j < nx;
+        const int jj = colok ? j : nx - 1;
+        const bool hw = jj > 0, he = jj < nx - 1;
+        const bool wstrip = x0 > 0, estrip = x0 + 64 < nx;
+        double2 SZ[TZ], IZ[TZ + 1];
+#pragma unroll
+        for (int t = 0; t < TZ; ++t) SZ[t] = sz[min(i0 + t, nz - 1)];
+#pragma unroll
+        for (int t = 0; t <= TZ; ++t) IZ[t] = isze[min(i0 + t, nz)];
+        double2 U[RB][TZ + 2], Bv[RB][TZ];
+        double E[TZ + 2];
+        double2 K[TZ];
+#pragma unroll
+        for (int t = 0; t < TZ + 2; ++t) {
+            const int i = i0 - 1 + t;
+            const bool rok = i >= 0 && i < nz;
+#pragma unroll
+            for (int q = 0; q < RB; ++q)
+                U[q][t] = (rok && r0 + q < nrhs) ? x[(int64_t)(r0 + q) * n + (int64_t)i * nx + jj]
+                                                 : make_double2(0.0, 0.0);
+            E[t] = rok ? eta[(int64_t)i * nx + jj] : 1.0;
+        }
+#pragma unroll
+        for (int t = 0; t < TZ; ++t) {
+            const bool rok = i0 + t < nz;
+            K[t] = rok ? kap[(int64_t)(i0 + t) * nx + jj] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < RB; ++q)
+                Bv[q][t] = (rok && r0 + q < nrhs) ? b[(int64_t)(r0 + q) * n + (int64_t)(i0 + t) * nx + jj]
+                                                  : make_double2(0.0, 0.0);
+        }
+        double2 HW = make_double2(0.0, 0.0), HE = make_double2(0.0, 0.0);
+        double EH = 1.0;  // lanes 0..TZ-1: west eta of row t; lanes 32..32+TZ-1: east eta
+        {
+            const int q = lane / TZ, t = lane % TZ, i = i0 + t;
+            if (lane < RB * TZ && i < nz && r0 + q < nrhs) {
+                const int64_t o = (int64_t)(r0 + q) * n + (int64_t)i * nx;
+                if (wstrip) HW = x[o + x0 - 1];
+                if (estrip) HE = x[o + x0 + 64];
+            }
+            if (lane < TZ && i0 + lane < nz && wstrip) EH = eta[(int64_t)(i0 + lane) * nx + x0 - 1];
+            if (lane >= 32 && lane < 32 + TZ && i0 + lane - 32 < nz && estrip)
+                EH = eta[(int64_t)(i0 + lane - 32) * nx + x0 + 64];
+        }
+        const double2 sxj = sx[jj], iw = isxe[jj], ie = isxe[jj + 1];
+        const bool cj = conj != 0;
+#pragma unroll
+        for (int t = 0; t < TZ; ++t) {
+            const int i = i0 + t;
+            const double en = E[t], ec = E[t + 1], es = E[t + 2];
+            double eww = __shfl_up(ec, 1, 64), eee = __shfl_down(ec, 1, 64);
+            const double ehw = readlane_d(EH, t), ehe = readlane_d(EH, 32 + t);
+            if (lane == 0) eww = ehw;
+            if (lane == 63) eee = ehe;
+            if (!hw) eww = 1.0;
+            if (!he) eee = 1.0;
+            Coef<double2> c;
+            c.fw = cscale(cmul(SZ[t], iw), hh * (ec + eww));
+            c.fe = cscale(cmul(SZ[t], ie), hh * (ec + eee));
+            c.fn = cscale(cmul(sxj, IZ[t]), hh * (ec + en));
+            c.fs = cscale(cmul(sxj, IZ[t + 1]), hh * (ec + es));
+            const double2 mass = cscale(cmul(cmul(SZ[t], sxj), K[t]), k0sq);
+            c.dg = csub(mass, cadd(cadd(c.fw, c.fe), cadd(c.fn, c.fs)));
+            if (cj) {
+                c.dg = cconj(c.dg);
+                c.fw = cconj(c.fw);
+                c.fe = cconj(c.fe);
+                c.fn = cconj(c.fn);
+                c.fs = cconj(c.fs);
+            }
+            const bool hn = i > 0, hs = i < nz - 1;
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const double2 uc = U[q][t + 1];
+                double2 w = shfl_up1d(uc), e = shfl_down1d(uc);
+                const double2 hwv = make_double2(readlane_d(HW.x, q * TZ + t), readlane_d(HW.y, q * TZ + t));
+                const double2 hev = make_double2(readlane_d(HE.x, q * TZ + t), readlane_d(HE.y, q * TZ + t));
+                if (lane == 0) w = hwv;
+                if (lane == 63) e = hev;
+                double2 acc = cmul(c.dg, uc);
+                if (hw) cfma(acc, c.fw, w);
+                if (he) cfma(acc, c.fe, e);
+                if (hn) cfma(acc, c.fn, U[q][t]);
+                if (hs) cfma(acc, c.fs, U[q][t + 2]);
+                const int r = r0 + q;
+                if (colok && r < nrhs && i < nz) {
+                    const double2 res = csub(Bv[q][t], acc);
+                    v0[(int64_t)r * n + (int64_t)i * nx + j] = make_float2((float)res.x, (float)res.y);
+                    nrm[q] += res.x * res.x + res.y * res.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        const double v = wave_sum(nrm[q]);
+        if (lane == 0) red[wave][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < RB && r0 + (int)threadIdx.x < nrhs) {
+        const int q = threadIdx.x;
+        const double tot = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+        partial[(int64_t)(r0 + q) * nchr + wg] = make_double2(tot, 0.0);
+    }
+}
+
 // v_{j+1}[r] = w[r] / h_{j+1,j}
 __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __restrict__ dst,
                           const double* __restrict__ inv, const int* __restrict__ cyc) {
@@ -969,10 +1126,18 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const int rgroups = (int)std::max<int64_t>(1, std::min<int64_t>(nrhs, (2048 + nchr - 1) / nchr));
     const int rrb = (nrhs + rgroups - 1) / rgroups;
     const bool v0raw = lazy && L.op == HELM_OP_FLUX5 && (int64_t)nrhs * nchr <= W.partial_len;
+    // its block form: 64 columns x RES_TZ rows x RES_RB RHS per wavefront, 4 wavefronts per workgroup
+    const dim3 bgrid((unsigned)((L.nx + 63) / 64), (unsigned)((L.nz + 4 * RES_TZ - 1) / (4 * RES_TZ)), 1);
+    const int nchb = (int)(bgrid.x * bgrid.y);
+    const int nchr_used = resid_block() ? nchb : nchr;
     for (;;) {
         // true residual in FP64
         hipEvent_t ev = pf ? prof_begin(pf, PK_RESIDUAL, s) : nullptr;
-        if (v0raw) {
+        if (v0raw && resid_block()) {
+            hipLaunchKernelGGL((k_residual_block<RES_RB, RES_TZ>), dim3(bgrid.x, bgrid.y, (nrhs + RES_RB - 1) / RES_RB),
+                               dim3(256), 0, s, L.nz, L.nx, n, nrhs, L.sx64, L.sz64, L.isxe64, L.isze64, L.eta64,
+                               L.kap64, k0sq, 0.5 / (L.h_m * L.h_m), (int)adj, (const double2*)x, b, W.V, W.partial);
+        } else if (v0raw) {
             hipLaunchKernelGGL(k_residual_v0, dim3(rgrid.x, rgrid.y, (nrhs + rrb - 1) / rrb), dim3(256), 0, s, L.nz,
                                L.nx, n, nrhs, rrb, L.sx64, L.sz64, L.isxe64, L.isze64, L.eta64, L.kap64, k0sq,
                                0.5 / (L.h_m * L.h_m), (int)adj, (const double2*)x, b, W.V, W.partial);
@@ -983,7 +1148,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         applications++;
         if (!v0raw)
             hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
-        hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, v0raw ? nchr : nch, mm, o->tolerance, W.partial,
+        hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, v0raw ? nchr_used : nch, mm, o->tolerance, W.partial,
                            W.bnorm, W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, W.dstat_rel,
                            W.dstat_active, W.dstat_iters, (int)v0raw);
         HELM_HIP(hipGetLastError());

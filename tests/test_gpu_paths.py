@@ -2,8 +2,8 @@
 
 Each fast path of the library keeps its predecessor behind an environment
 switch that is read once per process (HELM_DCGS_FUSED, HELM_LAZY_BASIS,
-HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS), so a test process only ever runs one
-side.  This test solves the golden problem (nsrno/theranostic/wave.py
+HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS), so a test process
+only ever runs one side.  This test solves the golden problem (nsrno/theranostic/wave.py
 Solver.solve, fields from tests/golden/theranostic_small.npz) in two child
 processes, one with every switch off and one with the defaults, and checks
 both against the reference's direct-solve fields.
@@ -43,6 +43,8 @@ np.save(sys.argv[3], x.cpu().numpy().reshape(4, -1).T)
 '''
 
 SWITCHES = {"HELM_DCGS_FUSED": "0", "HELM_LAZY_BASIS": "0", "HELM_STENCIL_BLOCK": "0", "HELM_MG_TAIL_LDS": "0"}
+# the per-node restart residual (the lazy basis stays on, so that kernel runs)
+SWITCHES_RESID = {"HELM_RESID_BLOCK": "0"}
 
 
 def rel(a, b):
@@ -52,9 +54,10 @@ def rel(a, b):
 def test_switched_off_paths_match_default_and_reference(tmp_path):
     golden = np.load(GOLDEN_PATH)
     fields = {}
-    defaults = {k: v for k, v in os.environ.items() if k not in SWITCHES}
-    for name, env in (("alternate", dict(defaults, **SWITCHES)), ("default", defaults)):
-        out = tmp_path / f"{name}.npy"
+    defaults = {k: v for k, v in os.environ.items() if k not in SWITCHES and k not in SWITCHES_RESID}
+    for name, env in (("alternate", dict(defaults, **SWITCHES)), ("per-node residual", dict(defaults, **SWITCHES_RESID)),
+                      ("default", defaults)):
+        out = tmp_path / f"{name.replace(' ', '_')}.npy"
         print(f"child solve: {name} paths", flush=True)
         subprocess.run([sys.executable, "-c", SOLVE, str(ROOT), str(GOLDEN_PATH), str(out)], env=env, check=True,
                        timeout=240)
@@ -62,3 +65,4 @@ def test_switched_off_paths_match_default_and_reference(tmp_path):
     for name, f in fields.items():
         assert rel(f, golden["fields"]) < 1e-7, name
     assert rel(fields["alternate"], fields["default"]) < 1e-7
+    assert rel(fields["per-node residual"], fields["default"]) < 1e-7

@@ -430,7 +430,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
         const int i = t;
         const bool rin = zin(i);
         // x: stage row i (and the halo columns) in LDS
-        __syncthreads();  // previous row's LDS readers are done
+        lds_barrier();  // previous row's LDS readers are done
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             const float2 v = U[q][0];  // U[q][k] holds row t + k
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         const float axa = ax[(int64_t)i * W + fa];
         const float axb = l < 7 ? ax[(int64_t)i * W + fb] : 0.f;
 #pragma unroll
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
                 if (!ADJ && pmlx) sg[q][f] = cm(pass ? gB : gA, F);
             }
         }
-        __syncthreads();
+        lds_barrier();
         // z PML terms of the forward output: g - 1 of faces i-4 .. i+3
         float2 gz[8];
         bool pzo = false;

@@ -59,6 +59,18 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = tr
     bz = t / (gx * gy);
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() fences global
+// memory too, which on this target waits for every outstanding vector-memory
+// load (vmcnt(0)) and so drains a prefetch issued before the barrier; where a
+// barrier only separates LDS writes from LDS reads, this one keeps the
+// prefetch in flight.  Not for kernels whose threads communicate through
+// global memory.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename R> struct cplx;
 template <> struct cplx<float> { using type = float2; };
 template <> struct cplx<double> { using type = double2; };

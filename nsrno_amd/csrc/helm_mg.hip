@@ -994,7 +994,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
     if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
         if (!PF) fetch(r);
-        __syncthreads();  // the previous RHS is done with the LDS arrays
+        lds_barrier();  // the previous RHS is done with the LDS arrays
 #pragma unroll
         for (int q = 0; q < PR_1PT; ++q) {
             const int e = tid + 256 * q;
@@ -1004,7 +1004,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
             }
         }
         if (PF && r + 1 < r1) fetch(r + 1);
-        __syncthreads();
+        lds_barrier();
         // u2 = u1 + (w/dg) (f - M u1) on the u2 region
         float2 v2[PR_2PT];
 #pragma unroll
@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
             }
             if (e < PR_2N) s2[e] = v2[q];
         }
-        __syncthreads();
+        lds_barrier();
         // residual f - M u2 on the 17 x 65 region (into s1) and the owned u2 outputs
 #pragma unroll
         for (int q = 0; q < PR_2PT; ++q) {
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
             if (i2[q] >= 0 && gi >= 2 * I0 && gi < 2 * I0 + 2 * RR_TZ && gj >= 2 * J0 && gj < 2 * J0 + 2 * RR_TX)
                 u[(int64_t)r * nf + i2[q]] = v2[q];
         }
-        __syncthreads();
+        lds_barrier();
         const int I = I0 + threadIdx.y, J = J0 + threadIdx.x;
         if (I < nzc && J < nxc) {
             const float2* base = s1 + (2 * threadIdx.y + 1) * PR_RX + 2 * threadIdx.x + 1;
@@ -1150,13 +1150,13 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
     if (PF) fetch(r0);
     for (int r = r0; r < r1; ++r) {
         if (!PF) fetch(r);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int q = 0; q < MP_CPT; ++q) {
             const int e = tid + 256 * q;
             if (e < MP_NC) scv[e] = cs[q];
         }
-        __syncthreads();
+        lds_barrier();
         // v = u + P u_c on the halo-2 block (bilinear weights without branches)
 #pragma unroll
         for (int q = 0; q < P2_PT; ++q) {
@@ -1180,7 +1180,7 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) fc[q] = fv[q];
         if (PF && r + 1 < r1) fetch(r + 1);
-        __syncthreads();
+        lds_barrier();
         // first sweep on the halo-1 block
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) {
@@ -1198,7 +1198,7 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
             }
             if (e < MT_NS) sv1[e] = v1;
         }
-        __syncthreads();
+        lds_barrier();
         // second sweep on the tile
 #pragma unroll
         for (int q = 0; q < MT_SPT; ++q) {

@@ -199,7 +199,9 @@ def stencil_kernel_name(tuning):
         return "k_apply_stream<4> (FP32 stencil, default 8-row runs)"
     rows, rhs, blk = tuning["stencil_rows"], tuning["stencil_rhs"], tuning.get("stencil_block", 0)
     if blk:
-        order = ", XCD-contiguous order" if blk == 2 else ""
+        order = ", XCD-contiguous order" if blk in (2, 4) else ""
+        if blk >= 3:
+            return f"k_apply_pair<{rhs},{rows}> (FP32 stencil, two columns per lane, {rows}-row runs{order})"
         return f"k_apply_block<{rhs},{rows}> (FP32 stencil, {rows}-row runs loaded up front{order})"
     return f"k_apply_stream<{rhs}> (FP32 stencil, {rows}-row runs)"
 
@@ -251,6 +253,7 @@ def pmc_summary(directory):
              "k_dcgs_norm_scale": "k_dcgs_norm_scale", "k_dcgs_update": "k_dcgs_update", "k_tail_lds": "k_tail_lds",
              "k_apply<float": "k_apply<float>",
              "k_apply_stream": "k_apply_stream<float>", "k_apply_block": "k_apply_block<float>",
+             "k_apply_pair": "k_apply_pair<float>",
              "k_mgt_pre2_rr": "k_mgt_pre2_rr", "k_mgt_post2": "k_mgt_post2", "k_mgt_jacobi": "k_mgt_jacobi",
              "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
              "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail(": "k_tail", "k_cond8s": "k_cond8s<float>"}

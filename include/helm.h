@@ -341,15 +341,20 @@ typedef struct {
     double stencil_us;         /* the chosen stencil's time per apply        */
     double vcycle_us;          /* the chosen V-cycle's time                  */
     int32_t stencil_block;     /* 1: the block form (whole run loaded first),
-                                  2: the same in XCD-contiguous order        */
+                                  2: the same in XCD-contiguous order,
+                                  3: the paired block form (two columns per
+                                  lane), 4: the same in XCD-contiguous order */
 } helm_tuning;
 int32_t helm_autotune(helm_plan* plan, int32_t nrhs, helm_tuning* result, void* stream);
 
 /* ---- fixed launch shape of the FP32 flux stencil: no counterpart in the
  * reference.  form 0: streaming (rows 2..1024 per run), 1: block (rows 2, 4,
- * 8 or 16 loaded up front), 2: block in XCD-contiguous workgroup order; rhs
- * 1, 2 or 4 per group.  rows = 0 returns to the built-in default.  For tests
- * and for callers that tuned once and reuse the result. */
+ * 8 or 16 loaded up front), 2: block in XCD-contiguous workgroup order, 3:
+ * paired block (two columns per lane; rows 2, 4 or 8; the block form where
+ * the grid has an odd row length or the fields are not 16-byte aligned), 4:
+ * paired block in XCD-contiguous order; rhs 1, 2 or 4 per group.  rows = 0
+ * returns to the built-in default.  For tests and for callers that tuned
+ * once and reuse the result. */
 int32_t helm_set_stencil_shape(helm_plan* plan, int32_t rows, int32_t rhs, int32_t form);
 
 /* ---- in-solve kernel timing: no counterpart in the reference (its path

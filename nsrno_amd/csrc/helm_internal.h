@@ -138,6 +138,7 @@ struct Level {
     int st_tz = 0, st_rb = 0;
     bool st_block = false;  // FP32 flux stencil: block form (whole run loaded up front)
     bool st_xcd = false;    // block form in XCD-contiguous workgroup order
+    bool st_pair = false;   // paired block form (two columns per lane)
 };
 
 struct Workspace {

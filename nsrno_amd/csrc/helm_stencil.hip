@@ -292,6 +292,141 @@ __global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, 
     }
 }
 
+// Paired form of the block kernel (nx even): each lane owns two adjacent
+// columns, so field and kappa rows move as 16-byte vectors, eta as 8-byte
+// ones, and a wavefront covers 128 columns; the face between a lane's two
+// columns is formed once, and only the outer neighbours (west of the first,
+// east of the second column) come from the adjacent lanes.  The arithmetic
+// per node, and its order, are those of k_apply_block.
+template <int RB, int TZ>
+__global__ __launch_bounds__(256) void k_apply_pair(int nz, int nx, int64_t n, int nrhs,
+                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                    const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                    const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                    float k0sq, float hh, float shift, int conj,
+                                                    const float2* __restrict__ u, float2* __restrict__ out, int xcd) {
+    static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
+    int bx, by, bz;
+    xcd_tile(bx, by, bz, (xcd & 1) != 0);
+    const int lane = threadIdx.x;
+    const int x0 = bx * 128, c0 = x0 + 2 * lane;  // columns c0, c0 + 1
+    const int i0 = __builtin_amdgcn_readfirstlane((by * 4 + threadIdx.y) * TZ);
+    if (i0 >= nz) return;
+    const int r0 = bz * RB;
+    const bool colok = c0 < nx;  // nx is even: both columns or neither
+    const int cc = colok ? c0 : nx - 2;  // lanes past the grid mirror the last pair (results dropped)
+    const bool hw = cc > 0, he = cc + 1 < nx - 1;  // west of c0, east of c0 + 1
+    const bool wstrip = x0 > 0, estrip = x0 + 128 < nx;
+    float2 SZ[TZ], IZ[TZ + 1];
+#pragma unroll
+    for (int t = 0; t < TZ; ++t) SZ[t] = sz[min(i0 + t, nz - 1)];
+#pragma unroll
+    for (int t = 0; t <= TZ; ++t) IZ[t] = isze[min(i0 + t, nz)];
+    const float4* u4 = reinterpret_cast<const float4*>(u);
+    const float4* k4 = reinterpret_cast<const float4*>(kap);
+    const float2* e2 = reinterpret_cast<const float2*>(eta);
+    float4 U[RB][TZ + 2];
+    float2 E[TZ + 2];
+    float4 K[TZ];
+#pragma unroll
+    for (int t = 0; t < TZ + 2; ++t) {
+        const int i = i0 - 1 + t;
+        const bool rok = i >= 0 && i < nz;
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+            U[q][t] = (rok && r0 + q < nrhs) ? u4[((int64_t)(r0 + q) * n + (int64_t)i * nx + cc) >> 1]
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+        E[t] = rok ? e2[((int64_t)i * nx + cc) >> 1] : make_float2(1.f, 1.f);  // ghost eta 1
+    }
+#pragma unroll
+    for (int t = 0; t < TZ; ++t)
+        K[t] = i0 + t < nz ? k4[((int64_t)(i0 + t) * nx + cc) >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 HW = make_float2(0.f, 0.f), HE = make_float2(0.f, 0.f);
+    float EH = 1.f;  // lanes 0..TZ-1: west eta of row t; lanes 32..32+TZ-1: east eta
+    {
+        const int q = lane / TZ, t = lane % TZ, i = i0 + t;
+        if (lane < RB * TZ && i < nz && r0 + q < nrhs) {
+            const int64_t o = (int64_t)(r0 + q) * n + (int64_t)i * nx;
+            if (wstrip) HW = u[o + x0 - 1];
+            if (estrip) HE = u[o + x0 + 128];
+        }
+        if (lane < TZ && i0 + lane < nz && wstrip) EH = eta[(int64_t)(i0 + lane) * nx + x0 - 1];
+        if (lane >= 32 && lane < 32 + TZ && i0 + lane - 32 < nz && estrip)
+            EH = eta[(int64_t)(i0 + lane - 32) * nx + x0 + 128];
+    }
+    const float2 sx0 = sx[cc], sx1 = sx[cc + 1], iw = isxe[cc], im = isxe[cc + 1], ie = isxe[cc + 2];
+    const bool cj = conj != 0;
+#pragma unroll
+    for (int t = 0; t < TZ; ++t) {
+        const int i = i0 + t;
+        const float2 en = E[t], ec = E[t + 1], es = E[t + 2];
+        float eww = __shfl_up(ec.y, 1, 64), eee = __shfl_down(ec.x, 1, 64);
+        const float ehw = readlane_f(EH, t), ehe = readlane_f(EH, 32 + t);
+        if (lane == 0) eww = ehw;
+        if (lane == 63) eee = ehe;
+        if (!hw) eww = 1.f;
+        if (!he) eee = 1.f;
+        const float2 szi = SZ[t], in_ = IZ[t], is_ = IZ[t + 1];
+        // the two nodes' stencils, as node_coef forms them (the middle face once)
+        const float2 fmid = cscale(cmul(szi, im), hh * (ec.x + ec.y));
+        Coef<float2> a, b;
+        a.fw = cscale(cmul(szi, iw), hh * (ec.x + eww));
+        a.fe = fmid;
+        a.fn = cscale(cmul(sx0, in_), hh * (ec.x + en.x));
+        a.fs = cscale(cmul(sx0, is_), hh * (ec.x + es.x));
+        b.fw = fmid;
+        b.fe = cscale(cmul(szi, ie), hh * (ec.y + eee));
+        b.fn = cscale(cmul(sx1, in_), hh * (ec.y + en.y));
+        b.fs = cscale(cmul(sx1, is_), hh * (ec.y + es.y));
+        float2 ma = cscale(cmul(cmul(szi, sx0), make_float2(K[t].x, K[t].y)), k0sq);
+        float2 mb = cscale(cmul(cmul(szi, sx1), make_float2(K[t].z, K[t].w)), k0sq);
+        if (shift != 0.f) {
+            ma = make_float2(ma.x - shift * ma.y, ma.y + shift * ma.x);
+            mb = make_float2(mb.x - shift * mb.y, mb.y + shift * mb.x);
+        }
+        a.dg = csub(ma, cadd(cadd(a.fw, a.fe), cadd(a.fn, a.fs)));
+        b.dg = csub(mb, cadd(cadd(b.fw, b.fe), cadd(b.fn, b.fs)));
+        if (cj) {
+            a.dg = cconj(a.dg);
+            a.fw = cconj(a.fw);
+            a.fe = cconj(a.fe);
+            a.fn = cconj(a.fn);
+            a.fs = cconj(a.fs);
+            b.dg = cconj(b.dg);
+            b.fw = cconj(b.fw);
+            b.fe = cconj(b.fe);
+            b.fn = cconj(b.fn);
+            b.fs = cconj(b.fs);
+        }
+        const bool hn = i > 0, hs = i < nz - 1;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const float4 uc = U[q][t + 1], un = U[q][t], us = U[q][t + 2];
+            const float2 u0 = make_float2(uc.x, uc.y), u1 = make_float2(uc.z, uc.w);
+            float2 w = make_float2(__shfl_up(uc.z, 1, 64), __shfl_up(uc.w, 1, 64));
+            float2 e = make_float2(__shfl_down(uc.x, 1, 64), __shfl_down(uc.y, 1, 64));
+            const float2 hwv = make_float2(readlane_f(HW.x, q * TZ + t), readlane_f(HW.y, q * TZ + t));
+            const float2 hev = make_float2(readlane_f(HE.x, q * TZ + t), readlane_f(HE.y, q * TZ + t));
+            if (lane == 0) w = hwv;
+            if (lane == 63) e = hev;
+            float2 acc0 = cmul(a.dg, u0);
+            if (hw) cfma(acc0, a.fw, w);
+            cfma(acc0, a.fe, u1);
+            if (hn) cfma(acc0, a.fn, make_float2(un.x, un.y));
+            if (hs) cfma(acc0, a.fs, make_float2(us.x, us.y));
+            float2 acc1 = cmul(b.dg, u1);
+            cfma(acc1, b.fw, u0);
+            if (he) cfma(acc1, b.fe, e);
+            if (hn) cfma(acc1, b.fn, make_float2(un.z, un.w));
+            if (hs) cfma(acc1, b.fs, make_float2(us.z, us.w));
+            const int r = r0 + q;
+            if (colok && r < nrhs && i < nz)
+                reinterpret_cast<float4*>(out)[((int64_t)r * n + (int64_t)i * nx + c0) >> 1] =
+                    make_float4(acc0.x, acc0.y, acc1.x, acc1.y);
+        }
+    }
+}
+
 // streaming-kernel shape (rows per run, RHS per group): tuning knobs read
 // once from HELM_STENCIL_TZ and HELM_STENCIL_RB
 static int stream_tz() {
@@ -383,6 +518,31 @@ static void launch_block(const Level& L, int tz, double k0sq, const float2* u, f
         launch_block_tz<RB, 16>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
 }
 
+template <int RB, int TZ>
+static void launch_pair_tz(const Level& L, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                           float shift, hipStream_t s) {
+    const int runs = (L.nz + TZ - 1) / TZ;
+    const dim3 block(64, 4), grid((L.nx + 127) / 128, (runs + 3) / 4, (nrhs + RB - 1) / RB);
+    hipLaunchKernelGGL((k_apply_pair<RB, TZ>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.sx32, L.sz32, L.isxe32,
+                       L.isze32, L.eta32, L.kap32, (float)k0sq, (float)(0.5 / (L.h_m * L.h_m)), shift, (int)adjoint,
+                       u, out, (int)L.st_xcd);
+}
+template <int RB>
+static void launch_pair(const Level& L, int tz, double k0sq, const float2* u, float2* out, int nrhs, bool adjoint,
+                        float shift, hipStream_t s) {
+    if (tz <= 2)
+        launch_pair_tz<RB, 2>(L, k0sq, u, out, nrhs, adjoint, shift, s);
+    else if (tz <= 4)
+        launch_pair_tz<RB, 4>(L, k0sq, u, out, nrhs, adjoint, shift, s);
+    else
+        launch_pair_tz<RB, 8>(L, k0sq, u, out, nrhs, adjoint, shift, s);
+}
+// the paired form needs an even row length and 16-byte aligned fields and kappa
+static bool pair_ok(const Level& L, const float2* u, const float2* out, const float2* b) {
+    auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+    return !b && L.nx % 2 == 0 && L.nx >= 128 && al(u, 16) && al(out, 16) && al(L.kap32, 16) && al(L.eta32, 8);
+}
+
 // right-hand sides per workgroup: enough groups for >= ~2048 workgroups
 static int rhs_per_group(const Level& L, int nrhs) {
     const int64_t bxy = (int64_t)((L.nx + 63) / 64) * ((L.nz + 3) / 4);
@@ -414,7 +574,13 @@ int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out,
                       const float2* b, float shift, hipStream_t s) {
     if (L.nx >= 64) {  // streaming kernel (a strip needs a full wavefront of columns to pay off)
         const int rb = L.st_rb ? L.st_rb : stream_rb();
-        if (L.st_tz ? L.st_block : stream_block()) {
+        if (L.st_tz && L.st_pair && pair_ok(L, u, out, b)) {
+            switch (rb) {
+                case 1: launch_pair<1>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
+                case 2: launch_pair<2>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
+                default: launch_pair<4>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
+            }
+        } else if (L.st_tz ? L.st_block : stream_block()) {
             const int tz = L.st_tz ? L.st_tz : block_tz();
             switch (rb) {
                 case 1: launch_block<1>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
@@ -535,14 +701,18 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     // the flux stencil's block form: runs of 2, 4 or 8 rows loaded up front (16 was never the fastest),
     // in the hardware's workgroup order and in the XCD-contiguous one
     const int nblk = cond ? 0 : 6, tzs_blk[3] = {2, 4, 8};
-    bool bblk = !cond, bxcd = false;
+    // and the paired form (two columns per lane; even rows of >= 128 nodes): runs of 2 or 4 rows
+    const int npair = (!cond && pair_ok(L, u, out, nullptr)) ? 2 : 0, tzs_pair[2] = {2, 4};
+    bool bblk = !cond, bxcd = false, bpair = false;
     const int reps = 10;  // launches timed per candidate
-    for (int a = 0; a < ntz + nblk && st == HELM_OK; ++a)
+    for (int a = 0; a < ntz + nblk + npair && st == HELM_OK; ++a)
         for (int b = 0; b < 3 && st == HELM_OK; ++b) {
-            const bool blk = a >= ntz;
-            L.st_tz = blk ? tzs_blk[(a - ntz) % 3] : tzs[a];
+            const bool pair = a >= ntz + nblk;
+            const bool blk = a >= ntz && !pair;
+            L.st_tz = pair ? tzs_pair[a - ntz - nblk] : (blk ? tzs_blk[(a - ntz) % 3] : tzs[a]);
             L.st_rb = rbs[b];
-            L.st_block = blk;
+            L.st_block = blk || pair;  // the paired form falls back to the block form
+            L.st_pair = pair;
             L.st_xcd = blk && a - ntz >= 3;
             for (int w = 0; w < 3 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e0, s);
@@ -554,14 +724,16 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
                 best = ms / reps;
                 btz = L.st_tz;
                 brb = rbs[b];
-                bblk = blk;
+                bblk = blk || pair;
                 bxcd = L.st_xcd;
+                bpair = pair;
             }
         }
     L.st_tz = btz;
     L.st_rb = brb;
     L.st_block = bblk;
     L.st_xcd = bxcd;
+    L.st_pair = bpair;
     *best_us = best * 1e3;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -715,7 +887,7 @@ extern "C" int32_t helm_autotune(helm_plan* p, int32_t nrhs, helm_tuning* result
         const Level& L = p->levels[0];
         result->stencil_rows = L.st_tz;
         result->stencil_rhs = L.st_rb;
-        result->stencil_block = L.st_block ? (L.st_xcd ? 2 : 1) : 0;
+        result->stencil_block = L.st_pair ? (L.st_xcd ? 4 : 3) : (L.st_block ? (L.st_xcd ? 2 : 1) : 0);
         result->mg_workgroups = p->mg_wg_target;
         result->stencil_us = st_us;
         result->vcycle_us = mg_us;
@@ -728,16 +900,19 @@ extern "C" int32_t helm_set_stencil_shape(helm_plan* p, int32_t rows, int32_t rh
     Level& L = p->levels[0];
     if (rows == 0) {  // back to the built-in defaults
         L.st_tz = L.st_rb = 0;
-        L.st_block = L.st_xcd = false;
+        L.st_block = L.st_xcd = L.st_pair = false;
     } else {
         HELM_CHECK(rhs == 1 || rhs == 2 || rhs == 4, HELM_ERR_ARG, "helm_set_stencil_shape: rhs must be 1, 2 or 4");
-        HELM_CHECK(form >= 0 && form <= 2, HELM_ERR_ARG, "helm_set_stencil_shape: form must be 0, 1 or 2");
-        HELM_CHECK(form == 0 ? rows >= 2 && rows <= 1024 : rows == 2 || rows == 4 || rows == 8 || rows == 16,
-                   HELM_ERR_ARG, "helm_set_stencil_shape: rows out of range for the form");
+        HELM_CHECK(form >= 0 && form <= 4, HELM_ERR_ARG, "helm_set_stencil_shape: form must be 0 .. 4");
+        const bool rows_ok = form == 0 ? rows >= 2 && rows <= 1024
+                             : form <= 2 ? rows == 2 || rows == 4 || rows == 8 || rows == 16
+                                         : rows == 2 || rows == 4 || rows == 8;
+        HELM_CHECK(rows_ok, HELM_ERR_ARG, "helm_set_stencil_shape: rows out of range for the form");
         L.st_tz = rows;
         L.st_rb = rhs;
-        L.st_block = form != 0;
-        L.st_xcd = form == 2;
+        L.st_block = form >= 1;
+        L.st_pair = form >= 3;  // falls back to the block form where the paired one cannot run
+        L.st_xcd = form == 2 || form == 4;
     }
     p->generation++;  // captured cycles hold the old launch shapes
     return HELM_OK;

@@ -182,14 +182,17 @@ def test_born_data_and_jacobian_match_oracle():
 
 
 @pytest.mark.parametrize("shape", [None, (8, 4, 0), (3, 1, 0), (2, 2, 1), (4, 4, 1), (8, 1, 1), (16, 4, 1),
-                                   (4, 2, 2), (16, 1, 2)])
+                                   (4, 2, 2), (16, 1, 2), (2, 2, 3), (4, 4, 3), (8, 1, 4), (2, 4, 4)])
 @pytest.mark.parametrize("nrhs", [1, 5])
-def test_apply_fp32_streaming_shapes(nrhs, shape):
-    """The FP32 stencil's streaming and block forms (rows per run, RHS per
-    group, workgroup order) on a grid that is no multiple of the strip (64
-    columns) or of the row runs, with a RHS count that leaves a partial group."""
+@pytest.mark.parametrize("width", [170, 171])
+def test_apply_fp32_streaming_shapes(nrhs, shape, width):
+    """The FP32 stencil's streaming, block and paired block forms (rows per
+    run, RHS per group, workgroup order) on grids that are no multiple of the
+    strip (64 or 128 columns) or of the row runs, with a RHS count that leaves
+    a partial group; the odd row length sends the paired form to its block
+    fallback, the even one runs it."""
     from nsrno_amd.helmholtz import HelmholtzPlan
-    grid = O.grid_with_pml(170, 131, 12, 0.05)
+    grid = O.grid_with_pml(width, 131, 12, 0.05)  # 194 or 195 columns
     nz, nx = grid.shape
     rng = np.random.default_rng(5)
     eta = 1 + 0.1 * rng.random((nz, nx))

@@ -323,23 +323,23 @@ __global__ __launch_bounds__(256) void k_cond8(int H, int W, int64_t n, int nrhs
 // reached the run, writes output row t.  Forward: the staged field is M u;
 // adjoint: M y, scaled per column (x) and per row (z) by conj(cA), conj(cB)
 // as the differences are formed.
-template <int RB, int MODE, bool ADJ>
+template <typename R, typename C, int RB, int MODE, bool ADJ>
 __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs, int tzr,
-                                               const float* __restrict__ ax, const float* __restrict__ az,
-                                               const float2* __restrict__ mass, const float2* __restrict__ tx,
-                                               const float2* __restrict__ tz, int bz0, int bx0, int bh, int bw,
-                                               float c0, float c1, float c2, float c3,
-                                               const float2* __restrict__ u, float2* __restrict__ out,
-                                               const float2* __restrict__ b) {
+                                               const R* __restrict__ ax, const R* __restrict__ az,
+                                               const C* __restrict__ mass, const C* __restrict__ tx,
+                                               const C* __restrict__ tz, int bz0, int bx0, int bh, int bw,
+                                               R c0, R c1, R c2, R c3,
+                                               const C* __restrict__ u, C* __restrict__ out,
+                                               const C* __restrict__ b) {
     constexpr int NS = 78, NF = 71;
-    __shared__ float2 sa[RB][NS], sb[RB][NS], sf[RB][NF], sg[RB][NF];
+    __shared__ C sa[RB][NS], sb[RB][NS], sf[RB][NF], sg[RB][NF];
     const int l = threadIdx.x;
     const int x0 = blockIdx.x * 64, j = x0 + l;
     const int i0 = blockIdx.y * tzr;
     if (i0 >= H) return;
     const int i1 = min(H, i0 + tzr);
     const int r0 = blockIdx.z * RB;
-    const float2 zero = make_float2(0.f, 0.f);
+    const C zero = C{R(0), R(0)};
     const bool jok = j < W;
     const int jc = cwrap(j, W);
     const bool xin = jc >= bx0 && jc < bx0 + bw;
@@ -348,22 +348,22 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
     const int ch0 = cwrap(x0 - 7 + l, W), ch1 = cwrap(x0 + 64 + l, W);
     const bool hin0 = ch0 >= bx0 && ch0 < bx0 + bw, hin1 = ch1 >= bx0 && ch1 < bx0 + bw;
     const int fa = cwrap(x0 - 4 + l, W), fb = cwrap(x0 + 60 + l, W);
-    const float2 svA = tx[fa], gA = tx[W + fa];
-    const float2 svB = l < 7 ? tx[fb] : zero, gB = l < 7 ? tx[W + fb] : zero;
-    const float2 cAj = tx[2 * W + jc], cBj = tx[3 * W + jc];
-    float2 cA0 = zero, cB0 = zero, cA1 = zero, cB1 = zero;
+    const C svA = tx[fa], gA = tx[W + fa];
+    const C svB = l < 7 ? tx[fb] : zero, gB = l < 7 ? tx[W + fb] : zero;
+    const C cAj = tx[2 * W + jc], cBj = tx[3 * W + jc];
+    C cA0 = zero, cB0 = zero, cA1 = zero, cB1 = zero;
     if (ADJ && l < 7) {
         cA0 = tx[2 * W + ch0];
         cB0 = tx[3 * W + ch0];
         cA1 = tx[2 * W + ch1];
         cB1 = tx[3 * W + ch1];
     }
-    const bool pmlx = __any((gA.x != 0.f || gA.y != 0.f) || (gB.x != 0.f || gB.y != 0.f));
+    const bool pmlx = __any((gA.x != R(0) || gA.y != R(0)) || (gB.x != R(0) || gB.y != R(0)));
 
-    float2 U[RB][8], G[RB][8];
-    auto ldu = [=](int q, int row, int col, bool colin) -> float2 {
+    C U[RB][8], G[RB][8];
+    auto ldu = [=](int q, int row, int col, bool colin) -> C {
         const int r = r0 + q;
-        if (r >= nrhs || !colin || row < bz0 || row >= bz0 + bh) return make_float2(0.f, 0.f);
+        if (r >= nrhs || !colin || row < bz0 || row >= bz0 + bh) return C{R(0), R(0)};
         return u[(int64_t)r * n + (int64_t)row * W + col];
     };
     // initial window: rows i0-8 .. i0-1
@@ -387,12 +387,12 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
         }
         // z face t+3: rows t .. t+7 are U[.][0..7]
         {
-            const float2 sv = tz[fz], gm = tz[H + fz];
-            const float a = az[(int64_t)fz * W + jc];
-            float2 cz[8], czb[8];
+            const C sv = tz[fz], gm = tz[H + fz];
+            const R a = az[(int64_t)fz * W + jc];
+            C cz[8], czb[8];
             bool pz = false;
             if (ADJ) {
-                pz = gm.x != 0.f || gm.y != 0.f;
+                pz = gm.x != R(0) || gm.y != R(0);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const int row = cwrap(t + k, H);
@@ -402,17 +402,17 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
             }
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
-                float2 d;
+                C d;
                 if (!ADJ) {
-                    d = c8diff<float2, float>(c0, c1, c2, c3, U[q][4], U[q][3], U[q][5], U[q][2], U[q][6], U[q][1],
+                    d = c8diff<C, R>(c0, c1, c2, c3, U[q][4], U[q][3], U[q][5], U[q][2], U[q][6], U[q][1],
                                               U[q][7], U[q][0]);
                     d = cscale(cm(sv, d), a);
                 } else {
-                    d = c8diff<float2, float>(c0, c1, c2, c3, cjm(cz[4], U[q][4]), cjm(cz[3], U[q][3]),
+                    d = c8diff<C, R>(c0, c1, c2, c3, cjm(cz[4], U[q][4]), cjm(cz[3], U[q][3]),
                                               cjm(cz[5], U[q][5]), cjm(cz[2], U[q][2]), cjm(cz[6], U[q][6]),
                                               cjm(cz[1], U[q][1]), cjm(cz[7], U[q][7]), cjm(cz[0], U[q][0]));
                     if (pz) {
-                        const float2 d2 = c8diff<float2, float>(
+                        const C d2 = c8diff<C, R>(
                             c0, c1, c2, c3, cjm(czb[4], U[q][4]), cjm(czb[3], U[q][3]), cjm(czb[5], U[q][5]),
                             cjm(czb[2], U[q][2]), cjm(czb[6], U[q][6]), cjm(czb[1], U[q][1]), cjm(czb[7], U[q][7]),
                             cjm(czb[0], U[q][0]));
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
         lds_barrier();  // previous row's LDS readers are done
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
-            const float2 v = U[q][0];  // U[q][k] holds row t + k
+            const C v = U[q][0];  // U[q][k] holds row t + k
             if (!ADJ) {
                 sa[q][l + 7] = v;
             } else {
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
                 if (pmlx) sb[q][l + 7] = cjm(cBj, v);
             }
             if (l < 7) {
-                const float2 h0 = ldu(q, i, ch0, hin0), h1 = ldu(q, i, ch1, hin1);
+                const C h0 = ldu(q, i, ch0, hin0), h1 = ldu(q, i, ch1, hin1);
                 if (!ADJ) {
                     sa[q][l] = h0;
                     sa[q][71 + l] = h1;
@@ -456,44 +456,44 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
             }
         }
         lds_barrier();
-        const float axa = ax[(int64_t)i * W + fa];
-        const float axb = l < 7 ? ax[(int64_t)i * W + fb] : 0.f;
+        const R axa = ax[(int64_t)i * W + fa];
+        const R axb = l < 7 ? ax[(int64_t)i * W + fb] : R(0);
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
 #pragma unroll
             for (int pass = 0; pass < 2; ++pass) {
                 if (pass == 1 && l >= 7) continue;
                 const int f = pass ? 64 + l : l;  // face index in the strip window (0 .. 70 = faces -4 .. 66)
-                const float2* s = sa[q] + f + 3;  // s[m] = node f - 4 + m  (node index - (x0-7) = f + 3 + m)
+                const C* s = sa[q] + f + 3;  // s[m] = node f - 4 + m  (node index - (x0-7) = f + 3 + m)
                 // D+ at face f-4: nodes f-4+m+1 - (f-4-m) -> s[m+1] - s[-m] with s at node f-4
-                float2 d = c8diff<float2, float>(c0, c1, c2, c3, s[1], s[0], s[2], s[-1], s[3], s[-2], s[4], s[-3]);
-                const float2 sv = pass ? svB : svA;
-                const float a = pass ? axb : axa;
+                C d = c8diff<C, R>(c0, c1, c2, c3, s[1], s[0], s[2], s[-1], s[3], s[-2], s[4], s[-3]);
+                const C sv = pass ? svB : svA;
+                const R a = pass ? axb : axa;
                 if (ADJ && pmlx) {
-                    const float2* sB = sb[q] + f + 3;
-                    const float2 d2 = c8diff<float2, float>(c0, c1, c2, c3, sB[1], sB[0], sB[2], sB[-1], sB[3], sB[-2],
+                    const C* sB = sb[q] + f + 3;
+                    const C d2 = c8diff<C, R>(c0, c1, c2, c3, sB[1], sB[0], sB[2], sB[-1], sB[3], sB[-2],
                                                             sB[4], sB[-3]);
                     d = cadd(d, cjm(pass ? gB : gA, d2));
                 }
-                const float2 F = ADJ ? cscale(cjm(sv, d), a) : cscale(cm(sv, d), a);
+                const C F = ADJ ? cscale(cjm(sv, d), a) : cscale(cm(sv, d), a);
                 sf[q][f] = F;
                 if (!ADJ && pmlx) sg[q][f] = cm(pass ? gB : gA, F);
             }
         }
         lds_barrier();
         // z PML terms of the forward output: g - 1 of faces i-4 .. i+3
-        float2 gz[8];
+        C gz[8];
         bool pzo = false;
         if (!ADJ) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 gz[k] = tz[H + cwrap(i - 4 + k, H)];
-                pzo = pzo || gz[k].x != 0.f || gz[k].y != 0.f;
+                pzo = pzo || gz[k].x != R(0) || gz[k].y != R(0);
             }
         }
-        const float2 cAz = tz[2 * H + cwrap(i, H)], cBz = tz[3 * H + cwrap(i, H)];
+        const C cAz = tz[2 * H + cwrap(i, H)], cBz = tz[3 * H + cwrap(i, H)];
         const bool inside = jok && xin && rin;
-        float2 mkc = zero;
+        C mkc = zero;
         if (inside) mkc = mass[(int64_t)i * W + jc];
         if (ADJ) mkc.y = -mkc.y;
 #pragma unroll
@@ -501,24 +501,24 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
             const int r = r0 + q;
             if (!jok || r >= nrhs) continue;
             const int64_t o = (int64_t)r * n + (int64_t)i * W + j;
-            float2 acc = zero;
+            C acc = zero;
             if (inside) {
-                const float2* fx = sf[q] + l + 4;  // fx[m] = face l + m
-                const float2 dx = c8diff<float2, float>(c0, c1, c2, c3, fx[0], fx[-1], fx[1], fx[-2], fx[2], fx[-3],
+                const C* fx = sf[q] + l + 4;  // fx[m] = face l + m
+                const C dx = c8diff<C, R>(c0, c1, c2, c3, fx[0], fx[-1], fx[1], fx[-2], fx[2], fx[-3],
                                                         fx[3], fx[-4]);
-                const float2 dz = c8diff<float2, float>(c0, c1, c2, c3, G[q][4], G[q][3], G[q][5], G[q][2], G[q][6],
+                const C dz = c8diff<C, R>(c0, c1, c2, c3, G[q][4], G[q][3], G[q][5], G[q][2], G[q][6],
                                                         G[q][1], G[q][7], G[q][0]);
                 if (!ADJ) {
                     acc = cm(cAj, dx);
                     cfma(acc, cAz, dz);
                     if (pmlx) {
-                        const float2* gx = sg[q] + l + 4;
-                        const float2 d2 = c8diff<float2, float>(c0, c1, c2, c3, gx[0], gx[-1], gx[1], gx[-2], gx[2],
+                        const C* gx = sg[q] + l + 4;
+                        const C d2 = c8diff<C, R>(c0, c1, c2, c3, gx[0], gx[-1], gx[1], gx[-2], gx[2],
                                                                 gx[-3], gx[3], gx[-4]);
                         cfma(acc, cBj, d2);
                     }
                     if (pzo) {
-                        const float2 d2 = c8diff<float2, float>(
+                        const C d2 = c8diff<C, R>(
                             c0, c1, c2, c3, cm(gz[4], G[q][4]), cm(gz[3], G[q][3]), cm(gz[5], G[q][5]),
                             cm(gz[2], G[q][2]), cm(gz[6], G[q][6]), cm(gz[1], G[q][1]), cm(gz[7], G[q][7]),
                             cm(gz[0], G[q][0]));
@@ -564,8 +564,9 @@ static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs
     const float2* tx = L.ctab32;
     const float2* tz = L.ctab32 + 4 * L.nx;
 #define HELM_C8S(M, A, BP)                                                                                       \
-    hipLaunchKernelGGL((k_cond8s<RB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr, L.ax32, L.az32,     \
-                       L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3, u, out, BP)
+    hipLaunchKernelGGL((k_cond8s<float, float2, RB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr,      \
+                       L.ax32, L.az32, L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
+                       c3, u, out, BP)
     if (b) {
         if (adjoint) HELM_C8S(1, true, b);
         else HELM_C8S(1, false, b);
@@ -574,6 +575,37 @@ static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs
         else HELM_C8S(0, false, (const float2*)nullptr);
     }
 #undef HELM_C8S
+}
+
+// FP64 (true residuals of the refinement): the streaming kernel with one RHS
+// per wavefront and 32-row runs (HELM_COND64_TILED=1: the LDS-tile kernel)
+static bool cond64_tiled() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_COND64_TILED");
+        return v && std::atoi(v) == 1;
+    }();
+    return t;
+}
+static void launch_cond8s64(const Level& L, const double2* u, double2* out, int nrhs, bool adjoint,
+                            const double2* b, hipStream_t s) {
+    constexpr int TZR = 32;
+    const dim3 block(64), grid((L.nx + 63) / 64, (L.nz + TZR - 1) / TZR, nrhs);
+    const double ih = 1.0 / L.h_m;
+    const double c0 = CD0 * ih, c1 = CD1 * ih, c2 = CD2 * ih, c3 = CD3 * ih;
+    const double2* tx = L.ctab64;
+    const double2* tz = L.ctab64 + 4 * L.nx;
+#define HELM_C8S64(M, A, BP)                                                                                     \
+    hipLaunchKernelGGL((k_cond8s<double, double2, 1, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, TZR,     \
+                       L.ax64, L.az64, L.cmass64, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
+                       c3, u, out, BP)
+    if (b) {
+        if (adjoint) HELM_C8S64(1, true, b);
+        else HELM_C8S64(1, false, b);
+    } else {
+        if (adjoint) HELM_C8S64(0, true, (const double2*)nullptr);
+        else HELM_C8S64(0, false, (const double2*)nullptr);
+    }
+#undef HELM_C8S64
 }
 
 template <int TZ>
@@ -622,6 +654,11 @@ int32_t launch_cond8_64(const Level& L, const double2* u, double2* out, int nrhs
                         hipStream_t s) {
     constexpr int TZ = 8;
     HELM_CHECK(L.ax64 && L.az64 && L.cmass64 && L.ctab64, HELM_ERR_STATE, "conditioned operator: medium not set");
+    if (!cond64_tiled()) {
+        launch_cond8s64(L, u, out, nrhs, adjoint, b, s);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const int rb = cond8_rhs_per_group<TZ>(L, nrhs);
     const dim3 block(64, 4), grid((L.nx + C8_TX - 1) / C8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
     const double ih = 1.0 / L.h_m;

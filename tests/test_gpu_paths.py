@@ -2,11 +2,13 @@
 
 Each fast path of the library keeps its predecessor behind an environment
 switch that is read once per process (HELM_DCGS_FUSED, HELM_LAZY_BASIS,
-HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS), so a test process
-only ever runs one side.  This test solves the golden problem (nsrno/theranostic/wave.py
-Solver.solve, fields from tests/golden/theranostic_small.npz) in two child
-processes, one with every switch off and one with the defaults, and checks
-both against the reference's direct-solve fields.
+HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS, HELM_COND64_TILED),
+so a test process only ever runs one side.  The tests below run child
+processes with the switches off and with the defaults and check both against
+the reference's golden outputs: the direct-solve fields of
+nsrno/theranostic/wave.py Solver.solve (tests/golden/theranostic_small.npz)
+and the conditioned operator of nsrno/dense_wave.py projected_operator /
+adjoint_operator in FP64 (tests/golden/conditioned_small.npz).
 """
 import os
 import subprocess
@@ -66,3 +68,36 @@ def test_switched_off_paths_match_default_and_reference(tmp_path):
         assert rel(f, golden["fields"]) < 1e-7, name
     assert rel(fields["alternate"], fields["default"]) < 1e-7
     assert rel(fields["per-node residual"], fields["default"]) < 1e-7
+
+
+COND_GOLDEN = Path(__file__).parent / "golden" / "conditioned_small.npz"
+
+# FP64 projected and adjoint applies of the conditioned operator on the golden input
+COND_APPLY = r'''
+import json, sys
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+from nsrno_amd.conditioned import ConditionedFrequencySystem, FrequencyGeometry, adjoint_operator, projected_operator
+
+g = dict(np.load(sys.argv[2]))
+geo = FrequencyGeometry(json.loads(str(g["spec_json"])))
+medium = torch.tensor(np.stack((g["c"], g["rho"])), dtype=torch.float64, device="cuda")
+s = ConditionedFrequencySystem(geo, medium, torch.tensor(g["alpha"], dtype=torch.float64, device="cuda"),
+                               float(g["f"]))
+p = torch.from_numpy(g["p"]).cuda()
+np.savez(sys.argv[3], proj=projected_operator(s, p).cpu().numpy(), adj=adjoint_operator(s, p).cpu().numpy())
+'''
+
+
+def test_conditioned_fp64_tile_and_streaming_match_reference(tmp_path):
+    golden = np.load(COND_GOLDEN)
+    base = {k: v for k, v in os.environ.items() if k != "HELM_COND64_TILED"}
+    for name, env in (("tiled", dict(base, HELM_COND64_TILED="1")), ("streaming", base)):
+        out = tmp_path / f"{name}.npz"
+        print(f"child apply: {name} FP64 conditioned kernel", flush=True)
+        subprocess.run([sys.executable, "-c", COND_APPLY, str(ROOT), str(COND_GOLDEN), str(out)], env=env,
+                       check=True, timeout=240)
+        got = np.load(out)
+        assert rel(got["proj"], golden["proj"]) < 1e-12, name
+        assert rel(got["adj"], golden["adj"]) < 1e-12, name

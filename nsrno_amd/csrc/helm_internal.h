@@ -40,23 +40,32 @@ __host__ __device__ inline void cfma(double2& a, double2 b, double2 c) {
 // turn (workgroup b to XCD b mod 8), each XCD with its own L2.  Remapping the
 // linear workgroup id so that each XCD works on one contiguous run of tiles
 // keeps the halos that neighbouring tiles share in one L2.  A bijection of
-// [0, total): XCD k takes ids [start_k, start_k + count_k).
-__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = true) {
+// [0, total): XCD k takes ids [start_k, start_k + count_k).  With rhs_fast
+// the z index (the right-hand-side group) runs fastest in that order, so the
+// groups of one tile share an XCD and read the tile's stencil coefficients
+// and halos through one L2; otherwise it runs slowest.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = true, bool rhs_fast = false) {
     if (!on) {
         bx = blockIdx.x;
         by = blockIdx.y;
         bz = blockIdx.z;
         return;
     }
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int total = gx * gy * gridDim.z;
+    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const int total = gx * gy * gz;
     const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const int q = total / 8, r = total % 8;
     const int k = b % 8, slot = b / 8;
     const int t = k < r ? k * (q + 1) + slot : r * (q + 1) + (k - r) * q + slot;
-    bx = t % gx;
-    by = (t / gx) % gy;
-    bz = t / (gx * gy);
+    if (rhs_fast) {
+        bz = t % gz;
+        bx = (t / gz) % gx;
+        by = t / (gz * gx);
+    } else {
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
 }
 
 // Workgroup barrier that orders LDS only.  __syncthreads() fences global
@@ -180,11 +189,15 @@ struct Workspace {
     std::vector<int> h_active;
     std::vector<int> h_iters;
     // restart status written by the device straight into pinned, mapped host
-    // memory (rel, active, iters per RHS): no copies per restart
+    // memory (rel, active, iters per RHS): no copies per restart.  Two slots of
+    // stat_bytes, used in turn, so the status of the cycle queued ahead does
+    // not overwrite the one the host is reading
     void* hstat = nullptr;      // host block (hipHostMalloc)
-    double* dstat_rel = nullptr;  // its device-side addresses
+    double* dstat_rel = nullptr;  // its device-side addresses (slot 0)
     int* dstat_active = nullptr;
     int* dstat_iters = nullptr;
+    size_t stat_bytes = 0;
+    hipEvent_t stat_ev[2] = {nullptr, nullptr};  // recorded after each slot's k_cycle_init
 };
 
 // HIP-event timing of the kernels inside a solve (helm_profile)
@@ -211,6 +224,9 @@ struct Profiler {
 hipEvent_t prof_begin(Profiler* pf, int id, hipStream_t s);  // nullptr when id is not timed
 void prof_end(Profiler* pf, int id, hipEvent_t a, double bytes, hipStream_t s);
 void prof_flush(Profiler* pf);  // after the stream has been synchronised
+// the first `count` pending records only (work known to have completed);
+// keep = false drops them instead of adding them to the totals
+void prof_flush_first(Profiler* pf, size_t count, bool keep = true);
 // event pair for a kernel launched with hipExtLaunchKernelGGL (the dispatch
 // itself records the events, so no marker packets are added); false when id
 // is not timed
@@ -287,9 +303,12 @@ int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool ad
                      const double* fscale = nullptr);
 // multigrid (helm_mg.hip)
 // fscale (nullable, per RHS): the V-cycle solves for fscale[r] * v[r] (applied as the
-// finest level's kernels load v); only where mg_scaled_input_ok holds
+// finest level's kernels load v); only where mg_scaled_input_ok holds.
+// active (nullable, per RHS, device): workgroups of the fused level kernels and
+// the tail whose RHS are all 0 there return at once, leaving z of those RHS
+// unspecified (for RHS whose Krylov cycle has stopped)
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
-                  const double* fscale = nullptr);
+                  const double* fscale = nullptr, const int* active = nullptr);
 bool mg_scaled_input_ok(const helm_plan* p);  // the finest level runs the fused kernels
 int32_t mg_build(helm_plan* p, hipStream_t s);
 // krylov (helm_krylov.hip)

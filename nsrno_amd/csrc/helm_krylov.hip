@@ -124,6 +124,14 @@ static bool lazy_basis() {
     }();
     return t;
 }
+// next restart cycle queued before the host reads the status (HELM_RESTART_AHEAD=0: after)
+static bool restart_ahead() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_RESTART_AHEAD");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
 // fused reduce+update and norm+scale launches (HELM_DCGS_FUSED=0: the separate kernels)
 static bool dcgs_fused() {
     static const bool t = [] {
@@ -940,8 +948,10 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     w.h_iters.assign(nrhs, 0);
     if (w.hstat) (void)hipHostFree(w.hstat);
     w.hstat = nullptr;
-    HELM_HIP(hipHostMalloc(&w.hstat, (size_t)nrhs * (sizeof(double) + 2 * sizeof(int)),
-                           hipHostMallocMapped | hipHostMallocCoherent));
+    w.stat_bytes = (size_t)nrhs * (sizeof(double) + 2 * sizeof(int));
+    HELM_HIP(hipHostMalloc(&w.hstat, 2 * w.stat_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    for (hipEvent_t& e : w.stat_ev)
+        if (!e) HELM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     {
         char* base = nullptr;
         HELM_HIP(hipHostGetDevicePointer((void**)&base, w.hstat, 0));
@@ -1130,8 +1140,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const dim3 bgrid((unsigned)((L.nx + 63) / 64), (unsigned)((L.nz + 4 * RES_TZ - 1) / (4 * RES_TZ)), 1);
     const int nchb = (int)(bgrid.x * bgrid.y);
     const int nchr_used = resid_block() ? nchb : nchr;
-    for (;;) {
-        // true residual in FP64
+    // restart status into pinned slot `slot`: the FP64 true residual, its norm
+    // and k_cycle_init, then the slot's event
+    auto enqueue_status = [&](int slot) -> int32_t {
         hipEvent_t ev = pf ? prof_begin(pf, PK_RESIDUAL, s) : nullptr;
         if (v0raw && resid_block()) {
             hipLaunchKernelGGL((k_residual_block<RES_RB, RES_TZ>), dim3(bgrid.x, bgrid.y, (nrhs + RES_RB - 1) / RES_RB),
@@ -1145,125 +1156,166 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
         }
         if (pf) prof_end(pf, PK_RESIDUAL, ev, resid_bytes, s);
-        applications++;
         if (!v0raw)
             hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
+        char* d = (char*)W.dstat_rel + (size_t)slot * W.stat_bytes;
         hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, v0raw ? nchr_used : nch, mm, o->tolerance, W.partial,
-                           W.bnorm, W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, W.dstat_rel,
-                           W.dstat_active, W.dstat_iters, (int)v0raw);
+                           W.bnorm, W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, (double*)d,
+                           (int*)(d + (size_t)W.nrhs * sizeof(double)),
+                           (int*)(d + (size_t)W.nrhs * (sizeof(double) + sizeof(int))), (int)v0raw);
         HELM_HIP(hipGetLastError());
-        HELM_HIP(hipStreamSynchronize(s));
+        HELM_HIP(hipEventRecord(W.stat_ev[slot], s));
+        return HELM_OK;
+    };
+    double vec_bytes = 8.0 * nn * nrhs;  // one c64 vector over the RHS active at the last status read
+    auto enqueue_cycle = [&](int nsteps, hipStream_t cs) -> int32_t {
+        if (!v0raw) hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
+        for (int j = 0; j < nsteps; ++j) {
+            float2* vj = W.V + (int64_t)j * vstride;
+            // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
+            float2* zj = W.Z + (int64_t)j * vstride;
+            // w = A z_j: in V_{j+1}'s slot when the basis is scaled lazily
+            float2* wj = lazy ? W.V + (int64_t)(j + 1) * vstride : W.w;
+            hipEvent_t ev;
+            if (pc != HELM_PC_NONE) {
+                ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
+                if (lazy && L.op == HELM_OP_COND8)
+                    HELM_TRY(cond_precond(p, vj, zj, nrhs, adj, cs, W.pscale));
+                else if (lazy)
+                    HELM_TRY(mg_vcycle(p, vj, zj, nrhs, adj, cs, W.pscale, W.cyc));
+                else
+                    HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));
+                if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, cs);
+            } else {
+                HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, cs));
+            }
+            ev = pf ? prof_begin(pf, PK_APPLY, cs) : nullptr;
+            HELM_TRY(launch_apply32(L, k0sq, zj, wj, nrhs, adj, nullptr, 0.0f, cs));
+            if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, cs);
+            hipEvent_t ea, eb;
+            if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
+                hipExtLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs,
+                                      ea, eb, 0, n, nrhs, nchd, kmax, j,
+                                      (const float2*)W.V, (const float2*)wj, (const int*)W.cyc, W.partial,
+                                      pd_part);
+                prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
+            } else {
+                hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs, n,
+                                   nrhs, nchd, kmax, j, W.V, wj,
+                                   W.cyc, W.partial, pd_part);
+            }
+            if (dcgs_fused()) {
+                if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
+                    hipExtLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs,
+                                          nch, nchd, mm, kmax, j, o->tolerance, (const double2*)W.partial,
+                                          (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g,
+                                          W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
+                    prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
+                } else {
+                    hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, nchd, mm,
+                                       kmax, j, o->tolerance, W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g,
+                                       W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
+                }
+                const bool last = j + 1 >= nsteps;
+                hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last || lazy ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm,
+                                   j, (int)last, pw2, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used,
+                                   wj, lazy ? wj : W.V + (int64_t)(last ? j : j + 1) * vstride,
+                                   lazy ? W.pscale : nullptr);
+                HELM_HIP(hipGetLastError());
+                continue;
+            }
+            hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, cs, nchd, mm, kmax, j, o->tolerance,
+                               W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc, W.used, W.coef);
+            if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
+                hipExtLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs, nch, kmax, j,
+                                      W.V, W.w, (const double2*)W.coef, (const int*)W.cyc, pw2);
+                prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
+            } else {
+                hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, kmax, j, W.V, W.w,
+                                   W.coef, W.cyc, pw2);
+            }
+            hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, cs, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
+                               W.g, W.est, W.inv, W.cyc, W.used);
+            if (j + 1 < nsteps)
+                hipLaunchKernelGGL(k_scale32, gvec, blk, 0, cs, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
+                                   W.cyc);
+            else
+                hipLaunchKernelGGL(k_dcgs_finish, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, j, W.H,
+                                   W.cs, W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
+            HELM_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, kmax, W.H, W.g, W.used,
+                           W.coef);
+        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, cs, n, nrhs, kmax, W.Z, W.coef, W.used, x);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    };
+    // one restart cycle; full cycles after the first (which performs any lazy
+    // allocation) replay a captured graph
+    auto run_cycle = [&](int msteps, int index) -> int32_t {
+        if (o->use_graph && !pf && msteps == m && index > 1) {
+            const uint64_t key[8] = {(uint64_t)m, (uint64_t)nrhs, (uint64_t)adj, (uint64_t)pc, (uint64_t)(uintptr_t)x,
+                                     (uint64_t)(uintptr_t)p->net, p->generation, (uint64_t)(o->tolerance * 1e18)};
+            return cycle_graph(W, key, enqueue_cycle, m, s);
+        }
+        return enqueue_cycle(msteps, s);
+    };
+    // The next cycle and its status are queued before the host reads the
+    // current status (HELM_RESTART_AHEAD=0: after), so the device does not
+    // idle while the host decides.  When the status says every RHS has
+    // stopped, the cycle queued ahead changes nothing: each kernel skips RHS
+    // with cyc = 0 (the multigrid whole groups of them), used stays 0 and x is
+    // updated only over used > 0.  The iteration budget is known on the host,
+    // so no cycle beyond it is queued.
+    const bool ahead = restart_ahead();
+    std::vector<int> prev_iters(nrhs, 0);
+    int slot = 0, last_steps = 0;
+    HELM_TRY(enqueue_status(slot));
+    applications = 1;
+    for (;;) {
+        const int msteps = std::min(m, o->max_iterations - steps);
+        const size_t done = pf ? pf->pending.size() : 0;  // profiled launches up to this status
+        const bool queued = ahead && msteps > 0;
+        if (queued) {
+            HELM_TRY(run_cycle(msteps, cycles + 1));
+            HELM_TRY(enqueue_status(slot ^ 1));
+        }
+        HELM_HIP(hipEventSynchronize(W.stat_ev[slot]));
         {
-            const char* hs = (const char*)W.hstat;
+            const char* hs = (const char*)W.hstat + (size_t)slot * W.stat_bytes;
             std::memcpy(W.h_rel.data(), hs, nrhs * sizeof(double));
             std::memcpy(W.h_active.data(), hs + (size_t)W.nrhs * sizeof(double), nrhs * sizeof(int));
             std::memcpy(W.h_iters.data(), hs + (size_t)W.nrhs * (sizeof(double) + sizeof(int)), nrhs * sizeof(int));
         }
-        if (pf) prof_flush(pf);
+        if (pf) {
+            // the launches of the cycle that just ended enter the profile only
+            // if every RHS ran all its steps (no launch skipped work)
+            bool full = true;
+            for (int r = 0; r < nrhs; ++r) full = full && W.h_iters[r] - prev_iters[r] == last_steps;
+            prof_flush_first(pf, queued ? done : pf->pending.size(), full);
+        }
+        prev_iters = W.h_iters;
         if (o->callback) o->callback(o->user, applications, W.h_rel.data(), nrhs);
         int n_active = 0;
         for (int r = 0; r < nrhs; ++r) n_active += W.h_active[r];
-        const double vec_bytes = 8.0 * nn * n_active;  // one c64 vector over the active RHS
-        if (n_active == 0) break;
-        if (steps >= o->max_iterations) {
-            budget = true;
+        if (n_active == 0 || steps >= o->max_iterations) {
+            budget = n_active > 0;
+            if (queued) {
+                HELM_HIP(hipStreamSynchronize(s));
+                if (pf) prof_flush_first(pf, pf->pending.size(), false);  // the no-op cycle
+            }
             break;
         }
-        cycles++;
-        const int msteps = std::min(m, o->max_iterations - steps);
-        // one restart cycle: V_0 = r/beta, msteps flexible DCGS2 steps, x += Z y
-        auto enqueue_cycle = [&](int nsteps, hipStream_t cs) -> int32_t {
-            if (!v0raw) hipLaunchKernelGGL(k_scale_to32, gvec, blk, 0, cs, n, W.r64, W.V, W.inv, W.cyc);
-            for (int j = 0; j < nsteps; ++j) {
-                float2* vj = W.V + (int64_t)j * vstride;
-                // the direction must be the V_j the operator sees (DCGS2 rewrites V_j afterwards)
-                float2* zj = W.Z + (int64_t)j * vstride;
-                // w = A z_j: in V_{j+1}'s slot when the basis is scaled lazily
-                float2* wj = lazy ? W.V + (int64_t)(j + 1) * vstride : W.w;
-                hipEvent_t ev;
-                if (pc != HELM_PC_NONE) {
-                    ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
-                    if (lazy && L.op == HELM_OP_COND8)
-                        HELM_TRY(cond_precond(p, vj, zj, nrhs, adj, cs, W.pscale));
-                    else if (lazy)
-                        HELM_TRY(mg_vcycle(p, vj, zj, nrhs, adj, cs, W.pscale));
-                    else
-                        HELM_TRY(precond_apply(p, pc, vj, zj, nrhs, adj, cs));
-                    if (pf) prof_end(pf, PK_PRECOND, ev, 0.0, cs);
-                } else {
-                    HELM_HIP(hipMemcpyAsync(zj, vj, (size_t)vstride * sizeof(float2), hipMemcpyDeviceToDevice, cs));
-                }
-                ev = pf ? prof_begin(pf, PK_APPLY, cs) : nullptr;
-                HELM_TRY(launch_apply32(L, k0sq, zj, wj, nrhs, adj, nullptr, 0.0f, cs));
-                if (pf) prof_end(pf, PK_APPLY, ev, apply_bytes, cs);
-                hipEvent_t ea, eb;
-                if (prof_pair(pf, PK_DOTS, &ea, &eb)) {
-                    hipExtLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs,
-                                          ea, eb, 0, n, nrhs, nchd, kmax, j,
-                                          (const float2*)W.V, (const float2*)wj, (const int*)W.cyc, W.partial,
-                                          pd_part);
-                    prof_push(pf, PK_DOTS, ea, eb, (j + 2) * vec_bytes);
-                } else {
-                    hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, cs, n,
-                                       nrhs, nchd, kmax, j, W.V, wj,
-                                       W.cyc, W.partial, pd_part);
-                }
-                if (dcgs_fused()) {
-                    if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
-                        hipExtLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs,
-                                              nch, nchd, mm, kmax, j, o->tolerance, (const double2*)W.partial,
-                                              (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g,
-                                              W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
-                        prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
-                    } else {
-                        hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, nchd, mm,
-                                           kmax, j, o->tolerance, W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g,
-                                           W.est, W.cyc, W.used, W.V, wj, pw2, (int)v0raw);
-                    }
-                    const bool last = j + 1 >= nsteps;
-                    hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(last || lazy ? 1 : nch, nrhs), blk, 0, cs, n, nch, mm,
-                                       j, (int)last, pw2, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc, W.used,
-                                       wj, lazy ? wj : W.V + (int64_t)(last ? j : j + 1) * vstride,
-                                       lazy ? W.pscale : nullptr);
-                    HELM_HIP(hipGetLastError());
-                    continue;
-                }
-                hipLaunchKernelGGL(k_dcgs_reduce, dim3(nrhs), dim3(256), 0, cs, nchd, mm, kmax, j, o->tolerance,
-                                   W.partial, pd_part, W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.cyc, W.used, W.coef);
-                if (prof_pair(pf, PK_UPDATE, &ea, &eb)) {
-                    hipExtLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, ea, eb, 0, n, nrhs, nch, kmax, j,
-                                          W.V, W.w, (const double2*)W.coef, (const int*)W.cyc, pw2);
-                    prof_push(pf, PK_UPDATE, ea, eb, (j + 4) * vec_bytes);
-                } else {
-                    hipLaunchKernelGGL(k_dcgs_update, dim3(nch, nrhs), blk, 0, cs, n, nrhs, nch, kmax, j, W.V, W.w,
-                                       W.coef, W.cyc, pw2);
-                }
-                hipLaunchKernelGGL(k_dcgs_norm, dim3(nrhs), dim3(64), 0, cs, nch, mm, j, pw2, W.bnorm, W.H, W.cs, W.sn,
-                                   W.g, W.est, W.inv, W.cyc, W.used);
-                if (j + 1 < nsteps)
-                    hipLaunchKernelGGL(k_scale32, gvec, blk, 0, cs, n, W.w, W.V + (int64_t)(j + 1) * vstride, W.inv,
-                                       W.cyc);
-                else
-                    hipLaunchKernelGGL(k_dcgs_finish, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, j, W.H,
-                                       W.cs, W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
-                HELM_HIP(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, kmax, W.H, W.g, W.used,
-                               W.coef);
-            hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, cs, n, nrhs, kmax, W.Z, W.coef, W.used, x);
-            HELM_HIP(hipGetLastError());
-            return HELM_OK;
-        };
-        // full cycles after the first (which performs any lazy allocation) replay a captured graph
-        if (o->use_graph && !pf && msteps == m && cycles > 1) {
-            const uint64_t key[8] = {(uint64_t)m, (uint64_t)nrhs, (uint64_t)adj, (uint64_t)pc, (uint64_t)(uintptr_t)x,
-                                     (uint64_t)(uintptr_t)p->net, p->generation, (uint64_t)(o->tolerance * 1e18)};
-            HELM_TRY(cycle_graph(W, key, enqueue_cycle, m, s));
-        } else {
-            HELM_TRY(enqueue_cycle(msteps, s));
+        vec_bytes = 8.0 * nn * n_active;
+        if (!queued) {
+            HELM_TRY(run_cycle(msteps, cycles + 1));
+            HELM_TRY(enqueue_status(slot ^ 1));
         }
+        cycles++;
         steps += msteps;
-        applications += msteps;
+        last_steps = msteps;
+        applications += msteps + 1;
+        slot ^= 1;
     }
     for (int r = 0; r < nrhs; ++r) {
         info[r].reason = W.h_active[r] ? HELM_ITERATION_BUDGET : HELM_CONVERGED;

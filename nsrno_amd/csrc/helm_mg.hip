@@ -183,14 +183,25 @@ static bool fused_levels() {
 // XCD-aware tile order in the fused kernels, per kernel and level
 // (HELM_XCD_ORDER bits: 1 pre-smoothing on the finest level, 2 pre-smoothing
 // on coarser levels, 4 post-smoothing on the finest level, 8 post-smoothing
-// on coarser levels; 15: all).  At 576^2 x 11 it cuts the finest level's
-// post-smoothing traffic from 208 to 136 MB per launch.
+// on coarser levels; 15: all; 0: the hardware order everywhere).  The default
+// (4, right-hand-side groups fastest) measured 34.3 us per finest-level
+// post-smoothing launch at 576^2 x 11 against 35.5 us in the hardware order;
+// the other kernels measured no faster in it.
 static int xcd_order() {
-    static const int t = (int)env_int("HELM_XCD_ORDER", 0, 0, 15);
+    static const int t = (int)env_int("HELM_XCD_ORDER", 4, 0, 15);
     return t;
 }
-static int xcd_pre(int l) { return (xcd_order() >> (l == 0 ? 0 : 1)) & 1; }
-static int xcd_post(int l) { return (xcd_order() >> (l == 0 ? 2 : 3)) & 1; }
+// in the XCD-aware order the right-hand-side groups of a tile follow each
+// other (one XCD, one L2 for the tile's coefficients); HELM_XCD_RHS_FAST=0:
+// the groups run slowest
+static int xcd_rhs_fast() {
+    static const int t = (int)env_int("HELM_XCD_RHS_FAST", 1, 0, 1);
+    return t;
+}
+// kernel argument: 0 hardware order, 1 XCD-contiguous, 2 XCD-contiguous with
+// the right-hand-side group fastest
+static int xcd_pre(int l) { return ((xcd_order() >> (l == 0 ? 0 : 1)) & 1) * (1 + xcd_rhs_fast()); }
+static int xcd_post(int l) { return ((xcd_order() >> (l == 0 ? 2 : 3)) & 1) * (1 + xcd_rhs_fast()); }
 // post-smoothing kernel variant: without prefetching (default) it fits three
 // waves per SIMD, which measured 25% faster at 576^2 x 11 than the prefetching
 // variant's two (HELM_MG_POST_PREFETCH=1)
@@ -286,9 +297,10 @@ __device__ void tail_sweeps(const LevelView& V, int sweeps, bool zero_start, boo
 
 __global__ __launch_bounds__(1024) void k_tail(const LevelView* __restrict__ views, int t, int last, int conj_i,
                                                int pre, int post, int coarse, const float2* __restrict__ f_top,
-                                               float2* __restrict__ u_top) {
+                                               float2* __restrict__ u_top, const int* __restrict__ act) {
     extern __shared__ float2 lds[];
     const int r = blockIdx.x;
+    if (act && !act[r]) return;
     const bool conj = conj_i != 0;
     int64_t off[16];
     {
@@ -421,9 +433,10 @@ __device__ void tail_sweeps_p(int nz, int nx, int n, const float2* cf, int sweep
 
 __global__ __launch_bounds__(1024) void k_tail_lds(const LevelView* __restrict__ views, int t, int last, int conj_i,
                                                    int pre, int post, int coarse, const float2* __restrict__ f_top,
-                                                   float2* __restrict__ u_top) {
+                                                   float2* __restrict__ u_top, const int* __restrict__ act) {
     extern __shared__ float2 lds[];
     const int r = blockIdx.x;
+    if (act && !act[r]) return;
     const bool conj = conj_i != 0;
     int off[16];  // level l's block: u, partner, f, 6 coefficient planes (9 n float2)
     {
@@ -913,6 +926,15 @@ __device__ __forceinline__ void level_coef(int gi, int gj, int node, int nz, int
     }
 }
 
+// false when act (per RHS, nullable) marks every RHS of [r0, r1) inactive: the
+// workgroup then has nothing to do (a converged RHS's multigrid output is unused)
+__device__ __forceinline__ bool group_active(const int* __restrict__ act, int r0, int r1) {
+    if (!act) return true;
+    bool any = false;
+    for (int r = r0; r < r1; ++r) any = any || act[r] != 0;
+    return any;
+}
+
 // ---- fused level kernels (pre = post = 2) ---------------------------------
 // k_mgt_pre2_rr: u2 = S(S(0)) and f_c = R (f - M u2) in one pass.  Works on
 // the 8 x 32 coarse tile of k_mgt_restrict_res: f is staged on the fine
@@ -936,13 +958,15 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
                                                      float shift, int conj, const float2* __restrict__ wid,
                                                      const float2* __restrict__ tcf, const float2* __restrict__ f,
                                                      const double* __restrict__ fscale, float2* __restrict__ u,
-                                                     float2* __restrict__ fcoarse, int xcd) {
+                                                     float2* __restrict__ fcoarse, int xcd,
+                                                     const int* __restrict__ act) {
     __shared__ float2 sf[PR_1N];
     __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
     __shared__ float2 s2[PR_2N];
     const int tid = threadIdx.y * RR_TX + threadIdx.x;
     int bx, by, bz;
-    xcd_tile(bx, by, bz, xcd != 0);
+    xcd_tile(bx, by, bz, xcd != 0, xcd == 2);
+    if (!group_active(act, bz * rb, min(nrhs, bz * rb + rb))) return;
     const int I0 = by * RR_TZ, J0 = bx * RR_TX;
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
     const int64_t nf = (int64_t)nzf * nxf, nc = (int64_t)nzc * nxc;
@@ -1086,12 +1110,13 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
                                                    const float2* __restrict__ wid, const float2* __restrict__ tcf,
                                                    const float2* __restrict__ uc, const float2* __restrict__ u,
                                                    const float2* __restrict__ f, const double* __restrict__ fscale,
-                                                   float2* __restrict__ out, int xcd) {
+                                                   float2* __restrict__ out, int xcd, const int* __restrict__ act) {
     __shared__ float2 sv2[P2_N];
     __shared__ float2 sv1[MT_NS];
     __shared__ float2 scv[MP_NC];
     int bx, by, bz;
-    xcd_tile(bx, by, bz, xcd != 0);
+    xcd_tile(bx, by, bz, xcd != 0, xcd == 2);
+    if (!group_active(act, bz * rb, min(nrhs, bz * rb + rb))) return;
     const int z0 = by * MT_TZ, x0 = bx * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
     const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
     const int64_t ncn = (int64_t)nzc * nxc;
@@ -1403,7 +1428,7 @@ static int32_t tail_views(helm_plan* p) {
 // output, on coarser levels L.mg_f and L.mg_u.  L.mg_r holds the residual,
 // L.mg_t is the Jacobi ping-pong partner of u.
 static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs, bool adj, hipStream_t s,
-                      const double* fscale = nullptr) {
+                      const double* fscale = nullptr, const int* act = nullptr) {
     Level& L = p->levels[l];
     const MGConfig& c = p->mg;
     HELM_CHECK(!fscale || (l == 0 && mg_scaled_input_ok(p)), HELM_ERR_STATE,
@@ -1412,11 +1437,11 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const size_t full = tail_lds_bytes_full(p, l);
         if (tail_all_lds() && full <= 160 * 1024)
             hipLaunchKernelGGL(k_tail_lds, dim3(nrhs), dim3(1024), full, s, (const LevelView*)p->mg_views, l,
-                               (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u);
+                               (int)p->levels.size() - 1, (int)adj, c.pre, c.post, c.coarse, f, u, act);
         else
             hipLaunchKernelGGL(k_tail, dim3(nrhs), dim3(1024), tail_lds_bytes(p, l), s,
                                (const LevelView*)p->mg_views, l, (int)p->levels.size() - 1, (int)adj, c.pre, c.post,
-                               c.coarse, f, u);
+                               c.coarse, f, u, act);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }
@@ -1447,29 +1472,29 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
                                   rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
-                                  sh, (int)adj, (const float2*)L.wid, tab, f, fscale, L.mg_t, C.mg_f, xcd_pre(l));
+                                  sh, (int)adj, (const float2*)L.wid, tab, f, fscale, L.mg_t, C.mg_f, xcd_pre(l), act);
             prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ), 0,
                                s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
-                               fscale, L.mg_t, C.mg_f, xcd_pre(l));
+                               fscale, L.mg_t, C.mg_f, xcd_pre(l), act);
         }
         HELM_HIP(hipGetLastError());
-        HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s));
+        HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s, nullptr, act));
         if (kpost >= 0 && prof_pair(pf, kpost, &ea, &eb)) {
             hipExtLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, ea, eb, 0,
                                   L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb,
                                   (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, (const float2*)C.mg_u,
-                                  (const float2*)L.mg_t, f, fscale, u, xcd_post(l));
+                                  (const float2*)L.mg_t, f, fscale, u, xcd_post(l), act);
             prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
         } else {
             hipLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, L.nz, L.nx,
                                L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                                L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
-                               L.mg_t, f, fscale, u, xcd_post(l));
+                               L.mg_t, f, fscale, u, xcd_post(l), act);
         }
         HELM_HIP(hipGetLastError());
         return HELM_OK;
@@ -1533,7 +1558,7 @@ bool mg_scaled_input_ok(const helm_plan* p) {
 }
 
 int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
-                  const double* fscale) {
+                  const double* fscale, const int* active) {
     HELM_CHECK(p->mg.levels > 0, HELM_ERR_STATE, "multigrid not set up");
     bool grew = false;
     for (auto& L : p->levels) {
@@ -1545,7 +1570,7 @@ int32_t mg_vcycle(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoi
         p->mg_views_valid = true;
         p->generation++;
     }
-    return vcycle(p, 0, v, z, nrhs, adjoint, s, fscale);
+    return vcycle(p, 0, v, z, nrhs, adjoint, s, fscale, active);
 }
 
 // Time one V-cycle for each workgroup target and keep the fastest.

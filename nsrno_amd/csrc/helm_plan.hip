@@ -5,6 +5,7 @@
 // nsrno/theranostic/wave.py:127-135; node, edge-midpoint and exterior-ghost
 // evaluation points follow wave.py:143-156 (the ghost edges sit at
 // x_0 - h/2 and x_end + h/2).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -115,6 +116,8 @@ void free_workspace(Workspace& w) {
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (w.hstat) (void)hipHostFree(w.hstat);
+    for (hipEvent_t e : w.stat_ev)
+        if (e) (void)hipEventDestroy(e);
     w = Workspace();
 }
 
@@ -204,10 +207,12 @@ void prof_push(Profiler* pf, int id, hipEvent_t a, hipEvent_t b, double bytes) {
     if (a && b) pf->pending.push_back({id, a, b, bytes});
 }
 
-void prof_flush(Profiler* pf) {
-    for (auto& r : pf->pending) {
+void prof_flush_first(Profiler* pf, size_t count, bool keep) {
+    count = std::min(count, pf->pending.size());
+    for (size_t i = 0; i < count; ++i) {
+        auto& r = pf->pending[i];
         float ms = 0.f;
-        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        if (hipEventSynchronize(r.b) == hipSuccess && keep && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
             pf->launches[r.id]++;
             pf->ms[r.id] += ms;
             pf->bytes[r.id] += r.bytes;
@@ -215,8 +220,10 @@ void prof_flush(Profiler* pf) {
         pf->pool.push_back(r.a);
         pf->pool.push_back(r.b);
     }
-    pf->pending.clear();
+    pf->pending.erase(pf->pending.begin(), pf->pending.begin() + count);
 }
+
+void prof_flush(Profiler* pf) { prof_flush_first(pf, pf->pending.size()); }
 
 static void prof_destroy(Profiler* pf) {
     if (!pf) return;

@@ -2,7 +2,8 @@
 
 Each fast path of the library keeps its predecessor behind an environment
 switch that is read once per process (HELM_DCGS_FUSED, HELM_LAZY_BASIS,
-HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS, HELM_COND64_TILED),
+HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS, HELM_XCD_ORDER,
+HELM_RESTART_AHEAD, HELM_COND64_TILED),
 so a test process only ever runs one side.  The tests below run child
 processes with the switches off and with the defaults and check both against
 the reference's golden outputs: the direct-solve fields of
@@ -44,9 +45,13 @@ assert all(i.reason == "converged" for i in infos), infos
 np.save(sys.argv[3], x.cpu().numpy().reshape(4, -1).T)
 '''
 
-SWITCHES = {"HELM_DCGS_FUSED": "0", "HELM_LAZY_BASIS": "0", "HELM_STENCIL_BLOCK": "0", "HELM_MG_TAIL_LDS": "0"}
+SWITCHES = {"HELM_DCGS_FUSED": "0", "HELM_LAZY_BASIS": "0", "HELM_STENCIL_BLOCK": "0", "HELM_MG_TAIL_LDS": "0",
+            "HELM_XCD_ORDER": "0"}
 # the per-node restart residual (the lazy basis stays on, so that kernel runs)
 SWITCHES_RESID = {"HELM_RESID_BLOCK": "0"}
+# the next restart cycle queued only after the host has read the status: the
+# same kernels on the same data, so the same bits
+SWITCHES_AHEAD = {"HELM_RESTART_AHEAD": "0"}
 
 
 def rel(a, b):
@@ -56,9 +61,10 @@ def rel(a, b):
 def test_switched_off_paths_match_default_and_reference(tmp_path):
     golden = np.load(GOLDEN_PATH)
     fields = {}
-    defaults = {k: v for k, v in os.environ.items() if k not in SWITCHES and k not in SWITCHES_RESID}
+    defaults = {k: v for k, v in os.environ.items()
+                if k not in SWITCHES and k not in SWITCHES_RESID and k not in SWITCHES_AHEAD}
     for name, env in (("alternate", dict(defaults, **SWITCHES)), ("per-node residual", dict(defaults, **SWITCHES_RESID)),
-                      ("default", defaults)):
+                      ("restart after status", dict(defaults, **SWITCHES_AHEAD)), ("default", defaults)):
         out = tmp_path / f"{name.replace(' ', '_')}.npy"
         print(f"child solve: {name} paths", flush=True)
         subprocess.run([sys.executable, "-c", SOLVE, str(ROOT), str(GOLDEN_PATH), str(out)], env=env, check=True,
@@ -68,6 +74,7 @@ def test_switched_off_paths_match_default_and_reference(tmp_path):
         assert rel(f, golden["fields"]) < 1e-7, name
     assert rel(fields["alternate"], fields["default"]) < 1e-7
     assert rel(fields["per-node residual"], fields["default"]) < 1e-7
+    assert np.array_equal(fields["restart after status"], fields["default"])
 
 
 COND_GOLDEN = Path(__file__).parent / "golden" / "conditioned_small.npz"

@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precond", default="mg", choices=["none", "mg", "learned"])
-    ap.add_argument("--restart", type=int, default=4,
+    ap.add_argument("--restart", type=int, default=3,
                     help="FGMRES restart length (short cycles: each restart is an FP64 refinement step)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=6000)

@@ -121,7 +121,7 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     return case
 
 
-def run_sweep(frequencies, angles, tol=1e-6, restart=4, workers=4, seed=0, log=None, autotune=True, graph=False,
+def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, autotune=True, graph=False,
               mg_shift=0.4):
     """Solve every frequency (largest grids first) on `workers` concurrent
     streams.  Returns (cases in input order, wall seconds)."""

@@ -1056,7 +1056,10 @@ int32_t warm_start(helm_plan* p, int pc, const double2* b, double2* x, int nrhs,
 
 // Capture (when the key changed) and launch the restart cycle as a HIP graph.
 // Capture runs on the workspace's own stream (the null stream cannot be
-// captured); the graph is launched on the caller's stream.
+// captured); the graph is launched on the caller's stream.  A capture that
+// fails (the runtime invalidated it, or a launch in it failed) has executed
+// nothing: the runtime's last error is cleared and the cycle is enqueued
+// directly on the caller's stream instead, which reports any genuine error.
 template <typename F>
 static int32_t cycle_graph(Workspace& W, const uint64_t key[8], F&& enqueue, int m, hipStream_t s) {
     if (!W.gexec || std::memcmp(W.gkey, key, sizeof(W.gkey)) != 0) {
@@ -1065,21 +1068,20 @@ static int32_t cycle_graph(Workspace& W, const uint64_t key[8], F&& enqueue, int
             W.gexec = nullptr;
         }
         if (!W.gstream) HELM_HIP(hipStreamCreateWithFlags(&W.gstream, hipStreamNonBlocking));
-        HELM_HIP(hipStreamBeginCapture(W.gstream, hipStreamCaptureModeThreadLocal));
-        const int32_t st = enqueue(m, W.gstream);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(W.gstream, &g);
-        if (st != HELM_OK) {
+        bool captured = hipStreamBeginCapture(W.gstream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (captured) {
+            const int32_t st = enqueue(m, W.gstream);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(W.gstream, &g);
+            captured = st == HELM_OK && e == hipSuccess && g &&
+                       hipGraphInstantiate(&W.gexec, g, nullptr, nullptr, 0) == hipSuccess;
             if (g) (void)hipGraphDestroy(g);
-            return st;
         }
-        HELM_HIP(e);
-        const hipError_t ei = hipGraphInstantiate(&W.gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) {
+        if (!captured) {
+            if (W.gexec) (void)hipGraphExecDestroy(W.gexec);
             W.gexec = nullptr;
-            set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-            return HELM_ERR_HIP;
+            (void)hipGetLastError();
+            return enqueue(m, s);
         }
         std::memcpy(W.gkey, key, sizeof(W.gkey));
     }

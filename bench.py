@@ -484,7 +484,7 @@ def main():
         pmc_summary(a.pmc_summary)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported by this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -86,6 +86,8 @@ template <> struct cplx<double> { using type = double2; };
 
 // ---------------------------------------------------------------- errors
 void set_error(const std::string& msg);
+// host-to-device copy, complete on return, that avoids the legacy null stream
+int32_t upload_h2d(void* dst, const void* src, size_t bytes);
 #define HELM_HIP(call)                                                                    \
     do {                                                                                  \
         hipError_t e_ = (call);                                                           \

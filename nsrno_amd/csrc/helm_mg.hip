@@ -1419,7 +1419,7 @@ static int32_t tail_views(helm_plan* p) {
         h[l] = LevelView{L.nz, L.nx, L.n, L.tcf, L.mg_f, L.mg_r};
     }
     if (!p->mg_views) HELM_HIP(hipMalloc(&p->mg_views, 16 * sizeof(LevelView)));
-    HELM_HIP(hipMemcpy(p->mg_views, h.data(), nl * sizeof(LevelView), hipMemcpyHostToDevice));
+    HELM_TRY(upload_h2d(p->mg_views, h.data(), nl * sizeof(LevelView)));
     return HELM_OK;
 }
 

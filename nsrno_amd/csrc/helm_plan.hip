@@ -44,13 +44,34 @@ static int32_t dev_alloc(T** ptr, size_t count) {
     return HELM_OK;
 }
 
+// Host-to-device copy that completes before returning, on a non-blocking
+// stream of the current device (one per device, created on first use) rather
+// than the legacy null stream: the runtime refuses a legacy-stream copy while
+// another host thread captures a graph (a solve on another plan).
+int32_t upload_h2d(void* dst, const void* src, size_t bytes) {
+    static std::mutex mu;
+    static hipStream_t streams[64] = {};
+    int dev = 0;
+    HELM_HIP(hipGetDevice(&dev));
+    HELM_CHECK(dev >= 0 && dev < 64, HELM_ERR_STATE, "upload: device index out of range");
+    hipStream_t st;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!streams[dev]) HELM_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+        st = streams[dev];
+    }
+    HELM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    HELM_HIP(hipStreamSynchronize(st));
+    return HELM_OK;
+}
+
 static int32_t upload_profile(const std::vector<double2>& h, double2** d64, float2** d32) {
     HELM_TRY(dev_alloc(d64, h.size()));
     HELM_TRY(dev_alloc(d32, h.size()));
     std::vector<float2> f(h.size());
     for (size_t i = 0; i < h.size(); ++i) f[i] = make_float2((float)h[i].x, (float)h[i].y);
-    HELM_HIP(hipMemcpy(*d64, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
-    HELM_HIP(hipMemcpy(*d32, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HELM_TRY(upload_h2d(*d64, h.data(), h.size() * sizeof(double2)));
+    HELM_TRY(upload_h2d(*d32, f.data(), f.size() * sizeof(float2)));
     return HELM_OK;
 }
 

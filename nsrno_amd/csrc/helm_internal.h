@@ -195,9 +195,7 @@ struct Workspace {
     // stat_bytes, used in turn, so the status of the cycle queued ahead does
     // not overwrite the one the host is reading
     void* hstat = nullptr;      // host block (hipHostMalloc)
-    double* dstat_rel = nullptr;  // its device-side addresses (slot 0)
-    int* dstat_active = nullptr;
-    int* dstat_iters = nullptr;
+    char* dstat = nullptr;      // its device-side address; per slot: rel [nrhs], active [nrhs], iters [nrhs]
     size_t stat_bytes = 0;
     hipEvent_t stat_ev[2] = {nullptr, nullptr};  // recorded after each slot's k_cycle_init
 };

@@ -955,9 +955,7 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
     {
         char* base = nullptr;
         HELM_HIP(hipHostGetDevicePointer((void**)&base, w.hstat, 0));
-        w.dstat_rel = (double*)base;
-        w.dstat_active = (int*)(base + (size_t)nrhs * sizeof(double));
-        w.dstat_iters = w.dstat_active + nrhs;
+        w.dstat = base;
     }
     return HELM_OK;
 }
@@ -1160,7 +1158,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         if (pf) prof_end(pf, PK_RESIDUAL, ev, resid_bytes, s);
         if (!v0raw)
             hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, W.r64, W.partial);
-        char* d = (char*)W.dstat_rel + (size_t)slot * W.stat_bytes;
+        char* d = W.dstat + (size_t)slot * W.stat_bytes;
         hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, v0raw ? nchr_used : nch, mm, o->tolerance, W.partial,
                            W.bnorm, W.beta, W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, W.pscale, (double*)d,
                            (int*)(d + (size_t)W.nrhs * sizeof(double)),

@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--cond-shape", type=int, nargs=2, default=[960, 1216],
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
-    ap.add_argument("--cond-restart", type=int, default=8, help="FGMRES restart length of the conditioned workload")
+    ap.add_argument("--cond-restart", type=int, default=6, help="FGMRES restart length of the conditioned workload")
     ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
                          "64 frequencies x 11 angles on uneven grids")

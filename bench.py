@@ -83,9 +83,10 @@ def parse():
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
     ap.add_argument("--cond-restart", type=int, default=6, help="FGMRES restart length of the conditioned workload")
-    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned"],
+    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned", "spectral", "net"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
-                         "64 frequencies x 11 angles on uneven grids")
+                         "64 frequencies x 11 angles on uneven grids; conditioned: the split-PML system; "
+                         "spectral: the S operator on a 1056x1216 box; net: the learned U-Net alone")
     ap.add_argument("--frequencies", type=int, default=64)
     ap.add_argument("--workers", type=int, default=4, help="frequencies in flight (frequencies workload)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -443,6 +444,132 @@ def conditioned_workload(a):
     system.close()
 
 
+def _event_time(fn, reps):
+    """Average seconds of fn() over reps calls, HIP events on torch's current stream."""
+    import torch
+    for _ in range(3):
+        fn()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def spectral_workload(a):
+    """The spectral (S) path of solve_frequency (wave_operator.py, wave_solver.py)
+    on the reference's own box size: a 961 x 1121 medium at dx = 50 um in a
+    1056 x 1216 periodic box with the 48-cell sponge, 5.9975 MHz (bin 360),
+    power-law attenuation, seeded smooth random c and rho, 11 plane-wave angles
+    in [-8, 8] deg as right-hand sides of one batched solve to rel-res a.tol
+    (the reference's default tolerance is 1e-4).  A step = set-up + incident
+    fields + right-hand sides + warm start + multigrid FGMRES + receiver
+    traces.  The 8-tap stencil k_stag8 (FP32, all 11 fields) is timed alone
+    against the HBM roofline: 16 + 16/R bytes per node per field."""
+    import numpy as np
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from nsrno_amd.wave_operator import WaveResidualOperator
+    from nsrno_amd.wave_solver import solve_frequency_batch
+    h, w = 961, 1121
+    dx = 50e-6
+    R = a.angles
+    spec = dict(medium_shape=[h, w], dx_m=dx, x0_m=-w * dx / 2, z0_m=-3 * dx,
+                angles_deg=list(np.linspace(-8.0, 8.0, R)), source_tref_s=[0.0] * R)
+    rng = np.random.default_rng(a.seed)
+
+    def smooth():
+        f = gaussian_filter(rng.normal(size=(h, w)), 8.0)
+        return f / np.abs(f).max()
+
+    medium = torch.tensor(np.stack((1540.0 + 50.0 * smooth(), 1000.0 + 60.0 * smooth()))[None],
+                          dtype=torch.float32, device="cuda")
+    f = 360 * 40e6 / 2401
+    model = WaveResidualOperator(spec, iterations=a.warm_steps, restart=a.restart, nrhs=R)
+    print(f"[bench] spectral box {model.H}x{model.W} medium {h}x{w} R={R} f={f / 1e6:.4f} MHz", flush=True)
+
+    def step():
+        return solve_frequency_batch(model, medium, f, range(R), tolerance=a.tol,
+                                     max_iterations=a.restart * (a.max_iterations // a.restart), restart=a.restart)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        preds, x, infos = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    data, _, _ = model.setup(medium, f)
+    v = torch.randn(R, model.H, model.W, dtype=torch.complex64, device="cuda")
+    out = torch.empty_like(v)
+    t = _event_time(lambda: data.plan.apply(v, adjoint=True, out=out), 30)
+    n = model.H * model.W
+    alg = (16.0 + 16.0 / R) * n * R
+    line = {
+        "metric": f"solves/s, spectral operator {model.H}x{model.W} box x {R} angles to rel-res {a.tol:g}",
+        "value": round(R * a.steps / dt, 4), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (FP64 reductions/refinement)",
+        "data": "synthetic (seeded smooth random c, rho)",
+        "config": {"workload": f"spectral {h}x{w} medium, box {model.H}x{model.W}, {f / 1e6:.4f} MHz x{R} angles",
+                   "tolerance": a.tol, "method": f"fgmres({a.restart}) + multigrid, warm start x{a.warm_steps}",
+                   "parallelism": "single GPU"},
+        "solver_iterations": [i["solver_iterations"] for i in infos],
+        "relative_residual": max(i["relative_residual"] for i in infos),
+        "reasons": sorted(set(i["reason"] for i in infos)),
+        "roofline": {"bound": "hbm", "kernel": "k_stag8<float> (8-tap periodic stencil, all RHS)",
+                     "achieved": round(alg / t / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile("k_stag8<float>"),
+                     "avg_launch_us": round(t * 1e6, 2), "algorithmic_bytes_per_launch": int(alg)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def net_workload(a):
+    """The learned preconditioner's BF16 U-Net (helm_net: 12 3x3 convolutions
+    of 32 channels over 4 levels, MFMA 32x32x16 bf16, FP32 accumulation) on the
+    576^2 canvas x 11 residuals: forward and backward passes timed with HIP
+    events; FLOPs counted as 2 * 9 * 32 * 32 per pixel per convolution
+    (backward: weight gradients of every layer, data gradients of all but
+    the first)."""
+    import numpy as np
+    import torch
+    from nsrno_amd.learned import LearnedNet, canvas
+    n = canvas(a.interior + 2 * a.pml)
+    B = a.angles
+    net = LearnedNet(n, n, B, training=True)
+    feats = (torch.randn(B, n, n, 32, device="cuda") * 0.5).to(torch.bfloat16)
+    out = net.forward(feats)
+    dout = (torch.randn_like(out, dtype=torch.float32) * 1e-2).to(torch.bfloat16)
+    tf = _event_time(lambda: net.forward(feats), 20)
+    tb = _event_time(lambda: net.backward(dout), 10)
+    # pixels per level and convolutions per level (encoder 2, decoder 1, output 1 on level 0)
+    px = [n * n, (n // 2) ** 2, (n // 4) ** 2, (n // 8) ** 2]
+    convs = [4, 3, 3, 2]  # 12 layers: the layers of each level of the U-Net
+    per_px = 2 * 9 * 32 * 32 * B
+    flop = sum(p * c for p, c in zip(px, convs)) * per_px
+    # backward: weight gradients of all 12 layers, data gradients of all but the first
+    flop_b = 2 * flop - px[0] * per_px
+    line = {
+        "metric": f"U-Net forward+backward TFLOP/s, {n}x{n} x {B}",
+        "value": round((flop + flop_b) / (tf + tb) / 1e12, 2),
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": 30, "warmup": 3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random features)",
+        "config": {"workload": f"learned preconditioner U-Net, canvas {n}^2, batch {B}", "parallelism": "single GPU"},
+        "forward_ms": round(tf * 1e3, 3), "backward_ms": round(tb * 1e3, 3),
+        "roofline": {"bound": "mfma", "kernel": "k_conv_fwd (whole forward pass)",
+                     "achieved": round(flop / tf / 1e12, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(flop / tf / 1e12 / 2500.0, 4), "traffic": None},
+        "backward_tflop_s": round(flop_b / tb / 1e12, 2),
+    }
+    print(json.dumps(line), flush=True)
+    net.close()
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -499,6 +626,12 @@ def main():
         return
     if a.workload == "conditioned":
         conditioned_workload(a)
+        return
+    if a.workload == "spectral":
+        spectral_workload(a)
+        return
+    if a.workload == "net":
+        net_workload(a)
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)

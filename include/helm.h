@@ -60,7 +60,8 @@ extern "C" {
 #define HELM_CONVERGED 0
 #define HELM_ITERATION_BUDGET 1
 #define HELM_BREAKDOWN 2
-#define HELM_STAGNATION 3
+#define HELM_STAGNATION 3     /* no progress over the stagnation window     */
+#define HELM_DIVERGED 4       /* BiCGSTAB: the recomputed residual exploded (dense_wave.py:140-142) */
 
 #define HELM_GMRES 0
 #define HELM_BICGSTAB 1
@@ -106,6 +107,21 @@ typedef struct {
     /* nonzero: GMRES replays every full restart cycle after the first as one
      * captured HIP graph (the cycle's kernels, preconditioner included) */
     int32_t use_graph;
+    /* Automatic fallback (nsrno/dense_wave.py:212-216, 233-235):
+     * - GMRES with HELM_PC_LEARNED: a right-hand side whose true residual has
+     *   not fallen below stagnation_ratio times its value stagnation_steps
+     *   Krylov steps earlier is stagnating; with fallback != 0 the solve goes
+     *   on with the multigrid preconditioner (info.fallback = 1), otherwise
+     *   that right-hand side stops with HELM_STAGNATION;
+     * - BiCGSTAB: with fallback != 0 the right-hand sides that end without
+     *   converging (breakdown, divergence, budget) are solved on by FGMRES with
+     *   the multigrid preconditioner (plain GMRES when none is set up) from
+     *   the iterate BiCGSTAB left (from the initial guess if that is not
+     *   finite), within the remaining iteration budget.
+     * stagnation_steps <= 0: 60; stagnation_ratio <= 0: 0.5. */
+    int32_t fallback;
+    int32_t stagnation_steps;
+    double stagnation_ratio;
 } helm_solve_opts;
 
 typedef struct {
@@ -114,6 +130,8 @@ typedef struct {
     int32_t applications;      /* all operator applications incl. residuals  */
     int32_t cycles;            /* restarts / refinement cycles               */
     double relative_residual;  /* true FP64 ||b - A x|| / ||b|| at return    */
+    int32_t fallback;          /* 1: finished by the fallback solver          */
+    int32_t first_reason;      /* reason of the first solver (before any fallback) */
 } helm_solve_info;
 
 /* ---- library ---------------------------------------------------------- */

@@ -9,9 +9,9 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libhelm.so"
 
-HELM_CONVERGED, HELM_ITERATION_BUDGET, HELM_BREAKDOWN, HELM_STAGNATION = 0, 1, 2, 3
+HELM_CONVERGED, HELM_ITERATION_BUDGET, HELM_BREAKDOWN, HELM_STAGNATION, HELM_DIVERGED = 0, 1, 2, 3, 4
 REASONS = {HELM_CONVERGED: "converged", HELM_ITERATION_BUDGET: "iteration_budget",
-           HELM_BREAKDOWN: "breakdown", HELM_STAGNATION: "stagnation"}
+           HELM_BREAKDOWN: "breakdown", HELM_STAGNATION: "stagnation", HELM_DIVERGED: "diverged"}
 HELM_GMRES, HELM_BICGSTAB = 0, 1
 PC_NONE, PC_MG, PC_LEARNED = 0, 1, 2
 
@@ -45,12 +45,13 @@ CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), c_int32)
 class HelmSolveOpts(ctypes.Structure):
     _fields_ = [("method", c_int32), ("precond", c_int32), ("restart", c_int32), ("max_iterations", c_int32),
                 ("tolerance", c_double), ("adjoint", c_int32), ("use_x0", c_int32),
-                ("callback", CALLBACK), ("user", c_void_p), ("use_graph", c_int32)]
+                ("callback", CALLBACK), ("user", c_void_p), ("use_graph", c_int32), ("fallback", c_int32),
+                ("stagnation_steps", c_int32), ("stagnation_ratio", c_double)]
 
 
 class HelmSolveInfo(ctypes.Structure):
     _fields_ = [("reason", c_int32), ("iterations", c_int32), ("applications", c_int32), ("cycles", c_int32),
-                ("relative_residual", c_double)]
+                ("relative_residual", c_double), ("fallback", c_int32), ("first_reason", c_int32)]
 
 
 class HelmTuning(ctypes.Structure):

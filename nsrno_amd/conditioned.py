@@ -223,7 +223,7 @@ def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5
     b = b.reshape(R, system.g.H, system.g.W)
     x0 = None if initial is None else initial.to(torch.complex128).reshape(R, system.g.H, system.g.W)
     x, infos = system.plan.solve(b, x0, tolerance=tolerance, restart=restart, max_iterations=restart * max_cycles,
-                                 precond="mg", method=method, adjoint=adjoint)
+                                 precond="mg", method=method, adjoint=adjoint, fallback=method == "bicgstab")
     residual = system.plan.apply(x, adjoint=adjoint) - b
     relative = float(residual.norm() / b.norm().clamp_min(1e-30))
     individual = residual.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1).clamp_min(1e-30)

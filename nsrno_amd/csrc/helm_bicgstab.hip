@@ -1,14 +1,17 @@
 // Right-preconditioned BiCGSTAB with FP64 refinement.
 //
 // Algorithm: nsrno/dense_wave.py:118-148 (short recurrence, shadow residual
-// r~ = r0, breakdown when |rho| or |omega| or |<r~, v>| vanish, divergence
-// check on the recomputed residual every 40 iterations) inside the mixed
-// FP32-inner / FP64-outer refinement of dense_wave.py:167-192: at most five
-// outer FP64 true-residual corrections, each running a complex64 BiCGSTAB on
-// the current residual to an inner tolerance.  Differences: every right-hand
-// side runs its own recurrence (per-RHS scalars on the device, lock-step
-// kernels, converged or broken-down RHS are skipped), all inner products are
-// accumulated in FP64, and the host synchronises only every CHECK steps.
+// r~ = r0, breakdown when |rho| or |omega| or |<r~, v>| vanish, the half-step
+// exit on ||s||, and every 40 iterations the residual recomputed from x with
+// 'diverged' when it is not finite or above 1e5 times the pass's scale)
+// inside the mixed FP32-inner / FP64-outer refinement of
+// dense_wave.py:167-192: outer FP64 true-residual corrections, each running a
+// complex64 BiCGSTAB on the current residual to an inner tolerance.
+// Differences: every right-hand side runs its own recurrence (per-RHS scalars
+// on the device, lock-step kernels, converged, broken-down or diverged RHS
+// are skipped) and keeps its own reason, all inner products are accumulated
+// in FP64, the 40-step refresh recomputes b - A x in FP64, and the host
+// synchronises only every CHECK steps.
 #include <cmath>
 
 #include "helm_internal.h"
@@ -199,12 +202,14 @@ __global__ void k_bi_scalar_rho(int nch, const double2* __restrict__ partial, do
 }
 
 // stage 1: alpha = rho / <shadow, v>
-__global__ void k_bi_scalar_alpha(int nch, const double2* __restrict__ partial, double2* __restrict__ sc) {
+__global__ void k_bi_scalar_alpha(int nch, const double2* __restrict__ partial, double2* __restrict__ sc,
+                                  int* __restrict__ apps) {
     const int rr = blockIdx.x;
     double2* s = sc + rr * 8;
     if (s[S_STATE].x != 0.0) return;
     double2 den = red(partial + (int64_t)rr * nch, nch, nullptr);
     if (threadIdx.x) return;
+    apps[rr] += 1;  // v = A phat
     if (cabs2(den) < 1e-30 || !isfinite(den.x)) {
         s[S_STATE].x = 2.0;
         return;
@@ -214,10 +219,10 @@ __global__ void k_bi_scalar_alpha(int nch, const double2* __restrict__ partial, 
 
 // stage 2: |s| test (converged after the half step -> flag for x += alpha phat)
 __global__ void k_bi_scalar_s(int nch, double tol_abs_scale, const double2* __restrict__ partial,
-                              double2* __restrict__ sc, int* __restrict__ half) {
+                              double2* __restrict__ sc, int* __restrict__ half, int* __restrict__ its) {
     const int rr = blockIdx.x;
     double2* s = sc + rr * 8;
-    half[rr] = 0;
+    if (threadIdx.x == 0) half[rr] = 0;
     if (s[S_STATE].x != 0.0) return;
     double2 ss = red(partial + (int64_t)rr * nch, nch, nullptr);
     if (threadIdx.x) return;
@@ -226,33 +231,78 @@ __global__ void k_bi_scalar_s(int nch, double tol_abs_scale, const double2* __re
         half[rr] = 1;
         s[S_RNORM].x = sn;
         s[S_STATE].x = 1.0;
+        its[rr] += 1;  // the step counts (dense_wave.py:135-136)
     }
 }
 
 // stage 3: omega = <t, s> / <t, t>
-__global__ void k_bi_scalar_omega(int nch, const double2* __restrict__ partial, double2* __restrict__ sc) {
+__global__ void k_bi_scalar_omega(int nch, const double2* __restrict__ partial, double2* __restrict__ sc,
+                                  int* __restrict__ apps) {
     const int rr = blockIdx.x;
     double2* s = sc + rr * 8;
     if (s[S_STATE].x != 0.0) return;
     double2 ts = red(partial + ((int64_t)rr * 2 + 0) * nch, nch, nullptr);
     double2 tt = red(partial + ((int64_t)rr * 2 + 1) * nch, nch, nullptr);
     if (threadIdx.x) return;
-    s[S_OMEGA] = tt.x > 1e-300 ? make_double2(ts.x / tt.x, ts.y / tt.x) : make_double2(0.0, 0.0);
+    apps[rr] += 1;  // t = A shat
+    // denominator clamped at 1e-30 (dense_wave.py:138)
+    s[S_OMEGA] = make_double2(ts.x / fmax(tt.x, 1e-30), ts.y / fmax(tt.x, 1e-30));
 }
 
 // stage 4: |r| and next rho; convergence
 __global__ void k_bi_scalar_r(int nch, double tol_abs_scale, const double2* __restrict__ partial,
-                              double2* __restrict__ sc) {
+                              double2* __restrict__ sc, int* __restrict__ its) {
     const int rr = blockIdx.x;
     double2* s = sc + rr * 8;
     if (s[S_STATE].x != 0.0) return;
     double2 r2 = red(partial + ((int64_t)rr * 2 + 0) * nch, nch, nullptr);
     if (threadIdx.x) return;
+    its[rr] += 1;
     s[S_RHO_OLD] = s[S_RHO];
     const double rn = sqrt(r2.x);
     s[S_RNORM].x = rn;
     if (!isfinite(rn)) s[S_STATE].x = 3.0;
     else if (rn < s[S_SCALE].x * tol_abs_scale) s[S_STATE].x = 1.0;
+}
+
+// 40-step refresh (dense_wave.py:140-142): r = c64(b - A x) for the running
+// RHS (r64 holds b - A x in FP64), partial |r|^2
+__global__ __launch_bounds__(256) void k_bi_refresh(int64_t n, int nch, const double2* __restrict__ r64,
+                                                    float2* __restrict__ r, const double2* __restrict__ sc,
+                                                    double2* __restrict__ partial) {
+    __shared__ double lds[8];
+    const int chunk = blockIdx.x, rr = blockIdx.y;
+    if (sc[rr * 8 + S_STATE].x != 0.0) return;
+    const int64_t beg = (int64_t)chunk * BCHUNK, end = min(beg + BCHUNK, n);
+    const int64_t off = (int64_t)rr * n;
+    double2 acc[1] = {make_double2(0.0, 0.0)};
+    for (int64_t i = beg + threadIdx.x; i < end; i += 256) {
+        const double2 a = r64[off + i];
+        r[off + i] = make_float2((float)a.x, (float)a.y);
+        acc[0].x += a.x * a.x + a.y * a.y;
+    }
+    block_write<1>(acc, lds, partial, rr, nch, chunk);
+}
+
+// diverged when the recomputed residual is not finite or exceeds 1e5 x the
+// pass's scale; converged when it meets the inner tolerance
+__global__ void k_bi_scalar_refresh(int nch, double tol_abs_scale, const double2* __restrict__ partial,
+                                    double2* __restrict__ sc, int* __restrict__ apps) {
+    const int rr = blockIdx.x;
+    double2* s = sc + rr * 8;
+    if (s[S_STATE].x != 0.0) return;
+    double2 r2 = red(partial + (int64_t)rr * nch, nch, nullptr);
+    if (threadIdx.x) return;
+    apps[rr] += 1;
+    const double rn = sqrt(r2.x);
+    s[S_RNORM].x = rn;
+    if (!isfinite(rn) || rn > 1e5 * s[S_SCALE].x) s[S_STATE].x = 3.0;
+    else if (rn < s[S_SCALE].x * tol_abs_scale) s[S_STATE].x = 1.0;
+}
+
+__global__ void k_bi_izero(int count, int* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
 }
 
 // initialise the inner recurrence: r = shadow = c64(r64), p = v = 0, scalars
@@ -363,11 +413,16 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
         const int64_t c = (int64_t)nrhs * n;
         hipLaunchKernelGGL(k_bi_zero, dim3((unsigned)((c + 255) / 256)), blk, 0, s, c, x);
     }
-    int total_steps = 0, applications = 0, outer = 0, worst_state = 0;
-    std::vector<int> steps_rhs(nrhs, 0);
+    // per-RHS counters on the device: completed steps (W.iters) and operator
+    // applications inside the recurrences (W.used)
+    hipLaunchKernelGGL(k_bi_izero, dim3((nrhs + 255) / 256), blk, 0, s, nrhs, W.iters);
+    hipLaunchKernelGGL(k_bi_izero, dim3((nrhs + 255) / 256), blk, 0, s, nrhs, W.used);
+    int total_steps = 0, outer = 0;
+    // state of each RHS at the end of its last inner pass (0 running/budget,
+    // 1 converged, 2 breakdown, 3 diverged); reset at every outer pass
+    std::vector<int> rstate(nrhs, 0);
     for (;; ++outer) {
         HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
-        applications++;
         hipLaunchKernelGGL(k_bi_norm_partial, gch, blk, 0, s, n, nch, W.r64, W.partial);
         if (outer == 0) {  // ||b||
             hipLaunchKernelGGL(k_bi_norm_partial, gch, blk, 0, s, n, nch, b, W.partial + (size_t)nrhs * nch);
@@ -380,10 +435,12 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
         HELM_HIP(hipMemcpyAsync(W.h_rel.data(), W.rel, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipMemcpyAsync(W.h_active.data(), W.active, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
         HELM_HIP(hipStreamSynchronize(s));
-        if (o->callback) o->callback(o->user, applications, W.h_rel.data(), nrhs);
+        if (o->callback) o->callback(o->user, outer + total_steps * 2, W.h_rel.data(), nrhs);
         int n_active = 0;
         for (int r = 0; r < nrhs; ++r) n_active += W.h_active[r];
         if (n_active == 0 || total_steps >= o->max_iterations || outer >= 5 + o->max_iterations / 400) break;
+        for (int r = 0; r < nrhs; ++r)
+            if (W.h_active[r]) rstate[r] = 0;
         // inner tolerance relative to the current residual: reach tol*||b|| with margin
         double worst = 0;
         for (int r = 0; r < nrhs; ++r)
@@ -409,9 +466,10 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
             HELM_TRY(launch_apply32(L, k0sq, ph, W.bv, nrhs, adj, nullptr, 0.0f, s));
             hipLaunchKernelGGL(k_bi_dots<1>, gch, blk, 0, s, n, nch, W.bshadow, W.bv, nullptr, nullptr, W.bsc,
                                W.partial);
-            hipLaunchKernelGGL(k_bi_scalar_alpha, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bsc);
+            hipLaunchKernelGGL(k_bi_scalar_alpha, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bsc, W.used);
             hipLaunchKernelGGL(k_bi_s, gch, blk, 0, s, n, nch, W.br, W.bv, W.bs, W.bsc, W.partial);
-            hipLaunchKernelGGL(k_bi_scalar_s, dim3(nrhs), dim3(64), 0, s, nch, inner, W.partial, W.bsc, W.cyc);
+            hipLaunchKernelGGL(k_bi_scalar_s, dim3(nrhs), dim3(64), 0, s, nch, inner, W.partial, W.bsc, W.cyc,
+                               W.iters);
             hipLaunchKernelGGL(k_bi_xonly, gvec, blk, 0, s, n, x, W.bphat, W.bsc, W.cyc);
             if (pc != HELM_PC_NONE) {
                 HELM_TRY(precond_apply(p, pc, W.bs, W.bshat, nrhs, adj, s));
@@ -420,14 +478,18 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
             }
             HELM_TRY(launch_apply32(L, k0sq, W.bshat, W.bt, nrhs, adj, nullptr, 0.0f, s));
             hipLaunchKernelGGL(k_bi_dots<2>, gch, blk, 0, s, n, nch, W.bt, W.bs, W.bt, W.bt, W.bsc, W.partial);
-            hipLaunchKernelGGL(k_bi_scalar_omega, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bsc);
+            hipLaunchKernelGGL(k_bi_scalar_omega, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bsc, W.used);
             hipLaunchKernelGGL(k_bi_xr, gch, blk, 0, s, n, nch, x, W.bphat, W.bshat, W.bs, W.bt, W.br, W.bshadow,
                                W.bsc, W.partial);
-            hipLaunchKernelGGL(k_bi_scalar_r, dim3(nrhs), dim3(64), 0, s, nch, inner, W.partial, W.bsc);
+            hipLaunchKernelGGL(k_bi_scalar_r, dim3(nrhs), dim3(64), 0, s, nch, inner, W.partial, W.bsc, W.iters);
+            if (it % 40 == 39) {  // residual refresh and divergence test (dense_wave.py:140-142)
+                HELM_TRY(launch_apply64(L, k0sq, x, W.r64, nrhs, adj, b, s));
+                hipLaunchKernelGGL(k_bi_refresh, gch, blk, 0, s, n, nch, W.r64, W.br, W.bsc, W.partial);
+                hipLaunchKernelGGL(k_bi_scalar_refresh, dim3(nrhs), dim3(64), 0, s, nch, inner, W.partial, W.bsc,
+                                   W.used);
+            }
             HELM_HIP(hipGetLastError());
             total_steps++;
-            applications += 2;
-            for (int r = 0; r < nrhs; ++r) steps_rhs[r] += W.h_active[r];
             if ((it + 1) % CHECK == 0 || total_steps >= o->max_iterations) {
                 HELM_HIP(hipMemcpyAsync(hsc.data(), W.bsc, hsc.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
                 HELM_HIP(hipStreamSynchronize(s));
@@ -435,8 +497,7 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
                 for (int r = 0; r < nrhs; ++r) {
                     const int st = (int)hsc[(size_t)r * 8 + S_STATE].x;
                     if (st == 0) any = true;
-                    if (st >= 2) worst_state = std::max(worst_state, st);
-                    if (st != 0) W.h_active[r] = 0;
+                    if (W.h_active[r]) rstate[r] = st;
                 }
                 if (!any) break;
             }
@@ -445,12 +506,21 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
                                W.partial);
         }
     }
+    std::vector<int> its(nrhs), apps(nrhs);
+    HELM_HIP(hipMemcpyAsync(its.data(), W.iters, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
+    HELM_HIP(hipMemcpyAsync(apps.data(), W.used, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
+    HELM_HIP(hipStreamSynchronize(s));
     for (int r = 0; r < nrhs; ++r) {
         const bool conv = !(W.h_rel[r] >= o->tolerance);
-        info[r].reason = conv ? HELM_CONVERGED : (worst_state == 2 ? HELM_BREAKDOWN : HELM_ITERATION_BUDGET);
-        if (!std::isfinite(W.h_rel[r])) info[r].reason = HELM_BREAKDOWN;
-        info[r].iterations = steps_rhs[r];
-        info[r].applications = 2 * steps_rhs[r] + outer + 1;
+        int reason = HELM_ITERATION_BUDGET;
+        if (conv) reason = HELM_CONVERGED;
+        else if (!std::isfinite(W.h_rel[r]) || rstate[r] == 3) reason = HELM_DIVERGED;
+        else if (rstate[r] == 2) reason = HELM_BREAKDOWN;
+        info[r].reason = reason;
+        info[r].first_reason = reason;
+        info[r].fallback = 0;
+        info[r].iterations = its[r];
+        info[r].applications = apps[r] + outer + 1;  // + one FP64 true residual per outer pass
         info[r].cycles = outer;
         info[r].relative_residual = W.h_rel[r];
     }

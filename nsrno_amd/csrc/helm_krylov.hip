@@ -889,6 +889,12 @@ __global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, 
     x[(int64_t)r * n + p] = acc;
 }
 
+// stop right-hand side r (stagnation): no further cycles run for it
+__global__ void k_stop_rhs(int r, int* __restrict__ active, int* __restrict__ cyc) {
+    active[r] = 0;
+    cyc[r] = 0;
+}
+
 __global__ void k_zero_c128(int64_t count, double2* p) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = make_double2(0.0, 0.0);
@@ -1100,7 +1106,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const int mm = W.restart;  // row stride of H / rotations (capacity)
     const double k0sq = p->k0 * p->k0;
     const bool adj = o->adjoint != 0;
-    const int pc = o->precond;
+    int pc = o->precond;  // the learned preconditioner can fall back to the multigrid mid-solve
     const dim3 blk(256);
     const dim3 gvec((unsigned)((n + 255) / 256), nrhs);
     const int64_t vstride = (int64_t)nrhs * n;
@@ -1269,6 +1275,14 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     // so no cycle beyond it is queued.
     const bool ahead = restart_ahead();
     std::vector<int> prev_iters(nrhs, 0);
+    // stagnation: a RHS whose true residual is above ratio x its value at least
+    // `window` Krylov steps earlier (checked for the learned preconditioner, or
+    // whenever the caller sets stagnation_steps)
+    const bool watch = pc == HELM_PC_LEARNED || o->stagnation_steps > 0;
+    const int window = o->stagnation_steps > 0 ? o->stagnation_steps : 60;
+    const double ratio = o->stagnation_ratio > 0 ? o->stagnation_ratio : 0.5;
+    std::vector<std::vector<std::pair<int, double>>> hist(nrhs);
+    std::vector<int> stopped(nrhs, 0), fell(nrhs, 0);
     int slot = 0, last_steps = 0;
     HELM_TRY(enqueue_status(slot));
     applications = 1;
@@ -1296,8 +1310,40 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         }
         prev_iters = W.h_iters;
         if (o->callback) o->callback(o->user, applications, W.h_rel.data(), nrhs);
+        if (watch) {
+            bool stagnant = false;
+            std::vector<int> which;
+            for (int r = 0; r < nrhs; ++r) {
+                if (!W.h_active[r] || stopped[r]) continue;
+                auto& h = hist[r];
+                h.emplace_back(W.h_iters[r], W.h_rel[r]);
+                // the newest status at least `window` steps old
+                const std::pair<int, double>* old_st = nullptr;
+                for (const auto& e : h)
+                    if (e.first <= W.h_iters[r] - window) old_st = &e;
+                if (old_st && !(W.h_rel[r] <= ratio * old_st->second)) {
+                    stagnant = true;
+                    which.push_back(r);
+                }
+            }
+            if (stagnant) {
+                if (o->fallback && pc == HELM_PC_LEARNED && p->mg.levels > 0) {
+                    // go on with the multigrid preconditioner for every RHS still running
+                    pc = HELM_PC_MG;
+                    for (int r = 0; r < nrhs; ++r)
+                        if (W.h_active[r] && !stopped[r]) fell[r] = 1;
+                    for (auto& h : hist) h.clear();
+                } else {
+                    for (int r : which) {
+                        stopped[r] = 1;
+                        hipLaunchKernelGGL(k_stop_rhs, dim3(1), dim3(1), 0, s, r, W.active, W.cyc);
+                    }
+                    HELM_HIP(hipGetLastError());
+                }
+            }
+        }
         int n_active = 0;
-        for (int r = 0; r < nrhs; ++r) n_active += W.h_active[r];
+        for (int r = 0; r < nrhs; ++r) n_active += W.h_active[r] && !stopped[r];
         if (n_active == 0 || steps >= o->max_iterations) {
             budget = n_active > 0;
             if (queued) {
@@ -1320,6 +1366,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     for (int r = 0; r < nrhs; ++r) {
         info[r].reason = W.h_active[r] ? HELM_ITERATION_BUDGET : HELM_CONVERGED;
         if (!std::isfinite(W.h_rel[r])) info[r].reason = HELM_BREAKDOWN;
+        if (stopped[r]) info[r].reason = HELM_STAGNATION;
+        info[r].fallback = fell[r];
+        info[r].first_reason = fell[r] ? HELM_STAGNATION : info[r].reason;
         info[r].iterations = W.h_iters[r];
         info[r].applications = W.h_iters[r] + cycles + 1;
         info[r].cycles = cycles;
@@ -1349,9 +1398,43 @@ extern "C" int32_t helm_solve(helm_plan* p, const void* rhs, void* x, int32_t nr
                "helm_solve: call helm_mg_setup before using the multigrid preconditioner");
     hipStream_t s = (hipStream_t)stream;
     if (o->method == HELM_GMRES) return solve_gmres(p, (const double2*)rhs, (double2*)x, nrhs, o, info, s);
-    if (o->method == HELM_BICGSTAB) return solve_bicgstab(p, (const double2*)rhs, (double2*)x, nrhs, o, info, s);
-    set_error("helm_solve: unknown method");
-    return HELM_ERR_ARG;
+    if (o->method != HELM_BICGSTAB) {
+        set_error("helm_solve: unknown method");
+        return HELM_ERR_ARG;
+    }
+    HELM_TRY(solve_bicgstab(p, (const double2*)rhs, (double2*)x, nrhs, o, info, s));
+    if (!o->fallback) return HELM_OK;
+    // dense_wave.py:212-216: right-hand sides BiCGSTAB left unconverged go on
+    // with GMRES and the fixed (here: multigrid) preconditioner
+    bool any = false;
+    for (int r = 0; r < nrhs; ++r) any = any || info[r].reason != HELM_CONVERGED;
+    if (!any) return HELM_OK;
+    const int64_t n = p->levels[0].n;
+    for (int r = 0; r < nrhs; ++r)
+        if (!std::isfinite(info[r].relative_residual))  // a non-finite iterate restarts from zero
+            hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                               (double2*)x + (int64_t)r * n);
+    HELM_HIP(hipGetLastError());
+    helm_solve_opts g = *o;
+    g.method = HELM_GMRES;
+    g.precond = p->mg.levels > 0 ? HELM_PC_MG : HELM_PC_NONE;
+    g.restart = std::min(24, std::max(1, p->desc.restart));
+    g.use_x0 = 1;
+    g.fallback = 0;
+    g.stagnation_steps = 0;
+    std::vector<helm_solve_info> gi(nrhs);
+    HELM_TRY(solve_gmres(p, (const double2*)rhs, (double2*)x, nrhs, &g, gi.data(), s));
+    for (int r = 0; r < nrhs; ++r) {
+        if (info[r].reason == HELM_CONVERGED) continue;
+        info[r].first_reason = info[r].reason;
+        info[r].fallback = 1;
+        info[r].reason = gi[r].reason;
+        info[r].iterations += gi[r].iterations;
+        info[r].applications += gi[r].applications;
+        info[r].cycles += gi[r].cycles;
+        info[r].relative_residual = gi[r].relative_residual;
+    }
+    return HELM_OK;
 }
 
 extern "C" int32_t helm_warm_start(helm_plan* p, int32_t precond, const void* b, void* x, int32_t nrhs,

@@ -35,6 +35,8 @@ class SolveInfo:
     applications: int
     cycles: int
     relative_residual: float
+    fallback: bool = False      # finished by the fallback solver (multigrid FGMRES)
+    first_reason: str = ""      # reason of the first solver, before any fallback
 
 
 class Ports:
@@ -238,12 +240,19 @@ class HelmholtzPlan:
         return x
 
     def solve(self, rhs, x0=None, *, tolerance=1e-6, restart=None, max_iterations=3000, precond="none",
-              method="gmres", adjoint=False, callback=None, graph=False):
+              method="gmres", adjoint=False, callback=None, graph=False, fallback=None, stagnation_steps=0,
+              stagnation_ratio=0.0):
         """Krylov solve of A x = rhs (A^H x = rhs) for all right-hand sides.
 
         rhs: complex128 [R, nz, nx] (converted if needed).  Returns
         (x complex128 [R, nz, nx], [SolveInfo] * R).  graph: replay full restart
         cycles after the first as one captured HIP graph (GMRES).
+        fallback (default: on for the learned preconditioner): a stagnating
+        learned FGMRES goes on with the multigrid, and BiCGSTAB right-hand
+        sides that end unconverged are finished by multigrid FGMRES
+        (dense_wave.py:212-216); off: stagnation stops a right-hand side with
+        reason "stagnation".  stagnation_steps / stagnation_ratio: the window
+        (Krylov steps) and the residual reduction it must reach (0: 60, 0.5).
         """
         rhs = torch.as_tensor(rhs, device=self.device).to(torch.complex128).contiguous()
         nrhs = self._check_field(rhs)
@@ -262,11 +271,15 @@ class HelmholtzPlan:
                 callback(int(applications), [rel[i] for i in range(n)])
 
         cfun = _lib.CALLBACK(_cb)
+        if fallback is None:
+            fallback = precond == "learned"
         opts = _lib.HelmSolveOpts(meth, pc, restart, int(max_iterations), float(tolerance), int(bool(adjoint)),
-                                  int(x0 is not None), cfun, None, int(bool(graph)))
+                                  int(x0 is not None), cfun, None, int(bool(graph)), int(bool(fallback)),
+                                  int(stagnation_steps), float(stagnation_ratio))
         infos = (_lib.HelmSolveInfo * nrhs)()
         check(lib().helm_solve(self._h, _ptr(rhs), _ptr(x), nrhs, ctypes.byref(opts), infos, _stream()),
               "helm_solve")
         out = [SolveInfo(_lib.REASONS.get(i.reason, "unknown"), i.iterations, i.applications, i.cycles,
-                         i.relative_residual) for i in infos]
+                         i.relative_residual, bool(i.fallback), _lib.REASONS.get(i.first_reason, "unknown"))
+               for i in infos]
         return x, out

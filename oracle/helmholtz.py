@@ -368,34 +368,80 @@ def gmres(operator, precondition, rhs, x, callback=None, restart=32, max_cycles=
 
 
 def bicgstab(operator, precondition, rhs, x, tolerance=1e-3, max_iterations=800):
-    """Right-preconditioned BiCGSTAB, restating nsrno/dense_wave.py:118-148
-    (shadow residual r~ = r0, scalars rho, alpha, omega; the 40-step true
-    residual refresh is omitted).  Returns (x, iterations, reason)."""
-    scale = max(np.linalg.norm(rhs), 1e-30)
+    """Right-preconditioned BiCGSTAB, restating nsrno/dense_wave.py:118-148.
+
+    One Krylov recurrence over the whole array ``rhs`` (all of its entries form
+    one vector, as the reference's shared-angle batch does).  Branches, in the
+    reference's order: convergence test on the recursive ||r|| (:126);
+    breakdown when rho = <r~, r> is not finite or |rho| or |omega| < 1e-30
+    (:127-129); breakdown when |<r~, v>| < 1e-30 (:132-133); the half-step
+    exit when ||s|| is below the tolerance (:135-136); omega with the
+    denominator clamped at 1e-30 (:138); every 40th iteration the residual is
+    recomputed from x and a non-finite or > 1e5 relative residual ends the
+    solve as 'diverged' (:140-142); the final true residual, which upgrades
+    any reason to 'converged' when it meets the tolerance (:143-144).
+    Returns (x, info) with the reference's keys reason, iterations, steps
+    (operator applications), residual, method.
+    """
+    def norm(a):
+        return float(np.linalg.norm(np.ravel(a)))
+
+    def dot(a, b):  # sum(conj(a) * b)
+        return complex(np.vdot(np.ravel(a), np.ravel(b)))
+
+    rhs = np.asarray(rhs)
+    x = np.array(x, dtype=np.result_type(x, rhs, np.complex64), copy=True)
+    scale = max(norm(rhs), 1e-30)
     r = rhs - operator(x)
     shadow = r.copy()
     p = np.zeros_like(r)
     v = np.zeros_like(r)
-    rho_old = alpha = omega = 1.0
+    rho_old = alpha = omega = 1.0 + 0.0j
+    calls, reason, used = 1, "iteration_budget", 0
     for it in range(max_iterations):
-        if np.linalg.norm(r) / scale < tolerance:
-            return x, it, "converged"
-        rho = np.vdot(shadow, r)
+        if norm(r) / scale < tolerance:
+            reason = "converged"
+            break
+        rho = dot(shadow, r)
         if not np.isfinite(rho) or abs(rho) < 1e-30 or abs(omega) < 1e-30:
-            return x, it, "breakdown"
+            reason = "breakdown"
+            break
         beta = (rho / rho_old) * (alpha / omega)
         p = r + beta * (p - omega * v)
         phat = precondition(p)
         v = operator(phat)
-        alpha = rho / np.vdot(shadow, v)
+        calls += 1
+        den = dot(shadow, v)
+        if abs(den) < 1e-30:
+            reason = "breakdown"
+            break
+        alpha = rho / den
         s = r - alpha * v
+        if norm(s) / scale < tolerance:
+            x = x + alpha * phat
+            r = s
+            used = it + 1
+            reason = "converged"
+            break
         shat = precondition(s)
         t = operator(shat)
-        omega = np.vdot(t, s) / max(np.vdot(t, t).real, 1e-300)
+        calls += 1
+        omega = dot(t, s) / max(float(np.sum(np.abs(t) ** 2)), 1e-30)
         x = x + alpha * phat + omega * shat
         r = s - omega * t
         rho_old = rho
-    return x, max_iterations, "iteration_budget"
+        used = it + 1
+        if it % 40 == 39:
+            r = rhs - operator(x)
+            calls += 1
+            if not np.all(np.isfinite(r)) or norm(r) / scale > 1e5:
+                reason = "diverged"
+                break
+    actual = norm(operator(x) - rhs) / scale
+    calls += 1
+    if actual < tolerance:
+        reason = "converged"
+    return x, dict(reason=reason, iterations=used, steps=calls, residual=actual, method="BiCGSTAB")
 
 
 # --------------------------------------------------------------------------

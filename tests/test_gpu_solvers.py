@@ -103,9 +103,9 @@ def test_bicgstab_recurrence_matches_oracle():
     x, infos = plan.solve(torch.from_numpy(b).cuda(), tolerance=1e-14, method="bicgstab", precond="none",
                           max_iterations=6)
     for r in range(2):
-        ref, it, _ = O.bicgstab(lambda u: a @ u, lambda u: u, b[r].ravel(), np.zeros(grid.size, complex),
-                                tolerance=1e-14, max_iterations=6)
-        assert it == 6
+        ref, info = O.bicgstab(lambda u: a @ u, lambda u: u, b[r].ravel(), np.zeros(grid.size, complex),
+                               tolerance=1e-14, max_iterations=6)
+        assert info["iterations"] == 6 and info["reason"] == "iteration_budget"
         assert rel(x[r].cpu().numpy().ravel(), ref) < 1e-4
 
 

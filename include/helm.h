@@ -184,6 +184,35 @@ int32_t helm_sample(helm_plan* plan, int32_t n_ports, const int32_t* port_ptr,
 int32_t helm_solve(helm_plan* plan, const void* rhs, void* x, int32_t nrhs,
                    const helm_solve_opts* opts, helm_solve_info* info, void* stream);
 
+/* ---- reverse-communication FGMRES: replaces nsrno/wave_solver.py:9-36
+ * (gmres(operator, precondition, rhs, x, callback, restart, max_cycles,
+ * tolerance)) for operators and preconditioners the CALLER applies (any
+ * tensor -> tensor code), and the batched cycle of nsrno/dense_wave.py:84-116.
+ * The library keeps the Krylov state of nrhs independent spaces of n complex
+ * entries: FP64 Hessenberg columns and Givens rotations, DCGS2 partial sums;
+ * the caller owns the vectors, all device memory:
+ *   V  c64 [restart + 1][nrhs][n]  basis        Z  c64 [restart][nrhs][n]  directions
+ *   w  c64 [nrhs][n]               A z_j        r, b, x  c128 [nrhs][n]
+ * One cycle: helm_krylov_begin(r = b - A x) -> rel; for j = 0..m-1 the
+ * caller writes Z_j = M^{-1}(V_j) and w = A Z_j, then helm_krylov_step(j);
+ * helm_krylov_end adds Z y to x.  A right-hand side whose Givens estimate
+ * reaches tolerance, or whose new basis vector vanishes (happy breakdown),
+ * stops its cycle early (later steps skip it); convergence is decided on the
+ * true residual r at the next helm_krylov_begin. */
+typedef struct helm_krylov helm_krylov;
+int32_t helm_krylov_create(int64_t n, int32_t nrhs, int32_t restart, helm_krylov** k);
+int32_t helm_krylov_destroy(helm_krylov* k);
+/* V_0 = r / ||r|| per RHS; first != 0 also takes ||b|| (b may be NULL
+ * otherwise).  rel[nrhs] (host) = ||r||/||b||, active[nrhs] (host) =
+ * rel >= tolerance.  Synchronises the stream. */
+int32_t helm_krylov_begin(helm_krylov* k, const void* b, const void* r, void* V, double tolerance, int32_t first,
+                          double* rel, int32_t* active, void* stream);
+/* orthogonalise w = A Z_j against V_0..V_j (DCGS2, FP64 sums), Hessenberg
+ * column and rotation, V_{j+1} = w / ||w||; last != 0 ends the cycle at j */
+int32_t helm_krylov_step(helm_krylov* k, int32_t j, int32_t last, void* V, void* w, double tolerance, void* stream);
+/* x += Z y (FP64); steps (host, may be NULL) = steps used per RHS this cycle */
+int32_t helm_krylov_end(helm_krylov* k, const void* Z, void* x, int32_t* steps, void* stream);
+
 /* ---- adjoint gradient: replaces the material-gradient step of
  * dense_wave.ImplicitChannels.backward (nsrno/dense_wave.py:240-256).
  * u, lam: c128 [nrhs][N] forward and adjoint (A^H lam = P^T g) fields;
@@ -205,6 +234,10 @@ int32_t helm_mg_setup(helm_plan* plan, int32_t levels, double shift, int32_t pre
 /* z = M^{-1} v for nrhs c64 fields (the preconditioner the solve applies). */
 int32_t helm_precond_apply(helm_plan* plan, int32_t precond, const void* v, void* z, int32_t nrhs,
                            void* stream);
+/* the same, adjoint != 0: z = M^{-H} v (what adjoint solves apply; for the
+ * spectral plan, k0^2 M^{-H} preconditions the reference's own operator L) */
+int32_t helm_precond_apply_ex(helm_plan* plan, int32_t precond, const void* v, void* z, int32_t nrhs,
+                              int32_t adjoint, void* stream);
 
 /* ---- learned preconditioner network ----------------------------------
  * A 4-level, 32-channel 3x3-conv U-Net (BF16 MFMA implicit-GEMM kernels, FP32

@@ -80,8 +80,15 @@ SIGNATURES = {
                               c_int32, c_void_p]),
     "helm_solve": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, POINTER(HelmSolveOpts),
                              POINTER(HelmSolveInfo), c_void_p]),
+    "helm_krylov_create": (c_int32, [ctypes.c_int64, c_int32, c_int32, POINTER(c_void_p)]),
+    "helm_krylov_destroy": (c_int32, [c_void_p]),
+    "helm_krylov_begin": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int32, POINTER(c_double),
+                                    POINTER(c_int32), c_void_p]),
+    "helm_krylov_step": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_double, c_void_p]),
+    "helm_krylov_end": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_int32), c_void_p]),
     "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
     "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
+    "helm_precond_apply_ex": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
     "helm_material_sensitivity": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]),
     "helm_net_create": (c_int32, [c_int32, c_int32, c_int32, c_int32, POINTER(c_void_p)]),
     "helm_net_destroy": (c_int32, [c_void_p]),

@@ -196,43 +196,111 @@ def scattered_rhs(system, reference_system, pref, line):
             + reference_system(pref))
 
 
+def _direction(preconditioner, system, adjoint):
+    """The callable a caller's preconditioner stands for.  Objects with a
+    ``setup(features)`` method (the reference's LearnedPreconditioner,
+    conditioned_wave.py:94-117, used as dense_wave.py:199-201 does) get the
+    features namespace (c, rho, alpha as float32 box maps, g, frequency, k0)
+    and return the callable; objects with ``adjoint(features)`` give the
+    adjoint solve's callable; any other callable is used as it is."""
+    from types import SimpleNamespace
+    if hasattr(preconditioner, "setup"):
+        features = SimpleNamespace(c=system.c.float(), rho=system.rho.float(), alpha=system.alpha.float(),
+                                   g=system.g, frequency=system.frequency, k0=system.k0)
+        if adjoint:
+            if not hasattr(preconditioner, "adjoint"):
+                raise ValueError("this preconditioner has no adjoint; adjoint solves take preconditioner=None")
+            return preconditioner.adjoint(features)
+        return preconditioner.setup(features)
+    if callable(preconditioner):
+        return preconditioner
+    raise TypeError("preconditioner must be callable or have a setup(features) method")
+
+
 def solve_system(system, rhs, preconditioner=None, adjoint=False, tolerance=1e-5, max_cycles=300, warm_steps=8,
                  mixed=False, method="gmres", initial=None, restart=8):
     """dense_wave.solve_system (:167-225) on the GPU.
 
-    rhs [R, H, W] (complex64 or complex128).  Every right-hand side runs its
-    own Krylov space to ``tolerance`` on its true FP64 residual, so the batch
-    residual the reference tests (all angles flattened into one vector,
-    dense_wave.py:88-89) is below ``tolerance`` as well.  The preconditioner
-    is the plan's multigrid V-cycle; the reference's learned network is not
-    accepted here (``preconditioner`` must be None, and ``warm_steps`` is then
-    unused, as in the reference).  ``mixed`` only changes the reported dtype:
-    the solve always runs FP32 vectors with FP64 refinement.  FGMRES(8) by
-    default (the reference's GMRES(24) budget ``max_cycles`` x 24 steps is
-    kept: ``max_cycles`` counts cycles of ``restart`` steps, default 300).
-    Returns (value, info) with the reference's info keys.
+    rhs [R, H, W] (complex64 or complex128).  Without a preconditioner every
+    right-hand side runs its own Krylov space to ``tolerance`` on its true
+    FP64 residual (multigrid FGMRES(``restart``) or BiCGSTAB, FP32 vectors,
+    FP64 refinement), so the batch residual the reference tests (all angles
+    flattened into one vector, dense_wave.py:88-89) is below ``tolerance`` as
+    well; a BiCGSTAB right-hand side that ends unconverged is finished by
+    multigrid FGMRES (the reference's fallback to GMRES with the fixed
+    preconditioner, :212-216).
+
+    ``preconditioner`` (a callable on [R, H, W] fields, or an object with
+    ``setup(features)`` such as the reference's LearnedPreconditioner) is used
+    as the reference uses it:
+    * forward solves with ``warm_steps`` > 0 and no ``initial``: a warm start
+      of one GMRES cycle of ``warm_steps`` steps from zero along its
+      directions, all angles one Krylov vector (batched_gmres_cycle,
+      :84-116, :196-201);
+    * method 'gmres_learned', or 'bicgstab' with a preconditioner: flexible
+      GMRES(24) with it throughout (libhelm.so's reverse-communication
+      kernels; the reference's BiCGSTAB with the learned preconditioner is
+      replaced by this), and any angle left unconverged is finished by
+      multigrid FGMRES from that iterate (fallback, :212-216);
+    * method 'gmres': after the warm start the fixed (here: multigrid)
+      preconditioner solves.
+    ``mixed`` only changes the reported dtype: the solve always runs FP32
+    vectors with FP64 refinement.  ``max_cycles`` counts cycles of
+    ``restart`` steps (default 300).  Returns (value, info) with the
+    reference's info keys.
     """
-    if preconditioner is not None:
-        raise ValueError("conditioned solves take the multigrid preconditioner; pass preconditioner=None")
-    if method not in ("gmres", "bicgstab"):
+    if method not in ("gmres", "bicgstab", "gmres_learned"):
         raise ValueError(f"unknown method {method!r}")
     start = time.perf_counter()
     dtype = rhs.dtype
     b = rhs.to(torch.complex128).contiguous()
     R = b.shape[0] if b.dim() == 3 else 1
     b = b.reshape(R, system.g.H, system.g.W)
-    x0 = None if initial is None else initial.to(torch.complex128).reshape(R, system.g.H, system.g.W)
+    x0 = None if initial is None else initial.to(torch.complex128).reshape(R, system.g.H, system.g.W).contiguous()
+    operator = system.adjoint if adjoint else system
+    steps_before, fell = 0, False
+    if preconditioner is not None:
+        from .krylov import fgmres, gmres_cycle
+        direction = _direction(preconditioner, system, adjoint)
+        if warm_steps and not adjoint and initial is None:
+            x0 = gmres_cycle(operator, direction, b, iterations=int(warm_steps)).contiguous()
+            steps_before = int(warm_steps)
+        if method in ("gmres_learned", "bicgstab"):
+            cycles = max(1, (restart * max_cycles) // 24)
+            seen = []
+            xs, reasons = fgmres(operator, direction, b, x0 if x0 is not None else torch.zeros_like(b), restart=24,
+                                 max_cycles=cycles, tolerance=tolerance, batched=True,
+                                 callback=lambda k, xx, r: seen.append(k))
+            x0 = xs.contiguous()
+            steps_before += seen[-1] if seen else 0
+            if all(r == "converged" for r in reasons):
+                residual = operator(x0) - b
+                return _info_result(x0, residual, b, rhs, dtype, "converged", steps_before, len(seen), start, mixed,
+                                    method, tolerance)
+            fell = True  # finished by the multigrid solve below
     x, infos = system.plan.solve(b, x0, tolerance=tolerance, restart=restart, max_iterations=restart * max_cycles,
-                                 precond="mg", method=method, adjoint=adjoint, fallback=method == "bicgstab")
-    residual = system.plan.apply(x, adjoint=adjoint) - b
+                                 precond="mg", method="bicgstab" if method == "bicgstab" and preconditioner is None
+                                 else "gmres", adjoint=adjoint, fallback=method == "bicgstab")
+    residual = operator(x) - b
     relative = float(residual.norm() / b.norm().clamp_min(1e-30))
-    individual = residual.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1).clamp_min(1e-30)
     reason = "converged" if relative < tolerance else next(
         (i.reason for i in infos if i.reason != "converged"), "iteration_budget")
-    info = {"reason": reason, "steps": max(i.iterations for i in infos), "residual": relative,
+    info = _info_result(x, residual, b, rhs, dtype, reason, steps_before + max(i.iterations for i in infos),
+                        max(i.cycles for i in infos), start, mixed, method, tolerance)[1]
+    if fell or any(i.fallback for i in infos):
+        info["fallback"] = True
+    return x.reshape(rhs.shape).to(dtype), info
+
+
+def _info_result(x, residual, b, rhs, dtype, reason, steps, refinements, start, mixed, method, tolerance):
+    relative = float(residual.norm() / b.norm().clamp_min(1e-30))
+    if relative >= tolerance and reason == "converged":
+        reason = "iteration_budget"
+    individual = residual.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1).clamp_min(1e-30)
+    info = {"reason": reason, "steps": int(steps), "residual": relative,
             "max_angle_residual": float(individual.max()), "seconds": time.perf_counter() - start,
             "dtype": "mixed FP32 inverse / FP64 residual" if mixed else str(dtype), "method": method,
-            "refinements": max(i.cycles for i in infos)}
+            "refinements": int(refinements)}
     return x.reshape(rhs.shape).to(dtype), info
 
 
@@ -313,15 +381,21 @@ class ImplicitChannels(torch.autograd.Function):
     converged forward solve times the receiver response; backward solves the
     adjoint system with the channel gradient injected at the receivers and
     forms the material gradient of -Re <lam, M (A (u + pref) - s)> on the GPU
-    (alpha fixed).  Non-convergence raises RuntimeError."""
+    (alpha fixed).  With a preconditioner the solves take method 'bicgstab'
+    and fall back to a solve without it when they do not converge, as the
+    reference does (:234-235, :247-249).  Non-convergence raises
+    RuntimeError."""
 
     @staticmethod
-    def forward(ctx, medium, alpha, case, tolerance):
+    def forward(ctx, medium, alpha, case, preconditioner, tolerance):
         system, rhs = case.assemble(medium.detach().double(), alpha.detach().double())
-        u, info = solve_system(system, rhs, tolerance=tolerance, mixed=True)
+        method = "bicgstab" if preconditioner is not None else "gmres"
+        u, info = solve_system(system, rhs, preconditioner, tolerance=tolerance, mixed=True, method=method)
+        if info["reason"] != "converged" and preconditioner is not None:
+            u, info = solve_system(system, rhs, None, tolerance=tolerance, mixed=True, method="gmres")
         if info["reason"] != "converged":
             raise RuntimeError("Forward failed to converge: " + str(info))
-        ctx.case, ctx.tolerance, ctx.info = case, tolerance, info
+        ctx.case, ctx.tolerance, ctx.info, ctx.preconditioner = case, tolerance, info, preconditioner
         ctx.save_for_backward(medium, alpha, u)
         return channels(case.g, u) * receiver_response(case.frequency)
 
@@ -333,12 +407,18 @@ class ImplicitChannels(torch.autograd.Function):
         system, _ = case.assemble(medium.detach().double(), alpha.detach().double())
         rhs = torch.zeros_like(u)
         rhs[..., g.face, g.rx_cols] = gradient.to(torch.complex128) * receiver_response(case.frequency)
-        lam, info = solve_system(system, rhs, adjoint=True, tolerance=ctx.tolerance, mixed=True)
+        pre = ctx.preconditioner if ctx.preconditioner is not None and hasattr(ctx.preconditioner, "adjoint") else None
+        lam, info = solve_system(system, rhs, pre, adjoint=True, tolerance=ctx.tolerance, mixed=True,
+                                 method="bicgstab" if pre is not None else "gmres")
+        if info["reason"] != "converged" and pre is not None:
+            lam, info = solve_system(system, rhs, None, adjoint=True, tolerance=ctx.tolerance, mixed=True)
         if info["reason"] != "converged":
             raise RuntimeError("Adjoint failed to converge: " + str(info))
         grad = material_gradient(system, u + case.pref, lam, case.lines)
-        return grad.to(medium.dtype), None, None, None
+        return grad.to(medium.dtype), None, None, None, None
 
 
-def implicit_channels(medium, alpha, case, tolerance=1e-5):
-    return ImplicitChannels.apply(medium, alpha, case, tolerance)
+def implicit_channels(medium, alpha, case, preconditioner=None, tolerance=1e-5):
+    """dense_wave.implicit_channels (:258-259); ``case`` (DenseCase) stands
+    for the reference's (data, bin_index, angles)."""
+    return ImplicitChannels.apply(medium, alpha, case, preconditioner, tolerance)

@@ -889,6 +889,11 @@ __global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, 
     x[(int64_t)r * n + p] = acc;
 }
 
+__global__ void k_bi_izero_k(int count, int* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
 // stop right-hand side r (stagnation): no further cycles run for it
 __global__ void k_stop_rhs(int r, int* __restrict__ active, int* __restrict__ cyc) {
     active[r] = 0;
@@ -1378,9 +1383,170 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     return HELM_OK;
 }
 
+// ---- reverse-communication FGMRES (helm_krylov_*) ---------------------------
+// The Krylov state of the fused solver without a plan: the caller applies the
+// operator and the preconditioner itself (any code, any stream order) and
+// the library keeps the Hessenberg columns, the Givens rotations and the
+// reduction partials.  V (m + 1 vectors), Z (m) and W (1) are caller-owned
+// c64 arrays [k][nrhs][n]; x and residuals are c128 [nrhs][n].
+struct KrylovState {
+    int64_t n = 0;
+    int nrhs = 0, m = 0;
+    Workspace W;  // only the small per-RHS arrays and the partials are used
+    std::vector<double> h_rel;
+    std::vector<int> h_act;
+};
+
+static int32_t krylov_alloc(KrylovState& k) {
+    Workspace& w = k.W;
+    const int64_t n = k.n;
+    const int nrhs = k.nrhs, restart = k.m, kmax = restart + 1;
+    w.nrhs = nrhs;
+    w.restart = restart;
+    w.n = n;
+    w.partial_len = (int64_t)nrhs * (2 * kmax * ndotchunks_max(n) + nchunks(n));
+    HELM_TRY(alloc(&w.partial, (size_t)w.partial_len));
+    HELM_TRY(alloc(&w.H, (size_t)nrhs * restart * (restart + 1)));
+    HELM_TRY(alloc(&w.cs, (size_t)nrhs * restart));
+    HELM_TRY(alloc(&w.sn, (size_t)nrhs * restart));
+    HELM_TRY(alloc(&w.g, (size_t)nrhs * (restart + 1)));
+    HELM_TRY(alloc(&w.coef, (size_t)nrhs * (2 * kmax + 1)));
+    HELM_TRY(alloc(&w.beta, nrhs));
+    HELM_TRY(alloc(&w.bnorm, nrhs));
+    HELM_TRY(alloc(&w.rel, nrhs));
+    HELM_TRY(alloc(&w.inv, nrhs));
+    HELM_TRY(alloc(&w.est, nrhs));
+    HELM_TRY(alloc(&w.active, nrhs));
+    HELM_TRY(alloc(&w.cyc, nrhs));
+    HELM_TRY(alloc(&w.used, nrhs));
+    HELM_TRY(alloc(&w.iters, nrhs));
+    k.h_rel.assign(nrhs, 0.0);
+    k.h_act.assign(nrhs, 0);
+    return HELM_OK;
+}
+
+static void krylov_free(KrylovState& k) {
+    Workspace& w = k.W;
+    void* bufs[] = {w.partial, w.H, w.cs, w.sn, w.g, w.coef, w.beta, w.bnorm, w.rel, w.inv, w.est, w.active,
+                    w.cyc, w.used, w.iters};
+    for (void* q : bufs)
+        if (q) (void)hipFree(q);
+}
+
 }  // namespace helm
 
+struct helm_krylov {
+    helm::KrylovState st;
+};
+
 using namespace helm;
+
+extern "C" int32_t helm_krylov_create(int64_t n, int32_t nrhs, int32_t restart, helm_krylov** out) {
+    HELM_CHECK(out, HELM_ERR_ARG, "helm_krylov_create: null argument");
+    *out = nullptr;
+    HELM_CHECK(n >= 1 && nrhs >= 1 && nrhs <= 65535 && n * nrhs < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_krylov_create: n and nrhs out of range");
+    HELM_CHECK(restart >= 1 && restart <= 63, HELM_ERR_ARG, "helm_krylov_create: restart must be in [1, 63]");
+    helm_krylov* k = new helm_krylov();
+    k->st.n = n;
+    k->st.nrhs = nrhs;
+    k->st.m = restart;
+    const int32_t st = krylov_alloc(k->st);
+    if (st != HELM_OK) {
+        krylov_free(k->st);
+        delete k;
+        return st;
+    }
+    *out = k;
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_krylov_destroy(helm_krylov* k) {
+    if (!k) return HELM_OK;
+    krylov_free(k->st);
+    delete k;
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_krylov_begin(helm_krylov* k, const void* b, const void* r, void* V, double tolerance,
+                                     int32_t first, double* rel, int32_t* active, void* stream) {
+    HELM_CHECK(k && r && V && rel && active && (b || !first), HELM_ERR_ARG, "helm_krylov_begin: null argument");
+    HELM_CHECK(std::isfinite(tolerance) && tolerance > 0, HELM_ERR_ARG, "helm_krylov_begin: bad tolerance");
+    KrylovState& K = k->st;
+    Workspace& W = K.W;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = K.n;
+    const int nrhs = K.nrhs, nch = nchunks(n);
+    const dim3 blk(256);
+    if (first) {
+        hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, (const double2*)b, W.partial);
+        hipLaunchKernelGGL(k_bnorm, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bnorm, W.active, W.iters);
+        hipLaunchKernelGGL(k_bi_izero_k, dim3((nrhs + 255) / 256), blk, 0, s, nrhs, W.used);
+    }
+    hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), blk, 0, s, n, nch, (const double2*)r, W.partial);
+    hipLaunchKernelGGL(k_cycle_init, dim3(nrhs), dim3(64), 0, s, nch, K.m, tolerance, W.partial, W.bnorm, W.beta,
+                       W.rel, W.inv, W.active, W.cyc, W.used, W.iters, W.g, (double*)nullptr, (double*)nullptr,
+                       (int*)nullptr, (int*)nullptr, 0);
+    hipLaunchKernelGGL(k_scale_to32, dim3((unsigned)((n + 255) / 256), nrhs), blk, 0, s, n, (const double2*)r,
+                       (float2*)V, W.inv, W.cyc);
+    HELM_HIP(hipGetLastError());
+    HELM_HIP(hipMemcpyAsync(K.h_rel.data(), W.rel, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+    HELM_HIP(hipMemcpyAsync(K.h_act.data(), W.active, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
+    HELM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < nrhs; ++i) {
+        rel[i] = K.h_rel[i];
+        active[i] = K.h_act[i];
+    }
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_krylov_step(helm_krylov* k, int32_t j, int32_t last, void* V, void* w, double tolerance,
+                                    void* stream) {
+    HELM_CHECK(k && V && w, HELM_ERR_ARG, "helm_krylov_step: null argument");
+    KrylovState& K = k->st;
+    HELM_CHECK(j >= 0 && j < K.m, HELM_ERR_ARG, "helm_krylov_step: step index out of range");
+    Workspace& W = K.W;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = K.n;
+    const int nrhs = K.nrhs, nch = nchunks(n), nchd = ndotchunks(n), kmax = K.m + 1, mm = K.m;
+    const dim3 blk(256);
+    const int64_t vstride = (int64_t)nrhs * n;
+    double2* pd_part = W.partial + (int64_t)nrhs * kmax * nchd;
+    double2* pw2 = W.partial + (int64_t)2 * nrhs * kmax * nchd;
+    float2* Vb = (float2*)V;
+    float2* wj = (float2*)w;
+    const bool fin = last || j + 1 >= K.m;
+    hipLaunchKernelGGL(dot_npt() == 4 ? k_dcgs_dots<4> : k_dcgs_dots<8>, dim3(nchd, nrhs), blk, 0, s, n, nrhs, nchd,
+                       kmax, j, (const float2*)Vb, (const float2*)wj, (const int*)W.cyc, W.partial, pd_part);
+    hipLaunchKernelGGL(k_dcgs_reduce_update, dim3(nch, nrhs), blk, 0, s, n, nrhs, nch, nchd, mm, kmax, j, tolerance,
+                       (const double2*)W.partial, (const double2*)pd_part, (const double*)W.bnorm, W.H, W.cs, W.sn,
+                       W.g, W.est, W.cyc, W.used, Vb, wj, pw2, 0);
+    hipLaunchKernelGGL(k_dcgs_norm_scale, dim3(fin ? 1 : nch, nrhs), blk, 0, s, n, nch, mm, j, (int)fin,
+                       (const double2*)pw2, (const double*)W.bnorm, W.H, W.cs, W.sn, W.g, W.est, W.inv, W.cyc,
+                       W.used, (const float2*)wj, Vb + (int64_t)(fin ? j : j + 1) * vstride, (double*)nullptr);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+extern "C" int32_t helm_krylov_end(helm_krylov* k, const void* Z, void* x, int32_t* steps, void* stream) {
+    HELM_CHECK(k && Z && x, HELM_ERR_ARG, "helm_krylov_end: null argument");
+    KrylovState& K = k->st;
+    Workspace& W = K.W;
+    hipStream_t s = (hipStream_t)stream;
+    const int nrhs = K.nrhs, kmax = K.m + 1;
+    hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, K.m, kmax, W.H, W.g, W.used,
+                       W.coef);
+    hipLaunchKernelGGL(k_xupdate, dim3((unsigned)((K.n + 255) / 256), nrhs), dim3(256), 0, s, K.n, nrhs, kmax,
+                       (const float2*)Z, W.coef, W.used, (double2*)x);
+    HELM_HIP(hipGetLastError());
+    if (steps) {
+        std::vector<int> u(nrhs);
+        HELM_HIP(hipMemcpyAsync(u.data(), W.used, nrhs * sizeof(int), hipMemcpyDeviceToHost, s));
+        HELM_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < nrhs; ++i) steps[i] = u[i];
+    }
+    return HELM_OK;
+}
 
 extern "C" int32_t helm_solve(helm_plan* p, const void* rhs, void* x, int32_t nrhs, const helm_solve_opts* o,
                               helm_solve_info* info, void* stream) {

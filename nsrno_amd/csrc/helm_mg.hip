@@ -1699,3 +1699,12 @@ extern "C" int32_t helm_precond_apply(helm_plan* p, int32_t precond, const void*
     HELM_CHECK(v != z, HELM_ERR_ARG, "helm_precond_apply: v and z must not alias");
     return precond_apply(p, precond, (const float2*)v, (float2*)z, nrhs, false, (hipStream_t)stream);
 }
+
+extern "C" int32_t helm_precond_apply_ex(helm_plan* p, int32_t precond, const void* v, void* z, int32_t nrhs,
+                                         int32_t adjoint, void* stream) {
+    HELM_CHECK(p && v && z, HELM_ERR_ARG, "helm_precond_apply_ex: null argument");
+    HELM_CHECK(p->medium_set, HELM_ERR_STATE, "helm_precond_apply_ex: call helm_set_medium first");
+    HELM_CHECK(nrhs >= 1 && nrhs <= 65535, HELM_ERR_ARG, "helm_precond_apply_ex: nrhs out of range");
+    HELM_CHECK(v != z, HELM_ERR_ARG, "helm_precond_apply_ex: v and z must not alias");
+    return precond_apply(p, precond, (const float2*)v, (float2*)z, nrhs, adjoint != 0, (hipStream_t)stream);
+}

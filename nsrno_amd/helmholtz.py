@@ -137,7 +137,12 @@ class HelmholtzPlan:
     def apply(self, u, adjoint=False, out=None):
         """A u (or A^H u) for fields [..., nz, nx]: FP32 for complex64, FP64 for complex128."""
         nrhs = self._check_field(u)
-        out = torch.empty_like(u) if out is None else out
+        if out is None:
+            out = torch.empty_like(u)
+        else:
+            self._check_field(out)
+            if out.shape != u.shape or out.dtype != u.dtype or out.device != u.device:
+                raise ValueError("out must match u in shape, dtype and device")
         prec = 32 if u.dtype == torch.complex64 else 64
         check(lib().helm_apply(self._h, _ptr(u), _ptr(out), nrhs, prec, int(bool(adjoint)), _stream()),
               "helm_apply")
@@ -174,12 +179,14 @@ class HelmholtzPlan:
                                   float(omega), _stream()), "helm_mg_setup")
         self.mg_levels = int(levels)
 
-    def precondition(self, v, kind=_lib.PC_MG):
+    def precondition(self, v, kind=_lib.PC_MG, adjoint=False):
+        """z = M^{-1} v (adjoint: M^{-H} v) for complex64 fields [..., nz, nx]."""
         nrhs = self._check_field(v)
         if v.dtype != torch.complex64:
             raise ValueError("preconditioner input must be complex64")
         z = torch.empty_like(v)
-        check(lib().helm_precond_apply(self._h, int(kind), _ptr(v), _ptr(z), nrhs, _stream()), "helm_precond_apply")
+        check(lib().helm_precond_apply_ex(self._h, int(kind), _ptr(v), _ptr(z), nrhs, int(bool(adjoint)), _stream()),
+              "helm_precond_apply_ex")
         return z
 
     # ------------------------------------------------------------ kernel timing
@@ -231,6 +238,8 @@ class HelmholtzPlan:
         """`steps` minimal-residual preconditioned steps on x in place (complex128 [R, nz, nx])."""
         if x.dtype != torch.complex128 or rhs.dtype != torch.complex128 or not x.is_contiguous():
             raise ValueError("warm_start takes contiguous complex128 rhs and x")
+        if x.device.type != "cuda":
+            raise ValueError("warm_start takes GPU tensors")
         nrhs = self._check_field(rhs)
         if x.shape != rhs.shape:
             raise ValueError("x and rhs must have the same shape")

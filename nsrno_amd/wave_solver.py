@@ -12,9 +12,10 @@ FP64 true-residual refinement and the multigrid preconditioner, so
 below the complex64 rounding floor are reached.
 
 ``gmres`` keeps the reference's callable-based signature
-(wave_solver.py:9-36) for operators built by this package
-(``PlanOperator``); ``solve_frequency_batch`` solves several angles of one
-frequency as one batched solve (shared coefficients, one launch sequence).
+(wave_solver.py:9-36) for any caller operator and preconditioner (this
+package's are ``PlanOperator`` and ``PlanPreconditioner``);
+``solve_frequency_batch`` solves several angles of one frequency as one
+batched solve (shared coefficients, one launch sequence).
 """
 from __future__ import annotations
 
@@ -35,30 +36,37 @@ class PlanOperator:
         return self.model.apply_operator(u, self.data)
 
 
+class PlanPreconditioner:
+    """r -> k0^2 M^{-H} r: the plan's multigrid V-cycle as a preconditioner of
+    the reference's operator L (the plan holds A = k0^2 conj(L), so
+    L^{-1} = k0^2 (A^H)^{-1}); replaces the Fourier shifted-Laplacian
+    ``precondition`` closure of wave_solver.py:58-61."""
+
+    def __init__(self, model: WaveResidualOperator, data):
+        self.model, self.data = model, data
+
+    def __call__(self, r):
+        v = r.to(self.model.device, torch.complex64).contiguous()
+        z = self.data.plan.precondition(v, adjoint=True) * self.data.k0 ** 2
+        return z.to(r.dtype)
+
+
 def gmres(operator, precondition, rhs, x, callback, restart=32, max_cycles=80, tolerance=1e-4):
     """Right-preconditioned restarted GMRES (wave_solver.py:9-36) -> (x, reason).
 
-    ``operator`` must be a PlanOperator; ``precondition`` is ignored in favour
-    of the plan's multigrid (None or any callable is accepted).  ``callback``
-    (applications, x, relative_residual) fires once per restart with the true
-    FP64 relative residual; ``x`` passed to it is the current iterate.
+    ``operator`` and ``precondition`` are any tensor -> tensor callables on
+    tensors of ``rhs``'s shape and dtype (``precondition=None``: none), as in
+    the reference; ``PlanOperator`` / ``PlanPreconditioner`` give this
+    package's operator and multigrid.  The whole ``rhs`` is one Krylov vector
+    (the reference's semantics); libhelm.so's reverse-communication kernels
+    orthogonalise (DCGS2, FP64 sums), solve the least squares (FP64 Givens)
+    and accumulate x in complex128.  ``callback(applications, x,
+    relative_residual)`` fires at every restart with the current iterate and
+    the true relative residual; reason is 'converged' or 'iteration_budget'.
     """
-    if not isinstance(operator, PlanOperator):
-        raise TypeError("gmres takes a PlanOperator built from WaveResidualOperator.setup")
-    model, data = operator.model, operator.data
-    k0sq = data.k0 ** 2
-    b = (rhs.to(model.device, torch.complex128) * k0sq).contiguous()
-    x0 = x.to(model.device, torch.complex128).contiguous()
-    calls = [0]
-
-    def cb(applications, rel):
-        calls[0] += 1
-        callback(applications - calls[0], None, rel[0])
-
-    xs, info = data.plan.solve(b.reshape(1, *b.shape[-2:]), x0.reshape(1, *b.shape[-2:]), tolerance=tolerance,
-                               restart=restart, max_iterations=restart * max_cycles, precond="mg", adjoint=True,
-                               callback=cb)
-    return xs.reshape(rhs.shape).to(rhs.dtype), info[0].reason
+    from .krylov import fgmres
+    pre = precondition if precondition is not None else (lambda r: r)
+    return fgmres(operator, pre, rhs, x, callback, restart=restart, max_cycles=max_cycles, tolerance=tolerance)
 
 
 def _check(medium, max_iterations, restart):

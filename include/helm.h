@@ -306,6 +306,18 @@ int32_t helm_set_medium_spectral(helm_plan* plan, const float* c, const float* r
  * Needs helm_set_medium_spectral first (geometry and k0). */
 int32_t helm_spectral_incident(helm_plan* plan, double sin_theta, double aperture_m, void* field,
                                void* stream);
+/* No counterpart in the reference: helm_spectral_incident for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and output, bit
+ * for bit; calls no hipMalloc/hipFree.  Its work buffer is a file-scope
+ * static device array of 4 x 4096 complex doubles (256 KiB per GPU, held
+ * until the process ends), so boxes wider than 4096 nodes are rejected with
+ * HELM_ERR_ARG.  Every call on one GPU shares that GPU's buffer: calls that
+ * overlap on one GPU (two streams, two threads) race and must be ordered by
+ * the caller.  The first call on a device resolves the buffer's address
+ * (hipGetSymbolAddress), which allocates nothing. */
+int32_t helm_spectral_incident_noalloc(helm_plan* plan, double sin_theta, double aperture_m, void* field,
+                                       void* stream);
 /* Scattered-field right-hand side of the reference's solve, scaled for
  * helm_solve(adjoint = 1): rhs = -k0^2 (L - L0) u_inc, where L0 has
  * ax = az = 1 and kappa = 1 - i sponge (wave_operator.py:142-143,

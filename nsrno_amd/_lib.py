@@ -104,6 +104,7 @@ SIGNATURES = {
     "helm_net_features": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_set_medium_spectral": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HelmSpectralMedium), c_void_p]),
     "helm_spectral_incident": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
+    "helm_spectral_incident_noalloc": (c_int32, [c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "helm_spectral_rhs": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_autotune": (c_int32, [c_void_p, c_int32, POINTER(HelmTuning), c_void_p]),
     "helm_set_stencil_shape": (c_int32, [c_void_p, c_int32, c_int32, c_int32]),

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <mutex>
+
 #include "helm_internal.h"
 
 namespace helm {
@@ -423,6 +425,39 @@ __global__ __launch_bounds__(256) void k_inc_field(int H, int W, int pz, double 
     }
 }
 
+// Scratch of helm_spectral_incident_noalloc: the incident-field work buffer
+// (twiddles, boundary trace, k_z, boundary spectrum: 4 W complex doubles) for
+// boxes up to 256 * INC_MAXX = 4096 nodes wide, 256 KiB per device, held
+// until the process ends.  Host code must not pass `scratch` itself to a
+// kernel (on the host it names a stand-in, not device memory): the device
+// address of each GPU's copy comes from hipGetSymbolAddress, once per device
+// with that device current, and is kept in scratch_ptr[device].
+constexpr int INC_SCRATCH = 4 * 256 * INC_MAXX;
+static __device__ __align__(256) double2 scratch[INC_SCRATCH];
+constexpr int MAX_DEVICES = 64;
+static double2* scratch_ptr[MAX_DEVICES] = {};
+static std::mutex scratch_mutex;
+
+static int32_t scratch_address(int device, double2** out) {
+    HELM_CHECK(device >= 0 && device < MAX_DEVICES, HELM_ERR_ARG, "no-alloc scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        int current = 0;
+        HELM_HIP(hipGetDevice(&current));
+        if (current != device) HELM_HIP(hipSetDevice(device));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (current != device) HELM_HIP(hipSetDevice(current));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress (incident scratch): ") + hipGetErrorString(e));
+            return HELM_ERR_HIP;
+        }
+        scratch_ptr[device] = (double2*)ptr;
+    }
+    *out = scratch_ptr[device];
+    return HELM_OK;
+}
+
 template <typename T>
 static int32_t sp_alloc(T** ptr, size_t count) {
     if (*ptr) return HELM_OK;
@@ -432,6 +467,20 @@ static int32_t sp_alloc(T** ptr, size_t count) {
         set_error(std::string("hipMalloc (spectral operator): ") + hipGetErrorString(e));
         return HELM_ERR_ALLOC;
     }
+    return HELM_OK;
+}
+
+// the three incident-field kernels on the work buffer buf (4 W complex doubles)
+static int32_t launch_incident(helm_plan* p, double sin_theta, double aperture_m, double2* buf, double2* field,
+                               hipStream_t s) {
+    const Level& L = p->levels[0];
+    const int W = L.nx, H = L.nz;
+    const helm_spectral_medium& m = p->spectral;
+    hipLaunchKernelGGL(k_inc_prepare, dim3((W + 255) / 256), dim3(256), 0, s, W, m.px, L.h_m, m.x0_m, p->k0,
+                       sin_theta, aperture_m, buf);
+    hipLaunchKernelGGL(k_inc_spectrum, dim3((W + 255) / 256), dim3(256), 0, s, W, buf);
+    hipLaunchKernelGGL(k_inc_field, dim3(H), dim3(256), 0, s, H, W, m.pz, L.h_m, m.z0_m, buf, field);
+    HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
 
@@ -509,14 +558,23 @@ int32_t helm_spectral_incident(helm_plan* p, double sin_theta, double aperture_m
     HELM_CHECK(W <= 256 * INC_MAXX, HELM_ERR_ARG, "helm_spectral_incident: box wider than 4096 nodes");
     hipStream_t s = (hipStream_t)stream;
     HELM_TRY(sp_alloc(&p->sp_buf, (size_t)4 * W));
-    const helm_spectral_medium& m = p->spectral;
-    hipLaunchKernelGGL(k_inc_prepare, dim3((W + 255) / 256), dim3(256), 0, s, W, m.px, L.h_m, m.x0_m, p->k0,
-                       sin_theta, aperture_m, p->sp_buf);
-    hipLaunchKernelGGL(k_inc_spectrum, dim3((W + 255) / 256), dim3(256), 0, s, W, p->sp_buf);
-    hipLaunchKernelGGL(k_inc_field, dim3(H), dim3(256), 0, s, H, W, m.pz, L.h_m, m.z0_m, p->sp_buf,
-                       (double2*)field);
-    HELM_HIP(hipGetLastError());
-    return HELM_OK;
+    return launch_incident(p, sin_theta, aperture_m, p->sp_buf, (double2*)field, s);
+}
+
+int32_t helm_spectral_incident_noalloc(helm_plan* p, double sin_theta, double aperture_m, void* field,
+                                       void* stream) {
+    HELM_CHECK(p && field, HELM_ERR_ARG, "helm_spectral_incident_noalloc: null argument");
+    HELM_CHECK(p->desc.op_kind == HELM_OP_STAG8 && p->medium_set, HELM_ERR_STATE,
+               "helm_spectral_incident_noalloc: call helm_set_medium_spectral first");
+    HELM_CHECK(std::isfinite(sin_theta) && std::isfinite(aperture_m) && aperture_m > 0, HELM_ERR_ARG,
+               "helm_spectral_incident_noalloc: invalid angle or aperture");
+    const int W = p->levels[0].nx;
+    HELM_CHECK(W >= 1 && 4 * W <= INC_SCRATCH, HELM_ERR_ARG,
+               "helm_spectral_incident_noalloc: box wider than 4096 nodes (the static scratch holds 4 x 4096 "
+               "complex doubles)");
+    double2* buf = nullptr;
+    HELM_TRY(scratch_address(p->device, &buf));
+    return launch_incident(p, sin_theta, aperture_m, buf, (double2*)field, (hipStream_t)stream);
 }
 
 int32_t helm_spectral_rhs(helm_plan* p, const void* inc, void* rhs, int32_t nrhs, void* stream) {

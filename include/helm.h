@@ -132,6 +132,8 @@ typedef struct {
     double relative_residual;  /* true FP64 ||b - A x|| / ||b|| at return    */
     int32_t fallback;          /* 1: finished by the fallback solver          */
     int32_t first_reason;      /* reason of the first solver (before any fallback) */
+    int32_t graph_fallbacks;   /* restart cycles run directly because their graph
+                                  capture failed (use_graph; 0 otherwise)     */
 } helm_solve_info;
 
 /* ---- library ---------------------------------------------------------- */

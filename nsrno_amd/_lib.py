@@ -51,7 +51,8 @@ class HelmSolveOpts(ctypes.Structure):
 
 class HelmSolveInfo(ctypes.Structure):
     _fields_ = [("reason", c_int32), ("iterations", c_int32), ("applications", c_int32), ("cycles", c_int32),
-                ("relative_residual", c_double), ("fallback", c_int32), ("first_reason", c_int32)]
+                ("relative_residual", c_double), ("fallback", c_int32), ("first_reason", c_int32),
+                ("graph_fallbacks", c_int32)]
 
 
 class HelmTuning(ctypes.Structure):

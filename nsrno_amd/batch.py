@@ -38,6 +38,9 @@ class FrequencyCase:
     traces: np.ndarray | None = None
     krylov_bytes: float = 0.0
     tuning: dict | None = None
+    graph_fallbacks: int = 0    # restart cycles run directly because their graph capture failed
+    fields: np.ndarray | None = None  # complex128 [R, n, n] when asked for (keep_fields)
+    setup: dict | None = None   # grid, medium and acquisition of the solve (keep_fields)
 
 
 class PhysicalMedium:
@@ -84,8 +87,12 @@ def _mg_levels(n, coarsest=40):
 
 
 def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False,
-               mg_shift=0.4):
-    """One frequency: all angles as one batched multigrid-FGMRES solve."""
+               mg_shift=0.4, keep_fields=False, plans=None):
+    """One frequency: all angles as one batched multigrid-FGMRES solve.
+
+    plans: a list that receives the plan instead of closing it here (run_sweep
+    closes them after every worker has finished, so that no plan's memory is
+    freed while another thread captures a restart cycle)."""
     p = medium.pml
     n = case.interior + 2 * p
     c, rho = medium.sample(case)
@@ -112,19 +119,29 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     case.relative_residual = max(i.relative_residual for i in info)
     case.reasons = tuple(sorted(set(i.reason for i in info)))
     case.traces = traces.cpu().numpy()
+    case.graph_fallbacks = max(i.graph_fallbacks for i in info)
+    if keep_fields:
+        case.fields = x.cpu().numpy()
+        case.setup = dict(work=work, weights=weights, amplitudes=amp)
     # algorithmic Arnoldi + operator bytes (DCGS2 step j: 8(j+2) + 8(j+4); stencil 16 + 12/R per node per RHS)
     R = len(angles)
     steps = max(case.iterations)
     js = np.arange(steps) % restart
     case.krylov_bytes = float(np.sum(16.0 * js + 48.0 + 16.0 + 12.0 / R) * n * n * R)
-    plan.close()
+    if plans is None:
+        plan.close()
+    else:
+        plans.append(plan)
     return case
 
 
 def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, autotune=True, graph=False,
-              mg_shift=0.4):
+              mg_shift=0.4, keep_fields=()):
     """Solve every frequency (largest grids first) on `workers` concurrent
-    streams.  Returns (cases in input order, wall seconds)."""
+    streams.  Returns (cases in input order, wall seconds).  keep_fields:
+    indices of the frequencies whose fields are copied to the host.  The
+    plans stay allocated until every worker has finished (a few GB for a
+    64-frequency sweep) and are closed after the timed region."""
     medium = PhysicalMedium(seed=seed)
     cases = [medium.case(f) for f in frequencies]
     order = sorted(range(len(cases)), key=lambda k: -cases[k].interior)
@@ -132,6 +149,8 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
     for k in order:
         work.put(k)
     errors = []
+    plans = []
+    keep = set(keep_fields)
 
     def worker():
         stream = torch.cuda.Stream()
@@ -143,7 +162,7 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
                     return
                 try:
                     solve_case(medium, cases[k], angles, tol, restart, stream=stream, autotune=autotune, graph=graph,
-                               mg_shift=mg_shift)
+                               mg_shift=mg_shift, keep_fields=k in keep, plans=plans)
                     if log:
                         c = cases[k]
                         log(f"{c.frequency_mhz:.3f} MHz {c.interior}^2 steps {max(c.iterations)} "
@@ -161,6 +180,8 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
         t.join()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    for p in plans:
+        p.close()
     if errors:
         raise errors[0]
     return cases, wall

@@ -519,6 +519,7 @@ int32_t solve_bicgstab(helm_plan* p, const double2* b, double2* x, int nrhs, con
         info[r].reason = reason;
         info[r].first_reason = reason;
         info[r].fallback = 0;
+        info[r].graph_fallbacks = 0;
         info[r].iterations = its[r];
         info[r].applications = apps[r] + outer + 1;  // + one FP64 true residual per outer pass
         info[r].cycles = outer;

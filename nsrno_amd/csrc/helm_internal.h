@@ -186,6 +186,7 @@ struct Workspace {
     hipGraphExec_t gexec = nullptr;
     uint64_t gkey[8] = {};
     hipStream_t gstream = nullptr;  // capture stream (capture is not allowed on the null stream)
+    int graph_fallbacks = 0;        // cycles of the current solve enqueued directly after a failed capture
     // host mirrors
     std::vector<double> h_rel;
     std::vector<int> h_active;

@@ -1090,6 +1090,7 @@ static int32_t cycle_graph(Workspace& W, const uint64_t key[8], F&& enqueue, int
             if (W.gexec) (void)hipGraphExecDestroy(W.gexec);
             W.gexec = nullptr;
             (void)hipGetLastError();
+            W.graph_fallbacks++;
             return enqueue(m, s);
         }
         std::memcpy(W.gkey, key, sizeof(W.gkey));
@@ -1132,6 +1133,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
 
     int steps = 0, cycles = 0, applications = 0;
     bool budget = false;
+    W.graph_fallbacks = 0;
     Profiler* pf = (p->prof && p->prof->on) ? p->prof : nullptr;
     const double nn = (double)n, rr = (double)nrhs;
     const double coef_bytes32 = L.op != HELM_OP_FLUX5 ? 16.0 : 12.0;  // per node, FP32 coefficients
@@ -1374,6 +1376,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
         if (stopped[r]) info[r].reason = HELM_STAGNATION;
         info[r].fallback = fell[r];
         info[r].first_reason = fell[r] ? HELM_STAGNATION : info[r].reason;
+        info[r].graph_fallbacks = W.graph_fallbacks;
         info[r].iterations = W.h_iters[r];
         info[r].applications = W.h_iters[r] + cycles + 1;
         info[r].cycles = cycles;
@@ -1599,6 +1602,7 @@ extern "C" int32_t helm_solve(helm_plan* p, const void* rhs, void* x, int32_t nr
         info[r].applications += gi[r].applications;
         info[r].cycles += gi[r].cycles;
         info[r].relative_residual = gi[r].relative_residual;
+        info[r].graph_fallbacks = gi[r].graph_fallbacks;
     }
     return HELM_OK;
 }

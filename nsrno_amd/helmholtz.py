@@ -37,6 +37,7 @@ class SolveInfo:
     relative_residual: float
     fallback: bool = False      # finished by the fallback solver (multigrid FGMRES)
     first_reason: str = ""      # reason of the first solver, before any fallback
+    graph_fallbacks: int = 0    # restart cycles run directly because their graph capture failed
 
 
 class Ports:
@@ -289,6 +290,6 @@ class HelmholtzPlan:
         check(lib().helm_solve(self._h, _ptr(rhs), _ptr(x), nrhs, ctypes.byref(opts), infos, _stream()),
               "helm_solve")
         out = [SolveInfo(_lib.REASONS.get(i.reason, "unknown"), i.iterations, i.applications, i.cycles,
-                         i.relative_residual, bool(i.fallback), _lib.REASONS.get(i.first_reason, "unknown"))
-               for i in infos]
+                         i.relative_residual, bool(i.fallback), _lib.REASONS.get(i.first_reason, "unknown"),
+                         int(i.graph_fallbacks)) for i in infos]
         return x, out

@@ -83,10 +83,12 @@ def parse():
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
     ap.add_argument("--cond-restart", type=int, default=6, help="FGMRES restart length of the conditioned workload")
-    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned", "spectral", "net"],
+    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned", "spectral", "net", "learned"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
                          "64 frequencies x 11 angles on uneven grids; conditioned: the split-PML system; "
-                         "spectral: the S operator on a 1056x1216 box; net: the learned U-Net alone")
+                         "spectral: the S operator on a 1056x1216 box; net: the learned U-Net alone; "
+                         "learned: config 5 (learned vs multigrid vs plain on held-out media, 3-9 MHz)")
+    ap.add_argument("--media", type=int, default=3, help="held-out media per frequency (learned workload)")
     ap.add_argument("--frequencies", type=int, default=64)
     ap.add_argument("--workers", type=int, default=4, help="frequencies in flight (frequencies workload)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -570,6 +572,79 @@ def net_workload(a):
     net.close()
 
 
+def learned_workload(a):
+    """Config 5: learned warm start + multigrid FGMRES against multigrid FGMRES
+    alone and plain GMRES(30), on held-out seeded media (seeds 1000 + k; the
+    net was trained on seeds 0-199 at 3 MHz) at 3, 6 and 9 MHz on the fixed
+    512^2 + 32-PML grid at 10 points per wavelength (dx = lambda_min / 10 at
+    each frequency), 11 angles, tol 1e-6.  Per frequency: mean operator
+    applications and milliseconds per 11-angle solve of each method (a solve =
+    set-up + injection + warm start + solve, timed after one untimed run)."""
+    import numpy as np
+    import torch
+    from nsrno_amd.learned import LearnedNet, attach, canvas
+    from nsrno_amd.train_preconditioner import Family, set_medium
+    mg = dict(levels=a.mg_levels, shift=a.mg_shift, pre=a.mg_pre, post=a.mg_post, coarse_sweeps=a.mg_coarse)
+    freqs = [3.0, 6.0, 9.0]
+    seeds = [1000 + k for k in range(a.media)]
+    rows = {}
+    total = {"learned": [0.0, 0], "mg": [0.0, 0], "plain": [0.0, 0]}
+    for f in freqs:
+        fam = Family(interior=a.interior, frequency_mhz=f)
+        plan = fam.plan(restart=max(a.restart, a.plain_restart))
+        net = LearnedNet(canvas(fam.nz), canvas(fam.nx), fam.R, params=LearnedNet.load_params(a.learned_params),
+                         linear=LearnedNet.load_linear(a.learned_params))
+        attach(plan, net)
+        stats = {k: {"applications": [], "ms": [], "converged": True} for k in total}
+        for seed in seeds:
+            med = fam.medium(seed)
+            for kind in ("learned", "mg", "plain"):
+                for rep in range(2 if kind != "plain" else 1):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    set_medium(plan, med, mg)
+                    b = plan.inject(fam.ports, fam.amp)
+                    x0, warm = None, 0
+                    if kind == "learned":
+                        x0 = torch.zeros_like(b)
+                        plan.warm_start(b, x0, a.warm_steps, precond="learned")
+                        warm = 2 * a.warm_steps
+                    if kind == "plain":
+                        x, info = plan.solve(b, tolerance=a.tol, restart=a.plain_restart,
+                                             max_iterations=a.plain_max_iterations, precond="none")
+                    else:
+                        x, info = plan.solve(b, x0=x0, tolerance=a.tol, restart=a.restart,
+                                             max_iterations=a.max_iterations, precond="mg", graph=bool(a.graph))
+                    torch.cuda.synchronize()
+                    dt = time.perf_counter() - t0
+                st = stats[kind]
+                st["applications"].append(max(i.applications for i in info) + warm)
+                st["ms"].append(dt * 1e3)
+                st["converged"] = st["converged"] and all(i.reason == "converged" for i in info)
+                total[kind][0] += dt
+                total[kind][1] += fam.R
+        rows[f"{f:g} MHz"] = {k: {"applications": round(float(np.mean(v["applications"])), 1),
+                                  "ms_per_solve": round(float(np.mean(v["ms"])), 2), "converged": v["converged"]}
+                              for k, v in stats.items()}
+        print(f"[bench] {f} MHz {rows[f'{f:g} MHz']}", flush=True)
+        net.close()
+        plan.close()
+    sps = {k: total[k][1] / total[k][0] for k in total}
+    line = {
+        "metric": "solves/s to rel-res 1e-6, 512^2 x 11 angles, 3-9 MHz held-out media: learned warm start + "
+                  "multigrid FGMRES", "value": round(sps["learned"], 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": len(seeds) * len(freqs), "warmup": 1, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (FP64 reductions/refinement); net bf16",
+        "data": f"synthetic (held-out seeded media {seeds[0]}-{seeds[-1]})",
+        "config": {"workload": f"learned preconditioner, {a.interior}^2+{a.pml} PML, 10 PPW, 3/6/9 MHz x 11 angles",
+                   "tolerance": a.tol, "method": f"fgmres({a.restart})", "warm_start": f"learned x{a.warm_steps}",
+                   "parallelism": "single GPU"},
+        "solves_per_s": {k: round(v, 3) for k, v in sps.items()},
+        "per_frequency": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -632,6 +707,9 @@ def main():
         return
     if a.workload == "net":
         net_workload(a)
+        return
+    if a.workload == "learned":
+        learned_workload(a)
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)

@@ -657,10 +657,44 @@ def traffic_from_profile(kernel):
         return None
 
 
-def cpu_baseline(bench, a):
-    """Direct complex128 sparse LU of the same operator and RHS (the
-    theranostic CPU algorithm, wave.py:240-273), via the CPU oracle."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_run(fn, reps=3):
+    """One untimed warm-up, then the median wall time of `reps` runs."""
+    out = fn()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(sorted(times)[len(times) // 2]), out
+
+
+def cpu_baseline(bench, a, plain_iterations=None):
+    """The reference's CPU algorithms on this host, on the bench's workload
+    (nsrno/theranostic/wave.py:240-273 and nsrno/wave_solver.py:9-36 as
+    restated by the oracle):
+
+    * complex128 sparse LU (assembly + splu factorisation + all 11 solves) of
+      the same operator and right-hand sides, one warm-up then the median of
+      3 runs, with the BLAS/OpenMP pools limited to 1 thread (the reference's
+      default, theranostic/__main__.py:23) and to every host core;
+    * plain restarted GMRES(30) (the reference's solver) in numpy on the same
+      system: a bounded sample of two restart cycles of one right-hand side
+      gives seconds per Krylov step, and the full solve is extrapolated with
+      the step counts the GPU's plain GMRES(30) needed (`plain_iterations`)."""
     import numpy as np
+    from scipy.sparse.linalg import splu
+    from threadpoolctl import threadpool_limits
     from oracle import helmholtz as O
     from nsrno_amd import media
     w = bench.work
@@ -671,13 +705,37 @@ def cpu_baseline(bench, a):
     W = O.source_weights(grid, pos, nrm, wid)
     amp = media.plane_wave_amplitudes(pos, bench.angles, w.frequency_mhz, w.c0)
     rhs = -(W @ amp.T) / grid.h_m ** 2
-    t0 = time.perf_counter()
-    x, diag = O.solve(grid, w.frequency_mhz, None, w.rho0 * w.c0 ** 2 / (w.rho * w.c ** 2) - 1,
-                      w.rho0 / w.rho - 1, kappa_factor=(1 + 1j * w.alpha_np_m * w.c / (2 * np.pi * w.frequency_mhz * 1e6)) ** 2,
-                      rhs=rhs, c0=w.c0, check_pml=False)
-    dt = time.perf_counter() - t0
-    return dict(seconds=dt, solves=rhs.shape[1], nodes=grid.size, residual=diag["relative_linear_residual"],
-                fields=x, grid_shape=grid.shape)
+    mk = w.rho0 * w.c0 ** 2 / (w.rho * w.c ** 2) - 1
+    me = w.rho0 / w.rho - 1
+    kf = (1 + 1j * w.alpha_np_m * w.c / (2 * np.pi * w.frequency_mhz * 1e6)) ** 2
+
+    def direct():
+        return O.solve(grid, w.frequency_mhz, None, mk, me, kappa_factor=kf, rhs=rhs, c0=w.c0, check_pml=False)
+
+    cores = os.cpu_count() or 1
+    legs = {}
+    for n in (1, cores):
+        with threadpool_limits(limits=n):
+            t, (x, diag) = _median_run(direct)
+        legs[n] = dict(seconds=t, x=x, residual=diag["relative_linear_residual"], lu_nonzeros=diag["lu_nonzeros"])
+    # plain GMRES(30), numpy complex128, one thread: seconds per Krylov step
+    mat = O.assemble(grid, w.frequency_mhz, mk, me, kappa_factor=kf, c0=w.c0, check_pml=False).tocsr()
+    b0 = rhs[:, 0]
+    with threadpool_limits(limits=1):
+        O.gmres(lambda u: mat @ u, lambda u: u, b0, np.zeros_like(b0), restart=30, max_cycles=1, tolerance=1e-30)
+        t0 = time.perf_counter()
+        O.gmres(lambda u: mat @ u, lambda u: u, b0, np.zeros_like(b0), restart=30, max_cycles=2, tolerance=1e-30)
+        per_step = (time.perf_counter() - t0) / 60
+    gm = None
+    if plain_iterations:
+        total = per_step * float(sum(plain_iterations))
+        gm = {"value": round(len(plain_iterations) / total, 5), "unit": "solves/s (extrapolated)", "cores": 1,
+              "seconds_per_step": round(per_step, 5),
+              "sample": f"numpy restatement of wave_solver.py:9-36, 2 GMRES(30) cycles of one right-hand side "
+                        f"({per_step * 1e3:.1f} ms per step), scaled to the {sum(plain_iterations)} steps the GPU's "
+                        f"plain GMRES(30) needed for the {len(plain_iterations)} angles"}
+    return dict(legs=legs, solves=rhs.shape[1], nodes=grid.size, grid_shape=grid.shape, cores=cores,
+                cpu_model=_cpu_model(), gmres=gm)
 
 
 def main():
@@ -777,16 +835,23 @@ def main():
                          "frac": round(bytes_alg / t_st / 1e9 / PEAK_HBM_GBS, 4)}
     cpu = None
     if not a.no_cpu:
-        c = cpu_baseline(b, a)
-        cpu = {"value": round(c["solves"] / c["seconds"], 5), "unit": "solves/s", "cores": 1, "kind": "port",
+        c = cpu_baseline(b, a, result.get("plain_gmres", {}).get("iterations"))
+        one, allc = c["legs"][1], c["legs"][c["cores"]]
+        cpu = {"value": round(c["solves"] / one["seconds"], 5), "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": f"complex128 splu (oracle restatement of wave.py:240-273) of the same "
                          f"{c['grid_shape'][0]}x{c['grid_shape'][1]} operator, {c['solves']} RHS, "
-                         f"assembly+factor+solve {c['seconds']:.2f} s, residual {c['residual']:.1e}"}
+                         f"assembly+factor+solve, median of 3 after a warm-up: {one['seconds']:.2f} s at 1 thread, "
+                         f"residual {one['residual']:.1e}, LU nonzeros {one['lu_nonzeros']}",
+               "cpu_model": c["cpu_model"], "cpu_count": c["cores"],
+               "all_cores": {"value": round(c["solves"] / allc["seconds"], 5), "unit": "solves/s",
+                             "cores": c["cores"], "seconds": round(allc["seconds"], 3)},
+               "plain_gmres30": c["gmres"]}
         if c["grid_shape"] == (b.work.nz, b.work.nx):
             xf = x.reshape(b.R, -1).T.cpu().numpy()
-            err = np.linalg.norm(xf - c["fields"], axis=0) / np.linalg.norm(c["fields"], axis=0)
+            err = np.linalg.norm(xf - one["x"], axis=0) / np.linalg.norm(one["x"], axis=0)
             result["field_error_vs_c128_direct"] = [float(f"{e:.3e}") for e in err]
-        print(f"[bench] cpu baseline {c['seconds']:.2f} s", flush=True)
+        print(f"[bench] cpu baseline {one['seconds']:.2f} s (1 thread), {allc['seconds']:.2f} s ({c['cores']} threads)",
+              flush=True)
     line = {
         "metric": "solves/s to rel-res 1e-6, 512²×11 angles: learned FGMRES vs plain GMRES",
         "value": round(solves_per_s, 4), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,

@@ -392,6 +392,37 @@ int32_t helm_conditioned_gradient(helm_plan* plan, const double* c, const double
 int32_t helm_warm_start(helm_plan* plan, int32_t precond, const void* b, void* x, int32_t nrhs,
                         int32_t steps, int32_t adjoint, void* stream);
 
+/* ---- Born Jacobian at zero contrast: replaces the reciprocal field products
+ * of nsrno/theranostic/wave.py:331-389 (Solver.born_jacobian).  fields c128
+ * [n_nodes][n_src] (row = node), pairs (tx[p], rx[p]) int32 [n_pairs], the
+ * material basis as CSC over nodes (node_ptr [n_basis+1], node_idx, node_w
+ * c128 = h^2 k0^2 sz sx B) and over edges (edge_ptr [n_basis+1], edge_i,
+ * edge_j (-1: ghost end), edge_w c128 = -stretch_e (avg B)_e); J c128
+ * [n_pairs][2 n_basis] = [m_kappa columns | m_eta columns].  The caller
+ * guarantees every index is in range (nodes < n_nodes, sources < n_src). */
+int32_t helm_born_jacobian(int64_t n_nodes, int32_t n_src, const void* fields, int32_t n_pairs, const int32_t* tx,
+                           const int32_t* rx, int32_t n_basis, const int32_t* node_ptr, const int32_t* node_idx,
+                           const void* node_w, const int32_t* edge_ptr, const int32_t* edge_i,
+                           const int32_t* edge_j, const void* edge_w, void* J, void* stream);
+
+/* ---- time -> frequency capture: replaces accumulate_pressure of
+ * nsrno/native_capture.cu:4-11 (the running DFT of the native time stepper's
+ * pressure into complex128 frequency fields).  pressure float32 [snapshots][n]
+ * (1 to 32 consecutive captures), phase c128 [frequencies][captures]
+ * (exp(-2 pi i f t_c) / 2401 in the reference, native_capture.py:37-40),
+ * first = capture index of pressure[0]; fields c128 [frequencies][n] +=
+ * sum_k pressure[k] phase[f][first + k]. */
+int32_t helm_capture_accumulate(const float* pressure, int32_t snapshots, const void* phase, int32_t first,
+                                int32_t captures, int32_t frequencies, int64_t n, void* fields, void* stream);
+
+/* ---- band RF synthesis: replaces the irfft(..., n=2401, norm='forward') of
+ * WaveResidualOperator.predict_band_rf (nsrno/wave_operator.py:169-180).
+ * spectra c64 [traces][n_bins] hold the spectrum at bins[b] (int32, distinct,
+ * 0 <= bins[b] <= samples/2); every other bin is zero.  rf float32
+ * [traces][samples] (samples <= 8192), summed in FP64. */
+int32_t helm_band_rf(const void* spectra, const int32_t* bins, int32_t n_bins, int32_t traces, int32_t samples,
+                     float* rf, void* stream);
+
 /* ---- autotuning of launch shapes: no counterpart in the reference.  Times,
  * with HIP events on the plan's own grid and nrhs right-hand sides, the
  * FP32 stencil's streaming shapes (rows per run x right-hand sides per group,

@@ -87,6 +87,12 @@ SIGNATURES = {
                                     POINTER(c_int32), c_void_p]),
     "helm_krylov_step": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_double, c_void_p]),
     "helm_krylov_end": (c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_int32), c_void_p]),
+    "helm_born_jacobian": (c_int32, [ctypes.c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_int32,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "helm_capture_accumulate": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, ctypes.c_int64,
+                                          c_void_p, c_void_p]),
+    "helm_band_rf": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
     "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
     "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_precond_apply_ex": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),

@@ -10,6 +10,7 @@ instead of an assembled CSC matrix.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from time import perf_counter
 
@@ -18,6 +19,7 @@ import torch
 from scipy import sparse
 from scipy.sparse.linalg import LinearOperator
 
+from ._lib import check, lib
 from .helmholtz import HelmholtzPlan, Ports
 
 
@@ -357,9 +359,11 @@ class Solver:
 
         Rows [tx, rx] flattened, columns [m_kappa basis | m_eta basis].  The
         reciprocal field products over the basis support and the contraction
-        with the (stretch- and mass-weighted) basis run on the GPU as complex
-        GEMMs; no conjugate enters the products (inversion of real parameters
-        uses Re(J^H r))."""
+        with the (stretch- and mass-weighted) basis run in libhelm.so's
+        helm_born_jacobian kernel (one thread per pair and basis function);
+        no conjugate enters the products (inversion of real parameters uses
+        Re(J^H r)).  ``pair_chunk`` is accepted for the reference's signature
+        and validated; the kernel takes all pairs at once."""
         fields = np.asarray(background_fields, dtype=np.complex128)
         if fields.ndim != 2 or fields.shape[0] != self.grid.size:
             raise ValueError("background_fields has incompatible shape")
@@ -384,33 +388,35 @@ class Solver:
         if np.any(self.grid.pml_mask.ravel()[active_nodes]):
             raise ValueError("Material basis must vanish in the PML")
         ei, ej, avg = self._edges()
-        edge_basis = (avg @ basis).tocsr()
+        edge_basis = (avg @ basis).tocsc()
         edge_basis.eliminate_zeros()
-        active_edges = np.flatnonzero(np.diff(edge_basis.indptr))
         k0, mass, stretch = self.coefficients(frequency_mhz)
         h2 = self.grid.h_m ** 2
-        dev = torch.device(self.device)
-        mass_b = torch.from_numpy((sparse.diags(k0 ** 2 * mass[active_nodes]) @ basis[active_nodes]).toarray()
-                                  .astype(np.complex128)).to(dev)
-        flux_b = torch.from_numpy((sparse.diags(-stretch[active_edges]) @ edge_basis[active_edges]).toarray()
-                                  .astype(np.complex128)).to(dev)
-        f_t = torch.from_numpy(fields).to(dev)
-        u_act = f_t[torch.from_numpy(active_nodes).to(dev)]
-        i_e = torch.from_numpy(ei[active_edges]).to(dev)
-        j_e = ej[active_edges]
-        valid = j_e >= 0
-        grads = -f_t[i_e]
-        vi = torch.from_numpy(np.flatnonzero(valid)).to(dev)
-        grads[vi] += f_t[torch.from_numpy(j_e[valid]).to(dev)]
-        grads /= self.grid.h_m
-        tx = np.repeat(transmitters, receivers.size)
-        rx = np.tile(receivers, transmitters.size)
+        node_basis = sparse.csc_matrix(sparse.diags(h2 * k0 ** 2 * mass) @ basis, dtype=np.complex128)
+        node_basis.sort_indices()
+        edge_w = sparse.csc_matrix(sparse.diags(-stretch) @ edge_basis, dtype=np.complex128)
+        edge_w.sort_indices()
+        tx = np.repeat(transmitters, receivers.size).astype(np.int32)
+        rx = np.tile(receivers, transmitters.size).astype(np.int32)
         nb = basis.shape[1]
+        dev = torch.device(self.device)
+
+        def up(a, dtype):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
+
+        # every index the kernel gathers with is checked here (node rows < N, sources < n_src)
+        e_idx = edge_w.indices
+        if (node_basis.indices.size and node_basis.indices.max() >= self.grid.size) or (
+                e_idx.size and (ei[e_idx].max() >= self.grid.size or ej[e_idx].max() >= self.grid.size)):
+            raise ValueError("material basis or edge topology indexes outside the grid")
+        f_t = up(fields, np.complex128)
         out = torch.empty((tx.size, 2 * nb), dtype=torch.complex128, device=dev)
-        for first in range(0, tx.size, pair_chunk):
-            end = min(first + pair_chunk, tx.size)
-            t_i = torch.from_numpy(tx[first:end]).to(dev)
-            r_i = torch.from_numpy(rx[first:end]).to(dev)
-            out[first:end, :nb] = (mass_b.T @ (u_act[:, r_i] * u_act[:, t_i])).T * h2
-            out[first:end, nb:] = (flux_b.T @ (grads[:, r_i] * grads[:, t_i])).T * h2
+        args = [up(tx, np.int32), up(rx, np.int32), up(node_basis.indptr, np.int32), up(node_basis.indices, np.int32),
+                up(node_basis.data, np.complex128), up(edge_w.indptr, np.int32), up(ei[e_idx], np.int32),
+                up(ej[e_idx], np.int32), up(edge_w.data, np.complex128)]
+        p = [ctypes.c_void_p(t.data_ptr()) if t.numel() else None for t in args]
+        check(lib().helm_born_jacobian(self.grid.size, fields.shape[1], ctypes.c_void_p(f_t.data_ptr()), tx.size,
+                                       p[0], p[1], nb, p[2], p[3], p[4], p[5], p[6], p[7], p[8],
+                                       ctypes.c_void_p(out.data_ptr()),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "helm_born_jacobian")
         return out.cpu().numpy()

@@ -250,6 +250,20 @@ class WaveResidualOperator:
 
     __call__ = forward
 
+    @torch.no_grad()
+    def predict_band_rf(self, medium, angle, bin_indices):
+        """Inference-only RF projected onto the given positive-frequency bins
+        (wave_operator.py:169-180): the prediction of ``forward`` at each bin
+        frequency index * 40 MHz / 2401, then the inverse real transform to
+        2401 samples (helm_band_rf) -> float32 [192, 2401]."""
+        from .spectra import band_rf
+        bins = [int(b) for b in bin_indices]
+        for index in bins:
+            if not 0 < index < 1201:
+                raise ValueError("Positive RF bin out of range")
+        preds = torch.stack([self(medium, index * 40e6 / 2401, angle)[0] for index in bins], dim=1)
+        return band_rf(preds, bins, 2401)
+
 
 def load_wave_operator(path, device="cuda", iterations=None):
     """Build the mirror from a reference checkpoint (weights_only load, nsrno/wave_operator.py:182-186)."""

@@ -83,11 +83,12 @@ def parse():
                     help="native medium of the conditioned workload (box 1024 x 1280)")
     ap.add_argument("--cond-tol", type=float, default=1e-5)
     ap.add_argument("--cond-restart", type=int, default=6, help="FGMRES restart length of the conditioned workload")
-    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned", "spectral", "net", "learned"],
+    ap.add_argument("--workload", default="angles", choices=["angles", "frequencies", "conditioned", "spectral", "net", "learned", "aux"],
                     help="angles: the 576^2 x 11-angle batch (default); frequencies: the batched sweep of "
                          "64 frequencies x 11 angles on uneven grids; conditioned: the split-PML system; "
                          "spectral: the S operator on a 1056x1216 box; net: the learned U-Net alone; "
-                         "learned: config 5 (learned vs multigrid vs plain on held-out media, 3-9 MHz)")
+                         "learned: config 5 (learned vs multigrid vs plain on held-out media, 3-9 MHz); "
+                         "aux: capture, band RF and Born Jacobian kernels")
     ap.add_argument("--media", type=int, default=3, help="held-out media per frequency (learned workload)")
     ap.add_argument("--frequencies", type=int, default=64)
     ap.add_argument("--workers", type=int, default=4, help="frequencies in flight (frequencies workload)")
@@ -645,6 +646,61 @@ def learned_workload(a):
     print(json.dumps(line), flush=True)
 
 
+def aux_workload(a):
+    """The kernels around the solve, timed with HIP events on their stream:
+    * capture (helm_capture_accumulate): 64 frequencies of a 1056 x 1216 box
+      (the reference's V5 box) accumulating 32 snapshots per launch; HBM
+      bytes per launch n (4 * 32 + 32 * 64) against 8 TB/s;
+    * band RF (helm_band_rf): 192 receivers x 1200 bins -> 2401 samples;
+    * Born Jacobian (helm_born_jacobian): the theranostic quick grid
+      (185 x 254 nodes), 64 ports, all 4096 pairs, a 12 x 16 bilinear basis."""
+    import numpy as np
+    import torch
+    from scipy import sparse
+    from nsrno_amd.spectra import FrequencyCapture, band_rf
+    from nsrno_amd import theranostic as T
+    out = {}
+    H, W, F, K = 1056, 1216, 64, 32
+    cap = FrequencyCapture((H, W), np.arange(1, F + 1) * 40e6 / 2401 * 18, 24010)
+    snaps = torch.randn(K, H, W, device="cuda")
+    t = _event_time(lambda: cap.add(snaps, 0), 20)
+    alg = H * W * (4.0 * K + 32.0 * F)
+    out["capture"] = {"kernel": "k_capture (64 frequencies, 32 snapshots per launch)", "avg_launch_us": round(t * 1e6, 1),
+                      "algorithmic_bytes": int(alg), "gb_s": round(alg / t / 1e9, 1),
+                      "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4)}
+    bins = np.arange(1, 1201)
+    spec = torch.randn(192, 1200, dtype=torch.complex64, device="cuda")
+    t = _event_time(lambda: band_rf(spec, bins), 10)
+    out["band_rf"] = {"kernel": "k_band_rf (192 x 1200 bins -> 2401 samples)", "avg_launch_us": round(t * 1e6, 1),
+                      "complex_macs_per_s": round(192 * 2401 * 1200 / t / 1e9, 1)}
+    ax = np.arange(254) * 0.1
+    az = np.arange(185) * 0.1
+    grid = T.Grid(ax, az, pml_mm=2.0)
+    solver = T.Solver(grid)
+    xs = np.linspace(3.0, 22.0, 64)
+    w = solver.source_weights(np.stack((xs, np.full(64, 2.6)), axis=1), np.tile([0.0, 1.0], (64, 1)), np.full(64, .25))
+    u0 = solver.solve(1.0, w)
+    xx, zz = np.meshgrid(ax, az)
+    cols = [(np.maximum(1 - np.abs(xx - x0) / 1.2, 0) * np.maximum(1 - np.abs(zz - z0) / 1.2, 0)).ravel()
+            for z0 in np.linspace(4.0, 15.0, 12) for x0 in np.linspace(4.0, 21.0, 16)]
+    basis = sparse.csr_matrix(np.column_stack(cols))
+    solver.born_jacobian(1.0, u0, np.arange(64), np.arange(64), basis)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        solver.born_jacobian(1.0, u0, np.arange(64), np.arange(64), basis)
+    torch.cuda.synchronize()
+    out["born_jacobian"] = {"case": "185x254 nodes, 64 x 64 pairs, 192 bilinear basis functions",
+                            "ms_per_call_incl_host_prep_and_copies": round((time.perf_counter() - t0) / reps * 1e3, 2)}
+    line = {"metric": "aux kernels (capture, band RF, Born Jacobian)", "value": out["capture"]["gb_s"],
+            "unit": "GB/s (capture)", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+            "roofline": {"bound": "hbm", "kernel": out["capture"]["kernel"], "achieved": out["capture"]["gb_s"],
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": out["capture"]["frac"], "traffic": None},
+            "kernels": out}
+    print(json.dumps(line), flush=True)
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
@@ -768,6 +824,9 @@ def main():
         return
     if a.workload == "learned":
         learned_workload(a)
+        return
+    if a.workload == "aux":
+        aux_workload(a)
         return
     b = Bench(a)
     print(f"[bench] workload {b.work.nz}x{b.work.nx} R={b.R} f={a.freq} MHz h={b.work.h_mm:.5f} mm", flush=True)

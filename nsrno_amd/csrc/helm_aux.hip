@@ -87,17 +87,27 @@ __global__ __launch_bounds__(256) void k_capture(int64_t n, int K, int F, int ca
         }
         __syncthreads();
         if (!in) continue;
-        for (int f = 0; f < nf; ++f) {
-            double2 acc = fields[(int64_t)(f0 + f) * n + i];
+        // four frequencies per pass: their four loads are in flight together
+        for (int f = 0; f < nf; f += 4) {
+            double2 acc[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[q] = f + q < nf ? fields[(int64_t)(f0 + f + q) * n + i] : make_double2(0.0, 0.0);
 #pragma unroll
             for (int k = 0; k < CAP_K; ++k) {
                 if (k < K) {
-                    const double2 w = ph[f * CAP_K + k];
-                    acc.x = fma((double)pk[k], w.x, acc.x);
-                    acc.y = fma((double)pk[k], w.y, acc.y);
+                    const double pv = (double)pk[k];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double2 w = ph[min(f + q, nf - 1) * CAP_K + k];
+                        acc[q].x = fma(pv, w.x, acc[q].x);
+                        acc[q].y = fma(pv, w.y, acc[q].y);
+                    }
                 }
             }
-            fields[(int64_t)(f0 + f) * n + i] = acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (f + q < nf) fields[(int64_t)(f0 + f + q) * n + i] = acc[q];
         }
     }
 }

@@ -69,6 +69,7 @@ extern "C" {
 #define HELM_PC_NONE 0     /* plain (unpreconditioned) Krylov               */
 #define HELM_PC_MG 1       /* complex-shifted-Laplacian multigrid V-cycle   */
 #define HELM_PC_LEARNED 2  /* learned conv-net correction + MG V-cycle      */
+#define HELM_PC_SPECTRAL 3 /* Fourier multiplier gain * ifft2(M fft2(v))    */
 
 #define HELM_OP_FLUX5 0    /* 5-point flux form, stretched PML (wave.py)   */
 #define HELM_OP_STAG8 1    /* 8th-order staggered, periodic + sponge       */
@@ -240,6 +241,24 @@ int32_t helm_precond_apply(helm_plan* plan, int32_t precond, const void* v, void
  * spectral plan, k0^2 M^{-H} preconditions the reference's own operator L) */
 int32_t helm_precond_apply_ex(helm_plan* plan, int32_t precond, const void* v, void* z, int32_t nrhs,
                               int32_t adjoint, void* stream);
+
+/* ---- Fourier-multiplier preconditioner: replaces the spectral
+ * preconditioner of nsrno/wave_operator.py:93-102,129-135 (direction
+ * gain * ifft2(multiplier * fft2(r)), incl. a trained model's context gain
+ * and spectral modulation) and conditioned_wave.py:90-92 (gain = the mask).
+ * multiplier, gain: device c128 [nz][nx] on the plan's periodic box (gain
+ * NULL: 1).  HELM_PC_SPECTRAL then applies it in helm_warm_start, helm_solve
+ * and helm_precond_apply_ex: the map as given preconditions the plan's own
+ * operator (forward solves), except on spectral plans (A = k0^2 conj(L)),
+ * where adjoint solves (the reference's L) take it as given; the other
+ * direction applies the conjugate map conj(P(conj v)).  FFTs are mixed-radix Stockham passes in FP32; the
+ * plan allocates two transform buffers of nrhs-capacity fields. */
+int32_t helm_spectral_set_preconditioner(helm_plan* plan, const void* multiplier, const void* gain, void* stream);
+/* 2-D DFT of batch boxes [batch][H][W], c64 (precision 32) or c128 (64)
+ * (inverse != 0: e^{+i}, unscaled), H, W >= 2; work: a buffer of the same
+ * size; in, out, work distinct.  Replaces torch.fft.fft2/ifft2 on the path. */
+int32_t helm_fft2(int32_t H, int32_t W, int32_t batch, int32_t inverse, int32_t precision, const void* in, void* out,
+                  void* work, void* stream);
 
 /* ---- learned preconditioner network ----------------------------------
  * A 4-level, 32-channel 3x3-conv U-Net (BF16 MFMA implicit-GEMM kernels, FP32

@@ -13,7 +13,7 @@ HELM_CONVERGED, HELM_ITERATION_BUDGET, HELM_BREAKDOWN, HELM_STAGNATION, HELM_DIV
 REASONS = {HELM_CONVERGED: "converged", HELM_ITERATION_BUDGET: "iteration_budget",
            HELM_BREAKDOWN: "breakdown", HELM_STAGNATION: "stagnation", HELM_DIVERGED: "diverged"}
 HELM_GMRES, HELM_BICGSTAB = 0, 1
-PC_NONE, PC_MG, PC_LEARNED = 0, 1, 2
+PC_NONE, PC_MG, PC_LEARNED, PC_SPECTRAL = 0, 1, 2, 3
 
 
 class HelmDesc(ctypes.Structure):
@@ -93,6 +93,8 @@ SIGNATURES = {
     "helm_capture_accumulate": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32, ctypes.c_int64,
                                           c_void_p, c_void_p]),
     "helm_band_rf": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
+    "helm_spectral_set_preconditioner": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "helm_fft2": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "helm_mg_setup": (c_int32, [c_void_p, c_int32, c_double, c_int32, c_int32, c_int32, c_double, c_void_p]),
     "helm_precond_apply": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]),
     "helm_precond_apply_ex": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),

@@ -145,9 +145,10 @@ class ConditionedFrequencySystem:
 
     def fixed_preconditioner(self, shift=.15):
         """conditioned_wave.py:90-92: r -> M ifft2(fft2(r) / (1 + lap/k0^2 - i shift)),
-        lap = -(|Dx|^2 + |Dz|^2) of the 8-tap symbols.  The Fourier transforms
-        are the platform FFT library's (torch.fft on the device); the solves
-        here use the multigrid preconditioner instead."""
+        lap = -(|Dx|^2 + |Dz|^2) of the 8-tap symbols, with the library's FFTs
+        (helm_fft2, in the precision of r); the solves here use the multigrid
+        preconditioner instead."""
+        from .spectra import fft2, ifft2
         g = self.g
         coeff = (1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168)
 
@@ -157,7 +158,7 @@ class ConditionedFrequencySystem:
 
         lap = -(symbol(g.W).abs().square()[None, :] + symbol(g.H).abs().square()[:, None])
         multiplier = 1 / (1 + lap / self.k0 ** 2 - 1j * shift)
-        return lambda r: g.mask * torch.fft.ifft2(multiplier * torch.fft.fft2(r))
+        return lambda r: g.mask * ifft2(multiplier.to(r.dtype) * fft2(r.to(g.device)))
 
 
 def source_rhs(system, line):

@@ -1,0 +1,336 @@
+// Two-dimensional FFTs of periodic boxes and the Fourier-multiplier
+// preconditioner built on them.
+//
+// The spectral operator's reference preconditions with a multiplier in
+// Fourier space, d = gain * ifft2(M * fft2(r)) (nsrno/wave_operator.py:93-102,
+// 129-135; nsrno/conditioned_wave.py:90-92 with gain = the medium mask).
+// Box sides are multiples of 32 with arbitrary cofactors (H = 32 ceil((h +
+// 64)/32)), so the transforms are mixed-radix Stockham passes: for a length N
+// = R_1 R_2 ... and the product Ns of the radices already applied, pass R maps
+//   v[r] = in[j + r N/R] e^{s 2 pi i k r/(Ns R)},  k = j mod Ns,
+//   out[(j div Ns) Ns R + k + q Ns] = sum_r v[r] e^{s 2 pi i r q/R},
+// which leaves the result in natural order after the last pass (the
+// autosorting formulation of Govindaraju et al., SC 2008).  Radices 2-13
+// are register kernels; any other prime factor runs a generic kernel that
+// forms one output per thread.  Rows (x, unit stride) and columns (z, stride
+// W) have their own kernels so that a wavefront always reads consecutive
+// complex numbers: along x it walks j, along z it walks x.  FP32 data with
+// twiddle phases formed from exact integer products.
+#include <cmath>
+
+#include "helm_internal.h"
+
+namespace helm {
+
+// e^{2 pi i m / span} (sign folded into m's sign) in the precision of C
+template <typename C> __device__ __forceinline__ C cis(int m, int span);
+template <> __device__ __forceinline__ float2 cis<float2>(int m, int span) {
+    float s, c;
+    sincospif(2.f * (float)m / (float)span, &s, &c);
+    return make_float2(c, s);
+}
+template <> __device__ __forceinline__ double2 cis<double2>(int m, int span) {
+    double s, c;
+    sincospi(2.0 * (double)m / (double)span, &s, &c);
+    return make_double2(c, s);
+}
+
+template <typename C, int R>
+__device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign) {
+    const int span = Ns * R;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        const int m = (int)(((int64_t)k * r) % span);
+        v[r] = cmul(v[r], cis<C>(sign * m, span));
+    }
+    C w[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) w[q] = cis<C>(sign * q, R);
+    C out[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        C acc = v[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) cfma(acc, v[r], w[(r * q) % R]);
+        out[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+}
+
+// transform along x: lines = rows of all batches (H * B of them), length W
+template <typename C, int R>
+__global__ __launch_bounds__(256) void k_fft_rows(int W, int64_t lines, int Ns, int sign, const C* __restrict__ in,
+                                                 C* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = W / R;
+    if (t >= (int64_t)M * lines) return;
+    const int j = (int)(t % M);
+    const int64_t base = (t / M) * W;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[base + j + r * M];
+    const int k = j % Ns;
+    fft_butterfly<C, R>(v, k, Ns, sign);
+    const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[base + o + q * Ns] = v[q];
+}
+
+// transform along z: for every batch and column x, length H with stride W
+template <typename C, int R>
+__global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, int sign, const C* __restrict__ in,
+                                                 C* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = H / R;
+    if (t >= (int64_t)W * M * B) return;
+    const int x = (int)(t % W);
+    const int j = (int)((t / W) % M);
+    const int64_t base = (t / ((int64_t)W * M)) * H * W + x;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[base + (int64_t)(j + r * M) * W];
+    const int k = j % Ns;
+    fft_butterfly<C, R>(v, k, Ns, sign);
+    const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[base + (int64_t)(o + q * Ns) * W] = v[q];
+}
+
+// any radix: one output (line, j, q) per thread; stride = element stride
+// of the transformed axis, lines addressed as base = (l / per) * bstride +
+// (l % per) * lstride
+template <typename C>
+__global__ __launch_bounds__(256) void k_fft_generic(int N, int R, int64_t lines, int per, int64_t bstride,
+                                                     int64_t lstride, int64_t stride, int Ns, int sign,
+                                                     const C* __restrict__ in, C* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = N / R;
+    if (t >= (int64_t)N * lines) return;
+    const int q = (int)(t % R);
+    const int j = (int)((t / R) % M);
+    const int64_t l = t / N;
+    const int64_t base = (l / per) * bstride + (l % per) * lstride;
+    const int k = j % Ns;
+    const int span = Ns * R;
+    C acc;
+    acc.x = 0;
+    acc.y = 0;
+    for (int r = 0; r < R; ++r) {
+        const int m1 = (int)(((int64_t)k * r) % span);
+        const int m2 = (r * q) % R;
+        // e^{s 2 pi i (k r/(Ns R) + r q / R)} = e^{s 2 pi i (k r + r q Ns) / (Ns R)}
+        const int m = (int)(((int64_t)m1 + (int64_t)m2 * Ns) % span);
+        cfma(acc, in[base + (int64_t)(j + r * M) * stride], cis<C>(sign * m, span));
+    }
+    const int o = (j / Ns) * Ns * R + k;
+    out[base + (int64_t)(o + q * Ns) * stride] = acc;
+}
+
+static std::vector<int> fft_radices(int n) {
+    std::vector<int> r;
+    while (n % 8 == 0) { r.push_back(8); n /= 8; }
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    for (int p = 3; n > 1; p += 2)
+        while (n % p == 0) { r.push_back(p); n /= p; }
+    return r;
+}
+
+template <typename C, int R>
+static void launch_rows(int W, int64_t lines, int Ns, int sign, const C* in, C* out, hipStream_t s) {
+    const int64_t cnt = (int64_t)(W / R) * lines;
+    hipLaunchKernelGGL((k_fft_rows<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W, lines, Ns, sign, in,
+                       out);
+}
+template <typename C, int R>
+static void launch_cols(int H, int W, int B, int Ns, int sign, const C* in, C* out, hipStream_t s) {
+    const int64_t cnt = (int64_t)W * (H / R) * B;
+    hipLaunchKernelGGL((k_fft_cols<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, B, Ns, sign,
+                       in, out);
+}
+
+// One axis of a 2-D transform of B boxes [H][W]: src -> (passes ping-pong
+// between a and b) -> returns the buffer holding the result.
+template <typename C>
+static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s) {
+    const int N = rows ? W : H;
+    const std::vector<int> rad = fft_radices(N);
+    const C* in = src;
+    C* out = a;
+    int Ns = 1;
+    for (int R : rad) {
+        if (rows) {
+            const int64_t lines = (int64_t)H * B;
+            switch (R) {
+                case 2: launch_rows<C, 2>(W, lines, Ns, sign, in, out, s); break;
+                case 3: launch_rows<C, 3>(W, lines, Ns, sign, in, out, s); break;
+                case 4: launch_rows<C, 4>(W, lines, Ns, sign, in, out, s); break;
+                case 5: launch_rows<C, 5>(W, lines, Ns, sign, in, out, s); break;
+                case 7: launch_rows<C, 7>(W, lines, Ns, sign, in, out, s); break;
+                case 8: launch_rows<C, 8>(W, lines, Ns, sign, in, out, s); break;
+                case 11: launch_rows<C, 11>(W, lines, Ns, sign, in, out, s); break;
+                case 13: launch_rows<C, 13>(W, lines, Ns, sign, in, out, s); break;
+                default: {
+                    const int64_t cnt = (int64_t)W * lines;
+                    hipLaunchKernelGGL(k_fft_generic<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W, R,
+                                       lines, (int)std::min<int64_t>(lines, 1 << 30), (int64_t)0, (int64_t)W,
+                                       (int64_t)1, Ns, sign, in, out);
+                }
+            }
+        } else {
+            switch (R) {
+                case 2: launch_cols<C, 2>(H, W, B, Ns, sign, in, out, s); break;
+                case 3: launch_cols<C, 3>(H, W, B, Ns, sign, in, out, s); break;
+                case 4: launch_cols<C, 4>(H, W, B, Ns, sign, in, out, s); break;
+                case 5: launch_cols<C, 5>(H, W, B, Ns, sign, in, out, s); break;
+                case 7: launch_cols<C, 7>(H, W, B, Ns, sign, in, out, s); break;
+                case 8: launch_cols<C, 8>(H, W, B, Ns, sign, in, out, s); break;
+                case 11: launch_cols<C, 11>(H, W, B, Ns, sign, in, out, s); break;
+                case 13: launch_cols<C, 13>(H, W, B, Ns, sign, in, out, s); break;
+                default: {
+                    const int64_t lines = (int64_t)W * B;
+                    const int64_t cnt = (int64_t)H * lines;
+                    hipLaunchKernelGGL(k_fft_generic<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, R,
+                                       lines, W, (int64_t)H * W, (int64_t)1, (int64_t)W, Ns, sign, in, out);
+                }
+            }
+        }
+        Ns *= R;
+        in = out;
+        out = (out == a) ? b : a;
+    }
+    return (C*)in;
+}
+
+// 2-D transform of B boxes (sign -1 forward, +1 inverse without 1/N);
+// returns the buffer (a or b) that holds the result.  src may be a or b
+// only if it is not overwritten before it is read (the first pass reads src
+// and writes a).
+template <typename C>
+C* fft2(int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s) {
+    C* r = fft_axis<C>(true, H, W, B, sign, src, a, b, s);
+    C* other = (r == a) ? b : a;
+    return fft_axis<C>(false, H, W, B, sign, r, other, r, s);
+}
+
+// z[r] = conj?(gain (.) buf[r] / N) ; buf = M (.) fft2(conj?(v))  (two kernels)
+__global__ void k_sp_conj_copy(int64_t cnt, const float2* __restrict__ v, float2* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt) out[e] = cconj(v[e]);
+}
+__global__ void k_sp_mult(int64_t n, int64_t cnt, const float2* __restrict__ M, float2* __restrict__ x) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt) x[e] = cmul(M[e % n], x[e]);
+}
+__global__ void k_sp_gain(int64_t n, int64_t cnt, float scale, int conj, const float2* __restrict__ gain,
+                          const float2* __restrict__ x, float2* __restrict__ z) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cnt) return;
+    float2 v = cscale(x[e], scale);
+    if (gain) v = cmul(gain[e % n], v);
+    z[e] = conj ? cconj(v) : v;
+}
+
+int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+    HELM_CHECK(p->fft_mult, HELM_ERR_STATE, "spectral preconditioner: call helm_spectral_set_preconditioner first");
+    HELM_CHECK(nrhs <= p->fft_cap, HELM_ERR_ARG, "spectral preconditioner: more right-hand sides than its buffers");
+    const Level& L = p->levels[0];
+    const int H = L.nz, W = L.nx;
+    const int64_t n = L.n, cnt = (int64_t)nrhs * n;
+    const dim3 g((unsigned)((cnt + 255) / 256)), blk(256);
+    float2 *a = p->fft_a, *b = p->fft_b;
+    // The map as given, d = gain ifft2(M fft2(v)), preconditions the operator
+    // the multiplier was designed for: the plan's own operator (forward
+    // solves) except on spectral plans, which store A = k0^2 conj(L) and whose
+    // reference operator L is reached by adjoint solves.  The other direction
+    // takes the conjugate map conj(P(conj v)) (for a multiplier even in k and a
+    // real gain this is the reference's adjoint preconditioner gain ifft2(conj(M)
+    // fft2(v)), dense_wave.py:195-196).
+    const bool as_given = L.op == HELM_OP_STAG8 ? adjoint : !adjoint;
+    const float2* src = v;
+    if (!as_given) {
+        hipLaunchKernelGGL(k_sp_conj_copy, g, blk, 0, s, cnt, v, b);
+        src = b;
+    }
+    float2* f = fft2<float2>(H, W, nrhs, -1, src, a, b, s);
+    hipLaunchKernelGGL(k_sp_mult, g, blk, 0, s, n, cnt, (const float2*)p->fft_mult, f);
+    float2* fo = (f == a) ? b : a;
+    float2* r = fft2<float2>(H, W, nrhs, +1, f, fo, f, s);
+    hipLaunchKernelGGL(k_sp_gain, g, blk, 0, s, n, cnt, (float)(1.0 / (double)n), (int)!as_given,
+                       (const float2*)p->fft_gain, (const float2*)r, z);
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+__global__ void k_c128_to_c64(int64_t n, const double2* __restrict__ src, float2* __restrict__ dst) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[e] = make_float2((float)src[e].x, (float)src[e].y);
+}
+
+}  // namespace helm
+
+using namespace helm;
+
+extern "C" {
+
+int32_t helm_spectral_set_preconditioner(helm_plan* p, const void* multiplier, const void* gain, void* stream) {
+    HELM_CHECK(p && multiplier, HELM_ERR_ARG, "helm_spectral_set_preconditioner: null argument");
+    const Level& L = p->levels[0];
+    HELM_CHECK(L.nz >= 2 && L.nx >= 2, HELM_ERR_ARG, "helm_spectral_set_preconditioner: box too small");
+    const int64_t n = L.n;
+    const int cap = std::max(1, p->desc.nrhs);
+    hipStream_t s = (hipStream_t)stream;
+    auto take = [&](float2** q, int64_t count) -> int32_t {
+        if (*q) return HELM_OK;
+        hipError_t e = hipMalloc((void**)q, (size_t)count * sizeof(float2) + 256);
+        if (e != hipSuccess) {
+            *q = nullptr;
+            set_error(std::string("hipMalloc (spectral preconditioner): ") + hipGetErrorString(e));
+            return HELM_ERR_ALLOC;
+        }
+        return HELM_OK;
+    };
+    HELM_TRY(take(&p->fft_mult, n));
+    HELM_TRY(take(&p->fft_a, (int64_t)cap * n));
+    HELM_TRY(take(&p->fft_b, (int64_t)cap * n));
+    p->fft_cap = cap;
+    hipLaunchKernelGGL(k_c128_to_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                       (const double2*)multiplier, p->fft_mult);
+    if (gain) {
+        HELM_TRY(take(&p->fft_gain, n));
+        hipLaunchKernelGGL(k_c128_to_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                           (const double2*)gain, p->fft_gain);
+    } else if (p->fft_gain) {
+        (void)hipFree(p->fft_gain);
+        p->fft_gain = nullptr;
+    }
+    HELM_HIP(hipGetLastError());
+    p->generation++;
+    return HELM_OK;
+}
+
+int32_t helm_fft2(int32_t H, int32_t W, int32_t batch, int32_t inverse, int32_t precision, const void* in, void* out,
+                  void* work, void* stream) {
+    HELM_CHECK(in && out && work && in != out && in != work && out != work, HELM_ERR_ARG,
+               "helm_fft2: null or aliased argument");
+    HELM_CHECK(H >= 2 && W >= 2 && batch >= 1 && (int64_t)H * W * batch < ((int64_t)1 << 31), HELM_ERR_ARG,
+               "helm_fft2: sizes out of range");
+    HELM_CHECK(precision == 32 || precision == 64, HELM_ERR_ARG, "helm_fft2: precision must be 32 or 64");
+    hipStream_t s = (hipStream_t)stream;
+    const int sign = inverse ? +1 : -1;
+    void* r;
+    size_t el;
+    if (precision == 32) {
+        r = fft2<float2>(H, W, batch, sign, (const float2*)in, (float2*)out, (float2*)work, s);
+        el = sizeof(float2);
+    } else {
+        r = fft2<double2>(H, W, batch, sign, (const double2*)in, (double2*)out, (double2*)work, s);
+        el = sizeof(double2);
+    }
+    if (r != out) HELM_HIP(hipMemcpyAsync(out, r, (size_t)H * W * batch * el, hipMemcpyDeviceToDevice, s));
+    HELM_HIP(hipGetLastError());
+    return HELM_OK;
+}
+
+}  // extern "C"

@@ -268,6 +268,11 @@ struct helm_plan {
     float2* cond_scr = nullptr;       // scaled preconditioner input (HELM_OP_COND8)
     int64_t cond_cap = 0;             // elements cond_scr holds
     double2* cond_grad = nullptr;     // material-gradient scratch: 3 * n c128
+    // Fourier-multiplier preconditioner (helm_spectral_set_preconditioner)
+    float2* fft_mult = nullptr;       // multiplier M [n]
+    float2* fft_gain = nullptr;       // gain [n] (nullptr: 1)
+    float2 *fft_a = nullptr, *fft_b = nullptr;  // transform buffers, fft_cap * n each
+    int fft_cap = 0;
 };
 
 namespace helm {
@@ -323,6 +328,9 @@ int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nr
 // launch-shape tuning (helm_stencil.hip, helm_mg.hip)
 int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
+// Fourier-multiplier preconditioner z = gain ifft2(M fft2(v)) (helm_fft.hip);
+// adjoint = false applies its conjugate map
+int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
 // learned correction z += N(v) (helm_net.hip)
 int32_t net_correction(helm_plan* p, struct helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s);
 }  // namespace helm

@@ -361,6 +361,8 @@ int32_t helm_plan_destroy(helm_plan* p) {
     if (p->sp_buf) (void)hipFree(p->sp_buf);
     if (p->cond_scr) (void)hipFree(p->cond_scr);
     if (p->cond_grad) (void)hipFree(p->cond_grad);
+    for (void* q : {(void*)p->fft_mult, (void*)p->fft_gain, (void*)p->fft_a, (void*)p->fft_b})
+        if (q) (void)hipFree(q);
     prof_destroy(p->prof);
     delete p;
     return HELM_OK;

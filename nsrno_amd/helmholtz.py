@@ -27,6 +27,9 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+PRECONDITIONERS = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED, "spectral": _lib.PC_SPECTRAL}
+
+
 @dataclass
 class SolveInfo:
     """Per right-hand-side result of a solve (keys of wave_solver.py:65-67)."""
@@ -180,6 +183,17 @@ class HelmholtzPlan:
                                   float(omega), _stream()), "helm_mg_setup")
         self.mg_levels = int(levels)
 
+    def set_spectral_preconditioner(self, multiplier, gain=None):
+        """Fourier-multiplier preconditioner gain * ifft2(multiplier * fft2(v))
+        on the plan's periodic box (helm_spectral_set_preconditioner); then
+        precond="spectral" / kind=PC_SPECTRAL use it."""
+        m = torch.as_tensor(multiplier, device=self.device).to(torch.complex128).reshape(self.nz, self.nx).contiguous()
+        g = None if gain is None else torch.as_tensor(gain, device=self.device).to(torch.complex128).reshape(
+            self.nz, self.nx).contiguous()
+        self._spectral = (m, g)
+        check(lib().helm_spectral_set_preconditioner(self._h, _ptr(m), None if g is None else _ptr(g), _stream()),
+              "helm_spectral_set_preconditioner")
+
     def precondition(self, v, kind=_lib.PC_MG, adjoint=False):
         """z = M^{-1} v (adjoint: M^{-H} v) for complex64 fields [..., nz, nx]."""
         nrhs = self._check_field(v)
@@ -244,7 +258,7 @@ class HelmholtzPlan:
         nrhs = self._check_field(rhs)
         if x.shape != rhs.shape:
             raise ValueError("x and rhs must have the same shape")
-        pc = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED}[precond]
+        pc = PRECONDITIONERS[precond]
         check(lib().helm_warm_start(self._h, pc, _ptr(rhs), _ptr(x), nrhs, int(steps), int(bool(adjoint)),
                                     _stream()), "helm_warm_start")
         return x
@@ -273,7 +287,7 @@ class HelmholtzPlan:
             x = torch.empty_like(rhs)
         else:
             x = torch.as_tensor(x0, device=self.device).to(torch.complex128).contiguous().clone()
-        pc = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED}[precond]
+        pc = PRECONDITIONERS[precond]
         meth = {"gmres": _lib.HELM_GMRES, "bicgstab": _lib.HELM_BICGSTAB}[method]
 
         def _cb(user, applications, rel, n):

@@ -1,4 +1,8 @@
-"""Frequency capture from time snapshots and band-limited RF synthesis on the GPU.
+"""Frequency capture from time snapshots, band-limited RF synthesis and 2-D
+FFTs on the GPU.
+
+* ``fft2`` / ``ifft2`` are the library's mixed-radix FFTs (helm_fft2), used
+  where the reference calls torch.fft on the path.
 
 * ``FrequencyCapture`` replaces the running DFT of nsrno/native_capture.cu
   (accumulate_pressure, :4-11) and its driver nsrno/native_capture.py:29-59:
@@ -36,6 +40,26 @@ def _s():
 
 def _p(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+def fft2(x, inverse=False):
+    """2-D DFT over the last two axes of a complex64/complex128 GPU tensor
+    (helm_fft2: mixed-radix Stockham passes; inverse unscaled, as e^{+i})."""
+    if x.device.type != "cuda" or x.dtype not in (torch.complex64, torch.complex128) or x.dim() < 2:
+        raise ValueError("fft2 takes a complex64 or complex128 GPU tensor [..., H, W]")
+    H, W = x.shape[-2:]
+    src = x.contiguous()
+    out = torch.empty_like(src)
+    work = torch.empty_like(src)
+    batch = src.numel() // (H * W)
+    check(lib().helm_fft2(int(H), int(W), int(batch), int(bool(inverse)), 32 if x.dtype == torch.complex64 else 64,
+                          _p(src), _p(out), _p(work), _s()), "helm_fft2")
+    return out
+
+
+def ifft2(x):
+    """Inverse 2-D DFT with the 1/(H W) of numpy/torch's ifft2."""
+    return fft2(x, inverse=True) / (x.shape[-1] * x.shape[-2])
 
 
 def capture_phase(frequencies_hz, steps, stride=STRIDE, dt=STEP_S, samples=SAMPLES, device="cuda"):

@@ -12,20 +12,24 @@ the same arguments and meaning; the arithmetic runs in libhelm.so:
 * the scattered-field right-hand side        helm_spectral_rhs (:142-143)
 * _iteration / the warm start of forward     helm_warm_start   (:129-135)
 
+* the warm start's direction      helm_spectral_set_preconditioner + the
+                                   library's FFTs: gain * ifft2(multiplier *
+                                   fft2(r)) exactly as the reference forms it
+                                   (:71-102, 129-135), with a trained model's
+                                   context gain, spectral modulation and shift
+
 Differences, by design:
-* the preconditioner of the warm start and of the solve is the library's
-  complex-shifted-Laplacian multigrid on a 5-point discretisation of the same
-  coefficients, not the Fourier-space shifted-Laplacian multiplier
-  (wave_operator.py:93-102); the learned context gain (a small conv net of the
-  trained model, :71-75) is not applied, so with a trained checkpoint the warm
-  start differs while the solution the solve converges to does not;
+* the Krylov solve after the warm start is preconditioned by the library's
+  complex-shifted-Laplacian multigrid (a 5-point discretisation of the same
+  coefficients) instead of the Fourier multiplier; the converged solution is
+  the same;
 * ``setup`` returns an opaque ``SpectralData`` (plan handle and k0) in place of
   the (ax, az, kappa_complex, k0) tensor tuple; ``coefficients`` gives the
   tensors when they are wanted.
 
-The trained scalars that change the solution are honoured: ``alpha_log``
-(attenuation) and ``response`` (receiver scale), as loaded by
-``load_wave_operator``.
+Every trained parameter is honoured (``load_wave_operator``): ``alpha_log``
+(attenuation), ``response`` (receiver scale), ``log_shift``, ``spectral`` and
+the ``context`` net.
 """
 from __future__ import annotations
 
@@ -95,6 +99,11 @@ class WaveResidualOperator:
         self.alpha_log = math.log(0.5)
         self.log_shift = math.log(0.15)
         self.response = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+        # learned preconditioner parts (wave_operator.py:37-41): the context net's
+        # weights (None: its zero-initialised last layer, gain 1) and the
+        # spectral modulation coefficients
+        self.context = None
+        self.spectral = np.zeros((2, 4))
         self.training = False
         self.restart_capacity = int(restart)
         self.nrhs_capacity = int(nrhs)
@@ -121,6 +130,11 @@ class WaveResidualOperator:
             self.log_shift = float(val("log_shift"))
         if "response" in state:
             self.response = val("response").reshape(2, 3)
+        if "spectral" in state:
+            self.spectral = val("spectral").reshape(2, 4)
+        keys = ("context.0.weight", "context.0.bias", "context.2.weight", "context.2.bias")
+        if all(k in state for k in keys):
+            self.context = [val(k) for k in keys]
         return self
 
     @property
@@ -172,8 +186,55 @@ class WaveResidualOperator:
             plan.setup_multigrid(levels=_mg_levels(self.H, self.W), shift=0.5, pre=2, post=2, coarse_sweeps=10,
                                  omega=0.0)
 
+    def _symbol(self, n):
+        q = 2 * np.pi * np.fft.fftfreq(n)
+        coeff = (1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168)
+        return 2j * sum(a * np.sin((i + .5) * q) for i, a in enumerate(coeff)) * np.exp(.5j * q) / self.dx
+
+    def preconditioner_maps(self, c, rho, frequency, learned=True):
+        """(gain [H, W] or None, multiplier [H, W]) of the reference's spectral
+        preconditioner (wave_operator.py:71-102, 'shifted' and
+        'shifted_tapered'): multiplier exp(.5 tanh(mod0) + .5 i tanh(mod1)) /
+        (1 - q^2 - i shift), the modulation from the spectral coefficients;
+        gain from the context net on the 4x pooled (log c/C0, -log a,
+        f/6 MHz - 1) maps (None when its last layer is zero: gain 1), times the
+        sponge taper for 'shifted_tapered'.  Host-side set-up (float64, the
+        context net in float32 as the reference's)."""
+        if self.preconditioner not in ("shifted", "shifted_tapered"):
+            raise NotImplementedError("the free-space Green preconditioner needs a 2H x 2W box; use 'shifted'")
+        k0 = self.wavenumber(frequency)
+        Dx = self._symbol(self.W)[None, :]
+        Dz = self._symbol(self.H)[:, None]
+        q2 = (np.abs(Dx) ** 2 + np.abs(Dz) ** 2) / k0 ** 2
+        shift = min(max(math.exp(self.log_shift), .03), 1.0) if learned else .15
+        mult = 1 / (1 - q2 - 1j * shift)
+        if learned and np.any(self.spectral):
+            angular = np.abs(Dx) ** 2 / (np.abs(Dx) ** 2 + np.abs(Dz) ** 2 + 1) * np.ones_like(q2)
+            basis = np.stack((np.ones_like(q2), q2 / (1 + q2), angular, angular * q2 / (1 + q2)))
+            mod = np.einsum("ck,khw->chw", self.spectral, basis)
+            mult = np.exp(.5 * np.tanh(mod[0]) + .5j * np.tanh(mod[1])) * mult
+        gain = None
+        if learned and self.context is not None:
+            import torch.nn.functional as F
+            cp = torch.as_tensor(c, dtype=torch.float32).cpu()
+            a = RHO0 / torch.as_tensor(rho, dtype=torch.float32).cpu()
+            cp = F.pad(cp[None, None], self.pad, mode="replicate")[0, 0]
+            a = F.pad(a[None, None], self.pad, mode="replicate")[0, 0]
+            feats = torch.stack(((cp / C0).log(), -a.log(), torch.ones_like(a) * (frequency / 6e6 - 1)))[None]
+            w0, b0, w2, b2 = (torch.as_tensor(v, dtype=torch.float32) for v in self.context)
+            g = F.conv2d(torch.tanh(F.conv2d(F.avg_pool2d(feats, 4), w0, b0, padding=1)), w2, b2, padding=1)
+            g = F.interpolate(g, size=(self.H, self.W), mode="bilinear", align_corners=False)[0].double().numpy()
+            gain = (1 + .5 * np.tanh(g[0])) * np.exp(.4j * np.tanh(g[1]))
+        if self.preconditioner == "shifted_tapered":
+            taper = np.exp(-k0 * 48 * self.dx / 3 * (self.sponge.numpy() / 2) ** 1.5)
+            gain = taper if gain is None else gain * taper
+        return gain, mult
+
     def setup(self, medium, frequency, learned=True):
-        """(data, gain, multiplier) as wave_operator.py:46-103; data is a SpectralData."""
+        """(data, gain, multiplier) as wave_operator.py:46-103; data is a
+        SpectralData whose plan also holds the spectral preconditioner
+        (gain * ifft2(multiplier * fft2(r)), helm_spectral_set_preconditioner)
+        that the warm start applies."""
         if medium.shape[0] != 1:
             raise ValueError("One native medium per frequency solve")
         if tuple(medium.shape[-2:]) != (self.h, self.w):
@@ -181,7 +242,9 @@ class WaveResidualOperator:
         c, rho = medium.detach().float().unbind(1)
         plan = self._plan(frequency)
         self._set_medium(plan, c[0], rho[0])
-        return SpectralData(plan, self.wavenumber(frequency)), 1.0, None
+        gain, mult = self.preconditioner_maps(c[0].cpu().numpy(), rho[0].cpu().numpy(), frequency, learned)
+        plan.set_spectral_preconditioner(mult, gain)
+        return SpectralData(plan, self.wavenumber(frequency)), (1.0 if gain is None else gain), mult
 
     def coefficients(self, data):
         """(kappa_S, L applied to ones) read back through the FP64 operator: kappa_S = L 1."""
@@ -238,11 +301,11 @@ class WaveResidualOperator:
         den = rhs.abs().square().mean().clamp_min(1e-20)
         trace = []
         for _ in range(self.iterations if return_trace else 0):
-            data.plan.warm_start(rhs, x, 1, precond="mg", adjoint=True)
+            data.plan.warm_start(rhs, x, 1, precond="spectral", adjoint=True)
             r = rhs - data.plan.apply(x, adjoint=True)
             trace.append(float((r.abs().square().mean() / den).sqrt()))
         if not return_trace:
-            data.plan.warm_start(rhs, x, self.iterations, precond="mg", adjoint=True)
+            data.plan.warm_start(rhs, x, self.iterations, precond="spectral", adjoint=True)
         r = rhs - data.plan.apply(x, adjoint=True)
         loss = r.abs().square().mean() / den
         pred = x[self.face, self.rx_cols.to(self.device)] * self.source_factor(frequency, angle).to(self.device)

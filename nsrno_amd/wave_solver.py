@@ -92,7 +92,7 @@ def solve_frequency_batch(model: WaveResidualOperator, medium, frequency_hz, ang
     inc = torch.stack([model.incident64(frequency_hz, a, data) for a in angles])
     rhs = model.scattered_rhs(inc, data)
     x = torch.zeros_like(rhs)
-    data.plan.warm_start(rhs, x, model.iterations, precond="mg", adjoint=True)
+    data.plan.warm_start(rhs, x, model.iterations, precond="spectral", adjoint=True)
     traces = [[] for _ in angles]
     calls = [0]
 

@@ -28,6 +28,11 @@ response polynomial are explicit arguments):
                       (oracle.helmholtz.gmres = wave_solver.py:9-36) with the
                       spectral preconditioner
 * ``solve_direct``    the same system by splu: the exact solution.
+* ``context_gain``    wave_operator.py:71-75 (learned context conv net gain)
+* ``modulated_multiplier`` wave_operator.py:93-102 with the learned spectral
+                      modulation and shift
+* ``forward``         wave_operator.py:129-154 (warm-start-only prediction,
+                      loss and trace)
 
 S convention: torch/NumPy forward FFT kernel, outgoing waves exp(-i k r),
 sponge and attenuation as -i terms (wave_operator.py:3,67).
@@ -41,6 +46,9 @@ How each function is pinned (tests/test_oracle_spectral.py):
   columns); real coefficients give a real symmetric matrix;
 * incident: the Newton root satisfies the discrete dispersion relation;
 * solve_direct: residual < 1e-10 of the assembled system;
+* context_gain, modulated_multiplier, forward: golden gain map, multiplier
+  and forward predictions/losses/traces of the reference's own model, at its
+  default initialisation and with perturbed context net, modulation and shift;
 * solve_gmres: equal to solve_direct at convergence; the warm-start steps
   never increase the residual (scripts/check_wave.py:18); the reference's
   own complex64 prediction (golden) within its stagnation error.
@@ -235,6 +243,98 @@ def multiplier(box, k0, shift=0.15):
     Dz = symbol(box.H, box.dx)[:, None]
     q2 = (np.abs(Dx) ** 2 + np.abs(Dz) ** 2) / k0 ** 2
     return 1 / (1 - q2 - 1j * shift)
+
+
+def _conv3x3(x, w, b):
+    """Conv2d(k=3, padding=1) of x [Cin, H, W] with w [Cout, Cin, 3, 3], b [Cout]."""
+    cin, H, W = x.shape
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1)))
+    out = np.empty((w.shape[0], H, W))
+    for o in range(w.shape[0]):
+        acc = np.full((H, W), float(b[o]))
+        for i in range(cin):
+            for dz in range(3):
+                for dx in range(3):
+                    acc += w[o, i, dz, dx] * xp[i, dz:dz + H, dx:dx + W]
+        out[o] = acc
+    return out
+
+
+def _bilinear(x, H, W):
+    """F.interpolate(x, size=(H, W), mode='bilinear', align_corners=False) of x [C, h, w]."""
+    c, h, w = x.shape
+
+    def taps(n_out, n_in):
+        s = (np.arange(n_out) + 0.5) * n_in / n_out - 0.5
+        s = np.clip(s, 0, None)
+        i0 = np.minimum(np.floor(s).astype(int), n_in - 1)
+        i1 = np.minimum(i0 + 1, n_in - 1)
+        return i0, i1, s - i0
+
+    z0, z1, fz = taps(H, h)
+    x0, x1, fx = taps(W, w)
+    top = x[:, z0][:, :, x0] * (1 - fx) + x[:, z0][:, :, x1] * fx
+    bot = x[:, z1][:, :, x0] * (1 - fx) + x[:, z1][:, :, x1] * fx
+    return top * (1 - fz)[None, :, None] + bot * fz[None, :, None]
+
+
+def context_gain(box, c, rho, frequency, params):
+    """The learned complex gain of wave_operator.py:71-75: a 3 -> 12 -> 2 conv
+    net (tanh) on the 4x average-pooled (log c/C0, -log a, f/6 MHz - 1) maps,
+    bilinearly upsampled to the box, gain = (1 + .5 tanh g0) exp(.4 i tanh g1).
+    params: context_0_weight/bias, context_2_weight/bias arrays."""
+    c = np.asarray(c, dtype=np.float64)
+    a = _pad(RHO0 / np.asarray(rho, dtype=np.float64), box)
+    cp = _pad(c, box)
+    feats = np.stack((np.log(cp / C0), -np.log(a), np.full(a.shape, frequency / 6e6 - 1)))
+    pooled = feats.reshape(3, box.H // 4, 4, box.W // 4, 4).mean(axis=(2, 4))
+    hid = np.tanh(_conv3x3(pooled, params["context_0_weight"], params["context_0_bias"]))
+    g = _bilinear(_conv3x3(hid, params["context_2_weight"], params["context_2_bias"]), box.H, box.W)
+    return (1 + 0.5 * np.tanh(g[0])) * np.exp(0.4j * np.tanh(g[1]))
+
+
+def modulated_multiplier(box, k0, shift=0.15, spectral=None):
+    """The 'shifted' multiplier with the learned spectral modulation
+    exp(.5 tanh(mod0) + .5 i tanh(mod1)), mod = spectral @ (1, q^2/(1+q^2),
+    angular, angular q^2/(1+q^2)) (wave_operator.py:93-102)."""
+    Dx = symbol(box.W, box.dx)[None, :]
+    Dz = symbol(box.H, box.dx)[:, None]
+    q2 = (np.abs(Dx) ** 2 + np.abs(Dz) ** 2) / k0 ** 2
+    green = 1 / (1 - q2 - 1j * shift)
+    if spectral is None:
+        return green
+    angular = np.abs(Dx) ** 2 / (np.abs(Dx) ** 2 + np.abs(Dz) ** 2 + 1)
+    basis = np.stack((np.ones_like(q2), q2 / (1 + q2), angular * np.ones_like(q2), angular * q2 / (1 + q2)))
+    mod = np.einsum("ck,khw->chw", np.asarray(spectral, dtype=np.float64), basis)
+    return np.exp(0.5 * np.tanh(mod[0]) + 0.5j * np.tanh(mod[1])) * green
+
+
+def forward(box, c, rho, frequency, theta_rad, tref, iterations=8, gain=1.0, mult=None, alpha_scale=0.5,
+            f0=7.5e6):
+    """The warm-start-only forward of wave_operator.py:137-154 in complex128:
+    `iterations` steps u += alpha d, d = gain ifft2(mult fft2(r)),
+    alpha = <L d, r> / ||L d||^2 (:129-135).  Returns (prediction, loss =
+    mean|r|^2 / mean|rhs|^2, trace of sqrt(loss) after every step)."""
+    ax, az, kc, k0 = setup(box, c, rho, frequency, alpha_scale)
+    inc, _, _ = incident(box, frequency, theta_rad)
+    rhs = scattered_rhs(box, inc, ax, az, kc, k0)
+    mult = multiplier(box, k0) if mult is None else mult
+
+    def op(u):
+        return apply_stencil(u, ax, az, kc, k0, box.dx)
+
+    den = max(np.mean(np.abs(rhs) ** 2), 1e-20)
+    u = np.zeros_like(rhs)
+    trace = []
+    for _ in range(iterations):
+        r = rhs - op(u)
+        d = gain * np.fft.ifft2(mult * np.fft.fft2(r))
+        image = op(d)
+        u = u + np.vdot(image, r) / max(np.vdot(image, image).real, 1e-24) * d
+        trace.append(float(np.sqrt(np.mean(np.abs(rhs - op(u)) ** 2) / den)))
+    loss = float(np.mean(np.abs(rhs - op(u)) ** 2) / den)
+    pred = u[box.face, box.rx_cols] * source_factor(frequency, tref, f0)
+    return pred, loss, trace
 
 
 def solve_gmres(box, c, rho, frequency, theta_rad, tref, iterations=8, tolerance=1e-10, restart=32,

@@ -137,3 +137,45 @@ def test_reference_prediction_within_its_stagnation():
     assert info["reason"] == "converged"
     assert str(G["pred_reason"]) == "iteration_budget" and float(G["pred_residual"]) < 1e-5
     assert rel(pred, G["pred"]) < 0.03
+
+
+def _learned_params():
+    return {k[len("learned_"):]: G[k].astype(np.float64) for k in G.files
+            if k.startswith("learned_context") or k in ("learned_spectral", "learned_log_shift")}
+
+
+def test_forward_matches_reference_default_model():
+    """wave_operator.forward of the default model (gain 1, no modulation): the
+    warm-start-only prediction, loss and trace (golden, complex64 reference)."""
+    box = S.Box(int(G["h"]), int(G["w"]), float(G["dx"]), float(G["x0"]), float(G["z0"]))
+    theta = float(G["angles"][1]) * np.pi / 180
+    pred, loss, trace = S.forward(box, np.float32(G["c"]), np.float32(G["rho"]), float(G["f"]), theta,
+                                  float(G["trefs"][1]), iterations=4)
+    # the reference runs in complex64: its right-hand side and first direction
+    # carry ~1e-5 relative rounding, its iterate ~4e-4; the receiver row holds
+    # values ~1e-4 of the field's maximum, where that is ~2 % of the prediction
+    ax, az, kc, k0 = S.setup(box, np.float32(G["c"]), np.float32(G["rho"]), float(G["f"]))
+    inc, _, _ = S.incident(box, float(G["f"]), theta)
+    rhs = S.scattered_rhs(box, inc, ax, az, kc, k0)
+    assert rel(rhs, G["fwd_rhs"]) < 5e-5
+    assert rel(np.fft.ifft2(S.multiplier(box, k0) * np.fft.fft2(rhs)), G["fwd_dir0"]) < 5e-5
+    assert rel(pred, G["fwd_pred"]) < 5e-2
+    assert abs(loss - float(G["fwd_loss"])) < 2e-3 * float(G["fwd_loss"])
+    assert np.allclose(trace, G["fwd_trace"], rtol=1e-3)
+
+
+def test_learned_gain_multiplier_and_forward_match_reference():
+    box = S.Box(int(G["h"]), int(G["w"]), float(G["dx"]), float(G["x0"]), float(G["z0"]))
+    prm = _learned_params()
+    f = float(G["f"])
+    gain = S.context_gain(box, np.float32(G["c"]), np.float32(G["rho"]), f, prm)
+    assert rel(gain, G["learned_gain"]) < 1e-5
+    k0 = S.wavenumber(f)
+    shift = float(np.clip(np.exp(prm["log_shift"]), 0.03, 1.0))
+    mult = S.modulated_multiplier(box, k0, shift, prm["spectral"])
+    assert rel(mult, G["learned_mult"]) < 1e-5
+    theta = float(G["angles"][1]) * np.pi / 180
+    pred, loss, trace = S.forward(box, np.float32(G["c"]), np.float32(G["rho"]), f, theta, float(G["trefs"][1]),
+                                  iterations=4, gain=gain, mult=mult)
+    assert rel(pred, G["learned_pred"]) < 5e-2  # complex64 reference, see the default-model test
+    assert np.allclose(trace, G["learned_trace"], rtol=1e-3)

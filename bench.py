@@ -91,7 +91,10 @@ def parse():
                          "aux: capture, band RF and Born Jacobian kernels")
     ap.add_argument("--media", type=int, default=3, help="held-out media per frequency (learned workload)")
     ap.add_argument("--frequencies", type=int, default=64)
-    ap.add_argument("--workers", type=int, default=4, help="frequencies in flight (frequencies workload)")
+    ap.add_argument("--workers", type=int, default=4,
+                    help="frequencies in flight, or set-up threads with --sweep-mode batched (frequencies workload)")
+    ap.add_argument("--sweep-mode", default="batched", choices=["batched", "threads"],
+                    help="frequencies workload: one segmented batched solve, or one solve per frequency on streams")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (PMC counter calibration) and exit")
     ap.add_argument("--pmc-summary", metavar="DIR",
@@ -303,17 +306,20 @@ def frequency_sweep(a):
     """Batched uneven solves: `a.frequencies` frequencies evenly spaced in
     4.0-8.9 MHz x 11 angles; each grid sized for 10 points per wavelength at
     1440 m/s (interior 384^2 at 4 MHz to 854^2 at 8.9 MHz, + 32-cell PML) on
-    one inclusion medium resampled; frequencies in flight on `a.workers` HIP
-    streams.  One JSON line."""
+    one inclusion medium resampled.  --sweep-mode batched (default): one
+    helm_batch_solve over all frequencies; threads: `a.workers` frequencies
+    in flight on HIP streams.  One JSON line."""
     import numpy as np
     from nsrno_amd.batch import run_sweep
     freqs = np.linspace(4.0, 8.9, a.frequencies)
     angles = np.linspace(-15.0, 15.0, a.angles)
-    for _ in range(a.warmup):  # one small frequency warms the kernels and allocator
-        run_sweep(freqs[:1], angles, tol=a.tol, restart=a.restart, workers=1, mg_shift=a.mg_shift)
+    batched = a.sweep_mode == "batched"
+    for _ in range(a.warmup):  # small frequencies warm the kernels and allocator
+        run_sweep(freqs[:2], angles, tol=a.tol, restart=a.restart, workers=1, mg_shift=a.mg_shift,
+                  mode=a.sweep_mode)
     cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers,
-                            log=lambda m: print("[bench] " + m, flush=True), autotune=bool(a.autotune),
-                            graph=bool(a.graph), mg_shift=a.mg_shift)
+                            log=lambda m: print("[bench] " + m, flush=True), autotune=bool(a.autotune) and not batched,
+                            graph=bool(a.graph), mg_shift=a.mg_shift, mode=a.sweep_mode)
     solves = len(freqs) * len(angles)
     kbytes = sum(c.krylov_bytes for c in cases)
     line = {
@@ -324,7 +330,10 @@ def frequency_sweep(a):
         "config": {"workload": "frequency sweep", "frequencies": len(freqs), "angles": len(angles),
                    "interiors": [cases[0].interior, cases[-1].interior], "pml": 32, "ppw": 10,
                    "tolerance": a.tol, "method": f"fgmres({a.restart}) + multigrid",
-                   "streams": a.workers, "autotune": bool(a.autotune), "graph": bool(a.graph),
+                   "mode": ("one helm_batch_solve: every kernel of a restart cycle one segmented launch over "
+                            "the frequencies still running, set-up on %d host threads" % a.workers) if batched
+                           else f"{a.workers} frequencies in flight on {a.workers} streams",
+                   "autotune": bool(a.autotune) and not batched, "graph": bool(a.graph),
                    "parallelism": "single GPU"},
         "krylov_operator_gb_s": round(kbytes / wall / 1e9, 1),
         "all_converged": all(c.reasons == ("converged",) for c in cases),

@@ -187,6 +187,22 @@ int32_t helm_sample(helm_plan* plan, int32_t n_ports, const int32_t* port_ptr,
 int32_t helm_solve(helm_plan* plan, const void* rhs, void* x, int32_t nrhs,
                    const helm_solve_opts* opts, helm_solve_info* info, void* stream);
 
+/* ---- batched solve over plans of different grid sizes: replaces the
+ * per-frequency loop of nsrno/reconstruct_dense.py:91 (frequencies handed to
+ * processes, each frequency's angles batched by dense_wave.solve_system,
+ * dense_wave.py:167-225).  plans[k], rhs[k], x[k] (c128 [nrhs][N_k]) for
+ * k < nplans (<= 256), the same nrhs (<= 64) for all; info is a HOST array
+ * [nplans][nrhs].  Multigrid FGMRES only (opts.method HELM_GMRES, precond
+ * HELM_PC_MG; every plan set up with pre = post = 2 and a finest level above
+ * the LDS tail size); opts.callback, fallback and stagnation are not used.
+ * Each kernel of a restart cycle is one segmented launch over every plan
+ * still running; with use_graph the cycle is captured once and replayed.
+ * Per plan the numerics are those of helm_solve on the same plan with its
+ * default launch shapes (the autotuned stencil shape is not used). */
+int32_t helm_batch_solve(int32_t nplans, helm_plan* const* plans, const void* const* rhs,
+                         void* const* x, int32_t nrhs, const helm_solve_opts* opts,
+                         helm_solve_info* info, void* stream);
+
 /* ---- reverse-communication FGMRES: replaces nsrno/wave_solver.py:9-36
  * (gmres(operator, precondition, rhs, x, callback, restart, max_cycles,
  * tolerance)) for operators and preconditioners the CALLER applies (any

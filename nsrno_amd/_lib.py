@@ -81,6 +81,8 @@ SIGNATURES = {
                               c_int32, c_void_p]),
     "helm_solve": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, POINTER(HelmSolveOpts),
                              POINTER(HelmSolveInfo), c_void_p]),
+    "helm_batch_solve": (c_int32, [c_int32, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p), c_int32,
+                                   POINTER(HelmSolveOpts), POINTER(HelmSolveInfo), c_void_p]),
     "helm_krylov_create": (c_int32, [ctypes.c_int64, c_int32, c_int32, POINTER(c_void_p)]),
     "helm_krylov_destroy": (c_int32, [c_void_p]),
     "helm_krylov_begin": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int32, POINTER(c_double),

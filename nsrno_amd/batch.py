@@ -21,7 +21,7 @@ import torch
 from scipy.ndimage import map_coordinates
 
 from . import media
-from .helmholtz import HelmholtzPlan, Ports
+from .helmholtz import HelmholtzPlan, Ports, solve_batch
 from .theranostic import Grid, Solver
 
 
@@ -86,13 +86,10 @@ def _mg_levels(n, coarsest=40):
     return levels
 
 
-def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False,
-               mg_shift=0.4, keep_fields=False, plans=None):
-    """One frequency: all angles as one batched multigrid-FGMRES solve.
-
-    plans: a list that receives the plan instead of closing it here (run_sweep
-    closes them after every worker has finished, so that no plan's memory is
-    freed while another thread captures a restart cycle)."""
+def setup_case(medium, case, angles, restart, n_elements=192, mg_shift=0.4, autotune=True):
+    """One frequency's plan on the current stream: medium resampled to the
+    case's grid, multigrid, ports and right-hand sides.  Returns (plan,
+    ports, rhs, setup) with setup = dict(work, weights, amplitudes)."""
     p = medium.pml
     n = case.interior + 2 * p
     c, rho = medium.sample(case)
@@ -100,7 +97,6 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     pos, nrm, wid = media.line_array(work, n_elements)
     weights = Solver(Grid(work.x_mm, work.z_mm, work.pml_mm), c0_m_s=work.c0).source_weights(pos, nrm, wid)
     amp = media.plane_wave_amplitudes(pos, angles, case.frequency_mhz, work.c0)
-    t0 = time.perf_counter()
     plan = HelmholtzPlan(n, n, case.h_mm, case.frequency_mhz, work.pml_mm, c0_m_s=work.c0, nrhs=len(angles),
                          restart=restart)
     ports = Ports(weights)
@@ -109,12 +105,14 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     if autotune:
         case.tuning = plan.autotune(len(angles))
     rhs = plan.inject(ports, torch.from_numpy(amp).cuda())
-    x, info = plan.solve(rhs, tolerance=tol, restart=restart, max_iterations=6000, precond="mg", graph=graph)
+    return plan, ports, rhs, dict(work=work, weights=weights, amplitudes=amp)
+
+
+def finish_case(case, plan, ports, x, info, restart, keep_fields=False, setup=None):
+    """Traces, convergence summary and algorithmic bytes of a solved case."""
+    n = plan.nz * plan.nx
     traces = plan.sample(ports, x)
-    if stream is not None:
-        stream.synchronize()
-    case.seconds = time.perf_counter() - t0
-    case.nodes = n * n
+    case.nodes = n
     case.iterations = [i.iterations for i in info]
     case.relative_residual = max(i.relative_residual for i in info)
     case.reasons = tuple(sorted(set(i.reason for i in info)))
@@ -122,12 +120,29 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
     case.graph_fallbacks = max(i.graph_fallbacks for i in info)
     if keep_fields:
         case.fields = x.cpu().numpy()
-        case.setup = dict(work=work, weights=weights, amplitudes=amp)
+        case.setup = setup
     # algorithmic Arnoldi + operator bytes (DCGS2 step j: 8(j+2) + 8(j+4); stencil 16 + 12/R per node per RHS)
-    R = len(angles)
+    R = len(info)
     steps = max(case.iterations)
     js = np.arange(steps) % restart
-    case.krylov_bytes = float(np.sum(16.0 * js + 48.0 + 16.0 + 12.0 / R) * n * n * R)
+    case.krylov_bytes = float(np.sum(16.0 * js + 48.0 + 16.0 + 12.0 / R) * n * R)
+    return case
+
+
+def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, autotune=True, graph=False,
+               mg_shift=0.4, keep_fields=False, plans=None):
+    """One frequency: all angles as one batched multigrid-FGMRES solve.
+
+    plans: a list that receives the plan instead of closing it here (run_sweep
+    closes them after every worker has finished, so that no plan's memory is
+    freed while another thread captures a restart cycle)."""
+    t0 = time.perf_counter()
+    plan, ports, rhs, setup = setup_case(medium, case, angles, restart, n_elements, mg_shift, autotune)
+    x, info = plan.solve(rhs, tolerance=tol, restart=restart, max_iterations=6000, precond="mg", graph=graph)
+    finish_case(case, plan, ports, x, info, restart, keep_fields, setup)
+    if stream is not None:
+        stream.synchronize()
+    case.seconds = time.perf_counter() - t0
     if plans is None:
         plan.close()
     else:
@@ -136,12 +151,20 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
 
 
 def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, autotune=True, graph=False,
-              mg_shift=0.4, keep_fields=()):
-    """Solve every frequency (largest grids first) on `workers` concurrent
-    streams.  Returns (cases in input order, wall seconds).  keep_fields:
-    indices of the frequencies whose fields are copied to the host.  The
-    plans stay allocated until every worker has finished (a few GB for a
-    64-frequency sweep) and are closed after the timed region."""
+              mg_shift=0.4, keep_fields=(), mode="threads"):
+    """Solve every frequency.  mode "batched": all frequencies in one
+    helm_batch_solve (every kernel of a restart cycle one segmented launch
+    over the frequencies still running; set-up on `workers` host threads);
+    mode "threads": each frequency its own solve, `workers` of them in
+    flight on concurrent streams (largest grids first).  Returns (cases in
+    input order, wall seconds).  keep_fields: indices of the frequencies
+    whose fields are copied to the host.  The plans stay allocated until
+    every frequency is solved (a few GB for a 64-frequency sweep) and are
+    closed after the timed region."""
+    if mode == "batched":
+        return run_sweep_batched(frequencies, angles, tol, restart, workers, seed, log, graph, mg_shift, keep_fields)
+    if mode != "threads":
+        raise ValueError(f"unknown sweep mode {mode!r}")
     medium = PhysicalMedium(seed=seed)
     cases = [medium.case(f) for f in frequencies]
     order = sorted(range(len(cases)), key=lambda k: -cases[k].interior)
@@ -184,4 +207,67 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
         p.close()
     if errors:
         raise errors[0]
+    return cases, wall
+
+
+def run_sweep_batched(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, graph=True,
+                      mg_shift=0.4, keep_fields=()):
+    """run_sweep(mode="batched"): set-up of every frequency (medium
+    resampling, plan, multigrid, ports; `workers` host threads, each on its
+    own stream), then ONE helm_batch_solve over all of them, then traces.
+    The launch shapes are the library defaults (one segmented launch serves
+    every grid, so there is no per-frequency autotuning).  Each case's
+    seconds is its set-up time; the wall covers set-up, solve and traces."""
+    medium = PhysicalMedium(seed=seed)
+    cases = [medium.case(f) for f in frequencies]
+    order = sorted(range(len(cases)), key=lambda k: -cases[k].interior)
+    built = [None] * len(cases)
+    errors = []
+    work = queue.Queue()
+    for k in order:
+        work.put(k)
+
+    def worker():
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            while True:
+                try:
+                    k = work.get_nowait()
+                except queue.Empty:
+                    break
+                try:
+                    t0 = time.perf_counter()
+                    built[k] = setup_case(medium, cases[k], angles, restart, mg_shift=mg_shift, autotune=False)
+                    cases[k].seconds = time.perf_counter() - t0
+                except Exception as e:  # surfaced after the join
+                    errors.append(e)
+                    break
+            stream.synchronize()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker) for _ in range(max(1, workers))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    try:
+        if errors:
+            raise errors[0]
+        t_setup = time.perf_counter() - t0
+        plans = [b[0] for b in built]
+        xs, infos = solve_batch(plans, [b[2] for b in built], tolerance=tol, restart=restart, max_iterations=6000,
+                                graph=graph)
+        keep = set(keep_fields)
+        for k, (plan, ports, rhs, setup) in enumerate(built):
+            finish_case(cases[k], plan, ports, xs[k], infos[k], restart, k in keep, setup)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if log:
+            log(f"batched sweep: set-up {t_setup:.2f} s, solve + traces {wall - t_setup:.2f} s, "
+                f"steps {max(max(c.iterations) for c in cases)}")
+    finally:
+        for b in built:
+            if b is not None:
+                b[0].close()
     return cases, wall

@@ -36,6 +36,45 @@ __host__ __device__ inline void cfma(double2& a, double2 b, double2 c) {
     a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
     a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
 }
+// Block indices of a launch as values: the hardware's, or one segment's
+// share of a segmented launch (one launch over the grids of several plans,
+// the batched solve).  Kernel bodies take a VBlock instead of reading
+// blockIdx / gridDim, so that the same body serves both launches.
+struct VBlock {
+    int x, y, z, gx, gy, gz;
+};
+__device__ __forceinline__ VBlock hw_block() {
+    return VBlock{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                  (int)gridDim.x,  (int)gridDim.y,  (int)gridDim.z};
+}
+
+// Segmented launch: a 1-D grid in which segment s owns the workgroups
+// [start_s, start_s + gx gy gz); starts increase and are multiples of 8, so
+// that a workgroup's XCD (its linear id mod 8) is that of its local id.
+struct SegDims {
+    int start, gx, gy, gz;
+};
+constexpr int SEG_MAX = 256;  // segments per launch (four ballots)
+// Segment and local block of this workgroup; false for the padding
+// workgroups at a segment's end.  Each wave finds the segment with one
+// ballot per 64 segments (lane l tests segment base + l).
+__device__ __forceinline__ bool seg_block(const SegDims* __restrict__ dims, int nseg, int& seg, VBlock& B) {
+    const int b = blockIdx.x;
+    const int lane = __lane_id();
+    int cnt = 0;
+    for (int base = 0; base < nseg; base += 64) {
+        const int i = base + lane;
+        cnt += __popcll(__ballot(i < nseg && dims[i].start <= b));
+    }
+    const int s = __builtin_amdgcn_readfirstlane(cnt - 1);
+    const SegDims d = dims[s];
+    const int lb = b - d.start, tot = d.gx * d.gy * d.gz;
+    seg = s;
+    if (lb >= tot) return false;
+    B = VBlock{lb % d.gx, (lb / d.gx) % d.gy, lb / (d.gx * d.gy), d.gx, d.gy, d.gz};
+    return true;
+}
+
 // XCD-aware tile order: workgroups are handed to the 8 XCDs of the device in
 // turn (workgroup b to XCD b mod 8), each XCD with its own L2.  Remapping the
 // linear workgroup id so that each XCD works on one contiguous run of tiles
@@ -44,16 +83,17 @@ __host__ __device__ inline void cfma(double2& a, double2 b, double2 c) {
 // the z index (the right-hand-side group) runs fastest in that order, so the
 // groups of one tile share an XCD and read the tile's stencil coefficients
 // and halos through one L2; otherwise it runs slowest.
-__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = true, bool rhs_fast = false) {
+__device__ __forceinline__ void xcd_tile(const VBlock& B, int& bx, int& by, int& bz, bool on = true,
+                                         bool rhs_fast = false) {
     if (!on) {
-        bx = blockIdx.x;
-        by = blockIdx.y;
-        bz = blockIdx.z;
+        bx = B.x;
+        by = B.y;
+        bz = B.z;
         return;
     }
-    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const int gx = B.gx, gy = B.gy, gz = B.gz;
     const int total = gx * gy * gz;
-    const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int b = B.x + gx * (B.y + gy * B.z);
     const int q = total / 8, r = total % 8;
     const int k = b % 8, slot = b / 8;
     const int t = k < r ? k * (q + 1) + slot : r * (q + 1) + (k - r) * q + slot;
@@ -66,6 +106,9 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = tr
         by = (t / gx) % gy;
         bz = t / (gx * gy);
     }
+}
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, bool on = true, bool rhs_fast = false) {
+    xcd_tile(hw_block(), bx, by, bz, on, rhs_fast);
 }
 
 // Workgroup barrier that orders LDS only.  __syncthreads() fences global
@@ -200,6 +243,102 @@ struct Workspace {
     size_t stat_bytes = 0;
     hipEvent_t stat_ev[2] = {nullptr, nullptr};  // recorded after each slot's k_cycle_init
 };
+
+// ---- segmented launches of the batched solve (helm_batch_solve) -----------
+// One plan's share of a segmented Krylov / stencil launch: its workspace,
+// grid and operator.  Filled on the host, read by the kernels (uniform per
+// workgroup, so the fields come in as scalar loads).
+struct SegKrylov {
+    int64_t n;
+    int nrhs, nch, nchd, nchb;  // RHS; chunks of the norm/update, dots and residual-block reductions
+    int nz, nx, ws_nrhs;        // grid; RHS capacity of the workspace (status slot layout)
+    int kmax, mm;               // basis capacity + 1 and the Hessenberg row stride of the workspace
+    float2 *V, *Z;
+    double2 *partial, *H, *sn, *g, *coef;
+    double *cs, *bnorm, *beta, *rel, *inv, *pscale, *est;
+    int *active, *cyc, *used, *iters;
+    double2* x;
+    const double2* b;
+    char* dstat;        // status slots (device view of the mapped host block)
+    int64_t stat_bytes;
+    // FP64 operator (restart residual) and FP32 operator (Krylov steps)
+    const double2 *sx64, *sz64, *isxe64, *isze64, *kap64;
+    const double* eta64;
+    double k0sq, hh;
+    const float2 *sx32, *sz32, *isxe32, *isze32, *kap32;
+    const float* eta32;
+    float k0sq32, hh32;
+};
+// One plan's grid level l of the fused V-cycle (pre = post = 2): the
+// operands of k_mgt_pre2_rr and k_mgt_post2 on that level.  On the finest
+// level f and u are the step's basis and direction vectors: f + j f_step.
+struct SegLevel {
+    int nz, nx, nzc, nxc, nrhs, rb_pre, rb_post, xcd_pre, xcd_post;
+    int64_t n;
+    const float2 *sx, *sz, *isxe, *isze, *kap, *wid, *tcf;
+    const float* eta;
+    float k0sq, hh, shift;
+    const float2* f;
+    int64_t f_step;
+    float2* u;
+    int64_t u_step;
+    const double* fscale;
+    float2 *t, *fc;     // smoothed field between the two launches; coarse right-hand side
+    const float2* uc;   // coarse correction
+    const int* act;
+};
+struct LevelView {
+    int nz, nx;
+    int64_t n;
+    const float2* cf;       // [n][6]: w/dg, dg, fw, fe, fn, fs
+    float2 *f, *r;          // [nrhs][n] global work arrays
+};
+// One plan's LDS-resident tail (levels t..last), one workgroup per RHS
+struct SegTail {
+    const LevelView* views;
+    int t, last, pre, post, coarse;
+    const float2* f;
+    float2* u;
+    const int* act;
+};
+
+// segmented kernel launches (grid: sum of the segments' padded block counts)
+void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s);
+void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s);
+void seg_mg_tail(const SegDims* dims, int nseg, int blocks, size_t lds, const SegTail* tv, int conj, hipStream_t s);
+void seg_apply32(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int conj, hipStream_t s);
+void seg_dots(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, hipStream_t s);
+void seg_reduce_update(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, double tol,
+                       hipStream_t s);
+void seg_norm_scale(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int last, hipStream_t s);
+void seg_backsolve(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s);
+void seg_xupdate(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s);
+void seg_residual(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int conj, hipStream_t s);
+void seg_cycle_init(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int slot, double tol,
+                    hipStream_t s);
+// grids of the Krylov launches of one plan (x, y, z workgroups), as the
+// single-plan solve launches them on its lazily scaled, fused path
+struct KrylovGrids {
+    int dots[3], update[3], norm[3], backsolve[3], xupdate[3], residual[3], init[3], apply[3];
+    int nch, nchd, nchb;
+};
+KrylovGrids krylov_grids(int nz, int nx, int nrhs);
+// start of a solve: ||b|| per RHS, active flags, used = 0, x = 0 unless x0 is used
+int32_t krylov_begin(helm_plan* p, const double2* b, double2* x, int nrhs, bool use_x0, hipStream_t s);
+// plan p's Krylov table entry (workspace, grid, operator) for b, x and nrhs RHS
+SegKrylov krylov_seg(helm_plan* p, const double2* b, double2* x, int nrhs);
+// multigrid: work arrays for nrhs RHS; first tail level (-1: the V-cycle
+// does not end in the LDS tail); the tail's dynamic LDS bytes
+int32_t mg_prepare(helm_plan* p, int nrhs);
+int mg_tail_level(const helm_plan* p);
+size_t mg_tail_lds(const helm_plan* p, int t);
+// level l of plan p in a batched V-cycle, with the grids of its pre- and
+// post-smoothing launches; level 0 reads f + j f_step and writes u + j u_step;
+// wg_target: workgroups one plan's launch aims for (sets the RHS per group)
+SegLevel mg_seg_level(helm_plan* p, int l, int nrhs, const float2* f, int64_t f_step, float2* u, int64_t u_step,
+                      const double* fscale, const int* act, int64_t wg_target, int pre_grid[3], int post_grid[3]);
+SegTail mg_seg_tail(helm_plan* p, int t, const int* act);
+bool mg_fused_ok(const helm_plan* p);  // pre = post = 2 on the fused level kernels
 
 // HIP-event timing of the kernels inside a solve (helm_profile)
 enum ProfKernel {

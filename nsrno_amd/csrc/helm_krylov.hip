@@ -144,13 +144,13 @@ static inline int ndotchunks(int64_t n) { return (int)((n + 256 * dot_npt() - 1)
 // partial-sum capacity: enough chunks for the smaller chunk size
 static inline int ndotchunks_max(int64_t n) { return (int)((n + 1023) / 1024); }
 template <int DOT_NPT>
-__global__ __launch_bounds__(256, DOT_NPT == 8 ? 3 : 4) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax,
+__device__ __forceinline__ void k_dcgs_dots_body(const VBlock& B, int64_t n, int nrhs, int nch, int kmax,
                                                                           int j,
                                                    const float2* __restrict__ V, const float2* __restrict__ w,
                                                    const int* __restrict__ cyc, double2* __restrict__ pc,
                                                    double2* __restrict__ pd) {
     __shared__ double4 red[64][4];  // [k][wave] = (c.re, c.im, d.re, d.im)
-    const int chunk = blockIdx.x, r = blockIdx.y;
+    const int chunk = B.x, r = B.y;
     if (!cyc[r]) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t beg = (int64_t)chunk * (256 * DOT_NPT);
@@ -210,6 +210,14 @@ __global__ __launch_bounds__(256, DOT_NPT == 8 ? 3 : 4) void k_dcgs_dots(int64_t
         pc[((int64_t)r * kmax + k) * nch + chunk] = make_double2(t.x, t.y);
         pd[((int64_t)r * kmax + k) * nch + chunk] = make_double2(t.z, t.w);
     }
+}
+template <int DOT_NPT>
+__global__ __launch_bounds__(256, DOT_NPT == 8 ? 3 : 4) void k_dcgs_dots(int64_t n, int nrhs, int nch, int kmax,
+                                                                          int j,
+                                                   const float2* __restrict__ V, const float2* __restrict__ w,
+                                                   const int* __restrict__ cyc, double2* __restrict__ pc,
+                                                   double2* __restrict__ pd) {
+    k_dcgs_dots_body<DOT_NPT>(hw_block(), n, nrhs, nch, kmax, j, V, w, cyc, pc, pd);
 }
 
 // apply rotations 0..c-1 to column c, create rotation c, update g; returns |g_{c+1}|
@@ -414,7 +422,7 @@ __global__ void k_dcgs_finish(int nrhs, int m, int j, double2* __restrict__ H, d
 // cyc before the stop: harmless, as its V_j and w are never read again (the
 // x update uses Z_0..Z_{j-1}).  Extra reads per workgroup: 2 (j + 1) nchd
 // partials (31 KB at j = 5, from L2).
-__global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs, int nch, int nchd, int m, int kmax,
+__device__ __forceinline__ void k_dcgs_reduce_update_body(const VBlock& B, int64_t n, int nrhs, int nch, int nchd, int m, int kmax,
                                                             int j, double tol, const double2* __restrict__ pc,
                                                             const double2* __restrict__ pd,
                                                             const double* __restrict__ bnorm, double2* __restrict__ H,
@@ -428,7 +436,7 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
     __shared__ float2 c_s[64], d_s[64];
     __shared__ float2 hjj_s;
     __shared__ float ialpha_s;
-    const int chunk = blockIdx.x, r = blockIdx.y;
+    const int chunk = B.x, r = B.y;
     if (!cyc[r]) return;
     const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     for (int k = threadIdx.x >> 6; k <= j; k += nw) {
@@ -516,6 +524,17 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
     const double2 tot = block_sum(make_double2(nrm, 0.0), lds);
     if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = tot;
 }
+__global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs, int nch, int nchd, int m, int kmax,
+                                                            int j, double tol, const double2* __restrict__ pc,
+                                                            const double2* __restrict__ pd,
+                                                            const double* __restrict__ bnorm, double2* __restrict__ H,
+                                                            double* __restrict__ cs, double2* __restrict__ sn,
+                                                            double2* __restrict__ g, double* __restrict__ est,
+                                                            int* __restrict__ cyc, int* __restrict__ used,
+                                                            float2* __restrict__ V, float2* __restrict__ w,
+                                                            double2* __restrict__ partial, int v0_raw) {
+    k_dcgs_reduce_update_body(hw_block(), n, nrhs, nch, nchd, m, kmax, j, tol, pc, pd, bnorm, H, cs, sn, g, est, cyc, used, V, w, partial, v0_raw);
+}
 
 // k_dcgs_norm + k_scale32 (last = 0) or k_dcgs_norm + k_dcgs_finish (last = 1,
 // one workgroup per RHS): every workgroup sums the |w|^2 partials of its RHS
@@ -529,7 +548,7 @@ __global__ __launch_bounds__(256) void k_dcgs_reduce_update(int64_t n, int nrhs,
 // by alpha' = ||w|| alpha and its coefficients c_k carry the factor ||w||, so
 // H[j+1][j] = 1 here reproduces H[j+1][j] = ||w|| alpha and
 // H[k][j] += ||w|| c_k exactly as with a normalised V_{j+1}.
-__global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int m, int j, int last,
+__device__ __forceinline__ void k_dcgs_norm_scale_body(const VBlock& B, int64_t n, int nch, int m, int j, int last,
                                                          const double2* __restrict__ partial,
                                                          const double* __restrict__ bnorm, double2* __restrict__ H,
                                                          double* __restrict__ cs, double2* __restrict__ sn,
@@ -539,7 +558,7 @@ __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int
                                                          float2* __restrict__ vnext, double* __restrict__ pscale) {
     __shared__ double lds[8];
     __shared__ double hn_s;
-    const int chunk = blockIdx.x, r = blockIdx.y;
+    const int chunk = B.x, r = B.y;
     if (!cyc[r]) return;
     const double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
     if (threadIdx.x == 0) hn_s = sqrt(s);
@@ -567,6 +586,16 @@ __global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int
         const float2 a = wr[p];
         vr[p] = make_float2(a.x * sc, a.y * sc);
     }
+}
+__global__ __launch_bounds__(256) void k_dcgs_norm_scale(int64_t n, int nch, int m, int j, int last,
+                                                         const double2* __restrict__ partial,
+                                                         const double* __restrict__ bnorm, double2* __restrict__ H,
+                                                         double* __restrict__ cs, double2* __restrict__ sn,
+                                                         double2* __restrict__ g, double* __restrict__ est,
+                                                         double* __restrict__ inv, int* __restrict__ cyc,
+                                                         int* __restrict__ used, const float2* __restrict__ w,
+                                                         float2* __restrict__ vnext, double* __restrict__ pscale) {
+    k_dcgs_norm_scale_body(hw_block(), n, nch, m, j, last, partial, bnorm, H, cs, sn, g, est, inv, cyc, used, w, vnext, pscale);
 }
 
 // partial[r*nch+chunk] = |v[r]|^2 over the chunk (c128 or c64 vector)
@@ -601,14 +630,14 @@ __global__ void k_bnorm(int nch, const double2* __restrict__ partial, double* __
 }
 
 // Start of a cycle: beta = ||b - A x||, decide convergence, reset rotations.
-__global__ void k_cycle_init(int nch, int m, double tol, const double2* __restrict__ partial,
+__device__ __forceinline__ void k_cycle_init_body(const VBlock& B, int nch, int m, double tol, const double2* __restrict__ partial,
                              const double* __restrict__ bnorm, double* __restrict__ beta, double* __restrict__ rel,
                              double* __restrict__ inv, int* __restrict__ active, int* __restrict__ cyc,
                              int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g,
                              double* __restrict__ pscale, double* __restrict__ hrel, int* __restrict__ hact,
                              int* __restrict__ hit, int pscale_v0) {
     __shared__ double lds[8];
-    const int r = blockIdx.x;
+    const int r = B.x;
     double s = reduce_chunks(partial + (int64_t)r * nch, nch, lds);
     if (threadIdx.x == 0) {
         double b = sqrt(s);
@@ -630,6 +659,14 @@ __global__ void k_cycle_init(int nch, int m, double tol, const double2* __restri
         g[(int64_t)r * (m + 1)] = make_double2(b, 0.0);
         for (int k = 1; k <= m; ++k) g[(int64_t)r * (m + 1) + k] = make_double2(0.0, 0.0);
     }
+}
+__global__  void k_cycle_init(int nch, int m, double tol, const double2* __restrict__ partial,
+                             const double* __restrict__ bnorm, double* __restrict__ beta, double* __restrict__ rel,
+                             double* __restrict__ inv, int* __restrict__ active, int* __restrict__ cyc,
+                             int* __restrict__ used, int* __restrict__ iters, double2* __restrict__ g,
+                             double* __restrict__ pscale, double* __restrict__ hrel, int* __restrict__ hact,
+                             int* __restrict__ hit, int pscale_v0) {
+    k_cycle_init_body(hw_block(), nch, m, tol, partial, bnorm, beta, rel, inv, active, cyc, used, iters, g, pscale, hrel, hact, hit, pscale_v0);
 }
 
 // v0[r] = r64[r] / beta_r  (c128 -> c64)
@@ -708,7 +745,7 @@ __device__ __forceinline__ double2 shfl_down1d(double2 v) {
 }
 
 template <int RB, int TZ>
-__global__ __launch_bounds__(256) void k_residual_block(int nz, int nx, int64_t n, int nrhs,
+__device__ __forceinline__ void k_residual_block_body(const VBlock& B, int nz, int nx, int64_t n, int nrhs,
                                                         const double2* __restrict__ sx,
                                                         const double2* __restrict__ sz,
                                                         const double2* __restrict__ isxe,
@@ -721,10 +758,10 @@ __global__ __launch_bounds__(256) void k_residual_block(int nz, int nx, int64_t 
     static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
     __shared__ double red[4][RB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 64, j = x0 + lane;
-    const int i0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + wave) * TZ);
-    const int r0 = blockIdx.z * RB;
-    const int nchr = gridDim.x * gridDim.y, wg = blockIdx.y * gridDim.x + blockIdx.x;
+    const int x0 = B.x * 64, j = x0 + lane;
+    const int i0 = __builtin_amdgcn_readfirstlane((B.y * 4 + wave) * TZ);
+    const int r0 = B.z * RB;
+    const int nchr = B.gx * B.gy, wg = B.y * B.gx + B.x;
     double nrm[RB];
 #pragma unroll
     for (int q = 0; q < RB; ++q) nrm[q] = 0.0;
@@ -834,6 +871,19 @@ __global__ __launch_bounds__(256) void k_residual_block(int nz, int nx, int64_t 
         partial[(int64_t)(r0 + q) * nchr + wg] = make_double2(tot, 0.0);
     }
 }
+template <int RB, int TZ>
+__global__ __launch_bounds__(256) void k_residual_block(int nz, int nx, int64_t n, int nrhs,
+                                                        const double2* __restrict__ sx,
+                                                        const double2* __restrict__ sz,
+                                                        const double2* __restrict__ isxe,
+                                                        const double2* __restrict__ isze,
+                                                        const double* __restrict__ eta,
+                                                        const double2* __restrict__ kap, double k0sq, double hh,
+                                                        int conj, const double2* __restrict__ x,
+                                                        const double2* __restrict__ b, float2* __restrict__ v0,
+                                                        double2* __restrict__ partial) {
+    k_residual_block_body<RB, TZ>(hw_block(), nz, nx, n, nrhs, sx, sz, isxe, isze, eta, kap, k0sq, hh, conj, x, b, v0, partial);
+}
 
 // v_{j+1}[r] = w[r] / h_{j+1,j}
 __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __restrict__ dst,
@@ -848,9 +898,9 @@ __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __r
 }
 
 // y = R^{-1} g (upper triangular, used[r] columns), stored in coef[r][k]
-__global__ void k_backsolve(int nrhs, int m, int kmax, const double2* __restrict__ H,
+__device__ __forceinline__ void k_backsolve_body(const VBlock& B, int nrhs, int m, int kmax, const double2* __restrict__ H,
                             const double2* __restrict__ g, const int* __restrict__ used, double2* __restrict__ y) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = B.x * blockDim.x + threadIdx.x;
     if (r >= nrhs) return;
     const int u = used[r];
     const double2* Hr = H + (int64_t)r * m * (m + 1);
@@ -864,18 +914,22 @@ __global__ void k_backsolve(int nrhs, int m, int kmax, const double2* __restrict
                        : make_double2(0.0, 0.0);
     }
 }
+__global__  void k_backsolve(int nrhs, int m, int kmax, const double2* __restrict__ H,
+                            const double2* __restrict__ g, const int* __restrict__ used, double2* __restrict__ y) {
+    k_backsolve_body(hw_block(), nrhs, m, kmax, H, g, used, y);
+}
 
 // x[r] += sum_{k<used[r]} y_k Z_k[r]   (FP64 accumulate into c128 x)
-__global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, const float2* __restrict__ Z,
+__device__ __forceinline__ void k_xupdate_body(const VBlock& B, int64_t n, int nrhs, int kmax, const float2* __restrict__ Z,
                                                  const double2* __restrict__ y, const int* __restrict__ used,
                                                  double2* __restrict__ x) {
     __shared__ double2 yy[64];
-    const int r = blockIdx.y;
+    const int r = B.y;
     const int u = used[r];
     if (u == 0) return;
     for (int k = threadIdx.x; k < u; k += blockDim.x) yy[k] = y[(int64_t)r * kmax + k];
     __syncthreads();
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t p = (int64_t)B.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t vstride = (int64_t)nrhs * n;
     const float2* zr = Z + (int64_t)r * n;
@@ -887,6 +941,11 @@ __global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, 
         acc.y += c.x * z.y + c.y * z.x;
     }
     x[(int64_t)r * n + p] = acc;
+}
+__global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, const float2* __restrict__ Z,
+                                                 const double2* __restrict__ y, const int* __restrict__ used,
+                                                 double2* __restrict__ x) {
+    k_xupdate_body(hw_block(), n, nrhs, kmax, Z, y, used, x);
 }
 
 __global__ void k_bi_izero_k(int count, int* p) {
@@ -903,6 +962,125 @@ __global__ void k_stop_rhs(int r, int* __restrict__ active, int* __restrict__ cy
 __global__ void k_zero_c128(int64_t count, double2* p) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = make_double2(0.0, 0.0);
+}
+
+// ---- segmented forms: one launch over the plans of a batched solve ---------
+// Each workgroup finds its plan (segment) and local block indices, then runs
+// the single-plan kernel body on that plan's arrays.  The batched solve takes
+// the single-plan solve's lazily scaled, fused path: w = A z_j sits in
+// V_{j+1}'s slot, the restart residual goes straight into V_0.
+__device__ __forceinline__ int64_t seg_vstride(const SegKrylov& a) { return (int64_t)a.nrhs * a.n; }
+__global__ __launch_bounds__(256, 3) void k_seg_dots(const SegDims* __restrict__ dims, int nseg,
+                                                     const SegKrylov* __restrict__ seg, int j) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    k_dcgs_dots_body<8>(B, a.n, a.nrhs, a.nchd, a.kmax, j, a.V, a.V + (j + 1) * seg_vstride(a), a.cyc, a.partial,
+                        a.partial + (int64_t)a.nrhs * a.kmax * a.nchd);
+}
+__global__ __launch_bounds__(256) void k_seg_reduce_update(const SegDims* __restrict__ dims, int nseg,
+                                                           const SegKrylov* __restrict__ seg, int j, double tol) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    const int64_t pd = (int64_t)a.nrhs * a.kmax * a.nchd;
+    k_dcgs_reduce_update_body(B, a.n, a.nrhs, a.nch, a.nchd, a.mm, a.kmax, j, tol, a.partial, a.partial + pd,
+                              a.bnorm, a.H, a.cs, a.sn, a.g, a.est, a.cyc, a.used, a.V,
+                              a.V + (j + 1) * seg_vstride(a), a.partial + 2 * pd, 1);
+}
+__global__ __launch_bounds__(256) void k_seg_norm_scale(const SegDims* __restrict__ dims, int nseg,
+                                                        const SegKrylov* __restrict__ seg, int j, int last) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    float2* wj = a.V + (j + 1) * seg_vstride(a);
+    k_dcgs_norm_scale_body(B, a.n, a.nch, a.mm, j, last, a.partial + (int64_t)2 * a.nrhs * a.kmax * a.nchd,
+                           a.bnorm, a.H, a.cs, a.sn, a.g, a.est, a.inv, a.cyc, a.used, wj, wj, a.pscale);
+}
+__global__ void k_seg_backsolve(const SegDims* __restrict__ dims, int nseg, const SegKrylov* __restrict__ seg) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    k_backsolve_body(B, a.nrhs, a.mm, a.kmax, a.H, a.g, a.used, a.coef);
+}
+__global__ __launch_bounds__(256) void k_seg_xupdate(const SegDims* __restrict__ dims, int nseg,
+                                                     const SegKrylov* __restrict__ seg) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    k_xupdate_body(B, a.n, a.nrhs, a.kmax, a.Z, a.coef, a.used, a.x);
+}
+__global__ __launch_bounds__(256) void k_seg_residual(const SegDims* __restrict__ dims, int nseg,
+                                                      const SegKrylov* __restrict__ seg, int conj) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    k_residual_block_body<RES_RB, RES_TZ>(B, a.nz, a.nx, a.n, a.nrhs, a.sx64, a.sz64, a.isxe64, a.isze64, a.eta64,
+                                          a.kap64, a.k0sq, a.hh, conj, a.x, a.b, a.V, a.partial);
+}
+__global__ void k_seg_cycle_init(const SegDims* __restrict__ dims, int nseg, const SegKrylov* __restrict__ seg,
+                                 int slot, double tol) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    char* d = a.dstat + (int64_t)slot * a.stat_bytes;
+    k_cycle_init_body(B, a.nchb, a.mm, tol, a.partial, a.bnorm, a.beta, a.rel, a.inv, a.active, a.cyc, a.used,
+                      a.iters, a.g, a.pscale, (double*)d, (int*)(d + (size_t)a.ws_nrhs * sizeof(double)),
+                      (int*)(d + (size_t)a.ws_nrhs * (sizeof(double) + sizeof(int))), 1);
+}
+
+void seg_dots(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_dots, dim3(blocks), dim3(256), 0, s, dims, nseg, k, j);
+}
+void seg_reduce_update(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, double tol,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_reduce_update, dim3(blocks), dim3(256), 0, s, dims, nseg, k, j, tol);
+}
+void seg_norm_scale(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int last, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_norm_scale, dim3(blocks), dim3(256), 0, s, dims, nseg, k, j, last);
+}
+void seg_backsolve(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_backsolve, dim3(blocks), dim3(64), 0, s, dims, nseg, k);
+}
+void seg_xupdate(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_xupdate, dim3(blocks), dim3(256), 0, s, dims, nseg, k);
+}
+void seg_residual(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int conj, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_residual, dim3(blocks), dim3(256), 0, s, dims, nseg, k, conj);
+}
+void seg_cycle_init(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int slot, double tol,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_cycle_init, dim3(blocks), dim3(64), 0, s, dims, nseg, k, slot, tol);
+}
+
+KrylovGrids krylov_grids(int nz, int nx, int nrhs) {
+    KrylovGrids g{};
+    const int64_t n = (int64_t)nz * nx;
+    g.nch = nchunks(n);
+    g.nchd = (int)((n + 2047) / 2048);  // k_dcgs_dots<8>: 2048-node chunks
+    const int bx = (nx + 63) / 64, by = (nz + 4 * RES_TZ - 1) / (4 * RES_TZ);
+    g.nchb = bx * by;
+    auto set = [](int* d, int x, int y, int z) {
+        d[0] = x;
+        d[1] = y;
+        d[2] = z;
+    };
+    set(g.dots, g.nchd, nrhs, 1);
+    set(g.update, g.nch, nrhs, 1);
+    set(g.norm, 1, nrhs, 1);
+    set(g.backsolve, (nrhs + 63) / 64, 1, 1);
+    set(g.xupdate, (int)((n + 255) / 256), nrhs, 1);
+    set(g.residual, bx, by, (nrhs + RES_RB - 1) / RES_RB);
+    set(g.init, nrhs, 1, 1);
+    set(g.apply, (nx + 63) / 64, ((nz + 3) / 4 + 3) / 4, (nrhs + 3) / 4);  // k_apply_block<4, 4>
+    return g;
 }
 
 template <typename T>
@@ -968,6 +1146,78 @@ int32_t ensure_workspace(helm_plan* p, int nrhs, int restart) {
         HELM_HIP(hipHostGetDevicePointer((void**)&base, w.hstat, 0));
         w.dstat = base;
     }
+    return HELM_OK;
+}
+
+SegKrylov krylov_seg(helm_plan* p, const double2* b, double2* x, int nrhs) {
+    const Workspace& W = p->ws;
+    const Level& L = p->levels[0];
+    const KrylovGrids g = krylov_grids(L.nz, L.nx, nrhs);
+    SegKrylov a{};
+    a.n = L.n;
+    a.nrhs = nrhs;
+    a.nch = g.nch;
+    a.nchd = g.nchd;
+    a.nchb = g.nchb;
+    a.nz = L.nz;
+    a.nx = L.nx;
+    a.ws_nrhs = W.nrhs;
+    a.kmax = W.restart + 1;
+    a.mm = W.restart;
+    a.V = W.V;
+    a.Z = W.Z;
+    a.partial = W.partial;
+    a.H = W.H;
+    a.sn = W.sn;
+    a.g = W.g;
+    a.coef = W.coef;
+    a.cs = W.cs;
+    a.bnorm = W.bnorm;
+    a.beta = W.beta;
+    a.rel = W.rel;
+    a.inv = W.inv;
+    a.pscale = W.pscale;
+    a.est = W.est;
+    a.active = W.active;
+    a.cyc = W.cyc;
+    a.used = W.used;
+    a.iters = W.iters;
+    a.sx64 = L.sx64;
+    a.sz64 = L.sz64;
+    a.isxe64 = L.isxe64;
+    a.isze64 = L.isze64;
+    a.kap64 = L.kap64;
+    a.eta64 = L.eta64;
+    a.sx32 = L.sx32;
+    a.sz32 = L.sz32;
+    a.isxe32 = L.isxe32;
+    a.isze32 = L.isze32;
+    a.kap32 = L.kap32;
+    a.eta32 = L.eta32;
+    a.x = x;
+    a.b = b;
+    a.dstat = W.dstat;
+    a.stat_bytes = (int64_t)W.stat_bytes;
+    a.k0sq = p->k0 * p->k0;
+    a.hh = 0.5 / (L.h_m * L.h_m);
+    a.k0sq32 = (float)a.k0sq;
+    a.hh32 = (float)a.hh;
+    return a;
+}
+
+// start of a solve, as in solve_gmres: ||b|| per RHS, active flags, used = 0, x = 0 (unless x holds the start)
+int32_t krylov_begin(helm_plan* p, const double2* b, double2* x, int nrhs, bool use_x0, hipStream_t s) {
+    Workspace& W = p->ws;
+    const int64_t n = p->levels[0].n;
+    const int nch = nchunks(n);
+    hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nch, nrhs), dim3(256), 0, s, n, nch, b, W.partial);
+    hipLaunchKernelGGL(k_bnorm, dim3(nrhs), dim3(64), 0, s, nch, W.partial, W.bnorm, W.active, W.iters);
+    HELM_HIP(hipMemsetAsync(W.used, 0, nrhs * sizeof(int), s));
+    if (!use_x0) {
+        const int64_t cnt = (int64_t)nrhs * n;
+        hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, cnt, x);
+    }
+    HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
 

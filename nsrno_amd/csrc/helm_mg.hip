@@ -225,12 +225,6 @@ static int64_t tail_nodes() {
 }
 
 
-struct LevelView {
-    int nz, nx;
-    int64_t n;
-    const float2* cf;       // [n][6]: w/dg, dg, fw, fe, fn, fs
-    float2 *f, *r;          // [nrhs][n] global work arrays
-};
 
 __global__ void k_tail_coef(int nz, int nx, const float2* __restrict__ sx, const float2* __restrict__ sz,
                             const float2* __restrict__ isxe, const float2* __restrict__ isze,
@@ -431,11 +425,11 @@ __device__ void tail_sweeps_p(int nz, int nx, int n, const float2* cf, int sweep
     }
 }
 
-__global__ __launch_bounds__(1024) void k_tail_lds(const LevelView* __restrict__ views, int t, int last, int conj_i,
+__device__ __forceinline__ void k_tail_lds_body(const VBlock& B, const LevelView* __restrict__ views, int t, int last, int conj_i,
                                                    int pre, int post, int coarse, const float2* __restrict__ f_top,
                                                    float2* __restrict__ u_top, const int* __restrict__ act) {
     extern __shared__ float2 lds[];
-    const int r = blockIdx.x;
+    const int r = B.x;
     if (act && !act[r]) return;
     const bool conj = conj_i != 0;
     int off[16];  // level l's block: u, partner, f, 6 coefficient planes (9 n float2)
@@ -532,6 +526,11 @@ __global__ __launch_bounds__(1024) void k_tail_lds(const LevelView* __restrict__
     }
     const int n = (int)views[t].n;
     for (int q = threadIdx.x; q < n; q += blockDim.x) u_top[(int64_t)r * n + q] = lds[off[0] + q];
+}
+__global__ __launch_bounds__(1024) void k_tail_lds(const LevelView* __restrict__ views, int t, int last, int conj_i,
+                                                   int pre, int post, int coarse, const float2* __restrict__ f_top,
+                                                   float2* __restrict__ u_top, const int* __restrict__ act) {
+    k_tail_lds_body(hw_block(), views, t, last, conj_i, pre, post, coarse, f_top, u_top, act);
 }
 
 // ---- fused grid-level V-cycle kernels ----------------------------------
@@ -949,7 +948,7 @@ constexpr int PR_2Z = PR_RZ + 2, PR_2X = PR_RX + 2, PR_2N = PR_2Z * PR_2X;  // u
 constexpr int PR_1Z = PR_RZ + 4, PR_1X = PR_RX + 4, PR_1N = PR_1Z * PR_1X;  // f, u1 region 21 x 69
 constexpr int PR_1PT = (PR_1N + 255) / 256, PR_2PT = (PR_2N + 255) / 256;
 template <bool PF>
-__global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs,
+__device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int nxf, int nzc, int nxc, int nrhs,
                                                                int rb,
                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
                                                      const float2* __restrict__ isxe,
@@ -965,7 +964,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
     __shared__ float2 s2[PR_2N];
     const int tid = threadIdx.y * RR_TX + threadIdx.x;
     int bx, by, bz;
-    xcd_tile(bx, by, bz, xcd != 0, xcd == 2);
+    xcd_tile(B, bx, by, bz, xcd != 0, xcd == 2);
     if (!group_active(act, bz * rb, min(nrhs, bz * rb + rb))) return;
     const int I0 = by * RR_TZ, J0 = bx * RR_TX;
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
@@ -1089,6 +1088,20 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
         }
     }
 }
+template <bool PF>
+__global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs,
+                                                               int rb,
+                                                     const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                     const float2* __restrict__ isxe,
+                                                     const float2* __restrict__ isze, const float* __restrict__ eta,
+                                                     const float2* __restrict__ kap, float k0sq, float hh,
+                                                     float shift, int conj, const float2* __restrict__ wid,
+                                                     const float2* __restrict__ tcf, const float2* __restrict__ f,
+                                                     const double* __restrict__ fscale, float2* __restrict__ u,
+                                                     float2* __restrict__ fcoarse, int xcd,
+                                                     const int* __restrict__ act) {
+    k_mgt_pre2_rr_body<PF>(hw_block(), nzf, nxf, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, f, fscale, u, fcoarse, xcd, act);
+}
 
 // k_mgt_post2: out = S(S(u + P u_c)) in one pass on the 16 x 64 fine tile:
 // v = u + P u_c is formed on the tile plus a 2-node halo (20 x 68), the first
@@ -1101,7 +1114,7 @@ constexpr int P2_PT = (P2_N + 255) / 256;
 // PF: the loads of the next right-hand side are issued before this one is
 // computed (more registers, two waves per SIMD); without, three waves per SIMD
 template <bool PF>
-__global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
+__device__ __forceinline__ void k_mgt_post2_body(const VBlock& B, int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
                                                              int rb,
                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
                                                    const float2* __restrict__ isxe, const float2* __restrict__ isze,
@@ -1115,7 +1128,7 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
     __shared__ float2 sv1[MT_NS];
     __shared__ float2 scv[MP_NC];
     int bx, by, bz;
-    xcd_tile(bx, by, bz, xcd != 0, xcd == 2);
+    xcd_tile(B, bx, by, bz, xcd != 0, xcd == 2);
     if (!group_active(act, bz * rb, min(nrhs, bz * rb + rb))) return;
     const int z0 = by * MT_TZ, x0 = bx * MT_TX, tid = threadIdx.y * 64 + threadIdx.x;
     const int cz0 = z0 / 2 - 1, cx0 = x0 / 2 - 1;
@@ -1241,6 +1254,19 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
             out[(int64_t)r * n + i1[q]] = v;
         }
     }
+}
+template <bool PF>
+__global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
+                                                             int rb,
+                                                   const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                   const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                   const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                   float k0sq, float hh, float shift, int conj,
+                                                   const float2* __restrict__ wid, const float2* __restrict__ tcf,
+                                                   const float2* __restrict__ uc, const float2* __restrict__ u,
+                                                   const float2* __restrict__ f, const double* __restrict__ fscale,
+                                                   float2* __restrict__ out, int xcd, const int* __restrict__ act) {
+    k_mgt_post2_body<PF>(hw_block(), nz, nx, n, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, uc, u, f, fscale, out, xcd, act);
 }
 
 // bilinear prolongation of the coarse correction at fine node (i, j) (zero
@@ -1550,6 +1576,142 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
     if (src != u)
         HELM_HIP(hipMemcpyAsync(u, src, (size_t)nrhs * L.n * sizeof(float2), hipMemcpyDeviceToDevice, s));
     return HELM_OK;
+}
+
+// ---- batched V-cycle (helm_batch_solve) ------------------------------------
+// The fused level kernels and the LDS tail over the plans of a batch, one
+// launch per level and kernel: level l's launch covers the plans for which
+// l is a grid level above the tail.  Plans are independent, so running every
+// plan's pre-smoothing of level l before any plan's level l + 1 keeps each
+// plan's own order (pre 0..t-1, tail at t, post t-1..0).
+__global__ __launch_bounds__(256, 3) void k_seg_mg_pre(const SegDims* __restrict__ dims, int nseg,
+                                                       const SegLevel* __restrict__ lv, int j, int conj) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegLevel& a = lv[s];
+    k_mgt_pre2_rr_body<false>(B, a.nz, a.nx, a.nzc, a.nxc, a.nrhs, a.rb_pre, a.sx, a.sz, a.isxe, a.isze, a.eta,
+                              a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.f + j * a.f_step, a.fscale, a.t,
+                              a.fc, a.xcd_pre, a.act);
+}
+__global__ __launch_bounds__(256, 3) void k_seg_mg_post(const SegDims* __restrict__ dims, int nseg,
+                                                        const SegLevel* __restrict__ lv, int j, int conj) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegLevel& a = lv[s];
+    k_mgt_post2_body<false>(B, a.nz, a.nx, a.n, a.nzc, a.nxc, a.nrhs, a.rb_post, a.sx, a.sz, a.isxe, a.isze, a.eta,
+                            a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.uc, a.t, a.f + j * a.f_step,
+                            a.fscale, a.u + j * a.u_step, a.xcd_post, a.act);
+}
+__global__ __launch_bounds__(1024) void k_seg_mg_tail(const SegDims* __restrict__ dims, int nseg,
+                                                      const SegTail* __restrict__ tv, int conj) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegTail& a = tv[s];
+    k_tail_lds_body(B, a.views, a.t, a.last, conj, a.pre, a.post, a.coarse, a.f, a.u, a.act);
+}
+void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_mg_pre, dim3(blocks), dim3(RR_TX, RR_TZ), 0, s, dims, nseg, lv, j, conj);
+}
+void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_mg_post, dim3(blocks), dim3(64, 4), 0, s, dims, nseg, lv, j, conj);
+}
+void seg_mg_tail(const SegDims* dims, int nseg, int blocks, size_t lds, const SegTail* tv, int conj, hipStream_t s) {
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)k_seg_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)attr;
+    hipLaunchKernelGGL(k_seg_mg_tail, dim3(blocks), dim3(1024), lds, s, dims, nseg, tv, conj);
+}
+
+// work arrays and tail views for nrhs right-hand sides (as mg_vcycle does before a cycle)
+int32_t mg_prepare(helm_plan* p, int nrhs) {
+    HELM_CHECK(p->mg.levels > 0, HELM_ERR_STATE, "multigrid not set up");
+    bool grew = false;
+    for (auto& L : p->levels) {
+        grew = grew || L.mg_cap < nrhs;
+        HELM_TRY(level_alloc_work(L, nrhs));
+    }
+    if (grew || !p->mg_views_valid) {
+        HELM_TRY(tail_views(p));
+        p->mg_views_valid = true;
+        p->generation++;
+    }
+    return HELM_OK;
+}
+// first level of the LDS-resident tail, or -1 when the V-cycle does not end
+// in it (a coarsest level above the tail size, or a tail too large for LDS)
+int mg_tail_level(const helm_plan* p) {
+    const int nl = (int)p->levels.size();
+    for (int l = 0; l < nl; ++l)
+        if (p->levels[l].n <= tail_nodes())
+            return tail_all_lds() && tail_lds_bytes_full(p, l) <= 160 * 1024 ? l : -1;
+    return -1;
+}
+size_t mg_tail_lds(const helm_plan* p, int t) { return tail_lds_bytes_full(p, t); }
+bool mg_fused_ok(const helm_plan* p) { return p->mg.pre == 2 && p->mg.post == 2 && fused_levels(); }
+SegLevel mg_seg_level(helm_plan* p, int l, int nrhs, const float2* f, int64_t f_step, float2* u, int64_t u_step,
+                      const double* fscale, const int* act, int64_t wg_target, int pre_grid[3], int post_grid[3]) {
+    Level& L = p->levels[l];
+    Level& C = p->levels[l + 1];
+    SegLevel a{};
+    a.nz = L.nz;
+    a.nx = L.nx;
+    a.nzc = C.nz;
+    a.nxc = C.nx;
+    a.nrhs = nrhs;
+    a.n = L.n;
+    auto group = [&](int64_t tiles) {
+        const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(nrhs, (wg_target + tiles - 1) / tiles));
+        return (int)((nrhs + groups - 1) / groups);
+    };
+    const int64_t tiles = (int64_t)((L.nx + MT_TX - 1) / MT_TX) * ((L.nz + MT_TZ - 1) / MT_TZ);
+    const int64_t bc = (int64_t)((C.nx + RR_TX - 1) / RR_TX) * ((C.nz + RR_TZ - 1) / RR_TZ);
+    a.rb_post = wg_target > 0 ? group(tiles) : rhs_group(p, tiles, nrhs);
+    a.rb_pre = wg_target > 0 ? group(bc) : rhs_group(p, bc, nrhs);
+    a.xcd_pre = xcd_pre(l);
+    a.xcd_post = xcd_post(l);
+    a.sx = L.sx32;
+    a.sz = L.sz32;
+    a.isxe = L.isxe32;
+    a.isze = L.isze32;
+    a.kap = L.kap32;
+    a.eta = L.eta32;
+    a.wid = L.wid;
+    a.tcf = (l > 0 || table0()) ? (const float2*)L.tcf : nullptr;
+    a.k0sq = (float)(p->k0 * p->k0);
+    a.hh = (float)(0.5 / (L.h_m * L.h_m));
+    a.shift = (float)p->mg.shift;
+    a.f = f;
+    a.f_step = f_step;
+    a.u = u;
+    a.u_step = u_step;
+    a.fscale = fscale;
+    a.t = L.mg_t;
+    a.fc = C.mg_f;
+    a.uc = C.mg_u;
+    a.act = act;
+    pre_grid[0] = (C.nx + RR_TX - 1) / RR_TX;
+    pre_grid[1] = (C.nz + RR_TZ - 1) / RR_TZ;
+    pre_grid[2] = (nrhs + a.rb_pre - 1) / a.rb_pre;
+    post_grid[0] = (L.nx + MT_TX - 1) / MT_TX;
+    post_grid[1] = (L.nz + MT_TZ - 1) / MT_TZ;
+    post_grid[2] = (nrhs + a.rb_post - 1) / a.rb_post;
+    return a;
+}
+SegTail mg_seg_tail(helm_plan* p, int t, const int* act) {
+    SegTail a{};
+    a.views = (const LevelView*)p->mg_views;
+    a.t = t;
+    a.last = (int)p->levels.size() - 1;
+    a.pre = p->mg.pre;
+    a.post = p->mg.post;
+    a.coarse = p->mg.coarse;
+    a.f = p->levels[t].mg_f;
+    a.u = p->levels[t].mg_u;
+    a.act = act;
+    return a;
 }
 
 bool mg_scaled_input_ok(const helm_plan* p) {

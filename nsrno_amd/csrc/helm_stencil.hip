@@ -182,7 +182,7 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
 }
 
 template <int RB, int TZ, int MODE>
-__global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, int nrhs,
+__device__ __forceinline__ void k_apply_block_body(const VBlock& B, int nz, int nx, int64_t n, int nrhs,
                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
                                                      const float2* __restrict__ isxe, const float2* __restrict__ isze,
                                                      const float* __restrict__ eta, const float2* __restrict__ kap,
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, 
                                                      const float2* __restrict__ b, int xcd) {
     static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
     int bx, by, bz;
-    xcd_tile(bx, by, bz, xcd != 0);
+    xcd_tile(B, bx, by, bz, xcd != 0);
     const int lane = threadIdx.x;
     const int x0 = bx * 64, j = x0 + lane;
     // the run's first row, uniform over the wavefront (so the row profiles below are scalar loads)
@@ -290,6 +290,16 @@ __global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, 
             }
         }
     }
+}
+template <int RB, int TZ, int MODE>
+__global__ __launch_bounds__(256) void k_apply_block(int nz, int nx, int64_t n, int nrhs,
+                                                     const float2* __restrict__ sx, const float2* __restrict__ sz,
+                                                     const float2* __restrict__ isxe, const float2* __restrict__ isze,
+                                                     const float* __restrict__ eta, const float2* __restrict__ kap,
+                                                     float k0sq, float hh, float shift, int conj,
+                                                     const float2* __restrict__ u, float2* __restrict__ out,
+                                                     const float2* __restrict__ b, int xcd) {
+    k_apply_block_body<RB, TZ, MODE>(hw_block(), nz, nx, n, nrhs, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, u, out, b, xcd);
 }
 
 // Paired form of the block kernel (nx even): each lane owns two adjacent
@@ -740,6 +750,27 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     (void)hipFree(u);
     (void)hipFree(out);
     return st;
+}
+
+// segmented FP32 apply of the batched solve: w = A z_j into V_{j+1}'s slot,
+// in the single-plan default shape (block form, 4 RHS per wavefront, 4-row
+// runs, hardware order) on every plan
+// (shift is a kernel argument, 0 at run time, as in the single-plan launch:
+// a literal 0 would let the compiler fuse the mass term into the diagonal
+// differently and round it differently)
+__global__ __launch_bounds__(256) void k_seg_apply32(const SegDims* __restrict__ dims, int nseg,
+                                                     const SegKrylov* __restrict__ seg, int j, int conj,
+                                                     float shift) {
+    int s;
+    VBlock B;
+    if (!seg_block(dims, nseg, s, B)) return;
+    const SegKrylov& a = seg[s];
+    const int64_t vs = (int64_t)a.nrhs * a.n;
+    k_apply_block_body<4, 4, 0>(B, a.nz, a.nx, a.n, a.nrhs, a.sx32, a.sz32, a.isxe32, a.isze32, a.eta32, a.kap32,
+                                a.k0sq32, a.hh32, shift, conj, a.Z + j * vs, a.V + (j + 1) * vs, nullptr, 0);
+}
+void seg_apply32(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int conj, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_apply32, dim3(blocks), dim3(64, 4), 0, s, dims, nseg, k, j, conj, 0.0f);
 }
 
 }  // namespace helm

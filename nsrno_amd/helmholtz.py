@@ -307,3 +307,49 @@ class HelmholtzPlan:
                          i.relative_residual, bool(i.fallback), _lib.REASONS.get(i.first_reason, "unknown"),
                          int(i.graph_fallbacks)) for i in infos]
         return x, out
+
+
+def solve_batch(plans, rhs, x0=None, *, tolerance=1e-6, restart=3, max_iterations=3000, adjoint=False, graph=True):
+    """Multigrid FGMRES on several plans at once (helm_batch_solve).
+
+    plans: HelmholtzPlans of any grid sizes, each with its medium and
+    multigrid set (pre = post = 2); rhs: one complex128 [R, nz_k, nx_k] per
+    plan, the same R for all; x0: starting fields (None: zero).  Every kernel
+    of a restart cycle is one segmented launch over all plans still running,
+    the cycle a captured HIP graph.  Each plan's numerics are those of
+    plan.solve(precond="mg") on its finest-level fused path.  Returns
+    (list of x, list of [SolveInfo] * R)."""
+    plans = list(plans)
+    if len(plans) != len(rhs) or not plans:
+        raise ValueError("solve_batch: one right-hand side batch per plan")
+    if x0 is not None and len(x0) != len(plans):
+        raise ValueError("solve_batch: one starting field batch per plan")
+    bs, xs = [], []
+    nrhs = None
+    for k, (p, b) in enumerate(zip(plans, rhs)):
+        b = torch.as_tensor(b, device=p.device).to(torch.complex128).contiguous()
+        r = p._check_field(b)
+        if nrhs is None:
+            nrhs = r
+        elif r != nrhs:
+            raise ValueError("solve_batch: every plan needs the same number of right-hand sides")
+        if r > p.nrhs_capacity:
+            raise ValueError(f"nrhs {r} exceeds the capacity {p.nrhs_capacity} of plan {k}")
+        bs.append(b)
+        if x0 is None:
+            xs.append(torch.empty_like(b))
+        else:
+            xs.append(torch.as_tensor(x0[k], device=p.device).to(torch.complex128).contiguous().clone())
+    n = len(plans)
+    hp = (ctypes.c_void_p * n)(*[p._h for p in plans])
+    hb = (ctypes.c_void_p * n)(*[_ptr(b) for b in bs])
+    hx = (ctypes.c_void_p * n)(*[_ptr(x) for x in xs])
+    opts = _lib.HelmSolveOpts(_lib.HELM_GMRES, PRECONDITIONERS["mg"], int(restart), int(max_iterations),
+                              float(tolerance), int(bool(adjoint)), int(x0 is not None), _lib.CALLBACK(), None,
+                              int(bool(graph)), 0, 0, 0.0)
+    infos = (_lib.HelmSolveInfo * (n * nrhs))()
+    check(lib().helm_batch_solve(n, hp, hb, hx, nrhs, ctypes.byref(opts), infos, _stream()), "helm_batch_solve")
+    out = [[SolveInfo(_lib.REASONS.get(i.reason, "unknown"), i.iterations, i.applications, i.cycles,
+                      i.relative_residual, bool(i.fallback), _lib.REASONS.get(i.first_reason, "unknown"),
+                      int(i.graph_fallbacks)) for i in infos[k * nrhs:(k + 1) * nrhs]] for k in range(n)]
+    return xs, out

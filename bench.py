@@ -102,6 +102,7 @@ def parse():
                          "write_copy) into profiles/pmc_traffic.json and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched frequency-sweep leg")
     ap.add_argument("--cpu-interior", type=int, default=None,
                     help="interior size of the CPU baseline sample (default: the full workload)")
     return ap.parse_args()
@@ -302,6 +303,50 @@ def pmc_summary(directory):
     print(json.dumps(out), flush=True)
 
 
+SWEEP_PMC = os.path.join(ROOT, "profiles", "pmc_frequency_sweep.json")
+
+
+def sweep_hbm(a, n_freq, n_angles, timing):
+    """HBM bytes the batched sweep moves (PMC FETCH_SIZE + WRITE_SIZE of its
+    segmented kernels, measured once on the default workload and kept in
+    profiles/pmc_frequency_sweep.json; the solve is deterministic, so the
+    bytes are those of every run of the same workload) over this run's solve
+    seconds.  None for another workload or mode."""
+    if a.sweep_mode != "batched" or not timing or not os.path.exists(SWEEP_PMC):
+        return None
+    with open(SWEEP_PMC) as fh:
+        pmc = json.load(fh)
+    w = pmc["workload"]
+    if (w["frequencies"], w["angles"], w["tolerance"], w["restart"]) != (n_freq, n_angles, a.tol, a.restart):
+        return None
+    return {"bytes": pmc["segmented_kernel_bytes"], "solve_seconds": round(timing["solve_seconds"], 3),
+            "gb_s": round(pmc["segmented_kernel_bytes"] / timing["solve_seconds"] / 1e9, 1),
+            "frac": round(pmc["segmented_kernel_bytes"] / timing["solve_seconds"] / 1e9 / PEAK_HBM_GBS, 4),
+            "source": "profiles/frequency_sweep_batched_64x11.md"}
+
+
+def sweep_summary(a):
+    """The batched 64-frequency x 11-angle sweep (config 3) as a compact
+    object for the default bench line."""
+    import numpy as np
+    from nsrno_amd.batch import run_sweep
+    freqs = np.linspace(4.0, 8.9, a.frequencies)
+    angles = np.linspace(-15.0, 15.0, a.angles)
+    run_sweep(freqs[:2], angles, tol=a.tol, restart=a.restart, workers=1, mg_shift=a.mg_shift, mode="batched")
+    timing = {}
+    cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers, graph=True,
+                            mg_shift=a.mg_shift, mode="batched", timing=timing)
+    sweep_args = argparse.Namespace(**{**vars(a), "sweep_mode": "batched"})
+    return {"workload": f"{len(freqs)} frequencies 4.0-8.9 MHz x {len(angles)} angles, interiors "
+                        f"{cases[0].interior}^2-{cases[-1].interior}^2 + 32 PML, one helm_batch_solve",
+            "solves_per_s": round(len(freqs) * len(angles) / wall, 3), "seconds": round(wall, 3),
+            "setup_seconds": round(timing["setup_seconds"], 3), "solve_seconds": round(timing["solve_seconds"], 3),
+            "hbm": sweep_hbm(sweep_args, len(freqs), len(angles), timing),
+            "all_converged": all(c.reasons == ("converged",) for c in cases),
+            "max_relative_residual": max(c.relative_residual for c in cases),
+            "steps_range": [min(max(c.iterations) for c in cases), max(max(c.iterations) for c in cases)]}
+
+
 def frequency_sweep(a):
     """Batched uneven solves: `a.frequencies` frequencies evenly spaced in
     4.0-8.9 MHz x 11 angles; each grid sized for 10 points per wavelength at
@@ -317,9 +362,10 @@ def frequency_sweep(a):
     for _ in range(a.warmup):  # small frequencies warm the kernels and allocator
         run_sweep(freqs[:2], angles, tol=a.tol, restart=a.restart, workers=1, mg_shift=a.mg_shift,
                   mode=a.sweep_mode)
+    timing = {}
     cases, wall = run_sweep(freqs, angles, tol=a.tol, restart=a.restart, workers=a.workers,
                             log=lambda m: print("[bench] " + m, flush=True), autotune=bool(a.autotune) and not batched,
-                            graph=bool(a.graph), mg_shift=a.mg_shift, mode=a.sweep_mode)
+                            graph=bool(a.graph), mg_shift=a.mg_shift, mode=a.sweep_mode, timing=timing)
     solves = len(freqs) * len(angles)
     kbytes = sum(c.krylov_bytes for c in cases)
     line = {
@@ -336,6 +382,7 @@ def frequency_sweep(a):
                    "autotune": bool(a.autotune) and not batched, "graph": bool(a.graph),
                    "parallelism": "single GPU"},
         "krylov_operator_gb_s": round(kbytes / wall / 1e9, 1),
+        "hbm": sweep_hbm(a, len(freqs), len(angles), timing),
         "all_converged": all(c.reasons == ("converged",) for c in cases),
         "max_relative_residual": max(c.relative_residual for c in cases),
         "steps_range": [min(max(c.iterations) for c in cases), max(max(c.iterations) for c in cases)],
@@ -873,6 +920,10 @@ def main():
             max_relative_residual=max(i.relative_residual for i in minf))
         print(f"[bench] mg-fgmres without the net: {mdt / a.steps * 1e3:.1f} ms/step, applications "
               f"{max(result['mg_fgmres_without_net']['applications'])}", flush=True)
+    if not a.no_sweep:
+        # config 3 beside the headline: the 64-frequency batched sweep (its own line: --workload frequencies)
+        result["frequency_sweep"] = sweep_summary(a)
+        print(f"[bench] frequency sweep: {result['frequency_sweep']['solves_per_s']} solves/s", flush=True)
     if not a.no_plain and a.precond != "none":
         pdt, _, pinf, _ = b.timed("none", 1, 0, a.plain_max_iterations)
         result["plain_gmres"] = dict(

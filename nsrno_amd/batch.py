@@ -151,7 +151,7 @@ def solve_case(medium, case, angles, tol, restart, n_elements=192, stream=None, 
 
 
 def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, autotune=True, graph=False,
-              mg_shift=0.4, keep_fields=(), mode="threads"):
+              mg_shift=0.4, keep_fields=(), mode="threads", timing=None):
     """Solve every frequency.  mode "batched": all frequencies in one
     helm_batch_solve (every kernel of a restart cycle one segmented launch
     over the frequencies still running; set-up on `workers` host threads);
@@ -160,9 +160,11 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
     input order, wall seconds).  keep_fields: indices of the frequencies
     whose fields are copied to the host.  The plans stay allocated until
     every frequency is solved (a few GB for a 64-frequency sweep) and are
-    closed after the timed region."""
+    closed after the timed region.  timing (a dict, batched mode): receives
+    the set-up and solve seconds."""
     if mode == "batched":
-        return run_sweep_batched(frequencies, angles, tol, restart, workers, seed, log, graph, mg_shift, keep_fields)
+        return run_sweep_batched(frequencies, angles, tol, restart, workers, seed, log, graph, mg_shift, keep_fields,
+                                 timing)
     if mode != "threads":
         raise ValueError(f"unknown sweep mode {mode!r}")
     medium = PhysicalMedium(seed=seed)
@@ -211,7 +213,7 @@ def run_sweep(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=N
 
 
 def run_sweep_batched(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=0, log=None, graph=True,
-                      mg_shift=0.4, keep_fields=()):
+                      mg_shift=0.4, keep_fields=(), timing=None):
     """run_sweep(mode="batched"): set-up of every frequency (medium
     resampling, plan, multigrid, ports; `workers` host threads, each on its
     own stream), then ONE helm_batch_solve over all of them, then traces.
@@ -263,6 +265,8 @@ def run_sweep_batched(frequencies, angles, tol=1e-6, restart=3, workers=4, seed=
             finish_case(cases[k], plan, ports, xs[k], infos[k], restart, k in keep, setup)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        if timing is not None:
+            timing.update(setup_seconds=t_setup, solve_seconds=wall - t_setup)
         if log:
             log(f"batched sweep: set-up {t_setup:.2f} s, solve + traces {wall - t_setup:.2f} s, "
                 f"steps {max(max(c.iterations) for c in cases)}")

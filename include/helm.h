@@ -474,7 +474,10 @@ typedef struct {
     int32_t stencil_block;     /* 1: the block form (whole run loaded first),
                                   2: the same in XCD-contiguous order,
                                   3: the paired block form (two columns per
-                                  lane), 4: the same in XCD-contiguous order */
+                                  lane), 4: the same in XCD-contiguous order,
+                                  5 / 6: the block / paired form in
+                                  XCD-contiguous order with the RHS groups
+                                  of a run next to each other */
 } helm_tuning;
 int32_t helm_autotune(helm_plan* plan, int32_t nrhs, helm_tuning* result, void* stream);
 
@@ -483,9 +486,11 @@ int32_t helm_autotune(helm_plan* plan, int32_t nrhs, helm_tuning* result, void* 
  * 8 or 16 loaded up front), 2: block in XCD-contiguous workgroup order, 3:
  * paired block (two columns per lane; rows 2, 4 or 8; the block form where
  * the grid has an odd row length or the fields are not 16-byte aligned), 4:
- * paired block in XCD-contiguous order; rhs 1, 2 or 4 per group.  rows = 0
- * returns to the built-in default.  For tests and for callers that tuned
- * once and reuse the result. */
+ * paired block in XCD-contiguous order, 5 / 6: block / paired block in
+ * XCD-contiguous order with the RHS groups of a run next to each other;
+ * rhs 1, 2 or 4 per group (also 8 in forms 1-6).  rows = 0 returns to the
+ * built-in default.  For tests and for callers that tuned once and reuse
+ * the result. */
 int32_t helm_set_stencil_shape(helm_plan* plan, int32_t rows, int32_t rhs, int32_t form);
 
 /* ---- in-solve kernel timing: no counterpart in the reference (its path

@@ -191,7 +191,7 @@ struct Level {
     // streaming-stencil launch shape chosen by helm_autotune (0: default)
     int st_tz = 0, st_rb = 0;
     bool st_block = false;  // FP32 flux stencil: block form (whole run loaded up front)
-    bool st_xcd = false;    // block form in XCD-contiguous workgroup order
+    int st_xcd = 0;         // workgroup order: 0 hardware, 1 XCD-contiguous, 2 the same with RHS groups fastest
     bool st_pair = false;   // paired block form (two columns per lane)
 };
 

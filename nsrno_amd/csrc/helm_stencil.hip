@@ -191,7 +191,7 @@ __device__ __forceinline__ void k_apply_block_body(const VBlock& B, int nz, int 
                                                      const float2* __restrict__ b, int xcd) {
     static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
     int bx, by, bz;
-    xcd_tile(B, bx, by, bz, xcd != 0);
+    xcd_tile(B, bx, by, bz, xcd != 0, xcd == 2);
     const int lane = threadIdx.x;
     const int x0 = bx * 64, j = x0 + lane;
     // the run's first row, uniform over the wavefront (so the row profiles below are scalar loads)
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_apply_pair(int nz, int nx, int64_t n, i
                                                     const float2* __restrict__ u, float2* __restrict__ out, int xcd) {
     static_assert(RB * TZ <= 64, "halo columns are packed one value per lane");
     int bx, by, bz;
-    xcd_tile(bx, by, bz, (xcd & 1) != 0);
+    xcd_tile(bx, by, bz, xcd != 0, xcd == 2);
     const int lane = threadIdx.x;
     const int x0 = bx * 128, c0 = x0 + 2 * lane;  // columns c0, c0 + 1
     const int i0 = __builtin_amdgcn_readfirstlane((by * 4 + threadIdx.y) * TZ);
@@ -522,10 +522,10 @@ static void launch_block(const Level& L, int tz, double k0sq, const float2* u, f
         launch_block_tz<RB, 2>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
     else if (tz <= 4)
         launch_block_tz<RB, 4>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
-    else if (tz <= 8)
+    else if (tz <= 8 || RB * 16 > 64)  // a run's halo packs RB x TZ values into one wavefront
         launch_block_tz<RB, 8>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
     else
-        launch_block_tz<RB, 16>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
+        launch_block_tz<RB, (RB * 16 <= 64 ? 16 : 8)>(L, k0sq, u, out, nrhs, adjoint, b, shift, s);
 }
 
 template <int RB, int TZ>
@@ -588,6 +588,7 @@ int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out,
             switch (rb) {
                 case 1: launch_pair<1>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
                 case 2: launch_pair<2>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
+                case 8: launch_pair<8>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
                 default: launch_pair<4>(L, L.st_tz, k0sq, u, out, nrhs, adjoint, shift, s); break;
             }
         } else if (L.st_tz ? L.st_block : stream_block()) {
@@ -595,6 +596,7 @@ int32_t launch_flux32(const Level& L, double k0sq, const float2* u, float2* out,
             switch (rb) {
                 case 1: launch_block<1>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
                 case 2: launch_block<2>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
+                case 8: launch_block<8>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
                 default: launch_block<4>(L, tz, k0sq, u, out, nrhs, adjoint, b, shift, s); break;
             }
         } else {
@@ -682,48 +684,61 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     *best_us = 0.0;
     const bool cond = L.op == HELM_OP_COND8;
     if (!cond && (L.op != HELM_OP_FLUX5 || L.nx < 64)) return HELM_OK;  // no streaming kernel there
-    const size_t bytes = (size_t)nrhs * L.n * sizeof(float2);
-    float2 *u = nullptr, *out = nullptr;
-    HELM_HIP(hipMalloc((void**)&u, bytes));
-    if (hipMalloc((void**)&out, bytes) != hipSuccess) {
-        (void)hipFree(u);
+    // Candidates are timed on inputs that are not in the 256 MB last-level
+    // cache (Infinity Cache): the launches rotate through `nbuf` field pairs,
+    // at least 512 MB in all, as the solve's applies meet fields that the
+    // other kernels of the step have pushed out.  Timing one pair again and
+    // again measured the cache and ranked shapes differently.
+    const size_t stride = ((size_t)nrhs * L.n + 1) & ~(size_t)1;  // float2 per field batch, 16-byte aligned
+    const size_t bytes = stride * sizeof(float2);
+    const int nbuf = (int)std::min<size_t>(16, std::max<size_t>(2, ((size_t)512 << 20) / (2 * bytes) + 1));
+    float2* pool = nullptr;
+    if (hipMalloc((void**)&pool, 2 * nbuf * bytes) != hipSuccess) {
+        (void)hipGetLastError();
         set_error("helm_autotune: scratch allocation failed");
         return HELM_ERR_ALLOC;
     }
+    float2* u = pool;
+    float2* out = pool + (size_t)nbuf * stride;
+    int turn = 0;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     int32_t st = HELM_OK;
     double best = 1e30;
     int btz = cond ? 64 : 4, brb = cond ? 2 : 4;
-    if (hipMemsetAsync(u, 0, bytes, s) != hipSuccess) st = HELM_ERR_HIP;
+    if (hipMemsetAsync(pool, 0, 2 * nbuf * bytes, s) != hipSuccess) st = HELM_ERR_HIP;
     // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up);
     // the flux stencil's streaming form is no longer a candidate (its block form measured faster on
     // every grid from 576^2 to 2048^2) but stays selectable with helm_set_stencil_shape
     const int ntz = cond ? 5 : 0;
-    const int tzs_flux[5] = {4, 8, 16, 16, 16}, tzs_cond[5] = {8, 16, 32, 64, 128}, rbs[3] = {1, 2, 4};
+    const int tzs_flux[5] = {4, 8, 16, 16, 16}, tzs_cond[5] = {8, 16, 32, 64, 128}, rbs[4] = {1, 2, 4, 8};
+    const int nrb = cond ? 3 : 4;  // the conditioned kernel groups at most 4 RHS
     const int* tzs = cond ? tzs_cond : tzs_flux;
     const double k0sq = p->k0 * p->k0;
     auto run = [&]() {
-        return cond ? launch_cond8_32(L, u, out, nrhs, false, nullptr, s)
-                    : launch_flux32(L, k0sq, u, out, nrhs, false, nullptr, 0.f, s);
+        const size_t o = (size_t)(turn++ % nbuf) * stride;
+        return cond ? launch_cond8_32(L, u + o, out + o, nrhs, false, nullptr, s)
+                    : launch_flux32(L, k0sq, u + o, out + o, nrhs, false, nullptr, 0.f, s);
     };
     // the flux stencil's block form: runs of 2, 4 or 8 rows loaded up front (16 was never the fastest),
-    // in the hardware's workgroup order and in the XCD-contiguous one
-    const int nblk = cond ? 0 : 6, tzs_blk[3] = {2, 4, 8};
-    // and the paired form (two columns per lane; even rows of >= 128 nodes): runs of 2 or 4 rows
-    const int npair = (!cond && pair_ok(L, u, out, nullptr)) ? 2 : 0, tzs_pair[2] = {2, 4};
-    bool bblk = !cond, bxcd = false, bpair = false;
-    const int reps = 10;  // launches timed per candidate
+    // and the paired form (two columns per lane; even rows of >= 128 nodes): runs of 2 or 4 rows; each
+    // in the hardware's workgroup order, the XCD-contiguous one, and that with the RHS groups of a
+    // run next to each other (one XCD's L2 then serves the run's coefficients to every group)
+    const int nblk = cond ? 0 : 9, tzs_blk[3] = {2, 4, 8};
+    const int npair = (!cond && pair_ok(L, u, out, nullptr)) ? 6 : 0, tzs_pair[2] = {2, 4};
+    bool bblk = !cond, bpair = false;
+    int bxcd = 0;
+    const int reps = std::max(10, nbuf);  // launches timed per candidate
     for (int a = 0; a < ntz + nblk + npair && st == HELM_OK; ++a)
-        for (int b = 0; b < 3 && st == HELM_OK; ++b) {
+        for (int b = 0; b < nrb && st == HELM_OK; ++b) {
             const bool pair = a >= ntz + nblk;
             const bool blk = a >= ntz && !pair;
-            L.st_tz = pair ? tzs_pair[a - ntz - nblk] : (blk ? tzs_blk[(a - ntz) % 3] : tzs[a]);
+            L.st_tz = pair ? tzs_pair[(a - ntz - nblk) % 2] : (blk ? tzs_blk[(a - ntz) % 3] : tzs[a]);
             L.st_rb = rbs[b];
             L.st_block = blk || pair;  // the paired form falls back to the block form
             L.st_pair = pair;
-            L.st_xcd = blk && a - ntz >= 3;
+            L.st_xcd = pair ? (a - ntz - nblk) / 2 : (blk ? (a - ntz) / 3 : 0);
             for (int w = 0; w < 3 && st == HELM_OK; ++w) st = run();
             (void)hipEventRecord(e0, s);
             for (int w = 0; w < reps && st == HELM_OK; ++w) st = run();
@@ -747,8 +762,7 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     *best_us = best * 1e3;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(u);
-    (void)hipFree(out);
+    (void)hipFree(pool);
     return st;
 }
 
@@ -918,7 +932,9 @@ extern "C" int32_t helm_autotune(helm_plan* p, int32_t nrhs, helm_tuning* result
         const Level& L = p->levels[0];
         result->stencil_rows = L.st_tz;
         result->stencil_rhs = L.st_rb;
-        result->stencil_block = L.st_pair ? (L.st_xcd ? 4 : 3) : (L.st_block ? (L.st_xcd ? 2 : 1) : 0);
+        const int xo[3] = {0, 1, 2};  // order -> form offset: hardware, XCD-contiguous, RHS groups fastest
+        result->stencil_block = L.st_pair ? (L.st_xcd == 2 ? 6 : 3 + xo[L.st_xcd])
+                                          : (L.st_block ? (L.st_xcd == 2 ? 5 : 1 + xo[L.st_xcd]) : 0);
         result->mg_workgroups = p->mg_wg_target;
         result->stencil_us = st_us;
         result->vcycle_us = mg_us;
@@ -931,19 +947,23 @@ extern "C" int32_t helm_set_stencil_shape(helm_plan* p, int32_t rows, int32_t rh
     Level& L = p->levels[0];
     if (rows == 0) {  // back to the built-in defaults
         L.st_tz = L.st_rb = 0;
-        L.st_block = L.st_xcd = L.st_pair = false;
+        L.st_block = L.st_pair = false;
+        L.st_xcd = 0;
     } else {
-        HELM_CHECK(rhs == 1 || rhs == 2 || rhs == 4, HELM_ERR_ARG, "helm_set_stencil_shape: rhs must be 1, 2 or 4");
-        HELM_CHECK(form >= 0 && form <= 4, HELM_ERR_ARG, "helm_set_stencil_shape: form must be 0 .. 4");
+        HELM_CHECK(rhs == 1 || rhs == 2 || rhs == 4 || (rhs == 8 && form >= 1), HELM_ERR_ARG,
+                   "helm_set_stencil_shape: rhs must be 1, 2, 4 (or 8 in the block and paired forms)");
+        HELM_CHECK(form >= 0 && form <= 6, HELM_ERR_ARG, "helm_set_stencil_shape: form must be 0 .. 6");
+        const bool pair = form == 3 || form == 4 || form == 6;
         const bool rows_ok = form == 0 ? rows >= 2 && rows <= 1024
-                             : form <= 2 ? rows == 2 || rows == 4 || rows == 8 || rows == 16
-                                         : rows == 2 || rows == 4 || rows == 8;
-        HELM_CHECK(rows_ok, HELM_ERR_ARG, "helm_set_stencil_shape: rows out of range for the form");
+                             : !pair   ? rows == 2 || rows == 4 || rows == 8 || rows == 16
+                                       : rows == 2 || rows == 4 || rows == 8;
+        HELM_CHECK(rows_ok && (form == 0 || rows * rhs <= 64), HELM_ERR_ARG,
+                   "helm_set_stencil_shape: rows out of range for the form (rows x rhs <= 64)");
         L.st_tz = rows;
         L.st_rb = rhs;
         L.st_block = form >= 1;
-        L.st_pair = form >= 3;  // falls back to the block form where the paired one cannot run
-        L.st_xcd = form == 2 || form == 4;
+        L.st_pair = pair;  // falls back to the block form where the paired one cannot run
+        L.st_xcd = (form == 2 || form == 4) ? 1 : (form >= 5 ? 2 : 0);
     }
     p->generation++;  // captured cycles hold the old launch shapes
     return HELM_OK;

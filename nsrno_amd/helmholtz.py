@@ -216,8 +216,10 @@ class HelmholtzPlan:
 
     def set_stencil_shape(self, rows=0, rhs=4, form=1):
         """Fix the FP32 stencil's launch shape (helm_set_stencil_shape): form 0
-        streaming, 1 block, 2 block in XCD-contiguous order; rows=0 restores
-        the default."""
+        streaming, 1 block, 2 block in XCD-contiguous order, 3 / 4 the paired
+        block form (hardware / XCD-contiguous order), 5 / 6 block / paired in
+        XCD-contiguous order with the RHS groups of a run together; rhs 1, 2,
+        4 (8 in forms 1-6); rows=0 restores the default."""
         check(lib().helm_set_stencil_shape(self._h, int(rows), int(rhs), int(form)), "helm_set_stencil_shape")
 
     PROFILE_KERNELS = ("operator_apply", "preconditioner", "dcgs_dots", "dcgs_update", "true_residual",

@@ -182,7 +182,8 @@ def test_born_data_and_jacobian_match_oracle():
 
 
 @pytest.mark.parametrize("shape", [None, (8, 4, 0), (3, 1, 0), (2, 2, 1), (4, 4, 1), (8, 1, 1), (16, 4, 1),
-                                   (4, 2, 2), (16, 1, 2), (2, 2, 3), (4, 4, 3), (8, 1, 4), (2, 4, 4)])
+                                   (4, 2, 2), (16, 1, 2), (2, 2, 3), (4, 4, 3), (8, 1, 4), (2, 4, 4),
+                                   (2, 8, 1), (4, 8, 3), (2, 2, 5), (8, 8, 5), (2, 4, 6), (4, 8, 6)])
 @pytest.mark.parametrize("nrhs", [1, 5])
 @pytest.mark.parametrize("width", [170, 171])
 def test_apply_fp32_streaming_shapes(nrhs, shape, width):

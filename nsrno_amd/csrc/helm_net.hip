@@ -115,9 +115,9 @@ static int32_t conv_wgrad(helm_net* n, int layer, int l, const bf16* x, const bf
     const int groups = (int)std::min<int64_t>(WGRAD_GROUPS, tiles);
     hipLaunchKernelGGL(k_conv_wgrad, dim3(groups), dim3(256), 0, s, n->batch, H, W, x, dy, tx, ty, tiles, n->wpart,
                        n->bpart);
-    hipLaunchKernelGGL(k_sum_partials, g1d(WSZ), dim3(256), 0, s, groups, WSZ, n->wpart, grad + (size_t)layer * PSZ);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(32), 0, s, groups, 32, n->bpart,
-                       grad + (size_t)layer * PSZ + WSZ);
+    // dW then dbias: grad[layer] holds the WSZ weights followed by the 32 biases
+    hipLaunchKernelGGL(k_sum_partials, g1d(WSZ + 32), dim3(256), 0, s, groups, WSZ, n->wpart, n->bpart,
+                       grad + (size_t)layer * PSZ);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

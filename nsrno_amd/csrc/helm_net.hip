@@ -86,22 +86,25 @@ static int32_t conv_fwd(helm_net* n, int layer, int l, const bf16* x, bf16* y, b
     const bf16* w = n->wb + (size_t)layer * WSZ;
     const float* b = n->bias + layer * 32;
     if (relu && !n->linear)
-        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
+        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y, (const bf16*)nullptr);
     else
-        hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y);
+        hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, x, w, b, y, (const bf16*)nullptr);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
 
-// dx = conv^T(dy) (no bias); optionally masked by relu'(act)
+// dx = conv^T(dy) (no bias); optionally masked by relu'(act) in the same launch
 static int32_t conv_bwd_data(helm_net* n, int layer, int l, const bf16* dy, bf16* dx, const bf16* act,
                              hipStream_t s) {
     const int H = n->H >> l, W = n->W >> l;
     dim3 grid((W + 31) / 32, (H + CONV_ROWS_HOST - 1) / CONV_ROWS_HOST, n->batch);
-    hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, dy, n->wf + (size_t)layer * WSZ,
-                       (const float*)nullptr, dx);
+    const bf16* wf = n->wf + (size_t)layer * WSZ;
     if (act && !n->linear)
-        hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), act, dx);
+        hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, s, n->batch, H, W, dy, wf, (const float*)nullptr, dx,
+                           act);
+    else
+        hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(256), 0, s, n->batch, H, W, dy, wf, (const float*)nullptr, dx,
+                           (const bf16*)nullptr);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }
@@ -154,11 +157,12 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
         HELM_TRY(conv_wgrad(n, up_layers[l], l, n->s[l], n->g0[l], grad, s));
         HELM_TRY(conv_bwd_data(n, up_layers[l], l, n->g0[l], n->g1[l], nullptr, s));   // ds_l
         // gradient to the coarser decoder output (or e3): up_bwd(ds_l)
-        hipLaunchKernelGGL(k_up2_bwd, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, n->batch, n->H >> l, n->W >> l,
-                           n->g1[l], n->g0[l + 1]);
+        hipLaunchKernelGGL(k_up2_bwd, dim3((unsigned)(((n->W >> (l + 1)) + 63) / 64), (unsigned)(n->H >> (l + 1)),
+                                           n->batch),
+                           dim3(256), 0, s, n->batch, n->H >> l, n->W >> l, n->g1[l], n->g0[l + 1]);
         const bf16* act = (l < 2) ? n->d[l + 1] : n->e[3];
         if (!n->linear)
-            hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l + 1)), dim3(256), 0, s, lvl_elems(n, l + 1), act,
+            hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l + 1) / 8), dim3(256), 0, s, lvl_elems(n, l + 1), act,
                                n->g0[l + 1]);
     }
     // g0[3] = de3 (masked).  encoder from the bottom up; g1[l] holds the skip gradient de_l (l <= 2)
@@ -166,9 +170,10 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
         bf16* de = n->g0[l];
         if (l < 3) {
             // de_l = skip gradient (g1[l]) + pool_bwd(dp_{l+1}) (in g2[l]) , then relu mask
-            hipLaunchKernelGGL(k_add_bf16, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->g2[l], n->g1[l]);
+            hipLaunchKernelGGL(k_add_bf16, g1d(lvl_elems(n, l) / 8), dim3(256), 0, s, lvl_elems(n, l), n->g2[l],
+                               n->g1[l]);
             if (!n->linear)
-                hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l)), dim3(256), 0, s, lvl_elems(n, l), n->e[l],
+                hipLaunchKernelGGL(k_relu_mask, g1d(lvl_elems(n, l) / 8), dim3(256), 0, s, lvl_elems(n, l), n->e[l],
                                    n->g1[l]);
             de = n->g1[l];
         }
@@ -178,8 +183,8 @@ int32_t net_backward(helm_net* n, const bf16* dout, float* grad, hipStream_t s) 
         HELM_TRY(conv_wgrad(n, C0A + 2 * l, l, xin, n->g2[l], grad, s));
         if (l > 0) {
             HELM_TRY(conv_bwd_data(n, C0A + 2 * l, l, n->g2[l], n->g0[l], nullptr, s));  // dp_l (reuse g0[l])
-            hipLaunchKernelGGL(k_pool2_bwd, g1d(lvl_elems(n, l - 1)), dim3(256), 0, s, n->batch, n->H >> (l - 1),
-                               n->W >> (l - 1), n->g0[l], n->g2[l - 1]);
+            hipLaunchKernelGGL(k_pool2_bwd, dim3((unsigned)(((n->W >> l) + 63) / 64), (unsigned)(n->H >> l), n->batch),
+                               dim3(256), 0, s, n->batch, n->H >> (l - 1), n->W >> (l - 1), n->g0[l], n->g2[l - 1]);
         }
     }
     HELM_HIP(hipGetLastError());

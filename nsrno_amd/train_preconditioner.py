@@ -254,6 +254,9 @@ def main(argv=None):
     ap.add_argument("--clip", type=float, default=2.0)
     ap.add_argument("--interior", type=int, default=512)
     ap.add_argument("--freq", type=float, default=3.0)
+    ap.add_argument("--freqs", default=None,
+                    help="comma-separated frequencies in MHz (warmstart loss): media spread over them in turn, one "
+                         "plan per frequency on the same grid (10 points per wavelength at each), one shared net")
     ap.add_argument("--init", default=None, help="start from these parameters")
     ap.add_argument("--log-every", type=int, default=50)
     ap.add_argument("--eval", default=None, help="only evaluate these parameters on held-out media")
@@ -283,13 +286,22 @@ def main(argv=None):
         return
     torch.manual_seed(0)
     mg = dict(levels=5, shift=0.5, pre=2, post=2, coarse_sweeps=10)
-    fam = Family(interior=a.interior, frequency_mhz=a.freq)
-    plan = fam.plan()
+    freqs = [float(v) for v in a.freqs.split(",")] if a.freqs else [a.freq]
+    if len(freqs) > 1 and a.loss != "warmstart":
+        raise ValueError("--freqs needs --loss warmstart")
+    fams = [Family(interior=a.interior, frequency_mhz=f) for f in freqs]
+    plans = [f.plan() for f in fams]
+    if len({(f.nz, f.nx) for f in fams}) != 1:
+        raise ValueError("--freqs: every frequency must give the same grid (one canvas for the net)")
+    fam, plan = fams[0], plans[0]
     t0 = time.perf_counter()
-    meds, data = [], []
+    meds, data, which = [], [], []
     for k in range(a.media):
+        fi = k % len(fams)  # medium k belongs to frequency k mod len(freqs)
+        fam, plan = fams[fi], plans[fi]
         med = fam.medium(a.seed0 + k)
         meds.append(med)
+        which.append(fi)
         if a.loss == "warmstart":  # the source fields themselves (unit RMS)
             set_medium(plan, med, mg)
             b = plan.inject(fam.ports, fam.amp).to(torch.complex64)
@@ -314,12 +326,14 @@ def main(argv=None):
         params = params.ravel()
     batch = a.unroll if a.loss == "subspace" else a.batch
     net = LearnedNet(canvas(fam.nz), canvas(fam.nx), batch, training=True, params=params, linear=a.linear)
-    attach(plan, net)
+    for p in plans:
+        attach(p, net)
     rng = np.random.default_rng(0)
     hist, ratios = [], []
     t1 = time.perf_counter()
     for step in range(a.steps):
         k = int(rng.integers(len(meds)))
+        plan = plans[which[k]]
         set_medium(plan, meds[k], mg)
         if a.loss == "subspace":
             idx = int(rng.integers(data[k].shape[0]))

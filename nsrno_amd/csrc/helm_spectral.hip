@@ -124,20 +124,66 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
         kc[k] = v;
     }
 
-    const int r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
-    for (int r = blockIdx.z * rb; r < r1; ++r) {
+    // staging map of this thread (the same for every RHS): rows rr = ty + 4 k
+    // of the halo-7 tile, columns cc = tx and tx + 64 (the corners are never read)
+    constexpr int NR = (TZ + 2 * HL + 3) / 4;
+    constexpr bool PF_TILE = sizeof(R) == 4;  // the register tile of the prefetch (FP32)
+    int64_t rowoff[NR];
+    int colg[2];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int rr = ty + 4 * k;
+        rowoff[k] = rr < TZ + 2 * HL ? (int64_t)wrap_index(z0 + rr - HL, H) * W : -1;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int cc = tx + 64 * c;
+        colg[c] = cc < SU ? wrap_index(x0 + cc - HL, W) : -1;
+    }
+    auto staged = [&](int k, int c) __attribute__((always_inline)) {
+        const int rr = ty + 4 * k, cc = tx + 64 * c;
+        const bool zh = rr < HL || rr >= TZ + HL, xh = cc < HL || cc >= TX + HL;
+        return rowoff[k] >= 0 && colg[c] >= 0 && !(zh && xh);
+    };
+    // the next RHS's tile is loaded into registers while the current one is
+    // computed; only LDS barriers separate the phases (a full barrier would
+    // wait for those loads)
+    C pre[PF_TILE ? NR : 1][2];
+    auto load = [&](int r) __attribute__((always_inline)) {
         const C* ur = u + (int64_t)r * n;
-        __syncthreads();  // the previous right-hand side's readers are done
-        for (int rr = ty; rr < TZ + 2 * HL; rr += 4) {
-            const bool zh = rr < HL || rr >= TZ + HL;
-            const int64_t gz = wrap_index(z0 + rr - HL, H);
-            for (int cc = tx; cc < SU; cc += 64) {
-                const bool xh = cc < HL || cc >= TX + HL;
-                if (zh && xh) continue;
-                su[rr * SU + cc] = ur[gz * W + wrap_index(x0 + cc - HL, W)];
-            }
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                if (staged(k, c)) pre[k][c] = ur[rowoff[k] + colg[c]];
+    };
+    // (FP32 only: in FP64 the register tile does not fit beside the face
+    // coefficients without spilling, so each RHS is staged straight into LDS
+    // after the previous one is done)
+    constexpr bool PF = sizeof(R) == 4;
+    const int r0 = blockIdx.z * rb, r1 = min(nrhs, (int)(blockIdx.z + 1) * rb);
+    if constexpr (PF) {
+        if (r0 < r1) load(r0);
+    }
+    for (int r = r0; r < r1; ++r) {
+        lds_barrier();  // the previous right-hand side's readers are done
+        if constexpr (PF) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    if (staged(k, c)) su[(ty + 4 * k) * SU + tx + 64 * c] = pre[k][c];
+            lds_barrier();
+            if (r + 1 < r1) load(r + 1);
+        } else {
+            const C* ur = u + (int64_t)r * n;
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    if (staged(k, c)) su[(ty + 4 * k) * SU + tx + 64 * c] = ur[rowoff[k] + colg[c]];
+            __syncthreads();
         }
-        __syncthreads();
 #pragma unroll
         for (int q = 0; q < FXT; ++q) {
             const int e = tid + q * 256;
@@ -164,7 +210,7 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
                 sgz[e] = d;
             }
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             const int i = ty + 4 * k, c = tx;

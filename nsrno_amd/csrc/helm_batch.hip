@@ -77,7 +77,7 @@ struct Batch {
     // device tables of the running plans
     void* dev = nullptr;
     size_t dev_cap = 0;
-    Launch dots, update, norm, backsolve, xupdate, residual, init, apply, tail;
+    Launch dots, update, norm, xupdate, residual, init, apply, tail;
     std::vector<Launch> pre, post;  // per level
     size_t tail_lds = 0;
     // captured cycle
@@ -100,7 +100,7 @@ struct Batch {
     int32_t build() {
         Packer pk;
         std::vector<SegKrylov> ks;
-        DimsBuilder dd, du, dn, db, dx, dr, di, da, dt;
+        DimsBuilder dd, du, dn, dx, dr, di, da, dt;
         std::vector<std::vector<SegLevel>> lv(max_levels);
         std::vector<DimsBuilder> dpre(max_levels), dpost(max_levels);
         std::vector<SegTail> tv;
@@ -122,7 +122,6 @@ struct Batch {
             dd.add(g.dots);
             du.add(g.update);
             dn.add(g.norm);
-            db.add(g.backsolve);
             dx.add(g.xupdate);
             dr.add(g.residual);
             di.add(g.init);
@@ -159,7 +158,6 @@ struct Batch {
         };
         share(update, du);
         share(norm, dn);
-        share(backsolve, db);
         share(xupdate, dx);
         share(residual, dr);
         share(init, di);
@@ -183,7 +181,7 @@ struct Batch {
             ln.dims = (const SegDims*)((char*)dev + ln.dims_off);
             ln.tab = (const void*)((char*)dev + ln.tab_off);
         };
-        for (Launch* ln : {&dots, &update, &norm, &backsolve, &xupdate, &residual, &init, &apply, &tail}) fix(*ln);
+        for (Launch* ln : {&dots, &update, &norm, &xupdate, &residual, &init, &apply, &tail}) fix(*ln);
         for (int l = 0; l < max_levels; ++l) {
             fix(pre[l]);
             fix(post[l]);
@@ -234,7 +232,6 @@ struct Batch {
             seg_reduce_update(update.dims, update.nseg, update.blocks, ktab(), j, tol, s);
             seg_norm_scale(norm.dims, norm.nseg, norm.blocks, ktab(), j, j + 1 >= nsteps ? 1 : 0, s);
         }
-        seg_backsolve(backsolve.dims, backsolve.nseg, backsolve.blocks, ktab(), s);
         seg_xupdate(xupdate.dims, xupdate.nseg, xupdate.blocks, ktab(), s);
         HELM_HIP(hipGetLastError());
         return HELM_OK;

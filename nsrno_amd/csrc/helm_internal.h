@@ -311,7 +311,6 @@ void seg_dots(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int
 void seg_reduce_update(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, double tol,
                        hipStream_t s);
 void seg_norm_scale(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int last, hipStream_t s);
-void seg_backsolve(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s);
 void seg_xupdate(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s);
 void seg_residual(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int conj, hipStream_t s);
 void seg_cycle_init(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int slot, double tol,
@@ -319,7 +318,7 @@ void seg_cycle_init(const SegDims* dims, int nseg, int blocks, const SegKrylov* 
 // grids of the Krylov launches of one plan (x, y, z workgroups), as the
 // single-plan solve launches them on its lazily scaled, fused path
 struct KrylovGrids {
-    int dots[3], update[3], norm[3], backsolve[3], xupdate[3], residual[3], init[3], apply[3];
+    int dots[3], update[3], norm[3], xupdate[3], residual[3], init[3], apply[3];
     int nch, nchd, nchb;
 };
 KrylovGrids krylov_grids(int nz, int nx, int nrhs);

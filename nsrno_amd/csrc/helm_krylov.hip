@@ -897,37 +897,30 @@ __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __r
     dst[(int64_t)r * n + p] = make_float2(a.x * s, a.y * s);
 }
 
-// y = R^{-1} g (upper triangular, used[r] columns), stored in coef[r][k]
-__device__ __forceinline__ void k_backsolve_body(const VBlock& B, int nrhs, int m, int kmax, const double2* __restrict__ H,
-                            const double2* __restrict__ g, const int* __restrict__ used, double2* __restrict__ y) {
-    const int r = B.x * blockDim.x + threadIdx.x;
-    if (r >= nrhs) return;
-    const int u = used[r];
-    const double2* Hr = H + (int64_t)r * m * (m + 1);
-    double2* yr = y + (int64_t)r * kmax;
-    for (int k = u - 1; k >= 0; --k) {
-        double2 acc = g[(int64_t)r * (m + 1) + k];
-        for (int q = k + 1; q < u; ++q) acc = csub(acc, cmul(Hr[(int64_t)q * (m + 1) + k], yr[q]));
-        double2 d = Hr[(int64_t)k * (m + 1) + k];
-        double dd = d.x * d.x + d.y * d.y;
-        yr[k] = dd > 0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
-                       : make_double2(0.0, 0.0);
-    }
-}
-__global__  void k_backsolve(int nrhs, int m, int kmax, const double2* __restrict__ H,
-                            const double2* __restrict__ g, const int* __restrict__ used, double2* __restrict__ y) {
-    k_backsolve_body(hw_block(), nrhs, m, kmax, H, g, used, y);
-}
-
-// x[r] += sum_{k<used[r]} y_k Z_k[r]   (FP64 accumulate into c128 x)
-__device__ __forceinline__ void k_xupdate_body(const VBlock& B, int64_t n, int nrhs, int kmax, const float2* __restrict__ Z,
-                                                 const double2* __restrict__ y, const int* __restrict__ used,
-                                                 double2* __restrict__ x) {
+// x[r] += sum_{k<used[r]} y_k Z_k[r] with y = R^{-1} g (upper triangular,
+// used[r] columns of the rotated Hessenberg; FP64 accumulate into c128 x).
+// Thread 0 of every workgroup solves its RHS's small triangle into LDS (the
+// same operations in the same order in every workgroup, so every workgroup
+// holds the same y) and the workgroup applies it to its chunk of x.
+__device__ __forceinline__ void k_xupdate_body(const VBlock& B, int64_t n, int nrhs, int m,
+                                               const double2* __restrict__ H, const double2* __restrict__ g,
+                                               const float2* __restrict__ Z, const int* __restrict__ used,
+                                               double2* __restrict__ x) {
     __shared__ double2 yy[64];
     const int r = B.y;
     const int u = used[r];
     if (u == 0) return;
-    for (int k = threadIdx.x; k < u; k += blockDim.x) yy[k] = y[(int64_t)r * kmax + k];
+    if (threadIdx.x == 0) {
+        const double2* Hr = H + (int64_t)r * m * (m + 1);
+        for (int k = u - 1; k >= 0; --k) {
+            double2 acc = g[(int64_t)r * (m + 1) + k];
+            for (int q = k + 1; q < u; ++q) acc = csub(acc, cmul(Hr[(int64_t)q * (m + 1) + k], yy[q]));
+            const double2 d = Hr[(int64_t)k * (m + 1) + k];
+            const double dd = d.x * d.x + d.y * d.y;
+            yy[k] = dd > 0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
+                           : make_double2(0.0, 0.0);
+        }
+    }
     __syncthreads();
     int64_t p = (int64_t)B.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -942,10 +935,10 @@ __device__ __forceinline__ void k_xupdate_body(const VBlock& B, int64_t n, int n
     }
     x[(int64_t)r * n + p] = acc;
 }
-__global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int kmax, const float2* __restrict__ Z,
-                                                 const double2* __restrict__ y, const int* __restrict__ used,
-                                                 double2* __restrict__ x) {
-    k_xupdate_body(hw_block(), n, nrhs, kmax, Z, y, used, x);
+__global__ __launch_bounds__(256) void k_xupdate(int64_t n, int nrhs, int m, const double2* __restrict__ H,
+                                                 const double2* __restrict__ g, const float2* __restrict__ Z,
+                                                 const int* __restrict__ used, double2* __restrict__ x) {
+    k_xupdate_body(hw_block(), n, nrhs, m, H, g, Z, used, x);
 }
 
 __global__ void k_bi_izero_k(int count, int* p) {
@@ -1000,20 +993,13 @@ __global__ __launch_bounds__(256) void k_seg_norm_scale(const SegDims* __restric
     k_dcgs_norm_scale_body(B, a.n, a.nch, a.mm, j, last, a.partial + (int64_t)2 * a.nrhs * a.kmax * a.nchd,
                            a.bnorm, a.H, a.cs, a.sn, a.g, a.est, a.inv, a.cyc, a.used, wj, wj, a.pscale);
 }
-__global__ void k_seg_backsolve(const SegDims* __restrict__ dims, int nseg, const SegKrylov* __restrict__ seg) {
-    int s;
-    VBlock B;
-    if (!seg_block(dims, nseg, s, B)) return;
-    const SegKrylov& a = seg[s];
-    k_backsolve_body(B, a.nrhs, a.mm, a.kmax, a.H, a.g, a.used, a.coef);
-}
 __global__ __launch_bounds__(256) void k_seg_xupdate(const SegDims* __restrict__ dims, int nseg,
                                                      const SegKrylov* __restrict__ seg) {
     int s;
     VBlock B;
     if (!seg_block(dims, nseg, s, B)) return;
     const SegKrylov& a = seg[s];
-    k_xupdate_body(B, a.n, a.nrhs, a.kmax, a.Z, a.coef, a.used, a.x);
+    k_xupdate_body(B, a.n, a.nrhs, a.mm, a.H, a.g, a.Z, a.used, a.x);
 }
 __global__ __launch_bounds__(256) void k_seg_residual(const SegDims* __restrict__ dims, int nseg,
                                                       const SegKrylov* __restrict__ seg, int conj) {
@@ -1046,9 +1032,6 @@ void seg_reduce_update(const SegDims* dims, int nseg, int blocks, const SegKrylo
 void seg_norm_scale(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int last, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_norm_scale, dim3(blocks), dim3(256), 0, s, dims, nseg, k, j, last);
 }
-void seg_backsolve(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s) {
-    hipLaunchKernelGGL(k_seg_backsolve, dim3(blocks), dim3(64), 0, s, dims, nseg, k);
-}
 void seg_xupdate(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_xupdate, dim3(blocks), dim3(256), 0, s, dims, nseg, k);
 }
@@ -1075,7 +1058,6 @@ KrylovGrids krylov_grids(int nz, int nx, int nrhs) {
     set(g.dots, g.nchd, nrhs, 1);
     set(g.update, g.nch, nrhs, 1);
     set(g.norm, 1, nrhs, 1);
-    set(g.backsolve, (nrhs + 63) / 64, 1, 1);
     set(g.xupdate, (int)((n + 255) / 256), nrhs, 1);
     set(g.residual, bx, by, (nrhs + RES_RB - 1) / RES_RB);
     set(g.init, nrhs, 1, 1);
@@ -1507,9 +1489,7 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
                                    W.cs, W.sn, W.g, W.est, W.bnorm, W.cyc, W.used);
             HELM_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, cs, nrhs, mm, kmax, W.H, W.g, W.used,
-                           W.coef);
-        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, cs, n, nrhs, kmax, W.Z, W.coef, W.used, x);
+        hipLaunchKernelGGL(k_xupdate, gvec, blk, 0, cs, n, nrhs, mm, W.H, W.g, W.Z, W.used, x);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     };
@@ -1787,10 +1767,8 @@ extern "C" int32_t helm_krylov_end(helm_krylov* k, const void* Z, void* x, int32
     Workspace& W = K.W;
     hipStream_t s = (hipStream_t)stream;
     const int nrhs = K.nrhs, kmax = K.m + 1;
-    hipLaunchKernelGGL(k_backsolve, dim3((nrhs + 63) / 64), dim3(64), 0, s, nrhs, K.m, kmax, W.H, W.g, W.used,
-                       W.coef);
-    hipLaunchKernelGGL(k_xupdate, dim3((unsigned)((K.n + 255) / 256), nrhs), dim3(256), 0, s, K.n, nrhs, kmax,
-                       (const float2*)Z, W.coef, W.used, (double2*)x);
+    hipLaunchKernelGGL(k_xupdate, dim3((unsigned)((K.n + 255) / 256), nrhs), dim3(256), 0, s, K.n, nrhs, K.m, W.H,
+                       W.g, (const float2*)Z, W.used, (double2*)x);
     HELM_HIP(hipGetLastError());
     if (steps) {
         std::vector<int> u(nrhs);

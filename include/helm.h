@@ -70,6 +70,9 @@ extern "C" {
 #define HELM_PC_MG 1       /* complex-shifted-Laplacian multigrid V-cycle   */
 #define HELM_PC_LEARNED 2  /* learned conv-net correction + MG V-cycle      */
 #define HELM_PC_SPECTRAL 3 /* Fourier multiplier gain * ifft2(M fft2(v))    */
+#define HELM_PC_SPECTRAL_PLAIN 4 /* the same without the gain: ifft2(M fft2(v)),
+                                  the preconditioner of solve_frequency's GMRES
+                                  (nsrno/wave_solver.py:56-59)             */
 
 #define HELM_OP_FLUX5 0    /* 5-point flux form, stretched PML (wave.py)   */
 #define HELM_OP_STAG8 1    /* 8th-order staggered, periodic + sponge       */

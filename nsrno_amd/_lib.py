@@ -13,7 +13,7 @@ HELM_CONVERGED, HELM_ITERATION_BUDGET, HELM_BREAKDOWN, HELM_STAGNATION, HELM_DIV
 REASONS = {HELM_CONVERGED: "converged", HELM_ITERATION_BUDGET: "iteration_budget",
            HELM_BREAKDOWN: "breakdown", HELM_STAGNATION: "stagnation", HELM_DIVERGED: "diverged"}
 HELM_GMRES, HELM_BICGSTAB = 0, 1
-PC_NONE, PC_MG, PC_LEARNED, PC_SPECTRAL = 0, 1, 2, 3
+PC_NONE, PC_MG, PC_LEARNED, PC_SPECTRAL, PC_SPECTRAL_PLAIN = 0, 1, 2, 3, 4
 
 
 class HelmDesc(ctypes.Structure):

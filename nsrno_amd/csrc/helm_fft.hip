@@ -10,9 +10,10 @@
 //   v[r] = in[j + r N/R] e^{s 2 pi i k r/(Ns R)},  k = j mod Ns,
 //   out[(j div Ns) Ns R + k + q Ns] = sum_r v[r] e^{s 2 pi i r q/R},
 // which leaves the result in natural order after the last pass (the
-// autosorting formulation of Govindaraju et al., SC 2008).  Radices 2-13
-// are register kernels; any other prime factor runs a generic kernel that
-// forms one output per thread.  Rows (x, unit stride) and columns (z, stride
+// autosorting formulation of Govindaraju et al., SC 2008).  Radices 2-31
+// (2, 3, 4, 5, 7, 8 and the primes 11-31) are register kernels; a larger
+// prime factor runs a generic kernel that forms one output per thread (and
+// so reads each input R times).  Rows (x, unit stride) and columns (z, stride
 // W) have their own kernels so that a wavefront always reads consecutive
 // complex numbers: along x it walks j, along z it walks x.  FP32 data with
 // twiddle phases formed from exact integer products.
@@ -171,6 +172,11 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
                 case 8: launch_rows<C, 8>(W, lines, Ns, sign, in, out, s); break;
                 case 11: launch_rows<C, 11>(W, lines, Ns, sign, in, out, s); break;
                 case 13: launch_rows<C, 13>(W, lines, Ns, sign, in, out, s); break;
+                case 17: launch_rows<C, 17>(W, lines, Ns, sign, in, out, s); break;
+                case 19: launch_rows<C, 19>(W, lines, Ns, sign, in, out, s); break;
+                case 23: launch_rows<C, 23>(W, lines, Ns, sign, in, out, s); break;
+                case 29: launch_rows<C, 29>(W, lines, Ns, sign, in, out, s); break;
+                case 31: launch_rows<C, 31>(W, lines, Ns, sign, in, out, s); break;
                 default: {
                     const int64_t cnt = (int64_t)W * lines;
                     hipLaunchKernelGGL(k_fft_generic<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W, R,
@@ -188,6 +194,11 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
                 case 8: launch_cols<C, 8>(H, W, B, Ns, sign, in, out, s); break;
                 case 11: launch_cols<C, 11>(H, W, B, Ns, sign, in, out, s); break;
                 case 13: launch_cols<C, 13>(H, W, B, Ns, sign, in, out, s); break;
+                case 17: launch_cols<C, 17>(H, W, B, Ns, sign, in, out, s); break;
+                case 19: launch_cols<C, 19>(H, W, B, Ns, sign, in, out, s); break;
+                case 23: launch_cols<C, 23>(H, W, B, Ns, sign, in, out, s); break;
+                case 29: launch_cols<C, 29>(H, W, B, Ns, sign, in, out, s); break;
+                case 31: launch_cols<C, 31>(H, W, B, Ns, sign, in, out, s); break;
                 default: {
                     const int64_t lines = (int64_t)W * B;
                     const int64_t cnt = (int64_t)H * lines;
@@ -232,7 +243,8 @@ __global__ void k_sp_gain(int64_t n, int64_t cnt, float scale, int conj, const f
     z[e] = conj ? cconj(v) : v;
 }
 
-int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
+int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                         bool use_gain) {
     HELM_CHECK(p->fft_mult, HELM_ERR_STATE, "spectral preconditioner: call helm_spectral_set_preconditioner first");
     HELM_CHECK(nrhs <= p->fft_cap, HELM_ERR_ARG, "spectral preconditioner: more right-hand sides than its buffers");
     const Level& L = p->levels[0];
@@ -258,7 +270,7 @@ int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, boo
     float2* fo = (f == a) ? b : a;
     float2* r = fft2<float2>(H, W, nrhs, +1, f, fo, f, s);
     hipLaunchKernelGGL(k_sp_gain, g, blk, 0, s, n, cnt, (float)(1.0 / (double)n), (int)!as_given,
-                       (const float2*)p->fft_gain, (const float2*)r, z);
+                       use_gain ? (const float2*)p->fft_gain : nullptr, (const float2*)r, z);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

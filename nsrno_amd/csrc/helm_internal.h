@@ -468,7 +468,8 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 // Fourier-multiplier preconditioner z = gain ifft2(M fft2(v)) (helm_fft.hip);
 // adjoint = false applies its conjugate map
-int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s);
+int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
+                         bool use_gain = true);
 // learned correction z += N(v) (helm_net.hip)
 int32_t net_correction(helm_plan* p, struct helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s);
 }  // namespace helm

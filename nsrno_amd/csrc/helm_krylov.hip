@@ -1789,9 +1789,9 @@ extern "C" int32_t helm_solve(helm_plan* p, const void* rhs, void* x, int32_t nr
     HELM_CHECK(o->max_iterations >= 1, HELM_ERR_ARG, "helm_solve: max_iterations must be positive");
     HELM_CHECK(std::isfinite(o->tolerance) && o->tolerance > 0, HELM_ERR_ARG,
                "helm_solve: tolerance must be finite and positive");
-    HELM_CHECK(o->precond >= HELM_PC_NONE && o->precond <= HELM_PC_SPECTRAL, HELM_ERR_ARG,
+    HELM_CHECK(o->precond >= HELM_PC_NONE && o->precond <= HELM_PC_SPECTRAL_PLAIN, HELM_ERR_ARG,
                "helm_solve: unknown preconditioner");
-    HELM_CHECK(o->precond != HELM_PC_SPECTRAL || p->fft_mult, HELM_ERR_STATE,
+    HELM_CHECK((o->precond != HELM_PC_SPECTRAL && o->precond != HELM_PC_SPECTRAL_PLAIN) || p->fft_mult, HELM_ERR_STATE,
                "helm_solve: call helm_spectral_set_preconditioner before using the spectral preconditioner");
     HELM_CHECK(o->precond != HELM_PC_MG || p->mg.levels > 0, HELM_ERR_STATE,
                "helm_solve: call helm_mg_setup before using the multigrid preconditioner");
@@ -1845,11 +1845,12 @@ extern "C" int32_t helm_warm_start(helm_plan* p, int32_t precond, const void* b,
     HELM_CHECK(nrhs >= 1 && nrhs <= 65535 && (int64_t)nrhs * p->levels[0].n < ((int64_t)1 << 31), HELM_ERR_ARG,
                "helm_warm_start: nrhs out of range");
     HELM_CHECK(steps >= 0 && steps <= 1000, HELM_ERR_ARG, "helm_warm_start: steps must be in [0, 1000]");
-    HELM_CHECK(precond >= HELM_PC_NONE && precond <= HELM_PC_SPECTRAL, HELM_ERR_ARG,
+    HELM_CHECK(precond >= HELM_PC_NONE && precond <= HELM_PC_SPECTRAL_PLAIN, HELM_ERR_ARG,
                "helm_warm_start: unknown preconditioner");
-    HELM_CHECK(precond == HELM_PC_NONE || precond == HELM_PC_SPECTRAL || p->mg.levels > 0, HELM_ERR_STATE,
+    HELM_CHECK(precond == HELM_PC_NONE || precond == HELM_PC_SPECTRAL || precond == HELM_PC_SPECTRAL_PLAIN ||
+                   p->mg.levels > 0, HELM_ERR_STATE,
                "helm_warm_start: call helm_mg_setup before using the multigrid preconditioner");
-    HELM_CHECK(precond != HELM_PC_SPECTRAL || p->fft_mult, HELM_ERR_STATE,
+    HELM_CHECK((precond != HELM_PC_SPECTRAL && precond != HELM_PC_SPECTRAL_PLAIN) || p->fft_mult, HELM_ERR_STATE,
                "helm_warm_start: call helm_spectral_set_preconditioner first");
     return warm_start(p, precond, (const double2*)b, (double2*)x, nrhs, steps, adjoint != 0, (hipStream_t)stream);
 }

@@ -1777,7 +1777,8 @@ int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
 }
 
 int32_t precond_apply(helm_plan* p, int kind, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s) {
-    if (kind == HELM_PC_SPECTRAL) return spectral_precond(p, v, z, nrhs, adjoint, s);
+    if (kind == HELM_PC_SPECTRAL || kind == HELM_PC_SPECTRAL_PLAIN)
+        return spectral_precond(p, v, z, nrhs, adjoint, s, kind == HELM_PC_SPECTRAL);
     if (p->levels[0].op == HELM_OP_COND8) {
         HELM_CHECK(kind == HELM_PC_MG, HELM_ERR_ARG, "conditioned plans take the multigrid preconditioner only");
         return cond_precond(p, v, z, nrhs, adjoint, s);

@@ -27,7 +27,8 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-PRECONDITIONERS = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED, "spectral": _lib.PC_SPECTRAL}
+PRECONDITIONERS = {"none": _lib.PC_NONE, "mg": _lib.PC_MG, "learned": _lib.PC_LEARNED, "spectral": _lib.PC_SPECTRAL,
+                   "spectral_plain": _lib.PC_SPECTRAL_PLAIN}
 
 
 @dataclass
@@ -186,7 +187,8 @@ class HelmholtzPlan:
     def set_spectral_preconditioner(self, multiplier, gain=None):
         """Fourier-multiplier preconditioner gain * ifft2(multiplier * fft2(v))
         on the plan's periodic box (helm_spectral_set_preconditioner); then
-        precond="spectral" / kind=PC_SPECTRAL use it."""
+        precond="spectral" / kind=PC_SPECTRAL use it, precond="spectral_plain" /
+        PC_SPECTRAL_PLAIN the same without the gain."""
         m = torch.as_tensor(multiplier, device=self.device).to(torch.complex128).reshape(self.nz, self.nx).contiguous()
         g = None if gain is None else torch.as_tensor(gain, device=self.device).to(torch.complex128).reshape(
             self.nz, self.nx).contiguous()

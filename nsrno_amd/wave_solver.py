@@ -7,9 +7,11 @@ positive multiple of restart), return value (prediction [192] complex64,
 info dict with exactly the keys reason, initial_neural_iterations,
 solver_iterations, relative_residual, seconds, trace) and semantics
 (wave_solver.py:38-67).  The solve itself is libhelm.so's flexible GMRES with
-FP64 true-residual refinement and the multigrid preconditioner, so
-``relative_residual`` is the true FP64 ||f - L u|| / ||f||, and tolerances
-below the complex64 rounding floor are reached.
+FP64 true-residual refinement and the reference's own preconditioner,
+ifft2(multiplier fft2(r)) without the gain (wave_solver.py:56-59; the
+project's mixed-radix FFT kernels), so ``relative_residual`` is the true
+FP64 ||f - L u|| / ||f||, and tolerances below the complex64 rounding floor
+are reached.
 
 ``gmres`` keeps the reference's callable-based signature
 (wave_solver.py:9-36) for any caller operator and preconditioner (this
@@ -101,8 +103,10 @@ def solve_frequency_batch(model: WaveResidualOperator, medium, frequency_hz, ang
         for t, r in zip(traces, rel):
             t.append(dict(iterations=applications - calls[0], relative_residual=float(r)))
 
+    # GMRES preconditioned by ifft2(multiplier fft2(r)) without the gain, as
+    # wave_solver.py:56-59 (the learned boundary taper is left out there)
     x, info = data.plan.solve(rhs, x, tolerance=tolerance, restart=restart, max_iterations=max_iterations,
-                              precond="mg", adjoint=True, callback=monitor)
+                              precond="spectral_plain", adjoint=True, callback=monitor)
     cols = model.rx_cols.to(model.device)
     preds = torch.stack([x[k, model.face, cols] * model.source_factor(frequency_hz, a).to(model.device)
                          for k, a in enumerate(angles)]).to(torch.complex64)

@@ -30,7 +30,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("H,W", [(32, 96), (1056 // 8, 1216 // 16), (64, 2 * 37), (88, 1216), (24, 44)])
+@pytest.mark.parametrize("H,W", [(32, 96), (1056 // 8, 1216 // 16), (64, 2 * 37), (88, 1216), (24, 44),
+                                 (2 * 31, 2 * 29), (2 * 23, 2 * 17), (4 * 41, 8)])
 def test_fft2_matches_numpy(H, W, precision):
     from nsrno_amd import _lib
     rng = np.random.default_rng(H * W)

@@ -216,11 +216,11 @@ struct Batch {
     void vcycle(int j, hipStream_t s) {
         for (int l = 0; l < max_levels; ++l)
             if (pre[l].nseg)
-                seg_mg_pre(pre[l].dims, pre[l].nseg, pre[l].blocks, (const SegLevel*)pre[l].tab, j, conj, s);
+                seg_mg_pre(pre[l].dims, pre[l].nseg, pre[l].blocks, (const SegLevel*)pre[l].tab, j, conj, l, s);
         seg_mg_tail(tail.dims, tail.nseg, tail.blocks, tail_lds, (const SegTail*)tail.tab, conj, s);
         for (int l = max_levels - 1; l >= 0; --l)
             if (post[l].nseg)
-                seg_mg_post(post[l].dims, post[l].nseg, post[l].blocks, (const SegLevel*)post[l].tab, j, conj, s);
+                seg_mg_post(post[l].dims, post[l].nseg, post[l].blocks, (const SegLevel*)post[l].tab, j, conj, l, s);
     }
 
     int32_t cycle(int nsteps, hipStream_t s) {

@@ -123,6 +123,23 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Load through a pointer known to address global memory.  A pointer read
+// from memory (an argument table) is a generic pointer to the compiler, and
+// generic loads count against the LDS counter as well, so waits for them also
+// wait for LDS operations; this emits a global load.  (The host pass of the
+// same source never executes device code and keeps a plain load.)
+__device__ __forceinline__ float2 ld_global(const float2* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using g64 = const __attribute__((address_space(1))) unsigned long long*;
+    const unsigned long long v = *(g64)(const unsigned long long*)p;  // C-style: an address-space cast
+    float2 r;
+    __builtin_memcpy(&r, &v, sizeof(r));
+    return r;
+#else
+    return *p;
+#endif
+}
+
 template <typename R> struct cplx;
 template <> struct cplx<float> { using type = float2; };
 template <> struct cplx<double> { using type = double2; };
@@ -303,8 +320,11 @@ struct SegTail {
 };
 
 // segmented kernel launches (grid: sum of the segments' padded block counts)
-void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s);
-void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s);
+// level: the grid level of every plan in the launch (level 0 forms its stencils from the medium)
+void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, int level,
+                hipStream_t s);
+void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, int level,
+                 hipStream_t s);
 void seg_mg_tail(const SegDims* dims, int nseg, int blocks, size_t lds, const SegTail* tv, int conj, hipStream_t s);
 void seg_apply32(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, int conj, hipStream_t s);
 void seg_dots(const SegDims* dims, int nseg, int blocks, const SegKrylov* k, int j, hipStream_t s);

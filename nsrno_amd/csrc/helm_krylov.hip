@@ -160,13 +160,25 @@ __device__ __forceinline__ void k_dcgs_dots_body(const VBlock& B, int64_t n, int
     const float2* vj = vr + (int64_t)j * vstride;
     float2 wv[DOT_NPT], jv[DOT_NPT], cur[DOT_NPT], nxt[DOT_NPT];
     const float2 zero = make_float2(0.f, 0.f);
+    // every load of the chunk is issued before the first is used (addresses
+    // clamped, out-of-range values zeroed after): one memory latency, where
+    // loads behind bounds branches are waited for element by element.  For
+    // j = 0, vr is vj, so cur = V_0 = jv as the products below need.
 #pragma unroll
     for (int i = 0; i < DOT_NPT; ++i) {
         const int64_t p = beg + threadIdx.x + 256 * i;
-        const bool in = p < n;
-        wv[i] = in ? wr[p] : zero;
-        jv[i] = in ? vj[p] : zero;
-        cur[i] = (in && j > 0) ? vr[p] : jv[i];
+        const int64_t pc = p < n ? p : 0;
+        wv[i] = wr[pc];
+        jv[i] = vj[pc];
+        cur[i] = vr[pc];
+    }
+#pragma unroll
+    for (int i = 0; i < DOT_NPT; ++i) {
+        if (beg + threadIdx.x + 256 * i >= n) {
+            wv[i] = zero;
+            jv[i] = zero;
+            cur[i] = zero;
+        }
     }
     for (int k = 0; k <= j; ++k) {
         if (k + 1 < j) {
@@ -174,8 +186,11 @@ __device__ __forceinline__ void k_dcgs_dots_body(const VBlock& B, int64_t n, int
 #pragma unroll
             for (int i = 0; i < DOT_NPT; ++i) {
                 const int64_t p = beg + threadIdx.x + 256 * i;
-                nxt[i] = p < n ? vn[p] : zero;
+                nxt[i] = vn[p < n ? p : 0];
             }
+#pragma unroll
+            for (int i = 0; i < DOT_NPT; ++i)
+                if (beg + threadIdx.x + 256 * i >= n) nxt[i] = zero;
         } else {
 #pragma unroll
             for (int i = 0; i < DOT_NPT; ++i) nxt[i] = jv[i];
@@ -443,9 +458,22 @@ __device__ __forceinline__ void k_dcgs_reduce_update_body(const VBlock& B, int64
         double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
         const double2* srcc = pc + ((int64_t)r * kmax + k) * nchd;
         const double2* srcd = pd + ((int64_t)r * kmax + k) * nchd;
-        for (int c = lane; c < nchd; c += 64) {
-            a = cadd(a, srcc[c]);
-            b = cadd(b, srcd[c]);
+        // four strides of partials in flight per wait, summed in the order
+        // c = lane, lane + 64, ... as one at a time
+        for (int c0 = lane; c0 < nchd; c0 += 256) {
+            double2 ta[4], tb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = c0 + 64 * q < nchd ? c0 + 64 * q : 0;
+                ta[q] = srcc[c];
+                tb[q] = srcd[c];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c0 + 64 * q < nchd) {
+                    a = cadd(a, ta[q]);
+                    b = cadd(b, tb[q]);
+                }
         }
         a.x = wave_sum(a.x);
         a.y = wave_sum(a.y);
@@ -505,21 +533,44 @@ __device__ __forceinline__ void k_dcgs_reduce_update_body(const VBlock& B, int64
     const float2 hjj = hjj_s;
     const float ia = ialpha_s;
     double nrm = 0.0;
-    for (int64_t p = beg + threadIdx.x; p < end; p += 256) {
-        float2 sc2 = make_float2(0.f, 0.f), sd2 = make_float2(0.f, 0.f);
-        for (int k = 0; k < j; ++k) {
-            const float2 v = vr[(int64_t)k * vstride + p];
-            cfma(sc2, c_s[k], v);
-            cfma(sd2, d_s[k], v);
+    // UPD_U nodes per thread per pass: their loads are issued before the first
+    // is used (the stores of V_j and w may alias the basis loads, so a plain
+    // loop would load, wait and store node by node); same arithmetic and
+    // accumulation order as one node at a time
+    constexpr int UPD_U = 4;
+    for (int64_t p0 = beg + threadIdx.x; p0 < end; p0 += 256 * UPD_U) {
+        int64_t pp[UPD_U];
+        float2 jvv[UPD_U], wvv[UPD_U], sc2[UPD_U], sd2[UPD_U];
+#pragma unroll
+        for (int q = 0; q < UPD_U; ++q) {
+            pp[q] = p0 + 256 * q < end ? p0 + 256 * q : beg;
+            jvv[q] = vj[pp[q]];
+            wvv[q] = wr[pp[q]];
+            sc2[q] = make_float2(0.f, 0.f);
+            sd2[q] = make_float2(0.f, 0.f);
         }
-        float2 jv = csub(vj[p], sc2);
-        jv = make_float2(jv.x * ia, jv.y * ia);
-        vj[p] = jv;
-        float2 wv = csub(wr[p], sd2);
-        wv.x -= hjj.x * jv.x - hjj.y * jv.y;
-        wv.y -= hjj.x * jv.y + hjj.y * jv.x;
-        wr[p] = wv;
-        nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+        for (int k = 0; k < j; ++k) {
+            float2 v[UPD_U];
+#pragma unroll
+            for (int q = 0; q < UPD_U; ++q) v[q] = vr[(int64_t)k * vstride + pp[q]];
+#pragma unroll
+            for (int q = 0; q < UPD_U; ++q) {
+                cfma(sc2[q], c_s[k], v[q]);
+                cfma(sd2[q], d_s[k], v[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < UPD_U; ++q) {
+            if (p0 + 256 * q >= end) continue;
+            float2 jv = csub(jvv[q], sc2[q]);
+            jv = make_float2(jv.x * ia, jv.y * ia);
+            vj[pp[q]] = jv;
+            float2 wv = csub(wvv[q], sd2[q]);
+            wv.x -= hjj.x * jv.x - hjj.y * jv.y;
+            wv.y -= hjj.x * jv.y + hjj.y * jv.x;
+            wr[pp[q]] = wv;
+            nrm += (double)wv.x * wv.x + (double)wv.y * wv.y;
+        }
     }
     const double2 tot = block_sum(make_double2(nrm, 0.0), lds);
     if (threadIdx.x == 0) partial[(int64_t)r * nch + chunk] = tot;
@@ -901,37 +952,81 @@ __global__ void k_scale32(int64_t n, const float2* __restrict__ src, float2* __r
 // used[r] columns of the rotated Hessenberg; FP64 accumulate into c128 x).
 // Thread 0 of every workgroup solves its RHS's small triangle into LDS (the
 // same operations in the same order in every workgroup, so every workgroup
-// holds the same y) and the workgroup applies it to its chunk of x.
+// holds the same y) and the workgroup applies it to its chunk of x.  The
+// loads of x and Z are issued first and stay in flight (LDS-only barriers)
+// while the triangle's entries, loaded into LDS by all threads at once, are
+// solved.
+constexpr int XU_MAX = 4;  // columns whose Z loads are issued ahead (more: loaded after the solve)
+constexpr int XU_LDS = 16;  // columns whose triangle is staged in LDS (more: read from global memory)
+// back substitution of the u x u triangle (entry (q, k) at t[q * ld + k]) for y
+template <typename T>
+__device__ __forceinline__ void xupdate_triangle(const T* __restrict__ t, int ld, const T* __restrict__ gv, int u,
+                                                 double2* __restrict__ yy) {
+    for (int k = u - 1; k >= 0; --k) {
+        double2 a = gv[k];
+        for (int q = k + 1; q < u; ++q) a = csub(a, cmul(t[q * ld + k], yy[q]));
+        const double2 d = t[k * ld + k];
+        const double dd = d.x * d.x + d.y * d.y;
+        yy[k] = dd > 0 ? make_double2((a.x * d.x + a.y * d.y) / dd, (a.y * d.x - a.x * d.y) / dd)
+                       : make_double2(0.0, 0.0);
+    }
+}
 __device__ __forceinline__ void k_xupdate_body(const VBlock& B, int64_t n, int nrhs, int m,
                                                const double2* __restrict__ H, const double2* __restrict__ g,
                                                const float2* __restrict__ Z, const int* __restrict__ used,
                                                double2* __restrict__ x) {
     __shared__ double2 yy[64];
+    __shared__ double2 sh[256], sg[256];  // one slot per thread (launched with 256 threads)
+    static_assert(XU_LDS * XU_LDS <= 256, "the staged triangle takes one slot per thread");
     const int r = B.y;
     const int u = used[r];
     if (u == 0) return;
-    if (threadIdx.x == 0) {
-        const double2* Hr = H + (int64_t)r * m * (m + 1);
-        for (int k = u - 1; k >= 0; --k) {
-            double2 acc = g[(int64_t)r * (m + 1) + k];
-            for (int q = k + 1; q < u; ++q) acc = csub(acc, cmul(Hr[(int64_t)q * (m + 1) + k], yy[q]));
-            const double2 d = Hr[(int64_t)k * (m + 1) + k];
-            const double dd = d.x * d.x + d.y * d.y;
-            yy[k] = dd > 0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
-                           : make_double2(0.0, 0.0);
-        }
-    }
-    __syncthreads();
-    int64_t p = (int64_t)B.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const int64_t p = (int64_t)B.x * blockDim.x + threadIdx.x;
+    const int64_t pc = p < n ? p : 0;
     const int64_t vstride = (int64_t)nrhs * n;
     const float2* zr = Z + (int64_t)r * n;
-    double2 acc = x[(int64_t)r * n + p];
-    for (int k = 0; k < u; ++k) {
-        float2 z = zr[(int64_t)k * vstride + p];
-        double2 c = yy[k];
-        acc.x += c.x * z.x - c.y * z.y;
-        acc.y += c.x * z.y + c.y * z.x;
+    const double2* Hr = H + (int64_t)r * m * (m + 1);
+    const double2* gr = g + (int64_t)r * (m + 1);
+    if (u > XU_LDS) {  // long cycles: the triangle from global memory
+        if (threadIdx.x == 0) xupdate_triangle(Hr, m + 1, gr, u, yy);
+        __syncthreads();
+    }
+    // The triangle's entries are loaded first (one per thread), then x and
+    // the first XU_MAX directions (addresses clamped, unused columns load Z_0
+    // again): waiting for the older entries leaves the data loads in flight
+    // through the staging and the solve, which read LDS only.
+    const bool staged = u <= XU_LDS;
+    const int e = (staged && (int)threadIdx.x < u * u) ? (int)threadIdx.x : 0;
+    const double2 hv = Hr[(int64_t)(e / u) * (m + 1) + e % u];
+    const double2 gv = gr[(int)threadIdx.x < u ? threadIdx.x : 0];
+    __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the data loads
+    double2 acc = x[(int64_t)r * n + pc];
+    float2 z[XU_MAX];
+#pragma unroll
+    for (int k = 0; k < XU_MAX; ++k) z[k] = zr[(int64_t)(k < u ? k : 0) * vstride + pc];
+    // every thread writes its slot (slots past u * u and u, and all of them
+    // when the triangle is not staged, hold unused copies): a store behind a
+    // branch would let the compiler sink the load to it, behind the data loads
+    sh[threadIdx.x] = hv;
+    sg[threadIdx.x] = gv;
+    if (staged) {
+        lds_barrier();
+        if (threadIdx.x == 0) xupdate_triangle(sh, u, sg, u, yy);
+        lds_barrier();
+    }
+    if (p >= n) return;
+#pragma unroll
+    for (int k = 0; k < XU_MAX; ++k)
+        if (k < u) {
+            const double2 c = yy[k];
+            acc.x += c.x * z[k].x - c.y * z[k].y;
+            acc.y += c.x * z[k].y + c.y * z[k].x;
+        }
+    for (int k = XU_MAX; k < u; ++k) {
+        const float2 zk = zr[(int64_t)k * vstride + p];
+        const double2 c = yy[k];
+        acc.x += c.x * zk.x - c.y * zk.y;
+        acc.y += c.x * zk.y + c.y * zk.x;
     }
     x[(int64_t)r * n + p] = acc;
 }

@@ -394,33 +394,86 @@ __device__ __forceinline__ float2 tail_mu_p(int nz, int nx, int n, int q, int i,
     return acc;
 }
 
-// k_tail's sweep sequence (same ping-pong order) on plane-layout coefficients
+// k_tail's sweep sequence (same ping-pong order) on plane-layout coefficients.
+// One workgroup runs a right-hand side's tail on one CU, so a sweep is bound
+// by that CU's LDS bandwidth: each thread keeps the stencil, Jacobi factor and
+// f of its nodes (up to TS_NPT) in registers for all the sweeps of the call,
+// and a sweep reads only the iterate's five stencil values from LDS (12 words
+// per node before).  Same operations in the same order as tail_mu_p.
+constexpr int TS_NPT = 3;
 __device__ void tail_sweeps_p(int nz, int nx, int n, const float2* cf, int sweeps, bool zero_start, bool conj,
                               const float2* f, float2* u, float2* t) {
     const float inx = 1.f / (float)nx;
+    const int bd = (int)blockDim.x;
     const float2* src = zero_start ? nullptr : u;
     if (!zero_start && sweeps % 2 == 1) {
-        for (int q = threadIdx.x; q < n; q += blockDim.x) t[q] = u[q];
+        for (int q = threadIdx.x; q < n; q += bd) t[q] = u[q];
         __syncthreads();
         src = t;
     }
-    for (int k = 0; k < sweeps; ++k) {
-        float2* dst = zero_start ? ((sweeps - k) % 2 == 1 ? u : t) : (src == u ? t : u);
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            const float2 fv = f[q];
-            if (src == nullptr) {
-                dst[q] = cmul(cc(cf[q], conj), fv);
-            } else {
-                float2 nu = src[q];
-                cfma(nu, cc(cf[q], conj), csub(fv, tail_mu_p(nz, nx, n, q, tail_row(q, inx), cf, src, conj)));
-                dst[q] = nu;
-            }
+    if (n <= TS_NPT * bd) {
+        float2 w[TS_NPT], dg[TS_NPT], fw[TS_NPT], fe[TS_NPT], fn[TS_NPT], fs[TS_NPT], fv[TS_NPT];
+        int qs[TS_NPT];
+        bool hw[TS_NPT], he[TS_NPT], hn[TS_NPT], hs[TS_NPT];
+#pragma unroll
+        for (int k = 0; k < TS_NPT; ++k) {
+            const int q = threadIdx.x + k * bd;
+            qs[k] = q < n ? q : -1;
+            const int qc = q < n ? q : 0;
+            const int i = tail_row(qc, inx), j = qc - i * nx;
+            hw[k] = j > 0;
+            he[k] = j < nx - 1;
+            hn[k] = i > 0;
+            hs[k] = i < nz - 1;
+            w[k] = cc(cf[qc], conj);
+            dg[k] = cc(cf[n + qc], conj);
+            fw[k] = cc(cf[2 * n + qc], conj);
+            fe[k] = cc(cf[3 * n + qc], conj);
+            fn[k] = cc(cf[4 * n + qc], conj);
+            fs[k] = cc(cf[5 * n + qc], conj);
+            fv[k] = f[qc];
         }
-        __syncthreads();
-        src = dst;
+        for (int k = 0; k < sweeps; ++k) {
+            float2* dst = zero_start ? ((sweeps - k) % 2 == 1 ? u : t) : (src == u ? t : u);
+#pragma unroll
+            for (int e = 0; e < TS_NPT; ++e) {
+                const int q = qs[e];
+                if (q < 0) continue;
+                if (src == nullptr) {
+                    dst[q] = cmul(w[e], fv[e]);
+                } else {
+                    float2 acc = cmul(dg[e], src[q]);
+                    if (hw[e]) cfma(acc, fw[e], src[q - 1]);
+                    if (he[e]) cfma(acc, fe[e], src[q + 1]);
+                    if (hn[e]) cfma(acc, fn[e], src[q - nx]);
+                    if (hs[e]) cfma(acc, fs[e], src[q + nx]);
+                    float2 nu = src[q];
+                    cfma(nu, w[e], csub(fv[e], acc));
+                    dst[q] = nu;
+                }
+            }
+            __syncthreads();
+            src = dst;
+        }
+    } else {
+        for (int k = 0; k < sweeps; ++k) {
+            float2* dst = zero_start ? ((sweeps - k) % 2 == 1 ? u : t) : (src == u ? t : u);
+            for (int q = threadIdx.x; q < n; q += bd) {
+                const float2 fq = f[q];
+                if (src == nullptr) {
+                    dst[q] = cmul(cc(cf[q], conj), fq);
+                } else {
+                    float2 nu = src[q];
+                    cfma(nu, cc(cf[q], conj), csub(fq, tail_mu_p(nz, nx, n, q, tail_row(q, inx), cf, src, conj)));
+                    dst[q] = nu;
+                }
+            }
+            __syncthreads();
+            src = dst;
+        }
     }
     if (zero_start && sweeps == 0) {
-        for (int q = threadIdx.x; q < n; q += blockDim.x) u[q] = make_float2(0.f, 0.f);
+        for (int q = threadIdx.x; q < n; q += bd) u[q] = make_float2(0.f, 0.f);
         __syncthreads();
     }
 }
@@ -440,19 +493,54 @@ __device__ __forceinline__ void k_tail_lds_body(const VBlock& B, const LevelView
             o += 9 * (int)views[l].n;
         }
     }
-    for (int l = t; l <= last; ++l) {
-        const LevelView& V = views[l];
-        const int n = (int)V.n;
-        float2* cf = lds + off[l - t] + 3 * n;
-        for (int k = threadIdx.x; k < 6 * n; k += blockDim.x) {
-            const int q = k / 6, c = k - 6 * q;
-            cf[c * n + q] = V.cf[k];
-        }
-    }
+    // Copy f and the coefficient tables ([node][6] -> six planes) into LDS.
+    // Every load of a batch is issued before its first LDS store (a plain
+    // copy loop waits for each load): f's loads first, then each level's
+    // table in batches of TAIL_CB values per thread, f's stores last.  The
+    // tail levels hold at most 160 KB / 72 B = 2275 nodes, so one batch of
+    // TAIL_FPT f values and of TAIL_CB table values per thread covers them.
     {
-        const int n = (int)views[t].n;
-        float2* f = lds + off[0] + 2 * n;
-        for (int q = threadIdx.x; q < n; q += blockDim.x) f[q] = f_top[(int64_t)r * n + q];
+        constexpr int TAIL_FPT = 3, TAIL_CB = 14;
+        const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+        const int nt = (int)views[t].n;
+        const float2* fs = f_top + (int64_t)r * nt;
+        float2 fv[TAIL_FPT];
+#pragma unroll
+        for (int i = 0; i < TAIL_FPT; ++i) {
+            const int k = i * bd + tid;
+            fv[i] = ld_global(fs + (k < nt ? k : 0));
+        }
+        int o = 0;
+        for (int l = t; l <= last; ++l) {
+            const int n = (int)views[l].n;
+            const float2* src = views[l].cf;
+            float2* cf = lds + o + 3 * n;
+            for (int base = 0; base < 6 * n; base += TAIL_CB * bd) {
+                float2 v[TAIL_CB];
+#pragma unroll
+                for (int i = 0; i < TAIL_CB; ++i) {
+                    const int k = base + i * bd + tid;
+                    v[i] = ld_global(src + (k < 6 * n ? k : 0));
+                }
+#pragma unroll
+                for (int i = 0; i < TAIL_CB; ++i) {
+                    // out-of-range values go to u[0] of the level, which the
+                    // sweeps write before they read it: a store behind a
+                    // branch would let the compiler sink its load to it
+                    const int k = base + i * bd + tid;
+                    const int q = k / 6, c = k - 6 * q;
+                    *(k < 6 * n ? cf + c * n + q : lds + o) = v[i];
+                }
+            }
+            o += 9 * n;
+        }
+        float2* f = lds + 2 * nt;  // level t's block starts at offset 0
+#pragma unroll
+        for (int i = 0; i < TAIL_FPT; ++i) {
+            const int k = i * bd + tid;
+            if (k < nt) f[k] = fv[i];
+        }
+        for (int k = TAIL_FPT * bd + tid; k < nt; k += bd) f[k] = ld_global(fs + k);
     }
     __syncthreads();
     // down: smooth, residual (into the partner buffer), restrict into the coarse level's f
@@ -901,28 +989,63 @@ __global__ __launch_bounds__(256) void k_mgt_restrict_res(int nzf, int nxf, int 
     }
 }
 
-// stencil and Jacobi factor of fine node `node`: from the level's
-// precomputed table (levels >= 1: [w/dg, dg, fw, fe, fn, fs] per node) or
-// formed from the medium (level 0, where the table would cost more bytes than
-// the formation saves)
-__device__ __forceinline__ void level_coef(int gi, int gj, int node, int nz, int nx, const float2* __restrict__ tcf,
-                                           const float2* __restrict__ sx, const float2* __restrict__ sz,
-                                           const float2* __restrict__ isxe, const float2* __restrict__ isze,
-                                           const float* __restrict__ eta, const float2* __restrict__ kap, float k0sq,
-                                           float hh, float shift, bool cj, const float2* __restrict__ wid,
-                                           Coef<float2>& c, float2& w) {
-    if (tcf) {
-        const float2* t = tcf + (int64_t)node * 6;
-        w = cc(t[0], cj);
-        c.dg = cc(t[1], cj);
-        c.fw = cc(t[2], cj);
-        c.fe = cc(t[3], cj);
-        c.fn = cc(t[4], cj);
-        c.fs = cc(t[5], cj);
+// Stencil and Jacobi factor of a level node: from the level's precomputed
+// table (levels >= 1: [w/dg, dg, fw, fe, fn, fs] per node, TAB) or formed from
+// the medium (level 0, where the table would cost more bytes than the
+// formation saves).  The fused level kernels load the operands of all of a
+// thread's nodes first (level_raw_load, addresses clamped, no branches) and
+// form the stencils after (level_raw_form), so the set-up waits for memory
+// once rather than once or twice per node.
+template <bool TAB>
+struct LevelRaw;
+template <>
+struct LevelRaw<true> {
+    float4 a, b, c;  // [w/dg, dg], [fw, fe], [fn, fs]
+};
+template <>
+struct LevelRaw<false> {
+    NodeRaw m;
+    float2 w;
+};
+// ok false: an in-range node is loaded and the result must not be used
+template <bool TAB>
+__device__ __forceinline__ void level_raw_load(int gi, int gj, bool ok, int nz, int nx,
+                                               const float2* __restrict__ tcf, const float2* __restrict__ sx,
+                                               const float2* __restrict__ sz, const float2* __restrict__ isxe,
+                                               const float2* __restrict__ isze, const float* __restrict__ eta,
+                                               const float2* __restrict__ kap, const float2* __restrict__ wid,
+                                               LevelRaw<TAB>& r) {
+    const int64_t node = ok ? (int64_t)gi * nx + gj : 0;
+    if constexpr (TAB) {
+        const float4* t = reinterpret_cast<const float4*>(tcf + node * 6);  // 48 B rows of a 256 B aligned table
+        r.a = t[0];
+        r.b = t[1];
+        r.c = t[2];
     } else {
-        node_coef<float, float2>(gi, gj, nz, nx, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, c);
-        w = cc(wid[node], cj);
+        node_raw_load(gi, gj, ok, nz, nx, sx, sz, isxe, isze, eta, kap, r.m);
+        r.w = wid[node];
     }
+}
+template <bool TAB>
+__device__ __forceinline__ void level_raw_form(const LevelRaw<TAB>& r, int gi, int gj, int nz, int nx, float k0sq,
+                                               float hh, float shift, bool cj, Coef<float2>& c, float2& w) {
+    if constexpr (TAB) {
+        w = cc(make_float2(r.a.x, r.a.y), cj);
+        c.dg = cc(make_float2(r.a.z, r.a.w), cj);
+        c.fw = cc(make_float2(r.b.x, r.b.y), cj);
+        c.fe = cc(make_float2(r.b.z, r.b.w), cj);
+        c.fn = cc(make_float2(r.c.x, r.c.y), cj);
+        c.fs = cc(make_float2(r.c.z, r.c.w), cj);
+    } else {
+        node_raw_form(r.m, gi, gj, nz, nx, k0sq, hh, shift, cj, c);
+        w = cc(r.w, cj);
+    }
+}
+// Jacobi factor alone (node clamped by the caller)
+template <bool TAB>
+__device__ __forceinline__ float2 level_w_load(int64_t node, const float2* __restrict__ tcf,
+                                               const float2* __restrict__ wid) {
+    return TAB ? tcf[node * 6] : wid[node];
 }
 
 // false when act (per RHS, nullable) marks every RHS of [r0, r1) inactive: the
@@ -947,7 +1070,7 @@ constexpr int PR_RZ = 2 * RR_TZ + 1, PR_RX = 2 * RR_TX + 1;  // residual region 
 constexpr int PR_2Z = PR_RZ + 2, PR_2X = PR_RX + 2, PR_2N = PR_2Z * PR_2X;  // u2 region 19 x 67
 constexpr int PR_1Z = PR_RZ + 4, PR_1X = PR_RX + 4, PR_1N = PR_1Z * PR_1X;  // f, u1 region 21 x 69
 constexpr int PR_1PT = (PR_1N + 255) / 256, PR_2PT = (PR_2N + 255) / 256;
-template <bool PF>
+template <bool PF, bool TAB>
 __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int nxf, int nzc, int nxc, int nrhs,
                                                                int rb,
                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
@@ -970,38 +1093,28 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;  // first residual node
     const int64_t nf = (int64_t)nzf * nxf, nc = (int64_t)nzc * nxc;
     const bool cj = conj != 0;
-    // f / u1 region nodes staged by this thread
+    // Set-up: the Jacobi factors of the f / u1 region nodes staged by this
+    // thread and the stencils of its u2 region nodes, every load issued before
+    // the first is used, then the first right-hand side's f with them
     int i1[PR_1PT];
     float2 w1[PR_1PT];
 #pragma unroll
     for (int q = 0; q < PR_1PT; ++q) {
         const int e = tid + 256 * q;
-        i1[q] = -1;
-        w1[q] = make_float2(0.f, 0.f);
-        if (e < PR_1N) {
-            const int gi = fi0 - 2 + e / PR_1X, gj = fj0 - 2 + e % PR_1X;
-            if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
-                i1[q] = gi * nxf + gj;
-                w1[q] = tcf ? cc(tcf[(int64_t)i1[q] * 6], cj) : cc(wid[i1[q]], cj);
-            }
-        }
+        const int gi = fi0 - 2 + e / PR_1X, gj = fj0 - 2 + e % PR_1X;
+        const bool ok = e < PR_1N && gi >= 0 && gi < nzf && gj >= 0 && gj < nxf;
+        i1[q] = ok ? gi * nxf + gj : -1;
+        w1[q] = level_w_load<TAB>(ok ? i1[q] : 0, tcf, wid);
     }
-    // u2 region nodes computed by this thread: stencil, Jacobi factor, roles
     int i2[PR_2PT];
-    Coef<float2> c2[PR_2PT];
-    float2 w2[PR_2PT];
+    LevelRaw<TAB> raw[PR_2PT];
 #pragma unroll
     for (int q = 0; q < PR_2PT; ++q) {
         const int e = tid + 256 * q;
-        i2[q] = -1;
-        if (e < PR_2N) {
-            const int gi = fi0 - 1 + e / PR_2X, gj = fj0 - 1 + e % PR_2X;
-            if (gi >= 0 && gi < nzf && gj >= 0 && gj < nxf) {
-                i2[q] = gi * nxf + gj;
-                level_coef(gi, gj, i2[q], nzf, nxf, tcf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, c2[q],
-                           w2[q]);
-            }
-        }
+        const int gi = fi0 - 1 + e / PR_2X, gj = fj0 - 1 + e % PR_2X;
+        const bool ok = e < PR_2N && gi >= 0 && gi < nzf && gj >= 0 && gj < nxf;
+        i2[q] = ok ? gi * nxf + gj : -1;
+        level_raw_load<TAB>(gi, gj, ok, nzf, nxf, tcf, sx, sz, isxe, isze, eta, kap, wid, raw[q]);
     }
     const int r0 = bz * rb, r1 = min(nrhs, r0 + rb);
     float2 fq[PR_1PT];
@@ -1014,9 +1127,19 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
             for (int q = 0; q < PR_1PT; ++q) fq[q] = make_float2(fq[q].x * fs, fq[q].y * fs);
         }
     };
-    if (PF) fetch(r0);
+    fetch(r0);
+#pragma unroll
+    for (int q = 0; q < PR_1PT; ++q) w1[q] = i1[q] >= 0 ? cc(w1[q], cj) : make_float2(0.f, 0.f);  // 0 on ghosts
+    Coef<float2> c2[PR_2PT];
+    float2 w2[PR_2PT];
+#pragma unroll
+    for (int q = 0; q < PR_2PT; ++q) {
+        const int e = tid + 256 * q;
+        level_raw_form<TAB>(raw[q], fi0 - 1 + e / PR_2X, fj0 - 1 + e % PR_2X, nzf, nxf, k0sq, hh, shift, cj, c2[q],
+                            w2[q]);
+    }
     for (int r = r0; r < r1; ++r) {
-        if (!PF) fetch(r);
+        if (!PF && r > r0) fetch(r);
         lds_barrier();  // the previous RHS is done with the LDS arrays
 #pragma unroll
         for (int q = 0; q < PR_1PT; ++q) {
@@ -1088,7 +1211,7 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
         }
     }
 }
-template <bool PF>
+template <bool PF, bool TAB>
 __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nzc, int nxc, int nrhs,
                                                                int rb,
                                                      const float2* __restrict__ sx, const float2* __restrict__ sz,
@@ -1100,7 +1223,7 @@ __global__ __launch_bounds__(256, 3) void k_mgt_pre2_rr(int nzf, int nxf, int nz
                                                      const double* __restrict__ fscale, float2* __restrict__ u,
                                                      float2* __restrict__ fcoarse, int xcd,
                                                      const int* __restrict__ act) {
-    k_mgt_pre2_rr_body<PF>(hw_block(), nzf, nxf, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, f, fscale, u, fcoarse, xcd, act);
+    k_mgt_pre2_rr_body<PF, TAB>(hw_block(), nzf, nxf, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, f, fscale, u, fcoarse, xcd, act);
 }
 
 // k_mgt_post2: out = S(S(u + P u_c)) in one pass on the 16 x 64 fine tile:
@@ -1113,7 +1236,7 @@ constexpr int P2_X = MT_TX + 4, P2_Z = MT_TZ + 4, P2_N = P2_X * P2_Z;
 constexpr int P2_PT = (P2_N + 255) / 256;
 // PF: the loads of the next right-hand side are issued before this one is
 // computed (more registers, two waves per SIMD); without, three waves per SIMD
-template <bool PF>
+template <bool PF, bool TAB>
 __device__ __forceinline__ void k_mgt_post2_body(const VBlock& B, int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
                                                              int rb,
                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
@@ -1154,21 +1277,18 @@ __device__ __forceinline__ void k_mgt_post2_body(const VBlock& B, int nz, int nx
             if (I >= 0 && I < nzc && J >= 0 && J < nxc) cix[q] = I * nxc + J;
         }
     }
-    int i1[MT_SPT];  // halo-1 nodes swept by this thread
-    Coef<float2> c1[MT_SPT];
-    float2 w1[MT_SPT];
+    // halo-1 nodes swept by this thread: every node's stencil operands are
+    // loaded before the first is formed, the first right-hand side's fields
+    // with them
+    int i1[MT_SPT];
+    LevelRaw<TAB> raw[MT_SPT];
 #pragma unroll
     for (int q = 0; q < MT_SPT; ++q) {
         const int e = tid + 256 * q;
-        i1[q] = -1;
-        if (e < MT_NS) {
-            const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
-            if (gi >= 0 && gi < nz && gj >= 0 && gj < nx) {
-                i1[q] = gi * nx + gj;
-                level_coef(gi, gj, i1[q], nz, nx, tcf, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, cj, wid, c1[q],
-                           w1[q]);
-            }
-        }
+        const int gi = z0 - 1 + e / MT_SW, gj = x0 - 1 + e % MT_SW;
+        const bool ok = e < MT_NS && gi >= 0 && gi < nz && gj >= 0 && gj < nx;
+        i1[q] = ok ? gi * nx + gj : -1;
+        level_raw_load<TAB>(gi, gj, ok, nz, nx, tcf, sx, sz, isxe, isze, eta, kap, wid, raw[q]);
     }
     const int r0 = bz * rb, r1 = min(nrhs, r0 + rb);
     float2 su[P2_PT], cs[MP_CPT], fv[MT_SPT];
@@ -1185,9 +1305,16 @@ __device__ __forceinline__ void k_mgt_post2_body(const VBlock& B, int nz, int nx
             for (int q = 0; q < MT_SPT; ++q) fv[q] = make_float2(fv[q].x * fs, fv[q].y * fs);
         }
     };
-    if (PF) fetch(r0);
+    fetch(r0);
+    Coef<float2> c1[MT_SPT];
+    float2 w1[MT_SPT];
+#pragma unroll
+    for (int q = 0; q < MT_SPT; ++q) {
+        const int e = tid + 256 * q;
+        level_raw_form<TAB>(raw[q], z0 - 1 + e / MT_SW, x0 - 1 + e % MT_SW, nz, nx, k0sq, hh, shift, cj, c1[q], w1[q]);
+    }
     for (int r = r0; r < r1; ++r) {
-        if (!PF) fetch(r);
+        if (!PF && r > r0) fetch(r);
         lds_barrier();
 #pragma unroll
         for (int q = 0; q < MP_CPT; ++q) {
@@ -1255,7 +1382,7 @@ __device__ __forceinline__ void k_mgt_post2_body(const VBlock& B, int nz, int nx
         }
     }
 }
-template <bool PF>
+template <bool PF, bool TAB>
 __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, int64_t n, int nzc, int nxc, int nrhs,
                                                              int rb,
                                                    const float2* __restrict__ sx, const float2* __restrict__ sz,
@@ -1266,7 +1393,7 @@ __global__ __launch_bounds__(256, PF ? 2 : 3) void k_mgt_post2(int nz, int nx, i
                                                    const float2* __restrict__ uc, const float2* __restrict__ u,
                                                    const float2* __restrict__ f, const double* __restrict__ fscale,
                                                    float2* __restrict__ out, int xcd, const int* __restrict__ act) {
-    k_mgt_post2_body<PF>(hw_block(), nz, nx, n, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, uc, u, f, fscale, out, xcd, act);
+    k_mgt_post2_body<PF, TAB>(hw_block(), nz, nx, n, nzc, nxc, nrhs, rb, sx, sz, isxe, isze, eta, kap, k0sq, hh, shift, conj, wid, tcf, uc, u, f, fscale, out, xcd, act);
 }
 
 // bilinear prolongation of the coarse correction at fine node (i, j) (zero
@@ -1449,6 +1576,18 @@ static int32_t tail_views(helm_plan* p) {
     return HELM_OK;
 }
 
+// the fused level kernels for this level: stencils from the level's table
+// (tab) or formed from the medium; HELM_MG_*_PREFETCH picks the variants that
+// load the next right-hand side ahead
+static decltype(&k_mgt_pre2_rr<false, false>) pre2_kernel(bool tab) {
+    if (pre2_prefetch()) return tab ? k_mgt_pre2_rr<true, true> : k_mgt_pre2_rr<true, false>;
+    return tab ? k_mgt_pre2_rr<false, true> : k_mgt_pre2_rr<false, false>;
+}
+static decltype(&k_mgt_post2<false, false>) post2_kernel(bool tab) {
+    if (post2_prefetch()) return tab ? k_mgt_post2<true, true> : k_mgt_post2<true, false>;
+    return tab ? k_mgt_post2<false, true> : k_mgt_post2<false, false>;
+}
+
 // Solve M_l u = f approximately.  f and u are level-l arrays that must not
 // alias the level's scratch: on level 0 they are the caller's input and
 // output, on coarser levels L.mg_f and L.mg_u.  L.mg_r holds the residual,
@@ -1494,14 +1633,14 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         const int kpre = (pf && (pf->mask >> PK_MG_PRE & 1)) ? PK_MG_PRE : (l == 0 ? PK_MG_PRE0 : -1);
         const int kpost = (pf && (pf->mask >> PK_MG_POST & 1)) ? PK_MG_POST : (l == 0 ? PK_MG_POST0 : -1);
         if (kpre >= 0 && prof_pair(pf, kpre, &ea, &eb)) {
-            hipExtLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ),
+            hipExtLaunchKernelGGL(pre2_kernel(tab != nullptr), g, dim3(RR_TX, RR_TZ),
                                   0, s, ea, eb, 0, L.nz, L.nx, C.nz, C.nx, nrhs,
                                   rbc, (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
                                   sh, (int)adj, (const float2*)L.wid, tab, f, fscale, L.mg_t, C.mg_f, xcd_pre(l), act);
             prof_push(pf, kpre, ea, eb, nrhs * (16.0 * nf + 8.0 * ncn) + coef);
         } else {
-            hipLaunchKernelGGL(pre2_prefetch() ? k_mgt_pre2_rr<true> : k_mgt_pre2_rr<false>, g, dim3(RR_TX, RR_TZ), 0,
+            hipLaunchKernelGGL(pre2_kernel(tab != nullptr), g, dim3(RR_TX, RR_TZ), 0,
                                s, L.nz, L.nx, C.nz, C.nx, nrhs, rbc, L.sx32,
                                L.sz32, L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, f,
                                fscale, L.mg_t, C.mg_f, xcd_pre(l), act);
@@ -1509,7 +1648,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
         HELM_HIP(hipGetLastError());
         HELM_TRY(vcycle(p, l + 1, C.mg_f, C.mg_u, nrhs, adj, s, nullptr, act));
         if (kpost >= 0 && prof_pair(pf, kpost, &ea, &eb)) {
-            hipExtLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, ea, eb, 0,
+            hipExtLaunchKernelGGL(post2_kernel(tab != nullptr), grd, blk, 0, s, ea, eb, 0,
                                   L.nz, L.nx, L.n, C.nz, C.nx, nrhs, rb,
                                   (const float2*)L.sx32, (const float2*)L.sz32, (const float2*)L.isxe32,
                                   (const float2*)L.isze32, (const float*)L.eta32, (const float2*)L.kap32, k0sq, hh,
@@ -1517,7 +1656,7 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
                                   (const float2*)L.mg_t, f, fscale, u, xcd_post(l), act);
             prof_push(pf, kpost, ea, eb, nrhs * (24.0 * nf + 8.0 * ncn) + coef);
         } else {
-            hipLaunchKernelGGL(post2_prefetch() ? k_mgt_post2<true> : k_mgt_post2<false>, grd, blk, 0, s, L.nz, L.nx,
+            hipLaunchKernelGGL(post2_kernel(tab != nullptr), grd, blk, 0, s, L.nz, L.nx,
                                L.n, C.nz, C.nx, nrhs, rb, L.sx32, L.sz32,
                                L.isxe32, L.isze32, L.eta32, L.kap32, k0sq, hh, sh, (int)adj, L.wid, tab, C.mg_u,
                                L.mg_t, f, fscale, u, xcd_post(l), act);
@@ -1584,25 +1723,29 @@ static int32_t vcycle(helm_plan* p, int l, const float2* f, float2* u, int nrhs,
 // l is a grid level above the tail.  Plans are independent, so running every
 // plan's pre-smoothing of level l before any plan's level l + 1 keeps each
 // plan's own order (pre 0..t-1, tail at t, post t-1..0).
+// TAB: the launch's level takes its stencils from the level table (every
+// plan of a launch is at the same level index, so all or none do)
+template <bool TAB>
 __global__ __launch_bounds__(256, 3) void k_seg_mg_pre(const SegDims* __restrict__ dims, int nseg,
                                                        const SegLevel* __restrict__ lv, int j, int conj) {
     int s;
     VBlock B;
     if (!seg_block(dims, nseg, s, B)) return;
     const SegLevel& a = lv[s];
-    k_mgt_pre2_rr_body<false>(B, a.nz, a.nx, a.nzc, a.nxc, a.nrhs, a.rb_pre, a.sx, a.sz, a.isxe, a.isze, a.eta,
-                              a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.f + j * a.f_step, a.fscale, a.t,
-                              a.fc, a.xcd_pre, a.act);
+    k_mgt_pre2_rr_body<false, TAB>(B, a.nz, a.nx, a.nzc, a.nxc, a.nrhs, a.rb_pre, a.sx, a.sz, a.isxe, a.isze, a.eta,
+                                   a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.f + j * a.f_step, a.fscale,
+                                   a.t, a.fc, a.xcd_pre, a.act);
 }
+template <bool TAB>
 __global__ __launch_bounds__(256, 3) void k_seg_mg_post(const SegDims* __restrict__ dims, int nseg,
                                                         const SegLevel* __restrict__ lv, int j, int conj) {
     int s;
     VBlock B;
     if (!seg_block(dims, nseg, s, B)) return;
     const SegLevel& a = lv[s];
-    k_mgt_post2_body<false>(B, a.nz, a.nx, a.n, a.nzc, a.nxc, a.nrhs, a.rb_post, a.sx, a.sz, a.isxe, a.isze, a.eta,
-                            a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.uc, a.t, a.f + j * a.f_step,
-                            a.fscale, a.u + j * a.u_step, a.xcd_post, a.act);
+    k_mgt_post2_body<false, TAB>(B, a.nz, a.nx, a.n, a.nzc, a.nxc, a.nrhs, a.rb_post, a.sx, a.sz, a.isxe, a.isze,
+                                 a.eta, a.kap, a.k0sq, a.hh, a.shift, conj, a.wid, a.tcf, a.uc, a.t,
+                                 a.f + j * a.f_step, a.fscale, a.u + j * a.u_step, a.xcd_post, a.act);
 }
 __global__ __launch_bounds__(1024) void k_seg_mg_tail(const SegDims* __restrict__ dims, int nseg,
                                                       const SegTail* __restrict__ tv, int conj) {
@@ -1612,11 +1755,15 @@ __global__ __launch_bounds__(1024) void k_seg_mg_tail(const SegDims* __restrict_
     const SegTail& a = tv[s];
     k_tail_lds_body(B, a.views, a.t, a.last, conj, a.pre, a.post, a.coarse, a.f, a.u, a.act);
 }
-void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s) {
-    hipLaunchKernelGGL(k_seg_mg_pre, dim3(blocks), dim3(RR_TX, RR_TZ), 0, s, dims, nseg, lv, j, conj);
+void seg_mg_pre(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, int level,
+                hipStream_t s) {
+    hipLaunchKernelGGL((level > 0 || table0()) ? k_seg_mg_pre<true> : k_seg_mg_pre<false>, dim3(blocks),
+                       dim3(RR_TX, RR_TZ), 0, s, dims, nseg, lv, j, conj);
 }
-void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, hipStream_t s) {
-    hipLaunchKernelGGL(k_seg_mg_post, dim3(blocks), dim3(64, 4), 0, s, dims, nseg, lv, j, conj);
+void seg_mg_post(const SegDims* dims, int nseg, int blocks, const SegLevel* lv, int j, int conj, int level,
+                 hipStream_t s) {
+    hipLaunchKernelGGL((level > 0 || table0()) ? k_seg_mg_post<true> : k_seg_mg_post<false>, dim3(blocks),
+                       dim3(64, 4), 0, s, dims, nseg, lv, j, conj);
 }
 void seg_mg_tail(const SegDims* dims, int nseg, int blocks, size_t lds, const SegTail* tv, int conj, hipStream_t s) {
     static const hipError_t attr =

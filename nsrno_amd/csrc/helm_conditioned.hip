@@ -361,10 +361,13 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
     const bool pmlx = __any((gA.x != R(0) || gA.y != R(0)) || (gB.x != R(0) || gB.y != R(0)));
 
     C U[RB][8], G[RB][8];
+    // u at (row, col) of RHS r0 + q, zero outside the box or past the last
+    // RHS; the load itself is unconditional (row and col are in range, the
+    // RHS is clamped), so no branch splits the loads a step issues
     auto ldu = [=](int q, int row, int col, bool colin) -> C {
         const int r = r0 + q;
-        if (r >= nrhs || !colin || row < bz0 || row >= bz0 + bh) return C{R(0), R(0)};
-        return u[(int64_t)r * n + (int64_t)row * W + col];
+        const C v = u[(int64_t)(r < nrhs ? r : nrhs - 1) * n + (int64_t)row * W + col];
+        return (r < nrhs && colin && row >= bz0 && row < bz0 + bh) ? v : C{R(0), R(0)};
     };
     // initial window: rows i0-8 .. i0-1
 #pragma unroll
@@ -376,19 +379,54 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
             G[q][k] = zero;
         }
     }
+    // Software pipeline: each step issues the loads of the next step (its
+    // staged row and face coefficient; the next output row's halo columns,
+    // x-face coefficients and mass) and works on what the step before loaded,
+    // so a step does not wait for memory it has just asked for.
+    C P[RB], ph0[RB], ph1[RB], pu[RB], pb[RB], pmk = zero;
+    R pa = R(0), pax = R(0), pbx = R(0);
+    auto pf_face = [&](int t) {  // step t: row t + 7, az at face t + 3
+#pragma unroll
+        for (int q = 0; q < RB; ++q) P[q] = ldu(q, cwrap(t + 7, H), jc, xin);
+        pa = az[(int64_t)cwrap(t + 3, H) * W + jc];
+    };
+    auto pf_out = [&](int i) {  // output row i (wrapped; past the run: unused): halo columns, ax, mass
+        const int ic = cwrap(i, H);
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {  // lanes 7.. load column ch0 / ch1 too and discard it
+            ph0[q] = ldu(q, ic, ch0, hin0 && l < 7);
+            ph1[q] = ldu(q, ic, ch1, hin1 && l < 7);
+        }
+        pax = ax[(int64_t)ic * W + fa];
+        const R axf = ax[(int64_t)ic * W + fb];
+        pbx = l < 7 ? axf : R(0);
+        const C m = mass[(int64_t)ic * W + jc];
+        pmk = (jok && xin && zin(ic)) ? m : zero;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {  // u (the identity rows outside the medium) and b (MODE 1) at the output
+            const int64_t o = (int64_t)(r0 + q < nrhs ? r0 + q : nrhs - 1) * n + (int64_t)ic * W + jc;
+            pu[q] = u[o];
+            pb[q] = MODE == 1 ? b[o] : zero;
+        }
+    };
+    pf_face(i0 - 7);
+    pf_out(i0);
     for (int t = i0 - 7; t < i1; ++t) {
-        const int rn = cwrap(t + 7, H);  // row staged now
         const int fz = cwrap(t + 3, H);  // z face formed now (face fz + 1/2)
+        C nrow[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) nrow[q] = P[q];
+        const R a = pa;
+        pf_face(t + 1);  // past the run: loads of valid rows, never used
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
 #pragma unroll
             for (int k = 0; k < 7; ++k) U[q][k] = U[q][k + 1];
-            U[q][7] = ldu(q, rn, jc, xin);
+            U[q][7] = nrow[q];
         }
         // z face t+3: rows t .. t+7 are U[.][0..7]
         {
             const C sv = tz[fz], gm = tz[H + fz];
-            const R a = az[(int64_t)fz * W + jc];
             C cz[8], czb[8];
             bool pz = false;
             if (ADJ) {
@@ -429,6 +467,20 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
         // ---- output row t ----
         const int i = t;
         const bool rin = zin(i);
+        C hh0[RB], hh1[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            hh0[q] = ph0[q];
+            hh1[q] = ph1[q];
+        }
+        const R axa = pax, axb = pbx;
+        C mkc = pmk, uo[RB], bo[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            uo[q] = pu[q];
+            bo[q] = pb[q];
+        }
+        pf_out(t + 1);
         // x: stage row i (and the halo columns) in LDS
         lds_barrier();  // previous row's LDS readers are done
 #pragma unroll
@@ -441,7 +493,7 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
                 if (pmlx) sb[q][l + 7] = cjm(cBj, v);
             }
             if (l < 7) {
-                const C h0 = ldu(q, i, ch0, hin0), h1 = ldu(q, i, ch1, hin1);
+                const C h0 = hh0[q], h1 = hh1[q];
                 if (!ADJ) {
                     sa[q][l] = h0;
                     sa[q][71 + l] = h1;
@@ -456,8 +508,6 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
             }
         }
         lds_barrier();
-        const R axa = ax[(int64_t)i * W + fa];
-        const R axb = l < 7 ? ax[(int64_t)i * W + fb] : R(0);
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
 #pragma unroll
@@ -486,15 +536,12 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
         bool pzo = false;
         if (!ADJ) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                gz[k] = tz[H + cwrap(i - 4 + k, H)];
-                pzo = pzo || gz[k].x != R(0) || gz[k].y != R(0);
-            }
+            for (int k = 0; k < 8; ++k) gz[k] = tz[H + cwrap(i - 4 + k, H)];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pzo = pzo | (gz[k].x != R(0)) | (gz[k].y != R(0));
         }
         const C cAz = tz[2 * H + cwrap(i, H)], cBz = tz[3 * H + cwrap(i, H)];
         const bool inside = jok && xin && rin;
-        C mkc = zero;
-        if (inside) mkc = mass[(int64_t)i * W + jc];
         if (ADJ) mkc.y = -mkc.y;
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
@@ -529,9 +576,9 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
                 }
                 cfma(acc, mkc, U[q][0]);
             } else {
-                acc = u[o];
+                acc = uo[q];
             }
-            if (MODE == 1) acc = csub(b[o], acc);
+            if (MODE == 1) acc = csub(bo[q], acc);
             out[o] = acc;
         }
     }

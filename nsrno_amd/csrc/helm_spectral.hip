@@ -81,30 +81,42 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
     const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
 
+    // face coefficients and kappa of the tile: every load issued before the
+    // first is used (indices wrapped or clamped into the arrays, values
+    // outside the tile zeroed after), one memory latency for the set-up
     R cfx[FXT], cfz[FZT];
 #pragma unroll
     for (int q = 0; q < FXT; ++q) {
         const int e = tid + q * 256;
-        R v = R(0);
-        if (e < TZ * NFX) {
-            const int r = e / NFX, f = e - r * NFX - 4;
-            v = ax[(int64_t)wrap_index(z0 + r, H) * W + wrap_index(x0 + f, W)];
-            if (MODE == 2) v -= R(1);
-        }
-        cfx[q] = v;
+        const int ec = e < TZ * NFX ? e : 0;
+        const int r = ec / NFX, f = ec - r * NFX - 4;
+        cfx[q] = ax[(int64_t)wrap_index(z0 + r, H) * W + wrap_index(x0 + f, W)];
     }
 #pragma unroll
     for (int q = 0; q < FZT; ++q) {
         const int e = tid + q * 256;
-        R v = R(0);
-        if (e < NFZ * TX) {
-            const int fr = e / TX - 4, c = e % TX;
-            v = az[(int64_t)wrap_index(z0 + fr, H) * W + wrap_index(x0 + c, W)];
-            if (MODE == 2) v -= R(1);
-        }
-        cfz[q] = v;
+        const int ec = e < NFZ * TX ? e : 0;
+        const int fr = ec / TX - 4, c = ec % TX;
+        cfz[q] = az[(int64_t)wrap_index(z0 + fr, H) * W + wrap_index(x0 + c, W)];
     }
     C kc[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = z0 + ty + 4 * k, j = x0 + tx;
+        kc[k] = kap[(int64_t)(i < H ? i : 0) * W + (j < W ? j : 0)];
+    }
+#pragma unroll
+    for (int q = 0; q < FXT; ++q) {
+        R v = tid + q * 256 < TZ * NFX ? cfx[q] : R(0);
+        if (MODE == 2 && tid + q * 256 < TZ * NFX) v -= R(1);
+        cfx[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < FZT; ++q) {
+        R v = tid + q * 256 < NFZ * TX ? cfz[q] : R(0);
+        if (MODE == 2 && tid + q * 256 < NFZ * TX) v -= R(1);
+        cfz[q] = v;
+    }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int i = z0 + ty + 4 * k, j = x0 + tx;
@@ -112,7 +124,7 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
         v.x = R(0);
         v.y = R(0);
         if (i < H && j < W) {
-            v = kap[(int64_t)i * W + j];
+            v = kc[k];
             if (MODE == 2) {
                 v.x -= R(1);
                 v.y -= sponge_at<R>(i, j, H, W, sp_cells, sp_amp);
@@ -165,6 +177,16 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
     if constexpr (PF) {
         if (r0 < r1) load(r0);
     }
+    // MODE 1: b at this thread's output nodes, loaded ahead of the next
+    // right-hand side's tile so that waiting for it leaves that tile in flight
+    C bv[RPT];
+    auto load_b = [&](int r) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int gi = z0 + ty + 4 * k, gj = x0 + tx;
+            bv[k] = b[(int64_t)r * n + (int64_t)(gi < H ? gi : 0) * W + (gj < W ? gj : 0)];
+        }
+    };
     for (int r = r0; r < r1; ++r) {
         lds_barrier();  // the previous right-hand side's readers are done
         if constexpr (PF) {
@@ -174,14 +196,28 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
                 for (int c = 0; c < 2; ++c)
                     if (staged(k, c)) su[(ty + 4 * k) * SU + tx + 64 * c] = pre[k][c];
             lds_barrier();
+            if (MODE == 1) load_b(r);
             if (r + 1 < r1) load(r + 1);
         } else {
+            // the whole tile is loaded before any of it is stored (addresses
+            // clamped for the nodes outside the staged region)
             const C* ur = u + (int64_t)r * n;
+            C tile[NR][2];
 #pragma unroll
             for (int k = 0; k < NR; ++k)
 #pragma unroll
-                for (int c = 0; c < 2; ++c)
-                    if (staged(k, c)) su[(ty + 4 * k) * SU + tx + 64 * c] = ur[rowoff[k] + colg[c]];
+                for (int c = 0; c < 2; ++c) tile[k][c] = ur[staged(k, c) ? rowoff[k] + colg[c] : 0];
+            if (MODE == 1) load_b(r);
+            // elements outside the staged region go to slot 0, a corner of the
+            // tile that is never read: a store behind a branch would let the
+            // compiler sink its load to it and wait for the loads one by one
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int keep = -(int)staged(k, c);  // all ones or zero: a mask, not a branch
+                    su[((ty + 4 * k) * SU + tx + 64 * c) & keep] = tile[k][c];
+                }
             __syncthreads();
         }
 #pragma unroll
@@ -226,7 +262,7 @@ __global__ __launch_bounds__(256) void k_stag8(int H, int W, int64_t n, int nrhs
                 acc.y += dz.y;
                 cfma(acc, kc[k], su[(i + HL) * SU + HL + c]);
                 const int64_t o = (int64_t)r * n + (int64_t)gi * W + gj;
-                if (MODE == 1) acc = csub(b[o], acc);
+                if (MODE == 1) acc = csub(bv[k], acc);
                 if (MODE == 2) {
                     acc.x = -acc.x;
                     acc.y = -acc.y;

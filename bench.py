@@ -281,17 +281,22 @@ def pmc_summary(directory):
              "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail(": "k_tail", "k_cond8s": "k_cond8s<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
     # multigrid kernels run on every level: their finest-level launches (the
-    # largest grid) are also summarised on their own ("<name> (finest level)")
+    # largest grid of the kernel, over all its template instances: the finest
+    # level forms its stencils from the medium, the others read a table) are
+    # also summarised on their own ("<name> (finest level)")
+    def label_of(name):
+        return next((label for key, label in short.items() if key in name), None)
+
     biggest = collections.defaultdict(int)
     rows_bench = {c: _pmc_rows(os.path.join(directory, sub), c)
                   for c, sub in (("FETCH_SIZE", "fetch_bench"), ("WRITE_SIZE", "write_bench"))}
     for name, _, grid in rows_bench["FETCH_SIZE"]:
-        biggest[name] = max(biggest[name], grid)
+        biggest[label_of(name)] = max(biggest[label_of(name)], grid)
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         for name, v, grid in rows_bench[counter]:
             for key, label in short.items():
                 if key in name:
-                    for lab in ([label, label + " (finest level)"] if key.startswith("k_mgt") and grid == biggest[name]
+                    for lab in ([label, label + " (finest level)"] if key.startswith("k_mgt") and grid == biggest[label]
                                 else [label]):
                         t = totals[lab][counter]
                         t[0] += v * scale[counter]

@@ -123,6 +123,19 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// keep ? v : 0 and keep ? v : alt formed with bit masks.  A select on a
+// loaded value that the compiler turns into a branch also sinks the load into
+// that branch, where it is then waited for as soon as it is issued; a mask
+// keeps the load unconditional, so it can be issued ahead with the others.
+__device__ __forceinline__ float2 keep_or_zero(float2 v, bool keep) {
+    const int m = -(int)keep;
+    return make_float2(__int_as_float(__float_as_int(v.x) & m), __int_as_float(__float_as_int(v.y) & m));
+}
+__device__ __forceinline__ float keep_or(float v, bool keep, float alt) {
+    const int m = -(int)keep;
+    return __int_as_float((__float_as_int(v) & m) | (__float_as_int(alt) & ~m));
+}
+
 // Load through a pointer known to address global memory.  A pointer read
 // from memory (an argument table) is a generic pointer to the compiler, and
 // generic loads count against the LDS counter as well, so waits for them also

@@ -183,12 +183,13 @@ static bool fused_levels() {
 // XCD-aware tile order in the fused kernels, per kernel and level
 // (HELM_XCD_ORDER bits: 1 pre-smoothing on the finest level, 2 pre-smoothing
 // on coarser levels, 4 post-smoothing on the finest level, 8 post-smoothing
-// on coarser levels; 15: all; 0: the hardware order everywhere).  The default
-// (4, right-hand-side groups fastest) measured 34.3 us per finest-level
-// post-smoothing launch at 576^2 x 11 against 35.5 us in the hardware order;
-// the other kernels measured no faster in it.
+// on coarser levels; 15: all; 0: the hardware order everywhere).  Default 15
+// with the right-hand-side groups fastest: at 576^2 x 11 the V-cycle takes
+// the same time in every order (126.9-128.6 us; the kernels are bound by
+// latency), and this one moves the fewest HBM bytes (finest pre-smoothing
+// 55 MB per launch against 105 MB in the hardware order).
 static int xcd_order() {
-    static const int t = (int)env_int("HELM_XCD_ORDER", 4, 0, 15);
+    static const int t = (int)env_int("HELM_XCD_ORDER", 15, 0, 15);
     return t;
 }
 // in the XCD-aware order the right-hand-side groups of a tile follow each

@@ -83,19 +83,24 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
                                                       const float2* __restrict__ b) {
     const int lane = threadIdx.x;
     const int x0 = blockIdx.x * 64, j = x0 + lane;
-    const int i0 = (blockIdx.y * 4 + threadIdx.y) * tz;
-    if (i0 >= nz) return;  // uniform over the wavefront
+    const int i0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + threadIdx.y) * tz);  // uniform per wavefront
+    if (i0 >= nz) return;
     const int i1 = min(nz, i0 + tz);
     const int r0 = blockIdx.z * RB;
     const bool colok = j < nx;
     const int jj = colok ? j : nx - 1;  // lanes past the grid mirror the last column (results dropped)
     const bool hw = jj > 0, he = jj < nx - 1;
     const bool cj = conj != 0;
+    // u and eta at (i, col), zero (eta: 1) outside the grid or past the last
+    // RHS; the load is unconditional (row and RHS clamped into range) so that
+    // no branch splits the loads a row step issues
     auto ld = [=](int r, int i, int col) -> float2 {
-        return (r < nrhs && i >= 0 && i < nz) ? u[(int64_t)r * n + (int64_t)i * nx + col] : make_float2(0.f, 0.f);
+        const float2 v = u[(int64_t)(r < nrhs ? r : nrhs - 1) * n + (int64_t)min(max(i, 0), nz - 1) * nx + col];
+        return keep_or_zero(v, r < nrhs && i >= 0 && i < nz);
     };
-    // eta with the operator's ghost value 1 outside the grid
-    auto le = [=](int i, int col) -> float { return (i >= 0 && i < nz) ? eta[(int64_t)i * nx + col] : 1.f; };
+    auto le = [=](int i, int col) -> float {
+        return keep_or(eta[(int64_t)min(max(i, 0), nz - 1) * nx + col], i >= 0 && i < nz, 1.f);
+    };
     // per-column stretch factors of this lane (constant down the run)
     const float2 sxj = sx[jj], iw = isxe[jj], ie = isxe[jj + 1];
     float2 un[RB], uc[RB], us[RB], up[RB];
@@ -107,20 +112,41 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
     }
     // eta window down the column: rows i-1, i, i+1
     float en = le(i0 - 1, jj), ec = le(i0, jj), es = le(i0 + 1, jj);
-    for (int i = i0; i < i1; ++i) {
-        float2 ew[RB], ee[RB];
+    // Row step i uses operands loaded one step earlier (u two rows ahead):
+    // row i + 2 of u and eta, and for row i + 1 the strip's halo columns
+    // (edge lanes), the edge lanes' outer eta and kappa.
+    const int xw = x0 > 0 ? x0 - 1 : 0, xe = x0 + 64 < nx ? x0 + 64 : nx - 1;
+    const int jwe = lane == 0 ? (hw ? jj - 1 : jj) : (he ? jj + 1 : jj);  // edge lanes' outer eta column
+    float2 pw[RB], pe[RB], pk;
+    float pup, peh;
+    auto pf_row = [&](int i) {  // operands of row step i (i may pass the run: loaded, unused)
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             up[q] = ld(r0 + q, i + 2, jj);
-            ew[q] = (lane == 0 && x0 > 0) ? ld(r0 + q, i, x0 - 1) : make_float2(0.f, 0.f);
-            ee[q] = (lane == 63 && x0 + 64 < nx) ? ld(r0 + q, i, x0 + 64) : make_float2(0.f, 0.f);
+            pw[q] = keep_or_zero(ld(r0 + q, i, xw), lane == 0 && x0 > 0);
+            pe[q] = keep_or_zero(ld(r0 + q, i, xe), lane == 63 && x0 + 64 < nx);
         }
-        const float eup = le(i + 2, jj);
-        const float2 kc = kap[(int64_t)i * nx + jj];
-        // west/east eta from the neighbouring lanes (edge lanes load theirs)
+        pup = le(i + 2, jj);
+        const int ic = min(i, nz - 1);
+        pk = kap[(int64_t)ic * nx + jj];
+        peh = eta[(int64_t)ic * nx + jwe];
+    };
+    pf_row(i0);
+    for (int i = i0; i < i1; ++i) {
+        float2 ew[RB], ee[RB], upc[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            ew[q] = pw[q];
+            ee[q] = pe[q];
+            upc[q] = up[q];
+        }
+        const float eup = pup, ehl = peh;
+        const float2 kc = pk;
+        pf_row(i + 1);
+        // west/east eta from the neighbouring lanes (edge lanes use theirs)
         float eww = __shfl_up(ec, 1, 64), eee = __shfl_down(ec, 1, 64);
-        if (lane == 0) eww = hw ? eta[(int64_t)i * nx + jj - 1] : 1.f;
-        if (lane == 63) eee = he ? eta[(int64_t)i * nx + jj + 1] : 1.f;
+        if (lane == 0) eww = hw ? ehl : 1.f;
+        if (lane == 63) eee = he ? ehl : 1.f;
         if (!hw) eww = 1.f;
         if (!he) eee = 1.f;
         // the node's stencil, as node_coef forms it
@@ -162,7 +188,7 @@ __global__ __launch_bounds__(256) void k_apply_stream(int nz, int nx, int64_t n,
             }
             un[q] = uc[q];
             uc[q] = us[q];
-            us[q] = up[q];
+            us[q] = upc[q];
         }
         en = ec;
         ec = es;

@@ -1050,12 +1050,18 @@ __device__ __forceinline__ float2 level_w_load(int64_t node, const float2* __res
 }
 
 // false when act (per RHS, nullable) marks every RHS of [r0, r1) inactive: the
-// workgroup then has nothing to do (a converged RHS's multigrid output is unused)
+// workgroup then has nothing to do (a converged RHS's multigrid output is
+// unused).  Lane k loads the flag of RHS r0 + k and a ballot combines them:
+// one memory round trip, where a loop that stops at the first active RHS
+// waits for each flag in turn.  (Testing the flags after the kernel's set-up
+// loads instead measured slower, V-cycle 131 against 127.5 us: the extra live
+// state split those loads into several rounds.)  Groups larger than a
+// wavefront are taken as active.
 __device__ __forceinline__ bool group_active(const int* __restrict__ act, int r0, int r1) {
     if (!act) return true;
-    bool any = false;
-    for (int r = r0; r < r1; ++r) any = any || act[r] != 0;
-    return any;
+    const int k = __lane_id(), n = r1 - r0;
+    const int v = act[r0 + (k < n ? k : 0)];
+    return n > 64 || __ballot(k < n && v != 0) != 0;
 }
 
 // ---- fused level kernels (pre = post = 2) ---------------------------------
@@ -1086,6 +1092,7 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
     __shared__ float2 sf[PR_1N];
     __shared__ float2 s1[PR_1N];  // u1, then the residual (17 x 65) once u2 is formed
     __shared__ float2 s2[PR_2N];
+    __shared__ float2 sw[PR_1N];  // w/dg on the f / u1 region (in LDS: as registers it spilled)
     const int tid = threadIdx.y * RR_TX + threadIdx.x;
     int bx, by, bz;
     xcd_tile(B, bx, by, bz, xcd != 0, xcd == 2);
@@ -1130,7 +1137,10 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
     };
     fetch(r0);
 #pragma unroll
-    for (int q = 0; q < PR_1PT; ++q) w1[q] = i1[q] >= 0 ? cc(w1[q], cj) : make_float2(0.f, 0.f);  // 0 on ghosts
+    for (int q = 0; q < PR_1PT; ++q) {
+        const int e = tid + 256 * q;
+        if (e < PR_1N) sw[e] = i1[q] >= 0 ? cc(w1[q], cj) : make_float2(0.f, 0.f);  // 0 on ghosts
+    }
     Coef<float2> c2[PR_2PT];
     float2 w2[PR_2PT];
 #pragma unroll
@@ -1147,7 +1157,7 @@ __device__ __forceinline__ void k_mgt_pre2_rr_body(const VBlock& B, int nzf, int
             const int e = tid + 256 * q;
             if (e < PR_1N) {
                 sf[e] = fq[q];
-                s1[e] = cmul(w1[q], fq[q]);  // u1 = (w/dg) f (0 on ghosts)
+                s1[e] = cmul(sw[e], fq[q]);  // u1 = (w/dg) f (0 on ghosts)
             }
         }
         if (PF && r + 1 < r1) fetch(r + 1);

@@ -33,6 +33,8 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <type_traits>
+#include <utility>
 
 #include "helm_internal.h"
 
@@ -314,25 +316,51 @@ __global__ __launch_bounds__(256) void k_cond8(int H, int W, int64_t n, int nrhs
     }
 }
 
-// ---- streaming FP32 form ------------------------------------------------
-// One wavefront per workgroup owns a strip of 64 columns and a run of tz
+// ---- streaming form ------------------------------------------------------
+// One wavefront per workgroup owns a strip of 64 columns and a run of tzr
 // output rows for RB right-hand sides, marching down the rows.  The z
-// differences use register windows (8 staged rows of the field, 8 z-face
-// fluxes); the x differences go through a per-wavefront LDS row (78 nodes,
-// 71 faces).  Iteration t stages row t+7, forms z face t+3 and, once t has
-// reached the run, writes output row t.  Forward: the staged field is M u;
-// adjoint: M y, scaled per column (x) and per row (z) by conj(cA), conj(cB)
-// as the differences are formed.
-template <typename R, typename C, int RB, int MODE, bool ADJ>
-__global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs, int tzr,
+// differences use two register rings: the field rows t .. t+7 and the z-face
+// fluxes t-4 .. t+3 of output row t, each row in slot (row - i0) mod 8.  The
+// row loop is unrolled eight times, so every slot is a compile-time constant
+// and nothing moves between registers from row to row.  The x differences go
+// through LDS once per block of NB output rows: the block's field rows (all in
+// the ring at its first row) and their 7-column halos are staged together and
+// its NB x 71 x-face fluxes formed in NB + 1 passes of the wavefront (NB passes
+// of 64 faces, one of the NB x 7 halo faces) where one row at a time takes two.
+// Field rows, z-face coefficients, mass and b are loaded PD rows ahead, a
+// block's halo columns and x-face coefficients one block ahead.  Forward: the
+// staged field is M u; adjoint: M y, scaled per column (x) and per row (z) by
+// conj(cA), conj(cB) as the differences are formed.
+// a row index at most a few periods outside [0, m) into it, with no division
+// (the streaming kernel wraps several rows per step)
+__device__ __forceinline__ int rwrap(int v, int m) {
+    while (v < 0) v += m;
+    while (v >= m) v -= m;
+    return v;
+}
+template <int K>
+using CK = std::integral_constant<int, K>;
+template <typename F, int... Ks>
+__device__ __forceinline__ void each_phase(std::integer_sequence<int, Ks...>, F&& f) {
+    (f(CK<Ks>{}), ...);
+}
+
+template <typename R, typename C, int RB, int NB, int MODE, bool ADJ>
+__global__ __launch_bounds__(64, (RB == 1 && sizeof(R) == 4) ? 4 : 1) void k_cond8s(int H, int W, int64_t n, int nrhs, int tzr,
                                                const R* __restrict__ ax, const R* __restrict__ az,
                                                const C* __restrict__ mass, const C* __restrict__ tx,
                                                const C* __restrict__ tz, int bz0, int bx0, int bh, int bw,
                                                R c0, R c1, R c2, R c3,
                                                const C* __restrict__ u, C* __restrict__ out,
                                                const C* __restrict__ b) {
-    constexpr int NS = 78, NF = 71;
-    __shared__ C sa[RB][NS], sb[RB][NS], sf[RB][NF], sg[RB][NF];
+    static_assert(NB == 4 || NB == 8, "blocks of 4 or 8 rows");
+    constexpr int NS = 78, NF = 71;  // staged nodes x0-7 .. x0+70, x faces x0-4 .. x0+66 (+1/2) of a row
+    constexpr int PD = 2;            // rows loaded ahead
+    constexpr int NHE = (14 * NB + 63) / 64;  // halo nodes per lane and block
+    __shared__ C st[RB][NB * NS];
+    __shared__ C sbt[ADJ ? RB : 1][ADJ ? NB * NS : 1];
+    __shared__ C sf[RB][NB * NF];
+    __shared__ C sgx[NF];  // g - 1 of the strip's x faces (forward PML term)
     const int l = threadIdx.x;
     const int x0 = blockIdx.x * 64, j = x0 + l;
     const int i0 = blockIdx.y * tzr;
@@ -344,244 +372,321 @@ __global__ __launch_bounds__(64) void k_cond8s(int H, int W, int64_t n, int nrhs
     const int jc = cwrap(j, W);
     const bool xin = jc >= bx0 && jc < bx0 + bw;
     auto zin = [&](int r) { return r >= bz0 && r < bz0 + bh; };
-    // per-lane column data: own node column, halo node columns (lanes < 7), faces l-4 and (lanes < 7) 60+l
-    const int ch0 = cwrap(x0 - 7 + l, W), ch1 = cwrap(x0 + 64 + l, W);
-    const bool hin0 = ch0 >= bx0 && ch0 < bx0 + bw, hin1 = ch1 >= bx0 && ch1 < bx0 + bw;
-    const int fa = cwrap(x0 - 4 + l, W), fb = cwrap(x0 + 60 + l, W);
-    const C svA = tx[fa], gA = tx[W + fa];
-    const C svB = l < 7 ? tx[fb] : zero, gB = l < 7 ? tx[W + fb] : zero;
-    const C cAj = tx[2 * W + jc], cBj = tx[3 * W + jc];
-    C cA0 = zero, cB0 = zero, cA1 = zero, cB1 = zero;
-    if (ADJ && l < 7) {
-        cA0 = tx[2 * W + ch0];
-        cB0 = tx[3 * W + ch0];
-        cA1 = tx[2 * W + ch1];
-        cB1 = tx[3 * W + ch1];
-    }
-    const bool pmlx = __any((gA.x != R(0) || gA.y != R(0)) || (gB.x != R(0) || gB.y != R(0)));
+    // row `row` of RHS r0+q (clamped into range) from base pointer f; rows are
+    // uniform, so a load is a scalar base plus the lane's column offset
+    auto rowp = [&](const C* f, int q, int row) -> const C* {
+        return f + (int64_t)(r0 + q < nrhs ? r0 + q : nrhs - 1) * n + (int64_t)row * W;
+    };
 
-    C U[RB][8], G[RB][8];
-    // u at (row, col) of RHS r0 + q, zero outside the box or past the last
-    // RHS; the load itself is unconditional (row and col are in range, the
-    // RHS is clamped), so no branch splits the loads a step issues
-    auto ldu = [=](int q, int row, int col, bool colin) -> C {
-        const int r = r0 + q;
-        const C v = u[(int64_t)(r < nrhs ? r : nrhs - 1) * n + (int64_t)row * W + col];
-        return (r < nrhs && colin && row >= bz0 && row < bz0 + bh) ? v : C{R(0), R(0)};
-    };
-    // initial window: rows i0-8 .. i0-1
+    // x faces: lane l forms face l of each block row (face column x0-4+l) and,
+    // in the halo pass (lanes < 7 NB), face 64 + l%7 of block row l/7
+    const int fa = cwrap(x0 - 4 + l, W);
+    const int hr = l < 7 * NB ? l / 7 : 0, hf = 64 + (l < 7 * NB ? l % 7 : 0);
+    const int fh = cwrap(x0 - 4 + hf, W);
+    const C svA = tx[fa], gA = tx[W + fa], svH = tx[fh], gH = tx[W + fh];
+    const C cAj = tx[2 * W + jc], cBj = tx[3 * W + jc];
+    sgx[l] = gA;
+    if (l < 7) sgx[64 + l] = gH;
+    const bool pmlx = __any((gA.x != R(0) || gA.y != R(0)) || (l < 7 * NB && (gH.x != R(0) || gH.y != R(0))));
+    // halo nodes: entry e = l + 64 h of 14 NB (block row e/14; columns x0-7.. x0-1, x0+64 .. x0+70)
+    int hrow[NHE], hcol[NHE], hpos[NHE];
+    bool hok[NHE], hin[NHE];
+    C hA[NHE], hB[NHE];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int row = cwrap(i0 - 8 + k, H);
-#pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            U[q][k] = ldu(q, row, jc, xin);
-            G[q][k] = zero;
-        }
+    for (int h = 0; h < NHE; ++h) {
+        const int e = l + 64 * h, ec = e < 14 * NB ? e : 0;
+        const int k = ec % 14;
+        hok[h] = e < 14 * NB;
+        hrow[h] = ec / 14;
+        hcol[h] = cwrap(k < 7 ? x0 - 7 + k : x0 + 57 + k, W);
+        hpos[h] = hrow[h] * NS + (k < 7 ? k : k + 64);
+        hin[h] = hcol[h] >= bx0 && hcol[h] < bx0 + bw;
+        hA[h] = ADJ ? tx[2 * W + hcol[h]] : zero;
+        hB[h] = ADJ ? tx[3 * W + hcol[h]] : zero;
     }
-    // Software pipeline: each step issues the loads of the next step (its
-    // staged row and face coefficient; the next output row's halo columns,
-    // x-face coefficients and mass) and works on what the step before loaded,
-    // so a step does not wait for memory it has just asked for.
-    C P[RB], ph0[RB], ph1[RB], pu[RB], pb[RB], pmk = zero;
-    R pa = R(0), pax = R(0), pbx = R(0);
-    auto pf_face = [&](int t) {  // step t: row t + 7, az at face t + 3
+
+    // block loads (rows tb .. tb+NB-1): x-face coefficients and halo nodes
+    R AXV[NB], AXH;
+    C HV[RB][NHE];
+    int hoff[NHE];  // halo node h of a block: row hrow[h] and column hcol[h] as an offset from the block's first row
 #pragma unroll
-        for (int q = 0; q < RB; ++q) P[q] = ldu(q, cwrap(t + 7, H), jc, xin);
-        pa = az[(int64_t)cwrap(t + 3, H) * W + jc];
-    };
-    auto pf_out = [&](int i) {  // output row i (wrapped; past the run: unused): halo columns, ax, mass
-        const int ic = cwrap(i, H);
+    for (int h = 0; h < NHE; ++h) hoff[h] = hrow[h] * W + hcol[h];
+    const int axh_off = hr * W + fh;
+    auto pf_block = [&](auto WRAPc, int tb) {
+        if constexpr (decltype(WRAPc)::value) {
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {  // lanes 7.. load column ch0 / ch1 too and discard it
-            ph0[q] = ldu(q, ic, ch0, hin0 && l < 7);
-            ph1[q] = ldu(q, ic, ch1, hin1 && l < 7);
+            for (int p = 0; p < NB; ++p) AXV[p] = (ax + (int64_t)rwrap(tb + p, H) * W)[fa];
+            AXH = ax[(int64_t)rwrap(tb + hr, H) * W + fh];
+#pragma unroll
+            for (int h = 0; h < NHE; ++h)
+#pragma unroll
+                for (int q = 0; q < RB; ++q)
+                    HV[q][h] = rowp(u, q, 0)[(int64_t)rwrap(tb + hrow[h], H) * W + hcol[h]];
+        } else {
+            const R* axb = ax + (int64_t)tb * W;
+#pragma unroll
+            for (int p = 0; p < NB; ++p) AXV[p] = axb[(int64_t)p * W + fa];
+            AXH = axb[axh_off];
+#pragma unroll
+            for (int h = 0; h < NHE; ++h)
+#pragma unroll
+                for (int q = 0; q < RB; ++q) HV[q][h] = rowp(u, q, tb)[hoff[h]];
         }
-        pax = ax[(int64_t)ic * W + fa];
-        const R axf = ax[(int64_t)ic * W + fb];
-        pbx = l < 7 ? axf : R(0);
-        const C m = mass[(int64_t)ic * W + jc];
-        pmk = (jok && xin && zin(ic)) ? m : zero;
-#pragma unroll
-        for (int q = 0; q < RB; ++q) {  // u (the identity rows outside the medium) and b (MODE 1) at the output
-            const int64_t o = (int64_t)(r0 + q < nrhs ? r0 + q : nrhs - 1) * n + (int64_t)ic * W + jc;
-            pu[q] = u[o];
-            pb[q] = MODE == 1 ? b[o] : zero;
-        }
     };
-    pf_face(i0 - 7);
-    pf_out(i0);
-    for (int t = i0 - 7; t < i1; ++t) {
-        const int fz = cwrap(t + 3, H);  // z face formed now (face fz + 1/2)
-        C nrow[RB];
-#pragma unroll
-        for (int q = 0; q < RB; ++q) nrow[q] = P[q];
-        const R a = pa;
-        pf_face(t + 1);  // past the run: loads of valid rows, never used
+    // value of a loaded node in the staged field M u (zero outside the box and past the last RHS)
+    auto masked = [&](int q, C v, int row, bool colin) -> C {
+        return (r0 + q < nrhs && colin && zin(row)) ? v : zero;
+    };
+
+    C U[RB][8], G[RB][8];  // rings: field row r in slot (r - i0) & 7, z face f (+1/2) in slot (f - i0) & 7
+    int gring = 0;         // bit s: the z face in ring slot s has g != 1 (forward PML term)
+    C PF[RB][PD], PM[PD], PB[RB][PD];
+    R PA[PD];
+    // per-lane load and store addresses of step t: field row t+7+PD, az row
+    // t+3+PD, mass and b of row t+PD, output row t (column jc)
+    const C* pU[RB];
+    const C* pB[RB];
+    C* pO[RB];
+    const R* pAz;
+    const C* pM;
+    auto setp = [&](int t) {
+        const int ru = rwrap(t + 7 + PD, H), rm = rwrap(t + PD, H), ro = rwrap(t, H);
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
-#pragma unroll
-            for (int k = 0; k < 7; ++k) U[q][k] = U[q][k + 1];
-            U[q][7] = nrow[q];
+            pU[q] = rowp(u, q, ru) + jc;
+            pB[q] = MODE == 1 ? rowp(b, q, rm) + jc : nullptr;
+            pO[q] = out + (rowp(u, q, ro) - u) + jc;
         }
-        // z face t+3: rows t .. t+7 are U[.][0..7]
+        pAz = az + (int64_t)rwrap(t + 3 + PD, H) * W + jc;
+        pM = mass + (int64_t)rm * W + jc;
+    };
+
+    // one row step: t = tb + K (K = (t - i0) & 7); OUT: t is an output row of
+    // the run; WRAP: rows are wrapped and addresses formed per step (runs at
+    // the box's ends), else they advance one row per step
+    auto step = [&](auto Kc, auto OUTc, auto WRAPc, int t, int tb) {
+        constexpr int K = decltype(Kc)::value;
+        constexpr bool OUT = decltype(OUTc)::value;
+        constexpr bool WRAP = decltype(WRAPc)::value;
+        constexpr int ks = K % PD;
+        auto wr = [&](int v) { return WRAP ? rwrap(v, H) : v; };
+        // field row t+7 into the ring; z-face coefficient of face t+3, mass and b of row t
         {
-            const C sv = tz[fz], gm = tz[H + fz];
+            const int row = wr(t + 7);
+#pragma unroll
+            for (int q = 0; q < RB; ++q) U[q][(K + 7) & 7] = masked(q, PF[q][ks], row, xin);
+        }
+        const R a = PA[ks];
+        const C mk = PM[ks];
+        C bo[RB];
+#pragma unroll
+        for (int q = 0; q < RB; ++q) bo[q] = PB[q][ks];
+        // loads PD rows ahead
+        if constexpr (WRAP) setp(t);
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            PF[q][ks] = *pU[q];
+            if constexpr (MODE == 1) PB[q][ks] = *pB[q];
+            else PB[q][ks] = zero;
+        }
+        PA[ks] = *pAz;
+        PM[ks] = *pM;
+        // uniform per-row coefficients: face t+3 and row t (in runs without
+        // wrap at constant offsets from the group's first row)
+        const C* tzf = tz + tb;          // tzf[k] = sv of face tb + k
+        const C* tzg = tz + H + tb;      // g - 1
+        const C* tza = tz + 2 * H + tb;  // cA of row tb + k
+        const C* tzb = tz + 3 * H + tb;  // cB
+        // z face t+3 from rows t .. t+7 (slots K .. K+7)
+        {
+            const int fz = wr(t + 3);
+            const C sv = WRAP ? tz[fz] : tzf[K + 3], gm = WRAP ? tz[H + fz] : tzg[K + 3];
             C cz[8], czb[8];
             bool pz = false;
-            if (ADJ) {
+            if constexpr (!ADJ) {
+                constexpr int bit = 1 << ((K + 3) & 7);
+                gring = (gring & ~bit) | ((gm.x != R(0) || gm.y != R(0)) ? bit : 0);
+            } else {
                 pz = gm.x != R(0) || gm.y != R(0);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    const int row = cwrap(t + k, H);
-                    cz[k] = tz[2 * H + row];
-                    czb[k] = pz ? tz[3 * H + row] : zero;
+                    const int row = wr(t + k);
+                    cz[k] = WRAP ? tz[2 * H + row] : tza[K + k];
+                    czb[k] = pz ? (WRAP ? tz[3 * H + row] : tzb[K + k]) : zero;
                 }
             }
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
+#define UR(k) U[q][(K + (k)) & 7]
                 C d;
-                if (!ADJ) {
-                    d = c8diff<C, R>(c0, c1, c2, c3, U[q][4], U[q][3], U[q][5], U[q][2], U[q][6], U[q][1],
-                                              U[q][7], U[q][0]);
+                if constexpr (!ADJ) {
+                    d = c8diff<C, R>(c0, c1, c2, c3, UR(4), UR(3), UR(5), UR(2), UR(6), UR(1), UR(7), UR(0));
                     d = cscale(cm(sv, d), a);
                 } else {
-                    d = c8diff<C, R>(c0, c1, c2, c3, cjm(cz[4], U[q][4]), cjm(cz[3], U[q][3]),
-                                              cjm(cz[5], U[q][5]), cjm(cz[2], U[q][2]), cjm(cz[6], U[q][6]),
-                                              cjm(cz[1], U[q][1]), cjm(cz[7], U[q][7]), cjm(cz[0], U[q][0]));
+                    d = c8diff<C, R>(c0, c1, c2, c3, cjm(cz[4], UR(4)), cjm(cz[3], UR(3)), cjm(cz[5], UR(5)),
+                                     cjm(cz[2], UR(2)), cjm(cz[6], UR(6)), cjm(cz[1], UR(1)), cjm(cz[7], UR(7)),
+                                     cjm(cz[0], UR(0)));
                     if (pz) {
-                        const C d2 = c8diff<C, R>(
-                            c0, c1, c2, c3, cjm(czb[4], U[q][4]), cjm(czb[3], U[q][3]), cjm(czb[5], U[q][5]),
-                            cjm(czb[2], U[q][2]), cjm(czb[6], U[q][6]), cjm(czb[1], U[q][1]), cjm(czb[7], U[q][7]),
-                            cjm(czb[0], U[q][0]));
+                        const C d2 = c8diff<C, R>(c0, c1, c2, c3, cjm(czb[4], UR(4)), cjm(czb[3], UR(3)),
+                                                  cjm(czb[5], UR(5)), cjm(czb[2], UR(2)), cjm(czb[6], UR(6)),
+                                                  cjm(czb[1], UR(1)), cjm(czb[7], UR(7)), cjm(czb[0], UR(0)));
                         d = cadd(d, cjm(gm, d2));
                     }
                     d = cscale(cjm(sv, d), a);  // conj(az sv), az real
                 }
-#pragma unroll
-                for (int k = 0; k < 7; ++k) G[q][k] = G[q][k + 1];
-                G[q][7] = d;  // G[.][k] = face t-4+k
+#undef UR
+                G[q][(K + 3) & 7] = d;
             }
         }
-        if (t < i0) continue;
-        // ---- output row t ----
-        const int i = t;
-        const bool rin = zin(i);
-        C hh0[RB], hh1[RB];
+        if constexpr (OUT) {
+            if constexpr (K % NB == 0) {
+                // x fluxes of block rows t .. t+NB-1 (ring slots K .. K+NB-1)
+                lds_barrier();  // the previous block's readers are done
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            hh0[q] = ph0[q];
-            hh1[q] = ph1[q];
-        }
-        const R axa = pax, axb = pbx;
-        C mkc = pmk, uo[RB], bo[RB];
+                for (int q = 0; q < RB; ++q) {
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            uo[q] = pu[q];
-            bo[q] = pb[q];
-        }
-        pf_out(t + 1);
-        // x: stage row i (and the halo columns) in LDS
-        lds_barrier();  // previous row's LDS readers are done
+                    for (int p = 0; p < NB; ++p) {
+                        const C v = U[q][(K + p) & 7];
+                        if constexpr (!ADJ) {
+                            st[q][p * NS + 7 + l] = v;
+                        } else {
+                            st[q][p * NS + 7 + l] = cjm(cAj, v);
+                            if (pmlx) sbt[q][p * NS + 7 + l] = cjm(cBj, v);
+                        }
+                    }
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            const C v = U[q][0];  // U[q][k] holds row t + k
-            if (!ADJ) {
-                sa[q][l + 7] = v;
-            } else {
-                sa[q][l + 7] = cjm(cAj, v);
-                if (pmlx) sb[q][l + 7] = cjm(cBj, v);
+                    for (int h = 0; h < NHE; ++h) {
+                        if (!hok[h]) continue;
+                        const C v = masked(q, HV[q][h], wr(t + hrow[h]), hin[h]);
+                        if constexpr (!ADJ) {
+                            st[q][hpos[h]] = v;
+                        } else {
+                            st[q][hpos[h]] = cjm(hA[h], v);
+                            if (pmlx) sbt[q][hpos[h]] = cjm(hB[h], v);
+                        }
+                    }
+                }
+                lds_barrier();
+                auto xflux = [&](int q, int base, C sv, C g, R axf) -> C {
+                    const C* s = st[q] + base;  // s[m] = node of face - m .. face + 1 + m
+                    C d = c8diff<C, R>(c0, c1, c2, c3, s[1], s[0], s[2], s[-1], s[3], s[-2], s[4], s[-3]);
+                    if constexpr (ADJ) {
+                        if (pmlx) {
+                            const C* sB = sbt[q] + base;
+                            const C d2 = c8diff<C, R>(c0, c1, c2, c3, sB[1], sB[0], sB[2], sB[-1], sB[3], sB[-2],
+                                                      sB[4], sB[-3]);
+                            d = cadd(d, cjm(g, d2));
+                        }
+                    }
+                    return ADJ ? cscale(cjm(sv, d), axf) : cscale(cm(sv, d), axf);
+                };
+#pragma unroll
+                for (int q = 0; q < RB; ++q) {
+#pragma unroll
+                    for (int p = 0; p < NB; ++p) sf[q][p * NF + l] = xflux(q, p * NS + l + 3, svA, gA, AXV[p]);
+                    if (l < 7 * NB) sf[q][hr * NF + hf] = xflux(q, hr * NS + hf + 3, svH, gH, AXH);
+                }
+                pf_block(WRAPc, t + NB);  // the next block's coefficients and halo nodes
+                lds_barrier();
             }
-            if (l < 7) {
-                const C h0 = hh0[q], h1 = hh1[q];
-                if (!ADJ) {
-                    sa[q][l] = h0;
-                    sa[q][71 + l] = h1;
+            // output row t
+            const int ic = wr(t);
+            const bool inside = jok && xin && zin(ic);
+            const C cAz = WRAP ? tz[2 * H + ic] : tza[K];
+            const bool pzo = !ADJ && gring != 0;  // some face t-4 .. t+3 has g != 1
+            C mkc = mk;
+            if (ADJ) mkc.y = -mkc.y;
+            constexpr int pb = K % NB;
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int r = r0 + q;
+                if (t >= i1 || !jok || r >= nrhs) continue;
+                C acc;
+                if (inside) {
+#define GR(k) G[q][(K + 4 + (k)) & 7]
+                    const C* fx = sf[q] + pb * NF + l + 4;  // fx[m] = face x0 + l + m (+1/2)
+                    const C dx = c8diff<C, R>(c0, c1, c2, c3, fx[0], fx[-1], fx[1], fx[-2], fx[2], fx[-3], fx[3],
+                                              fx[-4]);
+                    const C dz = c8diff<C, R>(c0, c1, c2, c3, GR(4), GR(3), GR(5), GR(2), GR(6), GR(1), GR(7), GR(0));
+                    if constexpr (!ADJ) {
+                        acc = cm(cAj, dx);
+                        cfma(acc, cAz, dz);
+                        if (pmlx) {
+                            const C* gg = sgx + l + 4;  // gg[m] = g - 1 at face x0 + l + m
+                            const C d2 = c8diff<C, R>(c0, c1, c2, c3, cm(gg[0], fx[0]), cm(gg[-1], fx[-1]),
+                                                      cm(gg[1], fx[1]), cm(gg[-2], fx[-2]), cm(gg[2], fx[2]),
+                                                      cm(gg[-3], fx[-3]), cm(gg[3], fx[3]), cm(gg[-4], fx[-4]));
+                            cfma(acc, cBj, d2);
+                        }
+                        if (pzo) {
+                            C gz[8];  // g - 1 of faces t-4 .. t+3
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) gz[k] = WRAP ? tz[H + rwrap(t - 4 + k, H)] : tzg[K - 4 + k];
+                            const C d2 = c8diff<C, R>(c0, c1, c2, c3, cm(gz[4], GR(4)), cm(gz[3], GR(3)),
+                                                      cm(gz[5], GR(5)), cm(gz[2], GR(2)), cm(gz[6], GR(6)),
+                                                      cm(gz[1], GR(1)), cm(gz[7], GR(7)), cm(gz[0], GR(0)));
+                            cfma(acc, WRAP ? tz[3 * H + ic] : tzb[K], d2);
+                        }
+                    } else {
+                        acc = cadd(dx, dz);
+                    }
+#undef GR
+                    cfma(acc, mkc, U[q][K]);
                 } else {
-                    sa[q][l] = cjm(cA0, h0);
-                    sa[q][71 + l] = cjm(cA1, h1);
-                    if (pmlx) {
-                        sb[q][l] = cjm(cB0, h0);
-                        sb[q][71 + l] = cjm(cB1, h1);
-                    }
+                    acc = u[pO[q] - out];  // identity rows outside the medium
                 }
+                if (MODE == 1) acc = csub(bo[q], acc);
+                *pO[q] = acc;
             }
         }
-        lds_barrier();
+        if constexpr (!WRAP) {
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-#pragma unroll
-            for (int pass = 0; pass < 2; ++pass) {
-                if (pass == 1 && l >= 7) continue;
-                const int f = pass ? 64 + l : l;  // face index in the strip window (0 .. 70 = faces -4 .. 66)
-                const C* s = sa[q] + f + 3;  // s[m] = node f - 4 + m  (node index - (x0-7) = f + 3 + m)
-                // D+ at face f-4: nodes f-4+m+1 - (f-4-m) -> s[m+1] - s[-m] with s at node f-4
-                C d = c8diff<C, R>(c0, c1, c2, c3, s[1], s[0], s[2], s[-1], s[3], s[-2], s[4], s[-3]);
-                const C sv = pass ? svB : svA;
-                const R a = pass ? axb : axa;
-                if (ADJ && pmlx) {
-                    const C* sB = sb[q] + f + 3;
-                    const C d2 = c8diff<C, R>(c0, c1, c2, c3, sB[1], sB[0], sB[2], sB[-1], sB[3], sB[-2],
-                                                            sB[4], sB[-3]);
-                    d = cadd(d, cjm(pass ? gB : gA, d2));
-                }
-                const C F = ADJ ? cscale(cjm(sv, d), a) : cscale(cm(sv, d), a);
-                sf[q][f] = F;
-                if (!ADJ && pmlx) sg[q][f] = cm(pass ? gB : gA, F);
+            for (int q = 0; q < RB; ++q) {
+                pU[q] += W;
+                if constexpr (MODE == 1) pB[q] += W;
+                pO[q] += W;
             }
+            pAz += W;
+            pM += W;
         }
-        lds_barrier();
-        // z PML terms of the forward output: g - 1 of faces i-4 .. i+3
-        C gz[8];
-        bool pzo = false;
-        if (!ADJ) {
+    };
+    // a run: rows i0-7 .. i0-1 (phases 1 .. 7) form z faces i0-4 .. i0+2,
+    // then the output rows in groups of eight (past the run's end: computed,
+    // not stored)
+    auto run = [&](auto WRAPc) {
+        pf_block(WRAPc, i0);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) gz[k] = tz[H + cwrap(i - 4 + k, H)];
+        for (int k = 1; k < 8; ++k) {  // rows i0-7 .. i0-1
+            const int row = rwrap(i0 - 8 + k, H);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) pzo = pzo | (gz[k].x != R(0)) | (gz[k].y != R(0));
+            for (int q = 0; q < RB; ++q) U[q][k] = masked(q, rowp(u, q, row)[jc], row, xin);
         }
-        const C cAz = tz[2 * H + cwrap(i, H)], cBz = tz[3 * H + cwrap(i, H)];
-        const bool inside = jok && xin && rin;
-        if (ADJ) mkc.y = -mkc.y;
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            const int r = r0 + q;
-            if (!jok || r >= nrhs) continue;
-            const int64_t o = (int64_t)r * n + (int64_t)i * W + j;
-            C acc = zero;
-            if (inside) {
-                const C* fx = sf[q] + l + 4;  // fx[m] = face l + m
-                const C dx = c8diff<C, R>(c0, c1, c2, c3, fx[0], fx[-1], fx[1], fx[-2], fx[2], fx[-3],
-                                                        fx[3], fx[-4]);
-                const C dz = c8diff<C, R>(c0, c1, c2, c3, G[q][4], G[q][3], G[q][5], G[q][2], G[q][6],
-                                                        G[q][1], G[q][7], G[q][0]);
-                if (!ADJ) {
-                    acc = cm(cAj, dx);
-                    cfma(acc, cAz, dz);
-                    if (pmlx) {
-                        const C* gx = sg[q] + l + 4;
-                        const C d2 = c8diff<C, R>(c0, c1, c2, c3, gx[0], gx[-1], gx[1], gx[-2], gx[2],
-                                                                gx[-3], gx[3], gx[-4]);
-                        cfma(acc, cBj, d2);
-                    }
-                    if (pzo) {
-                        const C d2 = c8diff<C, R>(
-                            c0, c1, c2, c3, cm(gz[4], G[q][4]), cm(gz[3], G[q][3]), cm(gz[5], G[q][5]),
-                            cm(gz[2], G[q][2]), cm(gz[6], G[q][6]), cm(gz[1], G[q][1]), cm(gz[7], G[q][7]),
-                            cm(gz[0], G[q][0]));
-                        cfma(acc, cBz, d2);
-                    }
-                } else {
-                    acc = cadd(dx, dz);
-                }
-                cfma(acc, mkc, U[q][0]);
-            } else {
-                acc = uo[q];
-            }
-            if (MODE == 1) acc = csub(bo[q], acc);
-            out[o] = acc;
+        for (int d = 0; d < PD; ++d) {
+            // row i0 + d (consumed at step i0+d-7, slot (d + 1) % PD); z face i0-4+d (step i0-7+d, slot (d + 1) % PD)
+            const int s = (d + 1) % PD;
+            const int row = rwrap(i0 + d, H);
+#pragma unroll
+            for (int q = 0; q < RB; ++q) PF[q][s] = rowp(u, q, row)[jc];
+            PA[s] = (az + (int64_t)rwrap(i0 - 4 + d, H) * W)[jc];
+            PM[s] = zero;
+#pragma unroll
+            for (int q = 0; q < RB; ++q) PB[q][s] = zero;
         }
-    }
+        if constexpr (!decltype(WRAPc)::value) setp(i0 - 7);
+        each_phase(std::integer_sequence<int, 1, 2, 3, 4, 5, 6, 7>{}, [&](auto Kc) {
+            step(Kc, std::false_type{}, WRAPc, i0 - 8 + decltype(Kc)::value, i0 - 8);
+        });
+        for (int tb = i0; tb < i1; tb += 8)
+            each_phase(std::integer_sequence<int, 0, 1, 2, 3, 4, 5, 6, 7>{}, [&](auto Kc) {
+                step(Kc, std::true_type{}, WRAPc, tb + decltype(Kc)::value, tb);
+            });
+    };
+    // every row the run touches (i0-8 .. its last group's row + 15 and the
+    // block loads beyond) inside [0, H): no wrap
+    const int last = i0 + ((i1 - i0 + 7) & ~7) - 8;
+    if (i0 >= 8 && last + 8 + NB + NB + PD + 8 <= H) run(std::false_type{});
+    else run(std::true_type{});
 }
 
 static int cond_stream_tz() {
@@ -601,9 +706,20 @@ static int cond_stream_rb() {
     return t;
 }
 
-template <int RB>
-static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
-                          hipStream_t s) {
+// rows per x-flux block of the streaming kernel (HELM_COND_NB: 4, the
+// default, or 8)
+static int cond_stream_nb(int) {
+    static const int t = [] {
+        const char* v = std::getenv("HELM_COND_NB");
+        const int x = v ? std::atoi(v) : 0;
+        return x == 8 ? 8 : 4;
+    }();
+    return t;
+}
+
+template <int RB, int NB>
+static void launch_cond8s_nb(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
+                             hipStream_t s) {
     const int tzr = L.st_tz ? L.st_tz : cond_stream_tz();
     const dim3 block(64), grid((L.nx + 63) / 64, (L.nz + tzr - 1) / tzr, (nrhs + RB - 1) / RB);
     const float ih = (float)(1.0 / L.h_m);
@@ -611,7 +727,7 @@ static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs
     const float2* tx = L.ctab32;
     const float2* tz = L.ctab32 + 4 * L.nx;
 #define HELM_C8S(M, A, BP)                                                                                       \
-    hipLaunchKernelGGL((k_cond8s<float, float2, RB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr,      \
+    hipLaunchKernelGGL((k_cond8s<float, float2, RB, NB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr,  \
                        L.ax32, L.az32, L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
                        c3, u, out, BP)
     if (b) {
@@ -622,6 +738,12 @@ static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs
         else HELM_C8S(0, false, (const float2*)nullptr);
     }
 #undef HELM_C8S
+}
+template <int RB>
+static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
+                          hipStream_t s) {
+    if (cond_stream_nb(RB) == 4) launch_cond8s_nb<RB, 4>(L, u, out, nrhs, adjoint, b, s);
+    else launch_cond8s_nb<RB, 8>(L, u, out, nrhs, adjoint, b, s);
 }
 
 // FP64 (true residuals of the refinement): the streaming kernel with one RHS
@@ -642,7 +764,7 @@ static void launch_cond8s64(const Level& L, const double2* u, double2* out, int 
     const double2* tx = L.ctab64;
     const double2* tz = L.ctab64 + 4 * L.nx;
 #define HELM_C8S64(M, A, BP)                                                                                     \
-    hipLaunchKernelGGL((k_cond8s<double, double2, 1, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, TZR,     \
+    hipLaunchKernelGGL((k_cond8s<double, double2, 1, 8, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, TZR,  \
                        L.ax64, L.az64, L.cmass64, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
                        c3, u, out, BP)
     if (b) {

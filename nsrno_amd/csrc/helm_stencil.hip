@@ -734,12 +734,15 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     double best = 1e30;
     int btz = cond ? 64 : 4, brb = cond ? 2 : 4;
     if (hipMemsetAsync(pool, 0, 2 * nbuf * bytes, s) != hipSuccess) st = HELM_ERR_HIP;
-    // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 15-row warm-up);
+    // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 7-row warm-up
+    // and march in groups of 8 rows; the intermediate row counts give grids that fill the device's
+    // wavefront slots more evenly);
     // the flux stencil's streaming form is no longer a candidate (its block form measured faster on
     // every grid from 576^2 to 2048^2) but stays selectable with helm_set_stencil_shape
-    const int ntz = cond ? 5 : 0;
-    const int tzs_flux[5] = {4, 8, 16, 16, 16}, tzs_cond[5] = {8, 16, 32, 64, 128}, rbs[4] = {1, 2, 4, 8};
-    const int nrb = cond ? 3 : 4;  // the conditioned kernel groups at most 4 RHS
+    const int ntz = cond ? 8 : 0;
+    const int tzs_flux[8] = {4, 8, 16, 16, 16, 16, 16, 16}, tzs_cond[8] = {8, 16, 24, 32, 40, 48, 64, 128},
+              rbs[4] = {1, 2, 4, 8};
+    const int nrb = cond ? 2 : 4;  // the conditioned kernel: 1 or 2 RHS (4 fits only one wavefront per SIMD)
     const int* tzs = cond ? tzs_cond : tzs_flux;
     const double k0sq = p->k0 * p->k0;
     auto run = [&]() {

@@ -693,7 +693,7 @@ static int cond_stream_tz() {
     static const int t = [] {
         const char* v = std::getenv("HELM_COND_TZ");
         const int x = v ? std::atoi(v) : 0;
-        return x >= 8 && x <= 4096 ? x : 64;
+        return x >= 8 && x <= 4096 ? x : 32;
     }();
     return t;
 }
@@ -701,7 +701,7 @@ static int cond_stream_rb() {
     static const int t = [] {
         const char* v = std::getenv("HELM_COND_RB");
         const int x = v ? std::atoi(v) : 0;
-        return (x == 1 || x == 2 || x == 4) ? x : 2;
+        return (x == 1 || x == 2 || x == 4) ? x : 1;
     }();
     return t;
 }

@@ -732,7 +732,7 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     (void)hipEventCreate(&e1);
     int32_t st = HELM_OK;
     double best = 1e30;
-    int btz = cond ? 64 : 4, brb = cond ? 2 : 4;
+    int btz = cond ? 32 : 4, brb = cond ? 1 : 4;
     if (hipMemsetAsync(pool, 0, 2 * nbuf * bytes, s) != hipSuccess) st = HELM_ERR_HIP;
     // rows per run x right-hand sides per group (the conditioned kernel's runs carry a 7-row warm-up
     // and march in groups of 8 rows; the intermediate row counts give grids that fill the device's

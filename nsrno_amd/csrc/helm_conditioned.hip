@@ -1046,6 +1046,32 @@ __global__ void k_cond_post(int H, int W, int64_t n, int nrhs, int bz0, int bx0,
     }
 }
 
+// forward form of k_cond_post: only the nodes outside the native box (the
+// frame of bands above, below, left and right of it) are written, z = v there
+__global__ void k_cond_frame(int H, int W, int64_t n, int nrhs, int bz0, int bx0, int bh, int bw,
+                             const float2* __restrict__ v, float2* z, const double* __restrict__ fscale) {
+    // frame nodes of one RHS: rows outside [bz0, bz0+bh) whole, then the left and right parts of the box rows
+    const int64_t top = (int64_t)bz0 * W, bottom = (int64_t)(H - bz0 - bh) * W;
+    const int side = W - bw;
+    const int64_t per = top + bottom + (int64_t)bh * side;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * nrhs) return;
+    const int r = (int)(e / per);
+    int64_t k = e - (int64_t)r * per, node;
+    if (k < top) {
+        node = k;
+    } else if (k < top + bottom) {
+        node = (int64_t)(bz0 + bh) * W + (k - top);
+    } else {
+        k -= top + bottom;
+        const int64_t row = k / side;
+        const int c = (int)(k - row * side);
+        node = (bz0 + row) * W + (c < bx0 ? c : c + bw);
+    }
+    const int64_t o = (int64_t)r * n + node;
+    z[o] = fscale ? cscale(v[o], (float)fscale[r]) : v[o];
+}
+
 int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
                      const double* fscale) {
     const Level& L = p->levels[0];
@@ -1068,8 +1094,15 @@ int32_t cond_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool ad
     } else {
         HELM_TRY(mg_vcycle(p, v, z, nrhs, false, s, fscale));
     }
-    hipLaunchKernelGGL(k_cond_post, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1], L.cbox[2],
-                       L.cbox[3], k0sq, (int)adjoint, L.sx32, L.sz32, v, z, fscale);
+    if (adjoint) {
+        hipLaunchKernelGGL(k_cond_post, grid, block, 0, s, L.nz, L.nx, L.n, nrhs, L.cbox[0], L.cbox[1], L.cbox[2],
+                           L.cbox[3], k0sq, (int)adjoint, L.sx32, L.sz32, v, z, fscale);
+    } else {
+        const int64_t frame = ((int64_t)L.nz * L.nx - (int64_t)L.cbox[2] * L.cbox[3]) * nrhs;
+        if (frame > 0)
+            hipLaunchKernelGGL(k_cond_frame, dim3((unsigned)((frame + 255) / 256)), block, 0, s, L.nz, L.nx, L.n,
+                               nrhs, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], v, z, fscale);
+    }
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

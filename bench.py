@@ -102,6 +102,8 @@ def parse():
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="summarise rocprofv3 --pmc runs under DIR (fetch_bench, write_bench, fetch_copy, "
                          "write_copy) into profiles/pmc_traffic.json and exit")
+    ap.add_argument("--pmc-labels", default="",
+                    help="with --pmc-summary: comma-separated kernel labels to update (default: every kernel seen)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched frequency-sweep leg")
@@ -257,10 +259,12 @@ def _pmc_rows(directory, counter):
     return out
 
 
-def pmc_summary(directory):
+def pmc_summary(directory, labels=None):
     """FETCH_SIZE/WRITE_SIZE (KiB per dispatch) of the bench kernels, each
     scaled by the factor that maps the same counter to the known bytes of the
-    calibration copy -> profiles/pmc_traffic.json."""
+    calibration copy -> profiles/pmc_traffic.json (only the entries named in
+    `labels` when given, so that a run of another workload updates its own
+    kernels and leaves the rest)."""
     import collections
     scale = {}
     for counter, sub in (("FETCH_SIZE", "fetch_copy"), ("WRITE_SIZE", "write_copy")):
@@ -278,7 +282,8 @@ def pmc_summary(directory):
              "k_apply_pair": "k_apply_pair<float>",
              "k_mgt_pre2_rr": "k_mgt_pre2_rr", "k_mgt_post2": "k_mgt_post2", "k_mgt_jacobi": "k_mgt_jacobi",
              "k_mgt_pre2": "k_mgt_pre2", "k_mgt_prolong_smooth": "k_mgt_prolong_smooth",
-             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail(": "k_tail", "k_cond8s": "k_cond8s<float>"}
+             "k_mgt_restrict_res": "k_mgt_restrict_res", "k_tail(": "k_tail", "k_cond8s<float": "k_cond8s<float>",
+             "k_stag8<float": "k_stag8<float>"}
     totals = collections.defaultdict(lambda: {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
     # multigrid kernels run on every level: their finest-level launches (the
     # largest grid of the kernel, over all its template instances: the finest
@@ -312,6 +317,8 @@ def pmc_summary(directory):
            "scale_fetch_bytes_per_count": scale["FETCH_SIZE"], "scale_write_bytes_per_count": scale["WRITE_SIZE"],
            "kernels": kept}
     for label, t in totals.items():
+        if labels and label not in labels:
+            continue
         f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
         if f[1] and w[1]:
             fb, wb = f[0] / f[1], w[0] / w[1]
@@ -465,26 +472,17 @@ def conditioned_workload(a):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"[bench] conditioned solve {dt / a.steps * 1e3:.1f} ms/step, {info}", flush=True)
-    # operator kernel alone: FP32 forward and adjoint over the R fields
-    v = torch.randn(R, g.H, g.W, dtype=torch.complex64, device="cuda")
-    out = torch.empty_like(v)
+    # operator kernel alone: FP32 forward and adjoint over the R fields, on
+    # inputs outside the last-level cache (cold, the roofline figure) and on
+    # one field pair again and again (warm)
     n = g.H * g.W
     alg = (16.0 + 16.0 / R) * n * R
     kern = {}
     for name, adj in (("forward", False), ("adjoint", True)):
-        for _ in range(3):
-            system.plan.apply(v, adjoint=adj, out=out)
-        s = torch.cuda.current_stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(s)
-        reps = 30
-        for _ in range(reps):
-            system.plan.apply(v, adjoint=adj, out=out)
-        e1.record(s)
-        torch.cuda.synchronize()
-        t = e0.elapsed_time(e1) / 1e3 / reps
+        t, tw = _cold_warm_time(lambda u, o: system.plan.apply(u, adjoint=adj, out=o), (R, g.H, g.W))
         kern[name] = {"avg_launch_us": round(t * 1e6, 2), "gb_s": round(alg / t / 1e9, 1),
-                      "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4)}
+                      "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4), "warm_us": round(tw * 1e6, 2),
+                      "warm_frac": round(alg / tw / 1e9 / PEAK_HBM_GBS, 4)}
     cpu = None
     if not a.no_cpu:
         from oracle import conditioned as oc
@@ -515,11 +513,44 @@ def conditioned_workload(a):
         "roofline": {"bound": "hbm", "kernel": "k_cond8s (FP32 streaming) forward, all RHS",
                      "achieved": kern["forward"]["gb_s"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": kern["forward"]["frac"], "traffic": traffic_from_profile("k_cond8s<float>"),
-                     "avg_launch_us": kern["forward"]["avg_launch_us"], "algorithmic_bytes_per_launch": int(alg)},
+                     "avg_launch_us": kern["forward"]["avg_launch_us"], "algorithmic_bytes_per_launch": int(alg),
+                     "inputs": "cold (rotating field pairs, >= 512 MB, none left in the 256 MB last-level cache)",
+                     "warm_us": kern["forward"]["warm_us"], "warm_frac": kern["forward"]["warm_frac"]},
         "adjoint_kernel": kern["adjoint"], "cpu_baseline": cpu, "autotune": tuning,
     }
     print(json.dumps(line), flush=True)
     system.close()
+
+
+def _cold_warm_time(apply, shape, reps=30):
+    """Average seconds per call of apply(u, out) on complex64 fields of
+    `shape`, HIP events on torch's current stream.  Cold: the calls rotate
+    through enough field pairs (>= 512 MB) that no input is still in the
+    256 MB last-level cache, as in a solve; warm: one pair again and again."""
+    import torch
+    elems = 1
+    for d in shape:
+        elems *= d
+    nbuf = max(2, min(16, (512 << 20) // (2 * 8 * elems) + 1))
+    us = [torch.randn(shape, dtype=torch.complex64, device="cuda") for _ in range(nbuf)]
+    outs = [torch.empty_like(us[0]) for _ in range(nbuf)]
+    s = torch.cuda.current_stream()
+
+    def timed(pairs):
+        for k in range(3):
+            apply(*pairs[k % len(pairs)])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for k in range(reps):
+            apply(*pairs[k % len(pairs)])
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    cold = timed(list(zip(us, outs)))
+    warm = timed([(us[0], outs[0])])
+    del us, outs
+    return cold, warm
 
 
 def _event_time(fn, reps):
@@ -584,9 +615,7 @@ def spectral_workload(a):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     data, _, _ = model.setup(medium, f)
-    v = torch.randn(R, model.H, model.W, dtype=torch.complex64, device="cuda")
-    out = torch.empty_like(v)
-    t = _event_time(lambda: data.plan.apply(v, adjoint=True, out=out), 30)
+    t, tw = _cold_warm_time(lambda u, o: data.plan.apply(u, adjoint=True, out=o), (R, model.H, model.W))
     n = model.H * model.W
     alg = (16.0 + 16.0 / R) * n * R
     line = {
@@ -606,7 +635,9 @@ def spectral_workload(a):
         "roofline": {"bound": "hbm", "kernel": "k_stag8<float> (8-tap periodic stencil, all RHS)",
                      "achieved": round(alg / t / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile("k_stag8<float>"),
-                     "avg_launch_us": round(t * 1e6, 2), "algorithmic_bytes_per_launch": int(alg)},
+                     "avg_launch_us": round(t * 1e6, 2), "algorithmic_bytes_per_launch": int(alg),
+                     "inputs": "cold (rotating field pairs, >= 512 MB, none left in the 256 MB last-level cache)",
+                     "warm_us": round(tw * 1e6, 2), "warm_frac": round(alg / tw / 1e9 / PEAK_HBM_GBS, 4)},
     }
     print(json.dumps(line), flush=True)
 
@@ -876,7 +907,7 @@ def cpu_baseline(bench, a, plain_iterations=None):
 def main():
     a = parse()
     if a.pmc_summary:
-        pmc_summary(a.pmc_summary)
+        pmc_summary(a.pmc_summary, [x for x in a.pmc_labels.split(",") if x] if a.pmc_labels else None)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported", file=sys.stderr)

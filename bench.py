@@ -104,6 +104,8 @@ def parse():
                          "write_copy) into profiles/pmc_traffic.json and exit")
     ap.add_argument("--pmc-labels", default="",
                     help="with --pmc-summary: comma-separated kernel labels to update (default: every kernel seen)")
+    ap.add_argument("--pmc-tag", default="",
+                    help="with --pmc-summary: stored as '<label> [TAG]' (a kernel measured on another workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-GMRES comparison leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched frequency-sweep leg")
@@ -259,7 +261,7 @@ def _pmc_rows(directory, counter):
     return out
 
 
-def pmc_summary(directory, labels=None):
+def pmc_summary(directory, labels=None, tag=""):
     """FETCH_SIZE/WRITE_SIZE (KiB per dispatch) of the bench kernels, each
     scaled by the factor that maps the same counter to the known bytes of the
     calibration copy -> profiles/pmc_traffic.json (only the entries named in
@@ -322,7 +324,7 @@ def pmc_summary(directory, labels=None):
         f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
         if f[1] and w[1]:
             fb, wb = f[0] / f[1], w[0] / w[1]
-            out["kernels"][label] = {"launches": f[1], "fetch_bytes_per_launch": round(fb),
+            out["kernels"][label + (f" [{tag}]" if tag else "")] = {"launches": f[1], "fetch_bytes_per_launch": round(fb),
                                      "write_bytes_per_launch": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
@@ -576,8 +578,9 @@ def spectral_workload(a):
     in [-8, 8] deg as right-hand sides of one batched solve to rel-res a.tol
     (the reference's default tolerance is 1e-4).  A step = set-up + incident
     fields + right-hand sides + warm start + FGMRES with the reference's
-    Fourier preconditioner + receiver traces.  The 8-tap stencil k_stag8 (FP32, all 11 fields) is timed alone
-    against the HBM roofline: 16 + 16/R bytes per node per field."""
+    Fourier preconditioner + receiver traces.  The 8-tap stencil (FP32, all
+    11 fields; the streaming kernel k_cond8s with unit split-PML factors) is
+    timed alone against the HBM roofline: 16 + 16/R bytes per node per field."""
     import numpy as np
     import torch
     from scipy.ndimage import gaussian_filter
@@ -632,9 +635,11 @@ def spectral_workload(a):
         "solver_iterations": [i["solver_iterations"] for i in infos],
         "relative_residual": max(i["relative_residual"] for i in infos),
         "reasons": sorted(set(i["reason"] for i in infos)),
-        "roofline": {"bound": "hbm", "kernel": "k_stag8<float> (8-tap periodic stencil, all RHS)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_cond8s<float> (the S operator's 8-tap periodic stencil as the streaming kernel "
+                               "with unit split-PML factors, all RHS)",
                      "achieved": round(alg / t / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile("k_stag8<float>"),
+                     "frac": round(alg / t / 1e9 / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile("k_cond8s<float> [spectral]"),
                      "avg_launch_us": round(t * 1e6, 2), "algorithmic_bytes_per_launch": int(alg),
                      "inputs": "cold (rotating field pairs, >= 512 MB, none left in the 256 MB last-level cache)",
                      "warm_us": round(tw * 1e6, 2), "warm_frac": round(alg / tw / 1e9 / PEAK_HBM_GBS, 4)},
@@ -907,7 +912,7 @@ def cpu_baseline(bench, a, plain_iterations=None):
 def main():
     a = parse()
     if a.pmc_summary:
-        pmc_summary(a.pmc_summary, [x for x in a.pmc_labels.split(",") if x] if a.pmc_labels else None)
+        pmc_summary(a.pmc_summary, [x for x in a.pmc_labels.split(",") if x] if a.pmc_labels else None, a.pmc_tag)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported", file=sys.stderr)

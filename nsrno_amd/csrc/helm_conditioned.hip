@@ -718,8 +718,8 @@ static int cond_stream_nb(int) {
 }
 
 template <int RB, int NB>
-static void launch_cond8s_nb(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
-                             hipStream_t s) {
+static void launch_cond8s_nb(const Level& L, const float2* mass, const float2* u, float2* out, int nrhs, bool adjoint,
+                             const float2* b, hipStream_t s) {
     const int tzr = L.st_tz ? L.st_tz : cond_stream_tz();
     const dim3 block(64), grid((L.nx + 63) / 64, (L.nz + tzr - 1) / tzr, (nrhs + RB - 1) / RB);
     const float ih = (float)(1.0 / L.h_m);
@@ -728,8 +728,8 @@ static void launch_cond8s_nb(const Level& L, const float2* u, float2* out, int n
     const float2* tz = L.ctab32 + 4 * L.nx;
 #define HELM_C8S(M, A, BP)                                                                                       \
     hipLaunchKernelGGL((k_cond8s<float, float2, RB, NB, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, tzr,  \
-                       L.ax32, L.az32, L.cmass32, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
-                       c3, u, out, BP)
+                       L.ax32, L.az32, mass, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3,   \
+                       u, out, BP)
     if (b) {
         if (adjoint) HELM_C8S(1, true, b);
         else HELM_C8S(1, false, b);
@@ -740,10 +740,10 @@ static void launch_cond8s_nb(const Level& L, const float2* u, float2* out, int n
 #undef HELM_C8S
 }
 template <int RB>
-static void launch_cond8s(const Level& L, const float2* u, float2* out, int nrhs, bool adjoint, const float2* b,
-                          hipStream_t s) {
-    if (cond_stream_nb(RB) == 4) launch_cond8s_nb<RB, 4>(L, u, out, nrhs, adjoint, b, s);
-    else launch_cond8s_nb<RB, 8>(L, u, out, nrhs, adjoint, b, s);
+static void launch_cond8s(const Level& L, const float2* mass, const float2* u, float2* out, int nrhs, bool adjoint,
+                          const float2* b, hipStream_t s) {
+    if (cond_stream_nb(RB) == 4) launch_cond8s_nb<RB, 4>(L, mass, u, out, nrhs, adjoint, b, s);
+    else launch_cond8s_nb<RB, 8>(L, mass, u, out, nrhs, adjoint, b, s);
 }
 
 // FP64 (true residuals of the refinement): the streaming kernel with one RHS
@@ -755,8 +755,8 @@ static bool cond64_tiled() {
     }();
     return t;
 }
-static void launch_cond8s64(const Level& L, const double2* u, double2* out, int nrhs, bool adjoint,
-                            const double2* b, hipStream_t s) {
+static void launch_cond8s64(const Level& L, const double2* mass, const double2* u, double2* out, int nrhs,
+                            bool adjoint, const double2* b, hipStream_t s) {
     constexpr int TZR = 32;
     const dim3 block(64), grid((L.nx + 63) / 64, (L.nz + TZR - 1) / TZR, nrhs);
     const double ih = 1.0 / L.h_m;
@@ -765,8 +765,8 @@ static void launch_cond8s64(const Level& L, const double2* u, double2* out, int 
     const double2* tz = L.ctab64 + 4 * L.nx;
 #define HELM_C8S64(M, A, BP)                                                                                     \
     hipLaunchKernelGGL((k_cond8s<double, double2, 1, 8, M, A>), grid, block, 0, s, L.nz, L.nx, L.n, nrhs, TZR,  \
-                       L.ax64, L.az64, L.cmass64, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2,  \
-                       c3, u, out, BP)
+                       L.ax64, L.az64, mass, tx, tz, L.cbox[0], L.cbox[1], L.cbox[2], L.cbox[3], c0, c1, c2, c3,   \
+                       u, out, BP)
     if (b) {
         if (adjoint) HELM_C8S64(1, true, b);
         else HELM_C8S64(1, false, b);
@@ -789,10 +789,14 @@ int32_t launch_cond8_32(const Level& L, const float2* u, float2* out, int nrhs, 
     constexpr int TZ = 16;
     HELM_CHECK(L.ax32 && L.az32 && L.cmass32 && L.ctab32, HELM_ERR_STATE, "conditioned operator: medium not set");
     if (cond_stream_tz() > 0 && !std::getenv("HELM_COND_TILED")) {
+        // a level with a conjugate mass takes its adjoint through the forward kernel
+        const bool hm = adjoint && L.cmassh32;
+        const float2* mass = hm ? L.cmassh32 : L.cmass32;
+        const bool adj = adjoint && !hm;
         switch (L.st_rb ? L.st_rb : cond_stream_rb()) {
-            case 1: launch_cond8s<1>(L, u, out, nrhs, adjoint, b, s); break;
-            case 4: launch_cond8s<4>(L, u, out, nrhs, adjoint, b, s); break;
-            default: launch_cond8s<2>(L, u, out, nrhs, adjoint, b, s); break;
+            case 1: launch_cond8s<1>(L, mass, u, out, nrhs, adj, b, s); break;
+            case 4: launch_cond8s<4>(L, mass, u, out, nrhs, adj, b, s); break;
+            default: launch_cond8s<2>(L, mass, u, out, nrhs, adj, b, s); break;
         }
         HELM_HIP(hipGetLastError());
         return HELM_OK;
@@ -824,7 +828,8 @@ int32_t launch_cond8_64(const Level& L, const double2* u, double2* out, int nrhs
     constexpr int TZ = 8;
     HELM_CHECK(L.ax64 && L.az64 && L.cmass64 && L.ctab64, HELM_ERR_STATE, "conditioned operator: medium not set");
     if (!cond64_tiled()) {
-        launch_cond8s64(L, u, out, nrhs, adjoint, b, s);
+        const bool hm = adjoint && L.cmassh64;
+        launch_cond8s64(L, hm ? L.cmassh64 : L.cmass64, u, out, nrhs, adjoint && !hm, b, s);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }

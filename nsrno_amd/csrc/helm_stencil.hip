@@ -708,7 +708,9 @@ __global__ void k_sample(int n_ports, int64_t n, int nrhs, const int* __restrict
 int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us) {
     Level& L = p->levels[0];
     *best_us = 0.0;
-    const bool cond = L.op == HELM_OP_COND8;
+    // the conditioned streaming kernel: the conditioned operator, and the S
+    // operator once its medium has given it the kernel's operands
+    const bool cond = L.op == HELM_OP_COND8 || (L.op == HELM_OP_STAG8 && L.cmass32 && L.ctab32);
     if (!cond && (L.op != HELM_OP_FLUX5 || L.nx < 64)) return HELM_OK;  // no streaming kernel there
     // Candidates are timed on inputs that are not in the 256 MB last-level
     // cache (Infinity Cache): the launches rotate through `nbuf` field pairs,

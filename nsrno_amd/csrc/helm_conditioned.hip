@@ -828,8 +828,7 @@ int32_t launch_cond8_64(const Level& L, const double2* u, double2* out, int nrhs
     constexpr int TZ = 8;
     HELM_CHECK(L.ax64 && L.az64 && L.cmass64 && L.ctab64, HELM_ERR_STATE, "conditioned operator: medium not set");
     if (!cond64_tiled()) {
-        const bool hm = adjoint && L.cmassh64;
-        launch_cond8s64(L, hm ? L.cmassh64 : L.cmass64, u, out, nrhs, adjoint && !hm, b, s);
+        launch_cond8s64(L, L.cmass64, u, out, nrhs, adjoint, b, s);
         HELM_HIP(hipGetLastError());
         return HELM_OK;
     }

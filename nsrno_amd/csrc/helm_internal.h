@@ -215,10 +215,9 @@ struct Level {
     // (z0, x0, h, w); eta/kap/sx/.. hold the multigrid's 5-point approximation
     double2* cmass64 = nullptr;
     float2* cmass32 = nullptr;
-    // conjugate mass, set where the adjoint differs from the operator only in
-    // it (the S operator: real faces): its adjoint is then the forward kernel
-    // with this mass
-    double2* cmassh64 = nullptr;
+    // conjugate mass (FP32), set where the adjoint differs from the operator
+    // only in it (the S operator: real faces): its adjoint is then the forward
+    // kernel with this mass
     float2* cmassh32 = nullptr;
     double2* ctab64 = nullptr;
     float2* ctab32 = nullptr;

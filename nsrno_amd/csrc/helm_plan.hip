@@ -122,7 +122,7 @@ int32_t upload_cond_profiles(helm_plan* p, int level) {
 static void free_level(Level& L) {
     void* ptrs[] = {L.sx64, L.sz64, L.isxe64, L.isze64, L.sx32, L.sz32, L.isxe32, L.isze32,
                     L.eta64, L.kap64, L.eta32, L.kap32, L.mg_u, L.mg_f, L.mg_r, L.mg_t, L.wgt, L.tcf,
-                    L.wid, L.ax32, L.az32, L.ax64, L.az64, L.cmass64, L.cmass32, L.ctab64, L.ctab32, L.cmassh64, L.cmassh32};
+                    L.wid, L.ax32, L.az32, L.ax64, L.az64, L.cmass64, L.cmass32, L.ctab64, L.ctab32, L.cmassh32};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     L = Level();

@@ -33,8 +33,10 @@ constexpr double FD0 = 1225.0 / 1024.0, FD1 = -245.0 / 3072.0, FD2 = 49.0 / 5120
 constexpr int S8_TX = 64;
 constexpr int S8_HALO = 7;
 
-// HELM_STAG8_TILED=1: the S operator's FP32 and FP64 (b - A u) applies through
-// the LDS-tile k_stag8 instead of the streaming kernel
+// HELM_STAG8_TILED=1: the S operator's FP32 applies through the LDS-tile
+// k_stag8 instead of the streaming kernel (the FP64 forms always take
+// k_stag8: 178 us per 11-field true residual on the 1056x1216 box against
+// 217 us for the streaming kernel in FP64)
 static bool stag8_tiled() {
     static const bool t = [] {
         const char* v = std::getenv("HELM_STAG8_TILED");
@@ -319,7 +321,6 @@ int32_t launch_stag8_64(const Level& L, double k0sq, const double2* u, double2* 
                         const double2* b, int mode, hipStream_t s) {
     constexpr int TZ = 8;
     HELM_CHECK(L.ax64 && L.az64, HELM_ERR_STATE, "8-tap operator: coefficients not set");
-    if (mode != 2 && !stag8_tiled() && L.cmass64 && L.ctab64) return launch_cond8_64(L, u, out, nrhs, adjoint, b, s);
     const int rb = stag8_rhs_per_group<TZ>(L, nrhs);
     const dim3 block(64, 4), grid((L.nx + S8_TX - 1) / S8_TX, (L.nz + TZ - 1) / TZ, (nrhs + rb - 1) / rb);
     const double ih = 1.0 / L.h_m;
@@ -585,17 +586,13 @@ static int32_t launch_incident(helm_plan* p, double sin_theta, double aperture_m
 }  // namespace helm
 
 namespace helm {
-// The S operator through the conditioned streaming kernel: mass k0^2 kappa_c
-// (FP32 rounded as k_stag8 forms it: float(k0^2) * kappa_c in FP32)
-// and its conjugate (the adjoint's mass; the faces are real)
-__global__ void k_sp_mass(int64_t n, double k0sq, const double2* __restrict__ kap64,
-                          const float2* __restrict__ kap32, double2* __restrict__ m64, float2* __restrict__ m32,
-                          double2* __restrict__ mh64, float2* __restrict__ mh32) {
+// The S operator's FP32 applies through the conditioned streaming kernel:
+// mass k0^2 kappa_c (rounded as k_stag8 forms it: float(k0^2) * kappa_c in
+// FP32) and its conjugate (the adjoint's mass; the faces are real)
+__global__ void k_sp_mass(int64_t n, double k0sq, const float2* __restrict__ kap32, float2* __restrict__ m32,
+                          float2* __restrict__ mh32) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const double2 k = kap64[e];
-    m64[e] = make_double2(k0sq * k.x, k0sq * k.y);
-    mh64[e] = make_double2(k0sq * k.x, k0sq * -k.y);
     const float2 k32 = kap32[e];
     const float s = (float)k0sq;
     m32[e] = make_float2(k32.x * s, k32.y * s);
@@ -658,32 +655,23 @@ int32_t helm_set_medium_spectral(helm_plan* p, const float* c, const float* rho,
     // k0 of wave_operator.py:55 (the relaxation form matches the 2.5 ns time stepper)
     p->k0 = m->relaxation ? 2.0 * std::sin(p->omega * m->relax_dt_s / 2.0) / m->relax_dt_s / p->desc.c0_m_s
                           : p->omega / p->desc.c0_m_s;
-    // the streaming kernel's operands: mass k0^2 kappa_c, unit per-column and
+    // the streaming kernel's FP32 operands: mass k0^2 kappa_c, unit per-column and
     // per-row factors (sv = 1, g - 1 = 0, cA = 1, cB = 0: no PML), the whole
     // box as the native box (periodic rows and columns, as here)
-    HELM_TRY(sp_alloc(&L.cmass64, L.n));
     HELM_TRY(sp_alloc(&L.cmass32, L.n));
-    HELM_TRY(sp_alloc(&L.cmassh64, L.n));
     HELM_TRY(sp_alloc(&L.cmassh32, L.n));
-    HELM_TRY(sp_alloc(&L.ctab64, (size_t)4 * (L.nx + L.nz)));
     HELM_TRY(sp_alloc(&L.ctab32, (size_t)4 * (L.nx + L.nz)));
-    hipLaunchKernelGGL(k_sp_mass, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.n, p->k0 * p->k0, L.kap64,
-                       L.kap32, L.cmass64, L.cmass32, L.cmassh64, L.cmassh32);
+    hipLaunchKernelGGL(k_sp_mass, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.n, p->k0 * p->k0, L.kap32,
+                       L.cmass32, L.cmassh32);
     HELM_HIP(hipGetLastError());
     {
-        std::vector<double2> t64((size_t)4 * (L.nx + L.nz));
-        std::vector<float2> t32(t64.size());
+        std::vector<float2> t32((size_t)4 * (L.nx + L.nz));
         auto fill = [&](int n, size_t base) {
             for (int k = 0; k < n; ++k)
-                for (int t = 0; t < 4; ++t) {
-                    const double v = (t == 0 || t == 2) ? 1.0 : 0.0;
-                    t64[base + (size_t)t * n + k] = make_double2(v, 0.0);
-                    t32[base + (size_t)t * n + k] = make_float2((float)v, 0.f);
-                }
+                for (int t = 0; t < 4; ++t) t32[base + (size_t)t * n + k] = make_float2((t == 0 || t == 2) ? 1.f : 0.f, 0.f);
         };
         fill(L.nx, 0);
         fill(L.nz, (size_t)4 * L.nx);
-        HELM_TRY(upload_h2d(L.ctab64, t64.data(), t64.size() * sizeof(double2)));
         HELM_TRY(upload_h2d(L.ctab32, t32.data(), t32.size() * sizeof(float2)));
     }
     L.cbox[0] = 0;

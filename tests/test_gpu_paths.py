@@ -3,13 +3,15 @@
 Each fast path of the library keeps its predecessor behind an environment
 switch that is read once per process (HELM_DCGS_FUSED, HELM_LAZY_BASIS,
 HELM_RESID_BLOCK, HELM_STENCIL_BLOCK, HELM_MG_TAIL_LDS, HELM_XCD_ORDER,
-HELM_RESTART_AHEAD, HELM_COND64_TILED),
+HELM_RESTART_AHEAD, HELM_COND64_TILED, HELM_STAG8_TILED),
 so a test process only ever runs one side.  The tests below run child
 processes with the switches off and with the defaults and check both against
 the reference's golden outputs: the direct-solve fields of
 nsrno/theranostic/wave.py Solver.solve (tests/golden/theranostic_small.npz)
-and the conditioned operator of nsrno/dense_wave.py projected_operator /
-adjoint_operator in FP64 (tests/golden/conditioned_small.npz).
+the conditioned operator of nsrno/dense_wave.py projected_operator /
+adjoint_operator in FP64 (tests/golden/conditioned_small.npz), and the S
+operator of nsrno/wave_operator.py apply_operator in FP32 against the oracle's
+restatement of it (oracle/spectral.py apply_stencil).
 """
 import os
 import subprocess
@@ -108,3 +110,44 @@ def test_conditioned_fp64_tile_and_streaming_match_reference(tmp_path):
         got = np.load(out)
         assert rel(got["proj"], golden["proj"]) < 1e-12, name
         assert rel(got["adj"], golden["adj"]) < 1e-12, name
+
+
+SPEC_GOLDEN = Path(__file__).parent / "golden" / "spectral_small.npz"
+
+# FP32 forward and adjoint applies of the S operator on the golden medium, and
+# the oracle's k0^2 conj(L) / k0^2 L of the same fields
+SPEC_APPLY = r'''
+import sys
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+from oracle import spectral as S
+from nsrno_amd.wave_operator import WaveResidualOperator
+
+G = np.load(sys.argv[2])
+spec = dict(medium_shape=[int(G["h"]), int(G["w"])], dx_m=float(G["dx"]), x0_m=float(G["x0"]), z0_m=float(G["z0"]),
+            angles_deg=list(G["angles"]), source_tref_s=list(G["trefs"]))
+model = WaveResidualOperator(spec, iterations=4)
+data, _, _ = model.setup(torch.tensor(np.stack((G["c"], G["rho"]))[None], dtype=torch.float32), float(G["f"]))
+box = S.Box(int(G["h"]), int(G["w"]), float(G["dx"]), float(G["x0"]), float(G["z0"]))
+ax, az, kc, k0 = S.setup(box, np.float32(G["c"]), np.float32(G["rho"]), float(G["f"]))
+u = np.random.default_rng(5).normal(size=(3, box.H, box.W)) + 1j * np.random.default_rng(6).normal(size=(3, box.H, box.W))
+u32 = torch.from_numpy(u.astype(np.complex64)).cuda()
+np.savez(sys.argv[3], fwd=data.plan.apply(u32).cpu().numpy(), adj=data.plan.apply(u32, adjoint=True).cpu().numpy(),
+         fwd_ref=k0 ** 2 * np.conj(S.apply_stencil(np.conj(u), ax, az, kc, k0, box.dx)),
+         adj_ref=k0 ** 2 * S.apply_stencil(u, ax, az, kc, k0, box.dx))
+'''
+
+
+def test_s_operator_fp32_tile_and_streaming_match_oracle(tmp_path):
+    base = {k: v for k, v in os.environ.items() if k != "HELM_STAG8_TILED"}
+    got = {}
+    for name, env in (("tiled", dict(base, HELM_STAG8_TILED="1")), ("streaming", base)):
+        out = tmp_path / f"{name}.npz"
+        print(f"child apply: {name} FP32 S operator", flush=True)
+        subprocess.run([sys.executable, "-c", SPEC_APPLY, str(ROOT), str(SPEC_GOLDEN), str(out)], env=env,
+                       check=True, timeout=240)
+        got[name] = np.load(out)
+        assert rel(got[name]["fwd"], got[name]["fwd_ref"]) < 5e-6, name
+        assert rel(got[name]["adj"], got[name]["adj_ref"]) < 5e-6, name
+    assert rel(got["tiled"]["fwd"], got["streaming"]["fwd"]) < 5e-6

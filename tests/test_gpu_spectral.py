@@ -54,8 +54,14 @@ def test_stag8_apply_fp64_fp32_batched():
     lu = model.apply_operator(torch.from_numpy(G["u"]).cuda(), data).cpu().numpy()
     assert rel(lu, G["lu"]) < 5e-6
     # the library's forward operator is k0^2 conj(L)
+    fwd_ref = k0 ** 2 * np.conj(S.apply_stencil(np.conj(u), ax, az, kc, k0, box.dx))
     fwd = data.plan.apply(torch.from_numpy(u).cuda()).cpu().numpy()
-    assert rel(fwd, k0 ** 2 * np.conj(S.apply_stencil(np.conj(u), ax, az, kc, k0, box.dx))) < 1e-12
+    assert rel(fwd, fwd_ref) < 1e-12
+    # in FP32 both directions (the adjoint takes the conjugate mass through the forward kernel)
+    fwd32 = data.plan.apply(torch.from_numpy(u.astype(np.complex64)).cuda()).cpu().numpy()
+    assert rel(fwd32, fwd_ref) < 5e-6
+    adj32 = data.plan.apply(torch.from_numpy(u.astype(np.complex64)).cuda(), adjoint=True).cpu().numpy()
+    assert rel(adj32, k0 ** 2 * ref) < 5e-6
 
 
 @pytest.mark.parametrize("attenuation", [None, "relaxation"])

@@ -60,28 +60,55 @@ __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign
 }
 
 // transform along x: lines = rows of all batches (H * B of them), length W
+// Pass options the preconditioner folds into its first and last passes: the
+// conjugate of the input on load (conj_in), and on store
+// out = conj?(mul[node] (scale v)) (scale 1 and no mul: the plain output;
+// nodes of the [H][W] box, the same for every batch)
+template <typename C>
+struct FftEpi {
+    int conj_in = 0;
+    const C* mul = nullptr;
+    float scale = 1.f;
+    int conj_out = 0;
+};
+template <typename C>
+__device__ __forceinline__ C fft_epi(C v, const FftEpi<C>& e, int64_t node) {
+    if (e.scale != 1.f) {
+        v.x *= e.scale;
+        v.y *= e.scale;
+    }
+    if (e.mul) v = cmul(e.mul[node], v);
+    if (e.conj_out) v.y = -v.y;
+    return v;
+}
+
 template <typename C, int R>
-__global__ __launch_bounds__(256) void k_fft_rows(int W, int64_t lines, int Ns, int sign, const C* __restrict__ in,
-                                                 C* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, int Ns, int sign,
+                                                 const C* __restrict__ in, C* __restrict__ out, FftEpi<C> ep) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = W / R;
     if (t >= (int64_t)M * lines) return;
     const int j = (int)(t % M);
-    const int64_t base = (t / M) * W;
+    const int64_t line = t / M, base = line * W;
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[base + j + r * M];
+    if (ep.conj_in) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
+    }
     const int k = j % Ns;
     fft_butterfly<C, R>(v, k, Ns, sign);
     const int o = (j / Ns) * Ns * R + k;
+    const int64_t nrow = (line % H) * W;
 #pragma unroll
-    for (int q = 0; q < R; ++q) out[base + o + q * Ns] = v[q];
+    for (int q = 0; q < R; ++q) out[base + o + q * Ns] = fft_epi(v[q], ep, nrow + o + q * Ns);
 }
 
 // transform along z: for every batch and column x, length H with stride W
 template <typename C, int R>
 __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, int sign, const C* __restrict__ in,
-                                                 C* __restrict__ out) {
+                                                 C* __restrict__ out, FftEpi<C> ep) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = H / R;
     if (t >= (int64_t)W * M * B) return;
@@ -91,11 +118,16 @@ __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, i
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[base + (int64_t)(j + r * M) * W];
+    if (ep.conj_in) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
+    }
     const int k = j % Ns;
     fft_butterfly<C, R>(v, k, Ns, sign);
     const int o = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) out[base + (int64_t)(o + q * Ns) * W] = v[q];
+    for (int q = 0; q < R; ++q)
+        out[base + (int64_t)(o + q * Ns) * W] = fft_epi(v[q], ep, (int64_t)(o + q * Ns) * W + x);
 }
 
 // any radix: one output (line, j, q) per thread; stride = element stride
@@ -139,44 +171,59 @@ static std::vector<int> fft_radices(int n) {
 }
 
 template <typename C, int R>
-static void launch_rows(int W, int64_t lines, int Ns, int sign, const C* in, C* out, hipStream_t s) {
+static void launch_rows(int H, int W, int64_t lines, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,
+                        hipStream_t s) {
     const int64_t cnt = (int64_t)(W / R) * lines;
-    hipLaunchKernelGGL((k_fft_rows<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W, lines, Ns, sign, in,
-                       out);
+    hipLaunchKernelGGL((k_fft_rows<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, lines, Ns,
+                       sign, in, out, ep);
 }
 template <typename C, int R>
-static void launch_cols(int H, int W, int B, int Ns, int sign, const C* in, C* out, hipStream_t s) {
+static void launch_cols(int H, int W, int B, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,
+                        hipStream_t s) {
     const int64_t cnt = (int64_t)W * (H / R) * B;
     hipLaunchKernelGGL((k_fft_cols<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, B, Ns, sign,
-                       in, out);
+                       in, out, ep);
 }
 
 // One axis of a 2-D transform of B boxes [H][W]: src -> (passes ping-pong
-// between a and b) -> returns the buffer holding the result.
+// between a and b) -> returns the buffer holding the result.  The first pass
+// takes first.conj_in, the last pass last's store options and writes to
+// last_out when given (the generic kernel of a radix above 31 takes neither).
 template <typename C>
-static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s) {
+static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s,
+                   const FftEpi<C>& first = FftEpi<C>(), const FftEpi<C>& last = FftEpi<C>(),
+                   C* last_out = nullptr) {
     const int N = rows ? W : H;
     const std::vector<int> rad = fft_radices(N);
     const C* in = src;
     C* out = a;
     int Ns = 1;
-    for (int R : rad) {
+    for (size_t p = 0; p < rad.size(); ++p) {
+        const int R = rad[p];
+        FftEpi<C> ep;
+        if (p == 0) ep.conj_in = first.conj_in;
+        if (p + 1 == rad.size()) {
+            ep.mul = last.mul;
+            ep.scale = last.scale;
+            ep.conj_out = last.conj_out;
+            if (last_out) out = last_out;
+        }
         if (rows) {
             const int64_t lines = (int64_t)H * B;
             switch (R) {
-                case 2: launch_rows<C, 2>(W, lines, Ns, sign, in, out, s); break;
-                case 3: launch_rows<C, 3>(W, lines, Ns, sign, in, out, s); break;
-                case 4: launch_rows<C, 4>(W, lines, Ns, sign, in, out, s); break;
-                case 5: launch_rows<C, 5>(W, lines, Ns, sign, in, out, s); break;
-                case 7: launch_rows<C, 7>(W, lines, Ns, sign, in, out, s); break;
-                case 8: launch_rows<C, 8>(W, lines, Ns, sign, in, out, s); break;
-                case 11: launch_rows<C, 11>(W, lines, Ns, sign, in, out, s); break;
-                case 13: launch_rows<C, 13>(W, lines, Ns, sign, in, out, s); break;
-                case 17: launch_rows<C, 17>(W, lines, Ns, sign, in, out, s); break;
-                case 19: launch_rows<C, 19>(W, lines, Ns, sign, in, out, s); break;
-                case 23: launch_rows<C, 23>(W, lines, Ns, sign, in, out, s); break;
-                case 29: launch_rows<C, 29>(W, lines, Ns, sign, in, out, s); break;
-                case 31: launch_rows<C, 31>(W, lines, Ns, sign, in, out, s); break;
+                case 2: launch_rows<C, 2>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 3: launch_rows<C, 3>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 4: launch_rows<C, 4>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 5: launch_rows<C, 5>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 7: launch_rows<C, 7>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 8: launch_rows<C, 8>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 11: launch_rows<C, 11>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 13: launch_rows<C, 13>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 17: launch_rows<C, 17>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 19: launch_rows<C, 19>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 23: launch_rows<C, 23>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 29: launch_rows<C, 29>(H, W, lines, Ns, sign, in, out, ep, s); break;
+                case 31: launch_rows<C, 31>(H, W, lines, Ns, sign, in, out, ep, s); break;
                 default: {
                     const int64_t cnt = (int64_t)W * lines;
                     hipLaunchKernelGGL(k_fft_generic<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W, R,
@@ -186,19 +233,19 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
             }
         } else {
             switch (R) {
-                case 2: launch_cols<C, 2>(H, W, B, Ns, sign, in, out, s); break;
-                case 3: launch_cols<C, 3>(H, W, B, Ns, sign, in, out, s); break;
-                case 4: launch_cols<C, 4>(H, W, B, Ns, sign, in, out, s); break;
-                case 5: launch_cols<C, 5>(H, W, B, Ns, sign, in, out, s); break;
-                case 7: launch_cols<C, 7>(H, W, B, Ns, sign, in, out, s); break;
-                case 8: launch_cols<C, 8>(H, W, B, Ns, sign, in, out, s); break;
-                case 11: launch_cols<C, 11>(H, W, B, Ns, sign, in, out, s); break;
-                case 13: launch_cols<C, 13>(H, W, B, Ns, sign, in, out, s); break;
-                case 17: launch_cols<C, 17>(H, W, B, Ns, sign, in, out, s); break;
-                case 19: launch_cols<C, 19>(H, W, B, Ns, sign, in, out, s); break;
-                case 23: launch_cols<C, 23>(H, W, B, Ns, sign, in, out, s); break;
-                case 29: launch_cols<C, 29>(H, W, B, Ns, sign, in, out, s); break;
-                case 31: launch_cols<C, 31>(H, W, B, Ns, sign, in, out, s); break;
+                case 2: launch_cols<C, 2>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 3: launch_cols<C, 3>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 4: launch_cols<C, 4>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 5: launch_cols<C, 5>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 7: launch_cols<C, 7>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 8: launch_cols<C, 8>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 11: launch_cols<C, 11>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 13: launch_cols<C, 13>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 17: launch_cols<C, 17>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 19: launch_cols<C, 19>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 23: launch_cols<C, 23>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 29: launch_cols<C, 29>(H, W, B, Ns, sign, in, out, ep, s); break;
+                case 31: launch_cols<C, 31>(H, W, B, Ns, sign, in, out, ep, s); break;
                 default: {
                     const int64_t lines = (int64_t)W * B;
                     const int64_t cnt = (int64_t)H * lines;
@@ -219,10 +266,19 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
 // only if it is not overwritten before it is read (the first pass reads src
 // and writes a).
 template <typename C>
-C* fft2(int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s) {
-    C* r = fft_axis<C>(true, H, W, B, sign, src, a, b, s);
+C* fft2(int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s, int conj_in = 0,
+        const FftEpi<C>& last = FftEpi<C>(), C* last_out = nullptr) {
+    FftEpi<C> first;
+    first.conj_in = conj_in;
+    C* r = fft_axis<C>(true, H, W, B, sign, src, a, b, s, first);
     C* other = (r == a) ? b : a;
-    return fft_axis<C>(false, H, W, B, sign, r, other, r, s);
+    return fft_axis<C>(false, H, W, B, sign, r, other, r, s, FftEpi<C>(), last, last_out);
+}
+// largest radix of a side (the pass-per-radix kernels take options up to 31)
+static int fft_max_radix(int n) {
+    int m = 1;
+    for (int r : fft_radices(n)) m = std::max(m, r);
+    return m;
 }
 
 // z[r] = conj?(gain (.) buf[r] / N) ; buf = M (.) fft2(conj?(v))  (two kernels)
@@ -260,6 +316,22 @@ int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, boo
     // real gain this is the reference's adjoint preconditioner gain ifft2(conj(M)
     // fft2(v)), dense_wave.py:195-196).
     const bool as_given = L.op == HELM_OP_STAG8 ? adjoint : !adjoint;
+    if (fft_max_radix(H) <= 31 && fft_max_radix(W) <= 31) {
+        // the conjugate of the input in the first pass, M in the last forward
+        // pass, 1/N, the gain and the conjugate in the last inverse pass,
+        // which writes z
+        FftEpi<float2> m;
+        m.mul = p->fft_mult;
+        float2* f = fft2<float2>(H, W, nrhs, -1, v, a, b, s, (int)!as_given, m);
+        float2* fo = (f == a) ? b : a;
+        FftEpi<float2> g;
+        g.scale = (float)(1.0 / (double)n);
+        g.mul = use_gain ? p->fft_gain : nullptr;
+        g.conj_out = (int)!as_given;
+        fft2<float2>(H, W, nrhs, +1, f, fo, f, s, 0, g, z);
+        HELM_HIP(hipGetLastError());
+        return HELM_OK;
+    }
     const float2* src = v;
     if (!as_given) {
         hipLaunchKernelGGL(k_sp_conj_copy, g, blk, 0, s, cnt, v, b);

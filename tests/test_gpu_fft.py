@@ -5,7 +5,8 @@ preconditioner, and the reference's warm-start-only forward through them.
   and odd prime factors (3, 11, 19, 37: register and generic kernels),
   forward and inverse, batched.
 * HELM_PC_SPECTRAL: gain * ifft2(M fft2(v)) (adjoint) and its conjugate map
-  against numpy.
+  against numpy, on boxes with radix 2-5 and radix 11 / 19 sides (the
+  multiply, gain and conjugates folded into the first and last passes).
 * WaveResidualOperator.forward (nsrno/wave_operator.py:137-154) on the
   reference's golden case: the default model and a model loaded with
   perturbed context net, spectral modulation and shift; gain maps against
@@ -49,10 +50,10 @@ def test_fft2_matches_numpy(H, W, precision):
         assert rel(out.cpu().numpy(), ref) < (3e-6 if precision == 32 else 1e-14), (H, W, inverse)
 
 
-def test_spectral_preconditioner_apply():
+@pytest.mark.parametrize("H,W", [(96, 160), (132, 152)])
+def test_spectral_preconditioner_apply(H, W):
     from nsrno_amd import _lib
     from nsrno_amd.helmholtz import HelmholtzPlan
-    H, W = 96, 160
     plan = HelmholtzPlan(H, W, 0.05, 3.0, 0.0, nrhs=2, restart=4)
     plan.set_medium(np.ones(H * W), np.ones(H * W, dtype=complex))
     rng = np.random.default_rng(2)

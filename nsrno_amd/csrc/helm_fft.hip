@@ -67,6 +67,7 @@ __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign
 template <typename C>
 struct FftEpi {
     int conj_in = 0;
+    const double* bscale = nullptr;  // first pass: input of batch b scaled by bscale[b]
     const C* mul = nullptr;
     float scale = 1.f;
     int conj_out = 0;
@@ -93,6 +94,14 @@ __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, i
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[base + j + r * M];
+    if (ep.bscale) {
+        const float bs = (float)ep.bscale[line / H];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r].x *= bs;
+            v[r].y *= bs;
+        }
+    }
     if (ep.conj_in) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
@@ -118,6 +127,14 @@ __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, i
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[base + (int64_t)(j + r * M) * W];
+    if (ep.bscale) {
+        const float bs = (float)ep.bscale[t / ((int64_t)W * M)];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r].x *= bs;
+            v[r].y *= bs;
+        }
+    }
     if (ep.conj_in) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
@@ -201,7 +218,10 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
     for (size_t p = 0; p < rad.size(); ++p) {
         const int R = rad[p];
         FftEpi<C> ep;
-        if (p == 0) ep.conj_in = first.conj_in;
+        if (p == 0) {
+            ep.conj_in = first.conj_in;
+            ep.bscale = first.bscale;
+        }
         if (p + 1 == rad.size()) {
             ep.mul = last.mul;
             ep.scale = last.scale;
@@ -267,9 +287,10 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
 // and writes a).
 template <typename C>
 C* fft2(int H, int W, int B, int sign, const C* src, C* a, C* b, hipStream_t s, int conj_in = 0,
-        const FftEpi<C>& last = FftEpi<C>(), C* last_out = nullptr) {
+        const FftEpi<C>& last = FftEpi<C>(), C* last_out = nullptr, const double* bscale = nullptr) {
     FftEpi<C> first;
     first.conj_in = conj_in;
+    first.bscale = bscale;
     C* r = fft_axis<C>(true, H, W, B, sign, src, a, b, s, first);
     C* other = (r == a) ? b : a;
     return fft_axis<C>(false, H, W, B, sign, r, other, r, s, FftEpi<C>(), last, last_out);
@@ -299,8 +320,13 @@ __global__ void k_sp_gain(int64_t n, int64_t cnt, float scale, int conj, const f
     z[e] = conj ? cconj(v) : v;
 }
 
+bool spectral_scaled_input_ok(const helm_plan* p) {
+    const Level& L = p->levels[0];
+    return p->fft_mult && fft_max_radix(L.nz) <= 31 && fft_max_radix(L.nx) <= 31;
+}
+
 int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
-                         bool use_gain) {
+                         bool use_gain, const double* fscale) {
     HELM_CHECK(p->fft_mult, HELM_ERR_STATE, "spectral preconditioner: call helm_spectral_set_preconditioner first");
     HELM_CHECK(nrhs <= p->fft_cap, HELM_ERR_ARG, "spectral preconditioner: more right-hand sides than its buffers");
     const Level& L = p->levels[0];
@@ -316,13 +342,15 @@ int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, boo
     // real gain this is the reference's adjoint preconditioner gain ifft2(conj(M)
     // fft2(v)), dense_wave.py:195-196).
     const bool as_given = L.op == HELM_OP_STAG8 ? adjoint : !adjoint;
+    HELM_CHECK(!fscale || spectral_scaled_input_ok(p), HELM_ERR_STATE,
+               "spectral preconditioner: scaled input needs the fused passes");
     if (fft_max_radix(H) <= 31 && fft_max_radix(W) <= 31) {
-        // the conjugate of the input in the first pass, M in the last forward
-        // pass, 1/N, the gain and the conjugate in the last inverse pass,
-        // which writes z
+        // the per-RHS input scale and the conjugate of the input in the first
+        // pass, M in the last forward pass, 1/N, the gain and the conjugate in
+        // the last inverse pass, which writes z
         FftEpi<float2> m;
         m.mul = p->fft_mult;
-        float2* f = fft2<float2>(H, W, nrhs, -1, v, a, b, s, (int)!as_given, m);
+        float2* f = fft2<float2>(H, W, nrhs, -1, v, a, b, s, (int)!as_given, m, nullptr, fscale);
         float2* fo = (f == a) ? b : a;
         FftEpi<float2> g;
         g.scale = (float)(1.0 / (double)n);

@@ -505,8 +505,11 @@ int32_t tune_stencil(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 int32_t tune_mg(helm_plan* p, int nrhs, hipStream_t s, double* best_us);
 // Fourier-multiplier preconditioner z = gain ifft2(M fft2(v)) (helm_fft.hip);
 // adjoint = false applies its conjugate map
+// fscale (nullable, per RHS, device): input of RHS r scaled by fscale[r] (the
+// lazily scaled Krylov basis), when spectral_scaled_input_ok
 int32_t spectral_precond(helm_plan* p, const float2* v, float2* z, int nrhs, bool adjoint, hipStream_t s,
-                         bool use_gain = true);
+                         bool use_gain = true, const double* fscale = nullptr);
+bool spectral_scaled_input_ok(const helm_plan* p);
 // learned correction z += N(v) (helm_net.hip)
 int32_t net_correction(helm_plan* p, struct helm_net* n, const float2* v, float2* z, int nrhs, hipStream_t s);
 }  // namespace helm

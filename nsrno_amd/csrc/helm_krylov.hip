@@ -1468,8 +1468,11 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
     const double resid_bytes = 48.0 * nn * rr + 2.0 * coef_bytes32 * nn;
     // lazily scaled basis: the multigrid's finest-level kernels normalise V_j as they load it
     // (conditioned plans: the prescale kernel applies it, or on adjoint solves the multigrid)
-    const bool lazy = dcgs_fused() && lazy_basis() && pc == HELM_PC_MG &&
-                      ((L.op == HELM_OP_COND8 && !adj) || mg_scaled_input_ok(p));
+    // (the spectral preconditioner: its first FFT pass)
+    const bool spc = pc == HELM_PC_SPECTRAL || pc == HELM_PC_SPECTRAL_PLAIN;
+    const bool lazy = dcgs_fused() && lazy_basis() &&
+                      ((pc == HELM_PC_MG && ((L.op == HELM_OP_COND8 && !adj) || mg_scaled_input_ok(p))) ||
+                       (spc && spectral_scaled_input_ok(p)));
     // ... and for the flux operator the restart residual is written straight into V_0 (k_residual_v0)
     const dim3 rgrid((unsigned)((L.nx + 63) / 64), (unsigned)((L.nz + 3) / 4), 1);
     const int nchr = (int)(rgrid.x * rgrid.y);
@@ -1519,7 +1522,9 @@ int32_t solve_gmres(helm_plan* p, const double2* b, double2* x, int nrhs, const 
             hipEvent_t ev;
             if (pc != HELM_PC_NONE) {
                 ev = pf ? prof_begin(pf, PK_PRECOND, cs) : nullptr;
-                if (lazy && L.op == HELM_OP_COND8)
+                if (lazy && spc)
+                    HELM_TRY(spectral_precond(p, vj, zj, nrhs, adj, cs, pc == HELM_PC_SPECTRAL, W.pscale));
+                else if (lazy && L.op == HELM_OP_COND8)
                     HELM_TRY(cond_precond(p, vj, zj, nrhs, adj, cs, W.pscale));
                 else if (lazy)
                     HELM_TRY(mg_vcycle(p, vj, zj, nrhs, adj, cs, W.pscale, W.cyc));

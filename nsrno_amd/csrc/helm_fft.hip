@@ -17,6 +17,7 @@
 // W) have their own kernels so that a wavefront always reads consecutive
 // complex numbers: along x it walks j, along z it walks x.  FP32 data with
 // twiddle phases formed from exact integer products.
+#include <algorithm>
 #include <cmath>
 
 #include "helm_internal.h"
@@ -59,6 +60,53 @@ __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign
     for (int q = 0; q < R; ++q) v[q] = out[q];
 }
 
+// Composite radix R = R1 R2 in one pass (fewer passes over the data): the
+// pass twiddles as fft_butterfly, then the R-point DFT by Cooley-Tukey in
+// registers, with r = r1 + R1 r2 and q = q2 + R2 q1:
+//   A[r1][q2] = sum_r2 v[r1 + R1 r2] w_R2^(r2 q2)       (R1 DFTs of size R2)
+//   out[q2 + R2 q1] = sum_r1 (w_R^(r1 q2) A[r1][q2]) w_R1^(r1 q1)
+// (R1 (R1 + R2) R2 products instead of R^2)
+template <typename C, int R1, int R2>
+__device__ __forceinline__ void fft_butterfly2(C (&v)[R1 * R2], int k, int Ns, int sign) {
+    constexpr int R = R1 * R2;
+    const int span = Ns * R;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        const int m = (int)(((int64_t)k * r) % span);
+        v[r] = cmul(v[r], cis<C>(sign * m, span));
+    }
+    C w2[R2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) w2[q] = cis<C>(sign * q, R2);
+    C a[R];
+#pragma unroll
+    for (int r1 = 0; r1 < R1; ++r1)
+#pragma unroll
+        for (int q2 = 0; q2 < R2; ++q2) {
+            C acc = v[r1];
+#pragma unroll
+            for (int r2 = 1; r2 < R2; ++r2) cfma(acc, v[r1 + R1 * r2], w2[(r2 * q2) % R2]);
+            a[r1 * R2 + q2] = (r1 && q2) ? cmul(acc, cis<C>(sign * ((r1 * q2) % R), R)) : acc;
+        }
+    C w1[R1];
+#pragma unroll
+    for (int q = 0; q < R1; ++q) w1[q] = cis<C>(sign * q, R1);
+#pragma unroll
+    for (int q2 = 0; q2 < R2; ++q2)
+#pragma unroll
+        for (int q1 = 0; q1 < R1; ++q1) {
+            C acc = a[q2];
+#pragma unroll
+            for (int r1 = 1; r1 < R1; ++r1) cfma(acc, a[r1 * R2 + q2], w1[(r1 * q1) % R1]);
+            v[q2 + R2 * q1] = acc;
+        }
+}
+template <typename C, int R1, int R2>
+__device__ __forceinline__ void fft_pass_butterfly(C (&v)[R1 * R2], int k, int Ns, int sign) {
+    if constexpr (R2 == 1) fft_butterfly<C, R1>(v, k, Ns, sign);
+    else fft_butterfly2<C, R1, R2>(v, k, Ns, sign);
+}
+
 // transform along x: lines = rows of all batches (H * B of them), length W
 // Pass options the preconditioner folds into its first and last passes: the
 // conjugate of the input on load (conj_in), and on store
@@ -83,9 +131,10 @@ __device__ __forceinline__ C fft_epi(C v, const FftEpi<C>& e, int64_t node) {
     return v;
 }
 
-template <typename C, int R>
+template <typename C, int R1, int R2 = 1>
 __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, int Ns, int sign,
                                                  const C* __restrict__ in, C* __restrict__ out, FftEpi<C> ep) {
+    constexpr int R = R1 * R2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = W / R;
     if (t >= (int64_t)M * lines) return;
@@ -107,7 +156,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, i
         for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
     }
     const int k = j % Ns;
-    fft_butterfly<C, R>(v, k, Ns, sign);
+    fft_pass_butterfly<C, R1, R2>(v, k, Ns, sign);
     const int o = (j / Ns) * Ns * R + k;
     const int64_t nrow = (line % H) * W;
 #pragma unroll
@@ -115,9 +164,10 @@ __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, i
 }
 
 // transform along z: for every batch and column x, length H with stride W
-template <typename C, int R>
+template <typename C, int R1, int R2 = 1>
 __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, int sign, const C* __restrict__ in,
                                                  C* __restrict__ out, FftEpi<C> ep) {
+    constexpr int R = R1 * R2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = H / R;
     if (t >= (int64_t)W * M * B) return;
@@ -140,7 +190,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, i
         for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
     }
     const int k = j % Ns;
-    fft_butterfly<C, R>(v, k, Ns, sign);
+    fft_pass_butterfly<C, R1, R2>(v, k, Ns, sign);
     const int o = (j / Ns) * Ns * R + k;
 #pragma unroll
     for (int q = 0; q < R; ++q)
@@ -187,19 +237,100 @@ static std::vector<int> fft_radices(int n) {
     return r;
 }
 
-template <typename C, int R>
+template <typename C, int R1, int R2 = 1>
 static void launch_rows(int H, int W, int64_t lines, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,
                         hipStream_t s) {
-    const int64_t cnt = (int64_t)(W / R) * lines;
-    hipLaunchKernelGGL((k_fft_rows<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, lines, Ns,
-                       sign, in, out, ep);
+    const int64_t cnt = (int64_t)(W / (R1 * R2)) * lines;
+    hipLaunchKernelGGL((k_fft_rows<C, R1, R2>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, lines,
+                       Ns, sign, in, out, ep);
 }
-template <typename C, int R>
+template <typename C, int R1, int R2 = 1>
 static void launch_cols(int H, int W, int B, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,
                         hipStream_t s) {
-    const int64_t cnt = (int64_t)W * (H / R) * B;
-    hipLaunchKernelGGL((k_fft_cols<C, R>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, B, Ns, sign,
-                       in, out, ep);
+    const int64_t cnt = (int64_t)W * (H / (R1 * R2)) * B;
+    hipLaunchKernelGGL((k_fft_cols<C, R1, R2>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, B, Ns,
+                       sign, in, out, ep);
+}
+
+// Composite passes (FP32 transforms): (R1, R2) pairs with kernels, and the
+// plan that packs a side's factors into them: odd primes paired (product up
+// to 40), powers of two attached to single odd primes, the rest of the power
+// of two in passes of 32 (8 x 4).  1216 = (2 x 19)(8 x 4) and 1056 =
+// (3 x 11)(8 x 4): two passes each instead of three and four.
+#define HELM_FFT_PAIRS(X) \
+    X(8, 4) X(8, 2) X(2, 3) X(2, 5) X(2, 7) X(2, 11) X(2, 13) X(2, 17) X(2, 19) X(4, 3) X(4, 5) X(4, 7) X(8, 3) \
+    X(8, 5) X(3, 5) X(3, 7) X(3, 11) X(3, 13) X(5, 7)
+static bool fft_pair_ok(int r1, int r2) {
+#define HELM_FFT_PAIR_EQ(a, b) if (r1 == a && r2 == b) return true;
+    HELM_FFT_PAIRS(HELM_FFT_PAIR_EQ)
+#undef HELM_FFT_PAIR_EQ
+    return false;
+}
+// HELM_FFT_COMPOSITE=0: one pass per prime-or-power radix in FP32 too
+static bool fft_composite_on() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_FFT_COMPOSITE");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
+static std::vector<std::pair<int, int>> fft_plan(int n, bool composite) {
+    std::vector<std::pair<int, int>> passes;
+    if (composite) {
+        int m = n, a = 0;
+        while (m % 2 == 0) {
+            m /= 2;
+            ++a;
+        }
+        std::vector<int> odd;
+        for (int p = 3; m > 1; p += 2)
+            while (m % p == 0) {
+                odd.push_back(p);
+                m /= p;
+            }
+        bool ok = true;
+        for (int p : odd) ok = ok && p <= 31;
+        if (ok) {
+            std::sort(odd.rbegin(), odd.rend());
+            std::vector<bool> used(odd.size(), false);
+            for (size_t i = 0; i < odd.size(); ++i) {
+                if (used[i]) continue;
+                used[i] = true;
+                int mate = -1;
+                for (size_t j = odd.size(); j-- > i + 1;)
+                    if (!used[j] && fft_pair_ok(odd[j], odd[i])) {
+                        mate = (int)j;
+                        break;
+                    }
+                if (mate >= 0) {
+                    used[mate] = true;
+                    passes.push_back({odd[mate], odd[i]});
+                } else {
+                    passes.push_back({odd[i], 1});
+                }
+            }
+            for (auto& pr : passes) {
+                if (pr.second != 1) continue;
+                for (int b = 8, lb = 3; b >= 2; b /= 2, --lb)
+                    if (a >= lb && fft_pair_ok(b, pr.first)) {
+                        pr = {b, pr.first};
+                        a -= lb;
+                        break;
+                    }
+            }
+            while (a >= 5) {
+                passes.push_back({8, 4});
+                a -= 5;
+            }
+            if (a == 4) passes.push_back({8, 2});
+            else if (a == 3) passes.push_back({8, 1});
+            else if (a == 2) passes.push_back({4, 1});
+            else if (a == 1) passes.push_back({2, 1});
+            return passes;
+        }
+    }
+    for (int r : fft_radices(n)) passes.push_back({r, 1});
+    return passes;
 }
 
 // One axis of a 2-D transform of B boxes [H][W]: src -> (passes ping-pong
@@ -211,12 +342,12 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
                    const FftEpi<C>& first = FftEpi<C>(), const FftEpi<C>& last = FftEpi<C>(),
                    C* last_out = nullptr) {
     const int N = rows ? W : H;
-    const std::vector<int> rad = fft_radices(N);
+    const std::vector<std::pair<int, int>> rad = fft_plan(N, sizeof(C) == sizeof(float2) && fft_composite_on());
     const C* in = src;
     C* out = a;
     int Ns = 1;
     for (size_t p = 0; p < rad.size(); ++p) {
-        const int R = rad[p];
+        const int R = rad[p].first * rad[p].second;
         FftEpi<C> ep;
         if (p == 0) {
             ep.conj_in = first.conj_in;
@@ -228,7 +359,19 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
             ep.conj_out = last.conj_out;
             if (last_out) out = last_out;
         }
-        if (rows) {
+        if (rad[p].second != 1) {
+            if constexpr (sizeof(C) == sizeof(float2)) {  // composite plans are FP32 only
+                const int r1 = rad[p].first, r2 = rad[p].second;
+                const int64_t lines = (int64_t)H * B;
+#define HELM_FFT_PAIR_LAUNCH(A, Bq)                                                           \
+    if (r1 == A && r2 == Bq) {                                                                \
+        if (rows) launch_rows<C, A, Bq>(H, W, lines, Ns, sign, in, out, ep, s);               \
+        else launch_cols<C, A, Bq>(H, W, B, Ns, sign, in, out, ep, s);                        \
+    }
+                HELM_FFT_PAIRS(HELM_FFT_PAIR_LAUNCH)
+#undef HELM_FFT_PAIR_LAUNCH
+            }
+        } else if (rows) {
             const int64_t lines = (int64_t)H * B;
             switch (R) {
                 case 2: launch_rows<C, 2>(H, W, lines, Ns, sign, in, out, ep, s); break;

@@ -481,9 +481,10 @@ constexpr int INC_MAXX = 16;  // x values per thread: W <= 4096
 __global__ __launch_bounds__(256) void k_inc_field(int H, int W, int pz, double dx, double z0,
                                                    const double2* __restrict__ buf, double2* __restrict__ field) {
     __shared__ double2 row[INC_CHUNK];
+    extern __shared__ double2 tw[];  // the W twiddles, gathered by every output from LDS
     const int i = blockIdx.x;
     const double az = fabs((double)(i - pz) * dx + z0);
-    const double2* tw = buf;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) tw[k] = buf[k];
     const double2* kz = buf + 2 * W;
     const double2* spec = buf + 3 * W;
     double2 acc[INC_MAXX];
@@ -578,7 +579,11 @@ static int32_t launch_incident(helm_plan* p, double sin_theta, double aperture_m
     hipLaunchKernelGGL(k_inc_prepare, dim3((W + 255) / 256), dim3(256), 0, s, W, m.px, L.h_m, m.x0_m, p->k0,
                        sin_theta, aperture_m, buf);
     hipLaunchKernelGGL(k_inc_spectrum, dim3((W + 255) / 256), dim3(256), 0, s, W, buf);
-    hipLaunchKernelGGL(k_inc_field, dim3(H), dim3(256), 0, s, H, W, m.pz, L.h_m, m.z0_m, buf, field);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)k_inc_field, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+    (void)attr;
+    hipLaunchKernelGGL(k_inc_field, dim3(H), dim3(256), (size_t)W * sizeof(double2), s, H, W, m.pz, L.h_m, m.z0_m,
+                       buf, field);
     HELM_HIP(hipGetLastError());
     return HELM_OK;
 }

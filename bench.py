@@ -91,7 +91,7 @@ def parse():
                          "aux: capture, band RF and Born Jacobian kernels")
     ap.add_argument("--media", type=int, default=3, help="held-out media per frequency (learned workload)")
     ap.add_argument("--frequencies", type=int, default=64)
-    ap.add_argument("--spectral-restart", type=int, default=8,
+    ap.add_argument("--spectral-restart", type=int, default=10,
                     help="FGMRES restart length of the spectral workload (the reference's Fourier preconditioner)")
     ap.add_argument("--workers", type=int, default=4,
                     help="frequencies in flight, or set-up threads with --sweep-mode batched (frequencies workload)")

@@ -19,6 +19,7 @@
 // twiddle phases formed from exact integer products.
 #include <algorithm>
 #include <cmath>
+#include <utility>
 
 #include "helm_internal.h"
 
@@ -37,6 +38,65 @@ template <> __device__ __forceinline__ double2 cis<double2>(int m, int span) {
     return make_double2(c, s);
 }
 
+// e^{2 pi i q / R} for compile-time R and q < R, evaluated at compile time:
+// octant reduction in integers (angle (pi/4)(o + x) with o = 8q div R), then
+// Taylor series on [0, pi/4] in double (error below 1e-16)
+constexpr double CE_PI = 3.14159265358979323846264338327950288;
+constexpr double ce_sin_r(double x) {
+    double t = x, s = x;
+    for (int n = 1; n < 12; ++n) {
+        t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        s += t;
+    }
+    return s;
+}
+constexpr double ce_cos_r(double x) {
+    double t = 1.0, s = 1.0;
+    for (int n = 1; n < 12; ++n) {
+        t *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        s += t;
+    }
+    return s;
+}
+struct CeCis {
+    double c, s;
+};
+constexpr CeCis ce_cis(int q, int R) {
+    const int m = 8 * (q % R), o = m / R, rr = m - o * R;
+    double c = 0.0, s = 0.0;
+    if (o % 2 == 0) {
+        const double x = CE_PI / 4.0 * rr / R;
+        c = ce_cos_r(x);
+        s = ce_sin_r(x);
+    } else {
+        const double y = CE_PI / 4.0 * (R - rr) / R;
+        c = ce_sin_r(y);
+        s = ce_cos_r(y);
+    }
+    switch ((o / 2) % 4) {
+        case 1: return CeCis{-s, c};
+        case 2: return CeCis{-c, -s};
+        case 3: return CeCis{s, -c};
+        default: return CeCis{c, s};
+    }
+}
+template <int R, typename Seq>
+struct RootTab;
+template <int R, int... Q>
+struct RootTab<R, std::integer_sequence<int, Q...>> {
+    static constexpr double re[sizeof...(Q)] = {ce_cis(Q, R).c...};
+    static constexpr double im[sizeof...(Q)] = {ce_cis(Q, R).s...};
+};
+// e^{sign 2 pi i q / R} (q < R) from the compile-time table, in C's precision
+template <typename C, int R>
+__device__ __forceinline__ C root(int q, int sign) {
+    using T = RootTab<R, std::make_integer_sequence<int, R>>;
+    C w;
+    w.x = (decltype(w.x))T::re[q];
+    w.y = (decltype(w.y))(sign < 0 ? -T::im[q] : T::im[q]);
+    return w;
+}
+
 template <typename C, int R>
 __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign) {
     const int span = Ns * R;
@@ -47,7 +107,7 @@ __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign
     }
     C w[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) w[q] = cis<C>(sign * q, R);
+    for (int q = 0; q < R; ++q) w[q] = root<C, R>(q, sign);
     C out[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -77,7 +137,7 @@ __device__ __forceinline__ void fft_butterfly2(C (&v)[R1 * R2], int k, int Ns, i
     }
     C w2[R2];
 #pragma unroll
-    for (int q = 0; q < R2; ++q) w2[q] = cis<C>(sign * q, R2);
+    for (int q = 0; q < R2; ++q) w2[q] = root<C, R2>(q, sign);
     C a[R];
 #pragma unroll
     for (int r1 = 0; r1 < R1; ++r1)
@@ -86,11 +146,11 @@ __device__ __forceinline__ void fft_butterfly2(C (&v)[R1 * R2], int k, int Ns, i
             C acc = v[r1];
 #pragma unroll
             for (int r2 = 1; r2 < R2; ++r2) cfma(acc, v[r1 + R1 * r2], w2[(r2 * q2) % R2]);
-            a[r1 * R2 + q2] = (r1 && q2) ? cmul(acc, cis<C>(sign * ((r1 * q2) % R), R)) : acc;
+            a[r1 * R2 + q2] = (r1 && q2) ? cmul(acc, root<C, R>((r1 * q2) % R, sign)) : acc;
         }
     C w1[R1];
 #pragma unroll
-    for (int q = 0; q < R1; ++q) w1[q] = cis<C>(sign * q, R1);
+    for (int q = 0; q < R1; ++q) w1[q] = root<C, R1>(q, sign);
 #pragma unroll
     for (int q2 = 0; q2 < R2; ++q2)
 #pragma unroll

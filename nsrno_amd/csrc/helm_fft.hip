@@ -97,69 +97,177 @@ __device__ __forceinline__ C root(int q, int sign) {
     return w;
 }
 
+// v * e^{sign 2 pi i q / R} for compile-time q and R: the multiples of a
+// quarter turn as sign flips and swaps, the others as one complex product
+template <typename C, int R, int Q>
+__device__ __forceinline__ C cmul_root(C v, int sign) {
+    constexpr int q = Q % R;
+    if constexpr (q == 0) {
+        return v;
+    } else if constexpr (2 * q == R) {
+        v.x = -v.x;
+        v.y = -v.y;
+        return v;
+    } else if constexpr (4 * q == R || 4 * q == 3 * R) {
+        // times +-i: i (x + iy) = -y + ix
+        const auto s = (4 * q == R) ? sign : -sign;
+        C o;
+        o.x = s > 0 ? -v.y : v.y;
+        o.y = s > 0 ? v.x : -v.x;
+        return o;
+    } else {
+        return cmul(v, root<C, R>(q, sign));
+    }
+}
+
+template <typename C, int R, int... Q>
+__device__ __forceinline__ void roots_times(C* o, int sign, std::integer_sequence<int, Q...>) {
+    ((o[Q] = cmul_root<C, R, Q>(o[Q], sign)), ...);
+}
+// a[r1][q2] *= w_{R1 R2}^(r1 q2)
+template <typename C, int R1, int R2, int... I>
+__device__ __forceinline__ void inner_twiddles(C (&a)[R1][R2], int sign, std::integer_sequence<int, I...>) {
+    ((a[I / R2][I % R2] = cmul_root<C, R1 * R2, (I / R2) * (I % R2)>(a[I / R2][I % R2], sign)), ...);
+}
+
+// In-place R-point DFT X[q] = sum_r x[r] e^{sign 2 pi i r q / R} for a
+// compile-time R.  Even R: one radix-2 split (two R/2-point DFTs of the even
+// and odd samples, X[q] = E[q] + w^q O[q], X[q + R/2] = E[q] - w^q O[q]).
+// Odd R: the symmetric pairs a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r}
+// (r = 1..(R-1)/2) give
+//   X[0] = x_0 + sum a_r,  X[q], X[R-q] = P_q +- i sign Q_q,
+//   P_q = x_0 + sum_r a_r cos(2 pi r q / R),  Q_q = sum_r b_r sin(2 pi r q / R)
+// with real coefficients: (R-1)^2 FMAs instead of 4 (R-1) R.
+template <typename C, int R>
+__device__ __forceinline__ void dft_small(C (&x)[R], int sign);
+template <typename C, int R>
+__device__ __forceinline__ void dft_small(C (&x)[R], int sign) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        const C a = x[0], b = x[1];
+        x[0].x = a.x + b.x;
+        x[0].y = a.y + b.y;
+        x[1].x = a.x - b.x;
+        x[1].y = a.y - b.y;
+    } else if constexpr (R % 2 == 0) {
+        constexpr int H = R / 2;
+        C e[H], o[H];
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+            e[r] = x[2 * r];
+            o[r] = x[2 * r + 1];
+        }
+        dft_small<C, H>(e, sign);
+        dft_small<C, H>(o, sign);
+        roots_times<C, R>(o, sign, std::make_integer_sequence<int, H>{});
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            x[q].x = e[q].x + o[q].x;
+            x[q].y = e[q].y + o[q].y;
+            x[q + H].x = e[q].x - o[q].x;
+            x[q + H].y = e[q].y - o[q].y;
+        }
+    } else {
+        constexpr int H = (R - 1) / 2;
+        using T = RootTab<R, std::make_integer_sequence<int, R>>;
+        using F = decltype(x[0].x);
+        C a[H], b[H];
+        C x0 = x[0], s0 = x[0];
+#pragma unroll
+        for (int r = 1; r <= H; ++r) {
+            a[r - 1].x = x[r].x + x[R - r].x;
+            a[r - 1].y = x[r].y + x[R - r].y;
+            b[r - 1].x = x[r].x - x[R - r].x;
+            b[r - 1].y = x[r].y - x[R - r].y;
+            s0.x += a[r - 1].x;
+            s0.y += a[r - 1].y;
+        }
+        x[0] = s0;
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+            C P = x0, Q;
+            Q.x = 0;
+            Q.y = 0;
+#pragma unroll
+            for (int r = 1; r <= H; ++r) {
+                const F c = (F)T::re[(r * q) % R], sn = (F)T::im[(r * q) % R];
+                P.x += a[r - 1].x * c;
+                P.y += a[r - 1].y * c;
+                Q.x += b[r - 1].x * sn;
+                Q.y += b[r - 1].y * sn;
+            }
+            // i sign Q = sign (-Q.y, Q.x)
+            const F iqx = sign > 0 ? -Q.y : Q.y, iqy = sign > 0 ? Q.x : -Q.x;
+            x[q].x = P.x + iqx;
+            x[q].y = P.y + iqy;
+            x[R - q].x = P.x - iqx;
+            x[R - q].y = P.y - iqy;
+        }
+    }
+}
+
+// pass twiddles: v[r] *= e^{sign 2 pi i k r / (Ns R)} for r = r1 + R1 r2,
+// formed as the product of the r1 and R1 r2 factors (R1 + R2 - 2 sine-cosine
+// evaluations instead of R - 1)
+template <typename C, int R1, int R2>
+__device__ __forceinline__ void fft_twiddle(C (&v)[R1 * R2], int k, int Ns, int sign) {
+    constexpr int R = R1 * R2;
+    const int span = Ns * R;
+    if (Ns == 1) return;  // k = 0
+    C t1[R1], t2[R2];
+#pragma unroll
+    for (int r1 = 1; r1 < R1; ++r1) t1[r1] = cis<C>(sign * (int)(((int64_t)k * r1) % span), span);
+#pragma unroll
+    for (int r2 = 1; r2 < R2; ++r2) t2[r2] = cis<C>(sign * (int)(((int64_t)k * R1 * r2) % span), span);
+#pragma unroll
+    for (int r2 = 0; r2 < R2; ++r2)
+#pragma unroll
+        for (int r1 = 0; r1 < R1; ++r1) {
+            if (r1 == 0 && r2 == 0) continue;
+            const C w = r2 == 0 ? t1[r1] : (r1 == 0 ? t2[r2] : cmul(t1[r1], t2[r2]));
+            v[r1 + R1 * r2] = cmul(v[r1 + R1 * r2], w);
+        }
+}
+
 template <typename C, int R>
 __device__ __forceinline__ void fft_butterfly(C (&v)[R], int k, int Ns, int sign) {
     const int span = Ns * R;
+    if (Ns > 1) {
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-        const int m = (int)(((int64_t)k * r) % span);
-        v[r] = cmul(v[r], cis<C>(sign * m, span));
+        for (int r = 1; r < R; ++r) {
+            const int m = (int)(((int64_t)k * r) % span);
+            v[r] = cmul(v[r], cis<C>(sign * m, span));
+        }
     }
-    C w[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) w[q] = root<C, R>(q, sign);
-    C out[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        C acc = v[0];
-#pragma unroll
-        for (int r = 1; r < R; ++r) cfma(acc, v[r], w[(r * q) % R]);
-        out[q] = acc;
-    }
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = out[q];
+    dft_small<C, R>(v, sign);
 }
 
 // Composite radix R = R1 R2 in one pass (fewer passes over the data): the
-// pass twiddles as fft_butterfly, then the R-point DFT by Cooley-Tukey in
-// registers, with r = r1 + R1 r2 and q = q2 + R2 q1:
+// pass twiddles, then the R-point DFT by Cooley-Tukey in registers, with
+// r = r1 + R1 r2 and q = q2 + R2 q1:
 //   A[r1][q2] = sum_r2 v[r1 + R1 r2] w_R2^(r2 q2)       (R1 DFTs of size R2)
 //   out[q2 + R2 q1] = sum_r1 (w_R^(r1 q2) A[r1][q2]) w_R1^(r1 q1)
-// (R1 (R1 + R2) R2 products instead of R^2)
 template <typename C, int R1, int R2>
 __device__ __forceinline__ void fft_butterfly2(C (&v)[R1 * R2], int k, int Ns, int sign) {
-    constexpr int R = R1 * R2;
-    const int span = Ns * R;
+    fft_twiddle<C, R1, R2>(v, k, Ns, sign);
+    C a[R1][R2];
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-        const int m = (int)(((int64_t)k * r) % span);
-        v[r] = cmul(v[r], cis<C>(sign * m, span));
+    for (int r1 = 0; r1 < R1; ++r1) {
+#pragma unroll
+        for (int r2 = 0; r2 < R2; ++r2) a[r1][r2] = v[r1 + R1 * r2];
+        dft_small<C, R2>(a[r1], sign);
     }
-    C w2[R2];
+    inner_twiddles<C, R1, R2>(a, sign, std::make_integer_sequence<int, R1 * R2>{});
 #pragma unroll
-    for (int q = 0; q < R2; ++q) w2[q] = root<C, R2>(q, sign);
-    C a[R];
+    for (int q2 = 0; q2 < R2; ++q2) {
+        C b[R1];
 #pragma unroll
-    for (int r1 = 0; r1 < R1; ++r1)
+        for (int r1 = 0; r1 < R1; ++r1) b[r1] = a[r1][q2];
+        dft_small<C, R1>(b, sign);
 #pragma unroll
-        for (int q2 = 0; q2 < R2; ++q2) {
-            C acc = v[r1];
-#pragma unroll
-            for (int r2 = 1; r2 < R2; ++r2) cfma(acc, v[r1 + R1 * r2], w2[(r2 * q2) % R2]);
-            a[r1 * R2 + q2] = (r1 && q2) ? cmul(acc, root<C, R>((r1 * q2) % R, sign)) : acc;
-        }
-    C w1[R1];
-#pragma unroll
-    for (int q = 0; q < R1; ++q) w1[q] = root<C, R1>(q, sign);
-#pragma unroll
-    for (int q2 = 0; q2 < R2; ++q2)
-#pragma unroll
-        for (int q1 = 0; q1 < R1; ++q1) {
-            C acc = a[q2];
-#pragma unroll
-            for (int r1 = 1; r1 < R1; ++r1) cfma(acc, a[r1 * R2 + q2], w1[(r1 * q1) % R1]);
-            v[q2 + R2 * q1] = acc;
-        }
+        for (int q1 = 0; q1 < R1; ++q1) v[q2 + R2 * q1] = b[q1];
+    }
 }
 template <typename C, int R1, int R2>
 __device__ __forceinline__ void fft_pass_butterfly(C (&v)[R1 * R2], int k, int Ns, int sign) {

@@ -299,36 +299,70 @@ __device__ __forceinline__ C fft_epi(C v, const FftEpi<C>& e, int64_t node) {
     return v;
 }
 
-template <typename C, int R1, int R2 = 1>
+// STAGE (passes with Ns < 16): the outputs o + q Ns of a thread are Ns apart,
+// so lanes j and j + 1 store R elements apart and every store touches 64
+// lines.  The block (256 threads = 256 / M whole lines) then writes its lines
+// into LDS and copies them out contiguously.
+extern __shared__ __align__(16) unsigned char fft_rows_lds[];
+template <typename C, int R1, int R2 = 1, bool STAGE = false>
 __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, int Ns, int sign,
                                                  const C* __restrict__ in, C* __restrict__ out, FftEpi<C> ep) {
     constexpr int R = R1 * R2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int M = W / R;
-    if (t >= (int64_t)M * lines) return;
+    const bool live = t < (int64_t)M * lines;
+    if (!STAGE && !live) return;
     const int j = (int)(t % M);
     const int64_t line = t / M, base = line * W;
+    const int k = j % Ns;
+    const int o = (j / Ns) * Ns * R + k;
     C v[R];
+    if (live) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = in[base + j + r * M];
-    if (ep.bscale) {
-        const float bs = (float)ep.bscale[line / H];
+        for (int r = 0; r < R; ++r) v[r] = in[base + j + r * M];
+        if (ep.bscale) {
+            const float bs = (float)ep.bscale[line / H];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            v[r].x *= bs;
-            v[r].y *= bs;
+            for (int r = 0; r < R; ++r) {
+                v[r].x *= bs;
+                v[r].y *= bs;
+            }
+        }
+        if (ep.conj_in) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
+        }
+        fft_pass_butterfly<C, R1, R2>(v, k, Ns, sign);
+    }
+    if constexpr (!STAGE) {
+        const int64_t nrow = (line % H) * W;
+#pragma unroll
+        for (int q = 0; q < R; ++q) out[base + o + q * Ns] = fft_epi(v[q], ep, nrow + o + q * Ns);
+    } else {
+        C* sb = reinterpret_cast<C*>(fft_rows_lds);
+        const int L = (int)blockDim.x / M;
+        const int64_t line0 = (int64_t)blockIdx.x * L;
+        if (live) {
+            const int lo = (int)(line - line0) * W + o;
+#pragma unroll
+            for (int q = 0; q < R; ++q) sb[lo + q * Ns] = v[q];
+        }
+        __syncthreads();
+        const int n = (int)min((int64_t)L, lines - line0) * W;
+        int pos = (int)threadIdx.x, ln = 0;
+        while (pos >= W) {
+            pos -= W;
+            ++ln;
+        }
+        for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
+            out[line0 * W + i] = fft_epi(sb[i], ep, ((line0 + ln) % H) * W + pos);
+            pos += (int)blockDim.x;
+            while (pos >= W) {
+                pos -= W;
+                ++ln;
+            }
         }
     }
-    if (ep.conj_in) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r].y = -v[r].y;
-    }
-    const int k = j % Ns;
-    fft_pass_butterfly<C, R1, R2>(v, k, Ns, sign);
-    const int o = (j / Ns) * Ns * R + k;
-    const int64_t nrow = (line % H) * W;
-#pragma unroll
-    for (int q = 0; q < R; ++q) out[base + o + q * Ns] = fft_epi(v[q], ep, nrow + o + q * Ns);
 }
 
 // transform along z: for every batch and column x, length H with stride W
@@ -408,9 +442,25 @@ static std::vector<int> fft_radices(int n) {
 template <typename C, int R1, int R2 = 1>
 static void launch_rows(int H, int W, int64_t lines, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,
                         hipStream_t s) {
-    const int64_t cnt = (int64_t)(W / (R1 * R2)) * lines;
-    hipLaunchKernelGGL((k_fft_rows<C, R1, R2>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, H, W, lines,
-                       Ns, sign, in, out, ep);
+    constexpr int R = R1 * R2;
+    const int M = W / R;
+    const int64_t cnt = (int64_t)M * lines;
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    // staged stores: whole lines per block, at most 80 KB of LDS (256 R elements)
+    constexpr size_t stage_bytes = (size_t)256 * R * sizeof(C);
+    if (Ns * (int)sizeof(C) < 128 && 256 % M == 0 && stage_bytes <= 80 * 1024) {
+        static bool attr = false;
+        if (!attr) {
+            attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_rows<C, R1, R2, true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_bytes) == hipSuccess;
+        }
+        if (attr) {
+            hipLaunchKernelGGL((k_fft_rows<C, R1, R2, true>), grid, dim3(256), stage_bytes, s, H, W, lines, Ns, sign,
+                               in, out, ep);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_fft_rows<C, R1, R2>), grid, dim3(256), 0, s, H, W, lines, Ns, sign, in, out, ep);
 }
 template <typename C, int R1, int R2 = 1>
 static void launch_cols(int H, int W, int B, int Ns, int sign, const C* in, C* out, const FftEpi<C>& ep,

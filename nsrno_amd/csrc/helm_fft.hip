@@ -365,6 +365,56 @@ __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, i
     }
 }
 
+// Both passes of a two-pass row plan (W = RA RB, passes (A1 x A2) with Ns = 1
+// then (B1 x B2) with Ns = RA) in one kernel: each block holds FUSE_L whole
+// lines in LDS between the passes, so the rows go through HBM once instead of
+// twice.  Every block reads only its own lines before it writes them, so in
+// may equal out.
+constexpr int FUSE_L = 8;
+template <typename C, int A1, int A2, int B1, int B2>
+__global__ __launch_bounds__(320) void k_fft_rows_2p(int H, int64_t lines, int sign, const C* in, C* out,
+                                                     FftEpi<C> ep_first, FftEpi<C> ep_last) {
+    constexpr int RA = A1 * A2, RB = B1 * B2, W = RA * RB;
+    static_assert(FUSE_L * RA <= 320 && FUSE_L * RB <= 320, "one thread per butterfly of each pass");
+    C* sb = reinterpret_cast<C*>(fft_rows_lds);
+    const int64_t line0 = (int64_t)blockIdx.x * FUSE_L;
+    const int nl = (int)min((int64_t)FUSE_L, lines - line0);
+    const int tid = (int)threadIdx.x;
+    if (tid < nl * RB) {
+        const int li = tid / RB, j = tid % RB;
+        const int64_t base = (line0 + li) * W;
+        C v[RA];
+#pragma unroll
+        for (int r = 0; r < RA; ++r) v[r] = in[base + j + r * RB];
+        if (ep_first.bscale) {
+            const float bs = (float)ep_first.bscale[(line0 + li) / H];
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                v[r].x *= bs;
+                v[r].y *= bs;
+            }
+        }
+        if (ep_first.conj_in) {
+#pragma unroll
+            for (int r = 0; r < RA; ++r) v[r].y = -v[r].y;
+        }
+        fft_pass_butterfly<C, A1, A2>(v, 0, 1, sign);
+#pragma unroll
+        for (int q = 0; q < RA; ++q) sb[li * W + j * RA + q] = v[q];
+    }
+    __syncthreads();
+    if (tid < nl * RA) {
+        const int li = tid / RA, k = tid % RA;
+        C v[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) v[r] = sb[li * W + k + r * RA];
+        fft_pass_butterfly<C, B1, B2>(v, k, RA, sign);
+        const int64_t base = (line0 + li) * W, nrow = ((line0 + li) % H) * W;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) out[base + k + q * RA] = fft_epi(v[q], ep_last, nrow + k + q * RA);
+    }
+}
+
 // transform along z: for every batch and column x, length H with stride W
 template <typename C, int R1, int R2 = 1>
 __global__ __launch_bounds__(256) void k_fft_cols(int H, int W, int B, int Ns, int sign, const C* __restrict__ in,
@@ -485,6 +535,14 @@ static bool fft_pair_ok(int r1, int r2) {
     return false;
 }
 // HELM_FFT_COMPOSITE=0: one pass per prime-or-power radix in FP32 too
+// fused two-pass row kernel (HELM_FFT_ROWS_FUSED=0: one kernel per pass)
+static bool fft_rows_fused_on() {
+    static const bool t = [] {
+        const char* v = std::getenv("HELM_FFT_ROWS_FUSED");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return t;
+}
 static bool fft_composite_on() {
     static const bool t = [] {
         const char* v = std::getenv("HELM_FFT_COMPOSITE");
@@ -563,6 +621,23 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
     const std::vector<std::pair<int, int>> rad = fft_plan(N, sizeof(C) == sizeof(float2) && fft_composite_on());
     const C* in = src;
     C* out = a;
+    if constexpr (sizeof(C) == sizeof(float2)) {
+        // two-pass row plans with a fused kernel (1216 = (2 x 19)(8 x 4))
+        if (rows && rad.size() == 2 && rad[0] == std::make_pair(2, 19) && rad[1] == std::make_pair(8, 4) &&
+            fft_rows_fused_on()) {
+            constexpr size_t lds = (size_t)FUSE_L * 1216 * sizeof(C);
+            static bool attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_rows_2p<C, 2, 19, 8, 4>),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)lds) == hipSuccess;
+            if (attr) {
+                const int64_t lines = (int64_t)H * B;
+                C* dst = last_out ? last_out : b;
+                hipLaunchKernelGGL((k_fft_rows_2p<C, 2, 19, 8, 4>), dim3((unsigned)((lines + FUSE_L - 1) / FUSE_L)),
+                                   dim3(320), lds, s, H, lines, sign, src, dst, first, last);
+                return dst;
+            }
+        }
+    }
     int Ns = 1;
     for (size_t p = 0; p < rad.size(); ++p) {
         const int R = rad[p].first * rad[p].second;

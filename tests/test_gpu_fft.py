@@ -3,7 +3,8 @@ preconditioner, and the reference's warm-start-only forward through them.
 
 * helm_fft2 against numpy's FFT on boxes whose sides have radix-8/4/2 parts
   and odd prime factors (3, 11, 19, 37: register and generic kernels),
-  forward and inverse, batched.
+  forward and inverse, batched; 1216-wide rows go through the kernel that
+  fuses both row passes, 1056-tall columns through the (3 x 11)(8 x 4) plan.
 * HELM_PC_SPECTRAL: gain * ifft2(M fft2(v)) (adjoint) and its conjugate map
   against numpy, on boxes with radix 2-5 and radix 11 / 19 sides (the
   multiply, gain and conjugates folded into the first and last passes).
@@ -31,7 +32,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-@pytest.mark.parametrize("H,W", [(32, 96), (1056 // 8, 1216 // 16), (64, 2 * 37), (88, 1216), (24, 44),
+@pytest.mark.parametrize("H,W", [(32, 96), (1056 // 8, 1216 // 16), (64, 2 * 37), (88, 1216), (1056, 48), (24, 44),
                                  (2 * 31, 2 * 29), (2 * 23, 2 * 17), (4 * 41, 8)])
 def test_fft2_matches_numpy(H, W, precision):
     from nsrno_amd import _lib

@@ -370,12 +370,14 @@ __global__ __launch_bounds__(256) void k_fft_rows(int H, int W, int64_t lines, i
 // lines in LDS between the passes, so the rows go through HBM once instead of
 // twice.  Every block reads only its own lines before it writes them, so in
 // may equal out.
-constexpr int FUSE_L = 8;
-template <typename C, int A1, int A2, int B1, int B2>
-__global__ __launch_bounds__(320) void k_fft_rows_2p(int H, int64_t lines, int sign, const C* in, C* out,
-                                                     FftEpi<C> ep_first, FftEpi<C> ep_last) {
+template <int FUSE_L, int RA, int RB>
+constexpr int fuse_threads() {
+    return (FUSE_L * (RA > RB ? RA : RB) + 63) / 64 * 64;
+}
+template <typename C, int A1, int A2, int B1, int B2, int FUSE_L>
+__global__ __launch_bounds__((fuse_threads<FUSE_L, A1 * A2, B1 * B2>())) void k_fft_rows_2p(
+    int H, int64_t lines, int sign, const C* in, C* out, FftEpi<C> ep_first, FftEpi<C> ep_last) {
     constexpr int RA = A1 * A2, RB = B1 * B2, W = RA * RB;
-    static_assert(FUSE_L * RA <= 320 && FUSE_L * RB <= 320, "one thread per butterfly of each pass");
     C* sb = reinterpret_cast<C*>(fft_rows_lds);
     const int64_t line0 = (int64_t)blockIdx.x * FUSE_L;
     const int nl = (int)min((int64_t)FUSE_L, lines - line0);
@@ -535,6 +537,9 @@ static bool fft_pair_ok(int r1, int r2) {
     return false;
 }
 // HELM_FFT_COMPOSITE=0: one pass per prime-or-power radix in FP32 too
+// lines per workgroup of the fused row kernel: 4 (39 KB of LDS for 1216-wide
+// lines, four workgroups per CU, so one's loads overlap another's stores)
+constexpr int FFT_FUSE_L = 4;
 // fused two-pass row kernel (HELM_FFT_ROWS_FUSED=0: one kernel per pass)
 static bool fft_rows_fused_on() {
     static const bool t = [] {
@@ -625,15 +630,16 @@ static C* fft_axis(bool rows, int H, int W, int B, int sign, const C* src, C* a,
         // two-pass row plans with a fused kernel (1216 = (2 x 19)(8 x 4))
         if (rows && rad.size() == 2 && rad[0] == std::make_pair(2, 19) && rad[1] == std::make_pair(8, 4) &&
             fft_rows_fused_on()) {
-            constexpr size_t lds = (size_t)FUSE_L * 1216 * sizeof(C);
-            static bool attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_rows_2p<C, 2, 19, 8, 4>),
+            constexpr int L = FFT_FUSE_L;
+            constexpr size_t lds = (size_t)L * 1216 * sizeof(C);
+            static bool attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_rows_2p<C, 2, 19, 8, 4, L>),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    (int)lds) == hipSuccess;
             if (attr) {
                 const int64_t lines = (int64_t)H * B;
                 C* dst = last_out ? last_out : b;
-                hipLaunchKernelGGL((k_fft_rows_2p<C, 2, 19, 8, 4>), dim3((unsigned)((lines + FUSE_L - 1) / FUSE_L)),
-                                   dim3(320), lds, s, H, lines, sign, src, dst, first, last);
+                hipLaunchKernelGGL((k_fft_rows_2p<C, 2, 19, 8, 4, L>), dim3((unsigned)((lines + L - 1) / L)),
+                                   dim3(fuse_threads<L, 38, 32>()), lds, s, H, lines, sign, src, dst, first, last);
                 return dst;
             }
         }
